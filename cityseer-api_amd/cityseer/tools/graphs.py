@@ -1,0 +1,189 @@
+"""Graph preparation for the centrality path.
+
+Restates ``nx_simple_geoms`` (pysrc/cityseer/tools/graphs.py:58-107),
+``nx_decompose`` (graphs.py:1829-1962) and ``nx_to_dual`` (graphs.py:1965-2149)
+on networkX ``MultiGraph`` inputs, with ``tools.util.LineString`` in place of
+shapely geometries. Graph cleaning (graphs.py:110-1828) is out of scope.
+"""
+
+from __future__ import annotations
+
+import logging
+import math
+
+import networkx as nx
+import numpy as np
+
+from . import util
+from .util import LineString
+
+logger = logging.getLogger(__name__)
+
+
+def _coords_of(geom) -> list[tuple[float, ...]]:
+    return [tuple(c) for c in (geom.coords if hasattr(geom, "coords") else geom)]
+
+
+def nx_simple_geoms(nx_multigraph: nx.MultiGraph) -> nx.MultiGraph:
+    """Straight-line edge geometries between node coordinates (graphs.py:58-107)."""
+    g = nx_multigraph.copy()
+    remove = []
+    for s, e, k in g.edges(keys=True):
+        sd, ed = g.nodes[s], g.nodes[e]
+        s_z = "z" in sd and sd["z"] is not None
+        e_z = "z" in ed and ed["z"] is not None
+        if s_z and e_z:
+            seg = LineString([(sd["x"], sd["y"], sd["z"]), (ed["x"], ed["y"], ed["z"])])
+        else:
+            seg = LineString([(sd["x"], sd["y"]), (ed["x"], ed["y"])])
+        if seg.length == 0:
+            if s != e:
+                logger.warning(f"Found zero length edge between {s} and {e}, removing from graph.")
+            remove.append((s, e, k))
+        else:
+            g[s][e][k]["geom"] = seg
+    for s, e, k in remove:
+        g.remove_edge(s, e, key=k)
+    return util.validate_cityseer_networkx_graph(g)
+
+
+def _substring(coords: list[tuple[float, ...]], start: float, end: float) -> list[tuple[float, ...]]:
+    """Part of a polyline between two distances along it (shapely ``ops.substring``)."""
+    seglens = [math.hypot(b[0] - a[0], b[1] - a[1]) for a, b in zip(coords[:-1], coords[1:], strict=True)]
+    total = sum(seglens)
+    start = min(max(start, 0.0), total)
+    end = min(max(end, 0.0), total)
+
+    def point_at(dist: float) -> tuple[float, ...]:
+        walked = 0.0
+        for (a, b), sl in zip(zip(coords[:-1], coords[1:], strict=True), seglens, strict=True):
+            if walked + sl >= dist:
+                t = 0.0 if sl == 0 else (dist - walked) / sl
+                return tuple(a[i] + (b[i] - a[i]) * t for i in range(len(a)))
+            walked += sl
+        return tuple(coords[-1])
+
+    out = [point_at(start)]
+    walked = 0.0
+    for i, sl in enumerate(seglens):
+        walked += sl
+        if start < walked < end:
+            out.append(tuple(coords[i + 1]))
+    out.append(point_at(end))
+    return out
+
+
+def nx_decompose(nx_multigraph: nx.MultiGraph, decompose_max: float) -> nx.MultiGraph:
+    """Split edges so that none exceeds ``decompose_max`` (graphs.py:1829-1962)."""
+    g = util.validate_cityseer_networkx_graph(nx_multigraph)
+    for s, e, data in list(nx_multigraph.edges(data=True)):
+        sxy = (nx_multigraph.nodes[s]["x"], nx_multigraph.nodes[s]["y"])
+        coords = util.align_linestring_coords(_coords_of(data["geom"]), sxy)
+        line_len = LineString(coords).length
+        cuts = int(np.ceil(line_len / decompose_max))
+        step_size = line_len / cuts
+        g.remove_edge(s, e)
+        step = 0.0
+        prior = s
+        for sub in range(cuts - 1):
+            seg = _substring(coords, step, step + step_size)
+            end_coord = seg[-1]
+            x, y = end_coord[0], end_coord[1]
+            z = end_coord[2] if len(end_coord) == 3 else None
+            names = []
+            for name in [s, sub, e]:
+                name = str(name)
+                if len(name) > 10:
+                    name = f"{name[:5]}|{name[-5:]}"
+                names.append(name)
+            new_name = "±".join(names)
+            if new_name in g:
+                raise ValueError(
+                    f"Attempted to add a duplicate node at x: {x}, y:{y}. "
+                    f"Check for existence of duplicate edges in the vicinity of {s}-{e}."
+                )
+            attrs = {"x": x, "y": y}
+            if z is not None:
+                attrs["z"] = z
+            g.add_node(new_name, **attrs)
+            if "live" in nx_multigraph.nodes[s] and "live" in nx_multigraph.nodes[e]:
+                live = bool(nx_multigraph.nodes[s]["live"]) or bool(nx_multigraph.nodes[e]["live"])
+                g.nodes[new_name]["live"] = live
+            copy = {k: v for k, v in data.items() if k != "geom"}
+            g.add_edge(prior, new_name, geom=LineString(seg), **copy)
+            prior = new_name
+            step += step_size
+        seg = _substring(coords, step, line_len)
+        copy = {k: v for k, v in data.items() if k != "geom"}
+        g.add_edge(prior, e, geom=LineString(seg), **copy)
+    return util.validate_cityseer_networkx_graph(g)
+
+
+def nx_to_dual(nx_multigraph: nx.MultiGraph) -> nx.MultiGraph:
+    """Primal -> dual: one node per primal edge, one edge per pair of edges meeting at a node (graphs.py:1965-2149).
+
+    Dual edge geometry is the near half of each primal edge welded at the
+    shared primal node, so its length is half + half and its angle sum is the
+    turn angle at that node. Dual edges carry ``primal_node_id``.
+    """
+    util.validate_cityseer_networkx_graph(nx_multigraph)
+    if "is_dual" in nx_multigraph.graph:
+        raise ValueError("Graph is already a dual graph.")
+    g_dual = nx.MultiGraph()
+    g_dual.graph["is_dual"] = True
+    if "crs" in nx_multigraph.graph:
+        g_dual.graph["crs"] = nx_multigraph.graph["crs"]
+
+    def half_geoms(a, b, k):
+        data = nx_multigraph[a][b][k]
+        a_xy = (nx_multigraph.nodes[a]["x"], nx_multigraph.nodes[a]["y"])
+        coords = util.align_linestring_coords(_coords_of(data["geom"]), a_xy)
+        total = LineString(coords).length
+        a_half = _substring(coords, 0.0, 0.5 * total)
+        b_half = _substring(coords, 0.5 * total, total)
+        a_half[0] = tuple([a_xy[0], a_xy[1]] + list(a_half[0][2:]))
+        mid = a_half[-1]
+        b_half[0] = mid
+        b_xy = (nx_multigraph.nodes[b]["x"], nx_multigraph.nodes[b]["y"])
+        b_half[-1] = tuple([b_xy[0], b_xy[1]] + list(b_half[-1][2:]))
+        return a_half, b_half
+
+    def key_of(s, e, k) -> str:
+        se = sorted([str(s), str(e)])
+        return f"{se[0]}_{se[1]}_k{k}"
+
+    for s, e, k, data in nx_multigraph.edges(data=True, keys=True):
+        geom = data["geom"]
+        mid = LineString(_coords_of(geom)).interpolate(0.5, normalized=True)
+        dk = key_of(s, e, k)
+        kwargs = {
+            "x": mid.x,
+            "y": mid.y,
+            "primal_edge": geom,
+            "primal_edge_node_a": s,
+            "primal_edge_node_b": e,
+            "primal_edge_idx": k,
+        }
+        if mid.has_z:
+            kwargs["z"] = mid.z
+        g_dual.add_node(dk, **kwargs)
+        if "live" in nx_multigraph.nodes[s] and "live" in nx_multigraph.nodes[e]:
+            live = bool(nx_multigraph.nodes[s]["live"]) or bool(nx_multigraph.nodes[e]["live"])
+            g_dual.nodes[dk]["live"] = live
+    for s, e, k in nx_multigraph.edges(keys=True):
+        hub = key_of(s, e, k)
+        s_half, e_half = half_geoms(s, e, k)
+        for n_side, m_side, half in ((s, e, s_half), (e, s, e_half)):
+            for nb in nx.neighbors(nx_multigraph, n_side):
+                if nb == m_side:
+                    continue
+                for k2 in nx_multigraph[n_side][nb]:
+                    spoke = key_of(n_side, nb, k2)
+                    if g_dual.has_edge(hub, spoke):
+                        continue
+                    spoke_half, _ = half_geoms(n_side, nb, k2)
+                    # hub midpoint -> shared primal node -> spoke midpoint
+                    hub_part = list(reversed(half)) if half[0][:2] == spoke_half[0][:2] else list(half)
+                    merged = hub_part + spoke_half[1:]
+                    g_dual.add_edge(hub, spoke, primal_node_id=n_side, geom=LineString(merged))
+    return util.validate_cityseer_networkx_graph(g_dual)

@@ -1,0 +1,907 @@
+// cs_host.cpp -- host side of the C ABI: graph construction, device
+// residency, threshold conversion and the extern "C" entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "cs_internal.hpp"
+
+namespace cs {
+
+namespace {
+thread_local std::string t_err;
+thread_local cs_stats t_stats;
+}  // namespace
+
+void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? CS_ENOMEM : CS_EDEVICE;
+    (void)hipGetLastError();
+    fail(code, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+cs_stats &thread_stats() { return t_stats; }
+
+// Tobler slope penalty and edge travel seconds (centrality.rs:970-1007).
+float edge_seconds(const Graph &g, uint32_t from, uint32_t to, uint32_t e, float speed,
+                   bool use_impedance) {
+  if (!std::isnan(g.esec[e])) return g.esec[e];
+  const float length = g.elen[e];
+  float slope_pen = 1.0f;
+  if (length > 0.0f) {
+    const double zf = g.z[from], zt = g.z[to];
+    if (!std::isnan(zf) && !std::isnan(zt)) {
+      const float slope = (float)(zt - zf) / length;
+      const float flat = 0.839457f;
+      const float factor = std::exp(-3.5f * std::fabs(slope + 0.05f));
+      slope_pen = flat / factor;
+    }
+  }
+  const float imp = use_impedance ? g.eimp[e] : 1.0f;
+  float s = length * imp;
+  s = s * slope_pen;
+  return s / speed;
+}
+
+namespace {
+
+// Build the petgraph-ordered adjacency (newest edge first), twins, canonical
+// flags and the dual endpoint slots.
+void build_topology(Graph &g) {
+  const uint32_t n = g.n, m = g.m;
+  g.rev_off.assign(n + 1, 0);
+  g.fwd_off.assign(n + 1, 0);
+  for (uint32_t e = 0; e < m; ++e) {
+    g.fwd_off[g.esrc[e] + 1]++;
+    g.rev_off[g.edst[e] + 1]++;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    g.max_out_deg = std::max(g.max_out_deg, g.fwd_off[i + 1]);
+    g.max_in_deg = std::max(g.max_in_deg, g.rev_off[i + 1]);
+    g.fwd_off[i + 1] += g.fwd_off[i];
+    g.rev_off[i + 1] += g.rev_off[i];
+  }
+  g.fwd_eid.resize(m);
+  g.rev_eid.resize(m);
+  std::vector<uint32_t> fpos(g.fwd_off.begin(), g.fwd_off.end() - 1);
+  std::vector<uint32_t> rpos(g.rev_off.begin(), g.rev_off.end() - 1);
+  // descending edge id within each row: petgraph lists the newest edge first
+  for (uint32_t k = m; k-- > 0;) {
+    g.fwd_eid[fpos[g.esrc[k]]++] = k;
+    g.rev_eid[rpos[g.edst[k]]++] = k;
+  }
+  g.has_loops = false;
+  for (uint32_t e = 0; e < m; ++e)
+    if (g.esrc[e] == g.edst[e]) g.has_loops = true;
+  // twin(e) = edges_connecting(dst, src).find(key) (graph.rs:1278-1292)
+  g.twin.assign(m, -1);
+  for (uint32_t e = 0; e < m; ++e) {
+    const uint32_t a = g.edst[e], b = g.esrc[e], key = g.ekey[e];
+    for (uint32_t j = g.fwd_off[a]; j < g.fwd_off[a + 1]; ++j) {
+      const uint32_t f = g.fwd_eid[j];
+      if (g.edst[f] == b && g.ekey[f] == key) {
+        g.twin[e] = (int32_t)f;
+        break;
+      }
+    }
+  }
+  // canonical edge of each undirected key (min, max, edge_idx); loops excluded
+  // (circuit_ranks_from_traversal de-duplicates with a HashSet, centrality.rs:486-506)
+  g.canon.assign(m, 0);
+  {
+    std::unordered_map<uint64_t, std::vector<std::pair<uint32_t, uint32_t>>> seen;
+    seen.reserve(m);
+    for (uint32_t e = 0; e < m; ++e) {
+      const uint32_t a = g.esrc[e], b = g.edst[e];
+      if (a == b) continue;
+      const uint32_t lo = std::min(a, b), hi = std::max(a, b);
+      const uint64_t hk = ((uint64_t)lo << 32) | hi;
+      auto &lst = seen[hk];
+      bool dup = false;
+      for (auto &kv : lst)
+        if (kv.first == g.ekey[e]) dup = true;
+      if (!dup) {
+        lst.emplace_back(g.ekey[e], e);
+        g.canon[e] = 1;
+      }
+    }
+  }
+  // dual endpoint slots over edges in index order (centrality.rs:533-566)
+  g.slot_src.assign(m, -1);
+  g.slot_dst.assign(m, -1);
+  g.slots_ok = true;
+  std::vector<int32_t> slots(2 * (size_t)n, -1);
+  for (uint32_t e = 0; e < m && g.slots_ok; ++e) {
+    const int32_t k = g.epk[e];
+    if (k < 0) {
+      g.slots_ok = false;
+      g.slots_err = "dual edge " + std::to_string(g.esrc[e]) + " -> " + std::to_string(g.edst[e]) +
+                    " is missing shared_primal_node_key metadata";
+      break;
+    }
+    for (uint32_t node : {g.esrc[e], g.edst[e]}) {
+      int32_t *s = &slots[2 * (size_t)node];
+      if (s[0] == k || s[1] == k) continue;
+      if (s[0] < 0)
+        s[0] = k;
+      else if (s[1] < 0)
+        s[1] = k;
+      else {
+        g.slots_ok = false;
+        g.slots_err = "dual node " + std::to_string(node) + " references more than two primal endpoints";
+        break;
+      }
+    }
+  }
+  if (g.slots_ok) {
+    for (uint32_t e = 0; e < m; ++e) {
+      const int32_t k = g.epk[e];
+      const int32_t *ss = &slots[2 * (size_t)g.esrc[e]];
+      const int32_t *sd = &slots[2 * (size_t)g.edst[e]];
+      g.slot_src[e] = (int8_t)(ss[0] == k ? 0 : 1);
+      g.slot_dst[e] = (int8_t)(sd[0] == k ? 0 : 1);
+    }
+  }
+}
+
+template <typename T>
+T *dev_upload(const std::vector<T> &v) {
+  T *p = nullptr;
+  hip_check(hipMalloc(&p, sizeof(T) * std::max<size_t>(v.size(), 1)), "hipMalloc graph");
+  if (!v.empty()) hip_check(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice), "upload");
+  return p;
+}
+
+void free_edges(DevEdges &d) {
+  (void)hipFree(d.rev_adj);
+  (void)hipFree(d.fwd_adj);
+  (void)hipFree(d.rev_tw);
+  (void)hipFree(d.fwd_twlen);
+  (void)hipFree(d.rev_eid);
+  (void)hipFree(d.fwd_eid);
+  (void)hipFree(d.ang_adj);
+}
+
+}  // namespace
+
+DevGraph &device_graph(Graph &g, int device) {
+  auto it = g.dev.find(device);
+  if (it != g.dev.end()) return *it->second;
+  auto dg = std::make_unique<DevGraph>();
+  dg->device = device;
+  dg->rev_off = dev_upload(g.rev_off);
+  dg->fwd_off = dev_upload(g.fwd_off);
+  DevGraph &ref = *dg;
+  g.dev[device] = std::move(dg);
+  return ref;
+}
+
+DevEdges &device_edges(Graph &g, DevGraph &dg, float speed) {
+  for (auto &e : dg.edges)
+    if (e->speed == speed) return *e;
+  if (dg.edges.size() >= 4) {
+    free_edges(*dg.edges.front());
+    dg.edges.erase(dg.edges.begin());
+  }
+  auto d = std::make_unique<DevEdges>();
+  d->speed = speed;
+  const uint32_t m = g.m;
+  std::vector<uint2> rev(m), fwd(m);
+  std::vector<float2> rtw(m);
+  std::vector<float> ftw(m);
+  std::vector<uint32_t> reid(m), feid(m);
+  float min_es = INFINITY, max_es = 0.0f;
+  for (uint32_t v = 0; v < g.n; ++v) {
+    for (uint32_t j = g.rev_off[v]; j < g.rev_off[v + 1]; ++j) {
+      const uint32_t e = g.rev_eid[j];
+      const uint32_t nb = g.esrc[e];
+      const float es = edge_seconds(g, nb, v, e, speed, true);
+      uint32_t bits;
+      std::memcpy(&bits, &es, 4);
+      rev[j] = make_uint2(nb | (g.canon[e] ? kCanonBit : 0u), bits);
+      const int32_t tw = g.twin[e];
+      rtw[j] = make_float2(tw >= 0 ? g.elen[tw] : NAN, tw >= 0 ? g.eimp[tw] : NAN);
+      reid[j] = e;
+      if (nb != v) {
+        min_es = std::min(min_es, es);
+        max_es = std::max(max_es, es);
+      }
+    }
+    for (uint32_t j = g.fwd_off[v]; j < g.fwd_off[v + 1]; ++j) {
+      const uint32_t e = g.fwd_eid[j];
+      const uint32_t u = g.edst[e];
+      const float es = edge_seconds(g, v, u, e, speed, true);
+      uint32_t bits;
+      std::memcpy(&bits, &es, 4);
+      fwd[j] = make_uint2(u | (g.canon[e] ? kCanonBit : 0u), bits);
+      const int32_t tw = g.twin[e];
+      ftw[j] = tw >= 0 ? g.elen[tw] : NAN;
+      feid[j] = e;
+    }
+  }
+  d->min_es = std::isfinite(min_es) ? min_es : 0.0f;
+  d->max_es = max_es;
+  d->rev_adj = dev_upload(rev);
+  d->fwd_adj = dev_upload(fwd);
+  d->rev_tw = dev_upload(rtw);
+  d->fwd_twlen = dev_upload(ftw);
+  d->rev_eid = dev_upload(reid);
+  d->fwd_eid = dev_upload(feid);
+  dg.edges.push_back(std::move(d));
+  return *dg.edges.back();
+}
+
+// ---------------------------------------------------------------------------
+// Threshold conversion (common.rs:98-270), f32 arithmetic, round half away
+// from zero like Rust's f32::round.
+// ---------------------------------------------------------------------------
+namespace {
+
+void betas_from_distances(const uint32_t *d, uint32_t n, float mtw, float *out) {
+  if (n == 0) fail(CS_EINVAL, "Input 'distances' cannot be empty.");
+  for (uint32_t i = 1; i < n; ++i)
+    if (d[i] <= d[i - 1]) fail(CS_EINVAL, "Distances must be unique and sorted in strictly increasing order.");
+  for (uint32_t i = 0; i < n; ++i) {
+    if (d[i] == 0) fail(CS_EINVAL, "Distances must be positive integers.");
+    const float beta = -std::log(mtw) / (float)d[i];
+    out[i] = std::round(beta * 100000.0f) / 100000.0f;
+  }
+}
+
+void distances_from_betas(const float *b, uint32_t n, float mtw, uint32_t *out) {
+  if (n == 0) fail(CS_EINVAL, "Input 'betas' cannot be empty.");
+  for (uint32_t i = 1; i < n; ++i)
+    if (b[i] >= b[i - 1]) fail(CS_EINVAL, "Betas must be unique and sorted in strictly decreasing order.");
+  for (uint32_t i = 0; i < n; ++i) {
+    if (b[i] <= 0.0f) fail(CS_EINVAL, "Beta values must be greater than zero.");
+    const float d = std::round(std::log(mtw) / -b[i]);
+    if (d <= 0.0f) fail(CS_EINVAL, "Derived distance must be positive. Check beta values.");
+    out[i] = (uint32_t)d;
+  }
+}
+
+void seconds_from_distances(const uint32_t *d, uint32_t n, float speed, uint32_t *out) {
+  if (n == 0) fail(CS_EINVAL, "Input 'distances' cannot be empty.");
+  if (speed <= 0.0f) fail(CS_EINVAL, "Speed must be positive.");
+  for (uint32_t i = 1; i < n; ++i)
+    if (d[i] <= d[i - 1]) fail(CS_EINVAL, "Distances must be unique and sorted in strictly increasing order.");
+  for (uint32_t i = 0; i < n; ++i) {
+    if (d[i] == 0) fail(CS_EINVAL, "Distances must be positive integers.");
+    const float t = std::round((float)d[i] / speed);
+    if (t <= 0.0f) fail(CS_EINVAL, "Derived time must be positive. Check distance and speed values.");
+    out[i] = (uint32_t)t;
+  }
+}
+
+void distances_from_seconds(const uint32_t *s, uint32_t n, float speed, uint32_t *out) {
+  if (n == 0) fail(CS_EINVAL, "Input 'seconds' cannot be empty.");
+  if (speed <= 0.0f) fail(CS_EINVAL, "Speed must be positive.");
+  for (uint32_t i = 1; i < n; ++i)
+    if (s[i] <= s[i - 1]) fail(CS_EINVAL, "Times must be unique and sorted in strictly increasing order.");
+  for (uint32_t i = 0; i < n; ++i) {
+    if (s[i] == 0) fail(CS_EINVAL, "Time values must be positive integers.");
+    const float d = std::round((float)s[i] * speed);
+    if (d <= 0.0f) fail(CS_EINVAL, "Derived distance must be positive. Check time and speed values.");
+    out[i] = (uint32_t)d;
+  }
+}
+
+int current_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    fail(CS_EDEVICE, "no HIP device available: the centrality kernels require an AMD GPU (gfx950)");
+  }
+  int d = 0;
+  hip_check(hipGetDevice(&d), "hipGetDevice");
+  return d;
+}
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    t_err.clear();
+    f();
+    return CS_OK;
+  } catch (const Error &e) {
+    t_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    t_err = "host allocation failed";
+    return CS_ENOMEM;
+  } catch (const std::exception &e) {
+    t_err = e.what();
+    return CS_EINTERNAL;
+  }
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  explicit DevBuf(size_t bytes) { hip_check(hipMalloc(&p, std::max<size_t>(bytes, 4)), "hipMalloc"); }
+  ~DevBuf() { (void)hipFree(p); }
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+void check_params(const Graph &g, const cs_params *p) {
+  if (!p) fail(CS_EINVAL, "params must not be NULL");
+  if (p->n_thresholds == 0 || p->n_thresholds > CS_MAX_THRESHOLDS)
+    fail(CS_EINVAL, "between 1 and " + std::to_string(CS_MAX_THRESHOLDS) + " distance thresholds are supported");
+  if (!p->compute_closeness && !p->compute_betweenness)
+    fail(CS_EINVAL,
+         "Either or both closeness and betweenness flags is required, but both parameters are False.");
+  if (!(p->speed_m_s > 0.0f) || !std::isfinite(p->speed_m_s))
+    fail(CS_EINVAL, "speed_m_s must be finite and positive");
+  for (uint32_t i = 0; i < p->n_sources; ++i)
+    if (p->sources[i] >= g.n)
+      fail(CS_EINVAL, "node index " + std::to_string(p->sources[i]) + " does not exist in the graph");
+}
+
+CallSpec make_spec(const cs_params *p) {
+  CallSpec c;
+  c.T = p->n_thresholds;
+  uint32_t maxs = 0;
+  for (uint32_t i = 0; i < c.T; ++i) {
+    c.dist[i] = (float)p->distances[i];
+    c.beta[i] = p->betas[i];
+    c.seconds[i] = p->seconds[i];
+    maxs = std::max(maxs, p->seconds[i]);
+  }
+  c.maxsec = (float)maxs;
+  c.speed = p->speed_m_s;
+  c.tolerance = p->tolerance;
+  c.closeness = p->compute_closeness != 0;
+  c.betweenness = p->compute_betweenness != 0;
+  c.track_reach = p->track_reach != 0;
+  c.ang_unit = p->angular_scaling_unit;
+  c.far_offset = p->farness_scaling_offset;
+  return c;
+}
+
+// Sources kept after NaN-weight filtering, with per-source weights and the
+// cycles weight wt / node_weight (centrality.rs:1711-1730).
+struct SourceSet {
+  std::vector<uint32_t> src;
+  std::vector<float> wt, cyc;
+};
+
+SourceSet source_set(const Graph &g, const cs_params *p) {
+  SourceSet s;
+  s.src.reserve(p->n_sources);
+  for (uint32_t i = 0; i < p->n_sources; ++i) {
+    const uint32_t v = p->sources[i];
+    if (!p->eligible[v]) continue;
+    const float w = p->source_wt ? p->source_wt[i] : g.weight[v];
+    if (std::isnan(w)) continue;
+    s.src.push_back(v);
+    s.wt.push_back(w);
+    s.cyc.push_back(w / g.weight[v]);
+  }
+  return s;
+}
+
+// acc [N][M][T] (device, node-major) -> out [M][T][N] (host)
+void unpack_acc(const std::vector<double> &acc, uint32_t N, uint32_t M, uint32_t T, double *out) {
+  for (uint32_t v = 0; v < N; ++v)
+    for (uint32_t m = 0; m < M; ++m)
+      for (uint32_t i = 0; i < T; ++i) out[((size_t)m * T + i) * N + v] = acc[((size_t)v * M + m) * T + i];
+}
+
+void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s) {
+  if (s == 1.0) return;
+  for (uint32_t m = m0; m <= m1; ++m)
+    for (uint32_t i = 0; i < T; ++i)
+      for (uint32_t v = 0; v < N; ++v) out[((size_t)m * T + i) * N + v] *= s;
+}
+
+void run_shortest_checks(const Graph &g, const cs_params *p) {
+  if (p->tolerance > kTieEps)
+    fail(CS_EINVAL, "tolerance above the internal 0.01% tie band is not supported by the GPU search yet");
+  if (p->compute_betweenness && g.max_out_deg > 64)
+    fail(CS_EINVAL, "nodes with more than 64 out-edges are not supported by the GPU search");
+}
+
+}  // namespace
+
+}  // namespace cs
+
+using namespace cs;
+
+struct cs_graph {
+  Graph g;
+};
+
+extern "C" {
+
+const char *cs_last_error(void) { return t_err.c_str(); }
+
+const char *cs_version(void) { return "cityseer-amd 0.1.0 (gfx950)"; }
+
+int cs_pair_distances_betas_time(float speed, int mode, const uint32_t *distances, const float *betas,
+                                 const float *minutes, uint32_t n, float mtw, uint32_t *od, float *ob,
+                                 uint32_t *os) {
+  return guarded([&] {
+    if (mode == 0) {
+      for (uint32_t i = 0; i < n; ++i) od[i] = distances[i];
+      betas_from_distances(od, n, mtw, ob);
+      seconds_from_distances(od, n, speed, os);
+    } else if (mode == 1) {
+      for (uint32_t i = 0; i < n; ++i) ob[i] = betas[i];
+      distances_from_betas(ob, n, mtw, od);
+      seconds_from_distances(od, n, speed, os);
+    } else if (mode == 2) {
+      for (uint32_t i = 0; i < n; ++i) os[i] = (uint32_t)std::round(minutes[i] * 60.0f);
+      distances_from_seconds(os, n, speed, od);
+      betas_from_distances(od, n, mtw, ob);
+    } else {
+      fail(CS_EINVAL,
+           "Please provide exactly one of the following arguments: 'distances', 'betas', or 'minutes'.");
+    }
+  });
+}
+
+int cs_distances_from_betas(const float *betas, uint32_t n, float mtw, uint32_t *out) {
+  return guarded([&] { distances_from_betas(betas, n, mtw, out); });
+}
+int cs_betas_from_distances(const uint32_t *d, uint32_t n, float mtw, float *out) {
+  return guarded([&] { betas_from_distances(d, n, mtw, out); });
+}
+int cs_distances_from_seconds(const uint32_t *s, uint32_t n, float speed, uint32_t *out) {
+  return guarded([&] { distances_from_seconds(s, n, speed, out); });
+}
+int cs_seconds_from_distances(const uint32_t *d, uint32_t n, float speed, uint32_t *out) {
+  return guarded([&] { seconds_from_distances(d, n, speed, out); });
+}
+
+int cs_graph_create(uint32_t n_nodes, const double *z, const uint8_t *live, const float *weight,
+                    uint32_t n_edges, const uint32_t *src, const uint32_t *dst, const uint32_t *edge_key,
+                    const float *length, const float *imp_factor, const float *angle_sum,
+                    const float *seconds, const int32_t *primal_key, int is_dual, cs_graph **out) {
+  return guarded([&] {
+    if (!out) fail(CS_EINVAL, "out must not be NULL");
+    if (n_nodes >= kNodeMask) fail(CS_EINVAL, "too many nodes");
+    if (n_edges && (!src || !dst || !length)) fail(CS_EINVAL, "edge arrays must not be NULL");
+    auto h = std::make_unique<cs_graph>();
+    Graph &g = h->g;
+    g.n = n_nodes;
+    g.m = n_edges;
+    g.is_dual = is_dual != 0;
+    g.z.resize(n_nodes);
+    g.live.resize(n_nodes);
+    g.weight.resize(n_nodes);
+    for (uint32_t i = 0; i < n_nodes; ++i) {
+      g.z[i] = z ? z[i] : NAN;
+      g.live[i] = live ? live[i] : 1;
+      g.weight[i] = weight ? weight[i] : 1.0f;
+    }
+    g.esrc.assign(src, src + n_edges);
+    g.edst.assign(dst, dst + n_edges);
+    g.ekey.resize(n_edges);
+    g.elen.assign(length, length + n_edges);
+    g.eimp.resize(n_edges);
+    g.eang.resize(n_edges);
+    g.esec.resize(n_edges);
+    g.epk.resize(n_edges);
+    for (uint32_t e = 0; e < n_edges; ++e) {
+      if (src[e] >= n_nodes || dst[e] >= n_nodes)
+        fail(CS_EINVAL, "edge " + std::to_string(e) + " references a node that does not exist");
+      g.ekey[e] = edge_key ? edge_key[e] : 0;
+      g.eimp[e] = imp_factor ? imp_factor[e] : 1.0f;
+      g.eang[e] = angle_sum ? angle_sum[e] : 0.0f;
+      g.esec[e] = seconds ? seconds[e] : NAN;
+      g.epk[e] = primal_key ? primal_key[e] : -1;
+      if (!(g.eimp[e] > 0.0f) || !std::isfinite(g.eimp[e]))
+        fail(CS_EINVAL, "Invalid impedance factor for edge " + std::to_string(e) +
+                            ": impedance must be finite and positive (> 0.0).");
+    }
+    build_topology(g);
+    *out = h.release();
+  });
+}
+
+void cs_graph_destroy(cs_graph *h) {
+  if (!h) return;
+  for (auto &kv : h->g.dev) {
+    int prev = 0;
+    if (hipGetDevice(&prev) == hipSuccess && hipSetDevice(kv.first) == hipSuccess) {
+      (void)hipFree(kv.second->rev_off);
+      (void)hipFree(kv.second->fwd_off);
+      for (auto &e : kv.second->edges) free_edges(*e);
+      (void)hipSetDevice(prev);
+    }
+  }
+  (void)hipGetLastError();
+  delete h;
+}
+
+int cs_graph_info(const cs_graph *h, uint32_t *n, uint32_t *m, uint32_t *mo, uint32_t *mi) {
+  return guarded([&] {
+    if (!h) fail(CS_EINVAL, "graph must not be NULL");
+    if (n) *n = h->g.n;
+    if (m) *m = h->g.m;
+    if (mo) *mo = h->g.max_out_deg;
+    if (mi) *mi = h->g.max_in_deg;
+  });
+}
+
+uint64_t cs_graph_progress(const cs_graph *h) { return h ? h->g.progress : 0; }
+
+int cs_last_stats(cs_stats *out) {
+  if (!out) return CS_EINVAL;
+  *out = t_stats;
+  return CS_OK;
+}
+
+int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_t *reach_totals) {
+  const auto t0 = std::chrono::steady_clock::now();
+  t_stats = cs_stats{};
+  int rc = guarded([&] {
+    if (!h || !out) fail(CS_EINVAL, "graph and out must not be NULL");
+    Graph &g = h->g;
+    check_params(g, p);
+    const int dev = current_device();
+    const CallSpec spec = make_spec(p);
+    const SourceSet ss = source_set(g, p);
+    const uint32_t N = g.n, T = spec.T, M = 7;
+    hipStream_t stream;
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    DevBuf acc(sizeof(double) * (size_t)N * M * T), reach(4 * kMaxT), d_src(4 * ss.src.size()),
+        d_wt(4 * ss.src.size()), d_cyc(4 * ss.src.size()), d_elig(N);
+    hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
+    hip_check(hipMemsetAsync(reach.p, 0, 4 * kMaxT, stream), "memset");
+    if (!ss.src.empty()) {
+      hip_check(hipMemcpyAsync(d_src.p, ss.src.data(), 4 * ss.src.size(), hipMemcpyHostToDevice, stream), "copy");
+      hip_check(hipMemcpyAsync(d_wt.p, ss.wt.data(), 4 * ss.wt.size(), hipMemcpyHostToDevice, stream), "copy");
+      hip_check(hipMemcpyAsync(d_cyc.p, ss.cyc.data(), 4 * ss.cyc.size(), hipMemcpyHostToDevice, stream), "copy");
+    }
+    hip_check(hipMemcpyAsync(d_elig.p, p->eligible, N, hipMemcpyHostToDevice, stream), "copy");
+    {
+      std::lock_guard<std::mutex> lk(g.mu);
+      DevGraph &dg = device_graph(g, dev);
+      DevEdges &de = device_edges(g, dg, spec.speed);
+      run_shortest_checks(g, p);
+      launch_shortest(g, dg, de, spec, d_src.as<uint32_t>(), d_wt.as<float>(), d_cyc.as<float>(),
+                      (uint32_t)ss.src.size(), d_elig.as<uint8_t>(), acc.as<double>(), reach.as<uint32_t>(),
+                      stream, t_stats);
+    }
+    std::vector<double> hacc((size_t)N * M * T);
+    uint32_t hreach[kMaxT];
+    hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
+              "copy out");
+    hip_check(hipMemcpyAsync(hreach, reach.p, 4 * kMaxT, hipMemcpyDeviceToHost, stream), "copy out");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    hip_check(hipStreamDestroy(stream), "stream destroy");
+    unpack_acc(hacc, N, M, T, out);
+    if (p->compute_betweenness) scale_rows(out, N, T, 5, 6, p->betw_scale);
+    if (reach_totals)
+      for (uint32_t i = 0; i < T; ++i) reach_totals[i] = hreach[i];
+    t_stats.sources = ss.src.size();
+    g.progress = p->n_sources;
+  });
+  t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int cs_centrality_shortest_device(cs_graph *h, const cs_params *p, const float *d_cycles_wt, double *d_acc,
+                                  uint32_t *d_reach, void *stream_ptr) {
+  const auto t0 = std::chrono::steady_clock::now();
+  t_stats = cs_stats{};
+  int rc = guarded([&] {
+    if (!h || !p || !d_acc) fail(CS_EINVAL, "graph, params and d_acc must not be NULL");
+    Graph &g = h->g;
+    if (p->n_thresholds == 0 || p->n_thresholds > CS_MAX_THRESHOLDS)
+      fail(CS_EINVAL, "between 1 and " + std::to_string(CS_MAX_THRESHOLDS) + " distance thresholds are supported");
+    if (!p->compute_closeness && !p->compute_betweenness)
+      fail(CS_EINVAL, "Either or both closeness and betweenness flags is required, but both parameters are False.");
+    if (!p->sources || !p->source_wt || !d_cycles_wt || !p->eligible)
+      fail(CS_EINVAL, "device sources, weights, cycle weights and eligibility must not be NULL");
+    const int dev = current_device();
+    const CallSpec spec = make_spec(p);
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    std::lock_guard<std::mutex> lk(g.mu);
+    DevGraph &dg = device_graph(g, dev);
+    DevEdges &de = device_edges(g, dg, spec.speed);
+    run_shortest_checks(g, p);
+    launch_shortest(g, dg, de, spec, p->sources, p->source_wt, d_cycles_wt, p->n_sources, p->eligible, d_acc,
+                    d_reach, stream, t_stats);
+    t_stats.sources = p->n_sources;
+    g.progress = p->n_sources;
+  });
+  t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int cs_segment_centrality(cs_graph *h, const cs_params *p, double *out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  t_stats = cs_stats{};
+  int rc = guarded([&] {
+    if (!h || !out) fail(CS_EINVAL, "graph and out must not be NULL");
+    Graph &g = h->g;
+    check_params(g, p);
+    const int dev = current_device();
+    const CallSpec spec = make_spec(p);
+    const uint32_t N = g.n, T = spec.T, M = 4;
+    // the reference looks up twin edges (graph.rs:1278-1292) and panics if one is missing
+    for (uint32_t e = 0; e < g.m; ++e)
+      if (g.twin[e] < 0)
+        fail(CS_EINVAL, "segment centrality needs every edge to have a reverse edge with the same key (edge " +
+                            std::to_string(e) + ")");
+    std::vector<uint32_t> srcs;
+    for (uint32_t i = 0; i < p->n_sources; ++i)
+      if (g.live[p->sources[i]]) srcs.push_back(p->sources[i]);
+    std::lock_guard<std::mutex> lk(g.mu);
+    DevGraph &dg = device_graph(g, dev);
+    DevEdges &de = device_edges(g, dg, spec.speed);
+    hipStream_t stream;
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    DevBuf acc(sizeof(double) * (size_t)N * M * T), d_src(4 * srcs.size());
+    hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
+    if (!srcs.empty())
+      hip_check(hipMemcpyAsync(d_src.p, srcs.data(), 4 * srcs.size(), hipMemcpyHostToDevice, stream), "copy");
+    launch_segment(g, dg, de, spec, d_src.as<uint32_t>(), (uint32_t)srcs.size(), acc.as<double>(), stream,
+                   t_stats);
+    std::vector<double> hacc((size_t)N * M * T);
+    hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
+              "copy out");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    hip_check(hipStreamDestroy(stream), "stream destroy");
+    unpack_acc(hacc, N, M, T, out);
+    t_stats.sources = srcs.size();
+    g.progress = p->n_sources;
+  });
+  t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int cs_centrality_simplest(cs_graph *h, const cs_params *p, double *out, uint32_t *reach_totals) {
+  const auto t0 = std::chrono::steady_clock::now();
+  t_stats = cs_stats{};
+  int rc = guarded([&] {
+    if (!h || !out) fail(CS_EINVAL, "graph and out must not be NULL");
+    Graph &g = h->g;
+    if (!g.is_dual)
+      fail(CS_EINVAL,
+           "centrality_simplest requires a dual graph for angular analysis. Convert the graph with "
+           "cityseer.tools.graphs.nx_to_dual(...) before ingesting it into NetworkStructure.");
+    check_params(g, p);
+    if (!g.slots_ok) fail(CS_EINVAL, g.slots_err);
+    const int dev = current_device();
+    const CallSpec spec = make_spec(p);
+    const SourceSet ss = source_set(g, p);
+    const uint32_t N = g.n, T = spec.T, M = 4;
+    std::lock_guard<std::mutex> lk(g.mu);
+    DevGraph &dg = device_graph(g, dev);
+    DevEdges &de = device_edges(g, dg, spec.speed);
+    hipStream_t stream;
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    DevBuf acc(sizeof(double) * (size_t)N * M * T), reach(4 * kMaxT), d_src(4 * ss.src.size()),
+        d_wt(4 * ss.src.size()), d_elig(N);
+    hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
+    hip_check(hipMemsetAsync(reach.p, 0, 4 * kMaxT, stream), "memset");
+    if (!ss.src.empty()) {
+      hip_check(hipMemcpyAsync(d_src.p, ss.src.data(), 4 * ss.src.size(), hipMemcpyHostToDevice, stream), "copy");
+      hip_check(hipMemcpyAsync(d_wt.p, ss.wt.data(), 4 * ss.wt.size(), hipMemcpyHostToDevice, stream), "copy");
+    }
+    hip_check(hipMemcpyAsync(d_elig.p, p->eligible, N, hipMemcpyHostToDevice, stream), "copy");
+    launch_simplest(g, dg, de, spec, d_src.as<uint32_t>(), d_wt.as<float>(), (uint32_t)ss.src.size(),
+                    d_elig.as<uint8_t>(), acc.as<double>(), reach.as<uint32_t>(), stream, t_stats);
+    std::vector<double> hacc((size_t)N * M * T);
+    uint32_t hreach[kMaxT];
+    hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
+              "copy out");
+    hip_check(hipMemcpyAsync(hreach, reach.p, 4 * kMaxT, hipMemcpyDeviceToHost, stream), "copy out");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    hip_check(hipStreamDestroy(stream), "stream destroy");
+    unpack_acc(hacc, N, M, T, out);
+    if (p->compute_betweenness) scale_rows(out, N, T, 3, 3, p->betw_scale);
+    if (reach_totals)
+      for (uint32_t i = 0; i < T; ++i) reach_totals[i] = hreach[i];
+    t_stats.sources = ss.src.size();
+    g.progress = p->n_sources;
+  });
+  t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Single-source search with device outputs copied back.
+struct Single {
+  std::vector<float> secs;
+  std::vector<int32_t> pred, last;
+  std::vector<double> sigma;
+  std::vector<uint32_t> preds, npreds;
+};
+
+Single run_single(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, float tolerance, int mode,
+                  uint32_t max_preds) {
+  Graph &g = h->g;
+  if (src >= g.n)
+    fail(CS_EINVAL, "src_idx " + std::to_string(src) + " out of range for network with node_bound " +
+                        std::to_string(g.n));
+  if (!std::isfinite(speed) || !(speed > 0.0f)) fail(CS_EINVAL, "speed_m_s must be finite and positive");
+  const int dev = current_device();
+  const uint32_t N = g.n;
+  std::lock_guard<std::mutex> lk(g.mu);
+  DevGraph &dg = device_graph(g, dev);
+  DevEdges &de = device_edges(g, dg, speed);
+  hipStream_t stream;
+  hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+  Single s;
+  s.secs.assign(N, INFINITY);
+  s.pred.assign(N, -1);
+  s.last.assign(N, -1);
+  DevBuf d_secs(4ull * N), d_pred(4ull * N), d_last(4ull * N), d_sigma(8ull * N), d_np(4ull * N),
+      d_pl(4ull * N * std::max<uint32_t>(max_preds, 1));
+  hip_check(hipMemcpyAsync(d_secs.p, s.secs.data(), 4ull * N, hipMemcpyHostToDevice, stream), "copy");
+  hip_check(hipMemsetAsync(d_pred.p, 0xff, 4ull * N, stream), "memset");
+  hip_check(hipMemsetAsync(d_last.p, 0xff, 4ull * N, stream), "memset");
+  hip_check(hipMemsetAsync(d_sigma.p, 0, 8ull * N, stream), "memset");
+  hip_check(hipMemsetAsync(d_np.p, 0, 4ull * N, stream), "memset");
+  TreeOut to;
+  to.secs = d_secs.as<float>();
+  to.pred = d_pred.as<int32_t>();
+  to.last_eid = d_last.as<int32_t>();
+  to.sigma = d_sigma.as<double>();
+  to.pred_list = d_pl.as<uint32_t>();
+  to.n_preds = d_np.as<uint32_t>();
+  to.max_preds = max_preds;
+  launch_single(g, dg, de, src, (float)max_seconds, tolerance, mode, to, stream);
+  hip_check(hipMemcpyAsync(s.secs.data(), d_secs.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
+  hip_check(hipMemcpyAsync(s.pred.data(), d_pred.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
+  hip_check(hipMemcpyAsync(s.last.data(), d_last.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
+  if (mode == 1) {
+    s.sigma.resize(N);
+    s.npreds.resize(N);
+    s.preds.resize((size_t)N * max_preds);
+    hip_check(hipMemcpyAsync(s.sigma.data(), d_sigma.p, 8ull * N, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipMemcpyAsync(s.npreds.data(), d_np.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
+    if (max_preds)
+      hip_check(hipMemcpyAsync(s.preds.data(), d_pl.p, 4ull * N * max_preds, hipMemcpyDeviceToHost, stream),
+                "copy");
+  }
+  hip_check(hipStreamSynchronize(stream), "sync");
+  hip_check(hipStreamDestroy(stream), "stream destroy");
+  return s;
+}
+
+// visited order = reference settle order: ascending (seconds, node)
+uint32_t settle_order(const std::vector<float> &secs, uint32_t *order) {
+  std::vector<uint32_t> idx;
+  for (uint32_t i = 0; i < secs.size(); ++i)
+    if (std::isfinite(secs[i])) idx.push_back(i);
+  std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+    return secs[a] < secs[b] || (secs[a] == secs[b] && a < b);
+  });
+  if (order) std::copy(idx.begin(), idx.end(), order);
+  return (uint32_t)idx.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+int cs_tree_shortest(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, uint8_t *visited,
+                     uint8_t *discovered, int32_t *pred, float *short_dist, float *agg_seconds, uint32_t *order,
+                     uint32_t *n_order) {
+  return guarded([&] {
+    if (!h) fail(CS_EINVAL, "graph must not be NULL");
+    Single s = run_single(h, src, max_seconds, speed, kTieEps, 0, 0);
+    const uint32_t N = h->g.n;
+    for (uint32_t v = 0; v < N; ++v) {
+      const bool r = std::isfinite(s.secs[v]);
+      visited[v] = r;
+      discovered[v] = r;
+      pred[v] = s.pred[v];
+      agg_seconds[v] = s.secs[v];
+      short_dist[v] = v == src ? 0.0f : (r ? s.secs[v] * speed : INFINITY);
+    }
+    const uint32_t k = settle_order(s.secs, order);
+    if (n_order) *n_order = k;
+  });
+}
+
+int cs_tree_segment(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, uint8_t *visited,
+                    uint8_t *discovered, int32_t *pred, float *short_dist, float *agg_seconds,
+                    int32_t *origin_seg, int32_t *last_seg, uint32_t *order, uint32_t *n_order,
+                    uint32_t *edge_order, uint32_t *n_edges) {
+  return guarded([&] {
+    if (!h) fail(CS_EINVAL, "graph must not be NULL");
+    Single s = run_single(h, src, max_seconds, speed, kTieEps, 0, 0);
+    const Graph &g = h->g;
+    const uint32_t N = g.n;
+    std::vector<uint32_t> ord(N);
+    const uint32_t k = settle_order(s.secs, ord.data());
+    for (uint32_t v = 0; v < N; ++v) {
+      const bool r = std::isfinite(s.secs[v]);
+      visited[v] = r;
+      discovered[v] = r;
+      pred[v] = s.pred[v];
+      agg_seconds[v] = s.secs[v];
+      short_dist[v] = v == src ? 0.0f : (r ? s.secs[v] * speed : INFINITY);
+      last_seg[v] = s.last[v];
+      origin_seg[v] = -1;
+    }
+    // origin segment: the first edge of each tree path, propagated in settle order
+    for (uint32_t i = 0; i < k; ++i) {
+      const uint32_t v = ord[i];
+      if (v == src || s.pred[v] < 0) continue;
+      origin_seg[v] = (uint32_t)s.pred[v] == src ? s.last[v] : origin_seg[s.pred[v]];
+    }
+    // visited edges: in-edges of each settled node whose source had not been
+    // settled yet, in settle order and petgraph order (centrality.rs:1553-1577)
+    uint32_t ne = 0;
+    std::vector<uint32_t> pos(N, UINT32_MAX);
+    for (uint32_t i = 0; i < k; ++i) pos[ord[i]] = i;
+    for (uint32_t i = 0; i < k; ++i) {
+      const uint32_t v = ord[i];
+      for (uint32_t j = g.rev_off[v]; j < g.rev_off[v + 1]; ++j) {
+        const uint32_t e = g.rev_eid[j];
+        const uint32_t nb = g.esrc[e];
+        if (nb != v && pos[nb] != UINT32_MAX && pos[nb] < i) continue;
+        if (edge_order) edge_order[ne] = e;
+        ++ne;
+      }
+    }
+    if (order) std::copy(ord.begin(), ord.begin() + k, order);
+    if (n_order) *n_order = k;
+    if (n_edges) *n_edges = ne;
+  });
+}
+
+int cs_brandes_shortest(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, float tolerance,
+                        float *secs, float *route, double *sigma, uint32_t *n_preds, uint32_t *preds,
+                        uint32_t max_preds) {
+  return guarded([&] {
+    if (!h) fail(CS_EINVAL, "graph must not be NULL");
+    if (tolerance > kTieEps)
+      fail(CS_EINVAL, "tolerance above the internal 0.01% tie band is not supported by the GPU search yet");
+    Single s = run_single(h, src, max_seconds, speed, tolerance, 1, max_preds);
+    const uint32_t N = h->g.n;
+    for (uint32_t v = 0; v < N; ++v) {
+      secs[v] = s.secs[v];
+      route[v] = std::isfinite(s.secs[v]) ? s.secs[v] * speed : INFINITY;
+      sigma[v] = s.sigma[v];
+      n_preds[v] = s.npreds[v];
+    }
+    if (max_preds) std::copy(s.preds.begin(), s.preds.end(), preds);
+  });
+}
+
+int cs_tree_simplest(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, uint8_t *visited,
+                     uint8_t *discovered, int32_t *pred, float *simpl_dist, float *agg_seconds, uint32_t *order,
+                     uint32_t *n_order) {
+  return guarded([&] {
+    if (!h) fail(CS_EINVAL, "graph must not be NULL");
+    if (!h->g.is_dual)
+      fail(CS_EINVAL,
+           "dijkstra_tree_simplest requires a dual graph for angular analysis. Convert the graph with "
+           "cityseer.tools.graphs.nx_to_dual(...) before ingesting it into NetworkStructure.");
+    fail(CS_EINVAL, "dijkstra_tree_simplest is not available on the GPU path yet");
+    (void)src;
+    (void)max_seconds;
+    (void)speed;
+    (void)visited;
+    (void)discovered;
+    (void)pred;
+    (void)simpl_dist;
+    (void)agg_seconds;
+    (void)order;
+    (void)n_order;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+// cs_internal.hpp -- host-side graph representation and kernel launch
+// interfaces shared by the host library (cs_host.cpp) and the HIP kernels
+// (cs_search.hip, cs_angular.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cityseer_amd.h"
+
+namespace cs {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;  // 4 independent waves per workgroup
+constexpr uint32_t kMaxT = CS_MAX_THRESHOLDS;
+constexpr uint32_t kNodeMask = 0x7fffffffu;
+constexpr uint32_t kCanonBit = 0x80000000u;
+constexpr float kTieEps = 1e-4f;  // centrality.rs:28
+constexpr float kAngularTieBreak = 1e-6f;  // centrality.rs:24
+
+// Thrown by host code; converted to a status + thread-local message at the C ABI.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string &msg);
+void hip_check(hipError_t e, const char *what);
+
+// Speed-dependent device edge arrays for one (device, speed).
+struct DevEdges {
+  float speed = 0.0f;
+  // shortest / segment (reverse search over in-edges, fold over out-edges)
+  uint2 *rev_adj = nullptr;     // {nb | canon<<31, bits(seconds of nb->v)}
+  uint2 *fwd_adj = nullptr;     // {u  | canon<<31, bits(seconds of v->u)}
+  float2 *rev_tw = nullptr;     // {twin length, twin imp} per rev position (segment)
+  float *fwd_twlen = nullptr;   // twin length per fwd position (segment)
+  uint32_t *rev_eid = nullptr;  // edge id per rev position (trees)
+  uint32_t *fwd_eid = nullptr;  // edge id per fwd position (trees)
+  float min_es = 0.0f;          // min seconds over non-loop edges (use_impedance)
+  float max_es = 0.0f;
+  // angular (forward search over out-edges on the 2-state dual)
+  uint4 *ang_adj = nullptr;  // {v | srcslot<<30 | dstslot<<31 ... see cs_angular.hip}
+  float min_es_ang = 0.0f;
+};
+
+struct DevGraph {
+  int device = -1;
+  uint32_t *rev_off = nullptr;
+  uint32_t *fwd_off = nullptr;
+  std::vector<std::unique_ptr<DevEdges>> edges;  // small cache keyed by speed
+};
+
+struct Graph {
+  uint32_t n = 0, m = 0;
+  bool is_dual = false;
+  std::vector<double> z;
+  std::vector<uint8_t> live;
+  std::vector<float> weight;
+  std::vector<uint32_t> esrc, edst, ekey;
+  std::vector<float> elen, eimp, eang, esec;
+  std::vector<int32_t> epk;
+  // topology in petgraph adjacency order (newest edge first)
+  std::vector<uint32_t> rev_off, rev_eid;  // in-edges per node
+  std::vector<uint32_t> fwd_off, fwd_eid;  // out-edges per node
+  std::vector<int32_t> twin;                // edges_connecting(dst, src).find(key)
+  std::vector<uint8_t> canon;               // first edge of its (min,max,key) triple
+  std::vector<int8_t> slot_src, slot_dst;   // dual endpoint slots (centrality.rs:533-566)
+  bool slots_ok = false;
+  std::string slots_err;
+  uint32_t max_out_deg = 0, max_in_deg = 0;
+  bool has_loops = false;
+
+  std::mutex mu;
+  std::map<int, std::unique_ptr<DevGraph>> dev;
+  uint64_t progress = 0;
+};
+
+// Host edge travel seconds (centrality.rs:970-1007).
+float edge_seconds(const Graph &g, uint32_t from, uint32_t to, uint32_t e, float speed,
+                   bool use_impedance);
+
+// Device residency (cs_host.cpp).
+DevGraph &device_graph(Graph &g, int device);
+DevEdges &device_edges(Graph &g, DevGraph &dg, float speed);
+
+// Per-call statistics (thread-local).
+cs_stats &thread_stats();
+
+// ----------------------------------------------------------------------
+// Kernel launchers (cs_search.hip)
+// ----------------------------------------------------------------------
+
+struct CallSpec {
+  uint32_t T = 0;
+  float dist[kMaxT] = {0};
+  float beta[kMaxT] = {0};
+  uint32_t seconds[kMaxT] = {0};
+  float maxsec = 0.0f;
+  float speed = 0.0f;
+  float tolerance = 1e-4f;
+  bool closeness = true, betweenness = true, track_reach = false;
+  float ang_unit = 180.0f, far_offset = 1.0f;
+};
+
+// Shortest-path centrality over `n_sources` sources; acc is a device array
+// [N][7][T] (node-major) that the call accumulates into; reach [T] u32.
+void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
+                     const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
+                     uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
+                     uint32_t *d_reach, hipStream_t stream, cs_stats &st);
+
+// Segment centrality; acc [N][4][T].
+void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
+                    const uint32_t *d_sources, uint32_t n_sources, double *d_acc,
+                    hipStream_t stream, cs_stats &st);
+
+// Single-source outputs (trees and Brandes state), device arrays of length N.
+struct TreeOut {
+  float *secs = nullptr;       // agg seconds (INF = unreached)
+  int32_t *pred = nullptr;     // pred node (-1)
+  int32_t *last_eid = nullptr; // edge id of v->pred (-1)
+  int32_t *origin_eid = nullptr;
+  double *sigma = nullptr;     // Brandes sigma
+  uint32_t *pred_list = nullptr;  // Brandes preds [N * max_preds]
+  uint32_t *n_preds = nullptr;
+  uint32_t max_preds = 0;
+};
+// mode 0: single-predecessor tree, 1: Brandes multi-predecessor state.
+void launch_single(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec,
+                   float tolerance, int mode, const TreeOut &out, hipStream_t stream);
+
+// Angular (simplest) centrality (cs_angular.hip); acc [N][4][T].
+void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
+                     const uint32_t *d_sources, const float *d_wt, uint32_t n_sources,
+                     const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
+                     hipStream_t stream, cs_stats &st);
+
+}  // namespace cs

@@ -1,0 +1,190 @@
+/*
+ * cityseer_amd.h -- C ABI of the MI355X implementation of cityseer's localised
+ * network-centrality path.
+ *
+ * Plain pointers and sizes only. Every compute entry point runs hand-written
+ * HIP kernels on the current HIP device; there is no CPU fallback. Functions
+ * return CS_OK (0) or a CS_E* status; cs_last_error() returns the calling
+ * thread's last message, whose fragments match the reference's ValueError
+ * messages where one exists.
+ *
+ * Each declaration names the reference interface it replaces (file:line in the
+ * cityseer repository).
+ */
+#ifndef CITYSEER_AMD_H
+#define CITYSEER_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_OK 0
+#define CS_EINVAL 1    /* argument error; the Python layer raises ValueError */
+#define CS_EDEVICE 2   /* no usable HIP device, or a HIP runtime error */
+#define CS_ENOMEM 3    /* device memory exhausted (or HBM budget exceeded) */
+#define CS_EINTERNAL 4 /* internal invariant violated */
+#define CS_EBUSY 5     /* serving queue full / request not finished */
+
+/* Last error message of the calling thread (empty string if none). */
+const char *cs_last_error(void);
+
+/* Library version string. */
+const char *cs_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Threshold conversion                                                   */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces rustalgos.pair_distances_betas_time (rust/src/common.rs:241-270).
+ * mode 0: distances given; 1: betas given; 2: minutes given. Each list has n
+ * entries; outputs receive n distances, betas and seconds. */
+int cs_pair_distances_betas_time(float speed_m_s, int mode, const uint32_t *distances,
+                                 const float *betas, const float *minutes, uint32_t n,
+                                 float min_threshold_wt, uint32_t *out_distances,
+                                 float *out_betas, uint32_t *out_seconds);
+
+/* Replaces rustalgos.distances_from_betas (rust/src/common.rs:98-131). */
+int cs_distances_from_betas(const float *betas, uint32_t n, float min_threshold_wt,
+                            uint32_t *out_distances);
+/* Replaces rustalgos.betas_from_distances (rust/src/common.rs:133-165). */
+int cs_betas_from_distances(const uint32_t *distances, uint32_t n, float min_threshold_wt,
+                            float *out_betas);
+/* Replaces rustalgos.distances_from_seconds (rust/src/common.rs:167-201). */
+int cs_distances_from_seconds(const uint32_t *seconds, uint32_t n, float speed_m_s,
+                              uint32_t *out_distances);
+/* Replaces rustalgos.seconds_from_distances (rust/src/common.rs:203-237). */
+int cs_seconds_from_distances(const uint32_t *distances, uint32_t n, float speed_m_s,
+                              uint32_t *out_seconds);
+
+/* ---------------------------------------------------------------------- */
+/* Graph                                                                  */
+/* ---------------------------------------------------------------------- */
+
+typedef struct cs_graph cs_graph;
+
+/* Replaces the NetworkStructure storage built by add_street_node /
+ * add_street_edge (rust/src/graph.rs:384-452,728-889). Nodes 0..n_nodes-1;
+ * directed edges in insertion order (edge id = position). z: NaN = no
+ * elevation (NULL = none). seconds: NaN = derive from length (NULL = all).
+ * primal_key: dual edges' shared primal node id, -1 = none (NULL = none).
+ * imp_factor / angle_sum NULL = 1.0 / 0.0. The handle is immutable after
+ * creation; concurrent calls on one handle are safe. */
+int cs_graph_create(uint32_t n_nodes, const double *z, const uint8_t *live, const float *weight,
+                    uint32_t n_edges, const uint32_t *src, const uint32_t *dst,
+                    const uint32_t *edge_key, const float *length, const float *imp_factor,
+                    const float *angle_sum, const float *seconds, const int32_t *primal_key,
+                    int is_dual, cs_graph **out);
+void cs_graph_destroy(cs_graph *g);
+
+/* Number of nodes/edges; maximum out/in degree. */
+int cs_graph_info(const cs_graph *g, uint32_t *n_nodes, uint32_t *n_edges, uint32_t *max_out_deg,
+                  uint32_t *max_in_deg);
+
+/* Progress counter: sources finished by the last or running call on this
+ * graph. Replaces NetworkStructure.progress (rust/src/graph.rs:407-415). */
+uint64_t cs_graph_progress(const cs_graph *g);
+
+/* ---------------------------------------------------------------------- */
+/* Centrality                                                             */
+/* ---------------------------------------------------------------------- */
+
+#define CS_MAX_THRESHOLDS 8
+
+typedef struct cs_params {
+  uint32_t n_thresholds;     /* 1..CS_MAX_THRESHOLDS */
+  const uint32_t *distances; /* [T] strictly increasing */
+  const float *betas;        /* [T] */
+  const uint32_t *seconds;   /* [T] */
+  float speed_m_s;
+  float tolerance;           /* decimal (user percent / 100), clamped >= 1e-4 */
+  int compute_closeness;
+  int compute_betweenness;
+  uint32_t n_sources;
+  const uint32_t *sources;   /* [n_sources] node indices */
+  const float *source_wt;    /* [n_sources] weight per source (NaN = skip) */
+  const uint8_t *eligible;   /* [N] source-eligible mask */
+  int track_reach;           /* fill reach_totals */
+  double betw_scale;         /* post-hoc betweenness scale (1.0 = none) */
+  float angular_scaling_unit;
+  float farness_scaling_offset;
+} cs_params;
+
+/* Replaces NetworkStructure.centrality_shortest (rust/src/centrality.rs:1624-1874).
+ * out: f64 [7][T][N] = density, farness, cycles, harmonic, beta, betweenness,
+ * betweenness_beta. reach_totals: u32 [T] (may be NULL). */
+int cs_centrality_shortest(cs_graph *g, const cs_params *p, double *out, uint32_t *reach_totals);
+
+/* Device-resident form of cs_centrality_shortest, for callers that keep inputs
+ * and outputs in HBM (batching, serving, benchmarks). p->distances/betas/
+ * seconds are host arrays; p->sources, p->source_wt and p->eligible are device
+ * arrays, as is d_cycles_wt [n_sources] (source weight / node weight). The call
+ * adds into d_acc, f64 [N][7][T] node-major (zeroed by the caller), and d_reach
+ * u32 [T] (may be NULL); betw_scale is not applied. stream may be NULL (the
+ * null stream). Same computation as centrality_shortest
+ * (rust/src/centrality.rs:1624-1874) without host copies. */
+int cs_centrality_shortest_device(cs_graph *g, const cs_params *p, const float *d_cycles_wt, double *d_acc,
+                                  uint32_t *d_reach, void *stream);
+
+/* Replaces NetworkStructure.centrality_simplest (rust/src/centrality.rs:1876-2132).
+ * out: f64 [4][T][N] = density, farness, harmonic, betweenness. */
+int cs_centrality_simplest(cs_graph *g, const cs_params *p, double *out, uint32_t *reach_totals);
+
+/* Replaces NetworkStructure.segment_centrality (rust/src/centrality.rs:2134-2407).
+ * out: f64 [4][T][N] = segment density, harmonic, beta, betweenness. */
+int cs_segment_centrality(cs_graph *g, const cs_params *p, double *out);
+
+/* ---------------------------------------------------------------------- */
+/* Single-source trees                                                    */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces NetworkStructure.dijkstra_tree_shortest (rust/src/centrality.rs:1499-1508).
+ * Arrays of length N; pred = -1 for none; order receives the settle order. */
+int cs_tree_shortest(cs_graph *g, uint32_t src, uint32_t max_seconds, float speed_m_s,
+                     uint8_t *visited, uint8_t *discovered, int32_t *pred, float *short_dist,
+                     float *agg_seconds, uint32_t *order, uint32_t *n_order);
+
+/* Replaces NetworkStructure.dijkstra_tree_simplest (rust/src/centrality.rs:1510-1521). */
+int cs_tree_simplest(cs_graph *g, uint32_t src, uint32_t max_seconds, float speed_m_s,
+                     uint8_t *visited, uint8_t *discovered, int32_t *pred, float *simpl_dist,
+                     float *agg_seconds, uint32_t *order, uint32_t *n_order);
+
+/* Replaces NetworkStructure.dijkstra_tree_segment (rust/src/centrality.rs:1523-1611).
+ * origin_seg / last_seg hold edge ids (-1 = none); edge_order receives the
+ * visited edges in visit order. */
+int cs_tree_segment(cs_graph *g, uint32_t src, uint32_t max_seconds, float speed_m_s,
+                    uint8_t *visited, uint8_t *discovered, int32_t *pred, float *short_dist,
+                    float *agg_seconds, int32_t *origin_seg, int32_t *last_seg, uint32_t *order,
+                    uint32_t *n_order, uint32_t *edge_order, uint32_t *n_edges);
+
+/* Multi-predecessor Brandes state of one shortest-path source, the state the
+ * reference builds in dijkstra_brandes_shortest (rust/src/centrality.rs:1344-1494):
+ * secs/route f32 [N], sigma f64 [N], n_preds u32 [N], preds u32 [N*max_preds]
+ * (predecessor node ids in insertion order). Exposed for parity checks. */
+int cs_brandes_shortest(cs_graph *g, uint32_t src, uint32_t max_seconds, float speed_m_s,
+                        float tolerance, float *secs, float *route, double *sigma,
+                        uint32_t *n_preds, uint32_t *preds, uint32_t max_preds);
+
+/* ---------------------------------------------------------------------- */
+/* Measurement                                                            */
+/* ---------------------------------------------------------------------- */
+
+typedef struct cs_stats {
+  double kernel_ms;        /* HIP-event time of the last call's search kernel(s) */
+  double total_ms;         /* host wall time of the last call */
+  uint64_t sources;        /* sources searched */
+  uint64_t nodes_reached;  /* sum over sources of R_s */
+  uint64_t edges_scanned;  /* sum over sources of traversed in-edges E_s */
+  uint64_t within_sum;     /* sum over sources and thresholds of R_{s,i} */
+  uint32_t waves;          /* persistent waves launched */
+  uint32_t retries;        /* sources re-run with larger workspaces */
+} cs_stats;
+
+/* Statistics of the calling thread's last compute call. */
+int cs_last_stats(cs_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,346 @@
+"""Pin the CPU oracle before trusting it (CPU only).
+
+Checks, per oracle function:
+- thresholds: the reference's expected values (test_common.py:32-186) and the
+  f32 arithmetic restated independently in numpy;
+- shortest closeness: diamond closed forms (test_centrality.py:451-510),
+  networkx harmonic closeness and a brute-force rederivation from networkx
+  distances including a brute-force circuit rank (test_centrality.py:247-341),
+  node-weight scaling (test_centrality.py:342-389);
+- trees: networkx shortest paths and distances (test_centrality.py:129-163);
+- betweenness: networkx betweenness_centrality(normalized=False)
+  (test_centrality.py:652-674) and the tolerance-drift graph (:893-930);
+- simplest: diamond angular harmonic (:519-535), plateau ratio 1.8 (:872-890),
+  node-order invariance (:710-772), expected dual paths (:166-234);
+- segment: diamond closed forms (:536-598) and decomposition invariance (:622-649);
+- heap order: the keyed order the GPU reproduces equals Rust's BinaryHeap
+  order on tie-free inputs (bit-identical outputs).
+"""
+
+from __future__ import annotations
+
+import math
+
+import networkx as nx
+import numpy as np
+import pytest
+
+from conftest import graph_from
+from cityseer import config
+from cityseer.tools import graphs, io, mock
+import oracle.oracle as O
+from oracle.oracle import OracleGraph
+
+ATOL, RTOL = config.ATOL, config.RTOL
+SPEED = config.SPEED_M_S
+
+
+@pytest.fixture(autouse=True)
+def _keyed_heap():
+    O.set_heap_mode(False)
+    yield
+    O.set_heap_mode(False)
+
+
+def _og(g):
+    _n, _e, ns = io.network_structure_from_nx(g)
+    return OracleGraph.from_network_structure(ns), ns, _n
+
+
+def _roundtrip_lengths(g):
+    """networkX copy with each edge's f32 cityseer length as weight."""
+    h = nx.MultiGraph()
+    for n, d in g.nodes(data=True):
+        h.add_node(n, **d)
+    for s, e, k, d in g.edges(keys=True, data=True):
+        h.add_edge(s, e, k, length=float(np.float32(d["geom"].length)))
+    return h
+
+
+# ---------------------------------------------------------------- thresholds
+def test_thresholds_reference_values(golden):
+    t = golden["thresholds"]
+    for betas, d in t["distances_from_betas"]:
+        assert O.thresholds(SPEED, betas=betas)[0] == [d]
+    for betas, mtw, d in t["distances_from_betas_mtw"]:
+        assert np.allclose(O.thresholds(SPEED, betas=betas, min_threshold_wt=mtw)[0], d, atol=ATOL, rtol=RTOL)
+    for dists, b in t["betas_from_distances"]:
+        assert np.allclose(O.thresholds(SPEED, distances=dists)[1], [b], atol=ATOL, rtol=RTOL)
+    for dists, mtw, b in t["betas_from_distances_mtw"]:
+        assert np.allclose(O.thresholds(SPEED, distances=dists, min_threshold_wt=mtw)[1], [b], atol=ATOL, rtol=RTOL)
+    ds = t["distances_from_seconds"]
+    assert O.thresholds(SPEED, minutes=ds["minutes"])[0] == ds["distances"]
+    with pytest.raises(ValueError):
+        O.thresholds(SPEED, distances=[100, 100])
+    with pytest.raises(ValueError):
+        O.thresholds(SPEED, betas=[0.0025, 0.04])
+
+
+def test_thresholds_f32_arithmetic():
+    """beta = round5(-ln(f32 mtw) / d) and seconds = round(d / speed) in f32 with half-away rounding."""
+    mtw = np.float32(0.01831563888873418)
+    ln = np.float32(np.log(np.float64(mtw)))  # exact here: ln(f32 mtw) rounds to -4.0
+    assert ln == np.float32(-4.0)
+    dists = list(range(50, 20001, 50))
+    d, b, s = O.thresholds(SPEED, distances=dists)
+    for di, bi, si in zip(dists, b, s, strict=True):
+        beta = np.float32(-ln / np.float32(di))
+        x = np.float32(beta * np.float32(100000.0))
+        r = np.float32(math.floor(x + 0.5) if x >= 0 else math.ceil(x - 0.5))
+        assert np.float32(bi) == np.float32(r / np.float32(100000.0)), di
+        t = np.float32(np.float32(di) / np.float32(SPEED))
+        assert si == int(math.floor(t + 0.5)), di
+    # 6400 m: 62.5 rounds away from zero (half-to-even would give 0.00062)
+    assert np.isclose(O.thresholds(SPEED, distances=[6400])[1][0], 0.00063, atol=1e-9)
+
+
+# ---------------------------------------------------------------- diamond (reference closed forms)
+@pytest.mark.parametrize("rust_heap", [True, False])
+def test_diamond_closed_forms(diamond_graph, golden, rust_heap):
+    O.set_heap_mode(rust_heap)
+    gold = golden["diamond"]
+    og, ns, _ = _og(diamond_graph)
+    out, _ = og.centrality_shortest(gold["distances"], betweenness=False)
+    for m, name in enumerate(["node_density", "node_farness", "node_cycles", "node_harmonic", "node_beta"]):
+        for i, d in enumerate(gold["distances"]):
+            assert np.allclose(out[m, i], gold[name][str(d)], atol=ATOL, rtol=RTOL), (name, d)
+    seg = og.segment_centrality(gold["distances"])
+    for m, name in enumerate(["segment_density", "segment_harmonic", "segment_beta"]):
+        for i, d in enumerate(gold["distances"]):
+            assert np.allclose(seg[m, i], gold[name][str(d)], atol=ATOL, rtol=RTOL), (name, d)
+    for i, d in enumerate(gold["distances"]):
+        expect = np.asarray(gold["segment_betweenness"][str(d)])
+        if rust_heap:
+            # exact tie between the routes via nodes 1 and 2: Rust's heap settles node 2 first
+            assert np.allclose(seg[3, i], expect, atol=ATOL, rtol=RTOL)
+        else:
+            # keyed order settles node 1 first: the same value lands on the mirror node
+            assert np.allclose(seg[3, i], expect[[0, 2, 1, 3]], atol=ATOL, rtol=RTOL)
+    dual = graphs.nx_to_dual(diamond_graph)
+    ogd, nsd, nd = _og(dual)
+    assert list(nd.index) == ["0_1_k0", "0_2_k0", "1_2_k0", "1_3_k0", "2_3_k0"]
+    outd, _ = ogd.centrality_simplest(gold["distances"], betweenness=False)
+    for i, d in enumerate(gold["distances"]):
+        assert np.allclose(outd[2, i], gold["simplest_harmonic"][str(d)], atol=ATOL, rtol=RTOL)
+
+
+def test_diamond_beta_exact_closed_form(diamond_graph):
+    """Node beta with the rounded betas the current reference computes: sum of exp(-beta * d)."""
+    og, _ns, _ = _og(diamond_graph)
+    out, _ = og.centrality_shortest([150], betweenness=False)
+    beta = np.float32(0.02667)
+    side = np.float32(np.float32(math.hypot(50.0, 86.60254)) / np.float32(SPEED)) * np.float32(SPEED)
+    term = float(np.exp(np.float32(-beta * side)))
+    assert np.isclose(out[4, 0, 0], 2 * term, rtol=1e-6)
+    assert np.isclose(out[4, 0, 1], 3 * term, rtol=1e-6)
+
+
+# ---------------------------------------------------------------- mock graph vs networkx
+def test_tree_shortest_vs_networkx(primal_graph):
+    O.set_heap_mode(True)
+    og, ns, _ = _og(primal_graph)
+    G = _roundtrip_lengths(primal_graph)
+    for max_dist in [0, 500, 2000, 5000]:
+        max_seconds = int(max_dist / SPEED)
+        for src in range(len(primal_graph)):
+            t = og.tree_shortest(src, max_seconds)
+            nx_dist, nx_path = nx.single_source_dijkstra(G, str(src), weight="length", cutoff=max_dist)
+            for key, path in nx_path.items():
+                j = int(key)
+                walk = [j]
+                while walk[-1] != src:
+                    walk.append(int(t["pred"][walk[-1]]))
+                assert list(reversed(walk)) == [int(k) for k in path]
+                assert abs(t["dist"][j] - nx_dist[key]) < ATOL
+
+
+def test_closeness_vs_networkx_and_brute_force(primal_graph):
+    og, ns, _ = _og(primal_graph)
+    G = _roundtrip_lengths(primal_graph)
+    betas = [0.02, 0.01, 0.005, 0.0008]
+    distances = O.thresholds(SPEED, betas=betas)[0]
+    out, _ = og.centrality_shortest(distances, betweenness=False)
+    d_last = out[0, -1]
+    assert set(np.unique(d_last).tolist()) <= {49.0, 3.0, 1.0, 0.0}
+    harm = nx.harmonic_centrality(G, distance="length")
+    for v in range(len(G)):
+        assert abs(harm[str(v)] - out[3, -1, v]) < ATOL
+    n = len(G)
+    dens = np.zeros((4, n))
+    far = np.zeros((4, n))
+    harm_bf = np.zeros((4, n))
+    grav = np.zeros((4, n))
+    cyc = np.zeros((4, n))
+    _, _betas, _ = O.thresholds(SPEED, distances=distances)
+    for src in range(n):
+        dists = nx.single_source_dijkstra_path_length(G, str(src), weight="length")
+        score = np.zeros(4)
+        for i, cut in enumerate(distances):
+            sub = {k for k, d in dists.items() if d <= cut}
+            eids = set()
+            for u, v, k in G.edges(keys=True):
+                if u != v and u in sub and v in sub:
+                    eids.add(tuple(sorted((u, v))) + (k,))
+            if sub:
+                score[i] = max(0, len(eids) - len(sub) + 1)
+        for key, d in dists.items():
+            t = int(key)
+            if t == src:
+                continue
+            for i, cut in enumerate(distances):
+                if d <= cut:
+                    dens[i, t] += 1
+                    far[i, t] += d
+                    harm_bf[i, t] += 1 / d
+                    grav[i, t] += math.exp(-_betas[i] * d)
+                    cyc[i, t] += score[i]
+    for i in range(4):
+        assert np.allclose(out[0, i], dens[i], atol=ATOL, rtol=RTOL)
+        assert np.allclose(out[1, i], far[i], atol=ATOL, rtol=RTOL)
+        assert np.allclose(out[2, i], cyc[i], atol=ATOL, rtol=RTOL)
+        assert np.allclose(out[3, i], harm_bf[i], atol=ATOL, rtol=RTOL)
+        assert np.allclose(out[4, i], grav[i], atol=ATOL, rtol=RTOL)
+
+
+@pytest.mark.parametrize("wt", [0.5, 2.0])
+def test_closeness_weight_scaling(primal_graph, wt):
+    og, ns, _ = _og(primal_graph)
+    base, _ = og.centrality_shortest([200, 800, 5000], betweenness=False)
+    g2 = primal_graph.copy()
+    for n in g2:
+        g2.nodes[n]["weight"] = wt
+    og2, _ns2, _ = _og(g2)
+    scaled, _ = og2.centrality_shortest([200, 800, 5000], betweenness=False)
+    for m in (0, 1, 3, 4):
+        assert np.allclose(base[m] * wt, scaled[m], atol=ATOL, rtol=RTOL)
+    assert np.allclose(base[2], scaled[2], atol=ATOL, rtol=RTOL)  # circuit rank ignores weights
+
+
+def test_betweenness_vs_networkx(primal_graph):
+    og, ns, _ = _og(primal_graph)
+    G = _roundtrip_lengths(primal_graph)
+    out, _ = og.centrality_shortest([5000], closeness=False)
+    nxb = nx.betweenness_centrality(G, normalized=False, weight="length")
+    for v in range(len(G)):
+        assert abs(nxb[str(v)] - out[5, 0, v]) < ATOL
+
+
+def test_tolerance_drift(golden):
+    g = graph_from(golden["tolerance_drift"]["nodes"], golden["tolerance_drift"]["edges"])
+    og, ns, nodes = _og(g)
+    idx = {k: i for i, k in enumerate(nodes.index)}
+    src = np.array([idx["S"]], dtype=np.uint32)
+    elig = np.zeros(len(idx), np.uint8)
+    elig[idx["S"]] = 1
+    n_live = len(idx)
+    exact, _ = og.centrality_shortest([20], tolerance=1e-4, closeness=False, sources=src, eligible=elig,
+                                      track=True, scale=n_live / 1)
+    tol, _ = og.centrality_shortest([20], tolerance=0.1, closeness=False, sources=src, eligible=elig, track=True,
+                                    scale=n_live / 1)
+    e = {k: exact[5, 0, i] for k, i in idx.items()}
+    t = {k: tol[5, 0, i] for k, i in idx.items()}
+    assert np.isclose(e["A"], 0.0, atol=ATOL) and np.isclose(e["B"], 0.0, atol=ATOL) and e["C"] > 0
+    assert np.isclose(t["A"], 0.0, atol=ATOL) and t["B"] > 0 and t["C"] > 0
+
+
+# ---------------------------------------------------------------- simplest
+def test_simplest_plateau_ratio(golden):
+    O.set_heap_mode(True)
+    g = graph_from(golden["plateau"]["nodes"], golden["plateau"]["edges"])
+    og, ns, nodes = _og(graphs.nx_to_dual(g))
+    out, _ = og.centrality_simplest([1000], closeness=False)
+    b = {k: out[3, 0, i] for i, k in enumerate(nodes.index)}
+    assert np.isclose(b["B_C_k0"] / b["C_D_k0"], golden["plateau"]["ratio"], atol=1e-6)
+
+
+def test_simplest_order_invariance():
+    base = {"A": (500000.0, 0.0), "B": (500000.0, 100.0), "C": (500100.0, 100.0), "D": (500050.0, 170.0)}
+    base_edges = [("A", "B"), ("B", "C"), ("B", "D")]
+    results = []
+    for ordering in (["A", "B", "C", "D"], ["D", "C", "B", "A"], ["C", "A", "D", "B"]):
+        lab = {k: str(i) for i, k in enumerate(ordering)}
+        inv = {v: k for k, v in lab.items()}
+        g = graph_from({lab[k]: base[k] for k in ordering}, [(lab[a], lab[b]) for a, b in base_edges])
+        og, ns, nodes = _og(graphs.nx_to_dual(g))
+        out, _ = og.centrality_simplest([500], closeness=False)
+        by_edge = {}
+        for pos, key in enumerate(nodes.index):
+            row = nodes.loc[key]
+            by_edge[tuple(sorted((inv[row["primal_edge_node_a"]], inv[row["primal_edge_node_b"]])))] = out[3, 0, pos]
+        results.append(by_edge)
+    for e in [("A", "B"), ("B", "C"), ("B", "D")]:
+        vals = [r[e] for r in results]
+        assert all(abs(v - vals[0]) < ATOL for v in vals)
+
+
+def test_simplest_dual_paths(dual_graph):
+    O.set_heap_mode(True)
+    og, ns, nodes = _og(dual_graph)
+    keys = nodes.index.tolist()
+    max_s = int(5000 / SPEED)
+
+    def path(t, src, tgt):
+        walk = [tgt]
+        while walk[-1] != src:
+            walk.append(int(t["pred"][walk[-1]]))
+        return [keys[i] for i in reversed(walk)]
+
+    src, tgt = keys.index("11_6_k0"), keys.index("39_40_k0")
+    assert path(og.tree_simplest(src, max_s), src, tgt) == [
+        "11_6_k0", "11_14_k0", "10_14_k0", "10_43_k0", "43_44_k0", "40_44_k0", "39_40_k0"]
+    assert path(og.tree_shortest(src, max_s), src, tgt) == [
+        "11_6_k0", "6_7_k0", "3_7_k0", "3_4_k0", "1_4_k0", "0_1_k0", "0_31_k0", "31_32_k0", "32_34_k0",
+        "34_37_k0", "37_39_k0", "39_40_k0"]
+    src, tgt = keys.index("10_43_k0"), keys.index("10_5_k0")
+    assert path(og.tree_simplest(src, max_s), src, tgt) == ["10_43_k0", "10_5_k0"]
+
+
+def test_simplest_differs_from_shortest(dual_graph):
+    og, ns, _ = _og(dual_graph)
+    sh, _ = og.centrality_shortest([5000], betweenness=False)
+    si, _ = og.centrality_simplest([5000], betweenness=False)
+    assert np.allclose(sh[0, 0], si[0, 0], atol=ATOL, rtol=RTOL)
+    assert not np.allclose(sh[1, 0], si[1, 0], atol=ATOL)
+    assert not np.allclose(sh[3, 0], si[2, 0], atol=ATOL)
+    shb, _ = og.centrality_shortest([500, 2000], closeness=False)
+    sib, _ = og.centrality_simplest([500, 2000], closeness=False)
+    for i in range(2):
+        assert not np.allclose(shb[5, i], sib[3, i], atol=ATOL)
+
+
+# ---------------------------------------------------------------- segment
+def test_segment_decomposition_invariance(primal_graph):
+    og, ns, _ = _og(primal_graph)
+    ogd, nsd, _ = _og(graphs.nx_decompose(primal_graph, 20))
+    a = og.segment_centrality([200, 400, 800, 5000])
+    b = ogd.segment_centrality([200, 400, 800, 5000])
+    for m in range(3):
+        assert np.allclose(a[m, 1].sum(), b[m, 1][:57].sum())
+
+
+# ---------------------------------------------------------------- heap order
+def test_keyed_order_equals_rust_heap_on_tie_free_graph():
+    lat = mock.perturbed_lattice(14, seed=11)
+    ns = mock.network_structure_from_lattice(lat)
+    og = OracleGraph.from_network_structure(ns)
+    O.set_heap_mode(False)
+    k_out, _ = og.centrality_shortest([300, 700, 1200])
+    k_seg = og.segment_centrality([300, 700, 1200])
+    k_tr = [og.brandes_shortest(s, 900) for s in (0, 17, 101)]
+    O.set_heap_mode(True)
+    r_out, _ = og.centrality_shortest([300, 700, 1200])
+    r_seg = og.segment_centrality([300, 700, 1200])
+    r_tr = [og.brandes_shortest(s, 900) for s in (0, 17, 101)]
+    assert np.array_equal(k_out, r_out)
+    assert np.array_equal(k_seg, r_seg)
+    for a, b in zip(k_tr, r_tr, strict=True):
+        assert np.array_equal(a["sigma"], b["sigma"]) and a["preds"] == b["preds"]
+        assert np.array_equal(a["order"], b["order"])
+    dual = mock.lattice_dual(mock.perturbed_lattice(10, seed=5))
+    nsd = mock.network_structure_from_lattice(dual)
+    ogd = OracleGraph.from_network_structure(nsd)
+    O.set_heap_mode(False)
+    k_ang, _ = ogd.centrality_simplest([400, 900])
+    O.set_heap_mode(True)
+    r_ang, _ = ogd.centrality_simplest([400, 900])
+    assert np.array_equal(k_ang, r_ang)
