@@ -31,7 +31,6 @@ namespace cs {
 namespace {
 
 constexpr uint32_t kFailCap = 1u, kFailBucket = 2u, kFailRing = 4u, kFailInternal = 8u;
-constexpr int kMaxCand = 8;  // register-resident fold candidates
 
 enum Mode { kBrandes = 0, kSegment = 1, kTree = 2, kBrandesExport = 3 };
 
@@ -69,11 +68,13 @@ struct SearchArgs {
   TreeOut tout;
 };
 
+// Wave-private workspace: only the owning wave reads or writes it, so
+// workgroup scope (coherent in the CU's L1 and the XCD's L2) is sufficient.
 __device__ __forceinline__ uint64_t ld64(const uint64_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ uint32_t ld32(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ float ldf(const float *p) {
   return __uint_as_float(ld32(reinterpret_cast<const uint32_t *>(p)));
@@ -82,14 +83,25 @@ __device__ __forceinline__ double ldd(const double *p) {
   return __longlong_as_double((long long)ld64(reinterpret_cast<const uint64_t *>(p)));
 }
 __device__ __forceinline__ void st32(uint32_t *p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void st64(uint64_t *p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void std_(double *p, double v) {
   st64(reinterpret_cast<uint64_t *>(p), (uint64_t)__double_as_longlong(v));
 }
+__device__ __forceinline__ uint64_t wmax64(uint64_t *p, uint64_t v) {
+  return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t wadd32(uint32_t *p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void wadd_f64(double *p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// Results shared by all waves (accumulators): device scope.
+__device__ __forceinline__ void gadd_f64(double *p, double v) { unsafeAtomicAdd(p, v); }
 // Orders this wave's earlier global writes before its later reads (all lanes).
 __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -188,38 +200,79 @@ __device__ __forceinline__ void fold_apply(Fold &f, float cs, double sigma_u, ui
   }
 }
 
-// Per-lane single-source state handed between phases.
+// A candidate offering more than final * (1+1e-4)^(deg+3) can neither stay a
+// predecessor nor change which candidates do: every later reset that must
+// remove it also removes anything it influenced (see DESIGN notes on the
+// fold). Only "near" candidates need the ordered fold; usually there is one.
+__device__ __forceinline__ double far_factor(uint32_t deg) {
+  double f = 1.0;
+  for (uint32_t i = 0; i < deg + 3; ++i) f *= 1.00010002;
+  return f;
+}
+
 struct SrcCtx {
   uint32_t src;
   uint32_t tag;
-  uint32_t k;  // current absolute bucket
-  uint32_t nslot;
-  uint32_t nb;  // buckets opened
+  uint32_t k;      // current absolute bucket
+  uint32_t nslot;  // settled nodes so far
+  uint32_t nb;     // buckets opened
   uint32_t fail;
 };
 
 // ---------------------------------------------------------------------------
-// Fold for node v (not the source) over its out-edges v->u (forward row).
-// Candidates: u settled in an earlier bucket, cs = secs_u + es <= maxsec.
-// Brandes: ordered multi-predecessor fold. Tree: strict-< first minimum.
+// Brandes fold for node v (not the source) over its out-edges v->u: candidates
+// are nodes u settled in an earlier bucket with cs = secs_u + es <= maxsec.
+// Also counts canonical out-edges to nodes settled before v (circuit rank).
+// Returns false on an invariant failure.
 // ---------------------------------------------------------------------------
-template <int MODE>
-__device__ __forceinline__ void settle_fold(const SearchArgs &a, const WaveWs &w, uint32_t tag,
-                                            uint32_t v, float secs_v, uint32_t kb, Fold &f,
-                                            uint32_t &tree_j, uint32_t &tree_u) {
+__device__ __forceinline__ bool brandes_fold(const SearchArgs &a, const WaveWs &w, uint32_t tag, uint32_t v,
+                                             float secs_v, uint32_t kb, bool cycles, Fold &f,
+                                             uint32_t &cnt_e) {
   const uint32_t r0 = a.fwd_off[v], r1 = a.fwd_off[v + 1];
-  const uint32_t deg = r1 - r0;
+  const double thr_far = (double)secs_v * far_factor(r1 - r0);
+  uint32_t n_near = 0, near_u = 0, near_j = 0;
+  float near_cs = INFINITY;
+  for (uint32_t j = r0; j < r1; ++j) {
+    const uint2 e = a.fwd_adj[j];
+    const uint32_t u = e.x & kNodeMask;
+    if (u == v) continue;
+    const uint64_t ku = ld64(&w.keyw[u]);
+    if (!key_current(ku, tag)) continue;
+    const float su = key_secs(ku);
+    if (cycles && (e.x & kCanonBit) && key_less(su, u, secs_v, v)) ++cnt_e;
+    if (bucket_of(su, a.inv_delta) >= kb) continue;
+    const float cs = su + __uint_as_float(e.y);
+    if (cs > a.maxsec) continue;
+    if ((double)cs <= thr_far) {
+      if (n_near == 0) {
+        near_u = u;
+        near_j = j - r0;
+        near_cs = cs;
+      }
+      ++n_near;
+    }
+  }
   f.best = INFINITY;
   f.sigma = 0.0;
   f.pmask = 0;
-  tree_j = 0xffffffffu;
-  tree_u = 0xffffffffu;
-  if constexpr (MODE == kSegment || MODE == kTree) {
-    // strict-< first minimum in settle order == lexicographic min (cs, secs_u, u, j)
-    float bcs = INFINITY, bsu = INFINITY;
-    uint32_t bu = 0xffffffffu, bj = 0xffffffffu;
-    for (uint32_t j = 0; j < deg; ++j) {
-      const uint2 e = a.fwd_adj[r0 + j];
+  if (n_near == 0) return false;
+  if (n_near == 1) {
+    f.best = near_cs;
+    f.pmask = 1ull << near_j;
+    f.sigma = ldd(reinterpret_cast<const double *>(&w.a8[ld32(&w.slotw[near_u])]));
+    return f.best == secs_v;
+  }
+  // Ordered fold over the near candidates: selection passes in settle order
+  // (secs_u, u), ties of one u (parallel edges) in row order.
+  bool have_last = false;
+  float lsu = 0.0f;
+  uint32_t lu = 0, lj = 0;
+  for (uint32_t step = 0; step < n_near; ++step) {
+    bool found = false;
+    float psu = INFINITY, pcs = 0.0f;
+    uint32_t pu = 0, pj = 0;
+    for (uint32_t j = r0; j < r1; ++j) {
+      const uint2 e = a.fwd_adj[j];
       const uint32_t u = e.x & kNodeMask;
       if (u == v) continue;
       const uint64_t ku = ld64(&w.keyw[u]);
@@ -227,168 +280,79 @@ __device__ __forceinline__ void settle_fold(const SearchArgs &a, const WaveWs &w
       const float su = key_secs(ku);
       if (bucket_of(su, a.inv_delta) >= kb) continue;
       const float cs = su + __uint_as_float(e.y);
-      if (cs > a.maxsec) continue;
-      const bool better = cs < bcs || (cs == bcs && (su < bsu || (su == bsu && (u < bu || (u == bu && j < bj)))));
-      if (better) {
-        bcs = cs;
-        bsu = su;
-        bu = u;
-        bj = j;
+      if (cs > a.maxsec || (double)cs > thr_far) continue;
+      const uint32_t jr = j - r0;
+      if (have_last && !(key_less(lsu, lu, su, u) || (su == lsu && u == lu && jr > lj))) continue;
+      if (!found || key_less(su, u, psu, pu) || (su == psu && u == pu && jr < pj)) {
+        found = true;
+        psu = su;
+        pu = u;
+        pj = jr;
+        pcs = cs;
       }
     }
-    f.best = bcs;
-    tree_j = bj;
-    tree_u = bu;
-  } else {
-    if (deg <= kMaxCand) {
-      uint32_t cu[kMaxCand], cj[kMaxCand];
-      float ccs[kMaxCand], csu[kMaxCand];
-      double csg[kMaxCand];
-      uint32_t n = 0;
-#pragma unroll
-      for (int c = 0; c < kMaxCand; ++c) {
-        cu[c] = 0;
-        cj[c] = 0;
-        ccs[c] = INFINITY;
-        csu[c] = INFINITY;
-        csg[c] = 0.0;
-      }
-#pragma unroll
-      for (int jj = 0; jj < kMaxCand; ++jj) {
-        if ((uint32_t)jj < deg) {
-          const uint2 e = a.fwd_adj[r0 + jj];
-          const uint32_t u = e.x & kNodeMask;
-          bool ok = u != v;
-          float su = INFINITY;
-          if (ok) {
-            const uint64_t ku = ld64(&w.keyw[u]);
-            ok = key_current(ku, tag);
-            su = key_secs(ku);
-            ok = ok && bucket_of(su, a.inv_delta) < kb;
-          }
-          float cs = su + __uint_as_float(e.y);
-          ok = ok && cs <= a.maxsec;
-          if (ok) {
-            const uint32_t slot_u = ld32(&w.slotw[u]);
-            const double sg = __longlong_as_double((long long)ld64(&w.a8[slot_u]));
-#pragma unroll
-            for (int c = 0; c < kMaxCand; ++c) {
-              if ((uint32_t)c == n) {
-                cu[c] = u;
-                cj[c] = (uint32_t)jj;
-                ccs[c] = cs;
-                csu[c] = su;
-                csg[c] = sg;
-              }
-            }
-            ++n;
-          }
-        }
-      }
-      uint32_t done = 0, inp = 0;  // processed / in-preds bit per candidate
-      for (uint32_t step = 0; step < n; ++step) {
-        int pick = -1;
-        float psu = INFINITY;
-        uint32_t pu = 0xffffffffu, pj = 0xffffffffu;
-#pragma unroll
-        for (int c = 0; c < kMaxCand; ++c) {
-          if ((uint32_t)c < n && !((done >> c) & 1u)) {
-            const bool lt = pick < 0 || key_less(csu[c], cu[c], psu, pu) ||
-                            (csu[c] == psu && cu[c] == pu && cj[c] < pj);
-            if (lt) {
-              pick = c;
-              psu = csu[c];
-              pu = cu[c];
-              pj = cj[c];
-            }
-          }
-        }
-        float pcs = 0.0f;
-        double psg = 0.0;
-        bool contains = false;
-#pragma unroll
-        for (int c = 0; c < kMaxCand; ++c) {
-          if (c == pick) {
-            pcs = ccs[c];
-            psg = csg[c];
-          }
-          if (((inp >> c) & 1u) && cu[c] == pu) contains = true;
-        }
-        done |= 1u << pick;
-        const uint64_t before = f.pmask;
-        fold_apply(f, pcs, psg, pj, contains);
-        if (f.pmask != before) {
-          // rebuild in-preds flags from the mask (a reset clears earlier preds)
-          uint32_t ni = 0;
-#pragma unroll
-          for (int c = 0; c < kMaxCand; ++c)
-            if ((uint32_t)c < n && ((done >> c) & 1u) && ((f.pmask >> cj[c]) & 1ull)) ni |= 1u << c;
-          inp = ni;
-        }
-      }
-    } else {
-      // Wide nodes: selection passes over the row in global memory.
-      bool have_last = false;
-      float lsu = 0.0f;
-      uint32_t lu = 0, lj = 0;
-      for (;;) {
-        bool found = false;
-        float psu = INFINITY, pcs = 0.0f;
-        uint32_t pu = 0, pj = 0;
-        for (uint32_t j = 0; j < deg; ++j) {
-          const uint2 e = a.fwd_adj[r0 + j];
-          const uint32_t u = e.x & kNodeMask;
-          if (u == v) continue;
-          const uint64_t ku = ld64(&w.keyw[u]);
-          if (!key_current(ku, tag)) continue;
-          const float su = key_secs(ku);
-          if (bucket_of(su, a.inv_delta) >= kb) continue;
-          const float cs = su + __uint_as_float(e.y);
-          if (cs > a.maxsec) continue;
-          // strictly after the last processed candidate in (secs, node, j) order
-          if (have_last && !(key_less(lsu, lu, su, u) || (su == lsu && u == lu && j > lj))) continue;
-          const bool lt = !found || key_less(su, u, psu, pu) || (su == psu && u == pu && j < pj);
-          if (lt) {
-            found = true;
-            psu = su;
-            pu = u;
-            pj = j;
-            pcs = cs;
-          }
-        }
-        if (!found) break;
-        bool contains = false;
-        for (uint64_t m = f.pmask; m; m &= m - 1) {
-          const uint32_t jb = __builtin_ctzll(m);
-          if ((a.fwd_adj[r0 + jb].x & kNodeMask) == pu) contains = true;
-        }
-        const uint32_t slot_u = ld32(&w.slotw[pu]);
-        const double sg = __longlong_as_double((long long)ld64(&w.a8[slot_u]));
-        fold_apply(f, pcs, sg, pj, contains);
-        have_last = true;
-        lsu = psu;
-        lu = pu;
-        lj = pj;
-      }
+    if (!found) return false;
+    bool contains = false;
+    for (uint64_t m = f.pmask; m; m &= m - 1) {
+      const uint32_t jb = __builtin_ctzll(m);
+      if ((a.fwd_adj[r0 + jb].x & kNodeMask) == pu) contains = true;
+    }
+    const double sg = ldd(reinterpret_cast<const double *>(&w.a8[ld32(&w.slotw[pu])]));
+    fold_apply(f, pcs, sg, pj, contains);
+    have_last = true;
+    lsu = psu;
+    lu = pu;
+    lj = pj;
+  }
+  return f.best == secs_v;
+}
+
+// Single-predecessor fold (dijkstra_tree_shortest / dijkstra_tree_segment,
+// centrality.rs:1186-1195,1589-1607): strict < over candidates in settle
+// order keeps the first minimum, i.e. the lexicographic min (cs, secs_u, u, j).
+__device__ __forceinline__ bool tree_fold(const SearchArgs &a, const WaveWs &w, uint32_t tag, uint32_t v,
+                                          float secs_v, uint32_t kb, uint32_t &tree_u, uint32_t &tree_j) {
+  const uint32_t r0 = a.fwd_off[v], r1 = a.fwd_off[v + 1];
+  float bcs = INFINITY, bsu = INFINITY;
+  uint32_t bu = 0xffffffffu, bj = 0xffffffffu;
+  for (uint32_t j = r0; j < r1; ++j) {
+    const uint2 e = a.fwd_adj[j];
+    const uint32_t u = e.x & kNodeMask;
+    if (u == v) continue;
+    const uint64_t ku = ld64(&w.keyw[u]);
+    if (!key_current(ku, tag)) continue;
+    const float su = key_secs(ku);
+    if (bucket_of(su, a.inv_delta) >= kb) continue;
+    const float cs = su + __uint_as_float(e.y);
+    if (cs > a.maxsec) continue;
+    const uint32_t jr = j - r0;
+    if (cs < bcs || (cs == bcs && (su < bsu || (su == bsu && (u < bu || (u == bu && jr < bj)))))) {
+      bcs = cs;
+      bsu = su;
+      bu = u;
+      bj = jr;
     }
   }
-  (void)secs_v;
+  tree_u = bu;
+  tree_j = bj;
+  return bj != 0xffffffffu && bcs == secs_v;
 }
 
 // ---------------------------------------------------------------------------
-// Search one source. Returns false if the source must be retried with a
-// larger workspace (fail flags set).
+// Search one source: bucket by bucket, each entry of the bucket is validated
+// (its key still equals the pushed value), given a slot, folded and relaxed in
+// one pass. NT = number of thresholds (compile time).
+// Returns false if the source must be retried with a larger workspace.
 // ---------------------------------------------------------------------------
-template <int MODE>
-__device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c,
-                              uint32_t (&ncnt)[kMaxT], uint32_t (&ecnt)[kMaxT],
-                              unsigned long long &st_edges) {
+template <int MODE, int NT>
+__device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c, uint32_t (&nband)[NT],
+                              uint32_t (&eband)[NT], unsigned long long &st_edges) {
   const uint32_t lane = lane_id();
   const float speed = a.speed;
-  const bool want_cycles = (MODE == kBrandes) && (a.flags & 1u);
+  const bool cycles = (MODE == kBrandes) && (a.flags & 1u);
   if (lane == 0) {
     st64(&w.keyw[c.src], pack_key(c.tag, 0.0f));
-    w.bkt[0] = make_uint2(c.src, 0u);
+    st64(reinterpret_cast<uint64_t *>(&w.bkt[0]), (uint64_t)c.src);
     st32(&w.bcnt[0], 1u);
   }
   wave_fence();
@@ -412,8 +376,12 @@ __device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c,
     if (cnt > a.bcap) c.fail |= kFailBucket;
     if (c.nb >= a.maxb) c.fail |= kFailInternal;
     if (c.fail) break;
-    const uint32_t start = c.nslot;
-    // ---- settle: entries whose key still equals the pushed value are final
+    if (lane == 0) {
+      st32(&w.bcnt[ring], 0u);
+      st32(&w.bnd[c.nb], c.nslot);
+    }
+    ++c.nb;
+    uint32_t pushes = 0, lfail = 0;
     for (uint32_t base = 0; base < cnt; base += kWave) {
       const uint32_t i = base + lane;
       bool valid = false;
@@ -430,131 +398,94 @@ __device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c,
         c.fail |= kFailCap;
         break;
       }
-      if (valid) {
-        const uint32_t slot = c.nslot + rank_below(m);
-        st32(&w.slotw[v], slot);
-        st32(&w.node[slot], v);
-        st32(reinterpret_cast<uint32_t *>(&w.secs[slot]), csb);
-      }
+      const uint32_t s = c.nslot + rank_below(m);
       c.nslot += cntv;
-    }
-    if (c.fail) break;
-    pending -= cnt;
-    if (lane == 0) {
-      st32(&w.bcnt[ring], 0u);
-      st32(&w.bnd[c.nb], start);
-    }
-    ++c.nb;
-    wave_fence();
-    // ---- fold + cycle counting + relaxation for the settled range
-    uint32_t pushes = 0;
-    uint32_t lfail = 0;
-    for (uint32_t s0 = start; s0 < c.nslot; s0 += kWave) {
-      const uint32_t s = s0 + lane;
-      if (s < c.nslot) {
-        const uint32_t v = ld32(&w.node[s]);
-        const float secs_v = ldf(&w.secs[s]);
-        const float route_v = secs_v * speed;
-        if constexpr (MODE == kBrandes || MODE == kBrandesExport) {
-          Fold f;
-          uint32_t tj = 0xffffffffu, tu = 0xffffffffu;
-          if (v == c.src) {
-            f.sigma = 1.0;
-            f.pmask = 0;
-            f.best = 0.0f;
+      if (!valid) continue;
+      const float secs_v = __uint_as_float(csb);
+      const float route_v = secs_v * speed;
+      st32(&w.slotw[v], s);
+      st32(&w.node[s], v);
+      st32(reinterpret_cast<uint32_t *>(&w.secs[s]), csb);
+      uint32_t cnt_e = 0;
+      if constexpr (MODE == kBrandes || MODE == kBrandesExport) {
+        Fold f;
+        if (v == c.src) {
+          f.sigma = 1.0;
+          f.pmask = 0;
+        } else if (!brandes_fold(a, w, c.tag, v, secs_v, c.k, cycles, f, cnt_e)) {
+          lfail |= kFailInternal;
+        }
+        st64(&w.a8[s], (uint64_t)__double_as_longlong(f.sigma));
+        st64(&w.b8[s], f.pmask);
+        if constexpr (MODE == kBrandes) {
+#pragma unroll
+          for (int t = 0; t < 2 * NT; ++t) std_(&w.acc[(size_t)s * 2 * NT + t], 0.0);
+        }
+      } else {
+        uint32_t tu = 0xffffffffu, tj = 0xffffffffu, pslot = 0xffffffffu;
+        float olen = 0.0f, llen = 0.0f;
+        if (v != c.src) {
+          if (!tree_fold(a, w, c.tag, v, secs_v, c.k, tu, tj)) {
+            lfail |= kFailInternal;
           } else {
-            settle_fold<kBrandes>(a, w, c.tag, v, secs_v, c.k, f, tj, tu);
-            if (f.best != secs_v) lfail |= kFailInternal;
+            pslot = ld32(&w.slotw[tu]);
+            llen = a.fwd_twlen[a.fwd_off[v] + tj];
+            olen = (tu == c.src) ? llen : __uint_as_float((uint32_t)(ld64(&w.b8[pslot]) >> 32));
           }
-          st64(&w.a8[s], (uint64_t)__double_as_longlong(f.sigma));
-          st64(&w.b8[s], f.pmask);
-          for (uint32_t i = 0; i < 2 * a.T; ++i) std_(&w.acc[(size_t)s * 2 * a.T + i], 0.0);
-        } else {
-          Fold f;
-          uint32_t tj = 0xffffffffu, tu = 0xffffffffu;
-          uint32_t pslot = 0xffffffffu;
-          float olen = 0.0f, llen = 0.0f;
-          if (v != c.src) {
-            settle_fold<kSegment>(a, w, c.tag, v, secs_v, c.k, f, tj, tu);
-            if (f.best != secs_v || tj == 0xffffffffu) {
-              lfail |= kFailInternal;
-            } else {
-              pslot = ld32(&w.slotw[tu]);
-              llen = a.fwd_twlen ? a.fwd_twlen[a.fwd_off[v] + tj] : 0.0f;
-              if (tu == c.src) {
-                olen = llen;
-              } else {
-                olen = __uint_as_float((uint32_t)(ld64(&w.b8[pslot]) >> 32));
-              }
-            }
-          }
-          // a8 = {pslot, last edge position}; b8 = {llen bits, olen bits}
-          st64(&w.a8[s], ((uint64_t)tj << 32) | pslot);
-          st64(&w.b8[s], ((uint64_t)__float_as_uint(olen) << 32) | __float_as_uint(llen));
-          for (uint32_t i = 0; i < a.T; ++i) std_(&w.acc[(size_t)s * a.T + i], 0.0);
         }
-        if (want_cycles) {
+        // a8 = {pslot | last-edge row position << 32}; b8 = {llen | olen << 32}
+        st64(&w.a8[s], ((uint64_t)tj << 32) | pslot);
+        st64(&w.b8[s], ((uint64_t)__float_as_uint(olen) << 32) | __float_as_uint(llen));
+        if constexpr (MODE == kSegment) {
 #pragma unroll
-          for (uint32_t i = 0; i < kMaxT; ++i)
-            if (i < a.T && route_v <= a.dist[i]) ++ncnt[i];
-          // canonical out-edges to nodes settled before v
-          const uint32_t f0 = a.fwd_off[v], f1 = a.fwd_off[v + 1];
-          uint32_t cnt_e = 0;
-          for (uint32_t j = f0; j < f1; ++j) {
-            const uint2 e = a.fwd_adj[j];
-            if (!(e.x & kCanonBit)) continue;
-            const uint32_t u = e.x & kNodeMask;
-            if (u == v) continue;
-            const uint64_t ku = ld64(&w.keyw[u]);
-            if (!key_current(ku, c.tag)) continue;
-            if (key_less(key_secs(ku), u, secs_v, v)) ++cnt_e;
-          }
-          const uint32_t r0 = a.rev_off[v], r1 = a.rev_off[v + 1];
-          for (uint32_t j = r0; j < r1; ++j) {
-            const uint2 e = a.rev_adj[j];
-            if (!(e.x & kCanonBit)) continue;
-            const uint32_t u = e.x & kNodeMask;
-            if (u == v) continue;
-            const uint64_t ku = ld64(&w.keyw[u]);
-            if (!key_current(ku, c.tag)) continue;
-            if (key_less(key_secs(ku), u, secs_v, v)) ++cnt_e;
-          }
-#pragma unroll
-          for (uint32_t i = 0; i < kMaxT; ++i)
-            if (i < a.T && route_v <= a.dist[i]) ecnt[i] += cnt_e;
+          for (int t = 0; t < NT; ++t) std_(&w.acc[(size_t)s * NT + t], 0.0);
         }
-        // relax in-edges nb -> v
-        const uint32_t r0 = a.rev_off[v], r1 = a.rev_off[v + 1];
-        st_edges += (r1 - r0);
-        for (uint32_t j = r0; j < r1; ++j) {
-          const uint2 e = a.rev_adj[j];
-          const uint32_t nb = e.x & kNodeMask;
-          if (nb == v) continue;
-          const float cs = secs_v + __uint_as_float(e.y);
-          if (cs > a.maxsec) continue;
+      }
+      // relax in-edges nb -> v; the atomic's old key also tells whether nb
+      // settled before v (canonical edges count toward the circuit rank)
+      const uint32_t r0 = a.rev_off[v], r1 = a.rev_off[v + 1];
+      st_edges += (r1 - r0);
+      for (uint32_t j = r0; j < r1; ++j) {
+        const uint2 e = a.rev_adj[j];
+        const uint32_t nb = e.x & kNodeMask;
+        if (nb == v) continue;
+        const float cs = secs_v + __uint_as_float(e.y);
+        uint64_t kn = 0;
+        if (cs <= a.maxsec) {
           const uint64_t nk = pack_key(c.tag, cs);
-          const uint64_t old = __hip_atomic_fetch_max(&w.keyw[nb], nk, __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT);
-          if (nk > old) {
+          kn = wmax64(&w.keyw[nb], nk);
+          if (nk > kn) {
             const uint32_t b = bucket_of(cs, a.inv_delta);
             if (b <= c.k || b - c.k >= a.nbk) {
               lfail |= (b <= c.k) ? kFailInternal : kFailRing;
-              continue;
-            }
-            const uint32_t rb = b % a.nbk;
-            const uint32_t pos = __hip_atomic_fetch_add(&w.bcnt[rb], 1u, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT);
-            if (pos < a.bcap) {
-              st64(reinterpret_cast<uint64_t *>(&w.bkt[rb * a.bcap + pos]),
-                   ((uint64_t)__float_as_uint(cs) << 32) | nb);
             } else {
-              lfail |= kFailBucket;
+              const uint32_t rb = b % a.nbk;
+              const uint32_t pos = wadd32(&w.bcnt[rb], 1u);
+              if (pos < a.bcap)
+                st64(reinterpret_cast<uint64_t *>(&w.bkt[rb * a.bcap + pos]),
+                     ((uint64_t)__float_as_uint(cs) << 32) | nb);
+              else
+                lfail |= kFailBucket;
+              ++pushes;
             }
-            ++pushes;
+          }
+        } else if (cycles && (e.x & kCanonBit)) {
+          kn = ld64(&w.keyw[nb]);
+        }
+        if (cycles && (e.x & kCanonBit) && key_current(kn, c.tag) && key_less(key_secs(kn), nb, secs_v, v))
+          ++cnt_e;
+      }
+      if (cycles) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          if (route_v <= a.dist[t]) {
+            ++nband[t];
+            eband[t] += cnt_e;
           }
         }
       }
     }
+    pending -= cnt;
     pending += wave_sum(pushes);
     c.fail |= uniform(wave_or(lfail));
     wave_fence();
@@ -573,12 +504,12 @@ __device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c,
 }
 
 // ---------------------------------------------------------------------------
-// Persistent kernel.
+// Persistent kernel: each wave loops over sources taken from a global counter.
 // ---------------------------------------------------------------------------
-template <int MODE>
+template <int MODE, int NT>
 __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
   const uint32_t lane = lane_id();
-  const uint32_t wid = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t wid = uniform((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
   const WaveWs w = wave_ws(a, wid);
   uint32_t epoch = a.epochs[wid];
   unsigned long long st_reached = 0, st_edges = 0, st_within = 0;
@@ -592,10 +523,10 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
     c.src = a.sources[si];
     ++epoch;
     c.tag = epoch;
-    uint32_t ncnt[kMaxT], ecnt[kMaxT];
+    uint32_t nband[NT], eband[NT];
 #pragma unroll
-    for (uint32_t i = 0; i < kMaxT; ++i) ncnt[i] = ecnt[i] = 0;
-    if (!search_source<MODE>(a, w, c, ncnt, ecnt, st_edges)) {
+    for (int t = 0; t < NT; ++t) nband[t] = eband[t] = 0;
+    if (!search_source<MODE, NT>(a, w, c, nband, eband, st_edges)) {
       if (lane == 0) {
         const uint32_t q = atomicAdd(a.n_fail, 1u);
         a.fail_list[q] = si;
@@ -609,51 +540,46 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
     if constexpr (MODE == kBrandes) {
       const float wt = a.src_wt[si];
       const float cyc = a.src_cyc[si];
-      const uint32_t T = a.T;
-      const uint32_t M = 7;
-      float rank[kMaxT];
-      if (a.flags & 1u) {
+      constexpr uint32_t M = 7;
+      float rank[NT];
+      uint32_t within[NT];
 #pragma unroll
-        for (uint32_t i = 0; i < kMaxT; ++i) {
-          const uint32_t nn = wave_sum(ncnt[i]);
-          const uint32_t ee = wave_sum(ecnt[i]);
-          const int r = (int)ee - (int)nn + 1;
-          rank[i] = (nn == 0) ? 0.0f : (float)(r > 0 ? r : 0);
-        }
+      for (int t = 0; t < NT; ++t) {
+        const uint32_t nn = wave_sum(nband[t]);
+        const uint32_t ee = wave_sum(eband[t]);
+        const int r = (int)ee - (int)nn + 1;
+        rank[t] = (nn == 0) ? 0.0f : (float)(r > 0 ? r : 0);
+        within[t] = 0;
       }
-      uint32_t rch[kMaxT];
-#pragma unroll
-      for (uint32_t i = 0; i < kMaxT; ++i) rch[i] = 0;
       // ---- closeness: scatter to targets (centrality.rs:1736-1779)
       for (uint32_t s = lane; s < R; s += kWave) {
         const uint32_t v = ld32(&w.node[s]);
         if (v == c.src) continue;
         const float route = ldf(&w.secs[s]) * speed;
-        double *accv = a.acc + (size_t)v * M * T;
+        double *accv = a.acc + (size_t)v * M * NT;
 #pragma unroll
-        for (uint32_t i = 0; i < kMaxT; ++i) {
-          if (i < T && route <= a.dist[i]) {
-            ++rch[i];
+        for (int t = 0; t < NT; ++t) {
+          if (route <= a.dist[t]) {
+            ++within[t];
             if (a.flags & 1u) {
-              unsafeAtomicAdd(&accv[0 * T + i], (double)wt);
-              unsafeAtomicAdd(&accv[1 * T + i], (double)(route * wt));
-              unsafeAtomicAdd(&accv[2 * T + i], (double)(rank[i] * cyc));
-              unsafeAtomicAdd(&accv[3 * T + i], (double)((1.0f / route) * wt));
-              unsafeAtomicAdd(&accv[4 * T + i], (double)(expf(-a.beta[i] * route) * wt));
+              gadd_f64(&accv[0 * NT + t], (double)wt);
+              gadd_f64(&accv[1 * NT + t], (double)(route * wt));
+              gadd_f64(&accv[2 * NT + t], (double)(rank[t] * cyc));
+              gadd_f64(&accv[3 * NT + t], (double)((1.0f / route) * wt));
+              gadd_f64(&accv[4 * NT + t], (double)(expf(-a.beta[t] * route) * wt));
             }
           }
         }
       }
 #pragma unroll
-      for (uint32_t i = 0; i < kMaxT; ++i) {
-        const uint32_t t = wave_sum(rch[i]);
-        st_within += (lane == 0) ? t : 0;
-        if ((a.flags & 4u) && (a.flags & 1u) && lane == 0 && i < T && t) atomicAdd(&a.reach[i], t);
+      for (int t = 0; t < NT; ++t) {
+        const uint32_t tot = wave_sum(within[t]);
+        st_within += (lane == 0) ? tot : 0;
+        if ((a.flags & 5u) == 5u && lane == 0 && tot) atomicAdd(&a.reach[t], tot);
       }
       // ---- betweenness: reverse bucket order (centrality.rs:1783-1835, 823-873)
       if (a.flags & 2u) {
-        const uint32_t nbk = c.nb;
-        for (int b = (int)nbk - 1; b >= 0; --b) {
+        for (int b = (int)c.nb - 1; b >= 0; --b) {
           const uint32_t s0 = ld32(&w.bnd[b]), s1 = ld32(&w.bnd[b + 1]);
           for (uint32_t s = s0 + lane; s < s1; s += kWave) {
             const uint32_t v = ld32(&w.node[s]);
@@ -663,42 +589,55 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             const bool is_src = v == c.src;
             const double pc = is_src ? 0.0 : (a.eligible[v] ? 0.5 : 1.0);
             const uint32_t f0 = a.fwd_off[v];
-            double *accv = a.acc + (size_t)v * M * T;
-            for (uint32_t i = 0; i < T; ++i) {
-              if (!(route <= a.dist[i])) continue;
-              const double seed = pc;
-              const double seed_b = pc * exp(-(double)a.beta[i] * (double)route);
-              const double dep = seed + ldd(&w.acc[(size_t)s * 2 * T + i]);
-              const double dep_b = seed_b + ldd(&w.acc[(size_t)s * 2 * T + T + i]);
-              if (dep == 0.0 && dep_b == 0.0) continue;
-              for (uint64_t m = pm; m; m &= m - 1) {
-                const uint32_t j = __builtin_ctzll(m);
-                const uint32_t u = a.fwd_adj[f0 + j].x & kNodeMask;
-                const uint32_t su = ld32(&w.slotw[u]);
-                const double sigma_u = ldd(reinterpret_cast<const double *>(&w.a8[su]));
-                if (sigma_u == 0.0) continue;
-                const double fct = sigma_u / sigma_w;
-                unsafeAtomicAdd(&w.acc[(size_t)su * 2 * T + i], fct * dep);
-                unsafeAtomicAdd(&w.acc[(size_t)su * 2 * T + T + i], fct * dep_b);
+            double *accv = a.acc + (size_t)v * M * NT;
+            double dep[NT], dep_b[NT], seed_b[NT];
+            bool any = false;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+              dep[t] = dep_b[t] = seed_b[t] = 0.0;
+              if (route <= a.dist[t]) {
+                seed_b[t] = pc * exp(-(double)a.beta[t] * (double)route);
+                dep[t] = pc + ldd(&w.acc[(size_t)s * 2 * NT + t]);
+                dep_b[t] = seed_b[t] + ldd(&w.acc[(size_t)s * 2 * NT + NT + t]);
+                any = any || dep[t] != 0.0 || dep_b[t] != 0.0;
               }
-              if (is_src) continue;
-              const double credit = dep - seed;
-              const double credit_b = dep_b - seed_b;
-              if (credit > 0.0) unsafeAtomicAdd(&accv[5 * T + i], credit * (double)wt);
-              if (credit_b > 0.0) unsafeAtomicAdd(&accv[6 * T + i], credit_b * (double)wt);
+            }
+            if (!any) continue;
+            for (uint64_t m = pm; m; m &= m - 1) {
+              const uint32_t j = __builtin_ctzll(m);
+              const uint32_t u = a.fwd_adj[f0 + j].x & kNodeMask;
+              const uint32_t su = ld32(&w.slotw[u]);
+              const double sigma_u = ldd(reinterpret_cast<const double *>(&w.a8[su]));
+              if (sigma_u == 0.0) continue;
+              const double fct = sigma_u / sigma_w;
+#pragma unroll
+              for (int t = 0; t < NT; ++t) {
+                if (dep[t] != 0.0 || dep_b[t] != 0.0) {
+                  wadd_f64(&w.acc[(size_t)su * 2 * NT + t], fct * dep[t]);
+                  wadd_f64(&w.acc[(size_t)su * 2 * NT + NT + t], fct * dep_b[t]);
+                }
+              }
+            }
+            if (is_src) continue;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+              if (dep[t] == 0.0 && dep_b[t] == 0.0) continue;
+              const double credit = dep[t] - pc;
+              const double credit_b = dep_b[t] - seed_b[t];
+              if (credit > 0.0) gadd_f64(&accv[5 * NT + t], credit * (double)wt);
+              if (credit_b > 0.0) gadd_f64(&accv[6 * NT + t], credit_b * (double)wt);
             }
           }
           wave_fence();
         }
       }
     } else if constexpr (MODE == kSegment) {
-      const uint32_t T = a.T;
-      const uint32_t M = 4;
+      constexpr uint32_t M = 4;
       // ---- closeness at the source over visited edges (centrality.rs:2201-2317)
-      double sd[kMaxT], sh[kMaxT], sb[kMaxT];
-#pragma unroll
-      for (uint32_t i = 0; i < kMaxT; ++i) sd[i] = sh[i] = sb[i] = 0.0;
       if (a.flags & 1u) {
+        double sd[NT], sh[NT], sb[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) sd[t] = sh[t] = sb[t] = 0.0;
         for (uint32_t s = lane; s < R; s += kWave) {
           const uint32_t v = ld32(&w.node[s]);
           const float secs_v = ldf(&w.secs[s]);
@@ -707,9 +646,8 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
           for (uint32_t j = r0; j < r1; ++j) {
             const uint32_t nb = a.rev_adj[j].x & kNodeMask;
             float sd_m = INFINITY;
-            bool visited_edge;
+            bool visited_edge = true;
             if (nb == v) {
-              visited_edge = true;
               sd_m = sd_n;
             } else {
               const uint64_t kn = ld64(&w.keyw[nb]);
@@ -717,8 +655,6 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
                 const float s_nb = key_secs(kn);
                 visited_edge = key_less(secs_v, v, s_nb, nb);
                 sd_m = s_nb * speed;
-              } else {
-                visited_edge = true;
               }
             }
             if (!visited_edge) continue;
@@ -733,10 +669,9 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             const float c_imp = a_imp + (cc - av) * imp;
             const float d_imp = c_imp;
 #pragma unroll
-            for (uint32_t i = 0; i < kMaxT; ++i) {
-              if (i >= T) continue;
-              const float dist = a.dist[i];
-              const float beta = a.beta[i];
+            for (int t = 0; t < NT; ++t) {
+              const float dist = a.dist[t];
+              const float beta = a.beta[t];
               const float nbeta = -beta;
               const float inv = beta != 0.0f ? 1.0f / nbeta : 0.0f;
               if (av < dist) {
@@ -745,11 +680,11 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
                   cur = dist;
                   curi = a_imp + (dist - av) * imp;
                 }
-                sd[i] += (double)(cur - av);
+                sd[t] += (double)(cur - av);
                 const float hh = a_imp < 1.0f ? logf(curi) : logf(fmaxf(curi / a_imp, 1.1920929e-07f));
-                sh[i] += (double)hh;
+                sh[t] += (double)hh;
                 const float bb = beta == 0.0f ? curi - a_imp : (expf(nbeta * curi) - expf(nbeta * a_imp)) * inv;
-                sb[i] += (double)bb;
+                sb[t] += (double)bb;
               }
               if (bv == dd) continue;
               if (bv <= dist) {
@@ -758,23 +693,23 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
                   cur = dist;
                   curi = b_imp + (dist - bv) * imp;
                 }
-                sd[i] += (double)(cur - bv);
+                sd[t] += (double)(cur - bv);
                 const float hh = b_imp < 1.0f ? logf(curi) : logf(fmaxf(curi / b_imp, 1.1920929e-07f));
-                sh[i] += (double)hh;
+                sh[t] += (double)hh;
                 const float bb = beta == 0.0f ? curi - b_imp : (expf(nbeta * curi) - expf(nbeta * b_imp)) * inv;
-                sb[i] += (double)bb;
+                sb[t] += (double)bb;
               }
             }
           }
         }
 #pragma unroll
-        for (uint32_t i = 0; i < kMaxT; ++i) {
-          const double t0 = wave_sum(sd[i]), t1 = wave_sum(sh[i]), t2 = wave_sum(sb[i]);
-          if (lane == 0 && i < T) {
-            double *accs = a.acc + (size_t)c.src * M * T;
-            unsafeAtomicAdd(&accs[0 * T + i], t0);
-            unsafeAtomicAdd(&accs[1 * T + i], t1);
-            unsafeAtomicAdd(&accs[2 * T + i], t2);
+        for (int t = 0; t < NT; ++t) {
+          const double t0 = wave_sum(sd[t]), t1 = wave_sum(sh[t]), t2 = wave_sum(sb[t]);
+          if (lane == 0) {
+            double *accs = a.acc + (size_t)c.src * M * NT;
+            gadd_f64(&accs[0 * NT + t], t0);
+            gadd_f64(&accs[1 * NT + t], t1);
+            gadd_f64(&accs[2 * NT + t], t2);
           }
         }
       }
@@ -786,21 +721,21 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             const uint32_t v = ld32(&w.node[s]);
             if (v == c.src) continue;
             const float sdist = ldf(&w.secs[s]) * speed;
-            const uint64_t pa = ld64(&w.a8[s]);
-            const uint32_t pslot = (uint32_t)pa;
+            const uint32_t pslot = (uint32_t)ld64(&w.a8[s]);
             const uint64_t lb = ld64(&w.b8[s]);
             const float llen = __uint_as_float((uint32_t)lb);
             const float olen = __uint_as_float((uint32_t)(lb >> 32));
             const uint32_t pred = ld32(&w.node[pslot]);
             const float min_span = sdist - olen - llen;
             const float o_1 = min_span, o_2 = min_span + olen, l_1 = min_span, l_2 = min_span + llen;
-            for (uint32_t i = 0; i < T; ++i) {
-              const double below = ldd(&w.acc[(size_t)s * T + i]);
-              if (below > 0.0) unsafeAtomicAdd(&a.acc[(size_t)v * M * T + 3 * T + i], below);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+              const double below = ldd(&w.acc[(size_t)s * NT + t]);
+              if (below > 0.0) gadd_f64(&a.acc[(size_t)v * M * NT + 3 * NT + t], below);
               double own = 0.0;
               if (v > c.src) {
-                const float dist = a.dist[i];
-                const float beta = a.beta[i];
+                const float dist = a.dist[t];
+                const float beta = a.beta[t];
                 if (min_span <= dist) {
                   const float o2s = fminf(o_2, dist), l2s = fminf(l_2, dist);
                   float auc;
@@ -814,7 +749,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
                 }
               }
               const double up = below + own;
-              if (pred != c.src && up != 0.0) unsafeAtomicAdd(&w.acc[(size_t)pslot * T + i], up);
+              if (pred != c.src && up != 0.0) wadd_f64(&w.acc[(size_t)pslot * NT + t], up);
             }
           }
           wave_fence();
@@ -824,22 +759,16 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
       // single-source exports (trees / Brandes state)
       for (uint32_t s = lane; s < R; s += kWave) {
         const uint32_t v = ld32(&w.node[s]);
-        const float secs = ldf(&w.secs[s]);
-        a.tout.secs[v] = secs;
+        a.tout.secs[v] = ldf(&w.secs[s]);
         if constexpr (MODE == kTree) {
           const uint64_t pa = ld64(&w.a8[s]);
-          const uint32_t pslot = (uint32_t)pa;
-          const uint32_t pj = (uint32_t)(pa >> 32);
           if (v != c.src) {
-            const uint32_t pred = ld32(&w.node[pslot]);
-            a.tout.pred[v] = (int32_t)pred;
-            const uint32_t eid = a.fwd_eid[a.fwd_off[v] + pj];
-            a.tout.last_eid[v] = (int32_t)eid;
+            a.tout.pred[v] = (int32_t)ld32(&w.node[(uint32_t)pa]);
+            a.tout.last_eid[v] = (int32_t)a.fwd_eid[a.fwd_off[v] + (uint32_t)(pa >> 32)];
           }
         } else if constexpr (MODE == kBrandesExport) {
-          const double sg = ldd(reinterpret_cast<const double *>(&w.a8[s]));
           const uint64_t pm = ld64(&w.b8[s]);
-          a.tout.sigma[v] = sg;
+          a.tout.sigma[v] = ldd(reinterpret_cast<const double *>(&w.a8[s]));
           uint32_t np = 0;
           const uint32_t f0 = a.fwd_off[v];
           for (uint64_t m = pm; m; m &= m - 1) {
@@ -868,6 +797,31 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
 // ---------------------------------------------------------------------------
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <int MODE>
+void launch_nt(uint32_t T, uint32_t blocks, hipStream_t stream, const SearchArgs &a) {
+  if constexpr (MODE == kTree || MODE == kBrandesExport) {
+    hipLaunchKernelGGL((k_search<MODE, 1>), dim3(blocks), dim3(kBlock), 0, stream, a);
+  } else {
+    switch (T) {
+#define CS_NT_CASE(n) \
+  case n:             \
+    hipLaunchKernelGGL((k_search<MODE, n>), dim3(blocks), dim3(kBlock), 0, stream, a); \
+    break;
+      CS_NT_CASE(1)
+      CS_NT_CASE(2)
+      CS_NT_CASE(3)
+      CS_NT_CASE(4)
+      CS_NT_CASE(5)
+      CS_NT_CASE(6)
+      CS_NT_CASE(7)
+      CS_NT_CASE(8)
+#undef CS_NT_CASE
+      default:
+        fail(CS_EINVAL, "unsupported number of thresholds");
+    }
+  }
+}
 
 struct Plan {
   float delta;
@@ -1099,7 +1053,7 @@ void run_search(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const 
     {
       const uint32_t blocks = waves / 4;
       hip_check(hipEventRecord(e0, stream), "event record");
-      hipLaunchKernelGGL(k_search<MODE>, dim3(blocks), dim3(kBlock), 0, stream, a);
+      launch_nt<MODE>(T, blocks, stream, a);
       hip_check(hipGetLastError(), "k_search launch");
       hip_check(hipEventRecord(e1, stream), "event record");
     }
