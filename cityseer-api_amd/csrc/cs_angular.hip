@@ -1,15 +1,838 @@
 // cs_angular.hip -- simplest-path (angular) centrality on the dual graph.
+//
+// Angular route increments are turn angles plus 1e-6 x length
+// (centrality.rs:665-667): straight continuations cost almost nothing, so no
+// bucket width separates a state from its predecessors and the reference's
+// order-dependent fold (centrality.rs:613-705) must see candidates strictly in
+// settle order. Each lane therefore runs one source's search serially with a
+// private binary heap keyed (route, seconds at push, state) -- 64 independent
+// sources per wave, persistent lanes pulling sources from a global counter.
+//
+// Per-lane state: epoch-tagged maps state -> slot and node -> slot (zero
+// memory is stale, nothing cleared per source), slot arrays for reached
+// states and nodes, the heap, and the settle order. Slot arrays and the heap
+// are interleaved across the 64 lanes of a wave so that lanes stepping in
+// lockstep (heap top, slot allocation) touch the same cache lines.
 #include "cs_internal.hpp"
+
+#include <cmath>
+#include <cstring>
 
 namespace cs {
 
-void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
-                     const uint32_t *d_sources, const float *d_wt, uint32_t n_sources,
-                     const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
-                     hipStream_t stream, cs_stats &st) {
-  (void)g; (void)dg; (void)de; (void)cs; (void)d_sources; (void)d_wt; (void)n_sources;
-  (void)d_eligible; (void)d_acc; (void)d_reach; (void)stream; (void)st;
-  fail(CS_EINVAL, "centrality_simplest is not available on the GPU path yet");
+namespace {
+
+constexpr uint32_t kFailCap = 1u, kFailHeap = 2u, kFailPreds = 4u;
+
+struct AngArgs {
+  uint32_t N;
+  const uint32_t *off;  // out-edge rows (petgraph order)
+  const uint4 *adj;     // {v | src_slot<<30 | dst_slot<<31, bits(es), bits(angle), bits(1e-6*len)}
+  const uint8_t *eligible;
+  uint32_t T;
+  float maxsec, tolerance, unit, offset;
+  float sec_thr[kMaxT];
+  uint32_t n_sources;
+  const uint32_t *sources;
+  const float *src_wt;
+  uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
+  double *acc;     // [N][4][T]
+  uint32_t *reach;
+  unsigned long long *stats;
+  char *ws;
+  size_t lane_stride;  // bytes of the lane-major maps per lane
+  size_t wave_stride;  // bytes of the interleaved arrays per wave
+  uint32_t scap, ncap, hcap, maxp;
+  size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_delta, o_seed, o_node, o_brt, o_bsec,
+      o_heap;
+  char *wave_ws;
+  uint32_t *epochs;  // per lane
+  uint32_t *next_source;
+  uint32_t *fail_list;
+  uint32_t *n_fail;
+  uint32_t *fail_flags;
+  // single-source tree (dijkstra_tree_simplest) outputs, N long
+  float *t_simpl;
+  float *t_secs;
+  int32_t *t_pred;
+  uint8_t *t_vis;
+  uint32_t *t_order;
+  uint32_t *t_norder;
+};
+
+// Interleaved element i of this lane in a wave-shared array of 64-lane rows.
+template <typename T>
+__device__ __forceinline__ T *il(char *base, size_t off, uint32_t i, uint32_t lane) {
+  return reinterpret_cast<T *>(base + off) + (size_t)i * kWave + lane;
+}
+
+struct LaneWs {
+  uint64_t *skey;  // [2N] epoch<<32 | state slot
+  uint64_t *nkey;  // [N]  epoch<<32 | node slot
+  char *w;         // wave-interleaved arrays
+  uint32_t lane;
+};
+
+__device__ __forceinline__ bool hless(float ka, float sa, uint32_t ia, float kb, float sb, uint32_t ib) {
+  if (ka != kb) return ka < kb;
+  if (sa != sb) return sa < sb;
+  return ia < ib;
+}
+
+struct Heap {
+  char *w;
+  size_t off;
+  uint32_t lane;
+  uint32_t n;
+  __device__ __forceinline__ uint4 *at(uint32_t i) { return il<uint4>(w, off, i, lane); }
+  __device__ __forceinline__ void push(float k, float s, uint32_t idx) {
+    uint32_t i = n++;
+    while (i > 0) {
+      const uint32_t p = (i - 1) / 2;
+      const uint4 pe = *at(p);
+      if (!hless(k, s, idx, __uint_as_float(pe.x), __uint_as_float(pe.y), pe.z)) break;
+      *at(i) = pe;
+      i = p;
+    }
+    *at(i) = make_uint4(__float_as_uint(k), __float_as_uint(s), idx, 0u);
+  }
+  __device__ __forceinline__ uint4 pop() {
+    const uint4 top = *at(0);
+    const uint4 last = *at(--n);
+    uint32_t i = 0;
+    for (;;) {
+      const uint32_t l = 2 * i + 1, r = l + 1;
+      uint32_t sidx = i;
+      uint4 best = last;
+      if (l < n) {
+        const uint4 le = *at(l);
+        if (hless(__uint_as_float(le.x), __uint_as_float(le.y), le.z, __uint_as_float(best.x),
+                  __uint_as_float(best.y), best.z)) {
+          sidx = l;
+          best = le;
+        }
+      }
+      if (r < n) {
+        const uint4 re = *at(r);
+        if (hless(__uint_as_float(re.x), __uint_as_float(re.y), re.z, __uint_as_float(best.x),
+                  __uint_as_float(best.y), best.z)) {
+          sidx = r;
+          best = re;
+        }
+      }
+      if (sidx == i) break;
+      *at(i) = best;
+      i = sidx;
+    }
+    if (n) *at(i) = last;
+    return top;
+  }
+};
+
+// One source, exactly as dijkstra_brandes_angular Phase 1 (centrality.rs:577-705).
+// Returns false (with fail bits) if a workspace capacity is exceeded.
+__device__ bool angular_source(const AngArgs &a, const LaneWs &L, uint32_t src, uint32_t epoch, uint32_t &n_states,
+                               uint32_t &n_nodes, uint32_t &n_order, uint32_t &fail) {
+  const float one_m = 1.0f - kTieEps, one_p = 1.0f + kTieEps;
+  char *w = L.w;
+  const uint32_t lane = L.lane;
+  n_states = n_nodes = n_order = 0;
+  Heap h{w, a.o_heap, lane, 0};
+  // node slot of the source
+  L.nkey[src] = ((uint64_t)epoch << 32) | 0u;
+  *il<uint32_t>(w, a.o_node, 0, lane) = src;
+  *il<float>(w, a.o_brt, 0, lane) = 0.0f;
+  *il<float>(w, a.o_bsec, 0, lane) = 0.0f;
+  n_nodes = 1;
+  for (uint32_t slot = 0; slot < 2; ++slot) {
+    const uint32_t s = 2 * src + slot;
+    const uint32_t ss = n_states++;
+    L.skey[s] = ((uint64_t)epoch << 32) | ss;
+    *il<float>(w, a.o_route, ss, lane) = 0.0f;
+    *il<float>(w, a.o_secs, ss, lane) = 0.0f;
+    *il<double>(w, a.o_sigma, ss, lane) = 1.0;
+    *il<uint32_t>(w, a.o_state, ss, lane) = s;  // visited flag in bit 31
+    *il<uint32_t>(w, a.o_np, ss, lane) = 0;
+    h.push(0.0f, 0.0f, s);
+  }
+  while (h.n > 0) {
+    const uint4 top = h.pop();
+    const uint32_t us = top.z;
+    const uint32_t uss = (uint32_t)L.skey[us];
+    uint32_t *ust = il<uint32_t>(w, a.o_state, uss, lane);
+    if (*ust & 0x80000000u) continue;
+    *ust = us | 0x80000000u;
+    if (n_order >= a.scap) {
+      fail |= kFailCap;
+      return false;
+    }
+    *il<uint32_t>(w, a.o_order, n_order++, lane) = uss;
+    const uint32_t u = us >> 1, entry = us & 1u;
+    const float u_route = *il<float>(w, a.o_route, uss, lane);
+    const float u_secs = *il<float>(w, a.o_secs, uss, lane);
+    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane);
+    for (uint32_t j = a.off[u]; j < a.off[u + 1]; ++j) {
+      const uint4 e = a.adj[j];
+      const uint32_t cur_slot = (e.x >> 30) & 1u;
+      if (cur_slot != 1u - entry) continue;
+      const uint32_t v = e.x & 0x3fffffffu;
+      const uint32_t vs = 2 * v + (e.x >> 31);
+      const float cs = u_secs + __uint_as_float(e.y);
+      if (cs > a.maxsec) continue;
+      const uint64_t sk = L.skey[vs];
+      uint32_t vss;
+      float cur;
+      if ((uint32_t)(sk >> 32) == epoch) {
+        vss = (uint32_t)sk;
+        if (*il<uint32_t>(w, a.o_state, vss, lane) & 0x80000000u) continue;  // visited
+        cur = *il<float>(w, a.o_route, vss, lane);
+      } else {
+        if (n_states >= a.scap) {
+          fail |= kFailCap;
+          return false;
+        }
+        vss = n_states++;
+        L.skey[vs] = ((uint64_t)epoch << 32) | vss;
+        *il<float>(w, a.o_route, vss, lane) = INFINITY;
+        *il<float>(w, a.o_secs, vss, lane) = INFINITY;
+        *il<double>(w, a.o_sigma, vss, lane) = 0.0;
+        *il<uint32_t>(w, a.o_state, vss, lane) = vs;
+        *il<uint32_t>(w, a.o_np, vss, lane) = 0;
+        cur = INFINITY;
+      }
+      float cr = u_route + __uint_as_float(e.z);
+      cr = cr + __uint_as_float(e.w);
+      uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane);
+      uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)vss * a.maxp) * kWave + lane;
+      if (cr < cur) {
+        double *sg = il<double>(w, a.o_sigma, vss, lane);
+        if (cr < cur * one_m) {
+          *np = 0;
+          *sg = u_sigma;
+        } else {
+          *sg = *sg + u_sigma;
+        }
+        *il<float>(w, a.o_route, vss, lane) = cr;
+        *il<float>(w, a.o_secs, vss, lane) = cs;
+        const uint32_t k = *np;
+        if (k >= a.maxp) {
+          fail |= kFailPreds;
+          return false;
+        }
+        preds[(size_t)k * kWave] = uss;
+        *np = k + 1;
+        if (h.n >= a.hcap) {
+          fail |= kFailHeap;
+          return false;
+        }
+        h.push(cr, cs, vs);
+      } else if (cr <= cur * one_p) {
+        const uint32_t k = *np;
+        bool has = false;
+        for (uint32_t q = 0; q < k; ++q)
+          if (preds[(size_t)q * kWave] == uss) has = true;
+        if (!has) {
+          float *vsec = il<float>(w, a.o_secs, vss, lane);
+          if (cs < *vsec) *vsec = cs;
+          if (k >= a.maxp) {
+            fail |= kFailPreds;
+            return false;
+          }
+          preds[(size_t)k * kWave] = uss;
+          *np = k + 1;
+          double *sg = il<double>(w, a.o_sigma, vss, lane);
+          *sg = *sg + u_sigma;
+        }
+      }
+      // per-node best route / seconds (centrality.rs:695-703)
+      const uint64_t nk = L.nkey[v];
+      uint32_t nsl;
+      if ((uint32_t)(nk >> 32) == epoch) {
+        nsl = (uint32_t)nk;
+      } else {
+        if (n_nodes >= a.ncap) {
+          fail |= kFailCap;
+          return false;
+        }
+        nsl = n_nodes++;
+        L.nkey[v] = ((uint64_t)epoch << 32) | nsl;
+        *il<uint32_t>(w, a.o_node, nsl, lane) = v;
+        *il<float>(w, a.o_brt, nsl, lane) = INFINITY;
+        *il<float>(w, a.o_bsec, nsl, lane) = INFINITY;
+      }
+      float *brt = il<float>(w, a.o_brt, nsl, lane);
+      float *bsec = il<float>(w, a.o_bsec, nsl, lane);
+      const float best = *brt;
+      if (cr < best * one_m) {
+        *brt = cr;
+        *bsec = cs;
+      } else if (cr <= best * one_p) {
+        *bsec = fminf(*bsec, cs);
+      }
+    }
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
+  const uint32_t lane = __lane_id();
+  const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;  // global lane
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(gl / kWave);
+  LaneWs L;
+  L.skey = reinterpret_cast<uint64_t *>(a.ws + (size_t)gl * a.lane_stride);
+  L.nkey = L.skey + 2 * (size_t)a.N;
+  L.w = a.wave_ws + (size_t)wid * a.wave_stride;
+  L.lane = lane;
+  uint32_t epoch = a.epochs[gl];
+  unsigned long long st_reached = 0, st_edges = 0, st_within = 0;
+  char *w = L.w;
+  for (;;) {
+    const uint32_t si = atomicAdd(a.next_source, 1u);
+    if (si >= a.n_sources) break;
+    const uint32_t src = a.sources[si];
+    const float wt = a.src_wt[si];
+    ++epoch;
+    uint32_t n_states, n_nodes, n_order, fail = 0;
+    if (!angular_source(a, L, src, epoch, n_states, n_nodes, n_order, fail)) {
+      const uint32_t q = atomicAdd(a.n_fail, 1u);
+      a.fail_list[q] = si;
+      atomicOr(a.fail_flags, fail);
+      continue;
+    }
+    st_reached += n_nodes;
+    st_edges += n_order;
+    const uint32_t T = a.T;
+    // ---- closeness at targets (centrality.rs:1995-2033)
+    for (uint32_t k = 1; k < n_nodes; ++k) {
+      const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane);
+      if (v == src) continue;
+      const float bsd = *il<float>(w, a.o_brt, k, lane);
+      const float bs = *il<float>(w, a.o_bsec, k, lane);
+      if (!isfinite(bsd) || !isfinite(bs)) continue;
+      double *accv = a.acc + (size_t)v * 4 * T;
+      for (uint32_t t = 0; t < T; ++t) {
+        if (bs <= a.sec_thr[t]) {
+          ++st_within;
+          if (a.flags & 4u) atomicAdd(&a.reach[t], 1u);
+          if (a.flags & 1u) {
+            unsafeAtomicAdd(&accv[0 * T + t], (double)wt);
+            const float far_ang = a.offset + (bsd / a.unit);
+            unsafeAtomicAdd(&accv[1 * T + t], (double)(far_ang * wt));
+            const float harm_ang = 1.0f + (bsd / a.unit);
+            unsafeAtomicAdd(&accv[2 * T + t], (double)((1.0f / harm_ang) * wt));
+          }
+        }
+      }
+    }
+    // ---- betweenness (centrality.rs:2037-2096, 823-873)
+    if (a.flags & 2u) {
+      for (uint32_t t = 0; t < T; ++t) {
+        const float sthr = a.sec_thr[t];
+        for (uint32_t q = 0; q < n_order; ++q) {
+          const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
+          *il<double>(w, a.o_delta, ss, lane) = 0.0;
+          *il<double>(w, a.o_seed, ss, lane) = 0.0;
+        }
+        // seeds: best terminal states split by sigma (best_angular_target_states)
+        for (uint32_t k = 1; k < n_nodes; ++k) {
+          const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane);
+          if (v == src) continue;
+          const float br = *il<float>(w, a.o_brt, k, lane);
+          const float bs = *il<float>(w, a.o_bsec, k, lane);
+          if (!isfinite(br) || !isfinite(bs) || bs > sthr) continue;
+          uint32_t best[2];
+          uint32_t nbst = 0;
+          double total = 0.0;
+          for (uint32_t slot = 0; slot < 2; ++slot) {
+            const uint64_t sk = L.skey[2 * v + slot];
+            if ((uint32_t)(sk >> 32) != epoch) continue;
+            const uint32_t ss = (uint32_t)sk;
+            const double sg = *il<double>(w, a.o_sigma, ss, lane);
+            if (sg == 0.0 || *il<float>(w, a.o_secs, ss, lane) > sthr) continue;
+            if (*il<float>(w, a.o_route, ss, lane) <= br * (1.0f + a.tolerance)) {
+              best[nbst++] = ss;
+            }
+          }
+          if (nbst == 0) continue;
+          for (uint32_t q = 0; q < nbst; ++q) total += *il<double>(w, a.o_sigma, best[q], lane);
+          if (total == 0.0) continue;
+          const double pc = a.eligible[v] ? 0.5 : 1.0;
+          for (uint32_t q = 0; q < nbst; ++q) {
+            double *sd = il<double>(w, a.o_seed, best[q], lane);
+            *sd = *sd + pc * (*il<double>(w, a.o_sigma, best[q], lane) / total);
+          }
+        }
+        // states by route descending, equal routes in settle order (stable sort)
+        int hi = (int)n_order - 1;
+        while (hi >= 0) {
+          const float r_hi = *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, hi, lane), lane);
+          int lo = hi;
+          while (lo > 0 && *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, lo - 1, lane), lane) == r_hi) --lo;
+          for (int q = lo; q <= hi; ++q) {
+            const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
+            if (!(*il<float>(w, a.o_secs, ss, lane) <= sthr)) continue;
+            const double sw = *il<double>(w, a.o_sigma, ss, lane);
+            if (sw == 0.0) continue;
+            const double seed = *il<double>(w, a.o_seed, ss, lane);
+            const double dep = seed + *il<double>(w, a.o_delta, ss, lane);
+            if (dep == 0.0) continue;
+            const uint32_t k = *il<uint32_t>(w, a.o_np, ss, lane);
+            const uint32_t *preds = reinterpret_cast<const uint32_t *>(w + a.o_preds) + ((size_t)ss * a.maxp) * kWave + lane;
+            for (uint32_t p = 0; p < k; ++p) {
+              const uint32_t ps = preds[(size_t)p * kWave];
+              const double sv = *il<double>(w, a.o_sigma, ps, lane);
+              if (sv == 0.0) continue;
+              double *dp = il<double>(w, a.o_delta, ps, lane);
+              *dp = *dp + (sv / sw) * dep;
+            }
+            const uint32_t node = (*il<uint32_t>(w, a.o_state, ss, lane) & 0x7fffffffu) >> 1;
+            if (node == src) continue;
+            const double credit = dep - seed;
+            if (credit > 0.0) unsafeAtomicAdd(&a.acc[(size_t)node * 4 * T + 3 * T + t], credit * (double)wt);
+          }
+          hi = lo - 1;
+        }
+      }
+    }
+  }
+  a.epochs[gl] = epoch;
+  atomicAdd(&a.stats[0], st_reached);
+  atomicAdd(&a.stats[1], st_edges);
+  atomicAdd(&a.stats[2], st_within);
+}
+
+// dijkstra_tree_simplest (centrality.rs:1202-1332) for one source on one lane:
+// absolute +-1e-4 bands, per-node view collapsed as the reference does.
+__global__ void k_tree_simplest(AngArgs a, uint32_t src) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const float eps = kTieEps;
+  LaneWs L;
+  L.skey = reinterpret_cast<uint64_t *>(a.ws);
+  L.nkey = L.skey + 2 * (size_t)a.N;
+  L.w = a.wave_ws;
+  L.lane = 0;
+  char *w = L.w;
+  const uint32_t epoch = a.epochs[0] + 1;
+  a.epochs[0] = epoch;
+  // state slots: route metric (o_route), agg seconds (o_secs), simplest
+  // distance (o_delta, an f32 value held exactly in an f64 slot)
+  uint32_t n_states = 0, no = 0;
+  Heap h{w, a.o_heap, 0, 0};
+  a.t_order[no++] = src;
+  a.t_vis[src] = 1;
+  a.t_simpl[src] = 0.0f;
+  a.t_secs[src] = 0.0f;
+  for (uint32_t slot = 0; slot < 2; ++slot) {
+    const uint32_t s = 2 * src + slot;
+    const uint32_t ss = n_states++;
+    L.skey[s] = ((uint64_t)epoch << 32) | ss;
+    *il<float>(w, a.o_route, ss, 0) = 0.0f;            // route metric
+    *il<float>(w, a.o_secs, ss, 0) = 0.0f;             // agg seconds
+    *il<double>(w, a.o_delta, ss, 0) = 0.0;            // simpl dist (f32 stored in f64 slot)
+    *il<uint32_t>(w, a.o_state, ss, 0) = s;
+    h.push(0.0f, 0.0f, s);
+  }
+  while (h.n > 0) {
+    const uint4 top = h.pop();
+    const uint32_t us = top.z;
+    const uint32_t uss = (uint32_t)L.skey[us];
+    uint32_t *ust = il<uint32_t>(w, a.o_state, uss, 0);
+    if (*ust & 0x80000000u) continue;
+    *ust = us | 0x80000000u;
+    const uint32_t u = us >> 1, entry = us & 1u;
+    const float u_secs = *il<float>(w, a.o_secs, uss, 0);
+    const float u_simpl = (float)*il<double>(w, a.o_delta, uss, 0);
+    for (uint32_t j = a.off[u]; j < a.off[u + 1]; ++j) {
+      const uint4 e = a.adj[j];
+      if (((e.x >> 30) & 1u) != 1u - entry) continue;
+      const uint32_t v = e.x & 0x3fffffffu;
+      const uint32_t vs = 2 * v + (e.x >> 31);
+      const float cs = u_secs + __uint_as_float(e.y);
+      if (cs > a.maxsec) continue;
+      const float csimpl = u_simpl + __uint_as_float(e.z);
+      const float cmetric = csimpl + __uint_as_float(e.w);
+      const uint64_t sk = L.skey[vs];
+      uint32_t vss;
+      float vmetric, vsecs;
+      if ((uint32_t)(sk >> 32) == epoch) {
+        vss = (uint32_t)sk;
+        vmetric = *il<float>(w, a.o_route, vss, 0);
+        vsecs = *il<float>(w, a.o_secs, vss, 0);
+      } else {
+        vss = 0xffffffffu;
+        vmetric = INFINITY;
+        vsecs = INFINITY;
+      }
+      const bool improved = cmetric + eps < vmetric;
+      const bool tied = fabsf(cmetric - vmetric) <= eps;
+      if (improved || (tied && cs < vsecs && us != vs)) {
+        if (vss == 0xffffffffu) {
+          if (n_states >= a.scap || h.n >= a.hcap) {
+            atomicOr(a.fail_flags, kFailCap);
+            return;
+          }
+          vss = n_states++;
+          L.skey[vs] = ((uint64_t)epoch << 32) | vss;
+          *il<uint32_t>(w, a.o_state, vss, 0) = vs;
+        }
+        *il<float>(w, a.o_route, vss, 0) = cmetric;
+        *il<double>(w, a.o_delta, vss, 0) = (double)csimpl;
+        *il<float>(w, a.o_secs, vss, 0) = cs;
+        if (h.n >= a.hcap) {
+          atomicOr(a.fail_flags, kFailHeap);
+          return;
+        }
+        h.push(cmetric, cs, vs);
+        // collapsed node view (centrality.rs:1307-1326); t_vis: 0 unreached, 1 reached
+        const bool first = a.t_vis[v] == 0;
+        if (first) a.t_order[no++] = v;
+        const float nsimpl = a.t_simpl[v];
+        const bool nimp = csimpl + eps < nsimpl;
+        const bool ntied = fabsf(csimpl - nsimpl) <= eps;
+        if (first || nimp || (ntied && cs < a.t_secs[v])) {
+          a.t_vis[v] = 1;
+          a.t_simpl[v] = csimpl;
+          a.t_secs[v] = cs;
+          a.t_pred[v] = (int32_t)u;
+        }
+      }
+    }
+  }
+  *a.t_norder = no;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct AngPlan {
+  uint32_t scap, ncap, hcap, maxp, lanes;
+  size_t lane_stride, wave_stride;
+  size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_delta, o_seed, o_node, o_brt, o_bsec, o_heap;
+};
+
+void ang_layout(AngPlan &p, uint32_t N) {
+  p.lane_stride = align256(8ull * 3 * N);  // skey [2N] + nkey [N]
+  size_t o = 0;
+  auto take = [&](size_t &field, size_t elem_bytes, size_t count) {
+    field = o;
+    o = align256(o + elem_bytes * count * kWave);
+  };
+  take(p.o_route, 4, p.scap);
+  take(p.o_secs, 4, p.scap);
+  take(p.o_sigma, 8, p.scap);
+  take(p.o_state, 4, p.scap);
+  take(p.o_np, 4, p.scap);
+  take(p.o_preds, 4, (size_t)p.scap * p.maxp);
+  take(p.o_order, 4, p.scap);
+  take(p.o_delta, 8, p.scap);
+  take(p.o_seed, 8, p.scap);
+  take(p.o_node, 4, p.ncap);
+  take(p.o_brt, 4, p.ncap);
+  take(p.o_bsec, 4, p.ncap);
+  take(p.o_heap, 16, p.hcap);
+  p.wave_stride = o;
+}
+
+struct AngWorkspace {
+  int device = -1;
+  char *lane_ws = nullptr, *wave_ws = nullptr;
+  uint32_t *epochs = nullptr;
+  size_t lane_bytes = 0, wave_bytes = 0;
+  uint32_t lanes = 0, N = 0;
+  size_t lane_stride = 0, wave_stride = 0;
+  uint64_t epoch_hi = 0;
+  std::mutex mu;
+};
+
+std::mutex g_ang_mu;
+std::map<int, std::unique_ptr<AngWorkspace>> g_ang;
+
+AngWorkspace &ang_ws(int device) {
+  std::lock_guard<std::mutex> lk(g_ang_mu);
+  auto &p = g_ang[device];
+  if (!p) {
+    p = std::make_unique<AngWorkspace>();
+    p->device = device;
+  }
+  return *p;
+}
+
+void prepare_ang(AngWorkspace &w, const AngPlan &p, uint32_t N, uint64_t sources, hipStream_t stream) {
+  const size_t lane_need = p.lane_stride * p.lanes;
+  const size_t wave_need = p.wave_stride * (p.lanes / kWave);
+  bool reinit = false;
+  if (lane_need > w.lane_bytes || wave_need > w.wave_bytes || w.N != N || w.lane_stride != p.lane_stride ||
+      w.wave_stride != p.wave_stride || p.lanes > w.lanes) {
+    (void)hipFree(w.lane_ws);
+    (void)hipFree(w.wave_ws);
+    (void)hipFree(w.epochs);
+    w.lane_ws = w.wave_ws = nullptr;
+    w.epochs = nullptr;
+    if (hipMalloc(&w.lane_ws, lane_need) != hipSuccess || hipMalloc(&w.wave_ws, wave_need) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(w.lane_ws);
+      w.lane_ws = nullptr;
+      w.lane_bytes = w.wave_bytes = 0;
+      fail(CS_ENOMEM, "could not allocate the angular search workspace");
+    }
+    hip_check(hipMalloc(&w.epochs, 4ull * p.lanes), "hipMalloc epochs");
+    w.lane_bytes = lane_need;
+    w.wave_bytes = wave_need;
+    w.lanes = p.lanes;
+    w.N = N;
+    w.lane_stride = p.lane_stride;
+    w.wave_stride = p.wave_stride;
+    reinit = true;
+  }
+  if (w.epoch_hi + sources + 2 >= 0xfffffff0ull) reinit = true;
+  if (reinit) {
+    hip_check(hipMemsetAsync(w.lane_ws, 0, lane_need, stream), "memset");
+    hip_check(hipMemsetAsync(w.epochs, 0, 4ull * w.lanes, stream), "memset");
+    w.epoch_hi = 0;
+  }
+  w.epoch_hi += sources;
+}
+
+}  // namespace
+
+// Forward adjacency of the dual with endpoint slots, angle and tie-break
+// increments (built once per speed; centrality.rs:533-566,638-667).
+static uint4 *angular_adjacency(Graph &g, DevEdges &de) {
+  if (de.ang_adj) return de.ang_adj;
+  std::vector<uint4> adj(g.m);
+  float min_es = INFINITY;
+  for (uint32_t v = 0; v < g.n; ++v) {
+    for (uint32_t j = g.fwd_off[v]; j < g.fwd_off[v + 1]; ++j) {
+      const uint32_t e = g.fwd_eid[j];
+      const uint32_t u = g.edst[e];
+      const float es = edge_seconds(g, v, u, e, de.speed, false);
+      const float tb = kAngularTieBreak * g.elen[e];
+      uint32_t eb, ab, tbb;
+      std::memcpy(&eb, &es, 4);
+      std::memcpy(&ab, &g.eang[e], 4);
+      std::memcpy(&tbb, &tb, 4);
+      adj[j] = make_uint4(u | ((uint32_t)g.slot_src[e] << 30) | ((uint32_t)g.slot_dst[e] << 31), eb, ab, tbb);
+      min_es = std::min(min_es, es);
+    }
+  }
+  hip_check(hipMalloc(&de.ang_adj, sizeof(uint4) * std::max<size_t>(adj.size(), 1)), "hipMalloc");
+  if (!adj.empty())
+    hip_check(hipMemcpy(de.ang_adj, adj.data(), sizeof(uint4) * adj.size(), hipMemcpyHostToDevice), "upload");
+  de.min_es_ang = min_es;
+  return de.ang_adj;
+}
+
+void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const uint32_t *d_sources,
+                     const float *d_wt, uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
+                     uint32_t *d_reach, hipStream_t stream, cs_stats &st) {
+  if (n_sources == 0) return;
+  if (g.n >= (1u << 30)) fail(CS_EINVAL, "dual graph too large for the angular search (node ids need 30 bits)");
+  const uint32_t N = g.n;
+  uint4 *adj = angular_adjacency(g, de);
+  AngPlan p;
+  p.maxp = std::max<uint32_t>(1, g.max_in_deg);
+  p.scap = std::min<uint32_t>(2 * N, 8192);
+  p.ncap = std::min<uint32_t>(N, p.scap);
+  p.hcap = 2 * p.scap;
+  int dev = dg.device;
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, dev), "props");
+  size_t free_b = 0, total_b = 0;
+  hip_check(hipMemGetInfo(&free_b, &total_b), "meminfo");
+  AngWorkspace &wsp = ang_ws(dev);
+  std::lock_guard<std::mutex> lk(wsp.mu);
+  uint32_t *d_ctl = nullptr, *d_fail = nullptr, *d_src_cur = nullptr;
+  float *d_wt_cur = nullptr;
+  unsigned long long *d_stats = nullptr;
+  hip_check(hipMalloc(&d_ctl, 16), "hipMalloc");
+  hip_check(hipMalloc(&d_fail, 4ull * n_sources), "hipMalloc");
+  hip_check(hipMalloc(&d_stats, 32), "hipMalloc");
+  hip_check(hipMemsetAsync(d_stats, 0, 32, stream), "memset");
+  const uint32_t *src_list = d_sources;
+  const float *wt_list = d_wt;
+  uint32_t n_list = n_sources;
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  double kms = 0.0;
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    ang_layout(p, N);
+    const size_t per_lane = p.lane_stride + p.wave_stride / kWave;
+    const size_t budget = free_b / 4 + wsp.lane_bytes + wsp.wave_bytes;
+    uint32_t lanes = (uint32_t)std::min<size_t>(budget / per_lane, (size_t)n_list);
+    lanes = std::min<uint32_t>(lanes, (uint32_t)prop.multiProcessorCount * 4 * kWave * 8);
+    lanes = (lanes + kBlock - 1) / kBlock * kBlock;
+    if (lanes == 0 || (size_t)lanes * per_lane > budget + per_lane * kBlock)
+      fail(CS_ENOMEM, "graph too large for the angular search workspace on this device");
+    p.lanes = lanes;
+    prepare_ang(wsp, p, N, n_list, stream);
+    hip_check(hipMemsetAsync(d_ctl, 0, 16, stream), "memset");
+    AngArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.N = N;
+    a.off = dg.fwd_off;
+    a.adj = adj;
+    a.eligible = d_eligible;
+    a.T = cs.T;
+    a.maxsec = cs.maxsec;
+    a.tolerance = cs.tolerance;
+    a.unit = cs.ang_unit;
+    a.offset = cs.far_offset;
+    for (uint32_t i = 0; i < kMaxT; ++i) a.sec_thr[i] = i < cs.T ? (float)cs.seconds[i] : 0.0f;
+    a.n_sources = n_list;
+    a.sources = src_list;
+    a.src_wt = wt_list;
+    a.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
+    a.acc = d_acc;
+    a.reach = d_reach;
+    a.stats = d_stats;
+    a.ws = wsp.lane_ws;
+    a.wave_ws = wsp.wave_ws;
+    a.lane_stride = p.lane_stride;
+    a.wave_stride = p.wave_stride;
+    a.scap = p.scap;
+    a.ncap = p.ncap;
+    a.hcap = p.hcap;
+    a.maxp = p.maxp;
+    a.o_route = p.o_route;
+    a.o_secs = p.o_secs;
+    a.o_sigma = p.o_sigma;
+    a.o_state = p.o_state;
+    a.o_np = p.o_np;
+    a.o_preds = p.o_preds;
+    a.o_order = p.o_order;
+    a.o_delta = p.o_delta;
+    a.o_seed = p.o_seed;
+    a.o_node = p.o_node;
+    a.o_brt = p.o_brt;
+    a.o_bsec = p.o_bsec;
+    a.o_heap = p.o_heap;
+    a.epochs = wsp.epochs;
+    a.next_source = d_ctl;
+    a.n_fail = d_ctl + 1;
+    a.fail_flags = d_ctl + 2;
+    a.fail_list = d_fail;
+    hip_check(hipEventRecord(e0, stream), "event");
+    hipLaunchKernelGGL(k_angular, dim3(lanes / kBlock), dim3(kBlock), 0, stream, a);
+    hip_check(hipGetLastError(), "k_angular launch");
+    hip_check(hipEventRecord(e1, stream), "event");
+    uint32_t ctl[4];
+    hip_check(hipMemcpyAsync(ctl, d_ctl, 16, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    kms += ms;
+    st.waves = lanes / kWave;
+    const uint32_t n_failed = ctl[1], flags = ctl[2];
+    if (n_failed == 0) break;
+    st.retries += n_failed;
+    if (!d_src_cur) {
+      hip_check(hipMalloc(&d_src_cur, 4ull * n_sources), "hipMalloc");
+      hip_check(hipMalloc(&d_wt_cur, 4ull * n_sources), "hipMalloc");
+    }
+    std::vector<uint32_t> fl(n_failed), srcs(n_list);
+    std::vector<float> wts(n_list);
+    hip_check(hipMemcpy(fl.data(), d_fail, 4ull * n_failed, hipMemcpyDeviceToHost), "copy");
+    hip_check(hipMemcpy(srcs.data(), src_list, 4ull * n_list, hipMemcpyDeviceToHost), "copy");
+    hip_check(hipMemcpy(wts.data(), wt_list, 4ull * n_list, hipMemcpyDeviceToHost), "copy");
+    std::vector<uint32_t> ns(n_failed);
+    std::vector<float> nw(n_failed);
+    for (uint32_t i = 0; i < n_failed; ++i) {
+      ns[i] = srcs[fl[i]];
+      nw[i] = wts[fl[i]];
+    }
+    hip_check(hipMemcpy(d_src_cur, ns.data(), 4ull * n_failed, hipMemcpyHostToDevice), "copy");
+    hip_check(hipMemcpy(d_wt_cur, nw.data(), 4ull * n_failed, hipMemcpyHostToDevice), "copy");
+    src_list = d_src_cur;
+    wt_list = d_wt_cur;
+    n_list = n_failed;
+    if (flags & kFailCap) p.scap = std::min<uint32_t>(2 * N, p.scap * 4);
+    p.ncap = std::min<uint32_t>(N, p.scap);
+    if (flags & (kFailHeap | kFailCap)) p.hcap = std::max<uint32_t>(p.hcap * 4, 2 * p.scap);
+    if (flags & kFailPreds) fail(CS_EINTERNAL, "predecessor list overflow in the angular search");
+    if (attempt == 5) fail(CS_EINTERNAL, "angular workspace retries exhausted");
+  }
+  unsigned long long stv[4];
+  hip_check(hipMemcpy(stv, d_stats, 32, hipMemcpyDeviceToHost), "copy");
+  st.kernel_ms += kms;
+  st.nodes_reached += stv[0];
+  st.edges_scanned += stv[1];
+  st.within_sum += stv[2];
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_ctl);
+  (void)hipFree(d_fail);
+  (void)hipFree(d_stats);
+  (void)hipFree(d_src_cur);
+  (void)hipFree(d_wt_cur);
+}
+
+void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec, float *d_simpl,
+                          float *d_secs, int32_t *d_pred, uint8_t *d_vis, uint32_t *d_order, uint32_t *d_norder,
+                          hipStream_t stream) {
+  const uint32_t N = g.n;
+  uint4 *adj = angular_adjacency(g, de);
+  AngPlan p;
+  p.maxp = 1;
+  p.scap = 2 * N;
+  p.ncap = 1;
+  p.hcap = 8 * N + 16;
+  p.lanes = kWave;
+  ang_layout(p, N);
+  // single lane: use a private workspace
+  char *lane_ws = nullptr, *wave_ws = nullptr;
+  uint32_t *epochs = nullptr, *ctl = nullptr;
+  hip_check(hipMalloc(&lane_ws, p.lane_stride), "hipMalloc");
+  hip_check(hipMalloc(&wave_ws, p.wave_stride), "hipMalloc");
+  hip_check(hipMalloc(&epochs, 4), "hipMalloc");
+  hip_check(hipMalloc(&ctl, 16), "hipMalloc");
+  hip_check(hipMemsetAsync(lane_ws, 0, p.lane_stride, stream), "memset");
+  hip_check(hipMemsetAsync(epochs, 0, 4, stream), "memset");
+  hip_check(hipMemsetAsync(ctl, 0, 16, stream), "memset");
+  AngArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.N = N;
+  a.off = dg.fwd_off;
+  a.adj = adj;
+  a.maxsec = maxsec;
+  a.ws = lane_ws;
+  a.wave_ws = wave_ws;
+  a.lane_stride = p.lane_stride;
+  a.wave_stride = p.wave_stride;
+  a.scap = p.scap;
+  a.ncap = p.ncap;
+  a.hcap = p.hcap;
+  a.maxp = p.maxp;
+  a.o_route = p.o_route;
+  a.o_secs = p.o_secs;
+  a.o_sigma = p.o_sigma;
+  a.o_state = p.o_state;
+  a.o_np = p.o_np;
+  a.o_preds = p.o_preds;
+  a.o_order = p.o_order;
+  a.o_delta = p.o_delta;
+  a.o_seed = p.o_seed;
+  a.o_node = p.o_node;
+  a.o_brt = p.o_brt;
+  a.o_bsec = p.o_bsec;
+  a.o_heap = p.o_heap;
+  a.epochs = epochs;
+  a.fail_flags = ctl + 2;
+  a.t_simpl = d_simpl;
+  a.t_secs = d_secs;
+  a.t_pred = d_pred;
+  a.t_vis = d_vis;
+  a.t_order = d_order;
+  a.t_norder = d_norder;
+  hipLaunchKernelGGL(k_tree_simplest, dim3(1), dim3(kWave), 0, stream, a, src);
+  hip_check(hipGetLastError(), "k_tree_simplest launch");
+  uint32_t hctl[4];
+  hip_check(hipMemcpyAsync(hctl, ctl, 16, hipMemcpyDeviceToHost, stream), "copy");
+  hip_check(hipStreamSynchronize(stream), "sync");
+  (void)hipFree(lane_ws);
+  (void)hipFree(wave_ws);
+  (void)hipFree(epochs);
+  (void)hipFree(ctl);
+  if (hctl[2]) fail(CS_EINTERNAL, "angular tree workspace exceeded");
 }
 
 }  // namespace cs

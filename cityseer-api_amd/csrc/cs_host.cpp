@@ -886,21 +886,52 @@ int cs_tree_simplest(cs_graph *h, uint32_t src, uint32_t max_seconds, float spee
                      uint32_t *n_order) {
   return guarded([&] {
     if (!h) fail(CS_EINVAL, "graph must not be NULL");
-    if (!h->g.is_dual)
+    Graph &g = h->g;
+    if (!g.is_dual)
       fail(CS_EINVAL,
            "dijkstra_tree_simplest requires a dual graph for angular analysis. Convert the graph with "
            "cityseer.tools.graphs.nx_to_dual(...) before ingesting it into NetworkStructure.");
-    fail(CS_EINVAL, "dijkstra_tree_simplest is not available on the GPU path yet");
-    (void)src;
-    (void)max_seconds;
-    (void)speed;
-    (void)visited;
-    (void)discovered;
-    (void)pred;
-    (void)simpl_dist;
-    (void)agg_seconds;
-    (void)order;
-    (void)n_order;
+    if (src >= g.n)
+      fail(CS_EINVAL, "src_idx " + std::to_string(src) + " out of range for network with node_bound " +
+                          std::to_string(g.n));
+    if (!std::isfinite(speed) || !(speed > 0.0f)) fail(CS_EINVAL, "speed_m_s must be finite and positive");
+    if (!g.slots_ok) fail(CS_EINVAL, g.slots_err);
+    const int dev = current_device();
+    const uint32_t N = g.n;
+    std::lock_guard<std::mutex> lk(g.mu);
+    DevGraph &dg = device_graph(g, dev);
+    DevEdges &de = device_edges(g, dg, speed);
+    hipStream_t stream;
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    std::vector<float> hs(N, INFINITY), hsec(N, INFINITY);
+    std::vector<int32_t> hp(N, -1);
+    std::vector<uint8_t> hv(N, 0);
+    std::vector<uint32_t> ho(N);
+    uint32_t hn = 0;
+    DevBuf d_s(4ull * N), d_sec(4ull * N), d_p(4ull * N), d_v(N), d_o(4ull * N), d_n(4);
+    hip_check(hipMemcpyAsync(d_s.p, hs.data(), 4ull * N, hipMemcpyHostToDevice, stream), "copy");
+    hip_check(hipMemcpyAsync(d_sec.p, hsec.data(), 4ull * N, hipMemcpyHostToDevice, stream), "copy");
+    hip_check(hipMemcpyAsync(d_p.p, hp.data(), 4ull * N, hipMemcpyHostToDevice, stream), "copy");
+    hip_check(hipMemsetAsync(d_v.p, 0, N, stream), "memset");
+    launch_tree_simplest(g, dg, de, src, (float)max_seconds, d_s.as<float>(), d_sec.as<float>(), d_p.as<int32_t>(),
+                         d_v.as<uint8_t>(), d_o.as<uint32_t>(), d_n.as<uint32_t>(), stream);
+    hip_check(hipMemcpyAsync(hs.data(), d_s.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipMemcpyAsync(hsec.data(), d_sec.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipMemcpyAsync(hp.data(), d_p.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipMemcpyAsync(hv.data(), d_v.p, N, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipMemcpyAsync(ho.data(), d_o.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipMemcpyAsync(&hn, d_n.p, 4, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    hip_check(hipStreamDestroy(stream), "stream destroy");
+    for (uint32_t v = 0; v < N; ++v) {
+      visited[v] = hv[v];
+      discovered[v] = hv[v];
+      pred[v] = hp[v];
+      simpl_dist[v] = hs[v];
+      agg_seconds[v] = hsec[v];
+    }
+    if (order) std::copy(ho.begin(), ho.begin() + hn, order);
+    if (n_order) *n_order = hn;
   });
 }
 

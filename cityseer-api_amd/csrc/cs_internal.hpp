@@ -142,4 +142,10 @@ void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
                      hipStream_t stream, cs_stats &st);
 
+// dijkstra_tree_simplest for one source (cs_angular.hip); device arrays of
+// length N, pre-filled by the caller (INF / -1 / 0).
+void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec, float *d_simpl,
+                          float *d_secs, int32_t *d_pred, uint8_t *d_vis, uint32_t *d_order, uint32_t *d_norder,
+                          hipStream_t stream);
+
 }  // namespace cs
