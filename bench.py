@@ -1,15 +1,20 @@
-"""Benchmark: source searches/sec of shortest-path centrality on one MI355X.
+"""Benchmark: source searches/sec (SSPS) of localised network centrality on one MI355X.
 
-Workload (the configuration the metric is quoted on): a seeded 200x200
-perturbed lattice (xy jitter +-20 m, 10% of edges dropped, 5% diagonal
-shortcuts, largest component, f32 lengths = Euclidean x U(1.0, 1.3)),
+Default workload (``--mode plain``, the configuration the metric is quoted on): a
+seeded 200x200 perturbed lattice (xy jitter +-20 m, 10% of edges dropped, 5%
+diagonal shortcuts, largest component, f32 lengths = Euclidean x U(1.0, 1.3)),
 node_centrality_shortest at [500, 1000, 2000, 5000] m, closeness and
-betweenness, all ~40k sources.
+betweenness, all ~40k sources. A step is one full centrality pass over every
+source with the graph, sources and output accumulators resident in HBM
+(cs_centrality_shortest_device); the timed region covers K steps between
+device synchronisations.
 
-A step is one full centrality pass over every source with the graph, sources
-and output accumulators resident in HBM (cs_centrality_shortest_device); the
-timed region covers K steps between device synchronisations. Prints one JSON
-line.
+``--mode simplest`` / ``--mode segment``: the dual (simplest-path, angular
+impedance) and the primal (segment) of a 150x150 perturbed lattice at
+[500, 1000, 2000] m through the NetworkStructure methods; ``value`` uses the
+kernel time of each call (graph resident, sources staged before the kernel).
+
+Prints one JSON line.
 """
 
 from __future__ import annotations
@@ -39,6 +44,19 @@ def build_workload(side: int, diag: float, seed: int):
     lat = mock.perturbed_lattice(side, seed=seed, diag_frac=diag)
     ns = mock.network_structure_from_lattice(lat)
     return lat, ns
+
+
+def profiled_traffic(name: str = "k_search_brandes_cfg2_hbm.md"):
+    """HBM bytes per launch of the dominant kernel on this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (counters scaled by a plain-copy calibration); None if absent."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(ns, target_s: float):
@@ -73,6 +91,280 @@ def cpu_baseline(ns, target_s: float):
     }
 
 
+VARIANT_DISTANCES = [500, 1000, 2000]
+VARIANT_SIDE = 150
+
+
+def variant_cpu_baseline(ns, kind: str, target_s: float):
+    """The oracle on the host cores over a bounded random sample of sources (simplest / segment)."""
+    import numpy as np
+
+    from oracle.oracle import OracleGraph
+
+    og = OracleGraph.from_network_structure(ns)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    n = ns.node_count()
+    rng = np.random.default_rng(321)
+
+    def run(k):
+        src = np.sort(rng.choice(n, k, replace=False)).astype(np.uint32)
+        t0 = time.perf_counter()
+        if kind == "simplest":
+            og.centrality_simplest(VARIANT_DISTANCES, sources=src, n_threads=threads)
+        else:
+            og.segment_centrality(VARIANT_DISTANCES, sources=src, n_threads=threads)
+        return time.perf_counter() - t0
+
+    k0 = max(threads * 4, 64)
+    rate = k0 / max(run(k0), 1e-6)
+    k = int(min(n, max(k0, rate * target_s)))
+    t = run(k)
+    return {
+        "value": round(k / t, 2),
+        "unit": "source searches/sec",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{k} random sources of the same workload ({t:.1f} s wall, {threads} threads)",
+    }
+
+
+def run_variant(args) -> int:
+    """Simplest-path centrality on the dual, or segment centrality on the primal, of a 150x150
+    perturbed lattice at [500, 1000, 2000] m, all sources."""
+    import numpy as np
+
+    from cityseer import _native
+    from cityseer.tools import mock
+
+    kind = args.mode
+    log(f"building {kind} workload")
+    lat = mock.perturbed_lattice(VARIANT_SIDE, seed=args.seed)
+    if kind == "simplest":
+        lat = mock.lattice_dual(lat)
+    ns = mock.network_structure_from_lattice(lat)
+    n, m = ns.node_count(), ns.edge_count
+
+    def step():
+        if kind == "simplest":
+            ns.centrality_simplest(distances=VARIANT_DISTANCES)
+        else:
+            ns.segment_centrality(distances=VARIANT_DISTANCES)
+        return _native.last_stats()
+
+    for _ in range(args.warmup):
+        step()
+    kernel_ms, total_ms = [], []
+    stats = None
+    for i in range(args.steps):
+        stats = step()
+        kernel_ms.append(stats["kernel_ms"])
+        total_ms.append(stats["total_ms"])
+        log(f"step {i}: kernel {stats['kernel_ms']:.2f} ms, call {stats['total_ms']:.2f} ms")
+    kms = float(np.mean(kernel_ms))
+    metrics = 4
+    algo_bytes = 12 * stats["edges_scanned"] + 16 * stats["nodes_reached"] + 16 * metrics * stats["within_sum"]
+    achieved = algo_bytes / (kms / 1000.0) / 1e9
+    cpu = None if args.no_cpu else variant_cpu_baseline(ns, kind, args.cpu_seconds)
+    call = "node_centrality_simplest (dual, angular)" if kind == "simplest" else "segment_centrality (primal)"
+    line = {
+        "metric": METRIC,
+        "value": round(n / (kms / 1000.0), 2),
+        "unit": "source searches/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(kms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 accumulators)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{VARIANT_SIDE}x{VARIANT_SIDE} perturbed lattice, {call} {VARIANT_DISTANCES} m, "
+                        "closeness+betweenness, all sources",
+            "nodes": n,
+            "directed_edges": m,
+            "sources": n,
+            "mode": kind,
+            "call_ms": round(float(np.mean(total_ms)), 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel_ms": round(kms, 3),
+            "algorithmic_bytes": int(algo_bytes),
+            "edges_scanned": int(stats["edges_scanned"]),
+            "nodes_reached": int(stats["nodes_reached"]),
+            "within_sum": int(stats["within_sum"]),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+BATCH_DISTANCES = [400, 800, 1600, 3200, 6400]
+
+
+def batch_workload(n_graphs: int, seed: int = 0):
+    """Graphs with node counts log-uniform in [1,000, 60,000] (side = round(sqrt(N))), seeds 0..n-1,
+    each with 2-4 distances drawn from {400, 800, 1600, 3200, 6400}."""
+    import numpy as np
+
+    from cityseer.tools import mock
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_graphs):
+        n_target = float(np.exp(rng.uniform(np.log(1e3), np.log(6e4))))
+        k = int(rng.integers(2, 5))
+        dists = sorted(rng.choice(BATCH_DISTANCES, k, replace=False).tolist())
+        side = max(3, int(round(np.sqrt(n_target))))
+        lat = mock.perturbed_lattice(side, seed=i)
+        out.append((mock.network_structure_from_lattice(lat), [int(d) for d in dists]))
+    return out
+
+
+def run_batch(args) -> int:
+    """Many graphs of uneven size, each with its own thresholds: one autotuned persistent launch
+    (hipGraph replay) against one plain device-resident launch per graph."""
+    import numpy as np
+    import torch
+
+    from cityseer import _native, rustalgos
+    from cityseer.batch import CentralityBatch
+
+    log(f"building {args.graphs} graphs")
+    work = batch_workload(args.graphs, args.seed)
+    total_sources = sum(ns.node_count() for ns, _ in work)
+    lib = _native.load()
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    stream = torch.cuda.current_stream(dev)
+
+    # plain: one device-resident launch per graph
+    plain = []
+    for ns, dists in work:
+        n = ns.node_count()
+        d, b, s = rustalgos.pair_distances_betas_time(1.33333, distances=dists)
+        t = len(d)
+        bufs = dict(
+            src=torch.arange(n, dtype=torch.int32, device=dev), wt=torch.ones(n, dtype=torch.float32, device=dev),
+            el=torch.ones(n, dtype=torch.uint8, device=dev), acc=torch.zeros(n * 7 * t, dtype=torch.float64, device=dev),
+            reach=torch.zeros(8, dtype=torch.int32, device=dev),
+            d=np.asarray(d, np.uint32), b=np.asarray(b, np.float32), s=np.asarray(s, np.uint32))
+        p = _native.CsParams()
+        p.n_thresholds = t
+        p.distances = _native.ptr(bufs["d"], ctypes.c_uint32)
+        p.betas = _native.ptr(bufs["b"], ctypes.c_float)
+        p.seconds = _native.ptr(bufs["s"], ctypes.c_uint32)
+        p.speed_m_s = 1.33333
+        p.tolerance = 1e-4
+        p.compute_closeness = 1
+        p.compute_betweenness = 1
+        p.n_sources = n
+        p.sources = ctypes.cast(bufs["src"].data_ptr(), ctypes.POINTER(ctypes.c_uint32))
+        p.source_wt = ctypes.cast(bufs["wt"].data_ptr(), ctypes.POINTER(ctypes.c_float))
+        p.eligible = ctypes.cast(bufs["el"].data_ptr(), ctypes.POINTER(ctypes.c_uint8))
+        p.betw_scale = 1.0
+        p.angular_scaling_unit = 180.0
+        p.farness_scaling_offset = 1.0
+        plain.append((ns._native_handle(), p, bufs))
+
+    def plain_pass():
+        kms = 0.0
+        for h, p, bufs in plain:
+            bufs["acc"].zero_()
+            _native.check(lib.cs_centrality_shortest_device(h, ctypes.byref(p), bufs["wt"].data_ptr(),
+                                                            bufs["acc"].data_ptr(), bufs["reach"].data_ptr(),
+                                                            stream.cuda_stream))
+            kms += _native.last_stats()["kernel_ms"]
+        torch.cuda.synchronize()
+        return kms
+
+    log("plain warmup")
+    plain_pass()
+    t0 = time.perf_counter()
+    plain_kms = 0.0
+    for _ in range(args.steps):
+        plain_kms += plain_pass()
+    plain_s = (time.perf_counter() - t0) / args.steps
+    log(f"plain: {plain_s * 1000:.1f} ms per pass over {len(work)} graphs")
+
+    batch = CentralityBatch()
+    for ns, dists in work:
+        batch.add_shortest(ns, distances=dists)
+    log("autotune")
+    t0 = time.perf_counter()
+    tune = batch.autotune(0.03)
+    tune_s = time.perf_counter() - t0
+    log(f"chosen {tune} in {tune_s:.1f} s")
+    for _ in range(max(args.warmup, 1)):
+        batch.run()
+    kernel_ms = []
+    stats = None
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        stats = batch.run()
+        kernel_ms.append(stats["kernel_ms"])
+        log(f"batched step {i}: {stats['kernel_ms']:.1f} ms kernel, {stats['total_ms']:.1f} ms call")
+    batch_s = (time.perf_counter() - t0) / args.steps
+    kms = float(np.mean(kernel_ms))
+    algo_bytes = 12 * stats["edges_scanned"] + 16 * stats["nodes_reached"] + 16 * 7 * stats["within_sum"]
+    achieved = algo_bytes / (kms / 1000.0) / 1e9
+    value = total_sources / batch_s
+    plain_value = total_sources / plain_s
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "source searches/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_s * 1000.0, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 accumulators)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(work)} perturbed lattices, N log-uniform in [1000, 60000], 2-4 distances each "
+                        f"from {BATCH_DISTANCES} m, node_centrality_shortest closeness+betweenness, all sources; "
+                        "one autotuned persistent launch replayed from a hipGraph",
+            "graphs": len(work),
+            "nodes_total": total_sources,
+            "sources": total_sources,
+            "mode": "batch",
+            "tuning": tune,
+            "autotune_s": round(tune_s, 2),
+            "plain_value": round(plain_value, 2),
+            "plain_ms_per_pass": round(plain_s * 1000.0, 3),
+            "plain_kernel_ms": round(plain_kms / args.steps, 3),
+            "batched_over_plain": round(value / plain_value, 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel_ms": round(kms, 3),
+            "algorithmic_bytes": int(algo_bytes),
+            "edges_scanned": int(stats["edges_scanned"]),
+            "nodes_reached": int(stats["nodes_reached"]),
+            "within_sum": int(stats["within_sum"]),
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +375,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["plain", "simplest", "segment", "batch"], default="plain")
+    ap.add_argument("--graphs", type=int, default=64, help="batch mode: number of graphs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (3 times), to scale HBM counters")
     args = ap.parse_args()
@@ -104,6 +398,11 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": src.numel() * 8, "copies": 3}), flush=True)
         return 0
+
+    if args.mode in ("simplest", "segment"):
+        return run_variant(args)
+    if args.mode == "batch":
+        return run_batch(args)
 
     log("building workload")
     lat, ns = build_workload(args.side, args.diag, args.seed)
@@ -199,7 +498,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(),
             "kernel_ms": round(kms, 3),
             "algorithmic_bytes": int(algo_bytes),
             "edges_scanned": int(stats["edges_scanned"]),

@@ -13,7 +13,7 @@ import importlib
 
 __version__ = "0.1.0"
 
-_SUBMODULES = ("config", "metrics", "rustalgos", "tools", "serving")
+_SUBMODULES = ("batch", "config", "metrics", "rustalgos", "tools", "serving")
 
 
 def __getattr__(name):

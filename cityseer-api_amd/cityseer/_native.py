@@ -64,6 +64,16 @@ class CsStats(ctypes.Structure):
     ]
 
 
+class CsTuning(ctypes.Structure):
+    _fields_ = [
+        ("block_threads", ctypes.c_uint32),
+        ("waves_per_simd", ctypes.c_uint32),
+        ("delta_scale", ctypes.c_float),
+        ("slot_capacity", ctypes.c_uint32),
+        ("xcd_split", ctypes.c_uint32),
+    ]
+
+
 # every symbol the header declares, with (restype, argtypes)
 _SIGNATURES = {
     "cs_last_error": (ctypes.c_char_p, []),
@@ -114,6 +124,15 @@ _SIGNATURES = {
          _u32p, _u32p, ctypes.c_uint32],
     ),
     "cs_last_stats": (ctypes.c_int, [ctypes.POINTER(CsStats)]),
+    "cs_set_tuning": (ctypes.c_int, [ctypes.POINTER(CsTuning)]),
+    "cs_get_tuning": (ctypes.c_int, [ctypes.POINTER(CsTuning)]),
+    "cs_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "cs_batch_destroy": (None, [ctypes.c_void_p]),
+    "cs_batch_add_shortest": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CsParams), _f64p, _u32p]),
+    "cs_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(CsTuning)]),
+    "cs_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CsTuning)]),
+    "cs_batch_run": (ctypes.c_int, [ctypes.c_void_p]),
+    "cs_batch_results": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 _lib = None

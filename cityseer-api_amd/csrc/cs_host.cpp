@@ -179,6 +179,10 @@ DevGraph &device_graph(Graph &g, int device) {
   dg->device = device;
   dg->rev_off = dev_upload(g.rev_off);
   dg->fwd_off = dev_upload(g.fwd_off);
+  std::vector<uint4> rows(g.n);
+  for (uint32_t v = 0; v < g.n; ++v)
+    rows[v] = make_uint4(g.fwd_off[v], g.fwd_off[v + 1], g.rev_off[v], g.rev_off[v + 1]);
+  dg->rows = dev_upload(rows);
   DevGraph &ref = *dg;
   g.dev[device] = std::move(dg);
   return ref;
@@ -187,10 +191,6 @@ DevGraph &device_graph(Graph &g, int device) {
 DevEdges &device_edges(Graph &g, DevGraph &dg, float speed) {
   for (auto &e : dg.edges)
     if (e->speed == speed) return *e;
-  if (dg.edges.size() >= 4) {
-    free_edges(*dg.edges.front());
-    dg.edges.erase(dg.edges.begin());
-  }
   auto d = std::make_unique<DevEdges>();
   d->speed = speed;
   const uint32_t m = g.m;
@@ -391,11 +391,15 @@ SourceSet source_set(const Graph &g, const cs_params *p) {
   return s;
 }
 
-// acc [N][M][T] (device, node-major) -> out [M][T][N] (host)
-void unpack_acc(const std::vector<double> &acc, uint32_t N, uint32_t M, uint32_t T, double *out) {
+// acc [N][M][stride] (device, node-major) -> out [M][T][N] (host)
+void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out) {
   for (uint32_t v = 0; v < N; ++v)
     for (uint32_t m = 0; m < M; ++m)
-      for (uint32_t i = 0; i < T; ++i) out[((size_t)m * T + i) * N + v] = acc[((size_t)v * M + m) * T + i];
+      for (uint32_t i = 0; i < T; ++i) out[((size_t)m * T + i) * N + v] = acc[((size_t)v * M + m) * stride + i];
+}
+
+void unpack_acc(const std::vector<double> &acc, uint32_t N, uint32_t M, uint32_t T, double *out) {
+  unpack_acc_strided(acc.data(), N, M, T, T, out);
 }
 
 void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s) {
@@ -417,6 +421,58 @@ void run_shortest_checks(const Graph &g, const cs_params *p) {
 }  // namespace cs
 
 using namespace cs;
+
+// Host side of a batch: each job's sources prepared as cs_centrality_shortest
+// prepares them, device buffers built on the first run.
+struct cs_batch {
+  struct JobHost {
+    cs_graph *h = nullptr;
+    CallSpec spec;
+    SourceSet ss;
+    std::vector<uint8_t> elig;
+    double *out = nullptr;
+    uint32_t *reach = nullptr;
+    bool betw = false;
+    double betw_scale = 1.0;
+    size_t acc_off = 0;
+  };
+  int device = -1;
+  std::vector<JobHost> jobs;
+  uint32_t NT = 1;
+  Tuning tune;
+  hipStream_t stream = nullptr;
+  std::vector<void *> bufs;
+  double *d_acc = nullptr;
+  size_t acc_words = 0;
+  uint32_t *d_reach = nullptr;
+  ShortestBatch *sb = nullptr;
+  bool have_results = false;
+
+  void drop_device() {
+    if (sb) batch_delete(sb);
+    sb = nullptr;
+    for (void *q : bufs) (void)hipFree(q);
+    bufs.clear();
+    if (d_acc) (void)hipFree(d_acc);
+    if (d_reach) (void)hipFree(d_reach);
+    d_acc = nullptr;
+    d_reach = nullptr;
+    have_results = false;
+  }
+  ~cs_batch() {
+    drop_device();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  template <typename T>
+  T *upload(const T *src, size_t n) {
+    void *q = nullptr;
+    hip_check(hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1)), "hipMalloc batch");
+    bufs.push_back(q);
+    if (n) hip_check(hipMemcpy(q, src, sizeof(T) * n, hipMemcpyHostToDevice), "upload batch");
+    return static_cast<T *>(q);
+  }
+  void build();
+};
 
 struct cs_graph {
   Graph g;
@@ -517,6 +573,7 @@ void cs_graph_destroy(cs_graph *h) {
     if (hipGetDevice(&prev) == hipSuccess && hipSetDevice(kv.first) == hipSuccess) {
       (void)hipFree(kv.second->rev_off);
       (void)hipFree(kv.second->fwd_off);
+      (void)hipFree(kv.second->rows);
       for (auto &e : kv.second->edges) free_edges(*e);
       (void)hipSetDevice(prev);
     }
@@ -932,6 +989,175 @@ int cs_tree_simplest(cs_graph *h, uint32_t src, uint32_t max_seconds, float spee
     }
     if (order) std::copy(ho.begin(), ho.begin() + hn, order);
     if (n_order) *n_order = hn;
+  });
+}
+
+}  // extern "C"
+
+void cs_batch::build() {
+  if (sb) return;
+  if (!stream) hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+  acc_words = 0;
+  for (auto &j : jobs) {
+    j.acc_off = acc_words;
+    acc_words += (size_t)j.h->g.n * 7 * NT;
+  }
+  hip_check(hipMalloc(&d_acc, 8 * std::max<size_t>(acc_words, 1)), "hipMalloc batch acc");
+  hip_check(hipMalloc(&d_reach, 32 * std::max<size_t>(jobs.size(), 1)), "hipMalloc batch reach");
+  std::vector<JobSpec> specs;
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    JobHost &j = jobs[i];
+    Graph &g = j.h->g;
+    const uint32_t n = (uint32_t)j.ss.src.size();
+    const uint32_t *d_src = upload(j.ss.src.data(), n);
+    const float *d_wt = upload(j.ss.wt.data(), n);
+    const float *d_cyc = upload(j.ss.cyc.data(), n);
+    const uint8_t *d_el = upload(j.elig.data(), j.elig.size());
+    std::lock_guard<std::mutex> lk(g.mu);
+    DevGraph &dg = device_graph(g, device);
+    DevEdges &de = device_edges(g, dg, j.spec.speed);
+    specs.push_back(JobSpec{&g, &dg, &de, j.spec, d_src, d_wt, d_cyc, n, d_el, d_acc + j.acc_off,
+                            d_reach + 8 * i, tune.delta_scale});
+  }
+  std::vector<ZeroRegion> zero{{reinterpret_cast<uint64_t *>(d_acc), acc_words},
+                               {reinterpret_cast<uint64_t *>(d_reach), 4 * jobs.size()}};
+  sb = batch_new(std::move(specs), std::move(zero), NT, device, tune);
+}
+
+namespace {
+Tuning from_c(const cs_tuning *t) {
+  if (!t) fail(CS_EINVAL, "tuning must not be NULL");
+  if (t->block_threads != 0 && (t->block_threads % 64 != 0 || t->block_threads > 256))
+    fail(CS_EINVAL, "block_threads must be 64, 128, 192 or 256");
+  if (!(t->delta_scale > 0.0f && t->delta_scale <= 1.0f)) fail(CS_EINVAL, "delta_scale must be in (0, 1]");
+  Tuning r;
+  r.block = t->block_threads ? t->block_threads : 256;
+  r.waves_per_simd = t->waves_per_simd;
+  r.delta_scale = t->delta_scale;
+  r.cap = t->slot_capacity;
+  r.xcd_split = t->xcd_split ? 1u : 0u;
+  return r;
+}
+cs_tuning to_c(const Tuning &t) {
+  cs_tuning r;
+  r.block_threads = t.block;
+  r.waves_per_simd = t.waves_per_simd;
+  r.delta_scale = t.delta_scale;
+  r.slot_capacity = t.cap;
+  r.xcd_split = t.xcd_split;
+  return r;
+}
+}  // namespace
+
+extern "C" {
+
+int cs_set_tuning(const cs_tuning *t) {
+  return guarded([&] { set_default_tuning(from_c(t)); });
+}
+
+int cs_get_tuning(cs_tuning *t) {
+  return guarded([&] {
+    if (!t) fail(CS_EINVAL, "tuning must not be NULL");
+    *t = to_c(default_tuning());
+  });
+}
+
+int cs_batch_create(cs_batch **out) {
+  return guarded([&] {
+    if (!out) fail(CS_EINVAL, "out must not be NULL");
+    auto b = std::make_unique<cs_batch>();
+    b->device = current_device();
+    b->tune = default_tuning();
+    *out = b.release();
+  });
+}
+
+void cs_batch_destroy(cs_batch *b) {
+  delete b;
+  (void)hipGetLastError();
+}
+
+int cs_batch_add_shortest(cs_batch *b, cs_graph *h, const cs_params *p, double *out, uint32_t *reach_totals) {
+  int idx = -1;
+  int rc = guarded([&] {
+    if (!b || !h || !out) fail(CS_EINVAL, "batch, graph and out must not be NULL");
+    Graph &g = h->g;
+    check_params(g, p);
+    run_shortest_checks(g, p);
+    cs_batch::JobHost j;
+    j.h = h;
+    j.spec = make_spec(p);
+    j.ss = source_set(g, p);
+    j.elig.assign(p->eligible, p->eligible + g.n);
+    j.out = out;
+    j.reach = reach_totals;
+    j.betw = p->compute_betweenness != 0;
+    j.betw_scale = p->betw_scale;
+    b->drop_device();
+    b->jobs.push_back(std::move(j));
+    b->NT = std::max(b->NT, p->n_thresholds);
+    idx = (int)b->jobs.size() - 1;
+  });
+  return rc == CS_OK ? idx : rc;
+}
+
+int cs_batch_set_tuning(cs_batch *b, const cs_tuning *t) {
+  return guarded([&] {
+    if (!b) fail(CS_EINVAL, "batch must not be NULL");
+    b->tune = from_c(t);
+    if (b->sb) batch_set_tuning(*b->sb, b->tune);
+  });
+}
+
+int cs_batch_autotune(cs_batch *b, double sample_fraction, cs_tuning *chosen) {
+  const auto t0 = std::chrono::steady_clock::now();
+  t_stats = cs_stats{};
+  int rc = guarded([&] {
+    if (!b) fail(CS_EINVAL, "batch must not be NULL");
+    if (!(sample_fraction > 0.0 && sample_fraction <= 1.0)) fail(CS_EINVAL, "sample_fraction must be in (0, 1]");
+    b->build();
+    b->tune = batch_autotune(*b->sb, sample_fraction, b->stream);
+    b->have_results = false;
+    if (chosen) *chosen = to_c(b->tune);
+  });
+  t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int cs_batch_run(cs_batch *b) {
+  const auto t0 = std::chrono::steady_clock::now();
+  t_stats = cs_stats{};
+  int rc = guarded([&] {
+    if (!b) fail(CS_EINVAL, "batch must not be NULL");
+    b->build();
+    batch_run(*b->sb, b->stream, t_stats);
+    uint64_t n = 0;
+    for (auto &j : b->jobs) n += j.ss.src.size();
+    t_stats.sources = n;
+    b->have_results = true;
+  });
+  t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int cs_batch_results(cs_batch *b) {
+  return guarded([&] {
+    if (!b) fail(CS_EINVAL, "batch must not be NULL");
+    if (!b->have_results) fail(CS_EINVAL, "the batch has no results: call cs_batch_run first");
+    std::vector<double> hacc(b->acc_words);
+    std::vector<uint32_t> hreach(8 * b->jobs.size());
+    if (b->acc_words)
+      hip_check(hipMemcpy(hacc.data(), b->d_acc, 8 * b->acc_words, hipMemcpyDeviceToHost), "copy out");
+    if (!hreach.empty())
+      hip_check(hipMemcpy(hreach.data(), b->d_reach, 4 * hreach.size(), hipMemcpyDeviceToHost), "copy out");
+    for (size_t i = 0; i < b->jobs.size(); ++i) {
+      const cs_batch::JobHost &j = b->jobs[i];
+      const uint32_t N = j.h->g.n, T = j.spec.T;
+      unpack_acc_strided(hacc.data() + j.acc_off, N, 7, T, b->NT, j.out);
+      if (j.betw) scale_rows(j.out, N, T, 5, 6, j.betw_scale);
+      if (j.reach)
+        for (uint32_t t = 0; t < T; ++t) j.reach[t] = hreach[8 * i + t];
+    }
   });
 }
 

@@ -54,6 +54,7 @@ struct DevGraph {
   int device = -1;
   uint32_t *rev_off = nullptr;
   uint32_t *fwd_off = nullptr;
+  uint4 *rows = nullptr;  // {fwd begin, fwd end, rev begin, rev end} per node: one 16-byte load
   std::vector<std::unique_ptr<DevEdges>> edges;  // small cache keyed by speed
 };
 
@@ -115,6 +116,55 @@ void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                      uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
                      uint32_t *d_reach, hipStream_t stream, cs_stats &st);
+
+// Launch shape of the search kernel: workgroup size (64..256), a cap on
+// resident waves per SIMD (0 = as many as fit), the bucket width as a fraction
+// of its safe maximum (<= 1) and the per-wave slot capacity (0 = default).
+struct Tuning {
+  uint32_t block = 256;
+  uint32_t waves_per_simd = 0;
+  float delta_scale = 1.0f;
+  uint32_t cap = 0;
+  uint32_t xcd_split = 1;  // sources dealt to XCDs in contiguous shares (1) or from one counter (0)
+};
+void set_default_tuning(const Tuning &t);
+Tuning default_tuning();
+
+// One (graph, request) pair of a batched shortest-path launch. d_acc is
+// [N][7][NT] for the launch's NT (>= cs.T); thresholds beyond cs.T stay zero.
+struct JobSpec {
+  Graph *g;
+  DevGraph *dg;
+  DevEdges *de;
+  CallSpec cs;
+  const uint32_t *d_sources;
+  const float *d_wt;
+  const float *d_cyc;
+  uint32_t n_sources;
+  const uint8_t *d_eligible;
+  double *d_acc;
+  uint32_t *d_reach;
+  float delta_scale;
+};
+// All sources of all jobs in one persistent launch.
+void launch_shortest_batch(const std::vector<JobSpec> &jobs, uint32_t NT, const Tuning &tune,
+                           hipStream_t stream, cs_stats &st);
+
+// A prepared batch of shortest-path jobs replayed as one captured hipGraph
+// (cs_search.hip). The zero regions (64-bit words) are cleared inside the
+// graph before each search: the jobs' accumulators and reach counters.
+struct ZeroRegion {
+  uint64_t *ptr;
+  size_t words;
+};
+struct ShortestBatch;
+ShortestBatch *batch_new(std::vector<JobSpec> specs, std::vector<ZeroRegion> zero, uint32_t NT, int device,
+                         const Tuning &tune);
+void batch_delete(ShortestBatch *b);
+void batch_set_tuning(ShortestBatch &b, const Tuning &t);
+Tuning batch_tuning(const ShortestBatch &b);
+Tuning batch_autotune(ShortestBatch &b, double fraction, hipStream_t stream);
+void batch_run(ShortestBatch &b, hipStream_t stream, cs_stats &st);
 
 // Segment centrality; acc [N][4][T].
 void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,

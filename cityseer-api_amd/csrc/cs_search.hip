@@ -1,7 +1,8 @@
 // cs_search.hip -- distance-bounded per-source search on gfx950.
 //
 // One wavefront (64 lanes) owns one source at a time and pulls the next source
-// from a global counter when it finishes (persistent waves, dynamic balance).
+// from a counter of its XCD's share of the source list when it finishes
+// (persistent waves, dynamic balance, work stealing across XCDs at the end).
 // The search is a Delta-bucketed Dijkstra over the reverse graph, the
 // direction the reference searches (rust/src/centrality.rs:1389-1392):
 //
@@ -17,12 +18,20 @@
 //  * Per-wave node state is an epoch-tagged 64-bit key (epoch<<32 | ~f32 bits
 //    of seconds) updated with a 64-bit atomic max: newer epochs and smaller
 //    seconds win, zero-filled memory is stale, and no O(N) state is cleared
-//    per source. Settled nodes get compact slots in settle order.
+//    per source. Settled nodes get compact slots in settle order; the source
+//    is always slot 0 (it is alone in bucket 0).
+//  * Each lane works through its node's in- and out-edges in chunks: the
+//    chunk's adjacency loads, neighbour key loads and relaxation atomics are
+//    all issued before any result is consumed, so a bucket costs a handful of
+//    memory round trips instead of two per edge. Bucket ring counters live in
+//    LDS.
 //  * Closeness terms are scattered to targets with f64 atomics
 //    (centrality.rs:1755-1778); betweenness back-propagates bucket by bucket
-//    in reverse (centrality.rs:823-873).
+//    in reverse (centrality.rs:823-873), reading the predecessor's slot that
+//    the forward fold stored.
 #include "cs_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -31,38 +40,53 @@ namespace cs {
 namespace {
 
 constexpr uint32_t kFailCap = 1u, kFailBucket = 2u, kFailRing = 4u, kFailInternal = 8u;
+constexpr int kChunk = 4;      // edges per lane per batch of memory operations
+constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dealt to them round-robin
 
 enum Mode { kBrandes = 0, kSegment = 1, kTree = 2, kBrandesExport = 3 };
 
-struct SearchArgs {
-  uint32_t N;
-  const uint32_t *rev_off;
+// One job: a graph and the parameters of one centrality request over it. A
+// launch runs the sources of one job (plain calls) or of many (batches).
+struct Job {
+  const uint4 *rows;  // {fwd begin, fwd end, rev begin, rev end} per node
   const uint2 *rev_adj;
-  const uint32_t *fwd_off;
   const uint2 *fwd_adj;
   const float2 *rev_tw;
   const float *fwd_twlen;
   const uint32_t *fwd_eid;
   const uint8_t *eligible;
-  uint32_t T;
-  float maxsec, speed, inv_delta, tolerance;
-  float dist[kMaxT];
-  float beta[kMaxT];
-  uint32_t nbk, bcap, cap, maxb;
-  uint32_t n_sources;
   const uint32_t *sources;
   const float *src_wt;
   const float *src_cyc;
-  uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
-  double *acc;     // [N][M][T]
+  double *acc;  // [N][M][NT] (threshold stride = the launch's NT)
   uint32_t *reach;
+  float maxsec, speed, inv_delta;
+  uint32_t nbk, maxb;
+  uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
+  uint32_t n_sources;
+  uint32_t pad;
+  float dist[kMaxT];  // thresholds beyond the job's own count hold -1 (never within)
+  float beta[kMaxT];
+};
+
+// Jobs are read through the constant address space, like kernel arguments:
+// scalar loads the compiler may repeat instead of holding every field.
+typedef __attribute__((address_space(4))) const Job CJob;
+
+struct SearchArgs {
+  const Job *jobs;
+  const uint2 *items;  // {job, source position}; null: job 0, position = item index
+  uint32_t n_items;
+  uint32_t bcap, cap, nbk_lds;
   unsigned long long *stats;  // reached, edges, within
   char *ws;
   size_t ws_stride;
-  size_t o_keyw, o_slotw, o_node, o_secs, o_a8, o_b8, o_acc, o_bkt, o_bcnt, o_bnd;
+  size_t o_keyw, o_slotw, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd;
   uint32_t *epochs;
-  uint32_t *next_source;
-  uint32_t *fail_list;
+  uint32_t *next;            // [kXcd] per-XCD item counters
+  uint32_t xbeg[kXcd + 1];   // XCD x owns item indices [xbeg[x], xbeg[x+1])
+  uint32_t xsplit;           // 0: every wave draws from share 0 (one global counter)
+  uint32_t *fail_list;       // failed item indices
   uint32_t *n_fail;
   uint32_t *fail_flags;
   TreeOut tout;
@@ -75,9 +99,6 @@ __device__ __forceinline__ uint64_t ld64(const uint64_t *p) {
 }
 __device__ __forceinline__ uint32_t ld32(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ float ldf(const float *p) {
-  return __uint_as_float(ld32(reinterpret_cast<const uint32_t *>(p)));
 }
 __device__ __forceinline__ double ldd(const double *p) {
   return __longlong_as_double((long long)ld64(reinterpret_cast<const uint64_t *>(p)));
@@ -94,15 +115,22 @@ __device__ __forceinline__ void std_(double *p, double v) {
 __device__ __forceinline__ uint64_t wmax64(uint64_t *p, uint64_t v) {
   return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-__device__ __forceinline__ uint32_t wadd32(uint32_t *p, uint32_t v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 __device__ __forceinline__ void wadd_f64(double *p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// LDS ring counters (one wave's private row).
+__device__ __forceinline__ uint32_t lds_ld(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_st(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t lds_add(uint32_t *p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 // Results shared by all waves (accumulators): device scope.
 __device__ __forceinline__ void gadd_f64(double *p, double v) { unsafeAtomicAdd(p, v); }
-// Orders this wave's earlier global writes before its later reads (all lanes).
+// Orders this wave's earlier memory operations before its later ones (all lanes).
 __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
@@ -119,6 +147,11 @@ __device__ __forceinline__ V wave_sum(V v) {
   return v;
 }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
 
 __device__ __forceinline__ uint64_t pack_key(uint32_t epoch, float secs) {
   return ((uint64_t)epoch << 32) | (uint32_t)(~__float_as_uint(secs));
@@ -128,11 +161,6 @@ __device__ __forceinline__ uint64_t pack_key_bits(uint32_t epoch, uint32_t secs_
 }
 __device__ __forceinline__ bool key_current(uint64_t k, uint32_t epoch) { return (uint32_t)(k >> 32) == epoch; }
 __device__ __forceinline__ float key_secs(uint64_t k) { return __uint_as_float(~(uint32_t)k); }
-__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-  return v;
-}
 
 __device__ __forceinline__ uint32_t bucket_of(float s, float inv_delta) {
   return (uint32_t)(s * inv_delta);
@@ -144,32 +172,34 @@ __device__ __forceinline__ bool key_less(float sa, uint32_t a, float sb, uint32_
   return sa < sb || (sa == sb && a < b);
 }
 
+// Per-slot record, two 64-bit words:
+//   lo = node | secs bits << 32
+//   hi = Brandes: first predecessor slot | (n_preds | eligible << 8) << 32
+//        trees:   predecessor slot | row position of the tree edge << 32
 struct WaveWs {
-  uint64_t *keyw;
-  uint32_t *slotw;
-  uint32_t *node;
-  float *secs;
-  uint64_t *a8;
-  uint64_t *b8;
-  double *acc;
-  uint2 *bkt;
-  uint32_t *bcnt;
-  uint32_t *bnd;
+  uint64_t *keyw;   // [N] epoch-tagged keys
+  uint32_t *slotw;  // [N] slot of a settled node
+  uint64_t *rec;    // [cap][2]
+  double *sig;      // [cap] sigma (Brandes) / unused
+  uint64_t *pm;     // [cap] predecessor row mask (Brandes) / {last len, origin len} (trees)
+  double *acc;      // [cap][2T] dependencies (Brandes) / [cap][T] subtree sums (segment)
+  uint2 *bkt;       // [nbk][bcap] bucket ring entries {node, secs bits}
+  uint32_t *bnd;    // [maxb + 2] slot bounds per opened bucket
+  uint32_t *ring;   // LDS [nbk] bucket ring counters
 };
 
-__device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid) {
+__device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uint32_t *ring) {
   char *base = a.ws + (size_t)wid * a.ws_stride;
   WaveWs w;
   w.keyw = reinterpret_cast<uint64_t *>(base + a.o_keyw);
   w.slotw = reinterpret_cast<uint32_t *>(base + a.o_slotw);
-  w.node = reinterpret_cast<uint32_t *>(base + a.o_node);
-  w.secs = reinterpret_cast<float *>(base + a.o_secs);
-  w.a8 = reinterpret_cast<uint64_t *>(base + a.o_a8);
-  w.b8 = reinterpret_cast<uint64_t *>(base + a.o_b8);
+  w.rec = reinterpret_cast<uint64_t *>(base + a.o_rec);
+  w.sig = reinterpret_cast<double *>(base + a.o_sig);
+  w.pm = reinterpret_cast<uint64_t *>(base + a.o_pm);
   w.acc = reinterpret_cast<double *>(base + a.o_acc);
   w.bkt = reinterpret_cast<uint2 *>(base + a.o_bkt);
-  w.bcnt = reinterpret_cast<uint32_t *>(base + a.o_bcnt);
   w.bnd = reinterpret_cast<uint32_t *>(base + a.o_bnd);
+  w.ring = ring;
   return w;
 }
 
@@ -202,68 +232,22 @@ __device__ __forceinline__ void fold_apply(Fold &f, float cs, double sigma_u, ui
 
 // A candidate offering more than final * (1+1e-4)^(deg+3) can neither stay a
 // predecessor nor change which candidates do: every later reset that must
-// remove it also removes anything it influenced (see DESIGN notes on the
-// fold). Only "near" candidates need the ordered fold; usually there is one.
+// remove it also removes anything it influenced. Only "near" candidates need
+// the ordered fold; usually there is exactly one.
 __device__ __forceinline__ double far_factor(uint32_t deg) {
   double f = 1.0;
   for (uint32_t i = 0; i < deg + 3; ++i) f *= 1.00010002;
   return f;
 }
 
-struct SrcCtx {
-  uint32_t src;
-  uint32_t tag;
-  uint32_t k;      // current absolute bucket
-  uint32_t nslot;  // settled nodes so far
-  uint32_t nb;     // buckets opened
-  uint32_t fail;
-};
-
-// ---------------------------------------------------------------------------
-// Brandes fold for node v (not the source) over its out-edges v->u: candidates
-// are nodes u settled in an earlier bucket with cs = secs_u + es <= maxsec.
-// Also counts canonical out-edges to nodes settled before v (circuit rank).
-// Returns false on an invariant failure.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ bool brandes_fold(const SearchArgs &a, const WaveWs &w, uint32_t tag, uint32_t v,
-                                             float secs_v, uint32_t kb, bool cycles, Fold &f,
-                                             uint32_t &cnt_e) {
-  const uint32_t r0 = a.fwd_off[v], r1 = a.fwd_off[v + 1];
-  const double thr_far = (double)secs_v * far_factor(r1 - r0);
-  uint32_t n_near = 0, near_u = 0, near_j = 0;
-  float near_cs = INFINITY;
-  for (uint32_t j = r0; j < r1; ++j) {
-    const uint2 e = a.fwd_adj[j];
-    const uint32_t u = e.x & kNodeMask;
-    if (u == v) continue;
-    const uint64_t ku = ld64(&w.keyw[u]);
-    if (!key_current(ku, tag)) continue;
-    const float su = key_secs(ku);
-    if (cycles && (e.x & kCanonBit) && key_less(su, u, secs_v, v)) ++cnt_e;
-    if (bucket_of(su, a.inv_delta) >= kb) continue;
-    const float cs = su + __uint_as_float(e.y);
-    if (cs > a.maxsec) continue;
-    if ((double)cs <= thr_far) {
-      if (n_near == 0) {
-        near_u = u;
-        near_j = j - r0;
-        near_cs = cs;
-      }
-      ++n_near;
-    }
-  }
+// Ordered fold over the near candidates of v (two or more): selection passes in
+// settle order (secs_u, u), ties of one u (parallel edges) in row order.
+__device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint32_t v, float secs_v,
+                                   uint32_t kb, uint32_t r0, uint32_t r1, double thr_far, uint32_t n_near,
+                                   Fold &f) {
   f.best = INFINITY;
   f.sigma = 0.0;
   f.pmask = 0;
-  if (n_near == 0) return false;
-  if (n_near == 1) {
-    f.best = near_cs;
-    f.pmask = 1ull << near_j;
-    f.sigma = ldd(reinterpret_cast<const double *>(&w.a8[ld32(&w.slotw[near_u])]));
-    return f.best == secs_v;
-  }
-  // Ordered fold over the near candidates: selection passes in settle order
-  // (secs_u, u), ties of one u (parallel edges) in row order.
   bool have_last = false;
   float lsu = 0.0f;
   uint32_t lu = 0, lj = 0;
@@ -271,17 +255,17 @@ __device__ __forceinline__ bool brandes_fold(const SearchArgs &a, const WaveWs &
     bool found = false;
     float psu = INFINITY, pcs = 0.0f;
     uint32_t pu = 0, pj = 0;
-    for (uint32_t j = r0; j < r1; ++j) {
-      const uint2 e = a.fwd_adj[j];
+    for (uint32_t p = r0; p < r1; ++p) {
+      const uint2 e = j.fwd_adj[p];
       const uint32_t u = e.x & kNodeMask;
       if (u == v) continue;
       const uint64_t ku = ld64(&w.keyw[u]);
       if (!key_current(ku, tag)) continue;
       const float su = key_secs(ku);
-      if (bucket_of(su, a.inv_delta) >= kb) continue;
+      if (bucket_of(su, j.inv_delta) >= kb) continue;
       const float cs = su + __uint_as_float(e.y);
-      if (cs > a.maxsec || (double)cs > thr_far) continue;
-      const uint32_t jr = j - r0;
+      if (cs > j.maxsec || (double)cs > thr_far) continue;
+      const uint32_t jr = p - r0;
       if (have_last && !(key_less(lsu, lu, su, u) || (su == lsu && u == lu && jr > lj))) continue;
       if (!found || key_less(su, u, psu, pu) || (su == psu && u == pu && jr < pj)) {
         found = true;
@@ -295,9 +279,9 @@ __device__ __forceinline__ bool brandes_fold(const SearchArgs &a, const WaveWs &
     bool contains = false;
     for (uint64_t m = f.pmask; m; m &= m - 1) {
       const uint32_t jb = __builtin_ctzll(m);
-      if ((a.fwd_adj[r0 + jb].x & kNodeMask) == pu) contains = true;
+      if ((j.fwd_adj[r0 + jb].x & kNodeMask) == pu) contains = true;
     }
-    const double sg = ldd(reinterpret_cast<const double *>(&w.a8[ld32(&w.slotw[pu])]));
+    const double sg = ldd(&w.sig[ld32(&w.slotw[pu])]);
     fold_apply(f, pcs, sg, pj, contains);
     have_last = true;
     lsu = psu;
@@ -307,36 +291,14 @@ __device__ __forceinline__ bool brandes_fold(const SearchArgs &a, const WaveWs &
   return f.best == secs_v;
 }
 
-// Single-predecessor fold (dijkstra_tree_shortest / dijkstra_tree_segment,
-// centrality.rs:1186-1195,1589-1607): strict < over candidates in settle
-// order keeps the first minimum, i.e. the lexicographic min (cs, secs_u, u, j).
-__device__ __forceinline__ bool tree_fold(const SearchArgs &a, const WaveWs &w, uint32_t tag, uint32_t v,
-                                          float secs_v, uint32_t kb, uint32_t &tree_u, uint32_t &tree_j) {
-  const uint32_t r0 = a.fwd_off[v], r1 = a.fwd_off[v + 1];
-  float bcs = INFINITY, bsu = INFINITY;
-  uint32_t bu = 0xffffffffu, bj = 0xffffffffu;
-  for (uint32_t j = r0; j < r1; ++j) {
-    const uint2 e = a.fwd_adj[j];
-    const uint32_t u = e.x & kNodeMask;
-    if (u == v) continue;
-    const uint64_t ku = ld64(&w.keyw[u]);
-    if (!key_current(ku, tag)) continue;
-    const float su = key_secs(ku);
-    if (bucket_of(su, a.inv_delta) >= kb) continue;
-    const float cs = su + __uint_as_float(e.y);
-    if (cs > a.maxsec) continue;
-    const uint32_t jr = j - r0;
-    if (cs < bcs || (cs == bcs && (su < bsu || (su == bsu && (u < bu || (u == bu && jr < bj)))))) {
-      bcs = cs;
-      bsu = su;
-      bu = u;
-      bj = jr;
-    }
-  }
-  tree_u = bu;
-  tree_j = bj;
-  return bj != 0xffffffffu && bcs == secs_v;
-}
+struct SrcCtx {
+  uint32_t src;
+  uint32_t tag;
+  uint32_t k;      // current absolute bucket
+  uint32_t nslot;  // settled nodes so far
+  uint32_t nb;     // buckets opened
+  uint32_t fail;
+};
 
 // ---------------------------------------------------------------------------
 // Search one source: bucket by bucket, each entry of the bucket is validated
@@ -345,15 +307,17 @@ __device__ __forceinline__ bool tree_fold(const SearchArgs &a, const WaveWs &w, 
 // Returns false if the source must be retried with a larger workspace.
 // ---------------------------------------------------------------------------
 template <int MODE, int NT>
-__device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c, uint32_t (&nband)[NT],
-                              uint32_t (&eband)[NT], unsigned long long &st_edges) {
+__device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, SrcCtx &c, uint32_t (&nband)[NT],
+                              uint32_t (&eband)[NT], uint32_t &st_edges) {
+  constexpr bool kMulti = (MODE == kBrandes || MODE == kBrandesExport);
   const uint32_t lane = lane_id();
-  const float speed = a.speed;
-  const bool cycles = (MODE == kBrandes) && (a.flags & 1u);
+  const float speed = j.speed;
+  const bool cycles = (MODE == kBrandes) && (j.flags & 1u);
+  const bool fold = (MODE != kBrandes) || (j.flags & 2u);
   if (lane == 0) {
     st64(&w.keyw[c.src], pack_key(c.tag, 0.0f));
     st64(reinterpret_cast<uint64_t *>(&w.bkt[0]), (uint64_t)c.src);
-    st32(&w.bcnt[0], 1u);
+    lds_st(&w.ring[0], 1u);
   }
   wave_fence();
   uint32_t pending = 1;
@@ -363,35 +327,39 @@ __device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c, u
   c.fail = 0;
   while (pending > 0) {
     // every bucket index is below maxsec / delta: a larger k means a broken count
-    if (c.k > a.maxb + a.nbk) {
+    if (c.k > j.maxb + j.nbk) {
       c.fail |= kFailInternal;
       break;
     }
-    const uint32_t ring = c.k % a.nbk;
-    const uint32_t cnt = uniform(ld32(&w.bcnt[ring]));
+    const uint32_t ring = c.k % j.nbk;
+    const uint32_t cnt = uniform(lds_ld(&w.ring[ring]));
     if (cnt == 0) {
       ++c.k;
       continue;
     }
     if (cnt > a.bcap) c.fail |= kFailBucket;
-    if (c.nb >= a.maxb) c.fail |= kFailInternal;
+    if (c.nb >= j.maxb) c.fail |= kFailInternal;
     if (c.fail) break;
     if (lane == 0) {
-      st32(&w.bcnt[ring], 0u);
+      lds_st(&w.ring[ring], 0u);
       st32(&w.bnd[c.nb], c.nslot);
     }
     ++c.nb;
     uint32_t pushes = 0, lfail = 0;
     for (uint32_t base = 0; base < cnt; base += kWave) {
       const uint32_t i = base + lane;
-      bool valid = false;
-      uint32_t v = 0, csb = 0;
+      uint32_t v = 0, csb = 0, el = 0;
+      uint64_t kv = 0;
+      uint4 rw = make_uint4(0u, 0u, 0u, 0u);
       if (i < cnt) {
         const uint64_t ent = ld64(reinterpret_cast<const uint64_t *>(&w.bkt[ring * a.bcap + i]));
         v = (uint32_t)ent;
         csb = (uint32_t)(ent >> 32);
-        valid = ld64(&w.keyw[v]) == pack_key_bits(c.tag, csb);
+        kv = ld64(&w.keyw[v]);
+        rw = j.rows[v];
+        if constexpr (MODE == kBrandes) el = j.eligible[v];
       }
+      const bool valid = (i < cnt) && kv == pack_key_bits(c.tag, csb);
       const uint64_t m = __ballot(valid);
       const uint32_t cntv = (uint32_t)__popcll(m);
       if (c.nslot + cntv > a.cap) {
@@ -400,88 +368,176 @@ __device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c, u
       }
       const uint32_t s = c.nslot + rank_below(m);
       c.nslot += cntv;
-      if (!valid) continue;
-      const float secs_v = __uint_as_float(csb);
-      const float route_v = secs_v * speed;
-      st32(&w.slotw[v], s);
-      st32(&w.node[s], v);
-      st32(reinterpret_cast<uint32_t *>(&w.secs[s]), csb);
       uint32_t cnt_e = 0;
-      if constexpr (MODE == kBrandes || MODE == kBrandesExport) {
-        Fold f;
-        if (v == c.src) {
-          f.sigma = 1.0;
-          f.pmask = 0;
-        } else if (!brandes_fold(a, w, c.tag, v, secs_v, c.k, cycles, f, cnt_e)) {
-          lfail |= kFailInternal;
-        }
-        st64(&w.a8[s], (uint64_t)__double_as_longlong(f.sigma));
-        st64(&w.b8[s], f.pmask);
-        if constexpr (MODE == kBrandes) {
-#pragma unroll
-          for (int t = 0; t < 2 * NT; ++t) std_(&w.acc[(size_t)s * 2 * NT + t], 0.0);
-        }
-      } else {
-        uint32_t tu = 0xffffffffu, tj = 0xffffffffu, pslot = 0xffffffffu;
-        float olen = 0.0f, llen = 0.0f;
-        if (v != c.src) {
-          if (!tree_fold(a, w, c.tag, v, secs_v, c.k, tu, tj)) {
-            lfail |= kFailInternal;
-          } else {
-            pslot = ld32(&w.slotw[tu]);
-            llen = a.fwd_twlen[a.fwd_off[v] + tj];
-            olen = (tu == c.src) ? llen : __uint_as_float((uint32_t)(ld64(&w.b8[pslot]) >> 32));
+      if (valid) {
+        const float secs_v = __uint_as_float(csb);
+        const bool is_src = (s == 0);
+        st32(&w.slotw[v], s);
+        const uint32_t f0 = rw.x, nf = rw.y - rw.x, r0 = rw.z, nr = rw.w - rw.z;
+        st_edges += nr;
+        // fold state (multi-predecessor: near candidates; trees: lexicographic min)
+        const double thr_far = kMulti ? (double)secs_v * far_factor(nf) : 0.0;
+        uint32_t n_near = 0, near_j = 0, near_slot = 0;
+        float near_cs = INFINITY;
+        float bcs = INFINITY, bsu = INFINITY;
+        uint32_t bu = 0xffffffffu, bj = 0xffffffffu, bslot = 0;
+        const uint32_t nmax = nf > nr ? nf : nr;
+        for (uint32_t cb = 0; cb < nmax; cb += kChunk) {
+          uint2 fe[kChunk], re[kChunk];
+  #pragma unroll
+          for (int q = 0; q < kChunk; ++q) {
+            fe[q] = (cb + q < nf) ? j.fwd_adj[f0 + cb + q] : make_uint2(v, 0u);
+            re[q] = (cb + q < nr) ? j.rev_adj[r0 + cb + q] : make_uint2(v, 0u);
           }
-        }
-        // a8 = {pslot | last-edge row position << 32}; b8 = {llen | olen << 32}
-        st64(&w.a8[s], ((uint64_t)tj << 32) | pslot);
-        st64(&w.b8[s], ((uint64_t)__float_as_uint(olen) << 32) | __float_as_uint(llen));
-        if constexpr (MODE == kSegment) {
-#pragma unroll
-          for (int t = 0; t < NT; ++t) std_(&w.acc[(size_t)s * NT + t], 0.0);
-        }
-      }
-      // relax in-edges nb -> v; the atomic's old key also tells whether nb
-      // settled before v (canonical edges count toward the circuit rank)
-      const uint32_t r0 = a.rev_off[v], r1 = a.rev_off[v + 1];
-      st_edges += (r1 - r0);
-      for (uint32_t j = r0; j < r1; ++j) {
-        const uint2 e = a.rev_adj[j];
-        const uint32_t nb = e.x & kNodeMask;
-        if (nb == v) continue;
-        const float cs = secs_v + __uint_as_float(e.y);
-        uint64_t kn = 0;
-        if (cs <= a.maxsec) {
-          const uint64_t nk = pack_key(c.tag, cs);
-          kn = wmax64(&w.keyw[nb], nk);
-          if (nk > kn) {
-            const uint32_t b = bucket_of(cs, a.inv_delta);
-            if (b <= c.k || b - c.k >= a.nbk) {
-              lfail |= (b <= c.k) ? kFailInternal : kFailRing;
-            } else {
-              const uint32_t rb = b % a.nbk;
-              const uint32_t pos = wadd32(&w.bcnt[rb], 1u);
-              if (pos < a.bcap)
-                st64(reinterpret_cast<uint64_t *>(&w.bkt[rb * a.bcap + pos]),
-                     ((uint64_t)__float_as_uint(cs) << 32) | nb);
-              else
-                lfail |= kFailBucket;
-              ++pushes;
+          // issue every key load and relaxation before consuming any result
+          uint64_t fk[kChunk], rk[kChunk];
+          uint32_t fs[kChunk];
+  #pragma unroll
+          for (int q = 0; q < kChunk; ++q) {
+            const uint32_t u = fe[q].x & kNodeMask;
+            fk[q] = 0;
+            fs[q] = 0;
+            if (!is_src && u != v && (fold || cycles)) {
+              fk[q] = ld64(&w.keyw[u]);
+              if (fold) fs[q] = ld32(&w.slotw[u]);
+            }
+            const uint32_t nb = re[q].x & kNodeMask;
+            const float cs = secs_v + __uint_as_float(re[q].y);
+            rk[q] = 0;
+            if (nb != v) {
+              if (cs <= j.maxsec)
+                rk[q] = wmax64(&w.keyw[nb], pack_key(c.tag, cs));
+              else if (cycles && (re[q].x & kCanonBit))
+                rk[q] = ld64(&w.keyw[nb]);
             }
           }
-        } else if (cycles && (e.x & kCanonBit)) {
-          kn = ld64(&w.keyw[nb]);
+          // fold candidates v -> u (u settled in an earlier bucket)
+  #pragma unroll
+          for (int q = 0; q < kChunk; ++q) {
+            const uint32_t u = fe[q].x & kNodeMask;
+            if (is_src || u == v) continue;
+            const uint64_t ku = fk[q];
+            if (!key_current(ku, c.tag)) continue;
+            const float su = key_secs(ku);
+            if (cycles && (fe[q].x & kCanonBit) && key_less(su, u, secs_v, v)) ++cnt_e;
+            if (!fold || bucket_of(su, j.inv_delta) >= c.k) continue;
+            const float cs = su + __uint_as_float(fe[q].y);
+            if (cs > j.maxsec) continue;
+            const uint32_t jr = cb + q;
+            if constexpr (kMulti) {
+              if ((double)cs <= thr_far) {
+                if (n_near == 0) {
+                  near_j = jr;
+                  near_cs = cs;
+                  near_slot = fs[q];
+                }
+                ++n_near;
+              }
+            } else {
+              // dijkstra_tree_shortest / dijkstra_tree_segment (centrality.rs:1186-1195,
+              // 1589-1607): strict < in settle order keeps the lexicographic min
+              // (cs, secs_u, u, row position)
+              if (cs < bcs || (cs == bcs && (su < bsu || (su == bsu && (u < bu || (u == bu && jr < bj)))))) {
+                bcs = cs;
+                bsu = su;
+                bu = u;
+                bj = jr;
+                bslot = fs[q];
+              }
+            }
+          }
+          // relaxation results nb -> v; the old key also tells whether nb settled
+          // before v (canonical edges count toward the circuit rank)
+  #pragma unroll
+          for (int q = 0; q < kChunk; ++q) {
+            const uint32_t nb = re[q].x & kNodeMask;
+            if (nb == v) continue;
+            const float cs = secs_v + __uint_as_float(re[q].y);
+            const uint64_t kn = rk[q];
+            if (cs <= j.maxsec) {
+              const uint64_t nk = pack_key(c.tag, cs);
+              if (nk > kn) {
+                const uint32_t b = bucket_of(cs, j.inv_delta);
+                if (b <= c.k || b - c.k >= j.nbk) {
+                  lfail |= (b <= c.k) ? kFailInternal : kFailRing;
+                } else {
+                  const uint32_t rb = b % j.nbk;
+                  const uint32_t pos = lds_add(&w.ring[rb], 1u);
+                  if (pos < a.bcap)
+                    st64(reinterpret_cast<uint64_t *>(&w.bkt[rb * a.bcap + pos]),
+                         ((uint64_t)__float_as_uint(cs) << 32) | nb);
+                  else
+                    lfail |= kFailBucket;
+                  ++pushes;
+                }
+              }
+            }
+            if (cycles && (re[q].x & kCanonBit) && key_current(kn, c.tag) && key_less(key_secs(kn), nb, secs_v, v))
+              ++cnt_e;
+          }
         }
-        if (cycles && (e.x & kCanonBit) && key_current(kn, c.tag) && key_less(key_secs(kn), nb, secs_v, v))
-          ++cnt_e;
+        // settle record
+        if constexpr (kMulti) {
+          Fold f;
+          f.sigma = 1.0;
+          f.pmask = 0;
+          uint32_t pslot0 = 0, npred = 0;
+          if (!is_src && fold) {
+            if (n_near == 1) {
+              f.pmask = 1ull << near_j;
+              f.sigma = ldd(&w.sig[near_slot]);
+              pslot0 = near_slot;
+              npred = 1;
+              if (near_cs != secs_v) lfail |= kFailInternal;
+            } else if (n_near == 0 ||
+                       !brandes_fold_exact(j, w, c.tag, v, secs_v, c.k, f0, f0 + nf, thr_far, n_near, f)) {
+              lfail |= kFailInternal;
+            } else {
+              npred = (uint32_t)__popcll(f.pmask);
+              const uint32_t u0 = j.fwd_adj[f0 + __builtin_ctzll(f.pmask)].x & kNodeMask;
+              pslot0 = ld32(&w.slotw[u0]);
+            }
+          }
+          st64(&w.rec[2 * (size_t)s], ((uint64_t)csb << 32) | v);
+          st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(npred | (el ? 0x100u : 0u)) << 32) | pslot0);
+          std_(&w.sig[s], f.sigma);
+          st64(&w.pm[s], f.pmask);
+          if constexpr (MODE == kBrandes) {
+            if (j.flags & 2u) {
+  #pragma unroll
+              for (int t = 0; t < 2 * NT; ++t) std_(&w.acc[(size_t)s * 2 * NT + t], 0.0);
+            }
+          }
+        } else {
+          uint32_t pslot = 0xffffffffu;
+          float olen = 0.0f, llen = 0.0f;
+          if (!is_src) {
+            if (bj == 0xffffffffu || bcs != secs_v) {
+              lfail |= kFailInternal;
+              bj = 0xffffffffu;
+            } else {
+              pslot = bslot;
+              llen = j.fwd_twlen[f0 + bj];
+              olen = (pslot == 0) ? llen : __uint_as_float((uint32_t)(ld64(&w.pm[pslot]) >> 32));
+            }
+          }
+          st64(&w.rec[2 * (size_t)s], ((uint64_t)csb << 32) | v);
+          st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)bj << 32) | pslot);
+          st64(&w.pm[s], ((uint64_t)__float_as_uint(olen) << 32) | __float_as_uint(llen));
+          if constexpr (MODE == kSegment) {
+  #pragma unroll
+            for (int t = 0; t < NT; ++t) std_(&w.acc[(size_t)s * NT + t], 0.0);
+          }
+        }
       }
+      // circuit-rank counts per threshold band, reduced over the wave
       if (cycles) {
+        const float route_v = __uint_as_float(csb) * speed;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          if (route_v <= a.dist[t]) {
-            ++nband[t];
-            eband[t] += cnt_e;
-          }
+          const bool in = valid && route_v <= j.dist[t];
+          nband[t] += uniform(wave_sum(in ? 1u : 0u));
+          eband[t] += uniform(wave_sum(in ? cnt_e : 0u));
         }
       }
     }
@@ -494,7 +550,7 @@ __device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c, u
   }
   if (c.fail) {
     // leave the ring empty for the next source
-    for (uint32_t r = lane; r < a.nbk; r += kWave) st32(&w.bcnt[r], 0u);
+    for (uint32_t r = lane; r < j.nbk; r += kWave) lds_st(&w.ring[r], 0u);
     wave_fence();
     return false;
   }
@@ -503,118 +559,175 @@ __device__ bool search_source(const SearchArgs &a, const WaveWs &w, SrcCtx &c, u
   return true;
 }
 
+// Next item for a wave on XCD `x`: its own share first, then the other XCDs'
+// shares in turn. Returns n_items when all are taken.
+__device__ __forceinline__ uint32_t next_item(const SearchArgs &a, uint32_t x) {
+  uint32_t si = a.n_items;
+  if (lane_id() == 0) {
+    for (uint32_t q = 0; q < kXcd; ++q) {
+      const uint32_t xx = (x + q) % kXcd;
+      const uint32_t len = a.xbeg[xx + 1] - a.xbeg[xx];
+      if (__hip_atomic_load(&a.next[xx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len) continue;
+      const uint32_t i = atomicAdd(&a.next[xx], 1u);
+      if (i < len) {
+        si = a.xbeg[xx] + i;
+        break;
+      }
+    }
+  }
+  return uniform(si);
+}
+
 // ---------------------------------------------------------------------------
-// Persistent kernel: each wave loops over sources taken from a global counter.
+// Persistent kernel: each wave loops over sources taken from the counters.
 // ---------------------------------------------------------------------------
 template <int MODE, int NT>
 __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
+  extern __shared__ uint32_t lds_rings[];
   const uint32_t lane = lane_id();
-  const uint32_t wid = uniform((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-  const WaveWs w = wave_ws(a, wid);
+  const uint32_t wib = uniform(threadIdx.x / kWave);
+  const uint32_t wid = uniform(blockIdx.x * (blockDim.x / kWave) + wib);
+  const uint32_t xcd = a.xsplit ? blockIdx.x % kXcd : 0u;
+  const WaveWs w = wave_ws(a, wid, lds_rings + (size_t)wib * a.nbk_lds);
+  for (uint32_t r = lane; r < a.nbk_lds; r += kWave) lds_st(&w.ring[r], 0u);
+  wave_fence();
   uint32_t epoch = a.epochs[wid];
-  unsigned long long st_reached = 0, st_edges = 0, st_within = 0;
-  const float speed = a.speed;
+  unsigned long long st_reached = 0, st_within = 0;  // wave-uniform
+  uint32_t st_edges = 0;                             // per lane
   for (;;) {
-    uint32_t si = 0;
-    if (lane == 0) si = atomicAdd(a.next_source, 1u);
-    si = uniform(si);
-    if (si >= a.n_sources) break;
+    const uint32_t item = next_item(a, xcd);
+    if (item >= a.n_items) break;
+    uint32_t job = 0, si = item;
+    if (a.items) {
+      const uint2 it = a.items[item];
+      job = uniform(it.x);
+      si = uniform(it.y);
+    }
+    CJob &j = ((CJob *)a.jobs)[job];
+    const float speed = j.speed;
     SrcCtx c;
-    c.src = a.sources[si];
+    c.src = j.sources[si];
     ++epoch;
     c.tag = epoch;
     uint32_t nband[NT], eband[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) nband[t] = eband[t] = 0;
-    if (!search_source<MODE, NT>(a, w, c, nband, eband, st_edges)) {
+    if (!search_source<MODE, NT>(a, j, w, c, nband, eband, st_edges)) {
       if (lane == 0) {
         const uint32_t q = atomicAdd(a.n_fail, 1u);
-        a.fail_list[q] = si;
+        a.fail_list[q] = item;
         atomicOr(a.fail_flags, c.fail);
       }
       continue;
     }
     const uint32_t R = c.nslot;
-    st_reached += (lane == 0) ? R : 0;
+    st_reached += R;
 
     if constexpr (MODE == kBrandes) {
-      const float wt = a.src_wt[si];
-      const float cyc = a.src_cyc[si];
+      const float wt = j.src_wt[si];
+      const float cyc = j.src_cyc[si];
       constexpr uint32_t M = 7;
       float rank[NT];
       uint32_t within[NT];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const uint32_t nn = wave_sum(nband[t]);
-        const uint32_t ee = wave_sum(eband[t]);
+        const uint32_t nn = nband[t];
+        const uint32_t ee = eband[t];
         const int r = (int)ee - (int)nn + 1;
         rank[t] = (nn == 0) ? 0.0f : (float)(r > 0 ? r : 0);
         within[t] = 0;
       }
-      // ---- closeness: scatter to targets (centrality.rs:1736-1779)
-      for (uint32_t s = lane; s < R; s += kWave) {
-        const uint32_t v = ld32(&w.node[s]);
-        if (v == c.src) continue;
-        const float route = ldf(&w.secs[s]) * speed;
-        double *accv = a.acc + (size_t)v * M * NT;
+      // ---- closeness: scatter to targets (centrality.rs:1736-1779); slot 0 is the source
+      for (uint32_t s = 1 + lane; s < R; s += kWave) {
+        const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+        const uint32_t v = (uint32_t)lo;
+        const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
+        double *accv = j.acc + (size_t)v * M * NT;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          if (route <= a.dist[t]) {
+          if (route <= j.dist[t]) {
             ++within[t];
-            if (a.flags & 1u) {
+            if (j.flags & 1u) {
               gadd_f64(&accv[0 * NT + t], (double)wt);
               gadd_f64(&accv[1 * NT + t], (double)(route * wt));
               gadd_f64(&accv[2 * NT + t], (double)(rank[t] * cyc));
               gadd_f64(&accv[3 * NT + t], (double)((1.0f / route) * wt));
-              gadd_f64(&accv[4 * NT + t], (double)(expf(-a.beta[t] * route) * wt));
+              gadd_f64(&accv[4 * NT + t], (double)(expf(-j.beta[t] * route) * wt));
             }
           }
         }
       }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        const uint32_t tot = wave_sum(within[t]);
-        st_within += (lane == 0) ? tot : 0;
-        if ((a.flags & 5u) == 5u && lane == 0 && tot) atomicAdd(&a.reach[t], tot);
+        const uint32_t tot = uniform(wave_sum(within[t]));
+        st_within += tot;
+        if ((j.flags & 5u) == 5u && lane == 0 && tot) atomicAdd(&j.reach[t], tot);
       }
       // ---- betweenness: reverse bucket order (centrality.rs:1783-1835, 823-873)
-      if (a.flags & 2u) {
+      if (j.flags & 2u) {
+        uint32_t s1 = R;
+        uint32_t s0 = uniform(ld32(&w.bnd[c.nb - 1]));
         for (int b = (int)c.nb - 1; b >= 0; --b) {
-          const uint32_t s0 = ld32(&w.bnd[b]), s1 = ld32(&w.bnd[b + 1]);
+          const uint32_t s0n = (b > 0) ? ld32(&w.bnd[b - 1]) : 0u;
           for (uint32_t s = s0 + lane; s < s1; s += kWave) {
-            const uint32_t v = ld32(&w.node[s]);
-            const float route = ldf(&w.secs[s]) * speed;
-            const double sigma_w = ldd(reinterpret_cast<const double *>(&w.a8[s]));
-            const uint64_t pm = ld64(&w.b8[s]);
-            const bool is_src = v == c.src;
-            const double pc = is_src ? 0.0 : (a.eligible[v] ? 0.5 : 1.0);
-            const uint32_t f0 = a.fwd_off[v];
-            double *accv = a.acc + (size_t)v * M * NT;
+            const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+            const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
+            const double sigma_w = ldd(&w.sig[s]);
+            double da[NT], db[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+              da[t] = ldd(&w.acc[(size_t)s * 2 * NT + t]);
+              db[t] = ldd(&w.acc[(size_t)s * 2 * NT + NT + t]);
+            }
+            const uint32_t v = (uint32_t)lo;
+            const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
+            const uint32_t pslot0 = (uint32_t)hi;
+            const uint32_t meta = (uint32_t)(hi >> 32);
+            const uint32_t npred = meta & 0xffu;
+            const bool is_src = (s == 0);
+            const double pc = is_src ? 0.0 : ((meta & 0x100u) ? 0.5 : 1.0);
+            double *accv = j.acc + (size_t)v * M * NT;
             double dep[NT], dep_b[NT], seed_b[NT];
             bool any = false;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
               dep[t] = dep_b[t] = seed_b[t] = 0.0;
-              if (route <= a.dist[t]) {
-                seed_b[t] = pc * exp(-(double)a.beta[t] * (double)route);
-                dep[t] = pc + ldd(&w.acc[(size_t)s * 2 * NT + t]);
-                dep_b[t] = seed_b[t] + ldd(&w.acc[(size_t)s * 2 * NT + NT + t]);
+              if (route <= j.dist[t]) {
+                seed_b[t] = pc * exp(-(double)j.beta[t] * (double)route);
+                dep[t] = pc + da[t];
+                dep_b[t] = seed_b[t] + db[t];
                 any = any || dep[t] != 0.0 || dep_b[t] != 0.0;
               }
             }
             if (!any) continue;
-            for (uint64_t m = pm; m; m &= m - 1) {
-              const uint32_t j = __builtin_ctzll(m);
-              const uint32_t u = a.fwd_adj[f0 + j].x & kNodeMask;
-              const uint32_t su = ld32(&w.slotw[u]);
-              const double sigma_u = ldd(reinterpret_cast<const double *>(&w.a8[su]));
-              if (sigma_u == 0.0) continue;
-              const double fct = sigma_u / sigma_w;
+            if (npred == 1) {
+              const double sigma_u = ldd(&w.sig[pslot0]);
+              if (sigma_u != 0.0) {
+                const double fct = sigma_u / sigma_w;
 #pragma unroll
-              for (int t = 0; t < NT; ++t) {
-                if (dep[t] != 0.0 || dep_b[t] != 0.0) {
-                  wadd_f64(&w.acc[(size_t)su * 2 * NT + t], fct * dep[t]);
-                  wadd_f64(&w.acc[(size_t)su * 2 * NT + NT + t], fct * dep_b[t]);
+                for (int t = 0; t < NT; ++t) {
+                  if (dep[t] != 0.0 || dep_b[t] != 0.0) {
+                    wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + t], fct * dep[t]);
+                    wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + NT + t], fct * dep_b[t]);
+                  }
+                }
+              }
+            } else if (npred > 1) {
+              const uint64_t pm = ld64(&w.pm[s]);
+              const uint32_t f0 = j.rows[v].x;
+              for (uint64_t mm = pm; mm; mm &= mm - 1) {
+                const uint32_t jb = __builtin_ctzll(mm);
+                const uint32_t u = j.fwd_adj[f0 + jb].x & kNodeMask;
+                const uint32_t su = ld32(&w.slotw[u]);
+                const double sigma_u = ldd(&w.sig[su]);
+                if (sigma_u == 0.0) continue;
+                const double fct = sigma_u / sigma_w;
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                  if (dep[t] != 0.0 || dep_b[t] != 0.0) {
+                    wadd_f64(&w.acc[(size_t)su * 2 * NT + t], fct * dep[t]);
+                    wadd_f64(&w.acc[(size_t)su * 2 * NT + NT + t], fct * dep_b[t]);
+                  }
                 }
               }
             }
@@ -629,75 +742,93 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             }
           }
           wave_fence();
+          s1 = s0;
+          s0 = uniform(s0n);
         }
       }
     } else if constexpr (MODE == kSegment) {
       constexpr uint32_t M = 4;
       // ---- closeness at the source over visited edges (centrality.rs:2201-2317)
-      if (a.flags & 1u) {
+      if (j.flags & 1u) {
         double sd[NT], sh[NT], sb[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) sd[t] = sh[t] = sb[t] = 0.0;
         for (uint32_t s = lane; s < R; s += kWave) {
-          const uint32_t v = ld32(&w.node[s]);
-          const float secs_v = ldf(&w.secs[s]);
+          const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+          const uint32_t v = (uint32_t)lo;
+          const float secs_v = __uint_as_float((uint32_t)(lo >> 32));
           const float sd_n = secs_v * speed;
-          const uint32_t r0 = a.rev_off[v], r1 = a.rev_off[v + 1];
-          for (uint32_t j = r0; j < r1; ++j) {
-            const uint32_t nb = a.rev_adj[j].x & kNodeMask;
-            float sd_m = INFINITY;
-            bool visited_edge = true;
-            if (nb == v) {
-              sd_m = sd_n;
-            } else {
-              const uint64_t kn = ld64(&w.keyw[nb]);
-              if (key_current(kn, c.tag)) {
-                const float s_nb = key_secs(kn);
+          const uint4 rw = j.rows[v];
+          const uint32_t r0 = rw.z, nr = rw.w - rw.z;
+          for (uint32_t cb = 0; cb < nr; cb += kChunk) {
+            uint2 re[kChunk];
+            float2 tw[kChunk];
+            uint64_t kn[kChunk];
+#pragma unroll
+            for (int q = 0; q < kChunk; ++q) {
+              const bool in = cb + q < nr;
+              re[q] = in ? j.rev_adj[r0 + cb + q] : make_uint2(v, 0u);
+              tw[q] = in ? j.rev_tw[r0 + cb + q] : make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int q = 0; q < kChunk; ++q) {
+              const uint32_t nb = re[q].x & kNodeMask;
+              kn[q] = (nb != v) ? ld64(&w.keyw[nb]) : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < kChunk; ++q) {
+              if (cb + q >= nr) continue;
+              const uint32_t nb = re[q].x & kNodeMask;
+              float sd_m = INFINITY;
+              bool visited_edge = true;
+              if (nb == v) {
+                sd_m = sd_n;
+              } else if (key_current(kn[q], c.tag)) {
+                const float s_nb = key_secs(kn[q]);
                 visited_edge = key_less(secs_v, v, s_nb, nb);
                 sd_m = s_nb * speed;
               }
-            }
-            if (!visited_edge) continue;
-            const float2 tw = a.rev_tw[j];
-            const bool n_nearer = sd_n <= sd_m;
-            const float av = n_nearer ? sd_n : sd_m;
-            const float bv = n_nearer ? sd_m : sd_n;
-            const float a_imp = av, b_imp = bv;
-            const float elen = tw.x, imp = tw.y;
-            const float cc = (elen + av + bv) / 2.0f;
-            const float dd = cc;
-            const float c_imp = a_imp + (cc - av) * imp;
-            const float d_imp = c_imp;
+              if (!visited_edge) continue;
+              const bool n_nearer = sd_n <= sd_m;
+              const float av = n_nearer ? sd_n : sd_m;
+              const float bv = n_nearer ? sd_m : sd_n;
+              const float a_imp = av, b_imp = bv;
+              const float elen = tw[q].x, imp = tw[q].y;
+              const float cc = (elen + av + bv) / 2.0f;
+              const float dd = cc;
+              const float c_imp = a_imp + (cc - av) * imp;
+              const float d_imp = c_imp;
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-              const float dist = a.dist[t];
-              const float beta = a.beta[t];
-              const float nbeta = -beta;
-              const float inv = beta != 0.0f ? 1.0f / nbeta : 0.0f;
-              if (av < dist) {
-                float cur = cc, curi = c_imp;
-                if (cur > dist) {
-                  cur = dist;
-                  curi = a_imp + (dist - av) * imp;
+              for (int t = 0; t < NT; ++t) {
+                const float dist = j.dist[t];
+                const float beta = j.beta[t];
+                const float nbeta = -beta;
+                const float inv = beta != 0.0f ? 1.0f / nbeta : 0.0f;
+                if (av < dist) {
+                  float cur = cc, curi = c_imp;
+                  if (cur > dist) {
+                    cur = dist;
+                    curi = a_imp + (dist - av) * imp;
+                  }
+                  sd[t] += (double)(cur - av);
+                  const float hh = a_imp < 1.0f ? logf(curi) : logf(fmaxf(curi / a_imp, 1.1920929e-07f));
+                  sh[t] += (double)hh;
+                  const float bb = beta == 0.0f ? curi - a_imp : (expf(nbeta * curi) - expf(nbeta * a_imp)) * inv;
+                  sb[t] += (double)bb;
                 }
-                sd[t] += (double)(cur - av);
-                const float hh = a_imp < 1.0f ? logf(curi) : logf(fmaxf(curi / a_imp, 1.1920929e-07f));
-                sh[t] += (double)hh;
-                const float bb = beta == 0.0f ? curi - a_imp : (expf(nbeta * curi) - expf(nbeta * a_imp)) * inv;
-                sb[t] += (double)bb;
-              }
-              if (bv == dd) continue;
-              if (bv <= dist) {
-                float cur = dd, curi = d_imp;
-                if (cur > dist) {
-                  cur = dist;
-                  curi = b_imp + (dist - bv) * imp;
+                if (bv == dd) continue;
+                if (bv <= dist) {
+                  float cur = dd, curi = d_imp;
+                  if (cur > dist) {
+                    cur = dist;
+                    curi = b_imp + (dist - bv) * imp;
+                  }
+                  sd[t] += (double)(cur - bv);
+                  const float hh = b_imp < 1.0f ? logf(curi) : logf(fmaxf(curi / b_imp, 1.1920929e-07f));
+                  sh[t] += (double)hh;
+                  const float bb = beta == 0.0f ? curi - b_imp : (expf(nbeta * curi) - expf(nbeta * b_imp)) * inv;
+                  sb[t] += (double)bb;
                 }
-                sd[t] += (double)(cur - bv);
-                const float hh = b_imp < 1.0f ? logf(curi) : logf(fmaxf(curi / b_imp, 1.1920929e-07f));
-                sh[t] += (double)hh;
-                const float bb = beta == 0.0f ? curi - b_imp : (expf(nbeta * curi) - expf(nbeta * b_imp)) * inv;
-                sb[t] += (double)bb;
               }
             }
           }
@@ -706,7 +837,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
         for (int t = 0; t < NT; ++t) {
           const double t0 = wave_sum(sd[t]), t1 = wave_sum(sh[t]), t2 = wave_sum(sb[t]);
           if (lane == 0) {
-            double *accs = a.acc + (size_t)c.src * M * NT;
+            double *accs = j.acc + (size_t)c.src * M * NT;
             gadd_f64(&accs[0 * NT + t], t0);
             gadd_f64(&accs[1 * NT + t], t1);
             gadd_f64(&accs[2 * NT + t], t2);
@@ -714,28 +845,33 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
         }
       }
       // ---- betweenness: subtree sums along the single-pred tree (centrality.rs:2319-2401)
-      if (a.flags & 2u) {
+      if (j.flags & 2u) {
+        uint32_t s1 = R;
+        uint32_t s0 = uniform(ld32(&w.bnd[c.nb - 1]));
         for (int b = (int)c.nb - 1; b >= 0; --b) {
-          const uint32_t s0 = ld32(&w.bnd[b]), s1 = ld32(&w.bnd[b + 1]);
+          const uint32_t s0n = (b > 0) ? ld32(&w.bnd[b - 1]) : 0u;
           for (uint32_t s = s0 + lane; s < s1; s += kWave) {
-            const uint32_t v = ld32(&w.node[s]);
-            if (v == c.src) continue;
-            const float sdist = ldf(&w.secs[s]) * speed;
-            const uint32_t pslot = (uint32_t)ld64(&w.a8[s]);
-            const uint64_t lb = ld64(&w.b8[s]);
+            if (s == 0) continue;
+            const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+            const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
+            const uint64_t lb = ld64(&w.pm[s]);
+            double below[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) below[t] = ldd(&w.acc[(size_t)s * NT + t]);
+            const uint32_t v = (uint32_t)lo;
+            const float sdist = __uint_as_float((uint32_t)(lo >> 32)) * speed;
+            const uint32_t pslot = (uint32_t)hi;
             const float llen = __uint_as_float((uint32_t)lb);
             const float olen = __uint_as_float((uint32_t)(lb >> 32));
-            const uint32_t pred = ld32(&w.node[pslot]);
             const float min_span = sdist - olen - llen;
             const float o_1 = min_span, o_2 = min_span + olen, l_1 = min_span, l_2 = min_span + llen;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-              const double below = ldd(&w.acc[(size_t)s * NT + t]);
-              if (below > 0.0) gadd_f64(&a.acc[(size_t)v * M * NT + 3 * NT + t], below);
+              if (below[t] > 0.0) gadd_f64(&j.acc[(size_t)v * M * NT + 3 * NT + t], below[t]);
               double own = 0.0;
               if (v > c.src) {
-                const float dist = a.dist[t];
-                const float beta = a.beta[t];
+                const float dist = j.dist[t];
+                const float beta = j.beta[t];
                 if (min_span <= dist) {
                   const float o2s = fminf(o_2, dist), l2s = fminf(l_2, dist);
                   float auc;
@@ -748,33 +884,37 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
                   if (isfinite(auc) && auc >= 0.0f) own = (double)auc;
                 }
               }
-              const double up = below + own;
-              if (pred != c.src && up != 0.0) wadd_f64(&w.acc[(size_t)pslot * NT + t], up);
+              const double up = below[t] + own;
+              if (pslot != 0 && up != 0.0) wadd_f64(&w.acc[(size_t)pslot * NT + t], up);
             }
           }
           wave_fence();
+          s1 = s0;
+          s0 = uniform(s0n);
         }
       }
     } else {
       // single-source exports (trees / Brandes state)
       for (uint32_t s = lane; s < R; s += kWave) {
-        const uint32_t v = ld32(&w.node[s]);
-        a.tout.secs[v] = ldf(&w.secs[s]);
+        const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+        const uint32_t v = (uint32_t)lo;
+        a.tout.secs[v] = __uint_as_float((uint32_t)(lo >> 32));
         if constexpr (MODE == kTree) {
-          const uint64_t pa = ld64(&w.a8[s]);
-          if (v != c.src) {
-            a.tout.pred[v] = (int32_t)ld32(&w.node[(uint32_t)pa]);
-            a.tout.last_eid[v] = (int32_t)a.fwd_eid[a.fwd_off[v] + (uint32_t)(pa >> 32)];
+          const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
+          if (s != 0) {
+            const uint32_t pslot = (uint32_t)hi;
+            a.tout.pred[v] = (int32_t)(uint32_t)ld64(&w.rec[2 * (size_t)pslot]);
+            a.tout.last_eid[v] = (int32_t)j.fwd_eid[j.rows[v].x + (uint32_t)(hi >> 32)];
           }
         } else if constexpr (MODE == kBrandesExport) {
-          const uint64_t pm = ld64(&w.b8[s]);
-          a.tout.sigma[v] = ldd(reinterpret_cast<const double *>(&w.a8[s]));
+          const uint64_t pm = ld64(&w.pm[s]);
+          a.tout.sigma[v] = ldd(&w.sig[s]);
           uint32_t np = 0;
-          const uint32_t f0 = a.fwd_off[v];
-          for (uint64_t m = pm; m; m &= m - 1) {
-            const uint32_t j = __builtin_ctzll(m);
+          const uint32_t f0 = j.rows[v].x;
+          for (uint64_t mm = pm; mm; mm &= mm - 1) {
+            const uint32_t jb = __builtin_ctzll(mm);
             if (np < a.tout.max_preds)
-              a.tout.pred_list[(size_t)v * a.tout.max_preds + np] = a.fwd_adj[f0 + j].x & kNodeMask;
+              a.tout.pred_list[(size_t)v * a.tout.max_preds + np] = j.fwd_adj[f0 + jb].x & kNodeMask;
             ++np;
           }
           a.tout.n_preds[v] = np;
@@ -788,68 +928,88 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
     atomicAdd(&a.stats[0], st_reached);
     atomicAdd(&a.stats[2], st_within);
   }
-  st_edges = wave_sum(st_edges);
-  if (lane == 0) atomicAdd(&a.stats[1], st_edges);
+  const unsigned long long edges = wave_sum((unsigned long long)st_edges);
+  if (lane == 0) atomicAdd(&a.stats[1], edges);
 }
 
 // ---------------------------------------------------------------------------
-// Host side: workspace sizing and launch with retry.
+// Host side: job tables, workspace sizing and launch with retry.
 // ---------------------------------------------------------------------------
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+template <int MODE, int NT>
+void launch_one(uint32_t blocks, uint32_t block, size_t lds, hipStream_t stream, const SearchArgs &a) {
+  hipLaunchKernelGGL((k_search<MODE, NT>), dim3(blocks), dim3(block), lds, stream, a);
+}
+
+template <int MODE, int NT>
+int occupancy_one(uint32_t block, size_t lds) {
+  int nb = 0;
+  hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_search<MODE, NT>, (int)block, lds),
+            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  return nb;
+}
+
+#define CS_NT_SWITCH(T, CALL)                          \
+  switch (T) {                                         \
+    case 1: CALL(1); break;                            \
+    case 2: CALL(2); break;                            \
+    case 3: CALL(3); break;                            \
+    case 4: CALL(4); break;                            \
+    case 5: CALL(5); break;                            \
+    case 6: CALL(6); break;                            \
+    case 7: CALL(7); break;                            \
+    case 8: CALL(8); break;                            \
+    default: fail(CS_EINVAL, "unsupported number of thresholds"); \
+  }
+
 template <int MODE>
-void launch_nt(uint32_t T, uint32_t blocks, hipStream_t stream, const SearchArgs &a) {
+void launch_nt(uint32_t T, uint32_t blocks, uint32_t block, size_t lds, hipStream_t stream, const SearchArgs &a) {
   if constexpr (MODE == kTree || MODE == kBrandesExport) {
-    hipLaunchKernelGGL((k_search<MODE, 1>), dim3(blocks), dim3(kBlock), 0, stream, a);
+    launch_one<MODE, 1>(blocks, block, lds, stream, a);
   } else {
-    switch (T) {
-#define CS_NT_CASE(n) \
-  case n:             \
-    hipLaunchKernelGGL((k_search<MODE, n>), dim3(blocks), dim3(kBlock), 0, stream, a); \
-    break;
-      CS_NT_CASE(1)
-      CS_NT_CASE(2)
-      CS_NT_CASE(3)
-      CS_NT_CASE(4)
-      CS_NT_CASE(5)
-      CS_NT_CASE(6)
-      CS_NT_CASE(7)
-      CS_NT_CASE(8)
-#undef CS_NT_CASE
-      default:
-        fail(CS_EINVAL, "unsupported number of thresholds");
-    }
+#define CS_CALL(n) launch_one<MODE, n>(blocks, block, lds, stream, a)
+    CS_NT_SWITCH(T, CS_CALL)
+#undef CS_CALL
   }
 }
 
+template <int MODE>
+int blocks_per_cu(uint32_t T, uint32_t block, size_t lds) {
+  int nb = 0;
+  if constexpr (MODE == kTree || MODE == kBrandesExport) {
+    nb = occupancy_one<MODE, 1>(block, lds);
+  } else {
+#define CS_CALL(n) nb = occupancy_one<MODE, n>(block, lds)
+    CS_NT_SWITCH(T, CS_CALL)
+#undef CS_CALL
+  }
+  return nb;
+}
+
 struct Plan {
-  float delta;
-  uint32_t nbk, bcap, cap, maxb, waves;
+  uint32_t bcap, cap, maxb, nbk, waves;
   size_t stride;
-  size_t o_keyw, o_slotw, o_node, o_secs, o_a8, o_b8, o_acc, o_bkt, o_bcnt, o_bnd;
+  size_t o_keyw, o_slotw, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd;
 };
 
-void layout(Plan &p, uint32_t N, uint32_t T) {
+void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   size_t o = 0;
   p.o_keyw = o;
   o = align256(o + 8ull * N);
   p.o_slotw = o;
   o = align256(o + 4ull * N);
-  p.o_node = o;
-  o = align256(o + 4ull * p.cap);
-  p.o_secs = o;
-  o = align256(o + 4ull * p.cap);
-  p.o_a8 = o;
-  o = align256(o + 8ull * p.cap);
-  p.o_b8 = o;
+  p.o_rec = o;
+  o = align256(o + 16ull * p.cap);
+  p.o_sig = o;
+  o = align256(o + (brandes ? 8ull * p.cap : 0));
+  p.o_pm = o;
   o = align256(o + 8ull * p.cap);
   p.o_acc = o;
-  o = align256(o + 8ull * p.cap * 2 * T);
+  o = align256(o + 8ull * p.cap * (brandes ? 2 * T : T));
   p.o_bkt = o;
   o = align256(o + 8ull * p.nbk * p.bcap);
-  p.o_bcnt = o;
-  o = align256(o + 4ull * p.nbk);
   p.o_bnd = o;
   o = align256(o + 4ull * (p.maxb + 2));
   p.stride = o;
@@ -937,58 +1097,148 @@ void prepare_ws(Workspace &w, uint32_t N, size_t stride, uint32_t waves, uint64_
   w.epoch_hi += sources;
 }
 
-template <int MODE>
-void run_search(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const uint32_t *d_sources,
-                const float *d_wt, const float *d_cyc, uint32_t n_sources, const uint8_t *d_eligible,
-                double *d_acc, uint32_t *d_reach, const TreeOut *tout, hipStream_t stream,
-                cs_stats &st) {
-  if (n_sources == 0) return;
-  const uint32_t N = g.n;
-  const uint32_t T = cs.T;
-  Plan p;
-  p.delta = choose_delta(g, de, cs.maxsec);
-  const float inv_delta = 1.0f / p.delta;
-  const uint32_t jump = (uint32_t)std::ceil((double)de.max_es / (double)p.delta) + 2;
-  p.nbk = jump + 1;
-  if (p.nbk > 4096) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
-  p.maxb = (uint32_t)((double)cs.maxsec * (double)inv_delta) + 4;
-  p.bcap = 1024;
-  p.cap = std::min<uint32_t>(N, 16384);
-  int dev = dg.device;
+struct DevInfo {
+  uint32_t cus = 0;
+};
+
+const DevInfo &dev_info(int dev) {
+  static std::mutex mu;
+  static std::map<int, DevInfo> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-  const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+  DevInfo d;
+  d.cus = (uint32_t)prop.multiProcessorCount;
+  return cache[dev] = d;
+}
+
+template <typename T>
+struct DevArr {
+  T *p = nullptr;
+  explicit DevArr(size_t n) {
+    if (n) hip_check(hipMalloc(&p, sizeof(T) * n), "hipMalloc");
+  }
+  ~DevArr() {
+    if (p) (void)hipFree(p);
+  }
+  DevArr(const DevArr &) = delete;
+  DevArr &operator=(const DevArr &) = delete;
+};
+
+Job make_job(const JobSpec &js, uint32_t NT) {
+  Graph &g = *js.g;
+  const DevEdges &de = *js.de;
+  const CallSpec &cs = js.cs;
+  Job j;
+  std::memset(&j, 0, sizeof(j));
+  const float delta = choose_delta(g, de, cs.maxsec) * js.delta_scale;
+  j.rows = js.dg->rows;
+  j.rev_adj = de.rev_adj;
+  j.fwd_adj = de.fwd_adj;
+  j.rev_tw = de.rev_tw;
+  j.fwd_twlen = de.fwd_twlen;
+  j.fwd_eid = de.fwd_eid;
+  j.eligible = js.d_eligible;
+  j.sources = js.d_sources;
+  j.src_wt = js.d_wt;
+  j.src_cyc = js.d_cyc;
+  j.acc = js.d_acc;
+  j.reach = js.d_reach;
+  j.maxsec = cs.maxsec;
+  j.speed = cs.speed;
+  j.inv_delta = 1.0f / delta;
+  const uint32_t jump = (uint32_t)std::ceil((double)de.max_es / (double)delta) + 2;
+  j.nbk = jump + 1;
+  if (j.nbk > 4096) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
+  j.maxb = (uint32_t)((double)cs.maxsec * (double)j.inv_delta) + 4;
+  j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
+  j.n_sources = js.n_sources;
+  for (uint32_t i = 0; i < kMaxT; ++i) {
+    j.dist[i] = i < cs.T ? cs.dist[i] : -1.0f;
+    j.beta[i] = i < cs.T ? cs.beta[i] : 0.0f;
+  }
+  (void)NT;
+  return j;
+}
+
+Tuning g_tuning;
+std::mutex g_tuning_mu;
+
+// Runs the sources of every job (or only `explicit_items`, {job, source
+// position} pairs) in one persistent launch, retrying sources that outgrew the
+// workspace with a larger one.
+template <int MODE>
+void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune, const TreeOut *tout,
+              hipStream_t stream, cs_stats &st, const std::vector<uint2> *explicit_items = nullptr) {
+  constexpr bool kBr = (MODE == kBrandes || MODE == kBrandesExport);
+  uint64_t n_items = 0;
+  for (auto &js : specs) n_items += js.n_sources;
+  if (explicit_items) n_items = explicit_items->size();
+  if (n_items == 0) return;
+  if (n_items >= 0xffffffffull) fail(CS_EINVAL, "too many sources in one launch");
+  const int dev = specs[0].dg->device;
+  const DevInfo &di = dev_info(dev);
+  std::vector<Job> jobs;
+  uint32_t maxN = 0;
+  Plan p;
+  p.nbk = 1;
+  p.maxb = 1;
+  for (auto &js : specs) {
+    jobs.push_back(make_job(js, NT));
+    maxN = std::max(maxN, js.g->n);
+    p.nbk = std::max(p.nbk, jobs.back().nbk);
+    p.maxb = std::max(p.maxb, jobs.back().maxb);
+  }
+  const uint32_t block = tune.block ? tune.block : kBlock;
+  if (block % kWave || block > (uint32_t)kBlock) fail(CS_EINVAL, "workgroup size must be 64, 128, 192 or 256");
+  const uint32_t wpb = block / kWave;
+  p.bcap = 1024;
+  p.cap = std::min<uint32_t>(maxN, tune.cap ? tune.cap : 16384);
+  const size_t lds = (size_t)wpb * p.nbk * 4;
+  const int bpc = std::max(1, blocks_per_cu<MODE>(NT, block, lds));
+  uint32_t resident = di.cus * (uint32_t)bpc * wpb;
+  if (tune.waves_per_simd) resident = std::min<uint32_t>(resident, di.cus * 4 * tune.waves_per_simd);
   size_t free_b = 0, total_b = 0;
   hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
 
-  // device scratch for the driver loop
-  uint32_t *d_ctl = nullptr;  // next_source, n_fail, fail_flags, pad
-  unsigned long long *d_stats = nullptr;
-  uint32_t *d_fail = nullptr;
-  uint32_t *d_src_cur = nullptr;
-  float *d_wt_cur = nullptr, *d_cyc_cur = nullptr;
-  hip_check(hipMalloc(&d_ctl, 16), "hipMalloc ctl");
-  hip_check(hipMalloc(&d_stats, 32), "hipMalloc stats");
-  hip_check(hipMalloc(&d_fail, 4ull * n_sources), "hipMalloc fail");
-  hip_check(hipMemsetAsync(d_stats, 0, 32, stream), "memset stats");
+  DevArr<Job> d_jobs(jobs.size());
+  hip_check(hipMemcpyAsync(d_jobs.p, jobs.data(), sizeof(Job) * jobs.size(), hipMemcpyHostToDevice, stream),
+            "copy jobs");
+  std::vector<uint2> items;  // explicit items only for several jobs or a retry pass
+  if (explicit_items) {
+    items = *explicit_items;
+  } else if (specs.size() > 1) {
+    items.reserve(n_items);
+    for (uint32_t jb = 0; jb < specs.size(); ++jb)
+      for (uint32_t i = 0; i < specs[jb].n_sources; ++i) items.push_back(make_uint2(jb, i));
+  }
+  DevArr<uint2> d_items(n_items);
+  if (!items.empty())
+    hip_check(hipMemcpyAsync(d_items.p, items.data(), sizeof(uint2) * items.size(), hipMemcpyHostToDevice, stream),
+              "copy items");
+  const bool use_items = !items.empty();
+  DevArr<uint32_t> d_ctl(16);
+  DevArr<unsigned long long> d_stats(4);
+  DevArr<uint32_t> d_fail(n_items);
+  hip_check(hipMemsetAsync(d_stats.p, 0, 32, stream), "memset stats");
 
   Workspace *wsp = acquire_ws(dev);
   WsGuard guard{wsp};
-  const uint32_t *src_list = d_sources;
-  const float *wt_list = d_wt;
-  const float *cyc_list = d_cyc;
-  uint32_t n_list = n_sources;
+  uint32_t n_list = (uint32_t)n_items;
+  bool list_items = use_items;
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
   double kms = 0.0;
   for (int attempt = 0; attempt < 6; ++attempt) {
-    layout(p, N, T);
-    // waves: enough to fill the chip, bounded by sources and by a quarter of free HBM
-    uint32_t want = std::min<uint32_t>(n_list, cus * 32);
+    layout(p, maxN, NT, kBr);
+    // waves: every wave resident at once, bounded by items and by a quarter of free HBM
+    const uint32_t want = std::min<uint32_t>(n_list, resident);
     const size_t budget = free_b / 4 + (wsp->bytes);
-    uint32_t by_mem = (uint32_t)std::min<size_t>(budget / p.stride, 1u << 20);
-    if (by_mem < 4) {
+    const uint32_t by_mem = (uint32_t)std::min<size_t>(budget / p.stride, 1u << 20);
+    if (by_mem < wpb) {
       if (p.cap > 4096) {
         p.cap = std::max<uint32_t>(4096, p.cap / 2);
         continue;
@@ -996,143 +1246,371 @@ void run_search(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const 
       fail(CS_ENOMEM, "graph too large for the search workspace on this device");
     }
     uint32_t waves = std::min(want, by_mem);
-    waves = std::max<uint32_t>(4, (waves + 3) / 4 * 4);
+    waves = std::max<uint32_t>(wpb, (waves + wpb - 1) / wpb * wpb);
     p.waves = waves;
-    prepare_ws(*wsp, N, p.stride, waves, n_list, stream);
-    hip_check(hipMemsetAsync(d_ctl, 0, 16, stream), "memset ctl");
+    prepare_ws(*wsp, maxN, p.stride, waves, n_list, stream);
+    hip_check(hipMemsetAsync(d_ctl.p, 0, 64, stream), "memset ctl");
     SearchArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.N = N;
-    a.rev_off = dg.rev_off;
-    a.rev_adj = de.rev_adj;
-    a.fwd_off = dg.fwd_off;
-    a.fwd_adj = de.fwd_adj;
-    a.rev_tw = de.rev_tw;
-    a.fwd_twlen = de.fwd_twlen;
-    a.fwd_eid = de.fwd_eid;
-    a.eligible = d_eligible;
-    a.T = T;
-    a.maxsec = cs.maxsec;
-    a.speed = cs.speed;
-    a.inv_delta = inv_delta;
-    a.tolerance = cs.tolerance;
-    for (uint32_t i = 0; i < kMaxT; ++i) {
-      a.dist[i] = i < T ? cs.dist[i] : 0.0f;
-      a.beta[i] = i < T ? cs.beta[i] : 0.0f;
-    }
-    a.nbk = p.nbk;
+    a.jobs = d_jobs.p;
+    a.items = list_items ? d_items.p : nullptr;
+    a.n_items = n_list;
     a.bcap = p.bcap;
     a.cap = p.cap;
-    a.maxb = p.maxb;
-    a.n_sources = n_list;
-    a.sources = src_list;
-    a.src_wt = wt_list;
-    a.src_cyc = cyc_list;
-    a.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
-    a.acc = d_acc;
-    a.reach = d_reach;
-    a.stats = d_stats;
+    a.nbk_lds = p.nbk;
+    a.stats = d_stats.p;
     a.ws = wsp->ws;
     a.ws_stride = p.stride;
     a.o_keyw = p.o_keyw;
     a.o_slotw = p.o_slotw;
-    a.o_node = p.o_node;
-    a.o_secs = p.o_secs;
-    a.o_a8 = p.o_a8;
-    a.o_b8 = p.o_b8;
+    a.o_rec = p.o_rec;
+    a.o_sig = p.o_sig;
+    a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
     a.o_bkt = p.o_bkt;
-    a.o_bcnt = p.o_bcnt;
     a.o_bnd = p.o_bnd;
     a.epochs = wsp->epochs;
-    a.next_source = d_ctl;
-    a.n_fail = d_ctl + 1;
-    a.fail_flags = d_ctl + 2;
-    a.fail_list = d_fail;
+    a.next = d_ctl.p;
+    a.xsplit = tune.xcd_split ? 1u : 0u;
+    for (uint32_t x = 0; x <= kXcd; ++x)
+      a.xbeg[x] = a.xsplit ? (uint32_t)((uint64_t)n_list * x / kXcd) : (x ? n_list : 0u);
+    a.n_fail = d_ctl.p + kXcd;
+    a.fail_flags = d_ctl.p + kXcd + 1;
+    a.fail_list = d_fail.p;
     if (tout) a.tout = *tout;
-    {
-      const uint32_t blocks = waves / 4;
-      hip_check(hipEventRecord(e0, stream), "event record");
-      launch_nt<MODE>(T, blocks, stream, a);
-      hip_check(hipGetLastError(), "k_search launch");
-      hip_check(hipEventRecord(e1, stream), "event record");
-    }
-    uint32_t ctl[4];
-    hip_check(hipMemcpyAsync(ctl, d_ctl, 16, hipMemcpyDeviceToHost, stream), "copy ctl");
+    hip_check(hipEventRecord(e0, stream), "event record");
+    launch_nt<MODE>(NT, waves / wpb, block, lds, stream, a);
+    hip_check(hipGetLastError(), "k_search launch");
+    hip_check(hipEventRecord(e1, stream), "event record");
+    uint32_t ctl[16];
+    hip_check(hipMemcpyAsync(ctl, d_ctl.p, 64, hipMemcpyDeviceToHost, stream), "copy ctl");
     hip_check(hipStreamSynchronize(stream), "sync");
     float ms = 0.0f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
     kms += ms;
     st.waves = waves;
-    const uint32_t n_failed = ctl[1], flags = ctl[2];
+    const uint32_t n_failed = ctl[kXcd], flags = ctl[kXcd + 1];
     if (n_failed == 0) break;
     if (flags & kFailInternal) fail(CS_EINTERNAL, "search invariant violated (bucket order)");
     if (flags & kFailRing) fail(CS_EINTERNAL, "bucket ring window exceeded");
     st.retries += n_failed;
-    // retry the failed sources with a larger workspace
-    if (!d_src_cur) {
-      hip_check(hipMalloc(&d_src_cur, 4ull * n_sources), "hipMalloc");
-      hip_check(hipMalloc(&d_wt_cur, 4ull * n_sources), "hipMalloc");
-      hip_check(hipMalloc(&d_cyc_cur, 4ull * n_sources), "hipMalloc");
-    }
-    // gather failed (source, wt, cyc) on the host: small lists
-    std::vector<uint32_t> fl(n_failed), srcs(n_list);
-    std::vector<float> wts(n_list, 0.0f), cycs(n_list, 0.0f);
-    hip_check(hipMemcpy(fl.data(), d_fail, 4ull * n_failed, hipMemcpyDeviceToHost), "copy");
-    hip_check(hipMemcpy(srcs.data(), src_list, 4ull * n_list, hipMemcpyDeviceToHost), "copy");
-    if (wt_list) hip_check(hipMemcpy(wts.data(), wt_list, 4ull * n_list, hipMemcpyDeviceToHost), "copy");
-    if (cyc_list) hip_check(hipMemcpy(cycs.data(), cyc_list, 4ull * n_list, hipMemcpyDeviceToHost), "copy");
-    std::vector<uint32_t> ns(n_failed);
-    std::vector<float> nw(n_failed), nc(n_failed);
-    for (uint32_t i = 0; i < n_failed; ++i) {
-      ns[i] = srcs[fl[i]];
-      nw[i] = wts[fl[i]];
-      nc[i] = cycs[fl[i]];
-    }
-    hip_check(hipMemcpy(d_src_cur, ns.data(), 4ull * n_failed, hipMemcpyHostToDevice), "copy");
-    hip_check(hipMemcpy(d_wt_cur, nw.data(), 4ull * n_failed, hipMemcpyHostToDevice), "copy");
-    hip_check(hipMemcpy(d_cyc_cur, nc.data(), 4ull * n_failed, hipMemcpyHostToDevice), "copy");
-    src_list = d_src_cur;
-    wt_list = wt_list ? d_wt_cur : nullptr;
-    cyc_list = cyc_list ? d_cyc_cur : nullptr;
+    // retry the failed items with a larger workspace
+    std::vector<uint32_t> fl(n_failed);
+    hip_check(hipMemcpy(fl.data(), d_fail.p, 4ull * n_failed, hipMemcpyDeviceToHost), "copy");
+    std::vector<uint2> retry(n_failed);
+    for (uint32_t i = 0; i < n_failed; ++i) retry[i] = list_items ? items[fl[i]] : make_uint2(0u, fl[i]);
+    items.swap(retry);
+    hip_check(hipMemcpy(d_items.p, items.data(), sizeof(uint2) * n_failed, hipMemcpyHostToDevice), "copy");
+    list_items = true;
     n_list = n_failed;
-    if (flags & kFailCap) p.cap = std::min<uint32_t>(N, p.cap * 4);
+    if (flags & kFailCap) p.cap = std::min<uint32_t>(maxN, p.cap * 4);
     if (flags & kFailBucket) p.bcap *= 4;
     if (attempt == 5) fail(CS_EINTERNAL, "search workspace retries exhausted");
   }
   unsigned long long stv[4];
-  hip_check(hipMemcpy(stv, d_stats, 32, hipMemcpyDeviceToHost), "copy stats");
+  hip_check(hipMemcpy(stv, d_stats.p, 32, hipMemcpyDeviceToHost), "copy stats");
   st.kernel_ms += kms;
   st.nodes_reached += stv[0];
   st.edges_scanned += stv[1];
   st.within_sum += stv[2];
   hip_check(hipEventDestroy(e0), "event");
   hip_check(hipEventDestroy(e1), "event");
-  hip_check(hipFree(d_ctl), "free");
-  hip_check(hipFree(d_stats), "free");
-  hip_check(hipFree(d_fail), "free");
-  if (d_src_cur) {
-    hip_check(hipFree(d_src_cur), "free");
-    hip_check(hipFree(d_wt_cur), "free");
-    hip_check(hipFree(d_cyc_cur), "free");
-  }
+}
+
+Tuning current_tuning() {
+  std::lock_guard<std::mutex> lk(g_tuning_mu);
+  return g_tuning;
 }
 
 }  // namespace
+
+// Clears `n` 64-bit words (a kernel rather than a memset node, so that it can
+// sit inside a captured hipGraph).
+__global__ void k_zero64(uint64_t *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// A fixed set of shortest-path jobs prepared once and launched many times:
+// job table, item list, control words and a workspace of its own, captured
+// with the clearing of the accumulators into one hipGraph that each run
+// replays. Sources that outgrow the workspace are retried outside the graph.
+struct ShortestBatch {
+  std::vector<JobSpec> specs;
+  std::vector<ZeroRegion> zero;
+  uint32_t NT = 1;
+  int device = 0;
+  Tuning tune;
+  bool ready = false;
+  uint64_t n_items = 0;
+  Plan p{};
+  uint32_t block = kBlock, wpb = kBlock / kWave, waves = 0;
+  size_t lds = 0;
+  std::vector<Job> jobs;
+  std::vector<uint2> items;
+  Job *d_jobs = nullptr;
+  uint2 *d_items = nullptr;
+  uint32_t *d_ctl = nullptr, *d_fail = nullptr;
+  unsigned long long *d_stats = nullptr;
+  Workspace ws;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~ShortestBatch() { release(); }
+
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    for (void *q : {(void *)d_jobs, (void *)d_items, (void *)d_ctl, (void *)d_fail, (void *)d_stats, (void *)ws.ws,
+                    (void *)ws.epochs})
+      if (q) (void)hipFree(q);
+    d_jobs = nullptr;
+    d_items = nullptr;
+    d_ctl = d_fail = nullptr;
+    d_stats = nullptr;
+    ws.ws = nullptr;
+    ws.epochs = nullptr;
+    ws.bytes = 0;
+    ws.waves = 0;
+    ready = false;
+  }
+
+  void prepare(hipStream_t stream) {
+    release();
+    n_items = 0;
+    for (auto &js : specs) n_items += js.n_sources;
+    if (n_items >= 0xffffffffull) fail(CS_EINVAL, "too many sources in one batch");
+    jobs.clear();
+    uint32_t maxN = 0;
+    p = Plan{};
+    p.nbk = 1;
+    p.maxb = 1;
+    for (auto &js : specs) {
+      JobSpec j2 = js;
+      j2.delta_scale = tune.delta_scale;
+      jobs.push_back(make_job(j2, NT));
+      maxN = std::max(maxN, js.g->n);
+      p.nbk = std::max(p.nbk, jobs.back().nbk);
+      p.maxb = std::max(p.maxb, jobs.back().maxb);
+    }
+    block = tune.block ? tune.block : kBlock;
+    if (block % kWave || block > (uint32_t)kBlock) fail(CS_EINVAL, "workgroup size must be 64, 128, 192 or 256");
+    wpb = block / kWave;
+    p.bcap = 1024;
+    p.cap = std::min<uint32_t>(maxN, tune.cap ? tune.cap : 16384);
+    lds = (size_t)wpb * p.nbk * 4;
+    const DevInfo &di = dev_info(device);
+    const int bpc = std::max(1, blocks_per_cu<kBrandes>(NT, block, lds));
+    uint32_t resident = di.cus * (uint32_t)bpc * wpb;
+    if (tune.waves_per_simd) resident = std::min<uint32_t>(resident, di.cus * 4 * tune.waves_per_simd);
+    layout(p, maxN, NT, true);
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const uint32_t by_mem = (uint32_t)std::min<size_t>((free_b / 4) / p.stride, 1u << 20);
+    if (by_mem < wpb) fail(CS_ENOMEM, "batch too large for the search workspace on this device");
+    waves = std::min<uint32_t>((uint32_t)std::min<uint64_t>(n_items, resident), by_mem);
+    waves = std::max<uint32_t>(wpb, (waves + wpb - 1) / wpb * wpb);
+    items.clear();
+    if (specs.size() > 1) {
+      items.reserve(n_items);
+      for (uint32_t jb = 0; jb < specs.size(); ++jb)
+        for (uint32_t i = 0; i < specs[jb].n_sources; ++i) items.push_back(make_uint2(jb, i));
+    }
+    hip_check(hipMalloc(&d_jobs, sizeof(Job) * jobs.size()), "hipMalloc jobs");
+    hip_check(hipMemcpy(d_jobs, jobs.data(), sizeof(Job) * jobs.size(), hipMemcpyHostToDevice), "copy jobs");
+    hip_check(hipMalloc(&d_items, sizeof(uint2) * std::max<uint64_t>(n_items, 1)), "hipMalloc items");
+    if (!items.empty())
+      hip_check(hipMemcpy(d_items, items.data(), sizeof(uint2) * items.size(), hipMemcpyHostToDevice), "copy");
+    hip_check(hipMalloc(&d_ctl, 64), "hipMalloc ctl");
+    hip_check(hipMalloc(&d_stats, 32), "hipMalloc stats");
+    hip_check(hipMalloc(&d_fail, 4 * std::max<uint64_t>(n_items, 1)), "hipMalloc fail");
+    ws.device = device;
+    prepare_ws(ws, maxN, p.stride, waves, 0, stream);
+    hip_check(hipStreamSynchronize(stream), "sync");
+    // capture: clear control words, statistics and accumulators, then search
+    SearchArgs a = args();
+    hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t *>(d_ctl), (size_t)8);
+    hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t *>(d_stats), (size_t)4);
+    for (const ZeroRegion &z : zero) {
+      const size_t grid = std::min<size_t>((z.words + 255) / 256, 4096);
+      if (z.words) hipLaunchKernelGGL(k_zero64, dim3((uint32_t)grid), dim3(256), 0, stream, z.ptr, z.words);
+    }
+    launch_nt<kBrandes>(NT, waves / wpb, block, lds, stream, a);
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(stream, &graph);
+    hip_check(le, "k_search capture");
+    hip_check(ce, "end capture");
+    hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    ready = true;
+  }
+
+  SearchArgs args() const {
+    SearchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.jobs = d_jobs;
+    a.items = items.empty() ? nullptr : d_items;
+    a.n_items = (uint32_t)n_items;
+    a.bcap = p.bcap;
+    a.cap = p.cap;
+    a.nbk_lds = p.nbk;
+    a.stats = d_stats;
+    a.ws = ws.ws;
+    a.ws_stride = p.stride;
+    a.o_keyw = p.o_keyw;
+    a.o_slotw = p.o_slotw;
+    a.o_rec = p.o_rec;
+    a.o_sig = p.o_sig;
+    a.o_pm = p.o_pm;
+    a.o_acc = p.o_acc;
+    a.o_bkt = p.o_bkt;
+    a.o_bnd = p.o_bnd;
+    a.epochs = ws.epochs;
+    a.next = d_ctl;
+    a.xsplit = tune.xcd_split ? 1u : 0u;
+    for (uint32_t x = 0; x <= kXcd; ++x)
+      a.xbeg[x] = a.xsplit ? (uint32_t)(n_items * x / kXcd) : (x ? (uint32_t)n_items : 0u);
+    a.n_fail = d_ctl + kXcd;
+    a.fail_flags = d_ctl + kXcd + 1;
+    a.fail_list = d_fail;
+    return a;
+  }
+
+  void run(hipStream_t stream, cs_stats &st) {
+    if (!ready) prepare(stream);
+    if (n_items == 0) return;
+    // epochs: every wave may use up to n_items of them per run
+    if (ws.epoch_hi + n_items + 2 >= 0xfffffff0ull) {
+      hip_check(hipMemsetAsync(ws.ws, 0, ws.bytes, stream), "hipMemset ws");
+      hip_check(hipMemsetAsync(ws.epochs, 0, 4ull * ws.waves, stream), "hipMemset epochs");
+      ws.epoch_hi = 0;
+    }
+    ws.epoch_hi += n_items;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, stream), "event record");
+    hip_check(hipGraphLaunch(exec, stream), "graph launch");
+    hip_check(hipEventRecord(e1, stream), "event record");
+    uint32_t ctl[16];
+    unsigned long long stv[4];
+    hip_check(hipMemcpyAsync(ctl, d_ctl, 64, hipMemcpyDeviceToHost, stream), "copy ctl");
+    hip_check(hipMemcpyAsync(stv, d_stats, 32, hipMemcpyDeviceToHost, stream), "copy stats");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st.kernel_ms += ms;
+    st.waves = waves;
+    st.nodes_reached += stv[0];
+    st.edges_scanned += stv[1];
+    st.within_sum += stv[2];
+    const uint32_t n_failed = ctl[kXcd], flags = ctl[kXcd + 1];
+    if (n_failed == 0) return;
+    if (flags & kFailInternal) fail(CS_EINTERNAL, "search invariant violated (bucket order)");
+    if (flags & kFailRing) fail(CS_EINTERNAL, "bucket ring window exceeded");
+    st.retries += n_failed;
+    std::vector<uint32_t> fl(n_failed);
+    hip_check(hipMemcpy(fl.data(), d_fail, 4ull * n_failed, hipMemcpyDeviceToHost), "copy");
+    std::vector<uint2> retry(n_failed);
+    for (uint32_t i = 0; i < n_failed; ++i) retry[i] = items.empty() ? make_uint2(0u, fl[i]) : items[fl[i]];
+    Tuning t2 = tune;
+    t2.cap = std::max<uint32_t>(p.cap * 4, 4096);
+    run_jobs<kBrandes>(specs, NT, t2, nullptr, stream, st, &retry);
+  }
+
+  // Times each candidate launch shape on every k-th source and keeps the
+  // fastest per source. The jobs' accumulators hold garbage afterwards; the
+  // next run clears them.
+  Tuning autotune(double fraction, hipStream_t stream) {
+    uint64_t total = 0;
+    for (auto &js : specs) total += js.n_sources;
+    if (total == 0) return tune;
+    const uint64_t stride = std::max<uint64_t>(1, (uint64_t)std::llround(1.0 / std::max(fraction, 1e-6)));
+    std::vector<uint2> sample;
+    uint64_t k = 0;
+    for (uint32_t jb = 0; jb < specs.size(); ++jb)
+      for (uint32_t i = 0; i < specs[jb].n_sources; ++i, ++k)
+        if (k % stride == 0) sample.push_back(make_uint2(jb, i));
+    Tuning best = tune;
+    double best_ms = INFINITY;
+    for (uint32_t blk : {64u, 128u, 256u})
+      for (uint32_t wps : {0u, 3u, 2u})
+        for (uint32_t xs : {1u, 0u})
+        for (float ds : {1.0f, 0.75f}) {
+          Tuning t;
+          t.block = blk;
+          t.waves_per_simd = wps;
+          t.delta_scale = ds;
+          t.xcd_split = xs;
+          std::vector<JobSpec> sp = specs;
+          for (auto &j : sp) j.delta_scale = ds;
+          cs_stats s1{};
+          try {
+            run_jobs<kBrandes>(sp, NT, t, nullptr, stream, s1, &sample);
+          } catch (const Error &) {
+            continue;  // a shape this device cannot run
+          }
+          if (s1.kernel_ms < best_ms) {
+            best_ms = s1.kernel_ms;
+            best = t;
+          }
+        }
+    tune = best;
+    ready = false;
+    return best;
+  }
+};
+
+void set_default_tuning(const Tuning &t) {
+  std::lock_guard<std::mutex> lk(g_tuning_mu);
+  g_tuning = t;
+}
+
+Tuning default_tuning() { return current_tuning(); }
 
 void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                      uint32_t n_sources, const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
                      hipStream_t stream, cs_stats &st) {
-  run_search<kBrandes>(g, dg, de, cs, d_sources, d_wt, d_cyc, n_sources, d_eligible, d_acc, d_reach,
-                       nullptr, stream, st);
+  const Tuning tune = current_tuning();
+  JobSpec js{&g, &dg, &de, cs, d_sources, d_wt, d_cyc, n_sources, d_eligible, d_acc, d_reach, tune.delta_scale};
+  run_jobs<kBrandes>({js}, cs.T, tune, nullptr, stream, st);
 }
+
+void launch_shortest_batch(const std::vector<JobSpec> &jobs, uint32_t NT, const Tuning &tune, hipStream_t stream,
+                           cs_stats &st) {
+  run_jobs<kBrandes>(jobs, NT, tune, nullptr, stream, st);
+}
+
+ShortestBatch *batch_new(std::vector<JobSpec> specs, std::vector<ZeroRegion> zero, uint32_t NT, int device,
+                         const Tuning &tune) {
+  auto *b = new ShortestBatch();
+  b->specs = std::move(specs);
+  b->zero = std::move(zero);
+  b->NT = NT;
+  b->device = device;
+  b->tune = tune;
+  return b;
+}
+void batch_delete(ShortestBatch *b) { delete b; }
+void batch_set_tuning(ShortestBatch &b, const Tuning &t) {
+  b.tune = t;
+  b.ready = false;
+}
+Tuning batch_tuning(const ShortestBatch &b) { return b.tune; }
+Tuning batch_autotune(ShortestBatch &b, double fraction, hipStream_t stream) { return b.autotune(fraction, stream); }
+void batch_run(ShortestBatch &b, hipStream_t stream, cs_stats &st) { b.run(stream, st); }
 
 void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                     const uint32_t *d_sources, uint32_t n_sources, double *d_acc, hipStream_t stream,
                     cs_stats &st) {
-  run_search<kSegment>(g, dg, de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, d_acc, nullptr,
-                       nullptr, stream, st);
+  const Tuning tune = current_tuning();
+  JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, d_acc, nullptr, tune.delta_scale};
+  run_jobs<kSegment>({js}, cs.T, tune, nullptr, stream, st);
 }
 
 void launch_single(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec, float tolerance,
@@ -1145,16 +1623,15 @@ void launch_single(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float max
   cs.tolerance = tolerance;
   cs.closeness = false;
   cs.betweenness = false;
-  uint32_t *d_src = nullptr;
-  hip_check(hipMalloc(&d_src, 4), "hipMalloc");
-  hip_check(hipMemcpy(d_src, &src, 4, hipMemcpyHostToDevice), "copy");
+  DevArr<uint32_t> d_src(1);
+  hip_check(hipMemcpy(d_src.p, &src, 4, hipMemcpyHostToDevice), "copy");
   cs_stats st{};
+  Tuning tune;  // single-source exports: defaults
+  JobSpec js{&g, &dg, &de, cs, d_src.p, nullptr, nullptr, 1, nullptr, nullptr, nullptr, 1.0f};
   if (mode == 0)
-    run_search<kTree>(g, dg, de, cs, d_src, nullptr, nullptr, 1, nullptr, nullptr, nullptr, &out, stream, st);
+    run_jobs<kTree>({js}, 1, tune, &out, stream, st);
   else
-    run_search<kBrandesExport>(g, dg, de, cs, d_src, nullptr, nullptr, 1, nullptr, nullptr, nullptr, &out,
-                               stream, st);
-  hip_check(hipFree(d_src), "free");
+    run_jobs<kBrandesExport>({js}, 1, tune, &out, stream, st);
 }
 
 }  // namespace cs

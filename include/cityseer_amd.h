@@ -184,6 +184,45 @@ typedef struct cs_stats {
 /* Statistics of the calling thread's last compute call. */
 int cs_last_stats(cs_stats *out);
 
+/* ---------------------------------------------------------------------- */
+/* Launch shape and batching                                              */
+/* ---------------------------------------------------------------------- */
+/* No counterpart in the reference, which runs one rayon loop per call
+ * (rust/src/centrality.rs:1702-1836); these entry points serve callers that
+ * run many centrality requests (many graphs, many threshold sets) at once. */
+
+typedef struct cs_tuning {
+  uint32_t block_threads;   /* workgroup size 64..256, a multiple of 64: sources in flight per workgroup = /64 */
+  uint32_t waves_per_simd;  /* cap on resident waves per SIMD; 0 = as many as fit */
+  float delta_scale;        /* bucket width as a fraction in (0, 1] of its safe maximum */
+  uint32_t slot_capacity;   /* settled-node slots per wave before a retry; 0 = default (16384) */
+  uint32_t xcd_split;       /* 1: each XCD's waves take a contiguous share of the sources first; 0: one counter */
+} cs_tuning;
+
+/* Launch shape of the plain shortest/segment entry points (process-wide). */
+int cs_set_tuning(const cs_tuning *t);
+int cs_get_tuning(cs_tuning *t);
+
+/* A batch of node_centrality_shortest jobs (any graphs, each with its own
+ * thresholds and sources) run as one persistent launch captured in a hipGraph.
+ * Graphs must outlive the batch. Not thread-safe: one thread drives a batch. */
+typedef struct cs_batch cs_batch;
+int cs_batch_create(cs_batch **out);
+void cs_batch_destroy(cs_batch *b);
+/* Adds a job with the arguments of cs_centrality_shortest; `out` ([7][T][N])
+ * and `reach_totals` ([T], may be NULL) are filled by cs_batch_results.
+ * Returns the job index (>= 0) or a negative status. */
+int cs_batch_add_shortest(cs_batch *b, cs_graph *g, const cs_params *p, double *out, uint32_t *reach_totals);
+/* Times candidate launch shapes on a `sample_fraction` of the batch's sources
+ * and keeps the fastest; `chosen` (may be NULL) receives it. */
+int cs_batch_autotune(cs_batch *b, double sample_fraction, cs_tuning *chosen);
+int cs_batch_set_tuning(cs_batch *b, const cs_tuning *t);
+/* Runs every job: the first run uploads the jobs and captures the launch,
+ * later runs replay it. Results stay in device memory until cs_batch_results. */
+int cs_batch_run(cs_batch *b);
+/* Copies the last run's results into the jobs' `out` / `reach_totals` arrays. */
+int cs_batch_results(cs_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
