@@ -365,6 +365,144 @@ def run_batch(args) -> int:
     return 0
 
 
+SERVE_DISTANCES = [400, 800, 1600]
+
+
+def serve_workload(n_requests: int, seed: int = 0):
+    """Graphs decomposed into <= 20 m edges with node counts log-uniform in [2,000, 100,000] (two of
+    them also as duals for simplest-path requests); a stream of requests, each on a random graph,
+    80% shortest / 20% simplest, a random 1%-30% source subset and 1-3 distances from
+    {400, 800, 1600} m."""
+    import numpy as np
+
+    from cityseer.tools import mock
+
+    rng = np.random.default_rng(seed)
+    primal, dual = [], []
+    for i in range(6):
+        target = float(np.exp(np.log(2e3) + (np.log(1e5) - np.log(2e3)) * i / 5))
+        side = max(4, int(round(np.sqrt(target / 11.0))))
+        lat = mock.decompose_lattice(mock.perturbed_lattice(side, seed=100 + i), 20.0)
+        primal.append(mock.network_structure_from_lattice(lat))
+    for i in range(2):
+        side = 8 + 10 * i
+        lat = mock.decompose_lattice(mock.perturbed_lattice(side, seed=200 + i), 20.0)
+        dual.append(mock.network_structure_from_lattice(mock.lattice_dual(lat)))
+    reqs = []
+    for _ in range(n_requests):
+        simplest = rng.random() < 0.2
+        ns = dual[int(rng.integers(len(dual)))] if simplest else primal[int(rng.integers(len(primal)))]
+        n = ns.node_count()
+        frac = float(rng.uniform(0.01, 0.3))
+        k = max(1, int(round(frac * n)))
+        src = np.sort(rng.choice(n, k, replace=False)).tolist()
+        nd = int(rng.integers(1, 4))
+        dists = sorted(rng.choice(SERVE_DISTANCES, nd, replace=False).tolist())
+        reqs.append(("simplest" if simplest else "shortest", ns, dict(distances=[int(d) for d in dists],
+                                                                       source_indices=src)))
+    return primal + dual, reqs
+
+
+def _fifo_latencies(arrivals, service):
+    """Latency of each request in a single first-come first-served server."""
+    free, lat = 0.0, []
+    for a, s in zip(arrivals, service):
+        start = max(a, free)
+        free = start + s
+        lat.append(free - a)
+    return lat
+
+
+def run_serve(args) -> int:
+    """Continuous-batching server vs the plain one-call-per-request path on the same request stream:
+    saturation throughput (all requests submitted at once) and p50/p99 latency under Poisson
+    arrivals at a fraction of the served saturation rate."""
+    import numpy as np
+
+    from cityseer.serving import CentralityServer
+
+    log("building serving workload")
+    graphs, reqs = serve_workload(args.requests, args.seed)
+    total_sources = sum(len(kw["source_indices"]) for _, _, kw in reqs)
+    for ns in graphs:  # device-resident graphs before any timing
+        ns.centrality_shortest(distances=[400], source_indices=[0])
+
+    def plain_call(kind, ns, kw):
+        return ns.centrality_shortest(**kw) if kind == "shortest" else ns.centrality_simplest(**kw)
+
+    log("plain: one call per request, in order")
+    service = []
+    for kind, ns, kw in reqs:
+        t0 = time.perf_counter()
+        plain_call(kind, ns, kw)
+        service.append(time.perf_counter() - t0)
+    plain_s = float(np.sum(service))
+
+    with CentralityServer() as srv:
+        log("served: all requests at once (saturation)")
+        t0 = time.perf_counter()
+        handles = [(srv.submit_shortest if k == "shortest" else srv.submit_simplest)(ns, **kw) for k, ns, kw in reqs]
+        for h in handles:
+            h.wait()
+        sat_s = time.perf_counter() - t0
+        sat_rate = len(reqs) / sat_s
+        lam = args.load * sat_rate
+        rng = np.random.default_rng(args.seed + 1)
+        gaps = rng.exponential(1.0 / lam, len(reqs))
+        arrivals = np.cumsum(gaps) - gaps[0]
+        log(f"served: Poisson arrivals at {lam:.2f} requests/s ({args.load:.0%} of saturation)")
+        handles = []
+        t0 = time.perf_counter()
+        for (k, ns, kw), at in zip(reqs, arrivals):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            handles.append((srv.submit_shortest if k == "shortest" else srv.submit_simplest)(ns, **kw))
+        infos = [h.wait() for h in handles]
+        stream_s = time.perf_counter() - t0
+    lat = np.array([i["latency_ms"] for i in infos])
+    plain_lat = np.array(_fifo_latencies(arrivals, service)) * 1000.0
+    launches = [i["launches"] for i in infos]
+    value = total_sources / sat_s
+    plain_value = total_sources / plain_s
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "source searches/sec",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(sat_s * 1000.0, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 accumulators)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(reqs)} requests over {len(graphs)} graphs decomposed to <= 20 m edges "
+                        f"({min(g.node_count() for g in graphs)}-{max(g.node_count() for g in graphs)} nodes), "
+                        f"80% shortest / 20% simplest, 1-30% of sources, 1-3 distances from {SERVE_DISTANCES} m; "
+                        "continuous-batching server vs one plain call per request",
+            "requests": len(reqs),
+            "sources": total_sources,
+            "mode": "serve",
+            "plain_value": round(plain_value, 2),
+            "served_over_plain": round(value / plain_value, 3),
+            "arrival_rate_per_s": round(lam, 3),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2),
+            "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "plain_p50_ms": round(float(np.percentile(plain_lat, 50)), 2),
+            "plain_p99_ms": round(float(np.percentile(plain_lat, 99)), 2),
+            "stream_s": round(stream_s, 3),
+            "mean_launches_per_request": round(float(np.mean(launches)), 2),
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -375,7 +513,10 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["plain", "simplest", "segment", "batch"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "simplest", "segment", "batch", "serve"], default="plain")
+    ap.add_argument("--requests", type=int, default=40, help="serve mode: requests in the stream")
+    ap.add_argument("--load", type=float, default=0.6,
+                    help="serve mode: Poisson arrival rate as a fraction of the served saturation rate")
     ap.add_argument("--graphs", type=int, default=64, help="batch mode: number of graphs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (3 times), to scale HBM counters")
@@ -403,6 +544,8 @@ def main() -> int:
         return run_variant(args)
     if args.mode == "batch":
         return run_batch(args)
+    if args.mode == "serve":
+        return run_serve(args)
 
     log("building workload")
     lat, ns = build_workload(args.side, args.diag, args.seed)

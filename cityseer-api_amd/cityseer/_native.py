@@ -74,6 +74,27 @@ class CsTuning(ctypes.Structure):
     ]
 
 
+class CsServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("max_active", ctypes.c_uint32),
+        ("slice_ms", ctypes.c_double),
+    ]
+
+
+class CsRequestInfo(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("queued_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+        ("sources", ctypes.c_uint64),
+        ("launches", ctypes.c_uint32),
+    ]
+
+
+CS_REQ_SHORTEST = 0
+CS_REQ_SIMPLEST = 1
+
 # every symbol the header declares, with (restype, argtypes)
 _SIGNATURES = {
     "cs_last_error": (ctypes.c_char_p, []),
@@ -133,6 +154,14 @@ _SIGNATURES = {
     "cs_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CsTuning)]),
     "cs_batch_run": (ctypes.c_int, [ctypes.c_void_p]),
     "cs_batch_results": (ctypes.c_int, [ctypes.c_void_p]),
+    "cs_server_create": (ctypes.c_int, [ctypes.POINTER(CsServerConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    "cs_server_destroy": (None, [ctypes.c_void_p]),
+    "cs_server_submit": (
+        ctypes.c_int64, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(CsParams), _f64p, _u32p]
+    ),
+    "cs_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(CsRequestInfo)]),
+    "cs_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(CsRequestInfo)]),
+    "cs_server_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
 }
 
 _lib = None

@@ -76,7 +76,7 @@ class CentralityBatch:
             _native.ptr(reach, ctypes.c_uint32),
         )
         if rc < 0:
-            _native.check(rc)
+            _native.check(-rc)
         self._jobs.append(dict(ns=ns, out=out, reach=reach, dists=dists, plan=plan, t=t))
         return rc
 

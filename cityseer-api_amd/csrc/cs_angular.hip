@@ -274,6 +274,64 @@ __device__ bool angular_source(const AngArgs &a, const LaneWs &L, uint32_t src, 
   return true;
 }
 
+// Phase 2 of the angular fold (centrality.rs:705-764): with a tolerance above
+// the internal tie band, predecessors and sigma are rebuilt from the final
+// routes, visiting settled states in settle order: a successor state settled
+// later gains u as a predecessor when route_u + angle + 1e-6 length <=
+// route_v * (1 + tol). Settle positions live in the delta slots, which the
+// betweenness pass clears before use.
+__device__ bool angular_phase2(const AngArgs &a, const LaneWs &L, uint32_t epoch, uint32_t n_order, uint32_t &fail) {
+  char *w = L.w;
+  const uint32_t lane = L.lane;
+  for (uint32_t q = 0; q < n_order; ++q) {
+    const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
+    *il<double>(w, a.o_delta, ss, lane) = (double)q;
+    *il<uint32_t>(w, a.o_np, ss, lane) = 0;
+    *il<double>(w, a.o_sigma, ss, lane) = 0.0;
+  }
+  // the source's two states are the first two slots
+  *il<double>(w, a.o_sigma, 0, lane) = 1.0;
+  *il<double>(w, a.o_sigma, 1, lane) = 1.0;
+  const float band = 1.0f + a.tolerance;
+  for (uint32_t pos = 0; pos < n_order; ++pos) {
+    const uint32_t uss = *il<uint32_t>(w, a.o_order, pos, lane);
+    const uint32_t us = *il<uint32_t>(w, a.o_state, uss, lane) & 0x7fffffffu;
+    const uint32_t u = us >> 1, entry = us & 1u;
+    const float u_route = *il<float>(w, a.o_route, uss, lane);
+    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane);
+    for (uint32_t j = a.off[u]; j < a.off[u + 1]; ++j) {
+      const uint4 e = a.adj[j];
+      if (((e.x >> 30) & 1u) != 1u - entry) continue;
+      const uint32_t v = e.x & 0x3fffffffu;
+      const uint32_t vs = 2 * v + (e.x >> 31);
+      const uint64_t sk = L.skey[vs];
+      if ((uint32_t)(sk >> 32) != epoch) continue;
+      const uint32_t vss = (uint32_t)sk;
+      if (!(*il<uint32_t>(w, a.o_state, vss, lane) & 0x80000000u)) continue;  // never settled
+      if (*il<double>(w, a.o_delta, vss, lane) <= (double)pos) continue;       // settled before u
+      float cr = u_route + __uint_as_float(e.z);
+      cr = cr + __uint_as_float(e.w);
+      if (!(cr <= *il<float>(w, a.o_route, vss, lane) * band)) continue;
+      uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane);
+      uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)vss * a.maxp) * kWave + lane;
+      const uint32_t k = *np;
+      bool has = false;
+      for (uint32_t q = 0; q < k; ++q)
+        if (preds[(size_t)q * kWave] == uss) has = true;
+      if (has) continue;
+      if (k >= a.maxp) {
+        fail |= kFailPreds;
+        return false;
+      }
+      preds[(size_t)k * kWave] = uss;
+      *np = k + 1;
+      double *sg = il<double>(w, a.o_sigma, vss, lane);
+      *sg = *sg + u_sigma;
+    }
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
   const uint32_t lane = __lane_id();
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;  // global lane
@@ -293,7 +351,9 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     const float wt = a.src_wt[si];
     ++epoch;
     uint32_t n_states, n_nodes, n_order, fail = 0;
-    if (!angular_source(a, L, src, epoch, n_states, n_nodes, n_order, fail)) {
+    bool ok = angular_source(a, L, src, epoch, n_states, n_nodes, n_order, fail);
+    if (ok && a.tolerance > kTieEps && (a.flags & 2u)) ok = angular_phase2(a, L, epoch, n_order, fail);
+    if (!ok) {
       const uint32_t q = atomicAdd(a.n_fail, 1u);
       a.fail_list[q] = si;
       atomicOr(a.fail_flags, fail);

@@ -1,0 +1,43 @@
+// cs_api.hpp -- host helpers shared by the C ABI translation units
+// (cs_host.cpp: graphs, plain calls and batches; cs_serve.cpp: the request
+// server): parameter checks, source preparation and result unpacking.
+#pragma once
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "cs_internal.hpp"
+
+struct cs_graph {
+  cs::Graph g;
+};
+
+namespace cs {
+
+// Current HIP device; raises CS_EDEVICE without a GPU.
+int current_device();
+
+// Runs f, turning exceptions into a status code and the thread's last error.
+int guarded_call(const std::function<void()> &f);
+
+// Argument checks shared by the centrality entry points (the reference's
+// ValueError messages, centrality.rs:895-959, common.rs:98-270).
+void check_params(const Graph &g, const cs_params *p);
+void run_shortest_checks(const Graph &g, const cs_params *p);
+void check_simplest_graph(const Graph &g);
+CallSpec make_spec(const cs_params *p);
+
+// Sources kept after NaN-weight filtering, with per-source weights and the
+// cycles weight wt / node_weight (centrality.rs:1711-1730).
+struct SourceSet {
+  std::vector<uint32_t> src;
+  std::vector<float> wt, cyc;
+};
+SourceSet source_set(const Graph &g, const cs_params *p);
+
+// acc [N][M][stride] (device, node-major) -> out [M][T][N] (host)
+void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out);
+void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s);
+
+}  // namespace cs

@@ -6,10 +6,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <unordered_map>
 
-#include "cs_internal.hpp"
+#include "cs_api.hpp"
 
 namespace cs {
 
@@ -294,6 +295,8 @@ void distances_from_seconds(const uint32_t *s, uint32_t n, float speed, uint32_t
   }
 }
 
+}  // namespace
+
 int current_device() {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
@@ -324,6 +327,8 @@ int guarded(F &&f) {
   }
 }
 
+int guarded_call(const std::function<void()> &f) { return guarded(f); }
+
 struct DevBuf {
   void *p = nullptr;
   explicit DevBuf(size_t bytes) { hip_check(hipMalloc(&p, std::max<size_t>(bytes, 4)), "hipMalloc"); }
@@ -343,6 +348,8 @@ void check_params(const Graph &g, const cs_params *p) {
          "Either or both closeness and betweenness flags is required, but both parameters are False.");
   if (!(p->speed_m_s > 0.0f) || !std::isfinite(p->speed_m_s))
     fail(CS_EINVAL, "speed_m_s must be finite and positive");
+  if (!(p->tolerance >= 0.0f && p->tolerance <= 1.0f))
+    fail(CS_EINVAL, "Tolerance must be between 0 and 100 (percent), got " + std::to_string(p->tolerance * 100.0f));
   for (uint32_t i = 0; i < p->n_sources; ++i)
     if (p->sources[i] >= g.n)
       fail(CS_EINVAL, "node index " + std::to_string(p->sources[i]) + " does not exist in the graph");
@@ -369,12 +376,6 @@ CallSpec make_spec(const cs_params *p) {
   return c;
 }
 
-// Sources kept after NaN-weight filtering, with per-source weights and the
-// cycles weight wt / node_weight (centrality.rs:1711-1730).
-struct SourceSet {
-  std::vector<uint32_t> src;
-  std::vector<float> wt, cyc;
-};
 
 SourceSet source_set(const Graph &g, const cs_params *p) {
   SourceSet s;
@@ -409,14 +410,19 @@ void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, d
       for (uint32_t v = 0; v < N; ++v) out[((size_t)m * T + i) * N + v] *= s;
 }
 
+void check_simplest_graph(const Graph &g) {
+  if (!g.is_dual)
+    fail(CS_EINVAL,
+         "centrality_simplest requires a dual graph for angular analysis. Convert the graph with "
+         "cityseer.tools.graphs.nx_to_dual(...) before ingesting it into NetworkStructure.");
+  if (!g.slots_ok) fail(CS_EINVAL, g.slots_err);
+}
+
 void run_shortest_checks(const Graph &g, const cs_params *p) {
-  if (p->tolerance > kTieEps)
-    fail(CS_EINVAL, "tolerance above the internal 0.01% tie band is not supported by the GPU search yet");
   if (p->compute_betweenness && g.max_out_deg > 64)
     fail(CS_EINVAL, "nodes with more than 64 out-edges are not supported by the GPU search");
 }
 
-}  // namespace
 
 }  // namespace cs
 
@@ -474,9 +480,6 @@ struct cs_batch {
   void build();
 };
 
-struct cs_graph {
-  Graph g;
-};
 
 extern "C" {
 
@@ -727,12 +730,8 @@ int cs_centrality_simplest(cs_graph *h, const cs_params *p, double *out, uint32_
   int rc = guarded([&] {
     if (!h || !out) fail(CS_EINVAL, "graph and out must not be NULL");
     Graph &g = h->g;
-    if (!g.is_dual)
-      fail(CS_EINVAL,
-           "centrality_simplest requires a dual graph for angular analysis. Convert the graph with "
-           "cityseer.tools.graphs.nx_to_dual(...) before ingesting it into NetworkStructure.");
+    check_simplest_graph(g);
     check_params(g, p);
-    if (!g.slots_ok) fail(CS_EINVAL, g.slots_err);
     const int dev = current_device();
     const CallSpec spec = make_spec(p);
     const SourceSet ss = source_set(g, p);
@@ -924,8 +923,8 @@ int cs_brandes_shortest(cs_graph *h, uint32_t src, uint32_t max_seconds, float s
                         uint32_t max_preds) {
   return guarded([&] {
     if (!h) fail(CS_EINVAL, "graph must not be NULL");
-    if (tolerance > kTieEps)
-      fail(CS_EINVAL, "tolerance above the internal 0.01% tie band is not supported by the GPU search yet");
+    if (!(tolerance >= 0.0f && tolerance <= 1.0f))
+      fail(CS_EINVAL, "Tolerance must be between 0 and 100 (percent), got " + std::to_string(tolerance * 100.0f));
     Single s = run_single(h, src, max_seconds, speed, tolerance, 1, max_preds);
     const uint32_t N = h->g.n;
     for (uint32_t v = 0; v < N; ++v) {
@@ -1098,7 +1097,7 @@ int cs_batch_add_shortest(cs_batch *b, cs_graph *h, const cs_params *p, double *
     b->NT = std::max(b->NT, p->n_thresholds);
     idx = (int)b->jobs.size() - 1;
   });
-  return rc == CS_OK ? idx : rc;
+  return rc == CS_OK ? idx : -rc;
 }
 
 int cs_batch_set_tuning(cs_batch *b, const cs_tuning *t) {

@@ -131,7 +131,8 @@ void set_default_tuning(const Tuning &t);
 Tuning default_tuning();
 
 // One (graph, request) pair of a batched shortest-path launch. d_acc is
-// [N][7][NT] for the launch's NT (>= cs.T); thresholds beyond cs.T stay zero.
+// [N][7][tstride] with tstride >= cs.T (0: the launch's NT); thresholds beyond
+// cs.T stay zero.
 struct JobSpec {
   Graph *g;
   DevGraph *dg;
@@ -145,10 +146,12 @@ struct JobSpec {
   double *d_acc;
   uint32_t *d_reach;
   float delta_scale;
+  uint32_t tstride = 0;
 };
-// All sources of all jobs in one persistent launch.
+// All sources of all jobs in one persistent launch, or only `items`
+// ({job, source position}) when given.
 void launch_shortest_batch(const std::vector<JobSpec> &jobs, uint32_t NT, const Tuning &tune,
-                           hipStream_t stream, cs_stats &st);
+                           hipStream_t stream, cs_stats &st, const std::vector<uint2> *items = nullptr);
 
 // A prepared batch of shortest-path jobs replayed as one captured hipGraph
 // (cs_search.hip). The zero regions (64-bit words) are cleared inside the

@@ -58,13 +58,14 @@ struct Job {
   const uint32_t *sources;
   const float *src_wt;
   const float *src_cyc;
-  double *acc;  // [N][M][NT] (threshold stride = the launch's NT)
+  double *acc;  // [N][M][tstride]
   uint32_t *reach;
   float maxsec, speed, inv_delta;
+  float tol;  // tie tolerance; above the internal band, Phase 2 rebuilds predecessors
   uint32_t nbk, maxb;
   uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
   uint32_t n_sources;
-  uint32_t pad;
+  uint32_t tstride;  // threshold stride of acc (>= the job's threshold count)
   float dist[kMaxT];  // thresholds beyond the job's own count hold -1 (never within)
   float beta[kMaxT];
 };
@@ -79,9 +80,11 @@ struct SearchArgs {
   uint32_t n_items;
   uint32_t bcap, cap, nbk_lds;
   unsigned long long *stats;  // reached, edges, within
-  char *ws;
+  uint64_t *keys;   // [waves][ncap] epoch-tagged keys
+  size_t ncap;
+  char *ws;         // [waves][ws_stride] scratch
   size_t ws_stride;
-  size_t o_keyw, o_slotw, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd;
+  size_t o_ninfo, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd;
   uint32_t *epochs;
   uint32_t *next;            // [kXcd] per-XCD item counters
   uint32_t xbeg[kXcd + 1];   // XCD x owns item indices [xbeg[x], xbeg[x+1])
@@ -178,7 +181,8 @@ __device__ __forceinline__ bool key_less(float sa, uint32_t a, float sb, uint32_
 //        trees:   predecessor slot | row position of the tree edge << 32
 struct WaveWs {
   uint64_t *keyw;   // [N] epoch-tagged keys
-  uint32_t *slotw;  // [N] slot of a settled node
+  uint64_t *ninfo;  // [N][2] of a settled node: {slot, payload}, payload = sigma bits (Brandes)
+                    // or origin-segment length bits (trees)
   uint64_t *rec;    // [cap][2]
   double *sig;      // [cap] sigma (Brandes) / unused
   uint64_t *pm;     // [cap] predecessor row mask (Brandes) / {last len, origin len} (trees)
@@ -191,8 +195,8 @@ struct WaveWs {
 __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uint32_t *ring) {
   char *base = a.ws + (size_t)wid * a.ws_stride;
   WaveWs w;
-  w.keyw = reinterpret_cast<uint64_t *>(base + a.o_keyw);
-  w.slotw = reinterpret_cast<uint32_t *>(base + a.o_slotw);
+  w.keyw = a.keys + (size_t)wid * a.ncap;
+  w.ninfo = reinterpret_cast<uint64_t *>(base + a.o_ninfo);
   w.rec = reinterpret_cast<uint64_t *>(base + a.o_rec);
   w.sig = reinterpret_cast<double *>(base + a.o_sig);
   w.pm = reinterpret_cast<uint64_t *>(base + a.o_pm);
@@ -281,7 +285,7 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
       const uint32_t jb = __builtin_ctzll(m);
       if ((j.fwd_adj[r0 + jb].x & kNodeMask) == pu) contains = true;
     }
-    const double sg = ldd(&w.sig[ld32(&w.slotw[pu])]);
+    const double sg = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)pu + 1]));
     fold_apply(f, pcs, sg, pj, contains);
     have_last = true;
     lsu = psu;
@@ -313,7 +317,9 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
   const uint32_t lane = lane_id();
   const float speed = j.speed;
   const bool cycles = (MODE == kBrandes) && (j.flags & 1u);
-  const bool fold = (MODE != kBrandes) || (j.flags & 2u);
+  // with Phase 2 ahead, the Phase 1 fold's predecessors would be discarded
+  const bool phase2 = kMulti && j.tol > kTieEps;
+  const bool fold = ((MODE != kBrandes) || (j.flags & 2u)) && !phase2;
   if (lane == 0) {
     st64(&w.keyw[c.src], pack_key(c.tag, 0.0f));
     st64(reinterpret_cast<uint64_t *>(&w.bkt[0]), (uint64_t)c.src);
@@ -372,34 +378,40 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
       if (valid) {
         const float secs_v = __uint_as_float(csb);
         const bool is_src = (s == 0);
-        st32(&w.slotw[v], s);
         const uint32_t f0 = rw.x, nf = rw.y - rw.x, r0 = rw.z, nr = rw.w - rw.z;
         st_edges += nr;
         // fold state (multi-predecessor: near candidates; trees: lexicographic min)
         const double thr_far = kMulti ? (double)secs_v * far_factor(nf) : 0.0;
         uint32_t n_near = 0, near_j = 0, near_slot = 0;
+        uint64_t near_pay = 0, bpay = 0;
         float near_cs = INFINITY;
         float bcs = INFINITY, bsu = INFINITY;
         uint32_t bu = 0xffffffffu, bj = 0xffffffffu, bslot = 0;
         const uint32_t nmax = nf > nr ? nf : nr;
         for (uint32_t cb = 0; cb < nmax; cb += kChunk) {
           uint2 fe[kChunk], re[kChunk];
-  #pragma unroll
+#pragma unroll
           for (int q = 0; q < kChunk; ++q) {
             fe[q] = (cb + q < nf) ? j.fwd_adj[f0 + cb + q] : make_uint2(v, 0u);
             re[q] = (cb + q < nr) ? j.rev_adj[r0 + cb + q] : make_uint2(v, 0u);
           }
           // issue every key load and relaxation before consuming any result
-          uint64_t fk[kChunk], rk[kChunk];
+          // (slot and payload are loaded with the keys: a later dependent load
+          // would add a round trip per bucket, which costs more than the bytes)
+          uint64_t fk[kChunk], rk[kChunk], fp[kChunk];
           uint32_t fs[kChunk];
-  #pragma unroll
+#pragma unroll
           for (int q = 0; q < kChunk; ++q) {
             const uint32_t u = fe[q].x & kNodeMask;
             fk[q] = 0;
             fs[q] = 0;
+            fp[q] = 0;
             if (!is_src && u != v && (fold || cycles)) {
               fk[q] = ld64(&w.keyw[u]);
-              if (fold) fs[q] = ld32(&w.slotw[u]);
+              if (fold) {
+                fs[q] = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
+                fp[q] = ld64(&w.ninfo[2 * (size_t)u + 1]);
+              }
             }
             const uint32_t nb = re[q].x & kNodeMask;
             const float cs = secs_v + __uint_as_float(re[q].y);
@@ -412,7 +424,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             }
           }
           // fold candidates v -> u (u settled in an earlier bucket)
-  #pragma unroll
+#pragma unroll
           for (int q = 0; q < kChunk; ++q) {
             const uint32_t u = fe[q].x & kNodeMask;
             if (is_src || u == v) continue;
@@ -430,6 +442,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
                   near_j = jr;
                   near_cs = cs;
                   near_slot = fs[q];
+                  near_pay = fp[q];
                 }
                 ++n_near;
               }
@@ -443,12 +456,13 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
                 bu = u;
                 bj = jr;
                 bslot = fs[q];
+                bpay = fp[q];
               }
             }
           }
           // relaxation results nb -> v; the old key also tells whether nb settled
           // before v (canonical edges count toward the circuit rank)
-  #pragma unroll
+#pragma unroll
           for (int q = 0; q < kChunk; ++q) {
             const uint32_t nb = re[q].x & kNodeMask;
             if (nb == v) continue;
@@ -485,7 +499,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           if (!is_src && fold) {
             if (n_near == 1) {
               f.pmask = 1ull << near_j;
-              f.sigma = ldd(&w.sig[near_slot]);
+              f.sigma = __longlong_as_double((long long)near_pay);
               pslot0 = near_slot;
               npred = 1;
               if (near_cs != secs_v) lfail |= kFailInternal;
@@ -495,16 +509,18 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             } else {
               npred = (uint32_t)__popcll(f.pmask);
               const uint32_t u0 = j.fwd_adj[f0 + __builtin_ctzll(f.pmask)].x & kNodeMask;
-              pslot0 = ld32(&w.slotw[u0]);
+              pslot0 = (uint32_t)ld64(&w.ninfo[2 * (size_t)u0]);
             }
           }
+          st64(&w.ninfo[2 * (size_t)v], s);
+          std_(reinterpret_cast<double *>(&w.ninfo[2 * (size_t)v + 1]), f.sigma);
           st64(&w.rec[2 * (size_t)s], ((uint64_t)csb << 32) | v);
           st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(npred | (el ? 0x100u : 0u)) << 32) | pslot0);
           std_(&w.sig[s], f.sigma);
           st64(&w.pm[s], f.pmask);
           if constexpr (MODE == kBrandes) {
             if (j.flags & 2u) {
-  #pragma unroll
+#pragma unroll
               for (int t = 0; t < 2 * NT; ++t) std_(&w.acc[(size_t)s * 2 * NT + t], 0.0);
             }
           }
@@ -518,14 +534,16 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             } else {
               pslot = bslot;
               llen = j.fwd_twlen[f0 + bj];
-              olen = (pslot == 0) ? llen : __uint_as_float((uint32_t)(ld64(&w.pm[pslot]) >> 32));
+              olen = (pslot == 0) ? llen : __uint_as_float((uint32_t)bpay);
             }
           }
+          st64(&w.ninfo[2 * (size_t)v], s);
+          st64(&w.ninfo[2 * (size_t)v + 1], __float_as_uint(olen));
           st64(&w.rec[2 * (size_t)s], ((uint64_t)csb << 32) | v);
           st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)bj << 32) | pslot);
           st64(&w.pm[s], ((uint64_t)__float_as_uint(olen) << 32) | __float_as_uint(llen));
           if constexpr (MODE == kSegment) {
-  #pragma unroll
+#pragma unroll
             for (int t = 0; t < NT; ++t) std_(&w.acc[(size_t)s * NT + t], 0.0);
           }
         }
@@ -578,11 +596,120 @@ __device__ __forceinline__ uint32_t next_item(const SearchArgs &a, uint32_t x) {
   return uniform(si);
 }
 
+// Phase 2 of the multi-predecessor fold (centrality.rs:1443-1482): with a
+// tolerance above the internal tie band, predecessors and sigma are rebuilt
+// from the final distances. u is a predecessor of v when u settles before v
+// and secs_u + es(v->u) <= secs_v * (1 + tol) (one entry per distinct u);
+// sigma_v sums sigma_u in u's settle order. A predecessor can settle in v's
+// own bucket, so each bucket resolves in rounds: a node is ready once all its
+// predecessors are. Record bit 0x200 of the metadata word marks a resolved
+// node. Returns false if a bucket fails to resolve (cannot happen: every
+// predecessor settles strictly earlier).
+constexpr uint32_t kP2Done = 0x200u;
+
+__device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
+  const uint32_t lane = lane_id();
+  const float band = 1.0f + j.tol;
+  for (uint32_t b = 0; b < c.nb; ++b) {
+    const uint32_t s0 = uniform(ld32(&w.bnd[b])), s1 = uniform(ld32(&w.bnd[b + 1]));
+    // predecessor masks over the out-row positions of each node
+    for (uint32_t s = s0 + lane; s < s1; s += kWave) {
+      const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+      const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
+      const uint32_t v = (uint32_t)lo;
+      const float secs_v = __uint_as_float((uint32_t)(lo >> 32));
+      uint32_t meta = (uint32_t)(hi >> 32) & 0x100u;  // keep the eligibility bit
+      uint64_t pm = 0;
+      if (s == 0) {
+        meta |= kP2Done;
+        std_(&w.sig[0], 1.0);
+      } else {
+        const uint4 rw = j.rows[v];
+        const float lim = secs_v * band;
+        for (uint32_t p = rw.x; p < rw.y; ++p) {
+          const uint2 e = j.fwd_adj[p];
+          const uint32_t u = e.x & kNodeMask;
+          if (u == v) continue;
+          const uint64_t ku = ld64(&w.keyw[u]);
+          if (!key_current(ku, c.tag)) continue;
+          const float su = key_secs(ku);
+          if (!key_less(su, u, secs_v, v)) continue;
+          const float ps = su + __uint_as_float(e.y);
+          if (!(ps <= lim)) continue;
+          bool dup = false;
+          for (uint64_t mm = pm; mm; mm &= mm - 1)
+            if ((j.fwd_adj[rw.x + __builtin_ctzll(mm)].x & kNodeMask) == u) dup = true;
+          if (!dup) pm |= 1ull << (p - rw.x);
+        }
+      }
+      st64(&w.pm[s], pm);
+      st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)meta << 32) | (uint32_t)hi);
+    }
+    wave_fence();
+    for (uint32_t round = 0;; ++round) {
+      uint32_t pending = 0;
+      for (uint32_t s = s0 + lane; s < s1; s += kWave) {
+        const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
+        const uint32_t meta = (uint32_t)(hi >> 32);
+        if (meta & kP2Done) continue;
+        const uint32_t v = (uint32_t)ld64(&w.rec[2 * (size_t)s]);
+        const uint64_t pm = ld64(&w.pm[s]);
+        const uint32_t f0 = j.rows[v].x;
+        bool ready = true;
+        for (uint64_t mm = pm; mm; mm &= mm - 1) {
+          const uint32_t u = j.fwd_adj[f0 + __builtin_ctzll(mm)].x & kNodeMask;
+          const uint32_t su = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
+          if (!((uint32_t)(ld64(&w.rec[2 * (size_t)su + 1]) >> 32) & kP2Done)) ready = false;
+        }
+        if (!ready) {
+          ++pending;
+          continue;
+        }
+        // sigma over the predecessors in their settle order (secs, node index)
+        double sigma = 0.0;
+        uint32_t pslot0 = 0;
+        bool have_last = false;
+        float lsu = 0.0f;
+        uint32_t lu = 0;
+        const uint32_t np = (uint32_t)__popcll(pm);
+        for (uint32_t step = 0; step < np; ++step) {
+          bool found = false;
+          float bsu = 0.0f;
+          uint32_t bu = 0, bslot = 0;
+          for (uint64_t mm = pm; mm; mm &= mm - 1) {
+            const uint32_t u = j.fwd_adj[f0 + __builtin_ctzll(mm)].x & kNodeMask;
+            const float su = key_secs(ld64(&w.keyw[u]));
+            if (have_last && !key_less(lsu, lu, su, u)) continue;
+            if (!found || key_less(su, u, bsu, bu)) {
+              found = true;
+              bsu = su;
+              bu = u;
+              bslot = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
+            }
+          }
+          sigma += ldd(&w.sig[bslot]);
+          have_last = true;
+          lsu = bsu;
+          lu = bu;
+        }
+        if (pm) pslot0 = (uint32_t)ld64(&w.ninfo[2 * (size_t)(j.fwd_adj[f0 + __builtin_ctzll(pm)].x & kNodeMask)]);
+        std_(&w.sig[s], sigma);
+        st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np) << 32) | pslot0);
+      }
+      pending = uniform(wave_sum(pending));
+      wave_fence();
+      if (pending == 0) break;
+      if (round > s1 - s0) return false;
+    }
+  }
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // Persistent kernel: each wave loops over sources taken from the counters.
 // ---------------------------------------------------------------------------
 template <int MODE, int NT>
-__global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_search(SearchArgs a) {
   extern __shared__ uint32_t lds_rings[];
   const uint32_t lane = lane_id();
   const uint32_t wib = uniform(threadIdx.x / kWave);
@@ -612,7 +739,14 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
     uint32_t nband[NT], eband[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) nband[t] = eband[t] = 0;
-    if (!search_source<MODE, NT>(a, j, w, c, nband, eband, st_edges)) {
+    bool ok = search_source<MODE, NT>(a, j, w, c, nband, eband, st_edges);
+    if constexpr (MODE == kBrandes || MODE == kBrandesExport) {
+      if (ok && j.tol > kTieEps && (MODE == kBrandesExport || (j.flags & 2u))) {
+        ok = phase2_rebuild(j, w, c);
+        if (!ok) c.fail |= kFailInternal;
+      }
+    }
+    if (!ok) {
       if (lane == 0) {
         const uint32_t q = atomicAdd(a.n_fail, 1u);
         a.fail_list[q] = item;
@@ -642,17 +776,18 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
         const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
         const uint32_t v = (uint32_t)lo;
         const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
-        double *accv = j.acc + (size_t)v * M * NT;
+        const uint32_t ts = j.tstride;
+        double *accv = j.acc + (size_t)v * M * ts;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           if (route <= j.dist[t]) {
             ++within[t];
             if (j.flags & 1u) {
-              gadd_f64(&accv[0 * NT + t], (double)wt);
-              gadd_f64(&accv[1 * NT + t], (double)(route * wt));
-              gadd_f64(&accv[2 * NT + t], (double)(rank[t] * cyc));
-              gadd_f64(&accv[3 * NT + t], (double)((1.0f / route) * wt));
-              gadd_f64(&accv[4 * NT + t], (double)(expf(-j.beta[t] * route) * wt));
+              gadd_f64(&accv[0 * ts + t], (double)wt);
+              gadd_f64(&accv[1 * ts + t], (double)(route * wt));
+              gadd_f64(&accv[2 * ts + t], (double)(rank[t] * cyc));
+              gadd_f64(&accv[3 * ts + t], (double)((1.0f / route) * wt));
+              gadd_f64(&accv[4 * ts + t], (double)(expf(-j.beta[t] * route) * wt));
             }
           }
         }
@@ -686,7 +821,8 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             const uint32_t npred = meta & 0xffu;
             const bool is_src = (s == 0);
             const double pc = is_src ? 0.0 : ((meta & 0x100u) ? 0.5 : 1.0);
-            double *accv = j.acc + (size_t)v * M * NT;
+            const uint32_t ts = j.tstride;
+            double *accv = j.acc + (size_t)v * M * ts;
             double dep[NT], dep_b[NT], seed_b[NT];
             bool any = false;
 #pragma unroll
@@ -701,15 +837,13 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             }
             if (!any) continue;
             if (npred == 1) {
-              const double sigma_u = ldd(&w.sig[pslot0]);
-              if (sigma_u != 0.0) {
-                const double fct = sigma_u / sigma_w;
+              // a single predecessor has sigma_u == sigma_w (sigma is the sum over
+              // the predecessors), so sigma_u / sigma_w is exactly 1
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                  if (dep[t] != 0.0 || dep_b[t] != 0.0) {
-                    wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + t], fct * dep[t]);
-                    wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + NT + t], fct * dep_b[t]);
-                  }
+              for (int t = 0; t < NT; ++t) {
+                if (dep[t] != 0.0 || dep_b[t] != 0.0) {
+                  wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + t], dep[t]);
+                  wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + NT + t], dep_b[t]);
                 }
               }
             } else if (npred > 1) {
@@ -718,7 +852,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
               for (uint64_t mm = pm; mm; mm &= mm - 1) {
                 const uint32_t jb = __builtin_ctzll(mm);
                 const uint32_t u = j.fwd_adj[f0 + jb].x & kNodeMask;
-                const uint32_t su = ld32(&w.slotw[u]);
+                const uint32_t su = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
                 const double sigma_u = ldd(&w.sig[su]);
                 if (sigma_u == 0.0) continue;
                 const double fct = sigma_u / sigma_w;
@@ -737,8 +871,8 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
               if (dep[t] == 0.0 && dep_b[t] == 0.0) continue;
               const double credit = dep[t] - pc;
               const double credit_b = dep_b[t] - seed_b[t];
-              if (credit > 0.0) gadd_f64(&accv[5 * NT + t], credit * (double)wt);
-              if (credit_b > 0.0) gadd_f64(&accv[6 * NT + t], credit_b * (double)wt);
+              if (credit > 0.0) gadd_f64(&accv[5 * ts + t], credit * (double)wt);
+              if (credit_b > 0.0) gadd_f64(&accv[6 * ts + t], credit_b * (double)wt);
             }
           }
           wave_fence();
@@ -837,10 +971,11 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
         for (int t = 0; t < NT; ++t) {
           const double t0 = wave_sum(sd[t]), t1 = wave_sum(sh[t]), t2 = wave_sum(sb[t]);
           if (lane == 0) {
-            double *accs = j.acc + (size_t)c.src * M * NT;
-            gadd_f64(&accs[0 * NT + t], t0);
-            gadd_f64(&accs[1 * NT + t], t1);
-            gadd_f64(&accs[2 * NT + t], t2);
+            const uint32_t ts = j.tstride;
+            double *accs = j.acc + (size_t)c.src * M * ts;
+            gadd_f64(&accs[0 * ts + t], t0);
+            gadd_f64(&accs[1 * ts + t], t1);
+            gadd_f64(&accs[2 * ts + t], t2);
           }
         }
       }
@@ -867,7 +1002,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             const float o_1 = min_span, o_2 = min_span + olen, l_1 = min_span, l_2 = min_span + llen;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-              if (below[t] > 0.0) gadd_f64(&j.acc[(size_t)v * M * NT + 3 * NT + t], below[t]);
+              if (below[t] > 0.0) gadd_f64(&j.acc[((size_t)v * M + 3) * j.tstride + t], below[t]);
               double own = 0.0;
               if (v > c.src) {
                 const float dist = j.dist[t];
@@ -991,15 +1126,16 @@ int blocks_per_cu(uint32_t T, uint32_t block, size_t lds) {
 struct Plan {
   uint32_t bcap, cap, maxb, nbk, waves;
   size_t stride;
-  size_t o_keyw, o_slotw, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd;
+  size_t o_ninfo, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd;
 };
 
+// Per-wave scratch layout. Everything here is written before it is read
+// within one source, so the scratch is never cleared; only the keys (a
+// separate arena) carry state across sources.
 void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   size_t o = 0;
-  p.o_keyw = o;
-  o = align256(o + 8ull * N);
-  p.o_slotw = o;
-  o = align256(o + 4ull * N);
+  p.o_ninfo = o;
+  o = align256(o + 16ull * N);
   p.o_rec = o;
   o = align256(o + 16ull * p.cap);
   p.o_sig = o;
@@ -1031,16 +1167,51 @@ float choose_delta(const Graph &g, const DevEdges &de, float maxsec) {
   return (float)delta;
 }
 
+// Grow-only device buffer.
+struct DevVec {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void *get(size_t need, const char *what) {
+    if (need > bytes) {
+      if (p) hip_check(hipFree(p), "hipFree");
+      p = nullptr;
+      bytes = 0;
+      if (hipMalloc(&p, need) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        fail(CS_ENOMEM, std::string("could not allocate ") + std::to_string(need >> 20) + " MiB of " + what);
+      }
+      bytes = need;
+    }
+    return p;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
 struct Workspace {
   int device = -1;
-  char *ws = nullptr;
-  size_t bytes = 0;
+  // epoch-tagged keys, [key_waves][ncap]: zeroed whenever (re)allocated or
+  // when the epochs would wrap; keys of earlier graphs are older epochs
+  uint64_t *keys = nullptr;
+  uint32_t ncap = 0, key_waves = 0;
   uint32_t *epochs = nullptr;
-  uint32_t waves = 0;
   uint64_t epoch_hi = 0;  // upper bound on epochs used
-  uint32_t n = 0;
-  size_t stride = 0;
+  DevVec scratch;         // [waves][stride], never cleared
+  DevVec jobs, items, ctl, stats, fail;  // launch control
   std::mutex mu;
+  void release() {
+    if (keys) (void)hipFree(keys);
+    if (epochs) (void)hipFree(epochs);
+    keys = nullptr;
+    epochs = nullptr;
+    ncap = key_waves = 0;
+    epoch_hi = 0;
+    for (DevVec *d : {&scratch, &jobs, &items, &ctl, &stats, &fail}) d->release();
+  }
 };
 
 std::mutex g_ws_mu;
@@ -1064,36 +1235,45 @@ struct WsGuard {
   ~WsGuard() { release_ws(w); }
 };
 
-// (Re)shape the workspace for `waves` waves of `stride` bytes over N nodes.
+// Device bytes the workspace still has to allocate for this shape.
+size_t ws_growth(const Workspace &w, uint32_t N, size_t stride, uint32_t waves) {
+  size_t g = 0;
+  if (N > w.ncap || waves > w.key_waves)
+    g += 8ull * std::max(N, w.ncap) * std::max(waves, w.key_waves);
+  if (stride * waves > w.scratch.bytes) g += stride * waves;
+  return g;
+}
+
+// Shape the workspace for `waves` waves over N nodes with `stride` bytes of
+// scratch each, reserving `sources` epochs.
 void prepare_ws(Workspace &w, uint32_t N, size_t stride, uint32_t waves, uint64_t sources,
                 hipStream_t stream) {
-  const size_t need = stride * waves;
-  bool reinit = false;
-  if (need > w.bytes || w.n != N || w.stride != stride || waves > w.waves) {
-    if (w.ws) hip_check(hipFree(w.ws), "hipFree");
+  bool clear = false;
+  if (N > w.ncap || waves > w.key_waves) {
+    const uint32_t ncap = std::max(N, w.ncap), kw = std::max(waves, w.key_waves);
+    if (w.keys) hip_check(hipFree(w.keys), "hipFree");
     if (w.epochs) hip_check(hipFree(w.epochs), "hipFree");
-    w.ws = nullptr;
+    w.keys = nullptr;
     w.epochs = nullptr;
-    hipError_t e = hipMalloc(&w.ws, need);
-    if (e != hipSuccess) {
+    w.ncap = w.key_waves = 0;
+    if (hipMalloc(&w.keys, 8ull * ncap * kw) != hipSuccess) {
       (void)hipGetLastError();
-      w.bytes = 0;
-      fail(CS_ENOMEM, "could not allocate " + std::to_string(need >> 20) + " MiB of search workspace");
+      w.keys = nullptr;
+      fail(CS_ENOMEM, "could not allocate " + std::to_string((8ull * ncap * kw) >> 20) + " MiB of search keys");
     }
-    hip_check(hipMalloc(&w.epochs, 4ull * waves), "hipMalloc epochs");
-    w.bytes = need;
-    w.waves = waves;
-    w.n = N;
-    w.stride = stride;
-    reinit = true;
+    hip_check(hipMalloc(&w.epochs, 4ull * kw), "hipMalloc epochs");
+    w.ncap = ncap;
+    w.key_waves = kw;
+    clear = true;
   }
-  if (w.epoch_hi + sources + 2 >= 0xfffffff0ull) reinit = true;
-  if (reinit) {
-    // zero memory: every key carries epoch 0 (never issued), every ring empty
-    hip_check(hipMemsetAsync(w.ws, 0, need, stream), "hipMemset ws");
-    hip_check(hipMemsetAsync(w.epochs, 0, 4ull * w.waves, stream), "hipMemset epochs");
+  if (w.epoch_hi + sources + 2 >= 0xfffffff0ull) clear = true;
+  if (clear) {
+    // every key carries epoch 0, which is never issued
+    hip_check(hipMemsetAsync(w.keys, 0, 8ull * w.ncap * w.key_waves, stream), "hipMemset keys");
+    hip_check(hipMemsetAsync(w.epochs, 0, 4ull * w.key_waves, stream), "hipMemset epochs");
     w.epoch_hi = 0;
   }
+  w.scratch.get(stride * waves, "search workspace");
   w.epoch_hi += sources;
 }
 
@@ -1147,6 +1327,7 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   j.acc = js.d_acc;
   j.reach = js.d_reach;
   j.maxsec = cs.maxsec;
+  j.tol = cs.tolerance;
   j.speed = cs.speed;
   j.inv_delta = 1.0f / delta;
   const uint32_t jump = (uint32_t)std::ceil((double)de.max_es / (double)delta) + 2;
@@ -1155,11 +1336,12 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   j.maxb = (uint32_t)((double)cs.maxsec * (double)j.inv_delta) + 4;
   j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
   j.n_sources = js.n_sources;
+  j.tstride = js.tstride ? js.tstride : NT;
+  if (j.tstride < cs.T) fail(CS_EINTERNAL, "accumulator stride below the threshold count");
   for (uint32_t i = 0; i < kMaxT; ++i) {
     j.dist[i] = i < cs.T ? cs.dist[i] : -1.0f;
     j.beta[i] = i < cs.T ? cs.beta[i] : 0.0f;
   }
-  (void)NT;
   return j;
 }
 
@@ -1200,11 +1382,11 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
   const int bpc = std::max(1, blocks_per_cu<MODE>(NT, block, lds));
   uint32_t resident = di.cus * (uint32_t)bpc * wpb;
   if (tune.waves_per_simd) resident = std::min<uint32_t>(resident, di.cus * 4 * tune.waves_per_simd);
-  size_t free_b = 0, total_b = 0;
-  hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
 
-  DevArr<Job> d_jobs(jobs.size());
-  hip_check(hipMemcpyAsync(d_jobs.p, jobs.data(), sizeof(Job) * jobs.size(), hipMemcpyHostToDevice, stream),
+  Workspace *wsp = acquire_ws(dev);
+  WsGuard guard{wsp};
+  Job *d_jobs = static_cast<Job *>(wsp->jobs.get(sizeof(Job) * jobs.size(), "job table"));
+  hip_check(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(Job) * jobs.size(), hipMemcpyHostToDevice, stream),
             "copy jobs");
   std::vector<uint2> items;  // explicit items only for several jobs or a retry pass
   if (explicit_items) {
@@ -1214,18 +1396,16 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     for (uint32_t jb = 0; jb < specs.size(); ++jb)
       for (uint32_t i = 0; i < specs[jb].n_sources; ++i) items.push_back(make_uint2(jb, i));
   }
-  DevArr<uint2> d_items(n_items);
+  uint2 *d_items = static_cast<uint2 *>(wsp->items.get(sizeof(uint2) * std::max<size_t>(items.size(), 1), "items"));
   if (!items.empty())
-    hip_check(hipMemcpyAsync(d_items.p, items.data(), sizeof(uint2) * items.size(), hipMemcpyHostToDevice, stream),
+    hip_check(hipMemcpyAsync(d_items, items.data(), sizeof(uint2) * items.size(), hipMemcpyHostToDevice, stream),
               "copy items");
   const bool use_items = !items.empty();
-  DevArr<uint32_t> d_ctl(16);
-  DevArr<unsigned long long> d_stats(4);
-  DevArr<uint32_t> d_fail(n_items);
-  hip_check(hipMemsetAsync(d_stats.p, 0, 32, stream), "memset stats");
+  uint32_t *d_ctl = static_cast<uint32_t *>(wsp->ctl.get(64, "control words"));
+  unsigned long long *d_stats = static_cast<unsigned long long *>(wsp->stats.get(32, "statistics"));
+  uint32_t *d_fail = static_cast<uint32_t *>(wsp->fail.get(4 * n_items, "failure list"));
+  hip_check(hipMemsetAsync(d_stats, 0, 32, stream), "memset stats");
 
-  Workspace *wsp = acquire_ws(dev);
-  WsGuard guard{wsp};
   uint32_t n_list = (uint32_t)n_items;
   bool list_items = use_items;
   hipEvent_t e0, e1;
@@ -1234,10 +1414,17 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
   double kms = 0.0;
   for (int attempt = 0; attempt < 6; ++attempt) {
     layout(p, maxN, NT, kBr);
-    // waves: every wave resident at once, bounded by items and by a quarter of free HBM
-    const uint32_t want = std::min<uint32_t>(n_list, resident);
-    const size_t budget = free_b / 4 + (wsp->bytes);
-    const uint32_t by_mem = (uint32_t)std::min<size_t>(budget / p.stride, 1u << 20);
+    // waves: every wave resident at once, bounded by items and, when the
+    // workspace must grow, by a quarter of free HBM
+    const uint32_t want = std::max<uint32_t>(wpb, (std::min<uint32_t>(n_list, resident) + wpb - 1) / wpb * wpb);
+    uint32_t by_mem = want;
+    if (ws_growth(*wsp, maxN, p.stride, want) > 0) {
+      size_t free_b = 0, total_b = 0;
+      hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+      const size_t have = wsp->scratch.bytes + 8ull * wsp->ncap * wsp->key_waves;
+      const size_t per_wave = p.stride + 8ull * maxN;
+      by_mem = (uint32_t)std::min<size_t>((free_b / 4 + have) / per_wave, want);
+    }
     if (by_mem < wpb) {
       if (p.cap > 4096) {
         p.cap = std::max<uint32_t>(4096, p.cap / 2);
@@ -1249,20 +1436,21 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     waves = std::max<uint32_t>(wpb, (waves + wpb - 1) / wpb * wpb);
     p.waves = waves;
     prepare_ws(*wsp, maxN, p.stride, waves, n_list, stream);
-    hip_check(hipMemsetAsync(d_ctl.p, 0, 64, stream), "memset ctl");
+    hip_check(hipMemsetAsync(d_ctl, 0, 64, stream), "memset ctl");
     SearchArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.jobs = d_jobs.p;
-    a.items = list_items ? d_items.p : nullptr;
+    a.jobs = d_jobs;
+    a.items = list_items ? d_items : nullptr;
     a.n_items = n_list;
     a.bcap = p.bcap;
     a.cap = p.cap;
     a.nbk_lds = p.nbk;
-    a.stats = d_stats.p;
-    a.ws = wsp->ws;
+    a.stats = d_stats;
+    a.keys = wsp->keys;
+    a.ncap = wsp->ncap;
+    a.ws = static_cast<char *>(wsp->scratch.p);
     a.ws_stride = p.stride;
-    a.o_keyw = p.o_keyw;
-    a.o_slotw = p.o_slotw;
+    a.o_ninfo = p.o_ninfo;
     a.o_rec = p.o_rec;
     a.o_sig = p.o_sig;
     a.o_pm = p.o_pm;
@@ -1270,20 +1458,20 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.epochs = wsp->epochs;
-    a.next = d_ctl.p;
+    a.next = d_ctl;
     a.xsplit = tune.xcd_split ? 1u : 0u;
     for (uint32_t x = 0; x <= kXcd; ++x)
       a.xbeg[x] = a.xsplit ? (uint32_t)((uint64_t)n_list * x / kXcd) : (x ? n_list : 0u);
-    a.n_fail = d_ctl.p + kXcd;
-    a.fail_flags = d_ctl.p + kXcd + 1;
-    a.fail_list = d_fail.p;
+    a.n_fail = d_ctl + kXcd;
+    a.fail_flags = d_ctl + kXcd + 1;
+    a.fail_list = d_fail;
     if (tout) a.tout = *tout;
     hip_check(hipEventRecord(e0, stream), "event record");
     launch_nt<MODE>(NT, waves / wpb, block, lds, stream, a);
     hip_check(hipGetLastError(), "k_search launch");
     hip_check(hipEventRecord(e1, stream), "event record");
     uint32_t ctl[16];
-    hip_check(hipMemcpyAsync(ctl, d_ctl.p, 64, hipMemcpyDeviceToHost, stream), "copy ctl");
+    hip_check(hipMemcpyAsync(ctl, d_ctl, 64, hipMemcpyDeviceToHost, stream), "copy ctl");
     hip_check(hipStreamSynchronize(stream), "sync");
     float ms = 0.0f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
@@ -1296,11 +1484,12 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     st.retries += n_failed;
     // retry the failed items with a larger workspace
     std::vector<uint32_t> fl(n_failed);
-    hip_check(hipMemcpy(fl.data(), d_fail.p, 4ull * n_failed, hipMemcpyDeviceToHost), "copy");
+    hip_check(hipMemcpy(fl.data(), d_fail, 4ull * n_failed, hipMemcpyDeviceToHost), "copy");
     std::vector<uint2> retry(n_failed);
     for (uint32_t i = 0; i < n_failed; ++i) retry[i] = list_items ? items[fl[i]] : make_uint2(0u, fl[i]);
     items.swap(retry);
-    hip_check(hipMemcpy(d_items.p, items.data(), sizeof(uint2) * n_failed, hipMemcpyHostToDevice), "copy");
+    d_items = static_cast<uint2 *>(wsp->items.get(sizeof(uint2) * n_failed, "items"));
+    hip_check(hipMemcpy(d_items, items.data(), sizeof(uint2) * n_failed, hipMemcpyHostToDevice), "copy");
     list_items = true;
     n_list = n_failed;
     if (flags & kFailCap) p.cap = std::min<uint32_t>(maxN, p.cap * 4);
@@ -1308,7 +1497,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     if (attempt == 5) fail(CS_EINTERNAL, "search workspace retries exhausted");
   }
   unsigned long long stv[4];
-  hip_check(hipMemcpy(stv, d_stats.p, 32, hipMemcpyDeviceToHost), "copy stats");
+  hip_check(hipMemcpy(stv, d_stats, 32, hipMemcpyDeviceToHost), "copy stats");
   st.kernel_ms += kms;
   st.nodes_reached += stv[0];
   st.edges_scanned += stv[1];
@@ -1363,17 +1552,13 @@ struct ShortestBatch {
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
-    for (void *q : {(void *)d_jobs, (void *)d_items, (void *)d_ctl, (void *)d_fail, (void *)d_stats, (void *)ws.ws,
-                    (void *)ws.epochs})
+    for (void *q : {(void *)d_jobs, (void *)d_items, (void *)d_ctl, (void *)d_fail, (void *)d_stats})
       if (q) (void)hipFree(q);
     d_jobs = nullptr;
     d_items = nullptr;
     d_ctl = d_fail = nullptr;
     d_stats = nullptr;
-    ws.ws = nullptr;
-    ws.epochs = nullptr;
-    ws.bytes = 0;
-    ws.waves = 0;
+    ws.release();
     ready = false;
   }
 
@@ -1408,7 +1593,7 @@ struct ShortestBatch {
     layout(p, maxN, NT, true);
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    const uint32_t by_mem = (uint32_t)std::min<size_t>((free_b / 4) / p.stride, 1u << 20);
+    const uint32_t by_mem = (uint32_t)std::min<size_t>((free_b / 4) / (p.stride + 8ull * maxN), 1u << 20);
     if (by_mem < wpb) fail(CS_ENOMEM, "batch too large for the search workspace on this device");
     waves = std::min<uint32_t>((uint32_t)std::min<uint64_t>(n_items, resident), by_mem);
     waves = std::max<uint32_t>(wpb, (waves + wpb - 1) / wpb * wpb);
@@ -1457,10 +1642,11 @@ struct ShortestBatch {
     a.cap = p.cap;
     a.nbk_lds = p.nbk;
     a.stats = d_stats;
-    a.ws = ws.ws;
+    a.keys = ws.keys;
+    a.ncap = ws.ncap;
+    a.ws = static_cast<char *>(ws.scratch.p);
     a.ws_stride = p.stride;
-    a.o_keyw = p.o_keyw;
-    a.o_slotw = p.o_slotw;
+    a.o_ninfo = p.o_ninfo;
     a.o_rec = p.o_rec;
     a.o_sig = p.o_sig;
     a.o_pm = p.o_pm;
@@ -1483,8 +1669,8 @@ struct ShortestBatch {
     if (n_items == 0) return;
     // epochs: every wave may use up to n_items of them per run
     if (ws.epoch_hi + n_items + 2 >= 0xfffffff0ull) {
-      hip_check(hipMemsetAsync(ws.ws, 0, ws.bytes, stream), "hipMemset ws");
-      hip_check(hipMemsetAsync(ws.epochs, 0, 4ull * ws.waves, stream), "hipMemset epochs");
+      hip_check(hipMemsetAsync(ws.keys, 0, 8ull * ws.ncap * ws.key_waves, stream), "hipMemset keys");
+      hip_check(hipMemsetAsync(ws.epochs, 0, 4ull * ws.key_waves, stream), "hipMemset epochs");
       ws.epoch_hi = 0;
     }
     ws.epoch_hi += n_items;
@@ -1582,8 +1768,8 @@ void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
 }
 
 void launch_shortest_batch(const std::vector<JobSpec> &jobs, uint32_t NT, const Tuning &tune, hipStream_t stream,
-                           cs_stats &st) {
-  run_jobs<kBrandes>(jobs, NT, tune, nullptr, stream, st);
+                           cs_stats &st, const std::vector<uint2> *items) {
+  run_jobs<kBrandes>(jobs, NT, tune, nullptr, stream, st, items);
 }
 
 ShortestBatch *batch_new(std::vector<JobSpec> specs, std::vector<ZeroRegion> zero, uint32_t NT, int device,

@@ -211,7 +211,7 @@ int cs_batch_create(cs_batch **out);
 void cs_batch_destroy(cs_batch *b);
 /* Adds a job with the arguments of cs_centrality_shortest; `out` ([7][T][N])
  * and `reach_totals` ([T], may be NULL) are filled by cs_batch_results.
- * Returns the job index (>= 0) or a negative status. */
+ * Returns the job index (>= 0) or minus a status code. */
 int cs_batch_add_shortest(cs_batch *b, cs_graph *g, const cs_params *p, double *out, uint32_t *reach_totals);
 /* Times candidate launch shapes on a `sample_fraction` of the batch's sources
  * and keeps the fastest; `chosen` (may be NULL) receives it. */
@@ -222,6 +222,47 @@ int cs_batch_set_tuning(cs_batch *b, const cs_tuning *t);
 int cs_batch_run(cs_batch *b);
 /* Copies the last run's results into the jobs' `out` / `reach_totals` arrays. */
 int cs_batch_results(cs_batch *b);
+
+/* ---------------------------------------------------------------------- */
+/* Serving: a request loop with continuous batching                       */
+/* ---------------------------------------------------------------------- */
+/* No counterpart in the reference. Requests are admitted first-come
+ * first-served under an HBM budget; admitted shortest-path requests share
+ * back-to-back batched launches (each launch takes part of every active
+ * request's remaining sources, so requests join and leave between launches);
+ * simplest-path requests run on a second stream. Graphs must outlive the
+ * requests that use them; `out` / `reach_totals` must stay valid until the
+ * request completes. */
+
+#define CS_REQ_SHORTEST 0 /* out [7][T][N], as cs_centrality_shortest */
+#define CS_REQ_SIMPLEST 1 /* out [4][T][N], as cs_centrality_simplest */
+
+typedef struct cs_server cs_server;
+typedef struct cs_server_config {
+  uint64_t hbm_budget_bytes; /* device bytes admitted requests may hold; 0 = a quarter of free HBM */
+  uint32_t max_active;       /* requests on the device at once; 0 = 64 */
+  double slice_ms;           /* target duration of one batched launch; 0 = 20 ms */
+} cs_server_config;
+typedef struct cs_request_info {
+  int32_t status;     /* 0 queued, 1 running, 2 done, or minus the error status */
+  double queued_ms;   /* submit -> admission (so far, if still queued) */
+  double latency_ms;  /* submit -> results in `out` (so far, if not done) */
+  uint64_t sources;   /* sources to search */
+  uint32_t launches;  /* batched launches the request took part in */
+} cs_request_info;
+
+int cs_server_create(const cs_server_config *cfg, cs_server **out);
+/* Stops after the running launches; queued requests fail with CS_EBUSY. No
+ * thread may be inside cs_server_wait on this server. */
+void cs_server_destroy(cs_server *s);
+/* Returns a request id (>= 0) or minus a status code. */
+int64_t cs_server_submit(cs_server *s, cs_graph *g, int kind, const cs_params *p, double *out,
+                         uint32_t *reach_totals);
+int cs_server_poll(cs_server *s, int64_t id, cs_request_info *info);
+/* Blocks until the request completes; returns its status. */
+int cs_server_wait(cs_server *s, int64_t id, cs_request_info *info);
+/* Forgets a completed request. */
+int cs_server_release(cs_server *s, int64_t id);
 
 #ifdef __cplusplus
 }

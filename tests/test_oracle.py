@@ -344,3 +344,46 @@ def test_keyed_order_equals_rust_heap_on_tie_free_graph():
     O.set_heap_mode(True)
     r_ang, _ = ogd.centrality_simplest([400, 900])
     assert np.array_equal(k_ang, r_ang)
+
+
+def _phase2_closed_form(ns, ref, tol, speed=1.33333):
+    """Phase 2 predecessor sets and sigma rebuilt from an oracle traversal's final seconds and settle
+    order, straight from the definition: u precedes v when u settles before v and
+    secs_u + es(v->u) <= secs_v * (1 + tol) (f32), sigma_v = sum of sigma_u in u's settle order."""
+    a = ns.arrays()
+    es = (a["length"].astype(np.float32) * a["imp"].astype(np.float32)) / np.float32(speed)
+    order = ref["order"].tolist()
+    pos = {v: i for i, v in enumerate(order)}
+    band = np.float32(np.float32(1.0) + np.float32(tol))
+    preds = {v: set() for v in order}
+    for e in range(len(a["src"])):
+        v, u = int(a["src"][e]), int(a["dst"][e])
+        if u == v or v not in pos or u not in pos or pos[u] >= pos[v]:
+            continue
+        if np.float32(ref["secs"][u] + es[e]) <= np.float32(ref["secs"][v] * band):
+            preds[v].add(u)
+    sigma = {v: 0.0 for v in order}
+    sigma[order[0]] = 1.0
+    for u in order:
+        for v in order:
+            if u in preds[v]:
+                sigma[v] += sigma[u]
+    return preds, sigma
+
+
+@pytest.mark.parametrize("tol", [0.01, 0.05, 0.3, 1.0])
+def test_phase2_matches_closed_form(tol):
+    O.set_heap_mode(False)
+    lat = mock.perturbed_lattice(9, seed=13, diag_frac=0.1)
+    ns = mock.network_structure_from_lattice(lat)
+    og = OracleGraph.from_network_structure(ns)
+    for src in (0, 17, 40):
+        ref = og.brandes_shortest(src, 900, tolerance=tol)
+        preds, sigma = _phase2_closed_form(ns, ref, tol)
+        for v in ref["order"].tolist():
+            assert ref["preds"][v] == sorted(preds[v]), (tol, src, v)
+            assert ref["sigma"][v] == sigma[v], (tol, src, v)
+        # a plausible mistake (comparing against the tolerance-free band) changes some sets
+        if tol >= 0.05:
+            loose, _ = _phase2_closed_form(ns, ref, 1e-4)
+            assert any(sorted(loose[v]) != ref["preds"][v] for v in ref["order"].tolist())
