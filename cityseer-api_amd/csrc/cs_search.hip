@@ -84,7 +84,7 @@ struct SearchArgs {
   size_t ncap;
   char *ws;         // [waves][ws_stride] scratch
   size_t ws_stride;
-  size_t o_ninfo, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd;
+  size_t o_ninfo, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
   uint32_t *epochs;
   uint32_t *next;            // [kXcd] per-XCD item counters
   uint32_t xbeg[kXcd + 1];   // XCD x owns item indices [xbeg[x], xbeg[x+1])
@@ -189,6 +189,7 @@ struct WaveWs {
   double *acc;      // [cap][2T] dependencies (Brandes) / [cap][T] subtree sums (segment)
   uint2 *bkt;       // [nbk][bcap] bucket ring entries {node, secs bits}
   uint32_t *bnd;    // [maxb + 2] slot bounds per opened bucket
+  uint32_t *lvl;    // [maxb + 2] highest Phase 2 resolution round per bucket
   uint32_t *ring;   // LDS [nbk] bucket ring counters
 };
 
@@ -203,6 +204,7 @@ __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uin
   w.acc = reinterpret_cast<double *>(base + a.o_acc);
   w.bkt = reinterpret_cast<uint2 *>(base + a.o_bkt);
   w.bnd = reinterpret_cast<uint32_t *>(base + a.o_bnd);
+  w.lvl = reinterpret_cast<uint32_t *>(base + a.o_lvl);
   w.ring = ring;
   return w;
 }
@@ -603,8 +605,10 @@ __device__ __forceinline__ uint32_t next_item(const SearchArgs &a, uint32_t x) {
 // sigma_v sums sigma_u in u's settle order. A predecessor can settle in v's
 // own bucket, so each bucket resolves in rounds: a node is ready once all its
 // predecessors are. Record bit 0x200 of the metadata word marks a resolved
-// node. Returns false if a bucket fails to resolve (cannot happen: every
-// predecessor settles strictly earlier).
+// node; bits 16-31 hold the round in which it resolved, its level: every
+// same-bucket predecessor has a lower level, so back-propagation can visit a
+// bucket level by level from the top. Returns false if a bucket fails to
+// resolve (cannot happen: every predecessor settles strictly earlier).
 constexpr uint32_t kP2Done = 0x200u;
 
 __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
@@ -659,7 +663,9 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
         for (uint64_t mm = pm; mm; mm &= mm - 1) {
           const uint32_t u = j.fwd_adj[f0 + __builtin_ctzll(mm)].x & kNodeMask;
           const uint32_t su = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
-          if (!((uint32_t)(ld64(&w.rec[2 * (size_t)su + 1]) >> 32) & kP2Done)) ready = false;
+          const uint32_t mu = (uint32_t)(ld64(&w.rec[2 * (size_t)su + 1]) >> 32);
+          // same-bucket predecessors must have resolved in an earlier round
+          if (!(mu & kP2Done) || (su >= s0 && su < s1 && (mu >> 16) >= round)) ready = false;
         }
         if (!ready) {
           ++pending;
@@ -694,12 +700,13 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
         }
         if (pm) pslot0 = (uint32_t)ld64(&w.ninfo[2 * (size_t)(j.fwd_adj[f0 + __builtin_ctzll(pm)].x & kNodeMask)]);
         std_(&w.sig[s], sigma);
-        st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np) << 32) | pslot0);
+        st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np | (round << 16)) << 32) | pslot0);
       }
       pending = uniform(wave_sum(pending));
+      if (pending == 0 && lane_id() == 0) st32(&w.lvl[b], round);
       wave_fence();
       if (pending == 0) break;
-      if (round > s1 - s0) return false;
+      if (round > s1 - s0 || round >= 0xffffu) return false;
     }
   }
   return true;
@@ -800,10 +807,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       }
       // ---- betweenness: reverse bucket order (centrality.rs:1783-1835, 823-873)
       if (j.flags & 2u) {
+        // with Phase 2, a successor may share its predecessor's bucket: visit
+        // each bucket level by level, highest first (levels are all 0 otherwise)
+        const bool p2 = j.tol > kTieEps;
         uint32_t s1 = R;
         uint32_t s0 = uniform(ld32(&w.bnd[c.nb - 1]));
         for (int b = (int)c.nb - 1; b >= 0; --b) {
           const uint32_t s0n = (b > 0) ? ld32(&w.bnd[b - 1]) : 0u;
+          const uint32_t top = p2 ? uniform(ld32(&w.lvl[b])) : 0u;
+          for (int L = (int)top; L >= 0; --L) {
           for (uint32_t s = s0 + lane; s < s1; s += kWave) {
             const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
             const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
@@ -814,6 +826,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
               da[t] = ldd(&w.acc[(size_t)s * 2 * NT + t]);
               db[t] = ldd(&w.acc[(size_t)s * 2 * NT + NT + t]);
             }
+            if (p2 && (uint32_t)(hi >> 48) != (uint32_t)L) continue;
             const uint32_t v = (uint32_t)lo;
             const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
             const uint32_t pslot0 = (uint32_t)hi;
@@ -876,6 +889,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             }
           }
           wave_fence();
+          }
           s1 = s0;
           s0 = uniform(s0n);
         }
@@ -1126,7 +1140,7 @@ int blocks_per_cu(uint32_t T, uint32_t block, size_t lds) {
 struct Plan {
   uint32_t bcap, cap, maxb, nbk, waves;
   size_t stride;
-  size_t o_ninfo, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd;
+  size_t o_ninfo, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
 };
 
 // Per-wave scratch layout. Everything here is written before it is read
@@ -1147,6 +1161,8 @@ void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   p.o_bkt = o;
   o = align256(o + 8ull * p.nbk * p.bcap);
   p.o_bnd = o;
+  o = align256(o + 4ull * (p.maxb + 2));
+  p.o_lvl = o;
   o = align256(o + 4ull * (p.maxb + 2));
   p.stride = o;
 }
@@ -1457,6 +1473,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.o_acc = p.o_acc;
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
+    a.o_lvl = p.o_lvl;
     a.epochs = wsp->epochs;
     a.next = d_ctl;
     a.xsplit = tune.xcd_split ? 1u : 0u;
@@ -1653,6 +1670,7 @@ struct ShortestBatch {
     a.o_acc = p.o_acc;
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
+    a.o_lvl = p.o_lvl;
     a.epochs = ws.epochs;
     a.next = d_ctl;
     a.xsplit = tune.xcd_split ? 1u : 0u;
