@@ -32,6 +32,11 @@
 #include "cs_internal.hpp"
 
 #include <algorithm>
+
+// 64-bit wave reductions of the device library (declared by HIP only under an
+// opt-in macro).
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_add_u64(unsigned long long);
+extern "C" __device__ __attribute__((const)) double __ockl_wfred_add_f64(double);
 #include <cmath>
 #include <cstring>
 
@@ -143,18 +148,13 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint32_t rank_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
-template <typename V>
-__device__ __forceinline__ V wave_sum(V v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
+// Wave-wide reductions over the active lanes (the device library's DPP
+// sequences: no LDS round trips), result in every lane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return __ockl_wfred_add_u32(v); }
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) { return __ockl_wfred_add_u64(v); }
+__device__ __forceinline__ double wave_sum(double v) { return __ockl_wfred_add_f64(v); }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-  return v;
-}
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) { return __ockl_wfred_or_u32(v); }
 
 __device__ __forceinline__ uint64_t pack_key(uint32_t epoch, float secs) {
   return ((uint64_t)epoch << 32) | (uint32_t)(~__float_as_uint(secs));
@@ -556,7 +556,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           const bool in = valid && route_v <= j.dist[t];
-          nband[t] += uniform(wave_sum(in ? 1u : 0u));
+          nband[t] += (uint32_t)__popcll(__ballot(in));
           eband[t] += uniform(wave_sum(in ? cnt_e : 0u));
         }
       }
@@ -716,7 +716,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
 // Persistent kernel: each wave loops over sources taken from the counters.
 // ---------------------------------------------------------------------------
 template <int MODE, int NT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_search(SearchArgs a) {
+__global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
   extern __shared__ uint32_t lds_rings[];
   const uint32_t lane = lane_id();
   const uint32_t wib = uniform(threadIdx.x / kWave);
