@@ -45,7 +45,7 @@ namespace cs {
 namespace {
 
 constexpr uint32_t kFailCap = 1u, kFailBucket = 2u, kFailRing = 4u, kFailInternal = 8u;
-constexpr int kChunk = 4;      // edges per lane per batch of memory operations
+constexpr int kChunk = 1;      // edges per lane per batch: one (wider batches oversubscribe the memory pipeline)
 constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dealt to them round-robin
 
 enum Mode { kBrandes = 0, kSegment = 1, kTree = 2, kBrandesExport = 3 };
@@ -89,7 +89,7 @@ struct SearchArgs {
   size_t ncap;
   char *ws;         // [waves][ws_stride] scratch
   size_t ws_stride;
-  size_t o_ninfo, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
+  size_t o_ninfo, o_rec, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
   uint32_t *epochs;
   uint32_t *next;            // [kXcd] per-XCD item counters
   uint32_t xbeg[kXcd + 1];   // XCD x owns item indices [xbeg[x], xbeg[x+1])
@@ -184,7 +184,6 @@ struct WaveWs {
   uint64_t *ninfo;  // [N][2] of a settled node: {slot, payload}, payload = sigma bits (Brandes)
                     // or origin-segment length bits (trees)
   uint64_t *rec;    // [cap][2]
-  double *sig;      // [cap] sigma (Brandes) / unused
   uint64_t *pm;     // [cap] predecessor row mask (Brandes) / {last len, origin len} (trees)
   double *acc;      // [cap][2T] dependencies (Brandes) / [cap][T] subtree sums (segment)
   uint2 *bkt;       // [nbk][bcap] bucket ring entries {node, secs bits}
@@ -199,7 +198,6 @@ __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uin
   w.keyw = a.keys + (size_t)wid * a.ncap;
   w.ninfo = reinterpret_cast<uint64_t *>(base + a.o_ninfo);
   w.rec = reinterpret_cast<uint64_t *>(base + a.o_rec);
-  w.sig = reinterpret_cast<double *>(base + a.o_sig);
   w.pm = reinterpret_cast<uint64_t *>(base + a.o_pm);
   w.acc = reinterpret_cast<double *>(base + a.o_acc);
   w.bkt = reinterpret_cast<uint2 *>(base + a.o_bkt);
@@ -516,14 +514,23 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           }
           st64(&w.ninfo[2 * (size_t)v], s);
           std_(reinterpret_cast<double *>(&w.ninfo[2 * (size_t)v + 1]), f.sigma);
-          st64(&w.rec[2 * (size_t)s], ((uint64_t)csb << 32) | v);
-          st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(npred | (el ? 0x100u : 0u)) << 32) | pslot0);
-          std_(&w.sig[s], f.sigma);
-          st64(&w.pm[s], f.pmask);
+          // the slot record is read only after this bucket's fence: one plain 16-byte store
+          *reinterpret_cast<uint4 *>(&w.rec[2 * (size_t)s]) =
+              make_uint4(v, csb, pslot0, npred | (el ? 0x100u : 0u));
+          // the predecessor mask is needed only to walk several predecessors
+          if (MODE == kBrandesExport || npred > 1) st64(&w.pm[s], f.pmask);
           if constexpr (MODE == kBrandes) {
             if (j.flags & 2u) {
+              // dependencies for thresholds the node lies beyond are never
+              // accumulated (successors lie farther still) nor read
+              const float route_v = secs_v * speed;
 #pragma unroll
-              for (int t = 0; t < 2 * NT; ++t) std_(&w.acc[(size_t)s * 2 * NT + t], 0.0);
+              for (int t = 0; t < NT; ++t) {
+                if (route_v <= j.dist[t]) {
+                  std_(&w.acc[(size_t)s * 2 * NT + t], 0.0);
+                  std_(&w.acc[(size_t)s * 2 * NT + NT + t], 0.0);
+                }
+              }
             }
           }
         } else {
@@ -626,7 +633,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
       uint64_t pm = 0;
       if (s == 0) {
         meta |= kP2Done;
-        std_(&w.sig[0], 1.0);
+        std_(reinterpret_cast<double *>(&w.ninfo[2 * (size_t)v + 1]), 1.0);
       } else {
         const uint4 rw = j.rows[v];
         const float lim = secs_v * band;
@@ -693,13 +700,13 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
               bslot = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
             }
           }
-          sigma += ldd(&w.sig[bslot]);
+          sigma += ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)bu + 1]));
           have_last = true;
           lsu = bsu;
           lu = bu;
         }
         if (pm) pslot0 = (uint32_t)ld64(&w.ninfo[2 * (size_t)(j.fwd_adj[f0 + __builtin_ctzll(pm)].x & kNodeMask)]);
-        std_(&w.sig[s], sigma);
+        std_(reinterpret_cast<double *>(&w.ninfo[2 * (size_t)v + 1]), sigma);
         st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np | (round << 16)) << 32) | pslot0);
       }
       pending = uniform(wave_sum(pending));
@@ -819,16 +826,18 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
           for (uint32_t s = s0 + lane; s < s1; s += kWave) {
             const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
             const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
-            const double sigma_w = ldd(&w.sig[s]);
-            double da[NT], db[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-              da[t] = ldd(&w.acc[(size_t)s * 2 * NT + t]);
-              db[t] = ldd(&w.acc[(size_t)s * 2 * NT + NT + t]);
-            }
             if (p2 && (uint32_t)(hi >> 48) != (uint32_t)L) continue;
             const uint32_t v = (uint32_t)lo;
             const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
+            double da[NT], db[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+              da[t] = db[t] = 0.0;
+              if (route <= j.dist[t]) {
+                da[t] = ldd(&w.acc[(size_t)s * 2 * NT + t]);
+                db[t] = ldd(&w.acc[(size_t)s * 2 * NT + NT + t]);
+              }
+            }
             const uint32_t pslot0 = (uint32_t)hi;
             const uint32_t meta = (uint32_t)(hi >> 32);
             const uint32_t npred = meta & 0xffu;
@@ -862,11 +871,12 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             } else if (npred > 1) {
               const uint64_t pm = ld64(&w.pm[s]);
               const uint32_t f0 = j.rows[v].x;
+              const double sigma_w = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)v + 1]));
               for (uint64_t mm = pm; mm; mm &= mm - 1) {
                 const uint32_t jb = __builtin_ctzll(mm);
                 const uint32_t u = j.fwd_adj[f0 + jb].x & kNodeMask;
                 const uint32_t su = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
-                const double sigma_u = ldd(&w.sig[su]);
+                const double sigma_u = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)u + 1]));
                 if (sigma_u == 0.0) continue;
                 const double fct = sigma_u / sigma_w;
 #pragma unroll
@@ -1057,7 +1067,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
           }
         } else if constexpr (MODE == kBrandesExport) {
           const uint64_t pm = ld64(&w.pm[s]);
-          a.tout.sigma[v] = ldd(&w.sig[s]);
+          a.tout.sigma[v] = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)v + 1]));
           uint32_t np = 0;
           const uint32_t f0 = j.rows[v].x;
           for (uint64_t mm = pm; mm; mm &= mm - 1) {
@@ -1140,7 +1150,7 @@ int blocks_per_cu(uint32_t T, uint32_t block, size_t lds) {
 struct Plan {
   uint32_t bcap, cap, maxb, nbk, waves;
   size_t stride;
-  size_t o_ninfo, o_rec, o_sig, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
+  size_t o_ninfo, o_rec, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
 };
 
 // Per-wave scratch layout. Everything here is written before it is read
@@ -1152,8 +1162,6 @@ void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   o = align256(o + 16ull * N);
   p.o_rec = o;
   o = align256(o + 16ull * p.cap);
-  p.o_sig = o;
-  o = align256(o + (brandes ? 8ull * p.cap : 0));
   p.o_pm = o;
   o = align256(o + 8ull * p.cap);
   p.o_acc = o;
@@ -1468,7 +1476,6 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.ws_stride = p.stride;
     a.o_ninfo = p.o_ninfo;
     a.o_rec = p.o_rec;
-    a.o_sig = p.o_sig;
     a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
     a.o_bkt = p.o_bkt;
@@ -1665,7 +1672,6 @@ struct ShortestBatch {
     a.ws_stride = p.stride;
     a.o_ninfo = p.o_ninfo;
     a.o_rec = p.o_rec;
-    a.o_sig = p.o_sig;
     a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
     a.o_bkt = p.o_bkt;
