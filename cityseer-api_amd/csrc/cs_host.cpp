@@ -207,8 +207,12 @@ DevEdges &device_edges(Graph &g, DevGraph &dg, float speed) {
       const float es = edge_seconds(g, nb, v, e, speed, true);
       uint32_t bits;
       std::memcpy(&bits, &es, 4);
-      rev[j] = make_uint2(nb | (g.canon[e] ? kCanonBit : 0u), bits);
       const int32_t tw = g.twin[e];
+      // circuit-rank flag of an in-edge: its undirected key is counted here
+      // (at the later-settled endpoint) when it or its reverse twin is the
+      // key's canonical edge
+      const bool count_in = g.canon[e] || (tw >= 0 && g.canon[tw]);
+      rev[j] = make_uint2(nb | (count_in ? kCanonBit : 0u), bits);
       rtw[j] = make_float2(tw >= 0 ? g.elen[tw] : NAN, tw >= 0 ? g.eimp[tw] : NAN);
       reid[j] = e;
       if (nb != v) {
@@ -222,8 +226,10 @@ DevEdges &device_edges(Graph &g, DevGraph &dg, float speed) {
       const float es = edge_seconds(g, v, u, e, speed, true);
       uint32_t bits;
       std::memcpy(&bits, &es, 4);
-      fwd[j] = make_uint2(u | (g.canon[e] ? kCanonBit : 0u), bits);
       const int32_t tw = g.twin[e];
+      // out-edges count only one-way canonical edges (no twin among the in-edges)
+      const bool count_out = g.canon[e] && tw < 0;
+      fwd[j] = make_uint2(u | (count_out ? kCanonBit : 0u), bits);
       ftw[j] = tw >= 0 ? g.elen[tw] : NAN;
       feid[j] = e;
     }

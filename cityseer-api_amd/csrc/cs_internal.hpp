@@ -37,8 +37,11 @@ void hip_check(hipError_t e, const char *what);
 struct DevEdges {
   float speed = 0.0f;
   // shortest / segment (reverse search over in-edges, fold over out-edges)
-  uint2 *rev_adj = nullptr;     // {nb | canon<<31, bits(seconds of nb->v)}
-  uint2 *fwd_adj = nullptr;     // {u  | canon<<31, bits(seconds of v->u)}
+  // bit 31 marks the entry that counts its undirected key toward the circuit
+  // rank: in-edges whose key's canonical edge is this edge or its reverse twin,
+  // out-edges that are canonical one-way edges
+  uint2 *rev_adj = nullptr;     // {nb | count<<31, bits(seconds of nb->v)}
+  uint2 *fwd_adj = nullptr;     // {u  | count<<31, bits(seconds of v->u)}
   float2 *rev_tw = nullptr;     // {twin length, twin imp} per rev position (segment)
   float *fwd_twlen = nullptr;   // twin length per fwd position (segment)
   uint32_t *rev_eid = nullptr;  // edge id per rev position (trees)

@@ -298,6 +298,7 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
 struct SrcCtx {
   uint32_t src;
   uint32_t tag;
+  float wt;        // source weight (closeness terms)
   uint32_t k;      // current absolute bucket
   uint32_t nslot;  // settled nodes so far
   uint32_t nb;     // buckets opened
@@ -406,7 +407,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             fk[q] = 0;
             fs[q] = 0;
             fp[q] = 0;
-            if (!is_src && u != v && (fold || cycles)) {
+            if (!is_src && u != v && (fold || (cycles && (fe[q].x & kCanonBit)))) {
               fk[q] = ld64(&w.keyw[u]);
               if (fold) {
                 fs[q] = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
@@ -520,10 +521,26 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           // the predecessor mask is needed only to walk several predecessors
           if (MODE == kBrandesExport || npred > 1) st64(&w.pm[s], f.pmask);
           if constexpr (MODE == kBrandes) {
+            const float route_v = secs_v * speed;
+            // closeness terms that need only this node's route go to the target
+            // now (centrality.rs:1755-1778); the cycles term waits for the
+            // source's final circuit rank
+            if ((j.flags & 1u) && !is_src) {
+              const uint32_t ts = j.tstride;
+              double *accv = j.acc + (size_t)v * 7 * ts;
+#pragma unroll
+              for (int t = 0; t < NT; ++t) {
+                if (route_v <= j.dist[t]) {
+                  gadd_f64(&accv[0 * ts + t], (double)c.wt);
+                  gadd_f64(&accv[1 * ts + t], (double)(route_v * c.wt));
+                  gadd_f64(&accv[3 * ts + t], (double)((1.0f / route_v) * c.wt));
+                  gadd_f64(&accv[4 * ts + t], (double)(expf(-j.beta[t] * route_v) * c.wt));
+                }
+              }
+            }
             if (j.flags & 2u) {
               // dependencies for thresholds the node lies beyond are never
               // accumulated (successors lie farther still) nor read
-              const float route_v = secs_v * speed;
 #pragma unroll
               for (int t = 0; t < NT; ++t) {
                 if (route_v <= j.dist[t]) {
@@ -557,14 +574,15 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           }
         }
       }
-      // circuit-rank counts per threshold band, reduced over the wave
-      if (cycles) {
+      // nodes within each threshold (circuit rank, reach) and the rank's edge
+      // counts, reduced over the wave
+      if constexpr (MODE == kBrandes) {
         const float route_v = __uint_as_float(csb) * speed;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           const bool in = valid && route_v <= j.dist[t];
           nband[t] += (uint32_t)__popcll(__ballot(in));
-          eband[t] += uniform(wave_sum(in ? cnt_e : 0u));
+          if (cycles) eband[t] += uniform(wave_sum(in ? cnt_e : 0u));
         }
       }
     }
@@ -688,7 +706,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
         for (uint32_t step = 0; step < np; ++step) {
           bool found = false;
           float bsu = 0.0f;
-          uint32_t bu = 0, bslot = 0;
+          uint32_t bu = 0;
           for (uint64_t mm = pm; mm; mm &= mm - 1) {
             const uint32_t u = j.fwd_adj[f0 + __builtin_ctzll(mm)].x & kNodeMask;
             const float su = key_secs(ld64(&w.keyw[u]));
@@ -697,7 +715,6 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
               found = true;
               bsu = su;
               bu = u;
-              bslot = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
             }
           }
           sigma += ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)bu + 1]));
@@ -748,6 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
     const float speed = j.speed;
     SrcCtx c;
     c.src = j.sources[si];
+    c.wt = (MODE == kBrandes) ? j.src_wt[si] : 0.0f;
     ++epoch;
     c.tag = epoch;
     uint32_t nband[NT], eband[NT];
@@ -776,41 +794,31 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
       const float cyc = j.src_cyc[si];
       constexpr uint32_t M = 7;
       float rank[NT];
-      uint32_t within[NT];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const uint32_t nn = nband[t];
         const uint32_t ee = eband[t];
         const int r = (int)ee - (int)nn + 1;
         rank[t] = (nn == 0) ? 0.0f : (float)(r > 0 ? r : 0);
-        within[t] = 0;
-      }
-      // ---- closeness: scatter to targets (centrality.rs:1736-1779); slot 0 is the source
-      for (uint32_t s = 1 + lane; s < R; s += kWave) {
-        const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
-        const uint32_t v = (uint32_t)lo;
-        const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
-        const uint32_t ts = j.tstride;
-        double *accv = j.acc + (size_t)v * M * ts;
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          if (route <= j.dist[t]) {
-            ++within[t];
-            if (j.flags & 1u) {
-              gadd_f64(&accv[0 * ts + t], (double)wt);
-              gadd_f64(&accv[1 * ts + t], (double)(route * wt));
-              gadd_f64(&accv[2 * ts + t], (double)(rank[t] * cyc));
-              gadd_f64(&accv[3 * ts + t], (double)((1.0f / route) * wt));
-              gadd_f64(&accv[4 * ts + t], (double)(expf(-j.beta[t] * route) * wt));
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const uint32_t tot = uniform(wave_sum(within[t]));
+        // targets within the threshold: every node counted but the source
+        const uint32_t tot = nn - 1;
         st_within += tot;
         if ((j.flags & 5u) == 5u && lane == 0 && tot) atomicAdd(&j.reach[t], tot);
+      }
+      const bool clo = (j.flags & 1u) != 0;
+      // ---- cycles term of closeness (centrality.rs:1764-1767): with betweenness it
+      // rides along the back-propagation walk below; slot 0 is the source
+      if (clo && !(j.flags & 2u)) {
+        for (uint32_t s = 1 + lane; s < R; s += kWave) {
+          const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+          const uint32_t v = (uint32_t)lo;
+          const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
+          const uint32_t ts = j.tstride;
+          double *accv = j.acc + (size_t)v * M * ts;
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            if (route <= j.dist[t]) gadd_f64(&accv[2 * ts + t], (double)(rank[t] * cyc));
+        }
       }
       // ---- betweenness: reverse bucket order (centrality.rs:1783-1835, 823-873)
       if (j.flags & 2u) {
@@ -845,6 +853,11 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             const double pc = is_src ? 0.0 : ((meta & 0x100u) ? 0.5 : 1.0);
             const uint32_t ts = j.tstride;
             double *accv = j.acc + (size_t)v * M * ts;
+            if (clo && !is_src) {
+#pragma unroll
+              for (int t = 0; t < NT; ++t)
+                if (route <= j.dist[t]) gadd_f64(&accv[2 * ts + t], (double)(rank[t] * cyc));
+            }
             double dep[NT], dep_b[NT], seed_b[NT];
             bool any = false;
 #pragma unroll
