@@ -36,7 +36,7 @@ struct AngArgs {
   const uint32_t *sources;
   const float *src_wt;
   uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
-  double *acc;     // [N][4][T]
+  double *acc;     // [4][T][N], metric-major
   uint32_t *reach;
   unsigned long long *stats;
   char *ws;
@@ -369,17 +369,18 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
       const float bsd = *il<float>(w, a.o_brt, k, lane);
       const float bs = *il<float>(w, a.o_bsec, k, lane);
       if (!isfinite(bsd) || !isfinite(bs)) continue;
-      double *accv = a.acc + (size_t)v * 4 * T;
+      double *accv = a.acc + v;
+      const size_t NN = a.N;
       for (uint32_t t = 0; t < T; ++t) {
         if (bs <= a.sec_thr[t]) {
           ++st_within;
           if (a.flags & 4u) atomicAdd(&a.reach[t], 1u);
           if (a.flags & 1u) {
-            unsafeAtomicAdd(&accv[0 * T + t], (double)wt);
+            unsafeAtomicAdd(&accv[((size_t)0 * T + t) * NN], (double)wt);
             const float far_ang = a.offset + (bsd / a.unit);
-            unsafeAtomicAdd(&accv[1 * T + t], (double)(far_ang * wt));
+            unsafeAtomicAdd(&accv[((size_t)1 * T + t) * NN], (double)(far_ang * wt));
             const float harm_ang = 1.0f + (bsd / a.unit);
-            unsafeAtomicAdd(&accv[2 * T + t], (double)((1.0f / harm_ang) * wt));
+            unsafeAtomicAdd(&accv[((size_t)2 * T + t) * NN], (double)((1.0f / harm_ang) * wt));
           }
         }
       }
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
             const uint32_t node = (*il<uint32_t>(w, a.o_state, ss, lane) & 0x7fffffffu) >> 1;
             if (node == src) continue;
             const double credit = dep - seed;
-            if (credit > 0.0) unsafeAtomicAdd(&a.acc[(size_t)node * 4 * T + 3 * T + t], credit * (double)wt);
+            if (credit > 0.0) unsafeAtomicAdd(&a.acc[((size_t)3 * T + t) * a.N + node], credit * (double)wt);
           }
           hi = lo - 1;
         }

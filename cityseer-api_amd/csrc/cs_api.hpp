@@ -36,7 +36,7 @@ struct SourceSet {
 };
 SourceSet source_set(const Graph &g, const cs_params *p);
 
-// acc [N][M][stride] (device, node-major) -> out [M][T][N] (host)
+// acc [M][stride][N] (device rows) -> out [M][T][N] (host)
 void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out);
 void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s);
 

@@ -398,11 +398,11 @@ SourceSet source_set(const Graph &g, const cs_params *p) {
   return s;
 }
 
-// acc [N][M][stride] (device, node-major) -> out [M][T][N] (host)
+// acc [M][stride][N] (device rows) -> out [M][T][N] (host): drops the padded thresholds
 void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out) {
-  for (uint32_t v = 0; v < N; ++v)
-    for (uint32_t m = 0; m < M; ++m)
-      for (uint32_t i = 0; i < T; ++i) out[((size_t)m * T + i) * N + v] = acc[((size_t)v * M + m) * stride + i];
+  for (uint32_t m = 0; m < M; ++m)
+    for (uint32_t i = 0; i < T; ++i)
+      std::memcpy(out + ((size_t)m * T + i) * N, acc + ((size_t)m * stride + i) * N, sizeof(double) * N);
 }
 
 void unpack_acc(const std::vector<double> &acc, uint32_t N, uint32_t M, uint32_t T, double *out) {

@@ -114,7 +114,8 @@ struct CallSpec {
 };
 
 // Shortest-path centrality over `n_sources` sources; acc is a device array
-// [N][7][T] (node-major) that the call accumulates into; reach [T] u32.
+// [7][T][N] (metric-major, the reference's row order) that the call
+// accumulates into; reach [T] u32.
 void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                      uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
@@ -134,7 +135,7 @@ void set_default_tuning(const Tuning &t);
 Tuning default_tuning();
 
 // One (graph, request) pair of a batched shortest-path launch. d_acc is
-// [N][7][tstride] with tstride >= cs.T (0: the launch's NT); thresholds beyond
+// [7][tstride][N] with tstride >= cs.T (0: the launch's NT); thresholds beyond
 // cs.T stay zero.
 struct JobSpec {
   Graph *g;
@@ -172,7 +173,7 @@ Tuning batch_tuning(const ShortestBatch &b);
 Tuning batch_autotune(ShortestBatch &b, double fraction, hipStream_t stream);
 void batch_run(ShortestBatch &b, hipStream_t stream, cs_stats &st);
 
-// Segment centrality; acc [N][4][T].
+// Segment centrality; acc [4][T][N].
 void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                     const uint32_t *d_sources, uint32_t n_sources, double *d_acc,
                     hipStream_t stream, cs_stats &st);

@@ -63,7 +63,7 @@ struct Job {
   const uint32_t *sources;
   const float *src_wt;
   const float *src_cyc;
-  double *acc;  // [N][M][tstride]
+  double *acc;  // [M][tstride][nacc], metric-major like the reference's result rows
   uint32_t *reach;
   float maxsec, speed, inv_delta;
   float tol;  // tie tolerance; above the internal band, Phase 2 rebuilds predecessors
@@ -71,6 +71,7 @@ struct Job {
   uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
   uint32_t n_sources;
   uint32_t tstride;  // threshold stride of acc (>= the job's threshold count)
+  uint32_t nacc;     // node count of acc (the job's graph)
   float dist[kMaxT];  // thresholds beyond the job's own count hold -1 (never within)
   float beta[kMaxT];
 };
@@ -527,14 +528,15 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             // source's final circuit rank
             if ((j.flags & 1u) && !is_src) {
               const uint32_t ts = j.tstride;
-              double *accv = j.acc + (size_t)v * 7 * ts;
+              const size_t nacc = j.nacc;
+              double *accv = j.acc + v;
 #pragma unroll
               for (int t = 0; t < NT; ++t) {
                 if (route_v <= j.dist[t]) {
-                  gadd_f64(&accv[0 * ts + t], (double)c.wt);
-                  gadd_f64(&accv[1 * ts + t], (double)(route_v * c.wt));
-                  gadd_f64(&accv[3 * ts + t], (double)((1.0f / route_v) * c.wt));
-                  gadd_f64(&accv[4 * ts + t], (double)(expf(-j.beta[t] * route_v) * c.wt));
+                  gadd_f64(&accv[((size_t)0 * ts + t) * nacc], (double)c.wt);
+                  gadd_f64(&accv[((size_t)1 * ts + t) * nacc], (double)(route_v * c.wt));
+                  gadd_f64(&accv[((size_t)3 * ts + t) * nacc], (double)((1.0f / route_v) * c.wt));
+                  gadd_f64(&accv[((size_t)4 * ts + t) * nacc], (double)(expf(-j.beta[t] * route_v) * c.wt));
                 }
               }
             }
@@ -792,7 +794,6 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
     if constexpr (MODE == kBrandes) {
       const float wt = j.src_wt[si];
       const float cyc = j.src_cyc[si];
-      constexpr uint32_t M = 7;
       float rank[NT];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
@@ -801,7 +802,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
         const int r = (int)ee - (int)nn + 1;
         rank[t] = (nn == 0) ? 0.0f : (float)(r > 0 ? r : 0);
         // targets within the threshold: every node counted but the source
-        const uint32_t tot = nn - 1;
+        const uint32_t tot = nn ? nn - 1 : 0u;  // padded thresholds hold nobody
         st_within += tot;
         if ((j.flags & 5u) == 5u && lane == 0 && tot) atomicAdd(&j.reach[t], tot);
       }
@@ -814,10 +815,11 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
           const uint32_t v = (uint32_t)lo;
           const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
           const uint32_t ts = j.tstride;
-          double *accv = j.acc + (size_t)v * M * ts;
+          const size_t nacc = j.nacc;
+          double *accv = j.acc + v;
 #pragma unroll
           for (int t = 0; t < NT; ++t)
-            if (route <= j.dist[t]) gadd_f64(&accv[2 * ts + t], (double)(rank[t] * cyc));
+            if (route <= j.dist[t]) gadd_f64(&accv[((size_t)2 * ts + t) * nacc], (double)(rank[t] * cyc));
         }
       }
       // ---- betweenness: reverse bucket order (centrality.rs:1783-1835, 823-873)
@@ -852,11 +854,12 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             const bool is_src = (s == 0);
             const double pc = is_src ? 0.0 : ((meta & 0x100u) ? 0.5 : 1.0);
             const uint32_t ts = j.tstride;
-            double *accv = j.acc + (size_t)v * M * ts;
+            const size_t nacc = j.nacc;
+          double *accv = j.acc + v;
             if (clo && !is_src) {
 #pragma unroll
               for (int t = 0; t < NT; ++t)
-                if (route <= j.dist[t]) gadd_f64(&accv[2 * ts + t], (double)(rank[t] * cyc));
+                if (route <= j.dist[t]) gadd_f64(&accv[((size_t)2 * ts + t) * nacc], (double)(rank[t] * cyc));
             }
             double dep[NT], dep_b[NT], seed_b[NT];
             bool any = false;
@@ -907,8 +910,8 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
               if (dep[t] == 0.0 && dep_b[t] == 0.0) continue;
               const double credit = dep[t] - pc;
               const double credit_b = dep_b[t] - seed_b[t];
-              if (credit > 0.0) gadd_f64(&accv[5 * ts + t], credit * (double)wt);
-              if (credit_b > 0.0) gadd_f64(&accv[6 * ts + t], credit_b * (double)wt);
+              if (credit > 0.0) gadd_f64(&accv[((size_t)5 * ts + t) * nacc], credit * (double)wt);
+              if (credit_b > 0.0) gadd_f64(&accv[((size_t)6 * ts + t) * nacc], credit_b * (double)wt);
             }
           }
           wave_fence();
@@ -918,7 +921,6 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
         }
       }
     } else if constexpr (MODE == kSegment) {
-      constexpr uint32_t M = 4;
       // ---- closeness at the source over visited edges (centrality.rs:2201-2317)
       if (j.flags & 1u) {
         double sd[NT], sh[NT], sb[NT];
@@ -1009,10 +1011,11 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
           const double t0 = wave_sum(sd[t]), t1 = wave_sum(sh[t]), t2 = wave_sum(sb[t]);
           if (lane == 0) {
             const uint32_t ts = j.tstride;
-            double *accs = j.acc + (size_t)c.src * M * ts;
-            gadd_f64(&accs[0 * ts + t], t0);
-            gadd_f64(&accs[1 * ts + t], t1);
-            gadd_f64(&accs[2 * ts + t], t2);
+            const size_t nacc = j.nacc;
+            double *accs = j.acc + c.src;
+            gadd_f64(&accs[((size_t)0 * ts + t) * nacc], t0);
+            gadd_f64(&accs[((size_t)1 * ts + t) * nacc], t1);
+            gadd_f64(&accs[((size_t)2 * ts + t) * nacc], t2);
           }
         }
       }
@@ -1039,7 +1042,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             const float o_1 = min_span, o_2 = min_span + olen, l_1 = min_span, l_2 = min_span + llen;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-              if (below[t] > 0.0) gadd_f64(&j.acc[((size_t)v * M + 3) * j.tstride + t], below[t]);
+              if (below[t] > 0.0) gadd_f64(&j.acc[((size_t)3 * j.tstride + t) * j.nacc + v], below[t]);
               double own = 0.0;
               if (v > c.src) {
                 const float dist = j.dist[t];
@@ -1374,6 +1377,7 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
   j.n_sources = js.n_sources;
   j.tstride = js.tstride ? js.tstride : NT;
+  j.nacc = g.n;
   if (j.tstride < cs.T) fail(CS_EINTERNAL, "accumulator stride below the threshold count");
   for (uint32_t i = 0; i < kMaxT; ++i) {
     j.dist[i] = i < cs.T ? cs.dist[i] : -1.0f;

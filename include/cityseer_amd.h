@@ -120,7 +120,7 @@ int cs_centrality_shortest(cs_graph *g, const cs_params *p, double *out, uint32_
  * and outputs in HBM (batching, serving, benchmarks). p->distances/betas/
  * seconds are host arrays; p->sources, p->source_wt and p->eligible are device
  * arrays, as is d_cycles_wt [n_sources] (source weight / node weight). The call
- * adds into d_acc, f64 [N][7][T] node-major (zeroed by the caller), and d_reach
+ * adds into d_acc, f64 [7][T][N] laid out like `out` (zeroed by the caller), and d_reach
  * u32 [T] (may be NULL); betw_scale is not applied. stream may be NULL (the
  * null stream). Same computation as centrality_shortest
  * (rust/src/centrality.rs:1624-1874) without host copies. */
