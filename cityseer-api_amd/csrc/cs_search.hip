@@ -47,6 +47,9 @@ namespace {
 constexpr uint32_t kFailCap = 1u, kFailBucket = 2u, kFailRing = 4u, kFailInternal = 8u;
 constexpr int kChunk = 1;      // edges per lane per batch: one (wider batches oversubscribe the memory pipeline)
 constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dealt to them round-robin
+constexpr uint32_t kScanQ = 4;     // key words per lane per step of the node-ordered result flush
+constexpr uint32_t kFlushList = kScanQ * 64;  // LDS list of one flush step (reached nodes, index order)
+constexpr uint32_t kScanSpan = 16; // scan the index span only while it is at most this many times the reach
 
 enum Mode { kBrandes = 0, kSegment = 1, kTree = 2, kBrandesExport = 3 };
 
@@ -191,6 +194,7 @@ struct WaveWs {
   uint32_t *bnd;    // [maxb + 2] slot bounds per opened bucket
   uint32_t *lvl;    // [maxb + 2] highest Phase 2 resolution round per bucket
   uint32_t *ring;   // LDS [nbk] bucket ring counters
+  uint32_t *list;   // LDS [kFlushList] node list of the result flush
 };
 
 __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uint32_t *ring) {
@@ -205,6 +209,7 @@ __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uin
   w.bnd = reinterpret_cast<uint32_t *>(base + a.o_bnd);
   w.lvl = reinterpret_cast<uint32_t *>(base + a.o_lvl);
   w.ring = ring;
+  w.list = ring + a.nbk_lds;
   return w;
 }
 
@@ -304,6 +309,7 @@ struct SrcCtx {
   uint32_t nslot;  // settled nodes so far
   uint32_t nb;     // buckets opened
   uint32_t fail;
+  uint32_t vlo, vhi;  // smallest and largest settled node index
 };
 
 // ---------------------------------------------------------------------------
@@ -333,6 +339,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
   c.nslot = 0;
   c.nb = 0;
   c.fail = 0;
+  uint32_t vlo = 0xffffffffu, vhi = 0u;  // this lane's settled index bounds
   while (pending > 0) {
     // every bucket index is below maxsec / delta: a larger k means a broken count
     if (c.k > j.maxb + j.nbk) {
@@ -378,6 +385,8 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
       c.nslot += cntv;
       uint32_t cnt_e = 0;
       if (valid) {
+        vlo = v < vlo ? v : vlo;
+        vhi = v > vhi ? v : vhi;
         const float secs_v = __uint_as_float(csb);
         const bool is_src = (s == 0);
         const uint32_t f0 = rw.x, nf = rw.y - rw.x, r0 = rw.z, nr = rw.w - rw.z;
@@ -523,23 +532,6 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           if (MODE == kBrandesExport || npred > 1) st64(&w.pm[s], f.pmask);
           if constexpr (MODE == kBrandes) {
             const float route_v = secs_v * speed;
-            // closeness terms that need only this node's route go to the target
-            // now (centrality.rs:1755-1778); the cycles term waits for the
-            // source's final circuit rank
-            if ((j.flags & 1u) && !is_src) {
-              const uint32_t ts = j.tstride;
-              const size_t nacc = j.nacc;
-              double *accv = j.acc + v;
-#pragma unroll
-              for (int t = 0; t < NT; ++t) {
-                if (route_v <= j.dist[t]) {
-                  gadd_f64(&accv[((size_t)0 * ts + t) * nacc], (double)c.wt);
-                  gadd_f64(&accv[((size_t)1 * ts + t) * nacc], (double)(route_v * c.wt));
-                  gadd_f64(&accv[((size_t)3 * ts + t) * nacc], (double)((1.0f / route_v) * c.wt));
-                  gadd_f64(&accv[((size_t)4 * ts + t) * nacc], (double)(expf(-j.beta[t] * route_v) * c.wt));
-                }
-              }
-            }
             if (j.flags & 2u) {
               // dependencies for thresholds the node lies beyond are never
               // accumulated (successors lie farther still) nor read
@@ -602,6 +594,8 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
     return false;
   }
   if (lane == 0) st32(&w.bnd[c.nb], c.nslot);
+  c.vlo = uniform(__ockl_wfred_min_u32(vlo));
+  c.vhi = uniform(__ockl_wfred_max_u32(vhi));
   wave_fence();
   return true;
 }
@@ -739,6 +733,92 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
 }
 
 // ---------------------------------------------------------------------------
+// Results of one source (centrality.rs:1733-1780 closeness, 1783-1835
+// betweenness credit): every reached target but the source adds, per threshold
+// it lies within, its five closeness terms and its two betweenness credits to
+// the shared accumulators.
+//
+// The settled targets are visited in node-index order where that is cheap: the
+// wave scans the epoch-tagged keys over [vlo, vhi], compacts the reached ones
+// into an LDS list and hands lane i the i-th, so the lanes of one atomic
+// instruction hit neighbouring addresses of a metric row (a few cache lines)
+// rather than 64 scattered ones. A sparse span falls back to slot order.
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ void flush_target(CJob &j, const WaveWs &w, uint32_t s, uint32_t v,
+                                             const float (&rank)[NT], float wt, float cyc) {
+  const uint4 r = *reinterpret_cast<const uint4 *>(&w.rec[2 * (size_t)s]);
+  const float route = __uint_as_float(r.y) * j.speed;
+  const double pc = (r.w & 0x100u) ? 0.5 : 1.0;
+  const size_t ts = j.tstride, nacc = j.nacc;
+  double *accv = j.acc + v;
+  if (j.flags & 1u) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (route <= j.dist[t]) {
+        gadd_f64(&accv[((size_t)0 * ts + t) * nacc], (double)wt);
+        gadd_f64(&accv[((size_t)1 * ts + t) * nacc], (double)(route * wt));
+        gadd_f64(&accv[((size_t)2 * ts + t) * nacc], (double)(rank[t] * cyc));
+        gadd_f64(&accv[((size_t)3 * ts + t) * nacc], (double)((1.0f / route) * wt));
+        gadd_f64(&accv[((size_t)4 * ts + t) * nacc], (double)(expf(-j.beta[t] * route) * wt));
+      }
+    }
+  }
+  if (j.flags & 2u) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (route <= j.dist[t]) {
+        // dependency = seed + delta; credit = dependency - seed (centrality.rs:848-866)
+        const double seed_b = pc * exp(-(double)j.beta[t] * (double)route);
+        const double credit = (pc + ldd(&w.acc[(size_t)s * 2 * NT + t])) - pc;
+        const double credit_b = (seed_b + ldd(&w.acc[(size_t)s * 2 * NT + NT + t])) - seed_b;
+        if (credit > 0.0) gadd_f64(&accv[((size_t)5 * ts + t) * nacc], credit * (double)wt);
+        if (credit_b > 0.0) gadd_f64(&accv[((size_t)6 * ts + t) * nacc], credit_b * (double)wt);
+      }
+    }
+  }
+}
+
+template <int NT>
+__device__ void flush_targets(CJob &j, const WaveWs &w, const SrcCtx &c, uint32_t R, const float (&rank)[NT],
+                              float wt, float cyc) {
+  if (R <= 1) return;
+  const uint32_t lane = lane_id();
+  const uint32_t lo = c.vlo, hi = c.vhi;
+  if ((uint64_t)(hi - lo) + 1 > (uint64_t)kScanSpan * R) {
+    for (uint32_t s = 1 + lane; s < R; s += kWave) {
+      const uint32_t v = (uint32_t)ld64(&w.rec[2 * (size_t)s]);
+      flush_target<NT>(j, w, s, v, rank, wt, cyc);
+    }
+    return;
+  }
+  for (uint64_t base = lo; base <= hi; base += (uint64_t)kScanQ * kWave) {
+    uint64_t key[kScanQ];
+#pragma unroll
+    for (uint32_t q = 0; q < kScanQ; ++q) {
+      const uint64_t v = base + q * kWave + lane;
+      key[q] = (v <= hi) ? ld64(&w.keyw[v]) : 0ull;
+    }
+    uint32_t n = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kScanQ; ++q) {
+      const uint32_t v = (uint32_t)(base + q * kWave + lane);
+      const bool hit = key_current(key[q], c.tag) && v != c.src;
+      const uint64_t m = __ballot(hit);
+      if (hit) lds_st(&w.list[n + rank_below(m)], v);
+      n += (uint32_t)__popcll(m);
+    }
+    wave_fence();
+    for (uint32_t i = lane; i < n; i += kWave) {
+      const uint32_t v = lds_ld(&w.list[i]);
+      const uint32_t s = (uint32_t)ld64(&w.ninfo[2 * (size_t)v]);
+      flush_target<NT>(j, w, s, v, rank, wt, cyc);
+    }
+    wave_fence();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Persistent kernel: each wave loops over sources taken from the counters.
 // ---------------------------------------------------------------------------
 template <int MODE, int NT>
@@ -748,7 +828,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
   const uint32_t wib = uniform(threadIdx.x / kWave);
   const uint32_t wid = uniform(blockIdx.x * (blockDim.x / kWave) + wib);
   const uint32_t xcd = a.xsplit ? blockIdx.x % kXcd : 0u;
-  const WaveWs w = wave_ws(a, wid, lds_rings + (size_t)wib * a.nbk_lds);
+  const WaveWs w = wave_ws(a, wid, lds_rings + (size_t)wib * (a.nbk_lds + kFlushList));
   for (uint32_t r = lane; r < a.nbk_lds; r += kWave) lds_st(&w.ring[r], 0u);
   wave_fence();
   uint32_t epoch = a.epochs[wid];
@@ -806,23 +886,8 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
         st_within += tot;
         if ((j.flags & 5u) == 5u && lane == 0 && tot) atomicAdd(&j.reach[t], tot);
       }
-      const bool clo = (j.flags & 1u) != 0;
-      // ---- cycles term of closeness (centrality.rs:1764-1767): with betweenness it
-      // rides along the back-propagation walk below; slot 0 is the source
-      if (clo && !(j.flags & 2u)) {
-        for (uint32_t s = 1 + lane; s < R; s += kWave) {
-          const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
-          const uint32_t v = (uint32_t)lo;
-          const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
-          const uint32_t ts = j.tstride;
-          const size_t nacc = j.nacc;
-          double *accv = j.acc + v;
-#pragma unroll
-          for (int t = 0; t < NT; ++t)
-            if (route <= j.dist[t]) gadd_f64(&accv[((size_t)2 * ts + t) * nacc], (double)(rank[t] * cyc));
-        }
-      }
-      // ---- betweenness: reverse bucket order (centrality.rs:1783-1835, 823-873)
+      // ---- betweenness dependencies: reverse bucket order (centrality.rs:1783-1835,
+      // 823-873); each slot's accumulated delta is credited by the flush below
       if (j.flags & 2u) {
         // with Phase 2, a successor may share its predecessor's bucket: visit
         // each bucket level by level, highest first (levels are all 0 otherwise)
@@ -833,93 +898,69 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
           const uint32_t s0n = (b > 0) ? ld32(&w.bnd[b - 1]) : 0u;
           const uint32_t top = p2 ? uniform(ld32(&w.lvl[b])) : 0u;
           for (int L = (int)top; L >= 0; --L) {
-          for (uint32_t s = s0 + lane; s < s1; s += kWave) {
-            const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
-            const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
-            if (p2 && (uint32_t)(hi >> 48) != (uint32_t)L) continue;
-            const uint32_t v = (uint32_t)lo;
-            const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
-            double da[NT], db[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-              da[t] = db[t] = 0.0;
-              if (route <= j.dist[t]) {
-                da[t] = ldd(&w.acc[(size_t)s * 2 * NT + t]);
-                db[t] = ldd(&w.acc[(size_t)s * 2 * NT + NT + t]);
-              }
-            }
-            const uint32_t pslot0 = (uint32_t)hi;
-            const uint32_t meta = (uint32_t)(hi >> 32);
-            const uint32_t npred = meta & 0xffu;
-            const bool is_src = (s == 0);
-            const double pc = is_src ? 0.0 : ((meta & 0x100u) ? 0.5 : 1.0);
-            const uint32_t ts = j.tstride;
-            const size_t nacc = j.nacc;
-          double *accv = j.acc + v;
-            if (clo && !is_src) {
-#pragma unroll
-              for (int t = 0; t < NT; ++t)
-                if (route <= j.dist[t]) gadd_f64(&accv[((size_t)2 * ts + t) * nacc], (double)(rank[t] * cyc));
-            }
-            double dep[NT], dep_b[NT], seed_b[NT];
-            bool any = false;
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-              dep[t] = dep_b[t] = seed_b[t] = 0.0;
-              if (route <= j.dist[t]) {
-                seed_b[t] = pc * exp(-(double)j.beta[t] * (double)route);
-                dep[t] = pc + da[t];
-                dep_b[t] = seed_b[t] + db[t];
-                any = any || dep[t] != 0.0 || dep_b[t] != 0.0;
-              }
-            }
-            if (!any) continue;
-            if (npred == 1) {
-              // a single predecessor has sigma_u == sigma_w (sigma is the sum over
-              // the predecessors), so sigma_u / sigma_w is exactly 1
+            // the source (slot 0) has no predecessors
+            for (uint32_t s = (s0 ? s0 : 1u) + lane; s < s1; s += kWave) {
+              const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+              const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
+              if (p2 && (uint32_t)(hi >> 48) != (uint32_t)L) continue;
+              const uint32_t v = (uint32_t)lo;
+              const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
+              const uint32_t pslot0 = (uint32_t)hi;
+              const uint32_t meta = (uint32_t)(hi >> 32);
+              const uint32_t npred = meta & 0xffu;
+              const double pc = (meta & 0x100u) ? 0.5 : 1.0;
+              double dep[NT], dep_b[NT];
+              bool any = false;
 #pragma unroll
               for (int t = 0; t < NT; ++t) {
-                if (dep[t] != 0.0 || dep_b[t] != 0.0) {
-                  wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + t], dep[t]);
-                  wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + NT + t], dep_b[t]);
+                dep[t] = dep_b[t] = 0.0;
+                if (route <= j.dist[t]) {
+                  const double seed_b = pc * exp(-(double)j.beta[t] * (double)route);
+                  dep[t] = pc + ldd(&w.acc[(size_t)s * 2 * NT + t]);
+                  dep_b[t] = seed_b + ldd(&w.acc[(size_t)s * 2 * NT + NT + t]);
+                  any = any || dep[t] != 0.0 || dep_b[t] != 0.0;
                 }
               }
-            } else if (npred > 1) {
-              const uint64_t pm = ld64(&w.pm[s]);
-              const uint32_t f0 = j.rows[v].x;
-              const double sigma_w = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)v + 1]));
-              for (uint64_t mm = pm; mm; mm &= mm - 1) {
-                const uint32_t jb = __builtin_ctzll(mm);
-                const uint32_t u = j.fwd_adj[f0 + jb].x & kNodeMask;
-                const uint32_t su = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
-                const double sigma_u = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)u + 1]));
-                if (sigma_u == 0.0) continue;
-                const double fct = sigma_u / sigma_w;
+              if (!any) continue;
+              if (npred == 1) {
+                // a single predecessor has sigma_u == sigma_w (sigma is the sum over
+                // the predecessors), so sigma_u / sigma_w is exactly 1
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
                   if (dep[t] != 0.0 || dep_b[t] != 0.0) {
-                    wadd_f64(&w.acc[(size_t)su * 2 * NT + t], fct * dep[t]);
-                    wadd_f64(&w.acc[(size_t)su * 2 * NT + NT + t], fct * dep_b[t]);
+                    wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + t], dep[t]);
+                    wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + NT + t], dep_b[t]);
+                  }
+                }
+              } else if (npred > 1) {
+                const uint64_t pm = ld64(&w.pm[s]);
+                const uint32_t f0 = j.rows[v].x;
+                const double sigma_w = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)v + 1]));
+                for (uint64_t mm = pm; mm; mm &= mm - 1) {
+                  const uint32_t jb = __builtin_ctzll(mm);
+                  const uint32_t u = j.fwd_adj[f0 + jb].x & kNodeMask;
+                  const uint32_t su = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
+                  const double sigma_u = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)u + 1]));
+                  if (sigma_u == 0.0) continue;
+                  const double fct = sigma_u / sigma_w;
+#pragma unroll
+                  for (int t = 0; t < NT; ++t) {
+                    if (dep[t] != 0.0 || dep_b[t] != 0.0) {
+                      wadd_f64(&w.acc[(size_t)su * 2 * NT + t], fct * dep[t]);
+                      wadd_f64(&w.acc[(size_t)su * 2 * NT + NT + t], fct * dep_b[t]);
+                    }
                   }
                 }
               }
             }
-            if (is_src) continue;
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-              if (dep[t] == 0.0 && dep_b[t] == 0.0) continue;
-              const double credit = dep[t] - pc;
-              const double credit_b = dep_b[t] - seed_b[t];
-              if (credit > 0.0) gadd_f64(&accv[((size_t)5 * ts + t) * nacc], credit * (double)wt);
-              if (credit_b > 0.0) gadd_f64(&accv[((size_t)6 * ts + t) * nacc], credit_b * (double)wt);
-            }
-          }
-          wave_fence();
+            wave_fence();
           }
           s1 = s0;
           s0 = uniform(s0n);
         }
       }
+      // ---- every target's closeness terms and betweenness credit, in node order
+      flush_targets<NT>(j, w, c, R, rank, wt, cyc);
     } else if constexpr (MODE == kSegment) {
       // ---- closeness at the source over visited edges (centrality.rs:2201-2317)
       if (j.flags & 1u) {
@@ -1372,7 +1413,7 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   j.inv_delta = 1.0f / delta;
   const uint32_t jump = (uint32_t)std::ceil((double)de.max_es / (double)delta) + 2;
   j.nbk = jump + 1;
-  if (j.nbk > 4096) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
+  if (j.nbk > 4096 - kFlushList) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
   j.maxb = (uint32_t)((double)cs.maxsec * (double)j.inv_delta) + 4;
   j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
   j.n_sources = js.n_sources;
@@ -1419,7 +1460,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
   const uint32_t wpb = block / kWave;
   p.bcap = 1024;
   p.cap = std::min<uint32_t>(maxN, tune.cap ? tune.cap : 16384);
-  const size_t lds = (size_t)wpb * p.nbk * 4;
+  const size_t lds = (size_t)wpb * (p.nbk + kFlushList) * 4;
   const int bpc = std::max(1, blocks_per_cu<MODE>(NT, block, lds));
   uint32_t resident = di.cus * (uint32_t)bpc * wpb;
   if (tune.waves_per_simd) resident = std::min<uint32_t>(resident, di.cus * 4 * tune.waves_per_simd);
@@ -1626,7 +1667,7 @@ struct ShortestBatch {
     wpb = block / kWave;
     p.bcap = 1024;
     p.cap = std::min<uint32_t>(maxN, tune.cap ? tune.cap : 16384);
-    lds = (size_t)wpb * p.nbk * 4;
+    lds = (size_t)wpb * (p.nbk + kFlushList) * 4;
     const DevInfo &di = dev_info(device);
     const int bpc = std::max(1, blocks_per_cu<kBrandes>(NT, block, lds));
     uint32_t resident = di.cus * (uint32_t)bpc * wpb;
