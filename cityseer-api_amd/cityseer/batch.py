@@ -32,11 +32,21 @@ class CentralityBatch:
         self._h = h
         self._jobs: list[dict] = []
 
-    def __del__(self) -> None:
+    def close(self) -> None:
+        """Free the batch's device memory (also done when the object is collected)."""
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             _native.load().cs_batch_destroy(h)
             self._h = None
+
+    def __del__(self) -> None:
+        self.close()
+
+    def __enter__(self) -> "CentralityBatch":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
 
     def add_shortest(
         self,
