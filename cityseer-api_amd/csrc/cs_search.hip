@@ -49,7 +49,10 @@ constexpr int kChunk = 1;      // edges per lane per batch: one (wider batches o
 constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dealt to them round-robin
 constexpr uint32_t kScanQ = 4;     // key words per lane per step of the node-ordered result flush
 constexpr uint32_t kFlushList = kScanQ * 64;  // LDS list of one flush step (reached nodes, index order)
+constexpr uint32_t kFlushWords = 3 * kFlushList;  // {node, seconds bits, slot} per list entry
 constexpr uint32_t kScanSpan = 16; // scan the index span only while it is at most this many times the reach
+constexpr uint32_t kEpochMax = 1023u;       // 10-bit epoch tags 1..1023; 0 is never issued
+constexpr uint32_t kSlotLimit = 1u << 22;   // 22-bit slots in the node key
 
 enum Mode { kBrandes = 0, kSegment = 1, kTree = 2, kBrandesExport = 3 };
 
@@ -89,11 +92,11 @@ struct SearchArgs {
   uint32_t n_items;
   uint32_t bcap, cap, nbk_lds;
   unsigned long long *stats;  // reached, edges, within
-  uint64_t *keys;   // [waves][ncap] epoch-tagged keys
+  uint64_t *keys;   // [waves][ncap][2] node records {epoch-tagged key, payload}
   size_t ncap;
   char *ws;         // [waves][ws_stride] scratch
   size_t ws_stride;
-  size_t o_ninfo, o_rec, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
+  size_t o_rec, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
   uint32_t *epochs;
   uint32_t *next;            // [kXcd] per-XCD item counters
   uint32_t xbeg[kXcd + 1];   // XCD x owns item indices [xbeg[x], xbeg[x+1])
@@ -160,14 +163,19 @@ __device__ __forceinline__ double wave_sum(double v) { return __ockl_wfred_add_f
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t wave_or(uint32_t v) { return __ockl_wfred_or_u32(v); }
 
+// Node key, one 64-bit word: epoch (10 bits) | ~f32 bits of the seconds (32) |
+// slot (22, zero until the node settles). A 64-bit atomic max keeps the newest
+// epoch and then the smallest seconds; a relaxation carries slot 0, so it can
+// never displace a settled key of equal seconds.
 __device__ __forceinline__ uint64_t pack_key(uint32_t epoch, float secs) {
-  return ((uint64_t)epoch << 32) | (uint32_t)(~__float_as_uint(secs));
+  return ((uint64_t)epoch << 54) | ((uint64_t)(uint32_t)~__float_as_uint(secs) << 22);
 }
 __device__ __forceinline__ uint64_t pack_key_bits(uint32_t epoch, uint32_t secs_bits) {
-  return ((uint64_t)epoch << 32) | (uint32_t)(~secs_bits);
+  return ((uint64_t)epoch << 54) | ((uint64_t)(uint32_t)~secs_bits << 22);
 }
-__device__ __forceinline__ bool key_current(uint64_t k, uint32_t epoch) { return (uint32_t)(k >> 32) == epoch; }
-__device__ __forceinline__ float key_secs(uint64_t k) { return __uint_as_float(~(uint32_t)k); }
+__device__ __forceinline__ bool key_current(uint64_t k, uint32_t epoch) { return (uint32_t)(k >> 54) == epoch; }
+__device__ __forceinline__ float key_secs(uint64_t k) { return __uint_as_float(~(uint32_t)(k >> 22)); }
+__device__ __forceinline__ uint32_t key_slot(uint64_t k) { return (uint32_t)k & (kSlotLimit - 1u); }
 
 __device__ __forceinline__ uint32_t bucket_of(float s, float inv_delta) {
   return (uint32_t)(s * inv_delta);
@@ -184,9 +192,8 @@ __device__ __forceinline__ bool key_less(float sa, uint32_t a, float sb, uint32_
 //   hi = Brandes: first predecessor slot | (n_preds | eligible << 8) << 32
 //        trees:   predecessor slot | row position of the tree edge << 32
 struct WaveWs {
-  uint64_t *keyw;   // [N] epoch-tagged keys
-  uint64_t *ninfo;  // [N][2] of a settled node: {slot, payload}, payload = sigma bits (Brandes)
-                    // or origin-segment length bits (trees)
+  uint64_t *node;   // [N][2] node records {key, payload}: one 16-byte line segment per node;
+                    // payload = sigma bits (Brandes) or origin-segment length bits (trees)
   uint64_t *rec;    // [cap][2]
   uint64_t *pm;     // [cap] predecessor row mask (Brandes) / {last len, origin len} (trees)
   double *acc;      // [cap][2T] dependencies (Brandes) / [cap][T] subtree sums (segment)
@@ -194,14 +201,13 @@ struct WaveWs {
   uint32_t *bnd;    // [maxb + 2] slot bounds per opened bucket
   uint32_t *lvl;    // [maxb + 2] highest Phase 2 resolution round per bucket
   uint32_t *ring;   // LDS [nbk] bucket ring counters
-  uint32_t *list;   // LDS [kFlushList] node list of the result flush
+  uint32_t *list;   // LDS [3][kFlushList] {node, seconds bits, slot} list of the result flush
 };
 
 __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uint32_t *ring) {
   char *base = a.ws + (size_t)wid * a.ws_stride;
   WaveWs w;
-  w.keyw = a.keys + (size_t)wid * a.ncap;
-  w.ninfo = reinterpret_cast<uint64_t *>(base + a.o_ninfo);
+  w.node = a.keys + (size_t)wid * a.ncap * 2;
   w.rec = reinterpret_cast<uint64_t *>(base + a.o_rec);
   w.pm = reinterpret_cast<uint64_t *>(base + a.o_pm);
   w.acc = reinterpret_cast<double *>(base + a.o_acc);
@@ -252,6 +258,15 @@ __device__ __forceinline__ double far_factor(uint32_t deg) {
 
 // Ordered fold over the near candidates of v (two or more): selection passes in
 // settle order (secs_u, u), ties of one u (parallel edges) in row order.
+__device__ __forceinline__ uint64_t *nkey(const WaveWs &w, uint32_t v) { return &w.node[2 * (size_t)v]; }
+__device__ __forceinline__ uint64_t *npay(const WaveWs &w, uint32_t v) { return &w.node[2 * (size_t)v + 1]; }
+// Key and payload of one node in one 16-byte load.
+__device__ __forceinline__ uint4 nrec(const WaveWs &w, uint32_t v) {
+  return *reinterpret_cast<const uint4 *>(&w.node[2 * (size_t)v]);
+}
+__device__ __forceinline__ uint64_t lo64(uint4 r) { return ((uint64_t)r.y << 32) | r.x; }
+__device__ __forceinline__ uint64_t hi64(uint4 r) { return ((uint64_t)r.w << 32) | r.z; }
+
 __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint32_t v, float secs_v,
                                    uint32_t kb, uint32_t r0, uint32_t r1, double thr_far, uint32_t n_near,
                                    Fold &f) {
@@ -269,7 +284,7 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
       const uint2 e = j.fwd_adj[p];
       const uint32_t u = e.x & kNodeMask;
       if (u == v) continue;
-      const uint64_t ku = ld64(&w.keyw[u]);
+      const uint64_t ku = ld64(nkey(w, u));
       if (!key_current(ku, tag)) continue;
       const float su = key_secs(ku);
       if (bucket_of(su, j.inv_delta) >= kb) continue;
@@ -291,7 +306,7 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
       const uint32_t jb = __builtin_ctzll(m);
       if ((j.fwd_adj[r0 + jb].x & kNodeMask) == pu) contains = true;
     }
-    const double sg = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)pu + 1]));
+    const double sg = ldd(reinterpret_cast<const double *>(npay(w, pu)));
     fold_apply(f, pcs, sg, pj, contains);
     have_last = true;
     lsu = psu;
@@ -329,7 +344,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
   const bool phase2 = kMulti && j.tol > kTieEps;
   const bool fold = ((MODE != kBrandes) || (j.flags & 2u)) && !phase2;
   if (lane == 0) {
-    st64(&w.keyw[c.src], pack_key(c.tag, 0.0f));
+    st64(nkey(w, c.src), pack_key(c.tag, 0.0f));
     st64(reinterpret_cast<uint64_t *>(&w.bkt[0]), (uint64_t)c.src);
     lds_st(&w.ring[0], 1u);
   }
@@ -370,7 +385,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
         const uint64_t ent = ld64(reinterpret_cast<const uint64_t *>(&w.bkt[ring * a.bcap + i]));
         v = (uint32_t)ent;
         csb = (uint32_t)(ent >> 32);
-        kv = ld64(&w.keyw[v]);
+        kv = ld64(nkey(w, v));
         rw = j.rows[v];
         if constexpr (MODE == kBrandes) el = j.eligible[v];
       }
@@ -418,10 +433,14 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             fs[q] = 0;
             fp[q] = 0;
             if (!is_src && u != v && (fold || (cycles && (fe[q].x & kCanonBit)))) {
-              fk[q] = ld64(&w.keyw[u]);
               if (fold) {
-                fs[q] = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
-                fp[q] = ld64(&w.ninfo[2 * (size_t)u + 1]);
+                // key, slot and payload of the candidate in one load
+                const uint4 r = nrec(w, u);
+                fk[q] = lo64(r);
+                fs[q] = key_slot(fk[q]);
+                fp[q] = hi64(r);
+              } else {
+                fk[q] = ld64(nkey(w, u));
               }
             }
             const uint32_t nb = re[q].x & kNodeMask;
@@ -429,9 +448,9 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             rk[q] = 0;
             if (nb != v) {
               if (cs <= j.maxsec)
-                rk[q] = wmax64(&w.keyw[nb], pack_key(c.tag, cs));
+                rk[q] = wmax64(nkey(w, nb), pack_key(c.tag, cs));
               else if (cycles && (re[q].x & kCanonBit))
-                rk[q] = ld64(&w.keyw[nb]);
+                rk[q] = ld64(nkey(w, nb));
             }
           }
           // fold candidates v -> u (u settled in an earlier bucket)
@@ -520,11 +539,17 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             } else {
               npred = (uint32_t)__popcll(f.pmask);
               const uint32_t u0 = j.fwd_adj[f0 + __builtin_ctzll(f.pmask)].x & kNodeMask;
-              pslot0 = (uint32_t)ld64(&w.ninfo[2 * (size_t)u0]);
+              pslot0 = key_slot(ld64(nkey(w, u0)));
             }
           }
-          st64(&w.ninfo[2 * (size_t)v], s);
-          std_(reinterpret_cast<double *>(&w.ninfo[2 * (size_t)v + 1]), f.sigma);
+          // the node's key gains its slot, the payload its sigma (read by folds,
+          // Phase 2 and back-propagation, none of which runs for closeness
+          // alone); relaxations still in flight carry larger seconds and leave
+          // the key unchanged
+          if (MODE != kBrandes || (j.flags & 2u)) {
+            if (s) st64(nkey(w, v), kv | s);
+            std_(reinterpret_cast<double *>(npay(w, v)), f.sigma);
+          }
           // the slot record is read only after this bucket's fence: one plain 16-byte store
           *reinterpret_cast<uint4 *>(&w.rec[2 * (size_t)s]) =
               make_uint4(v, csb, pslot0, npred | (el ? 0x100u : 0u));
@@ -557,8 +582,8 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
               olen = (pslot == 0) ? llen : __uint_as_float((uint32_t)bpay);
             }
           }
-          st64(&w.ninfo[2 * (size_t)v], s);
-          st64(&w.ninfo[2 * (size_t)v + 1], __float_as_uint(olen));
+          if (s) st64(nkey(w, v), kv | s);
+          st64(npay(w, v), __float_as_uint(olen));
           st64(&w.rec[2 * (size_t)s], ((uint64_t)csb << 32) | v);
           st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)bj << 32) | pslot);
           st64(&w.pm[s], ((uint64_t)__float_as_uint(olen) << 32) | __float_as_uint(llen));
@@ -647,7 +672,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
       uint64_t pm = 0;
       if (s == 0) {
         meta |= kP2Done;
-        std_(reinterpret_cast<double *>(&w.ninfo[2 * (size_t)v + 1]), 1.0);
+        std_(reinterpret_cast<double *>(npay(w, v)), 1.0);
       } else {
         const uint4 rw = j.rows[v];
         const float lim = secs_v * band;
@@ -655,7 +680,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
           const uint2 e = j.fwd_adj[p];
           const uint32_t u = e.x & kNodeMask;
           if (u == v) continue;
-          const uint64_t ku = ld64(&w.keyw[u]);
+          const uint64_t ku = ld64(nkey(w, u));
           if (!key_current(ku, c.tag)) continue;
           const float su = key_secs(ku);
           if (!key_less(su, u, secs_v, v)) continue;
@@ -683,7 +708,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
         bool ready = true;
         for (uint64_t mm = pm; mm; mm &= mm - 1) {
           const uint32_t u = j.fwd_adj[f0 + __builtin_ctzll(mm)].x & kNodeMask;
-          const uint32_t su = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
+          const uint32_t su = key_slot(ld64(nkey(w, u)));
           const uint32_t mu = (uint32_t)(ld64(&w.rec[2 * (size_t)su + 1]) >> 32);
           // same-bucket predecessors must have resolved in an earlier round
           if (!(mu & kP2Done) || (su >= s0 && su < s1 && (mu >> 16) >= round)) ready = false;
@@ -705,7 +730,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
           uint32_t bu = 0;
           for (uint64_t mm = pm; mm; mm &= mm - 1) {
             const uint32_t u = j.fwd_adj[f0 + __builtin_ctzll(mm)].x & kNodeMask;
-            const float su = key_secs(ld64(&w.keyw[u]));
+            const float su = key_secs(ld64(nkey(w, u)));
             if (have_last && !key_less(lsu, lu, su, u)) continue;
             if (!found || key_less(su, u, bsu, bu)) {
               found = true;
@@ -713,13 +738,13 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
               bu = u;
             }
           }
-          sigma += ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)bu + 1]));
+          sigma += ldd(reinterpret_cast<const double *>(npay(w, bu)));
           have_last = true;
           lsu = bsu;
           lu = bu;
         }
-        if (pm) pslot0 = (uint32_t)ld64(&w.ninfo[2 * (size_t)(j.fwd_adj[f0 + __builtin_ctzll(pm)].x & kNodeMask)]);
-        std_(reinterpret_cast<double *>(&w.ninfo[2 * (size_t)v + 1]), sigma);
+        if (pm) pslot0 = key_slot(ld64(nkey(w, (j.fwd_adj[f0 + __builtin_ctzll(pm)].x & kNodeMask))));
+        std_(reinterpret_cast<double *>(npay(w, v)), sigma);
         st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np | (round << 16)) << 32) | pslot0);
       }
       pending = uniform(wave_sum(pending));
@@ -740,16 +765,16 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
 //
 // The settled targets are visited in node-index order where that is cheap: the
 // wave scans the epoch-tagged keys over [vlo, vhi], compacts the reached ones
-// into an LDS list and hands lane i the i-th, so the lanes of one atomic
-// instruction hit neighbouring addresses of a metric row (a few cache lines)
-// rather than 64 scattered ones. A sparse span falls back to slot order.
+// (node, seconds, slot) into an LDS list and hands lane i the i-th, so the
+// lanes of one atomic instruction hit neighbouring addresses of a metric row
+// (a few cache lines) rather than 64 scattered ones. The route comes from the
+// key and the eligibility from the graph, so closeness reads no slot record.
+// A sparse span falls back to slot order.
 // ---------------------------------------------------------------------------
 template <int NT>
-__device__ __forceinline__ void flush_target(CJob &j, const WaveWs &w, uint32_t s, uint32_t v,
+__device__ __forceinline__ void flush_target(CJob &j, const WaveWs &w, uint32_t v, uint32_t secs_bits, uint32_t s,
                                              const float (&rank)[NT], float wt, float cyc) {
-  const uint4 r = *reinterpret_cast<const uint4 *>(&w.rec[2 * (size_t)s]);
-  const float route = __uint_as_float(r.y) * j.speed;
-  const double pc = (r.w & 0x100u) ? 0.5 : 1.0;
+  const float route = __uint_as_float(secs_bits) * j.speed;
   const size_t ts = j.tstride, nacc = j.nacc;
   double *accv = j.acc + v;
   if (j.flags & 1u) {
@@ -765,6 +790,7 @@ __device__ __forceinline__ void flush_target(CJob &j, const WaveWs &w, uint32_t 
     }
   }
   if (j.flags & 2u) {
+    const double pc = j.eligible[v] ? 0.5 : 1.0;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       if (route <= j.dist[t]) {
@@ -787,8 +813,8 @@ __device__ void flush_targets(CJob &j, const WaveWs &w, const SrcCtx &c, uint32_
   const uint32_t lo = c.vlo, hi = c.vhi;
   if ((uint64_t)(hi - lo) + 1 > (uint64_t)kScanSpan * R) {
     for (uint32_t s = 1 + lane; s < R; s += kWave) {
-      const uint32_t v = (uint32_t)ld64(&w.rec[2 * (size_t)s]);
-      flush_target<NT>(j, w, s, v, rank, wt, cyc);
+      const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+      flush_target<NT>(j, w, (uint32_t)lo, (uint32_t)(lo >> 32), s, rank, wt, cyc);
     }
     return;
   }
@@ -797,7 +823,7 @@ __device__ void flush_targets(CJob &j, const WaveWs &w, const SrcCtx &c, uint32_
 #pragma unroll
     for (uint32_t q = 0; q < kScanQ; ++q) {
       const uint64_t v = base + q * kWave + lane;
-      key[q] = (v <= hi) ? ld64(&w.keyw[v]) : 0ull;
+      key[q] = (v <= hi) ? ld64(nkey(w, v)) : 0ull;
     }
     uint32_t n = 0;
 #pragma unroll
@@ -805,15 +831,18 @@ __device__ void flush_targets(CJob &j, const WaveWs &w, const SrcCtx &c, uint32_
       const uint32_t v = (uint32_t)(base + q * kWave + lane);
       const bool hit = key_current(key[q], c.tag) && v != c.src;
       const uint64_t m = __ballot(hit);
-      if (hit) lds_st(&w.list[n + rank_below(m)], v);
+      if (hit) {
+        const uint32_t i = n + rank_below(m);
+        lds_st(&w.list[i], v);
+        lds_st(&w.list[kFlushList + i], ~(uint32_t)(key[q] >> 22));
+        lds_st(&w.list[2 * kFlushList + i], key_slot(key[q]));
+      }
       n += (uint32_t)__popcll(m);
     }
     wave_fence();
-    for (uint32_t i = lane; i < n; i += kWave) {
-      const uint32_t v = lds_ld(&w.list[i]);
-      const uint32_t s = (uint32_t)ld64(&w.ninfo[2 * (size_t)v]);
-      flush_target<NT>(j, w, s, v, rank, wt, cyc);
-    }
+    for (uint32_t i = lane; i < n; i += kWave)
+      flush_target<NT>(j, w, lds_ld(&w.list[i]), lds_ld(&w.list[kFlushList + i]), lds_ld(&w.list[2 * kFlushList + i]),
+                       rank, wt, cyc);
     wave_fence();
   }
 }
@@ -828,7 +857,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
   const uint32_t wib = uniform(threadIdx.x / kWave);
   const uint32_t wid = uniform(blockIdx.x * (blockDim.x / kWave) + wib);
   const uint32_t xcd = a.xsplit ? blockIdx.x % kXcd : 0u;
-  const WaveWs w = wave_ws(a, wid, lds_rings + (size_t)wib * (a.nbk_lds + kFlushList));
+  const WaveWs w = wave_ws(a, wid, lds_rings + (size_t)wib * (a.nbk_lds + kFlushWords));
   for (uint32_t r = lane; r < a.nbk_lds; r += kWave) lds_st(&w.ring[r], 0u);
   wave_fence();
   uint32_t epoch = a.epochs[wid];
@@ -848,6 +877,13 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
     SrcCtx c;
     c.src = j.sources[si];
     c.wt = (MODE == kBrandes) ? j.src_wt[si] : 0.0f;
+    if (epoch >= kEpochMax) {
+      // the 10-bit tags are used up: clear this wave's node records and start over
+      uint4 *nr = reinterpret_cast<uint4 *>(w.node);
+      for (size_t i = lane; i < a.ncap; i += kWave) nr[i] = make_uint4(0u, 0u, 0u, 0u);
+      wave_fence();
+      epoch = 0;
+    }
     ++epoch;
     c.tag = epoch;
     uint32_t nband[NT], eband[NT];
@@ -935,12 +971,13 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
               } else if (npred > 1) {
                 const uint64_t pm = ld64(&w.pm[s]);
                 const uint32_t f0 = j.rows[v].x;
-                const double sigma_w = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)v + 1]));
+                const double sigma_w = ldd(reinterpret_cast<const double *>(npay(w, v)));
                 for (uint64_t mm = pm; mm; mm &= mm - 1) {
                   const uint32_t jb = __builtin_ctzll(mm);
                   const uint32_t u = j.fwd_adj[f0 + jb].x & kNodeMask;
-                  const uint32_t su = (uint32_t)ld64(&w.ninfo[2 * (size_t)u]);
-                  const double sigma_u = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)u + 1]));
+                  const uint4 ru = nrec(w, u);
+                  const uint32_t su = key_slot(lo64(ru));
+                  const double sigma_u = __longlong_as_double((long long)hi64(ru));
                   if (sigma_u == 0.0) continue;
                   const double fct = sigma_u / sigma_w;
 #pragma unroll
@@ -987,7 +1024,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
 #pragma unroll
             for (int q = 0; q < kChunk; ++q) {
               const uint32_t nb = re[q].x & kNodeMask;
-              kn[q] = (nb != v) ? ld64(&w.keyw[nb]) : 0;
+              kn[q] = (nb != v) ? ld64(nkey(w, nb)) : 0;
             }
 #pragma unroll
             for (int q = 0; q < kChunk; ++q) {
@@ -1124,7 +1161,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
           }
         } else if constexpr (MODE == kBrandesExport) {
           const uint64_t pm = ld64(&w.pm[s]);
-          a.tout.sigma[v] = ldd(reinterpret_cast<const double *>(&w.ninfo[2 * (size_t)v + 1]));
+          a.tout.sigma[v] = ldd(reinterpret_cast<const double *>(npay(w, v)));
           uint32_t np = 0;
           const uint32_t f0 = j.rows[v].x;
           for (uint64_t mm = pm; mm; mm &= mm - 1) {
@@ -1207,7 +1244,7 @@ int blocks_per_cu(uint32_t T, uint32_t block, size_t lds) {
 struct Plan {
   uint32_t bcap, cap, maxb, nbk, waves;
   size_t stride;
-  size_t o_ninfo, o_rec, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
+  size_t o_rec, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
 };
 
 // Per-wave scratch layout. Everything here is written before it is read
@@ -1215,8 +1252,6 @@ struct Plan {
 // separate arena) carry state across sources.
 void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   size_t o = 0;
-  p.o_ninfo = o;
-  o = align256(o + 16ull * N);
   p.o_rec = o;
   o = align256(o + 16ull * p.cap);
   p.o_pm = o;
@@ -1275,8 +1310,9 @@ struct DevVec {
 
 struct Workspace {
   int device = -1;
-  // epoch-tagged keys, [key_waves][ncap]: zeroed whenever (re)allocated or
-  // when the epochs would wrap; keys of earlier graphs are older epochs
+  // node records {epoch-tagged key, payload}, [key_waves][ncap][2]: zeroed
+  // whenever (re)allocated; each wave clears its own when its 10-bit epochs
+  // wrap, and records of earlier graphs carry older epochs
   uint64_t *keys = nullptr;
   uint32_t ncap = 0, key_waves = 0;
   uint32_t *epochs = nullptr;
@@ -1320,7 +1356,7 @@ struct WsGuard {
 size_t ws_growth(const Workspace &w, uint32_t N, size_t stride, uint32_t waves) {
   size_t g = 0;
   if (N > w.ncap || waves > w.key_waves)
-    g += 8ull * std::max(N, w.ncap) * std::max(waves, w.key_waves);
+    g += 16ull * std::max(N, w.ncap) * std::max(waves, w.key_waves);
   if (stride * waves > w.scratch.bytes) g += stride * waves;
   return g;
 }
@@ -1337,10 +1373,10 @@ void prepare_ws(Workspace &w, uint32_t N, size_t stride, uint32_t waves, uint64_
     w.keys = nullptr;
     w.epochs = nullptr;
     w.ncap = w.key_waves = 0;
-    if (hipMalloc(&w.keys, 8ull * ncap * kw) != hipSuccess) {
+    if (hipMalloc(&w.keys, 16ull * ncap * kw) != hipSuccess) {
       (void)hipGetLastError();
       w.keys = nullptr;
-      fail(CS_ENOMEM, "could not allocate " + std::to_string((8ull * ncap * kw) >> 20) + " MiB of search keys");
+      fail(CS_ENOMEM, "could not allocate " + std::to_string((16ull * ncap * kw) >> 20) + " MiB of node records");
     }
     hip_check(hipMalloc(&w.epochs, 4ull * kw), "hipMalloc epochs");
     w.ncap = ncap;
@@ -1350,7 +1386,7 @@ void prepare_ws(Workspace &w, uint32_t N, size_t stride, uint32_t waves, uint64_
   if (w.epoch_hi + sources + 2 >= 0xfffffff0ull) clear = true;
   if (clear) {
     // every key carries epoch 0, which is never issued
-    hip_check(hipMemsetAsync(w.keys, 0, 8ull * w.ncap * w.key_waves, stream), "hipMemset keys");
+    hip_check(hipMemsetAsync(w.keys, 0, 16ull * w.ncap * w.key_waves, stream), "hipMemset keys");
     hip_check(hipMemsetAsync(w.epochs, 0, 4ull * w.key_waves, stream), "hipMemset epochs");
     w.epoch_hi = 0;
   }
@@ -1413,7 +1449,7 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   j.inv_delta = 1.0f / delta;
   const uint32_t jump = (uint32_t)std::ceil((double)de.max_es / (double)delta) + 2;
   j.nbk = jump + 1;
-  if (j.nbk > 4096 - kFlushList) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
+  if (j.nbk > 4096 - kFlushWords) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
   j.maxb = (uint32_t)((double)cs.maxsec * (double)j.inv_delta) + 4;
   j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
   j.n_sources = js.n_sources;
@@ -1459,8 +1495,8 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
   if (block % kWave || block > (uint32_t)kBlock) fail(CS_EINVAL, "workgroup size must be 64, 128, 192 or 256");
   const uint32_t wpb = block / kWave;
   p.bcap = 1024;
-  p.cap = std::min<uint32_t>(maxN, tune.cap ? tune.cap : 16384);
-  const size_t lds = (size_t)wpb * (p.nbk + kFlushList) * 4;
+  p.cap = std::min<uint32_t>({maxN, tune.cap ? tune.cap : 16384u, kSlotLimit});
+  const size_t lds = (size_t)wpb * (p.nbk + kFlushWords) * 4;
   const int bpc = std::max(1, blocks_per_cu<MODE>(NT, block, lds));
   uint32_t resident = di.cus * (uint32_t)bpc * wpb;
   if (tune.waves_per_simd) resident = std::min<uint32_t>(resident, di.cus * 4 * tune.waves_per_simd);
@@ -1503,8 +1539,8 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     if (ws_growth(*wsp, maxN, p.stride, want) > 0) {
       size_t free_b = 0, total_b = 0;
       hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-      const size_t have = wsp->scratch.bytes + 8ull * wsp->ncap * wsp->key_waves;
-      const size_t per_wave = p.stride + 8ull * maxN;
+      const size_t have = wsp->scratch.bytes + 16ull * wsp->ncap * wsp->key_waves;
+      const size_t per_wave = p.stride + 16ull * maxN;
       by_mem = (uint32_t)std::min<size_t>((free_b / 4 + have) / per_wave, want);
     }
     if (by_mem < wpb) {
@@ -1532,7 +1568,6 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.ncap = wsp->ncap;
     a.ws = static_cast<char *>(wsp->scratch.p);
     a.ws_stride = p.stride;
-    a.o_ninfo = p.o_ninfo;
     a.o_rec = p.o_rec;
     a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
@@ -1574,7 +1609,9 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     hip_check(hipMemcpy(d_items, items.data(), sizeof(uint2) * n_failed, hipMemcpyHostToDevice), "copy");
     list_items = true;
     n_list = n_failed;
-    if (flags & kFailCap) p.cap = std::min<uint32_t>(maxN, p.cap * 4);
+    if ((flags & kFailCap) && p.cap >= kSlotLimit)
+      fail(CS_EINVAL, "a source reaches more than 4,194,304 nodes, the search's slot limit");
+    if (flags & kFailCap) p.cap = std::min<uint32_t>({maxN, p.cap * 4, kSlotLimit});
     if (flags & kFailBucket) p.bcap *= 4;
     if (attempt == 5) fail(CS_EINTERNAL, "search workspace retries exhausted");
   }
@@ -1666,8 +1703,8 @@ struct ShortestBatch {
     if (block % kWave || block > (uint32_t)kBlock) fail(CS_EINVAL, "workgroup size must be 64, 128, 192 or 256");
     wpb = block / kWave;
     p.bcap = 1024;
-    p.cap = std::min<uint32_t>(maxN, tune.cap ? tune.cap : 16384);
-    lds = (size_t)wpb * (p.nbk + kFlushList) * 4;
+    p.cap = std::min<uint32_t>({maxN, tune.cap ? tune.cap : 16384u, kSlotLimit});
+    lds = (size_t)wpb * (p.nbk + kFlushWords) * 4;
     const DevInfo &di = dev_info(device);
     const int bpc = std::max(1, blocks_per_cu<kBrandes>(NT, block, lds));
     uint32_t resident = di.cus * (uint32_t)bpc * wpb;
@@ -1675,7 +1712,7 @@ struct ShortestBatch {
     layout(p, maxN, NT, true);
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    const uint32_t by_mem = (uint32_t)std::min<size_t>((free_b / 4) / (p.stride + 8ull * maxN), 1u << 20);
+    const uint32_t by_mem = (uint32_t)std::min<size_t>((free_b / 4) / (p.stride + 16ull * maxN), 1u << 20);
     if (by_mem < wpb) fail(CS_ENOMEM, "batch too large for the search workspace on this device");
     waves = std::min<uint32_t>((uint32_t)std::min<uint64_t>(n_items, resident), by_mem);
     waves = std::max<uint32_t>(wpb, (waves + wpb - 1) / wpb * wpb);
@@ -1728,7 +1765,6 @@ struct ShortestBatch {
     a.ncap = ws.ncap;
     a.ws = static_cast<char *>(ws.scratch.p);
     a.ws_stride = p.stride;
-    a.o_ninfo = p.o_ninfo;
     a.o_rec = p.o_rec;
     a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
@@ -1751,7 +1787,7 @@ struct ShortestBatch {
     if (n_items == 0) return;
     // epochs: every wave may use up to n_items of them per run
     if (ws.epoch_hi + n_items + 2 >= 0xfffffff0ull) {
-      hip_check(hipMemsetAsync(ws.keys, 0, 8ull * ws.ncap * ws.key_waves, stream), "hipMemset keys");
+      hip_check(hipMemsetAsync(ws.keys, 0, 16ull * ws.ncap * ws.key_waves, stream), "hipMemset keys");
       hip_check(hipMemsetAsync(ws.epochs, 0, 4ull * ws.key_waves, stream), "hipMemset epochs");
       ws.epoch_hi = 0;
     }
@@ -1786,7 +1822,8 @@ struct ShortestBatch {
     std::vector<uint2> retry(n_failed);
     for (uint32_t i = 0; i < n_failed; ++i) retry[i] = items.empty() ? make_uint2(0u, fl[i]) : items[fl[i]];
     Tuning t2 = tune;
-    t2.cap = std::max<uint32_t>(p.cap * 4, 4096);
+    if (p.cap >= kSlotLimit) fail(CS_EINVAL, "a source reaches more than 4,194,304 nodes, the search's slot limit");
+    t2.cap = std::min<uint32_t>(std::max<uint32_t>(p.cap * 4, 4096), kSlotLimit);
     run_jobs<kBrandes>(specs, NT, t2, nullptr, stream, st, &retry);
   }
 

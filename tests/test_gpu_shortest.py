@@ -232,3 +232,20 @@ def test_cfg2_subset_and_full_size_properties():
     c = stack(ns2.centrality_shortest(distances=d), SHORTEST_KEYS, d)
     scale = np.array([2, 2, 1, 2, 2, 2, 2], dtype=np.float64)[:, None, None]
     assert np.allclose(c, a * scale, rtol=1e-12, atol=0)
+
+
+def test_epoch_wrap_keeps_results_exact():
+    """Each wave tags its node records with 10-bit epochs and clears them when the tags run out.
+    1,300 back-to-back calls of 144 sources each (144 waves, one source each on average) take
+    every wave through a wrap; every call must return the first call's result."""
+    lat = mock.perturbed_lattice(12, seed=5, diag_frac=0.1)
+    ns = mock.network_structure_from_lattice(lat)
+    og = OracleGraph.from_network_structure(ns)
+    d = [300, 700]
+    first, _ = check_shortest(ns, og, d)
+    want = stack(first, SHORTEST_KEYS, d)
+    say(f"{ns.node_count()} sources per call, 1300 calls")
+    for i in range(1300):
+        got = stack(ns.centrality_shortest(distances=d), SHORTEST_KEYS, d)
+        assert np.array_equal(got[0], want[0]), f"call {i}: density changed"
+        assert np.allclose(got, want, rtol=1e-12, atol=0), f"call {i}: metrics changed"
