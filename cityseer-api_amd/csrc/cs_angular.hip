@@ -79,48 +79,51 @@ __device__ __forceinline__ bool hless(float ka, float sa, uint32_t ia, float kb,
   return ia < ib;
 }
 
+// Per-lane 4-ary min-heap on (route, seconds at push, state). Every key is
+// distinct (a state is pushed again only with a strictly smaller route), so the
+// pop sequence is the key order whatever the heap's shape; four children per
+// level halve the depth of the dependent loads a pop walks down.
 struct Heap {
   char *w;
   size_t off;
   uint32_t lane;
   uint32_t n;
   __device__ __forceinline__ uint4 *at(uint32_t i) { return il<uint4>(w, off, i, lane); }
+  __device__ __forceinline__ static bool less(const uint4 &a, const uint4 &b) {
+    return hless(__uint_as_float(a.x), __uint_as_float(a.y), a.z, __uint_as_float(b.x), __uint_as_float(b.y), b.z);
+  }
   __device__ __forceinline__ void push(float k, float s, uint32_t idx) {
+    const uint4 e = make_uint4(__float_as_uint(k), __float_as_uint(s), idx, 0u);
     uint32_t i = n++;
     while (i > 0) {
-      const uint32_t p = (i - 1) / 2;
+      const uint32_t p = (i - 1) / 4;
       const uint4 pe = *at(p);
-      if (!hless(k, s, idx, __uint_as_float(pe.x), __uint_as_float(pe.y), pe.z)) break;
+      if (!less(e, pe)) break;
       *at(i) = pe;
       i = p;
     }
-    *at(i) = make_uint4(__float_as_uint(k), __float_as_uint(s), idx, 0u);
+    *at(i) = e;
   }
   __device__ __forceinline__ uint4 pop() {
     const uint4 top = *at(0);
     const uint4 last = *at(--n);
     uint32_t i = 0;
     for (;;) {
-      const uint32_t l = 2 * i + 1, r = l + 1;
-      uint32_t sidx = i;
-      uint4 best = last;
-      if (l < n) {
-        const uint4 le = *at(l);
-        if (hless(__uint_as_float(le.x), __uint_as_float(le.y), le.z, __uint_as_float(best.x),
-                  __uint_as_float(best.y), best.z)) {
-          sidx = l;
-          best = le;
+      const uint32_t c0 = 4 * i + 1;
+      if (c0 >= n) break;
+      // the (up to) four children are independent loads
+      uint4 ch[4];
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) ch[q] = (c0 + q < n) ? *at(c0 + q) : make_uint4(0u, 0u, 0u, 0u);
+      uint32_t sidx = c0;
+      uint4 best = ch[0];
+#pragma unroll
+      for (uint32_t q = 1; q < 4; ++q)
+        if (c0 + q < n && less(ch[q], best)) {
+          sidx = c0 + q;
+          best = ch[q];
         }
-      }
-      if (r < n) {
-        const uint4 re = *at(r);
-        if (hless(__uint_as_float(re.x), __uint_as_float(re.y), re.z, __uint_as_float(best.x),
-                  __uint_as_float(best.y), best.z)) {
-          sidx = r;
-          best = re;
-        }
-      }
-      if (sidx == i) break;
+      if (!less(best, last)) break;
       *at(i) = best;
       i = sidx;
     }
@@ -385,74 +388,89 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
         }
       }
     }
-    // ---- betweenness (centrality.rs:2037-2096, 823-873)
+    // ---- betweenness (centrality.rs:2037-2096, 823-873): every threshold in one
+    // walk over the settled states; each threshold keeps its own seeds and
+    // dependencies ([T][scap]) and sees its terms in the reference's order
     if (a.flags & 2u) {
-      for (uint32_t t = 0; t < T; ++t) {
-        const float sthr = a.sec_thr[t];
-        for (uint32_t q = 0; q < n_order; ++q) {
-          const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
-          *il<double>(w, a.o_delta, ss, lane) = 0.0;
-          *il<double>(w, a.o_seed, ss, lane) = 0.0;
+      const size_t sc = a.scap;
+      for (uint32_t q = 0; q < n_order; ++q) {
+        const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
+        for (uint32_t t = 0; t < T; ++t) {
+          *il<double>(w, a.o_delta, t * sc + ss, lane) = 0.0;
+          *il<double>(w, a.o_seed, t * sc + ss, lane) = 0.0;
         }
-        // seeds: best terminal states split by sigma (best_angular_target_states)
-        for (uint32_t k = 1; k < n_nodes; ++k) {
-          const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane);
-          if (v == src) continue;
-          const float br = *il<float>(w, a.o_brt, k, lane);
-          const float bs = *il<float>(w, a.o_bsec, k, lane);
-          if (!isfinite(br) || !isfinite(bs) || bs > sthr) continue;
+      }
+      // seeds: best terminal states split by sigma (best_angular_target_states)
+      for (uint32_t k = 1; k < n_nodes; ++k) {
+        const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane);
+        if (v == src) continue;
+        const float br = *il<float>(w, a.o_brt, k, lane);
+        const float bs = *il<float>(w, a.o_bsec, k, lane);
+        if (!isfinite(br) || !isfinite(bs)) continue;
+        uint32_t cand[2];
+        uint32_t ncand = 0;
+        for (uint32_t slot = 0; slot < 2; ++slot) {
+          const uint64_t sk = L.skey[2 * v + slot];
+          if ((uint32_t)(sk >> 32) != epoch) continue;
+          cand[ncand++] = (uint32_t)sk;
+        }
+        const double pc = a.eligible[v] ? 0.5 : 1.0;
+        for (uint32_t t = 0; t < T; ++t) {
+          const float sthr = a.sec_thr[t];
+          if (bs > sthr) continue;
           uint32_t best[2];
           uint32_t nbst = 0;
           double total = 0.0;
-          for (uint32_t slot = 0; slot < 2; ++slot) {
-            const uint64_t sk = L.skey[2 * v + slot];
-            if ((uint32_t)(sk >> 32) != epoch) continue;
-            const uint32_t ss = (uint32_t)sk;
+          for (uint32_t q = 0; q < ncand; ++q) {
+            const uint32_t ss = cand[q];
             const double sg = *il<double>(w, a.o_sigma, ss, lane);
             if (sg == 0.0 || *il<float>(w, a.o_secs, ss, lane) > sthr) continue;
-            if (*il<float>(w, a.o_route, ss, lane) <= br * (1.0f + a.tolerance)) {
-              best[nbst++] = ss;
-            }
+            if (*il<float>(w, a.o_route, ss, lane) <= br * (1.0f + a.tolerance)) best[nbst++] = ss;
           }
           if (nbst == 0) continue;
           for (uint32_t q = 0; q < nbst; ++q) total += *il<double>(w, a.o_sigma, best[q], lane);
           if (total == 0.0) continue;
-          const double pc = a.eligible[v] ? 0.5 : 1.0;
           for (uint32_t q = 0; q < nbst; ++q) {
-            double *sd = il<double>(w, a.o_seed, best[q], lane);
+            double *sd = il<double>(w, a.o_seed, t * sc + best[q], lane);
             *sd = *sd + pc * (*il<double>(w, a.o_sigma, best[q], lane) / total);
           }
         }
-        // states by route descending, equal routes in settle order (stable sort)
-        int hi = (int)n_order - 1;
-        while (hi >= 0) {
-          const float r_hi = *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, hi, lane), lane);
-          int lo = hi;
-          while (lo > 0 && *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, lo - 1, lane), lane) == r_hi) --lo;
-          for (int q = lo; q <= hi; ++q) {
-            const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
-            if (!(*il<float>(w, a.o_secs, ss, lane) <= sthr)) continue;
-            const double sw = *il<double>(w, a.o_sigma, ss, lane);
-            if (sw == 0.0) continue;
-            const double seed = *il<double>(w, a.o_seed, ss, lane);
-            const double dep = seed + *il<double>(w, a.o_delta, ss, lane);
+      }
+      // states by route descending, equal routes in settle order (stable sort)
+      int hi = (int)n_order - 1;
+      while (hi >= 0) {
+        const float r_hi = *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, hi, lane), lane);
+        int lo = hi;
+        while (lo > 0 && *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, lo - 1, lane), lane) == r_hi) --lo;
+        for (int q = lo; q <= hi; ++q) {
+          const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
+          const float secs_ss = *il<float>(w, a.o_secs, ss, lane);
+          bool any = false;
+          for (uint32_t t = 0; t < T; ++t) any = any || secs_ss <= a.sec_thr[t];
+          if (!any) continue;
+          const double sw = *il<double>(w, a.o_sigma, ss, lane);
+          if (sw == 0.0) continue;
+          const uint32_t k = *il<uint32_t>(w, a.o_np, ss, lane);
+          const uint32_t *preds = reinterpret_cast<const uint32_t *>(w + a.o_preds) + ((size_t)ss * a.maxp) * kWave + lane;
+          const uint32_t node = (*il<uint32_t>(w, a.o_state, ss, lane) & 0x7fffffffu) >> 1;
+          for (uint32_t t = 0; t < T; ++t) {
+            if (!(secs_ss <= a.sec_thr[t])) continue;
+            const double seed = *il<double>(w, a.o_seed, t * sc + ss, lane);
+            const double dep = seed + *il<double>(w, a.o_delta, t * sc + ss, lane);
             if (dep == 0.0) continue;
-            const uint32_t k = *il<uint32_t>(w, a.o_np, ss, lane);
-            const uint32_t *preds = reinterpret_cast<const uint32_t *>(w + a.o_preds) + ((size_t)ss * a.maxp) * kWave + lane;
             for (uint32_t p = 0; p < k; ++p) {
               const uint32_t ps = preds[(size_t)p * kWave];
               const double sv = *il<double>(w, a.o_sigma, ps, lane);
               if (sv == 0.0) continue;
-              double *dp = il<double>(w, a.o_delta, ps, lane);
+              double *dp = il<double>(w, a.o_delta, t * sc + ps, lane);
               *dp = *dp + (sv / sw) * dep;
             }
-            const uint32_t node = (*il<uint32_t>(w, a.o_state, ss, lane) & 0x7fffffffu) >> 1;
             if (node == src) continue;
             const double credit = dep - seed;
             if (credit > 0.0) unsafeAtomicAdd(&a.acc[((size_t)3 * T + t) * a.N + node], credit * (double)wt);
           }
-          hi = lo - 1;
         }
+        hi = lo - 1;
       }
     }
   }
@@ -570,7 +588,7 @@ struct AngPlan {
   size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_delta, o_seed, o_node, o_brt, o_bsec, o_heap;
 };
 
-void ang_layout(AngPlan &p, uint32_t N) {
+void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
   p.lane_stride = align256(8ull * 3 * N);  // skey [2N] + nkey [N]
   size_t o = 0;
   auto take = [&](size_t &field, size_t elem_bytes, size_t count) {
@@ -584,8 +602,8 @@ void ang_layout(AngPlan &p, uint32_t N) {
   take(p.o_np, 4, p.scap);
   take(p.o_preds, 4, (size_t)p.scap * p.maxp);
   take(p.o_order, 4, p.scap);
-  take(p.o_delta, 8, p.scap);
-  take(p.o_seed, 8, p.scap);
+  take(p.o_delta, 8, (size_t)p.scap * std::max<uint32_t>(T, 1));
+  take(p.o_seed, 8, (size_t)p.scap * std::max<uint32_t>(T, 1));
   take(p.o_node, 4, p.ncap);
   take(p.o_brt, 4, p.ncap);
   take(p.o_bsec, 4, p.ncap);
@@ -716,7 +734,7 @@ void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, c
   hip_check(hipEventCreate(&e1), "event");
   double kms = 0.0;
   for (int attempt = 0; attempt < 6; ++attempt) {
-    ang_layout(p, N);
+    ang_layout(p, N, cs.T);
     const size_t per_lane = p.lane_stride + p.wave_stride / kWave;
     const size_t budget = free_b / 4 + wsp.lane_bytes + wsp.wave_bytes;
     uint32_t lanes = (uint32_t)std::min<size_t>(budget / per_lane, (size_t)n_list);
