@@ -495,6 +495,10 @@ def run_serve(args) -> int:
             "plain_p99_ms": round(float(np.percentile(plain_lat, 99)), 2),
             "stream_s": round(stream_s, 3),
             "mean_launches_per_request": round(float(np.mean(launches)), 2),
+            "plain_s_by_kind": {k: round(float(sum(sv for (kk, _, _), sv in zip(reqs, service) if kk == k)), 3)
+                                for k in ("shortest", "simplest")},
+            "sources_by_kind": {k: int(sum(len(kw["source_indices"]) for kk, _, kw in reqs if kk == k))
+                                for k in ("shortest", "simplest")},
         },
         "roofline": None,
         "cpu_baseline": None,

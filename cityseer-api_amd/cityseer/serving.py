@@ -7,8 +7,9 @@ first-served under an HBM budget. Admitted shortest-path requests share
 back-to-back batched GPU launches: every launch takes part of each active
 request's remaining sources, so a request that arrives while a launch runs
 joins the next one and a finished request leaves without waiting for the
-others. Simplest-path requests run the same way on a second stream, so the two
-kernel families overlap on the device.
+others. Simplest-path requests run on a second stream, so the two kernel
+families overlap on the device, and every simplest-path launch carries all the
+simplest-path requests active at that moment.
 
 Each request is prepared exactly as the corresponding ``NetworkStructure``
 method prepares it, so its result equals the plain call's within

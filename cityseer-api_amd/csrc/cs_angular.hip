@@ -23,21 +23,35 @@ namespace cs {
 namespace {
 
 constexpr uint32_t kFailCap = 1u, kFailHeap = 2u, kFailPreds = 4u;
+constexpr uint32_t kNoSource = 0xffffffffu;  // padding item
 
-struct AngArgs {
-  uint32_t N;
+// One (graph, request) pair of a launch: a plain call has one, a served slice
+// one per active request.
+struct AngJob {
+  uint32_t N;           // nodes of the graph (accumulator rows)
   const uint32_t *off;  // out-edge rows (petgraph order)
   const uint4 *adj;     // {v | src_slot<<30 | dst_slot<<31, bits(es), bits(angle), bits(1e-6*len)}
   const uint8_t *eligible;
   uint32_t T;
   float maxsec, tolerance, unit, offset;
   float sec_thr[kMaxT];
-  uint32_t n_sources;
   const uint32_t *sources;
   const float *src_wt;
   uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
   double *acc;     // [4][T][N], metric-major
   uint32_t *reach;
+};
+
+// Jobs are read-only for the whole launch: reading them through the constant
+// address space lets the compiler keep their fields in registers across the
+// workspace stores instead of reloading them.
+typedef __attribute__((address_space(4))) const AngJob CAngJob;
+
+struct AngArgs {
+  uint32_t N;           // largest graph of the launch (lane map layout)
+  const AngJob *jobs;
+  const uint2 *items;   // {job, source position}; NULL: job 0, position = item
+  uint32_t n_items;
   unsigned long long *stats;
   char *ws;
   size_t lane_stride;  // bytes of the lane-major maps per lane
@@ -134,7 +148,7 @@ struct Heap {
 
 // One source, exactly as dijkstra_brandes_angular Phase 1 (centrality.rs:577-705).
 // Returns false (with fail bits) if a workspace capacity is exceeded.
-__device__ bool angular_source(const AngArgs &a, const LaneWs &L, uint32_t src, uint32_t epoch, uint32_t &n_states,
+__device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, uint32_t src, uint32_t epoch, uint32_t &n_states,
                                uint32_t &n_nodes, uint32_t &n_order, uint32_t &fail) {
   const float one_m = 1.0f - kTieEps, one_p = 1.0f + kTieEps;
   char *w = L.w;
@@ -174,14 +188,14 @@ __device__ bool angular_source(const AngArgs &a, const LaneWs &L, uint32_t src, 
     const float u_route = *il<float>(w, a.o_route, uss, lane);
     const float u_secs = *il<float>(w, a.o_secs, uss, lane);
     const double u_sigma = *il<double>(w, a.o_sigma, uss, lane);
-    for (uint32_t j = a.off[u]; j < a.off[u + 1]; ++j) {
-      const uint4 e = a.adj[j];
+    for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
+      const uint4 e = jb.adj[j];
       const uint32_t cur_slot = (e.x >> 30) & 1u;
       if (cur_slot != 1u - entry) continue;
       const uint32_t v = e.x & 0x3fffffffu;
       const uint32_t vs = 2 * v + (e.x >> 31);
       const float cs = u_secs + __uint_as_float(e.y);
-      if (cs > a.maxsec) continue;
+      if (cs > jb.maxsec) continue;
       const uint64_t sk = L.skey[vs];
       uint32_t vss;
       float cur;
@@ -283,7 +297,7 @@ __device__ bool angular_source(const AngArgs &a, const LaneWs &L, uint32_t src, 
 // later gains u as a predecessor when route_u + angle + 1e-6 length <=
 // route_v * (1 + tol). Settle positions live in the delta slots, which the
 // betweenness pass clears before use.
-__device__ bool angular_phase2(const AngArgs &a, const LaneWs &L, uint32_t epoch, uint32_t n_order, uint32_t &fail) {
+__device__ bool angular_phase2(const AngArgs &a, CAngJob &jb, const LaneWs &L, uint32_t epoch, uint32_t n_order, uint32_t &fail) {
   char *w = L.w;
   const uint32_t lane = L.lane;
   for (uint32_t q = 0; q < n_order; ++q) {
@@ -295,15 +309,15 @@ __device__ bool angular_phase2(const AngArgs &a, const LaneWs &L, uint32_t epoch
   // the source's two states are the first two slots
   *il<double>(w, a.o_sigma, 0, lane) = 1.0;
   *il<double>(w, a.o_sigma, 1, lane) = 1.0;
-  const float band = 1.0f + a.tolerance;
+  const float band = 1.0f + jb.tolerance;
   for (uint32_t pos = 0; pos < n_order; ++pos) {
     const uint32_t uss = *il<uint32_t>(w, a.o_order, pos, lane);
     const uint32_t us = *il<uint32_t>(w, a.o_state, uss, lane) & 0x7fffffffu;
     const uint32_t u = us >> 1, entry = us & 1u;
     const float u_route = *il<float>(w, a.o_route, uss, lane);
     const double u_sigma = *il<double>(w, a.o_sigma, uss, lane);
-    for (uint32_t j = a.off[u]; j < a.off[u + 1]; ++j) {
-      const uint4 e = a.adj[j];
+    for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
+      const uint4 e = jb.adj[j];
       if (((e.x >> 30) & 1u) != 1u - entry) continue;
       const uint32_t v = e.x & 0x3fffffffu;
       const uint32_t vs = 2 * v + (e.x >> 31);
@@ -347,24 +361,34 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
   uint32_t epoch = a.epochs[gl];
   unsigned long long st_reached = 0, st_edges = 0, st_within = 0;
   char *w = L.w;
-  for (;;) {
-    const uint32_t si = atomicAdd(a.next_source, 1u);
-    if (si >= a.n_sources) break;
-    const uint32_t src = a.sources[si];
-    const float wt = a.src_wt[si];
+  // Items are dealt statically, lane gl taking gl, gl + lanes, ...; each job's
+  // items are padded to whole waves, so the 64 items of one wave belong to one
+  // job and its fields load once per wave into scalar registers.
+  const uint32_t lanes_total = gridDim.x * blockDim.x;
+  for (uint32_t it = gl; it < a.n_items; it += lanes_total) {
+    uint32_t job = 0, si = it;
+    if (a.items) {
+      const uint2 x = a.items[it];
+      job = __builtin_amdgcn_readfirstlane(x.x);
+      si = x.y;
+      if (si == kNoSource) continue;  // padding
+    }
+    CAngJob &jb = ((CAngJob *)a.jobs)[job];
+    const uint32_t src = jb.sources[si];
+    const float wt = jb.src_wt[si];
     ++epoch;
     uint32_t n_states, n_nodes, n_order, fail = 0;
-    bool ok = angular_source(a, L, src, epoch, n_states, n_nodes, n_order, fail);
-    if (ok && a.tolerance > kTieEps && (a.flags & 2u)) ok = angular_phase2(a, L, epoch, n_order, fail);
+    bool ok = angular_source(a, jb, L, src, epoch, n_states, n_nodes, n_order, fail);
+    if (ok && jb.tolerance > kTieEps && (jb.flags & 2u)) ok = angular_phase2(a, jb, L, epoch, n_order, fail);
     if (!ok) {
       const uint32_t q = atomicAdd(a.n_fail, 1u);
-      a.fail_list[q] = si;
+      a.fail_list[q] = it;
       atomicOr(a.fail_flags, fail);
       continue;
     }
     st_reached += n_nodes;
     st_edges += n_order;
-    const uint32_t T = a.T;
+    const uint32_t T = jb.T;
     // ---- closeness at targets (centrality.rs:1995-2033)
     for (uint32_t k = 1; k < n_nodes; ++k) {
       const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane);
@@ -372,17 +396,17 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
       const float bsd = *il<float>(w, a.o_brt, k, lane);
       const float bs = *il<float>(w, a.o_bsec, k, lane);
       if (!isfinite(bsd) || !isfinite(bs)) continue;
-      double *accv = a.acc + v;
-      const size_t NN = a.N;
+      double *accv = jb.acc + v;
+      const size_t NN = jb.N;
       for (uint32_t t = 0; t < T; ++t) {
-        if (bs <= a.sec_thr[t]) {
+        if (bs <= jb.sec_thr[t]) {
           ++st_within;
-          if (a.flags & 4u) atomicAdd(&a.reach[t], 1u);
-          if (a.flags & 1u) {
+          if (jb.flags & 4u) atomicAdd(&jb.reach[t], 1u);
+          if (jb.flags & 1u) {
             unsafeAtomicAdd(&accv[((size_t)0 * T + t) * NN], (double)wt);
-            const float far_ang = a.offset + (bsd / a.unit);
+            const float far_ang = jb.offset + (bsd / jb.unit);
             unsafeAtomicAdd(&accv[((size_t)1 * T + t) * NN], (double)(far_ang * wt));
-            const float harm_ang = 1.0f + (bsd / a.unit);
+            const float harm_ang = 1.0f + (bsd / jb.unit);
             unsafeAtomicAdd(&accv[((size_t)2 * T + t) * NN], (double)((1.0f / harm_ang) * wt));
           }
         }
@@ -391,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     // ---- betweenness (centrality.rs:2037-2096, 823-873): every threshold in one
     // walk over the settled states; each threshold keeps its own seeds and
     // dependencies ([T][scap]) and sees its terms in the reference's order
-    if (a.flags & 2u) {
+    if (jb.flags & 2u) {
       const size_t sc = a.scap;
       for (uint32_t q = 0; q < n_order; ++q) {
         const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
@@ -414,9 +438,9 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           if ((uint32_t)(sk >> 32) != epoch) continue;
           cand[ncand++] = (uint32_t)sk;
         }
-        const double pc = a.eligible[v] ? 0.5 : 1.0;
+        const double pc = jb.eligible[v] ? 0.5 : 1.0;
         for (uint32_t t = 0; t < T; ++t) {
-          const float sthr = a.sec_thr[t];
+          const float sthr = jb.sec_thr[t];
           if (bs > sthr) continue;
           uint32_t best[2];
           uint32_t nbst = 0;
@@ -425,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
             const uint32_t ss = cand[q];
             const double sg = *il<double>(w, a.o_sigma, ss, lane);
             if (sg == 0.0 || *il<float>(w, a.o_secs, ss, lane) > sthr) continue;
-            if (*il<float>(w, a.o_route, ss, lane) <= br * (1.0f + a.tolerance)) best[nbst++] = ss;
+            if (*il<float>(w, a.o_route, ss, lane) <= br * (1.0f + jb.tolerance)) best[nbst++] = ss;
           }
           if (nbst == 0) continue;
           for (uint32_t q = 0; q < nbst; ++q) total += *il<double>(w, a.o_sigma, best[q], lane);
@@ -446,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
           const float secs_ss = *il<float>(w, a.o_secs, ss, lane);
           bool any = false;
-          for (uint32_t t = 0; t < T; ++t) any = any || secs_ss <= a.sec_thr[t];
+          for (uint32_t t = 0; t < T; ++t) any = any || secs_ss <= jb.sec_thr[t];
           if (!any) continue;
           const double sw = *il<double>(w, a.o_sigma, ss, lane);
           if (sw == 0.0) continue;
@@ -454,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           const uint32_t *preds = reinterpret_cast<const uint32_t *>(w + a.o_preds) + ((size_t)ss * a.maxp) * kWave + lane;
           const uint32_t node = (*il<uint32_t>(w, a.o_state, ss, lane) & 0x7fffffffu) >> 1;
           for (uint32_t t = 0; t < T; ++t) {
-            if (!(secs_ss <= a.sec_thr[t])) continue;
+            if (!(secs_ss <= jb.sec_thr[t])) continue;
             const double seed = *il<double>(w, a.o_seed, t * sc + ss, lane);
             const double dep = seed + *il<double>(w, a.o_delta, t * sc + ss, lane);
             if (dep == 0.0) continue;
@@ -467,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
             }
             if (node == src) continue;
             const double credit = dep - seed;
-            if (credit > 0.0) unsafeAtomicAdd(&a.acc[((size_t)3 * T + t) * a.N + node], credit * (double)wt);
+            if (credit > 0.0) unsafeAtomicAdd(&jb.acc[((size_t)3 * T + t) * jb.N + node], credit * (double)wt);
           }
         }
         hi = lo - 1;
@@ -484,6 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
 // absolute +-1e-4 bands, per-node view collapsed as the reference does.
 __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  CAngJob &jb = ((CAngJob *)a.jobs)[0];
   const float eps = kTieEps;
   LaneWs L;
   L.skey = reinterpret_cast<uint64_t *>(a.ws);
@@ -521,13 +546,13 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
     const uint32_t u = us >> 1, entry = us & 1u;
     const float u_secs = *il<float>(w, a.o_secs, uss, 0);
     const float u_simpl = (float)*il<double>(w, a.o_delta, uss, 0);
-    for (uint32_t j = a.off[u]; j < a.off[u + 1]; ++j) {
-      const uint4 e = a.adj[j];
+    for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
+      const uint4 e = jb.adj[j];
       if (((e.x >> 30) & 1u) != 1u - entry) continue;
       const uint32_t v = e.x & 0x3fffffffu;
       const uint32_t vs = 2 * v + (e.x >> 31);
       const float cs = u_secs + __uint_as_float(e.y);
-      if (cs > a.maxsec) continue;
+      if (cs > jb.maxsec) continue;
       const float csimpl = u_simpl + __uint_as_float(e.z);
       const float cmetric = csimpl + __uint_as_float(e.w);
       const uint64_t sk = L.skey[vs];
@@ -700,41 +725,84 @@ static uint4 *angular_adjacency(Graph &g, DevEdges &de) {
   return de.ang_adj;
 }
 
-void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const uint32_t *d_sources,
-                     const float *d_wt, uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
-                     uint32_t *d_reach, hipStream_t stream, cs_stats &st) {
-  if (n_sources == 0) return;
-  if (g.n >= (1u << 30)) fail(CS_EINVAL, "dual graph too large for the angular search (node ids need 30 bits)");
-  const uint32_t N = g.n;
-  uint4 *adj = angular_adjacency(g, de);
+// Items grouped by job (stable) with each job's run padded to whole waves.
+std::vector<uint2> wave_aligned(const std::vector<uint2> &in, size_t n_jobs) {
+  std::vector<std::vector<uint2>> by(n_jobs);
+  for (const uint2 &x : in) by[x.x].push_back(x);
+  std::vector<uint2> out;
+  for (uint32_t j = 0; j < n_jobs; ++j) {
+    for (const uint2 &x : by[j]) out.push_back(x);
+    while (out.size() % kWave) out.push_back(make_uint2(j, kNoSource));
+  }
+  return out;
+}
+
+void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<uint2> *explicit_items,
+                          uint32_t n_implicit, hipStream_t stream, cs_stats &st) {
+  if (specs.empty()) return;
+  std::vector<uint2> items;
+  if (explicit_items) items = wave_aligned(*explicit_items, specs.size());
+  uint32_t n_list = explicit_items ? (uint32_t)items.size() : n_implicit;
+  if (n_list == 0) return;
+  uint32_t N = 1, maxp = 1, T = 1;
+  std::vector<AngJob> jobs(specs.size());
+  for (size_t i = 0; i < specs.size(); ++i) {
+    const AngSpec &sp = specs[i];
+    Graph &g = *sp.g;
+    if (g.n >= (1u << 30)) fail(CS_EINVAL, "dual graph too large for the angular search (node ids need 30 bits)");
+    N = std::max(N, g.n);
+    maxp = std::max<uint32_t>(maxp, g.max_in_deg);
+    T = std::max(T, sp.cs.T);
+    AngJob &j = jobs[i];
+    std::memset(&j, 0, sizeof(j));
+    j.N = g.n;
+    j.off = sp.dg->fwd_off;
+    j.adj = angular_adjacency(g, *sp.de);
+    j.eligible = sp.d_eligible;
+    j.T = sp.cs.T;
+    j.maxsec = sp.cs.maxsec;
+    j.tolerance = sp.cs.tolerance;
+    j.unit = sp.cs.ang_unit;
+    j.offset = sp.cs.far_offset;
+    for (uint32_t t = 0; t < kMaxT; ++t) j.sec_thr[t] = t < sp.cs.T ? (float)sp.cs.seconds[t] : 0.0f;
+    j.sources = sp.d_sources;
+    j.src_wt = sp.d_wt;
+    j.flags = (sp.cs.closeness ? 1u : 0u) | (sp.cs.betweenness ? 2u : 0u) | (sp.cs.track_reach ? 4u : 0u);
+    j.acc = sp.d_acc;
+    j.reach = sp.d_reach;
+  }
   AngPlan p;
-  p.maxp = std::max<uint32_t>(1, g.max_in_deg);
+  p.maxp = maxp;
   p.scap = std::min<uint32_t>(2 * N, 8192);
   p.ncap = std::min<uint32_t>(N, p.scap);
   p.hcap = 2 * p.scap;
-  int dev = dg.device;
+  const int dev = specs[0].dg->device;
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, dev), "props");
   size_t free_b = 0, total_b = 0;
   hip_check(hipMemGetInfo(&free_b, &total_b), "meminfo");
   AngWorkspace &wsp = ang_ws(dev);
   std::lock_guard<std::mutex> lk(wsp.mu);
-  uint32_t *d_ctl = nullptr, *d_fail = nullptr, *d_src_cur = nullptr;
-  float *d_wt_cur = nullptr;
+  uint32_t *d_ctl = nullptr, *d_fail = nullptr;
+  uint2 *d_items = nullptr;
+  AngJob *d_jobs = nullptr;
   unsigned long long *d_stats = nullptr;
   hip_check(hipMalloc(&d_ctl, 16), "hipMalloc");
-  hip_check(hipMalloc(&d_fail, 4ull * n_sources), "hipMalloc");
+  hip_check(hipMalloc(&d_fail, 4ull * n_list), "hipMalloc");
   hip_check(hipMalloc(&d_stats, 32), "hipMalloc");
+  hip_check(hipMalloc(&d_jobs, sizeof(AngJob) * jobs.size()), "hipMalloc");
+  hip_check(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(AngJob) * jobs.size(), hipMemcpyHostToDevice, stream), "copy");
+  if (explicit_items) {
+    hip_check(hipMalloc(&d_items, sizeof(uint2) * n_list), "hipMalloc");
+    hip_check(hipMemcpyAsync(d_items, items.data(), sizeof(uint2) * n_list, hipMemcpyHostToDevice, stream), "copy");
+  }
   hip_check(hipMemsetAsync(d_stats, 0, 32, stream), "memset");
-  const uint32_t *src_list = d_sources;
-  const float *wt_list = d_wt;
-  uint32_t n_list = n_sources;
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
   double kms = 0.0;
   for (int attempt = 0; attempt < 6; ++attempt) {
-    ang_layout(p, N, cs.T);
+    ang_layout(p, N, T);
     const size_t per_lane = p.lane_stride + p.wave_stride / kWave;
     const size_t budget = free_b / 4 + wsp.lane_bytes + wsp.wave_bytes;
     uint32_t lanes = (uint32_t)std::min<size_t>(budget / per_lane, (size_t)n_list);
@@ -748,21 +816,9 @@ void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, c
     AngArgs a;
     std::memset(&a, 0, sizeof(a));
     a.N = N;
-    a.off = dg.fwd_off;
-    a.adj = adj;
-    a.eligible = d_eligible;
-    a.T = cs.T;
-    a.maxsec = cs.maxsec;
-    a.tolerance = cs.tolerance;
-    a.unit = cs.ang_unit;
-    a.offset = cs.far_offset;
-    for (uint32_t i = 0; i < kMaxT; ++i) a.sec_thr[i] = i < cs.T ? (float)cs.seconds[i] : 0.0f;
-    a.n_sources = n_list;
-    a.sources = src_list;
-    a.src_wt = wt_list;
-    a.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
-    a.acc = d_acc;
-    a.reach = d_reach;
+    a.jobs = d_jobs;
+    a.items = d_items;
+    a.n_items = n_list;
     a.stats = d_stats;
     a.ws = wsp.lane_ws;
     a.wave_ws = wsp.wave_ws;
@@ -803,32 +859,22 @@ void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, c
     st.waves = lanes / kWave;
     const uint32_t n_failed = ctl[1], flags = ctl[2];
     if (n_failed == 0) break;
+    if (flags & kFailPreds) fail(CS_EINTERNAL, "predecessor list overflow in the angular search");
+    if (attempt == 5) fail(CS_EINTERNAL, "angular workspace retries exhausted");
     st.retries += n_failed;
-    if (!d_src_cur) {
-      hip_check(hipMalloc(&d_src_cur, 4ull * n_sources), "hipMalloc");
-      hip_check(hipMalloc(&d_wt_cur, 4ull * n_sources), "hipMalloc");
-    }
-    std::vector<uint32_t> fl(n_failed), srcs(n_list);
-    std::vector<float> wts(n_list);
+    // retry the failed items with a larger workspace
+    std::vector<uint32_t> fl(n_failed);
     hip_check(hipMemcpy(fl.data(), d_fail, 4ull * n_failed, hipMemcpyDeviceToHost), "copy");
-    hip_check(hipMemcpy(srcs.data(), src_list, 4ull * n_list, hipMemcpyDeviceToHost), "copy");
-    hip_check(hipMemcpy(wts.data(), wt_list, 4ull * n_list, hipMemcpyDeviceToHost), "copy");
-    std::vector<uint32_t> ns(n_failed);
-    std::vector<float> nw(n_failed);
-    for (uint32_t i = 0; i < n_failed; ++i) {
-      ns[i] = srcs[fl[i]];
-      nw[i] = wts[fl[i]];
-    }
-    hip_check(hipMemcpy(d_src_cur, ns.data(), 4ull * n_failed, hipMemcpyHostToDevice), "copy");
-    hip_check(hipMemcpy(d_wt_cur, nw.data(), 4ull * n_failed, hipMemcpyHostToDevice), "copy");
-    src_list = d_src_cur;
-    wt_list = d_wt_cur;
-    n_list = n_failed;
+    std::vector<uint2> retry(n_failed);
+    for (uint32_t i = 0; i < n_failed; ++i) retry[i] = d_items ? items[fl[i]] : make_uint2(0u, fl[i]);
+    items = wave_aligned(retry, specs.size());
+    (void)hipFree(d_items);
+    hip_check(hipMalloc(&d_items, sizeof(uint2) * items.size()), "hipMalloc");
+    hip_check(hipMemcpy(d_items, items.data(), sizeof(uint2) * items.size(), hipMemcpyHostToDevice), "copy");
+    n_list = (uint32_t)items.size();
     if (flags & kFailCap) p.scap = std::min<uint32_t>(2 * N, p.scap * 4);
     p.ncap = std::min<uint32_t>(N, p.scap);
     if (flags & (kFailHeap | kFailCap)) p.hcap = std::max<uint32_t>(p.hcap * 4, 2 * p.scap);
-    if (flags & kFailPreds) fail(CS_EINTERNAL, "predecessor list overflow in the angular search");
-    if (attempt == 5) fail(CS_EINTERNAL, "angular workspace retries exhausted");
   }
   unsigned long long stv[4];
   hip_check(hipMemcpy(stv, d_stats, 32, hipMemcpyDeviceToHost), "copy");
@@ -841,8 +887,16 @@ void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, c
   (void)hipFree(d_ctl);
   (void)hipFree(d_fail);
   (void)hipFree(d_stats);
-  (void)hipFree(d_src_cur);
-  (void)hipFree(d_wt_cur);
+  (void)hipFree(d_jobs);
+  (void)hipFree(d_items);
+}
+
+void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const uint32_t *d_sources,
+                     const float *d_wt, uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
+                     uint32_t *d_reach, hipStream_t stream, cs_stats &st) {
+  if (n_sources == 0) return;
+  std::vector<AngSpec> specs{AngSpec{&g, &dg, &de, cs, d_sources, d_wt, d_eligible, d_acc, d_reach}};
+  launch_simplest_jobs(specs, nullptr, n_sources, stream, st);
 }
 
 void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec, float *d_simpl,
@@ -867,12 +921,19 @@ void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, fl
   hip_check(hipMemsetAsync(lane_ws, 0, p.lane_stride, stream), "memset");
   hip_check(hipMemsetAsync(epochs, 0, 4, stream), "memset");
   hip_check(hipMemsetAsync(ctl, 0, 16, stream), "memset");
+  AngJob job;
+  std::memset(&job, 0, sizeof(job));
+  job.N = N;
+  job.off = dg.fwd_off;
+  job.adj = adj;
+  job.maxsec = maxsec;
+  AngJob *d_job = nullptr;
+  hip_check(hipMalloc(&d_job, sizeof(AngJob)), "hipMalloc");
+  hip_check(hipMemcpyAsync(d_job, &job, sizeof(AngJob), hipMemcpyHostToDevice, stream), "copy");
   AngArgs a;
   std::memset(&a, 0, sizeof(a));
   a.N = N;
-  a.off = dg.fwd_off;
-  a.adj = adj;
-  a.maxsec = maxsec;
+  a.jobs = d_job;
   a.ws = lane_ws;
   a.wave_ws = wave_ws;
   a.lane_stride = p.lane_stride;
@@ -911,6 +972,7 @@ void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, fl
   (void)hipFree(wave_ws);
   (void)hipFree(epochs);
   (void)hipFree(ctl);
+  (void)hipFree(d_job);
   if (hctl[2]) fail(CS_EINTERNAL, "angular tree workspace exceeded");
 }
 

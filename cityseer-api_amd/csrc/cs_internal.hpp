@@ -194,6 +194,24 @@ void launch_single(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float max
                    float tolerance, int mode, const TreeOut &out, hipStream_t stream);
 
 // Angular (simplest) centrality (cs_angular.hip); acc [N][4][T].
+// One (graph, request) pair of an angular launch; sources index d_sources.
+struct AngSpec {
+  Graph *g;
+  DevGraph *dg;
+  DevEdges *de;
+  CallSpec cs;
+  const uint32_t *d_sources;
+  const float *d_wt;
+  const uint8_t *d_eligible;
+  double *d_acc;  // [4][T][N]
+  uint32_t *d_reach;
+};
+
+// Simplest-path centrality for several requests in one launch: the items
+// {job, source position} (or, without a list, sources 0..n_implicit-1 of job 0).
+void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<uint2> *explicit_items,
+                          uint32_t n_implicit, hipStream_t stream, cs_stats &st);
+
 void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, uint32_t n_sources,
                      const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,

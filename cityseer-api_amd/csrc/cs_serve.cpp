@@ -9,9 +9,10 @@
 // estimate of the per-source cost so that one slice lasts about `slice_ms`.
 // A request that arrives while a slice runs joins the next one; a request
 // completes when its last sources finish, and leaves. Simplest-path requests
-// run the same way on a second worker and stream, so the two kernel families
-// overlap on the device. Results are copied out and unpacked by the worker
-// that finishes a request.
+// run on a second worker and stream, so the two kernel families overlap on the
+// device; since an angular launch lasts about as long as its slowest source,
+// each simplest slice packs every active simplest request into one launch.
+// Results are copied out and unpacked by the worker that finishes a request.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +32,9 @@ using Clock = std::chrono::steady_clock;
 
 namespace cs {
 namespace {
+
+// Sources of one simplest-path slice at most (about the angular kernel's lane budget).
+constexpr uint64_t kSimplestSlice = 262144;
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -220,9 +224,11 @@ struct cs_server {
         for (Req *r : active)
           if (r->kind == kind && r->ready && r->handed < r->n()) mine.push_back(r);
         if (mine.empty()) continue;
-        const double per = ms_per_src[kind] > 0.0 ? ms_per_src[kind] : (kind == CS_REQ_SHORTEST ? 0.004 : 0.01);
-        uint64_t target = (uint64_t)std::clamp(cfg.slice_ms / per, 4096.0, 1048576.0);
-        if (kind == CS_REQ_SIMPLEST) mine.resize(1);  // the angular kernel runs one request per launch
+        const double per = ms_per_src[kind] > 0.0 ? ms_per_src[kind] : 0.004;
+        // an angular launch lasts about as long as its slowest source whatever
+        // its width, so a simplest slice takes everything up to the lane budget
+        uint64_t target = kind == CS_REQ_SHORTEST ? (uint64_t)std::clamp(cfg.slice_ms / per, 4096.0, 1048576.0)
+                                                  : kSimplestSlice;
         std::vector<uint32_t> take(mine.size(), 0);
         uint64_t left = target;
         for (int pass = 0; pass < 2 && left > 0; ++pass) {
@@ -266,11 +272,16 @@ struct cs_server {
           n_src = items.size();
           launch_shortest_batch(specs, NT, default_tuning(), stream, st, &items);
         } else {
-          Req &r = *slice_reqs[0];
-          const uint32_t first = r.handed - slice_take[0];
-          n_src = slice_take[0];
-          launch_simplest(r.h->g, *r.dg, *r.de, r.spec, r.d_src + first, r.d_wt + first, slice_take[0], r.d_el,
-                          r.d_acc, r.d_reach, stream, st);
+          std::vector<AngSpec> specs;
+          std::vector<uint2> items;
+          for (size_t i = 0; i < slice_reqs.size(); ++i) {
+            Req &r = *slice_reqs[i];
+            specs.push_back(AngSpec{&r.h->g, r.dg, r.de, r.spec, r.d_src, r.d_wt, r.d_el, r.d_acc, r.d_reach});
+            const uint32_t first = r.handed - slice_take[i];
+            for (uint32_t k = 0; k < slice_take[i]; ++k) items.push_back(make_uint2((uint32_t)i, first + k));
+          }
+          n_src = items.size();
+          launch_simplest_jobs(specs, &items, 0, stream, st);
         }
       } catch (const Error &e) {
         err = e.code;
