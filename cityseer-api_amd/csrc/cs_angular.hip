@@ -24,6 +24,7 @@ namespace {
 
 constexpr uint32_t kFailCap = 1u, kFailHeap = 2u, kFailPreds = 4u;
 constexpr uint32_t kNoSource = 0xffffffffu;  // padding item
+constexpr uint32_t kRelax = 4;               // out-edges whose loads one relaxation step batches
 
 // One (graph, request) pair of a launch: a plain call has one, a served slice
 // one per active request.
@@ -188,103 +189,161 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
     const float u_route = *il<float>(w, a.o_route, uss, lane);
     const float u_secs = *il<float>(w, a.o_secs, uss, lane);
     const double u_sigma = *il<double>(w, a.o_sigma, uss, lane);
-    for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
-      const uint4 e = jb.adj[j];
-      const uint32_t cur_slot = (e.x >> 30) & 1u;
-      if (cur_slot != 1u - entry) continue;
-      const uint32_t v = e.x & 0x3fffffffu;
-      const uint32_t vs = 2 * v + (e.x >> 31);
-      const float cs = u_secs + __uint_as_float(e.y);
-      if (cs > jb.maxsec) continue;
-      const uint64_t sk = L.skey[vs];
-      uint32_t vss;
-      float cur;
-      if ((uint32_t)(sk >> 32) == epoch) {
-        vss = (uint32_t)sk;
-        if (*il<uint32_t>(w, a.o_state, vss, lane) & 0x80000000u) continue;  // visited
-        cur = *il<float>(w, a.o_route, vss, lane);
-      } else {
-        if (n_states >= a.scap) {
-          fail |= kFailCap;
-          return false;
-        }
-        vss = n_states++;
-        L.skey[vs] = ((uint64_t)epoch << 32) | vss;
-        *il<float>(w, a.o_route, vss, lane) = INFINITY;
-        *il<float>(w, a.o_secs, vss, lane) = INFINITY;
-        *il<double>(w, a.o_sigma, vss, lane) = 0.0;
-        *il<uint32_t>(w, a.o_state, vss, lane) = vs;
-        *il<uint32_t>(w, a.o_np, vss, lane) = 0;
-        cur = INFINITY;
+    // out-edges in chunks of kRelax: the adjacency entries, then the state and
+    // node keys, then the state and node fields are loaded for the whole chunk
+    // at once; the edges are then applied one by one in row order, re-reading
+    // a state or node that an earlier edge of the chunk has just written
+    const uint32_t j0 = jb.off[u], j1 = jb.off[u + 1];
+    for (uint32_t jc = j0; jc < j1; jc += kRelax) {
+      uint4 e[kRelax];
+#pragma unroll
+      for (uint32_t q = 0; q < kRelax; ++q) e[q] = (jc + q < j1) ? jb.adj[jc + q] : make_uint4(0u, 0u, 0u, 0u);
+      bool ok[kRelax];
+      uint32_t vv[kRelax], vsv[kRelax];
+      float csv[kRelax];
+      uint64_t skv[kRelax], nkv[kRelax];
+#pragma unroll
+      for (uint32_t q = 0; q < kRelax; ++q) {
+        vv[q] = e[q].x & 0x3fffffffu;
+        vsv[q] = 2 * vv[q] + (e[q].x >> 31);
+        csv[q] = u_secs + __uint_as_float(e[q].y);
+        ok[q] = jc + q < j1 && ((e[q].x >> 30) & 1u) == 1u - entry && !(csv[q] > jb.maxsec);
+        skv[q] = ok[q] ? L.skey[vsv[q]] : 0ull;
+        nkv[q] = ok[q] ? L.nkey[vv[q]] : 0ull;
       }
-      float cr = u_route + __uint_as_float(e.z);
-      cr = cr + __uint_as_float(e.w);
-      uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane);
-      uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)vss * a.maxp) * kWave + lane;
-      if (cr < cur) {
-        double *sg = il<double>(w, a.o_sigma, vss, lane);
-        if (cr < cur * one_m) {
-          *np = 0;
-          *sg = u_sigma;
+      uint32_t flv[kRelax];
+      float rtv[kRelax], brv[kRelax], bsv[kRelax];
+#pragma unroll
+      for (uint32_t q = 0; q < kRelax; ++q) {
+        flv[q] = 0u;
+        rtv[q] = brv[q] = bsv[q] = INFINITY;
+        if (ok[q] && (uint32_t)(skv[q] >> 32) == epoch) {
+          flv[q] = *il<uint32_t>(w, a.o_state, (uint32_t)skv[q], lane);
+          rtv[q] = *il<float>(w, a.o_route, (uint32_t)skv[q], lane);
+        }
+        if (ok[q] && (uint32_t)(nkv[q] >> 32) == epoch) {
+          brv[q] = *il<float>(w, a.o_brt, (uint32_t)nkv[q], lane);
+          bsv[q] = *il<float>(w, a.o_bsec, (uint32_t)nkv[q], lane);
+        }
+      }
+#pragma unroll
+      for (uint32_t q = 0; q < kRelax; ++q) {
+        if (!ok[q]) continue;
+        bool dup_s = false, dup_n = false;
+#pragma unroll
+        for (uint32_t r = 0; r < q; ++r) {
+          dup_s = dup_s || (ok[r] && vsv[r] == vsv[q]);
+          dup_n = dup_n || (ok[r] && vv[r] == vv[q]);
+        }
+        const uint32_t v = vv[q], vs = vsv[q];
+        const float cs = csv[q];
+        uint64_t sk = skv[q];
+        uint32_t fl = flv[q];
+        float rt = rtv[q];
+        if (dup_s) {
+          sk = L.skey[vs];
+          if ((uint32_t)(sk >> 32) == epoch) {
+            fl = *il<uint32_t>(w, a.o_state, (uint32_t)sk, lane);
+            rt = *il<float>(w, a.o_route, (uint32_t)sk, lane);
+          }
+        }
+        uint32_t vss;
+        float cur;
+        if ((uint32_t)(sk >> 32) == epoch) {
+          vss = (uint32_t)sk;
+          if (fl & 0x80000000u) continue;  // visited
+          cur = rt;
         } else {
-          *sg = *sg + u_sigma;
+          if (n_states >= a.scap) {
+            fail |= kFailCap;
+            return false;
+          }
+          vss = n_states++;
+          L.skey[vs] = ((uint64_t)epoch << 32) | vss;
+          *il<float>(w, a.o_route, vss, lane) = INFINITY;
+          *il<float>(w, a.o_secs, vss, lane) = INFINITY;
+          *il<double>(w, a.o_sigma, vss, lane) = 0.0;
+          *il<uint32_t>(w, a.o_state, vss, lane) = vs;
+          *il<uint32_t>(w, a.o_np, vss, lane) = 0;
+          cur = INFINITY;
         }
-        *il<float>(w, a.o_route, vss, lane) = cr;
-        *il<float>(w, a.o_secs, vss, lane) = cs;
-        const uint32_t k = *np;
-        if (k >= a.maxp) {
-          fail |= kFailPreds;
-          return false;
-        }
-        preds[(size_t)k * kWave] = uss;
-        *np = k + 1;
-        if (h.n >= a.hcap) {
-          fail |= kFailHeap;
-          return false;
-        }
-        h.push(cr, cs, vs);
-      } else if (cr <= cur * one_p) {
-        const uint32_t k = *np;
-        bool has = false;
-        for (uint32_t q = 0; q < k; ++q)
-          if (preds[(size_t)q * kWave] == uss) has = true;
-        if (!has) {
-          float *vsec = il<float>(w, a.o_secs, vss, lane);
-          if (cs < *vsec) *vsec = cs;
+        float cr = u_route + __uint_as_float(e[q].z);
+        cr = cr + __uint_as_float(e[q].w);
+        uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane);
+        uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)vss * a.maxp) * kWave + lane;
+        if (cr < cur) {
+          double *sg = il<double>(w, a.o_sigma, vss, lane);
+          if (cr < cur * one_m) {
+            *np = 0;
+            *sg = u_sigma;
+          } else {
+            *sg = *sg + u_sigma;
+          }
+          *il<float>(w, a.o_route, vss, lane) = cr;
+          *il<float>(w, a.o_secs, vss, lane) = cs;
+          const uint32_t k = *np;
           if (k >= a.maxp) {
             fail |= kFailPreds;
             return false;
           }
           preds[(size_t)k * kWave] = uss;
           *np = k + 1;
-          double *sg = il<double>(w, a.o_sigma, vss, lane);
-          *sg = *sg + u_sigma;
+          if (h.n >= a.hcap) {
+            fail |= kFailHeap;
+            return false;
+          }
+          h.push(cr, cs, vs);
+        } else if (cr <= cur * one_p) {
+          const uint32_t k = *np;
+          bool has = false;
+          for (uint32_t r = 0; r < k; ++r)
+            if (preds[(size_t)r * kWave] == uss) has = true;
+          if (!has) {
+            float *vsec = il<float>(w, a.o_secs, vss, lane);
+            if (cs < *vsec) *vsec = cs;
+            if (k >= a.maxp) {
+              fail |= kFailPreds;
+              return false;
+            }
+            preds[(size_t)k * kWave] = uss;
+            *np = k + 1;
+            double *sg = il<double>(w, a.o_sigma, vss, lane);
+            *sg = *sg + u_sigma;
+          }
         }
-      }
-      // per-node best route / seconds (centrality.rs:695-703)
-      const uint64_t nk = L.nkey[v];
-      uint32_t nsl;
-      if ((uint32_t)(nk >> 32) == epoch) {
-        nsl = (uint32_t)nk;
-      } else {
-        if (n_nodes >= a.ncap) {
-          fail |= kFailCap;
-          return false;
+        // per-node best route / seconds (centrality.rs:695-703)
+        uint64_t nk = nkv[q];
+        float best = brv[q], bsec = bsv[q];
+        if (dup_n) {
+          nk = L.nkey[v];
+          if ((uint32_t)(nk >> 32) == epoch) {
+            best = *il<float>(w, a.o_brt, (uint32_t)nk, lane);
+            bsec = *il<float>(w, a.o_bsec, (uint32_t)nk, lane);
+          }
         }
-        nsl = n_nodes++;
-        L.nkey[v] = ((uint64_t)epoch << 32) | nsl;
-        *il<uint32_t>(w, a.o_node, nsl, lane) = v;
-        *il<float>(w, a.o_brt, nsl, lane) = INFINITY;
-        *il<float>(w, a.o_bsec, nsl, lane) = INFINITY;
-      }
-      float *brt = il<float>(w, a.o_brt, nsl, lane);
-      float *bsec = il<float>(w, a.o_bsec, nsl, lane);
-      const float best = *brt;
-      if (cr < best * one_m) {
-        *brt = cr;
-        *bsec = cs;
-      } else if (cr <= best * one_p) {
-        *bsec = fminf(*bsec, cs);
+        uint32_t nsl;
+        if ((uint32_t)(nk >> 32) == epoch) {
+          nsl = (uint32_t)nk;
+        } else {
+          if (n_nodes >= a.ncap) {
+            fail |= kFailCap;
+            return false;
+          }
+          nsl = n_nodes++;
+          L.nkey[v] = ((uint64_t)epoch << 32) | nsl;
+          *il<uint32_t>(w, a.o_node, nsl, lane) = v;
+          best = bsec = INFINITY;
+        }
+        if (cr < best * one_m) {
+          *il<float>(w, a.o_brt, nsl, lane) = cr;
+          *il<float>(w, a.o_bsec, nsl, lane) = cs;
+        } else if (cr <= best * one_p) {
+          *il<float>(w, a.o_bsec, nsl, lane) = fminf(bsec, cs);
+        } else if (!((uint32_t)(nk >> 32) == epoch)) {
+          // a new node slot keeps its initial infinite best
+          *il<float>(w, a.o_brt, nsl, lane) = INFINITY;
+          *il<float>(w, a.o_bsec, nsl, lane) = INFINITY;
+        }
       }
     }
   }
