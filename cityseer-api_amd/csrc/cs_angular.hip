@@ -107,8 +107,9 @@ struct Heap {
   __device__ __forceinline__ static bool less(const uint4 &a, const uint4 &b) {
     return hless(__uint_as_float(a.x), __uint_as_float(a.y), a.z, __uint_as_float(b.x), __uint_as_float(b.y), b.z);
   }
-  __device__ __forceinline__ void push(float k, float s, uint32_t idx) {
-    const uint4 e = make_uint4(__float_as_uint(k), __float_as_uint(s), idx, 0u);
+  // `slot` rides along outside the key (the state's workspace slot)
+  __device__ __forceinline__ void push(float k, float s, uint32_t idx, uint32_t slot = 0u) {
+    const uint4 e = make_uint4(__float_as_uint(k), __float_as_uint(s), idx, slot);
     uint32_t i = n++;
     while (i > 0) {
       const uint32_t p = (i - 1) / 4;
@@ -171,29 +172,31 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
     *il<double>(w, a.o_sigma, ss, lane) = 1.0;
     *il<uint32_t>(w, a.o_state, ss, lane) = s;  // visited flag in bit 31
     *il<uint32_t>(w, a.o_np, ss, lane) = 0;
-    h.push(0.0f, 0.0f, s);
+    h.push(0.0f, 0.0f, s, ss);
   }
   while (h.n > 0) {
     const uint4 top = h.pop();
     const uint32_t us = top.z;
-    const uint32_t uss = (uint32_t)L.skey[us];
+    const uint32_t uss = top.w;  // the state's slot, carried by the heap entry
+    const uint32_t u = us >> 1, entry = us & 1u;
+    // the state's fields and its out-row bounds are independent loads
     uint32_t *ust = il<uint32_t>(w, a.o_state, uss, lane);
-    if (*ust & 0x80000000u) continue;
+    const uint32_t ustate = *ust;
+    const float u_route = *il<float>(w, a.o_route, uss, lane);
+    const float u_secs = *il<float>(w, a.o_secs, uss, lane);
+    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane);
+    const uint32_t j0 = jb.off[u], j1 = jb.off[u + 1];
+    if (ustate & 0x80000000u) continue;
     *ust = us | 0x80000000u;
     if (n_order >= a.scap) {
       fail |= kFailCap;
       return false;
     }
     *il<uint32_t>(w, a.o_order, n_order++, lane) = uss;
-    const uint32_t u = us >> 1, entry = us & 1u;
-    const float u_route = *il<float>(w, a.o_route, uss, lane);
-    const float u_secs = *il<float>(w, a.o_secs, uss, lane);
-    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane);
     // out-edges in chunks of kRelax: the adjacency entries, then the state and
     // node keys, then the state and node fields are loaded for the whole chunk
     // at once; the edges are then applied one by one in row order, re-reading
     // a state or node that an earlier edge of the chunk has just written
-    const uint32_t j0 = jb.off[u], j1 = jb.off[u + 1];
     for (uint32_t jc = j0; jc < j1; jc += kRelax) {
       uint4 e[kRelax];
 #pragma unroll
@@ -292,7 +295,7 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
             fail |= kFailHeap;
             return false;
           }
-          h.push(cr, cs, vs);
+          h.push(cr, cs, vs, vss);
         } else if (cr <= cur * one_p) {
           const uint32_t k = *np;
           bool has = false;
