@@ -57,7 +57,7 @@ struct AngArgs {
   char *ws;
   size_t lane_stride;  // bytes of the lane-major maps per lane
   size_t wave_stride;  // bytes of the interleaved arrays per wave
-  uint32_t scap, ncap, hcap, maxp;
+  uint32_t scap, ncap, hcap, maxp, tcap;  // per-lane capacities (tcap: thresholds of the delta/seed blocks)
   size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_delta, o_seed, o_node, o_brt, o_bsec,
       o_heap;
   char *wave_ws;
@@ -75,10 +75,13 @@ struct AngArgs {
   uint32_t *t_norder;
 };
 
-// Interleaved element i of this lane in a wave-shared array of 64-lane rows.
+// Element i of this lane's own block in a wave-shared array of 64 lane blocks
+// of `count` elements. Lanes search different sources at unrelated indices, so
+// nothing coalesces across lanes; a lane's neighbouring elements (the four
+// children of a heap node, consecutive settle positions) share a cache line.
 template <typename T>
-__device__ __forceinline__ T *il(char *base, size_t off, uint32_t i, uint32_t lane) {
-  return reinterpret_cast<T *>(base + off) + (size_t)i * kWave + lane;
+__device__ __forceinline__ T *il(char *base, size_t off, uint32_t i, uint32_t lane, size_t count) {
+  return reinterpret_cast<T *>(base + off) + (size_t)lane * count + i;
 }
 
 struct LaneWs {
@@ -103,7 +106,8 @@ struct Heap {
   size_t off;
   uint32_t lane;
   uint32_t n;
-  __device__ __forceinline__ uint4 *at(uint32_t i) { return il<uint4>(w, off, i, lane); }
+  uint32_t cap;
+  __device__ __forceinline__ uint4 *at(uint32_t i) { return il<uint4>(w, off, i, lane, cap); }
   __device__ __forceinline__ static bool less(const uint4 &a, const uint4 &b) {
     return hless(__uint_as_float(a.x), __uint_as_float(a.y), a.z, __uint_as_float(b.x), __uint_as_float(b.y), b.z);
   }
@@ -156,22 +160,22 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
   char *w = L.w;
   const uint32_t lane = L.lane;
   n_states = n_nodes = n_order = 0;
-  Heap h{w, a.o_heap, lane, 0};
+  Heap h{w, a.o_heap, lane, 0, a.hcap};
   // node slot of the source
   L.nkey[src] = ((uint64_t)epoch << 32) | 0u;
-  *il<uint32_t>(w, a.o_node, 0, lane) = src;
-  *il<float>(w, a.o_brt, 0, lane) = 0.0f;
-  *il<float>(w, a.o_bsec, 0, lane) = 0.0f;
+  *il<uint32_t>(w, a.o_node, 0, lane, a.ncap) = src;
+  *il<float>(w, a.o_brt, 0, lane, a.ncap) = 0.0f;
+  *il<float>(w, a.o_bsec, 0, lane, a.ncap) = 0.0f;
   n_nodes = 1;
   for (uint32_t slot = 0; slot < 2; ++slot) {
     const uint32_t s = 2 * src + slot;
     const uint32_t ss = n_states++;
     L.skey[s] = ((uint64_t)epoch << 32) | ss;
-    *il<float>(w, a.o_route, ss, lane) = 0.0f;
-    *il<float>(w, a.o_secs, ss, lane) = 0.0f;
-    *il<double>(w, a.o_sigma, ss, lane) = 1.0;
-    *il<uint32_t>(w, a.o_state, ss, lane) = s;  // visited flag in bit 31
-    *il<uint32_t>(w, a.o_np, ss, lane) = 0;
+    *il<float>(w, a.o_route, ss, lane, a.scap) = 0.0f;
+    *il<float>(w, a.o_secs, ss, lane, a.scap) = 0.0f;
+    *il<double>(w, a.o_sigma, ss, lane, a.scap) = 1.0;
+    *il<uint32_t>(w, a.o_state, ss, lane, a.scap) = s;  // visited flag in bit 31
+    *il<uint32_t>(w, a.o_np, ss, lane, a.scap) = 0;
     h.push(0.0f, 0.0f, s, ss);
   }
   while (h.n > 0) {
@@ -180,11 +184,11 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
     const uint32_t uss = top.w;  // the state's slot, carried by the heap entry
     const uint32_t u = us >> 1, entry = us & 1u;
     // the state's fields and its out-row bounds are independent loads
-    uint32_t *ust = il<uint32_t>(w, a.o_state, uss, lane);
+    uint32_t *ust = il<uint32_t>(w, a.o_state, uss, lane, a.scap);
     const uint32_t ustate = *ust;
-    const float u_route = *il<float>(w, a.o_route, uss, lane);
-    const float u_secs = *il<float>(w, a.o_secs, uss, lane);
-    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane);
+    const float u_route = *il<float>(w, a.o_route, uss, lane, a.scap);
+    const float u_secs = *il<float>(w, a.o_secs, uss, lane, a.scap);
+    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane, a.scap);
     const uint32_t j0 = jb.off[u], j1 = jb.off[u + 1];
     if (ustate & 0x80000000u) continue;
     *ust = us | 0x80000000u;
@@ -192,7 +196,7 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
       fail |= kFailCap;
       return false;
     }
-    *il<uint32_t>(w, a.o_order, n_order++, lane) = uss;
+    *il<uint32_t>(w, a.o_order, n_order++, lane, a.scap) = uss;
     // out-edges in chunks of kRelax: the adjacency entries, then the state and
     // node keys, then the state and node fields are loaded for the whole chunk
     // at once; the edges are then applied one by one in row order, re-reading
@@ -221,12 +225,12 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
         flv[q] = 0u;
         rtv[q] = brv[q] = bsv[q] = INFINITY;
         if (ok[q] && (uint32_t)(skv[q] >> 32) == epoch) {
-          flv[q] = *il<uint32_t>(w, a.o_state, (uint32_t)skv[q], lane);
-          rtv[q] = *il<float>(w, a.o_route, (uint32_t)skv[q], lane);
+          flv[q] = *il<uint32_t>(w, a.o_state, (uint32_t)skv[q], lane, a.scap);
+          rtv[q] = *il<float>(w, a.o_route, (uint32_t)skv[q], lane, a.scap);
         }
         if (ok[q] && (uint32_t)(nkv[q] >> 32) == epoch) {
-          brv[q] = *il<float>(w, a.o_brt, (uint32_t)nkv[q], lane);
-          bsv[q] = *il<float>(w, a.o_bsec, (uint32_t)nkv[q], lane);
+          brv[q] = *il<float>(w, a.o_brt, (uint32_t)nkv[q], lane, a.ncap);
+          bsv[q] = *il<float>(w, a.o_bsec, (uint32_t)nkv[q], lane, a.ncap);
         }
       }
 #pragma unroll
@@ -246,8 +250,8 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
         if (dup_s) {
           sk = L.skey[vs];
           if ((uint32_t)(sk >> 32) == epoch) {
-            fl = *il<uint32_t>(w, a.o_state, (uint32_t)sk, lane);
-            rt = *il<float>(w, a.o_route, (uint32_t)sk, lane);
+            fl = *il<uint32_t>(w, a.o_state, (uint32_t)sk, lane, a.scap);
+            rt = *il<float>(w, a.o_route, (uint32_t)sk, lane, a.scap);
           }
         }
         uint32_t vss;
@@ -263,33 +267,33 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
           }
           vss = n_states++;
           L.skey[vs] = ((uint64_t)epoch << 32) | vss;
-          *il<float>(w, a.o_route, vss, lane) = INFINITY;
-          *il<float>(w, a.o_secs, vss, lane) = INFINITY;
-          *il<double>(w, a.o_sigma, vss, lane) = 0.0;
-          *il<uint32_t>(w, a.o_state, vss, lane) = vs;
-          *il<uint32_t>(w, a.o_np, vss, lane) = 0;
+          *il<float>(w, a.o_route, vss, lane, a.scap) = INFINITY;
+          *il<float>(w, a.o_secs, vss, lane, a.scap) = INFINITY;
+          *il<double>(w, a.o_sigma, vss, lane, a.scap) = 0.0;
+          *il<uint32_t>(w, a.o_state, vss, lane, a.scap) = vs;
+          *il<uint32_t>(w, a.o_np, vss, lane, a.scap) = 0;
           cur = INFINITY;
         }
         float cr = u_route + __uint_as_float(e[q].z);
         cr = cr + __uint_as_float(e[q].w);
-        uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane);
-        uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)vss * a.maxp) * kWave + lane;
+        uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane, a.scap);
+        uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)lane * a.scap + vss) * a.maxp;
         if (cr < cur) {
-          double *sg = il<double>(w, a.o_sigma, vss, lane);
+          double *sg = il<double>(w, a.o_sigma, vss, lane, a.scap);
           if (cr < cur * one_m) {
             *np = 0;
             *sg = u_sigma;
           } else {
             *sg = *sg + u_sigma;
           }
-          *il<float>(w, a.o_route, vss, lane) = cr;
-          *il<float>(w, a.o_secs, vss, lane) = cs;
+          *il<float>(w, a.o_route, vss, lane, a.scap) = cr;
+          *il<float>(w, a.o_secs, vss, lane, a.scap) = cs;
           const uint32_t k = *np;
           if (k >= a.maxp) {
             fail |= kFailPreds;
             return false;
           }
-          preds[(size_t)k * kWave] = uss;
+          preds[k] = uss;
           *np = k + 1;
           if (h.n >= a.hcap) {
             fail |= kFailHeap;
@@ -300,17 +304,17 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
           const uint32_t k = *np;
           bool has = false;
           for (uint32_t r = 0; r < k; ++r)
-            if (preds[(size_t)r * kWave] == uss) has = true;
+            if (preds[r] == uss) has = true;
           if (!has) {
-            float *vsec = il<float>(w, a.o_secs, vss, lane);
+            float *vsec = il<float>(w, a.o_secs, vss, lane, a.scap);
             if (cs < *vsec) *vsec = cs;
             if (k >= a.maxp) {
               fail |= kFailPreds;
               return false;
             }
-            preds[(size_t)k * kWave] = uss;
+            preds[k] = uss;
             *np = k + 1;
-            double *sg = il<double>(w, a.o_sigma, vss, lane);
+            double *sg = il<double>(w, a.o_sigma, vss, lane, a.scap);
             *sg = *sg + u_sigma;
           }
         }
@@ -320,8 +324,8 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
         if (dup_n) {
           nk = L.nkey[v];
           if ((uint32_t)(nk >> 32) == epoch) {
-            best = *il<float>(w, a.o_brt, (uint32_t)nk, lane);
-            bsec = *il<float>(w, a.o_bsec, (uint32_t)nk, lane);
+            best = *il<float>(w, a.o_brt, (uint32_t)nk, lane, a.ncap);
+            bsec = *il<float>(w, a.o_bsec, (uint32_t)nk, lane, a.ncap);
           }
         }
         uint32_t nsl;
@@ -334,18 +338,18 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
           }
           nsl = n_nodes++;
           L.nkey[v] = ((uint64_t)epoch << 32) | nsl;
-          *il<uint32_t>(w, a.o_node, nsl, lane) = v;
+          *il<uint32_t>(w, a.o_node, nsl, lane, a.ncap) = v;
           best = bsec = INFINITY;
         }
         if (cr < best * one_m) {
-          *il<float>(w, a.o_brt, nsl, lane) = cr;
-          *il<float>(w, a.o_bsec, nsl, lane) = cs;
+          *il<float>(w, a.o_brt, nsl, lane, a.ncap) = cr;
+          *il<float>(w, a.o_bsec, nsl, lane, a.ncap) = cs;
         } else if (cr <= best * one_p) {
-          *il<float>(w, a.o_bsec, nsl, lane) = fminf(bsec, cs);
+          *il<float>(w, a.o_bsec, nsl, lane, a.ncap) = fminf(bsec, cs);
         } else if (!((uint32_t)(nk >> 32) == epoch)) {
           // a new node slot keeps its initial infinite best
-          *il<float>(w, a.o_brt, nsl, lane) = INFINITY;
-          *il<float>(w, a.o_bsec, nsl, lane) = INFINITY;
+          *il<float>(w, a.o_brt, nsl, lane, a.ncap) = INFINITY;
+          *il<float>(w, a.o_bsec, nsl, lane, a.ncap) = INFINITY;
         }
       }
     }
@@ -363,21 +367,21 @@ __device__ bool angular_phase2(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
   char *w = L.w;
   const uint32_t lane = L.lane;
   for (uint32_t q = 0; q < n_order; ++q) {
-    const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
-    *il<double>(w, a.o_delta, ss, lane) = (double)q;
-    *il<uint32_t>(w, a.o_np, ss, lane) = 0;
-    *il<double>(w, a.o_sigma, ss, lane) = 0.0;
+    const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane, a.scap);
+    *il<double>(w, a.o_delta, ss, lane, (size_t)a.scap * a.tcap) = (double)q;
+    *il<uint32_t>(w, a.o_np, ss, lane, a.scap) = 0;
+    *il<double>(w, a.o_sigma, ss, lane, a.scap) = 0.0;
   }
   // the source's two states are the first two slots
-  *il<double>(w, a.o_sigma, 0, lane) = 1.0;
-  *il<double>(w, a.o_sigma, 1, lane) = 1.0;
+  *il<double>(w, a.o_sigma, 0, lane, a.scap) = 1.0;
+  *il<double>(w, a.o_sigma, 1, lane, a.scap) = 1.0;
   const float band = 1.0f + jb.tolerance;
   for (uint32_t pos = 0; pos < n_order; ++pos) {
-    const uint32_t uss = *il<uint32_t>(w, a.o_order, pos, lane);
-    const uint32_t us = *il<uint32_t>(w, a.o_state, uss, lane) & 0x7fffffffu;
+    const uint32_t uss = *il<uint32_t>(w, a.o_order, pos, lane, a.scap);
+    const uint32_t us = *il<uint32_t>(w, a.o_state, uss, lane, a.scap) & 0x7fffffffu;
     const uint32_t u = us >> 1, entry = us & 1u;
-    const float u_route = *il<float>(w, a.o_route, uss, lane);
-    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane);
+    const float u_route = *il<float>(w, a.o_route, uss, lane, a.scap);
+    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane, a.scap);
     for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
       const uint4 e = jb.adj[j];
       if (((e.x >> 30) & 1u) != 1u - entry) continue;
@@ -386,25 +390,25 @@ __device__ bool angular_phase2(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
       const uint64_t sk = L.skey[vs];
       if ((uint32_t)(sk >> 32) != epoch) continue;
       const uint32_t vss = (uint32_t)sk;
-      if (!(*il<uint32_t>(w, a.o_state, vss, lane) & 0x80000000u)) continue;  // never settled
-      if (*il<double>(w, a.o_delta, vss, lane) <= (double)pos) continue;       // settled before u
+      if (!(*il<uint32_t>(w, a.o_state, vss, lane, a.scap) & 0x80000000u)) continue;  // never settled
+      if (*il<double>(w, a.o_delta, vss, lane, (size_t)a.scap * a.tcap) <= (double)pos) continue;       // settled before u
       float cr = u_route + __uint_as_float(e.z);
       cr = cr + __uint_as_float(e.w);
-      if (!(cr <= *il<float>(w, a.o_route, vss, lane) * band)) continue;
-      uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane);
-      uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)vss * a.maxp) * kWave + lane;
+      if (!(cr <= *il<float>(w, a.o_route, vss, lane, a.scap) * band)) continue;
+      uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane, a.scap);
+      uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)lane * a.scap + vss) * a.maxp;
       const uint32_t k = *np;
       bool has = false;
       for (uint32_t q = 0; q < k; ++q)
-        if (preds[(size_t)q * kWave] == uss) has = true;
+        if (preds[q] == uss) has = true;
       if (has) continue;
       if (k >= a.maxp) {
         fail |= kFailPreds;
         return false;
       }
-      preds[(size_t)k * kWave] = uss;
+      preds[k] = uss;
       *np = k + 1;
-      double *sg = il<double>(w, a.o_sigma, vss, lane);
+      double *sg = il<double>(w, a.o_sigma, vss, lane, a.scap);
       *sg = *sg + u_sigma;
     }
   }
@@ -453,10 +457,10 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     const uint32_t T = jb.T;
     // ---- closeness at targets (centrality.rs:1995-2033)
     for (uint32_t k = 1; k < n_nodes; ++k) {
-      const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane);
+      const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane, a.ncap);
       if (v == src) continue;
-      const float bsd = *il<float>(w, a.o_brt, k, lane);
-      const float bs = *il<float>(w, a.o_bsec, k, lane);
+      const float bsd = *il<float>(w, a.o_brt, k, lane, a.ncap);
+      const float bs = *il<float>(w, a.o_bsec, k, lane, a.ncap);
       if (!isfinite(bsd) || !isfinite(bs)) continue;
       double *accv = jb.acc + v;
       const size_t NN = jb.N;
@@ -480,18 +484,18 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     if (jb.flags & 2u) {
       const size_t sc = a.scap;
       for (uint32_t q = 0; q < n_order; ++q) {
-        const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
+        const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane, a.scap);
         for (uint32_t t = 0; t < T; ++t) {
-          *il<double>(w, a.o_delta, t * sc + ss, lane) = 0.0;
-          *il<double>(w, a.o_seed, t * sc + ss, lane) = 0.0;
+          *il<double>(w, a.o_delta, t * sc + ss, lane, (size_t)a.scap * a.tcap) = 0.0;
+          *il<double>(w, a.o_seed, t * sc + ss, lane, (size_t)a.scap * a.tcap) = 0.0;
         }
       }
       // seeds: best terminal states split by sigma (best_angular_target_states)
       for (uint32_t k = 1; k < n_nodes; ++k) {
-        const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane);
+        const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane, a.ncap);
         if (v == src) continue;
-        const float br = *il<float>(w, a.o_brt, k, lane);
-        const float bs = *il<float>(w, a.o_bsec, k, lane);
+        const float br = *il<float>(w, a.o_brt, k, lane, a.ncap);
+        const float bs = *il<float>(w, a.o_bsec, k, lane, a.ncap);
         if (!isfinite(br) || !isfinite(bs)) continue;
         uint32_t cand[2];
         uint32_t ncand = 0;
@@ -509,46 +513,46 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           double total = 0.0;
           for (uint32_t q = 0; q < ncand; ++q) {
             const uint32_t ss = cand[q];
-            const double sg = *il<double>(w, a.o_sigma, ss, lane);
-            if (sg == 0.0 || *il<float>(w, a.o_secs, ss, lane) > sthr) continue;
-            if (*il<float>(w, a.o_route, ss, lane) <= br * (1.0f + jb.tolerance)) best[nbst++] = ss;
+            const double sg = *il<double>(w, a.o_sigma, ss, lane, a.scap);
+            if (sg == 0.0 || *il<float>(w, a.o_secs, ss, lane, a.scap) > sthr) continue;
+            if (*il<float>(w, a.o_route, ss, lane, a.scap) <= br * (1.0f + jb.tolerance)) best[nbst++] = ss;
           }
           if (nbst == 0) continue;
-          for (uint32_t q = 0; q < nbst; ++q) total += *il<double>(w, a.o_sigma, best[q], lane);
+          for (uint32_t q = 0; q < nbst; ++q) total += *il<double>(w, a.o_sigma, best[q], lane, a.scap);
           if (total == 0.0) continue;
           for (uint32_t q = 0; q < nbst; ++q) {
-            double *sd = il<double>(w, a.o_seed, t * sc + best[q], lane);
-            *sd = *sd + pc * (*il<double>(w, a.o_sigma, best[q], lane) / total);
+            double *sd = il<double>(w, a.o_seed, t * sc + best[q], lane, (size_t)a.scap * a.tcap);
+            *sd = *sd + pc * (*il<double>(w, a.o_sigma, best[q], lane, a.scap) / total);
           }
         }
       }
       // states by route descending, equal routes in settle order (stable sort)
       int hi = (int)n_order - 1;
       while (hi >= 0) {
-        const float r_hi = *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, hi, lane), lane);
+        const float r_hi = *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, hi, lane, a.scap), lane, a.scap);
         int lo = hi;
-        while (lo > 0 && *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, lo - 1, lane), lane) == r_hi) --lo;
+        while (lo > 0 && *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, lo - 1, lane, a.scap), lane, a.scap) == r_hi) --lo;
         for (int q = lo; q <= hi; ++q) {
-          const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane);
-          const float secs_ss = *il<float>(w, a.o_secs, ss, lane);
+          const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane, a.scap);
+          const float secs_ss = *il<float>(w, a.o_secs, ss, lane, a.scap);
           bool any = false;
           for (uint32_t t = 0; t < T; ++t) any = any || secs_ss <= jb.sec_thr[t];
           if (!any) continue;
-          const double sw = *il<double>(w, a.o_sigma, ss, lane);
+          const double sw = *il<double>(w, a.o_sigma, ss, lane, a.scap);
           if (sw == 0.0) continue;
-          const uint32_t k = *il<uint32_t>(w, a.o_np, ss, lane);
-          const uint32_t *preds = reinterpret_cast<const uint32_t *>(w + a.o_preds) + ((size_t)ss * a.maxp) * kWave + lane;
-          const uint32_t node = (*il<uint32_t>(w, a.o_state, ss, lane) & 0x7fffffffu) >> 1;
+          const uint32_t k = *il<uint32_t>(w, a.o_np, ss, lane, a.scap);
+          const uint32_t *preds = reinterpret_cast<const uint32_t *>(w + a.o_preds) + ((size_t)lane * a.scap + ss) * a.maxp;
+          const uint32_t node = (*il<uint32_t>(w, a.o_state, ss, lane, a.scap) & 0x7fffffffu) >> 1;
           for (uint32_t t = 0; t < T; ++t) {
             if (!(secs_ss <= jb.sec_thr[t])) continue;
-            const double seed = *il<double>(w, a.o_seed, t * sc + ss, lane);
-            const double dep = seed + *il<double>(w, a.o_delta, t * sc + ss, lane);
+            const double seed = *il<double>(w, a.o_seed, t * sc + ss, lane, (size_t)a.scap * a.tcap);
+            const double dep = seed + *il<double>(w, a.o_delta, t * sc + ss, lane, (size_t)a.scap * a.tcap);
             if (dep == 0.0) continue;
             for (uint32_t p = 0; p < k; ++p) {
-              const uint32_t ps = preds[(size_t)p * kWave];
-              const double sv = *il<double>(w, a.o_sigma, ps, lane);
+              const uint32_t ps = preds[p];
+              const double sv = *il<double>(w, a.o_sigma, ps, lane, a.scap);
               if (sv == 0.0) continue;
-              double *dp = il<double>(w, a.o_delta, t * sc + ps, lane);
+              double *dp = il<double>(w, a.o_delta, t * sc + ps, lane, (size_t)a.scap * a.tcap);
               *dp = *dp + (sv / sw) * dep;
             }
             if (node == src) continue;
@@ -583,7 +587,7 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
   // state slots: route metric (o_route), agg seconds (o_secs), simplest
   // distance (o_delta, an f32 value held exactly in an f64 slot)
   uint32_t n_states = 0, no = 0;
-  Heap h{w, a.o_heap, 0, 0};
+  Heap h{w, a.o_heap, 0, 0, a.hcap};
   a.t_order[no++] = src;
   a.t_vis[src] = 1;
   a.t_simpl[src] = 0.0f;
@@ -592,22 +596,22 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
     const uint32_t s = 2 * src + slot;
     const uint32_t ss = n_states++;
     L.skey[s] = ((uint64_t)epoch << 32) | ss;
-    *il<float>(w, a.o_route, ss, 0) = 0.0f;            // route metric
-    *il<float>(w, a.o_secs, ss, 0) = 0.0f;             // agg seconds
-    *il<double>(w, a.o_delta, ss, 0) = 0.0;            // simpl dist (f32 stored in f64 slot)
-    *il<uint32_t>(w, a.o_state, ss, 0) = s;
+    *il<float>(w, a.o_route, ss, 0, a.scap) = 0.0f;            // route metric
+    *il<float>(w, a.o_secs, ss, 0, a.scap) = 0.0f;             // agg seconds
+    *il<double>(w, a.o_delta, ss, 0, (size_t)a.scap * a.tcap) = 0.0;            // simpl dist (f32 stored in f64 slot)
+    *il<uint32_t>(w, a.o_state, ss, 0, a.scap) = s;
     h.push(0.0f, 0.0f, s);
   }
   while (h.n > 0) {
     const uint4 top = h.pop();
     const uint32_t us = top.z;
     const uint32_t uss = (uint32_t)L.skey[us];
-    uint32_t *ust = il<uint32_t>(w, a.o_state, uss, 0);
+    uint32_t *ust = il<uint32_t>(w, a.o_state, uss, 0, a.scap);
     if (*ust & 0x80000000u) continue;
     *ust = us | 0x80000000u;
     const uint32_t u = us >> 1, entry = us & 1u;
-    const float u_secs = *il<float>(w, a.o_secs, uss, 0);
-    const float u_simpl = (float)*il<double>(w, a.o_delta, uss, 0);
+    const float u_secs = *il<float>(w, a.o_secs, uss, 0, a.scap);
+    const float u_simpl = (float)*il<double>(w, a.o_delta, uss, 0, (size_t)a.scap * a.tcap);
     for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
       const uint4 e = jb.adj[j];
       if (((e.x >> 30) & 1u) != 1u - entry) continue;
@@ -622,8 +626,8 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
       float vmetric, vsecs;
       if ((uint32_t)(sk >> 32) == epoch) {
         vss = (uint32_t)sk;
-        vmetric = *il<float>(w, a.o_route, vss, 0);
-        vsecs = *il<float>(w, a.o_secs, vss, 0);
+        vmetric = *il<float>(w, a.o_route, vss, 0, a.scap);
+        vsecs = *il<float>(w, a.o_secs, vss, 0, a.scap);
       } else {
         vss = 0xffffffffu;
         vmetric = INFINITY;
@@ -639,11 +643,11 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
           }
           vss = n_states++;
           L.skey[vs] = ((uint64_t)epoch << 32) | vss;
-          *il<uint32_t>(w, a.o_state, vss, 0) = vs;
+          *il<uint32_t>(w, a.o_state, vss, 0, a.scap) = vs;
         }
-        *il<float>(w, a.o_route, vss, 0) = cmetric;
-        *il<double>(w, a.o_delta, vss, 0) = (double)csimpl;
-        *il<float>(w, a.o_secs, vss, 0) = cs;
+        *il<float>(w, a.o_route, vss, 0, a.scap) = cmetric;
+        *il<double>(w, a.o_delta, vss, 0, (size_t)a.scap * a.tcap) = (double)csimpl;
+        *il<float>(w, a.o_secs, vss, 0, a.scap) = cs;
         if (h.n >= a.hcap) {
           atomicOr(a.fail_flags, kFailHeap);
           return;
@@ -670,12 +674,13 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct AngPlan {
-  uint32_t scap, ncap, hcap, maxp, lanes;
+  uint32_t scap, ncap, hcap, maxp, lanes, tcap;
   size_t lane_stride, wave_stride;
   size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_delta, o_seed, o_node, o_brt, o_bsec, o_heap;
 };
 
 void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
+  p.tcap = std::max<uint32_t>(T, 1);
   p.lane_stride = align256(8ull * 3 * N);  // skey [2N] + nkey [N]
   size_t o = 0;
   auto take = [&](size_t &field, size_t elem_bytes, size_t count) {
@@ -890,6 +895,7 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     a.ncap = p.ncap;
     a.hcap = p.hcap;
     a.maxp = p.maxp;
+    a.tcap = p.tcap;
     a.o_route = p.o_route;
     a.o_secs = p.o_secs;
     a.o_sigma = p.o_sigma;
@@ -1004,6 +1010,7 @@ void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, fl
   a.ncap = p.ncap;
   a.hcap = p.hcap;
   a.maxp = p.maxp;
+  a.tcap = p.tcap;
   a.o_route = p.o_route;
   a.o_secs = p.o_secs;
   a.o_sigma = p.o_sigma;
