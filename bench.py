@@ -46,13 +46,13 @@ def build_workload(side: int, diag: float, seed: int):
     return lat, ns
 
 
-def profiled_traffic(name: str = "k_search_brandes_cfg2_hbm.md"):
+def profiled_traffic(key: str = "traffic_bytes_per_launch", name: str = "k_search_brandes_cfg2_hbm.md"):
     """HBM bytes per launch of the dominant kernel on this workload, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (counters scaled by a plain-copy calibration); None if absent."""
     try:
         with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return int(float(line.split(":", 1)[1]))
     except OSError:
         pass
@@ -194,7 +194,7 @@ def run_variant(args) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic("simplest_traffic_bytes_per_launch") if kind == "simplest" else None,
             "kernel_ms": round(kms, 3),
             "algorithmic_bytes": int(algo_bytes),
             "edges_scanned": int(stats["edges_scanned"]),
