@@ -35,6 +35,7 @@
 
 // 64-bit wave reductions of the device library (declared by HIP only under an
 // opt-in macro).
+extern "C" __device__ uint32_t __ockl_wfscan_add_u32(uint32_t, bool);
 extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_add_u64(unsigned long long);
 extern "C" __device__ __attribute__((const)) double __ockl_wfred_add_f64(double);
 #include <cmath>
@@ -45,11 +46,14 @@ namespace cs {
 namespace {
 
 constexpr uint32_t kFailCap = 1u, kFailBucket = 2u, kFailRing = 4u, kFailInternal = 8u;
-constexpr int kChunk = 1;      // edges per lane per batch: one (wider batches oversubscribe the memory pipeline)
+constexpr uint32_t kFoldChunk = 4;  // out-row candidates whose loads one fold step batches
+constexpr int kChunk = 1;           // in-edges per lane per step of the segment closeness walk
 constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dealt to them round-robin
 constexpr uint32_t kScanQ = 4;     // key words per lane per step of the node-ordered result flush
 constexpr uint32_t kFlushList = kScanQ * 64;  // LDS list of one flush step (reached nodes, index order)
 constexpr uint32_t kFlushWords = 3 * kFlushList;  // {node, seconds bits, slot} per list entry
+constexpr uint32_t kRelWords = 5 * 64;  // per lane {first edge, in-degree, node, seconds, row start} of a batch
+constexpr uint32_t kWaveLds = kFlushWords + kRelWords;  // LDS words per wave besides the ring
 constexpr uint32_t kScanSpan = 16; // scan the index span only while it is at most this many times the reach
 constexpr uint32_t kEpochMax = 1023u;       // 10-bit epoch tags 1..1023; 0 is never issued
 constexpr uint32_t kSlotLimit = 1u << 22;   // 22-bit slots in the node key
@@ -202,6 +206,7 @@ struct WaveWs {
   uint32_t *lvl;    // [maxb + 2] highest Phase 2 resolution round per bucket
   uint32_t *ring;   // LDS [nbk] bucket ring counters
   uint32_t *list;   // LDS [3][kFlushList] {node, seconds bits, slot} list of the result flush
+  uint32_t *rel;    // LDS [5][64] edge-parallel relaxation table of one batch
 };
 
 __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uint32_t *ring) {
@@ -216,6 +221,7 @@ __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uin
   w.lvl = reinterpret_cast<uint32_t *>(base + a.o_lvl);
   w.ring = ring;
   w.list = ring + a.nbk_lds;
+  w.rel = w.list + kFlushWords;
   return w;
 }
 
@@ -404,7 +410,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
         vhi = v > vhi ? v : vhi;
         const float secs_v = __uint_as_float(csb);
         const bool is_src = (s == 0);
-        const uint32_t f0 = rw.x, nf = rw.y - rw.x, r0 = rw.z, nr = rw.w - rw.z;
+        const uint32_t f0 = rw.x, nf = rw.y - rw.x, nr = rw.w - rw.z;
         st_edges += nr;
         // fold state (multi-predecessor: near candidates; trees: lexicographic min)
         const double thr_far = kMulti ? (double)secs_v * far_factor(nf) : 0.0;
@@ -413,26 +419,22 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
         float near_cs = INFINITY;
         float bcs = INFINITY, bsu = INFINITY;
         uint32_t bu = 0xffffffffu, bj = 0xffffffffu, bslot = 0;
-        const uint32_t nmax = nf > nr ? nf : nr;
-        for (uint32_t cb = 0; cb < nmax; cb += kChunk) {
-          uint2 fe[kChunk], re[kChunk];
+        // ---- fold over the out-row, one node per lane: the candidates' loads
+        // go out four at a time, then the candidates are weighed in row order
+        const uint32_t nfold = (is_src || !(fold || cycles)) ? 0u : nf;
+        for (uint32_t cb = 0; cb < nfold; cb += kFoldChunk) {
+          uint2 fe[kFoldChunk];
+          uint64_t fk[kFoldChunk], fp[kFoldChunk];
+          uint32_t fs[kFoldChunk];
 #pragma unroll
-          for (int q = 0; q < kChunk; ++q) {
-            fe[q] = (cb + q < nf) ? j.fwd_adj[f0 + cb + q] : make_uint2(v, 0u);
-            re[q] = (cb + q < nr) ? j.rev_adj[r0 + cb + q] : make_uint2(v, 0u);
-          }
-          // issue every key load and relaxation before consuming any result
-          // (slot and payload are loaded with the keys: a later dependent load
-          // would add a round trip per bucket, which costs more than the bytes)
-          uint64_t fk[kChunk], rk[kChunk], fp[kChunk];
-          uint32_t fs[kChunk];
+          for (uint32_t q = 0; q < kFoldChunk; ++q) fe[q] = (cb + q < nf) ? j.fwd_adj[f0 + cb + q] : make_uint2(v, 0u);
 #pragma unroll
-          for (int q = 0; q < kChunk; ++q) {
+          for (uint32_t q = 0; q < kFoldChunk; ++q) {
             const uint32_t u = fe[q].x & kNodeMask;
             fk[q] = 0;
             fs[q] = 0;
             fp[q] = 0;
-            if (!is_src && u != v && (fold || (cycles && (fe[q].x & kCanonBit)))) {
+            if (u != v && (fold || (cycles && (fe[q].x & kCanonBit)))) {
               if (fold) {
                 // key, slot and payload of the candidate in one load
                 const uint4 r = nrec(w, u);
@@ -443,21 +445,12 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
                 fk[q] = ld64(nkey(w, u));
               }
             }
-            const uint32_t nb = re[q].x & kNodeMask;
-            const float cs = secs_v + __uint_as_float(re[q].y);
-            rk[q] = 0;
-            if (nb != v) {
-              if (cs <= j.maxsec)
-                rk[q] = wmax64(nkey(w, nb), pack_key(c.tag, cs));
-              else if (cycles && (re[q].x & kCanonBit))
-                rk[q] = ld64(nkey(w, nb));
-            }
           }
           // fold candidates v -> u (u settled in an earlier bucket)
 #pragma unroll
-          for (int q = 0; q < kChunk; ++q) {
+          for (uint32_t q = 0; q < kFoldChunk; ++q) {
             const uint32_t u = fe[q].x & kNodeMask;
-            if (is_src || u == v) continue;
+            if (u == v) continue;
             const uint64_t ku = fk[q];
             if (!key_current(ku, c.tag)) continue;
             const float su = key_secs(ku);
@@ -489,35 +482,6 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
                 bpay = fp[q];
               }
             }
-          }
-          // relaxation results nb -> v; the old key also tells whether nb settled
-          // before v (canonical edges count toward the circuit rank)
-#pragma unroll
-          for (int q = 0; q < kChunk; ++q) {
-            const uint32_t nb = re[q].x & kNodeMask;
-            if (nb == v) continue;
-            const float cs = secs_v + __uint_as_float(re[q].y);
-            const uint64_t kn = rk[q];
-            if (cs <= j.maxsec) {
-              const uint64_t nk = pack_key(c.tag, cs);
-              if (nk > kn) {
-                const uint32_t b = bucket_of(cs, j.inv_delta);
-                if (b <= c.k || b - c.k >= j.nbk) {
-                  lfail |= (b <= c.k) ? kFailInternal : kFailRing;
-                } else {
-                  const uint32_t rb = b % j.nbk;
-                  const uint32_t pos = lds_add(&w.ring[rb], 1u);
-                  if (pos < a.bcap)
-                    st64(reinterpret_cast<uint64_t *>(&w.bkt[rb * a.bcap + pos]),
-                         ((uint64_t)__float_as_uint(cs) << 32) | nb);
-                  else
-                    lfail |= kFailBucket;
-                  ++pushes;
-                }
-              }
-            }
-            if (cycles && (re[q].x & kCanonBit) && key_current(kn, c.tag) && key_less(key_secs(kn), nb, secs_v, v))
-              ++cnt_e;
           }
         }
         // settle record
@@ -593,8 +557,84 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           }
         }
       }
+      // ---- relaxations of this batch's settled nodes, one in-edge per lane: an
+      // exclusive scan of the in-degrees gives each node's first edge; the
+      // owner of edge g is the last lane whose first edge is <= g. The old key
+      // returned by the atomic max also tells whether the neighbour settled
+      // before the node (canonical edges count toward the circuit rank).
+      {
+        const uint32_t nrl = valid ? rw.w - rw.z : 0u;
+        const uint32_t first = __ockl_wfscan_add_u32(nrl, false);
+        const uint32_t tot = uniform(wave_sum(nrl));
+        if (tot) {
+          lds_st(&w.rel[lane], first);
+          lds_st(&w.rel[kWave + lane], nrl);
+          lds_st(&w.rel[2 * kWave + lane], v);
+          lds_st(&w.rel[3 * kWave + lane], csb);
+          lds_st(&w.rel[4 * kWave + lane], rw.z);
+          wave_fence();
+          for (uint32_t g0 = 0; g0 < tot; g0 += kWave) {
+            const uint32_t g = g0 + lane;
+            bool ce = false;
+            float route_o = 0.0f;
+            if (g < tot) {
+              uint32_t lo = 0, hi = kWave - 1;
+              while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (lds_ld(&w.rel[mid]) <= g) lo = mid;
+                else hi = mid - 1;
+              }
+              const uint32_t k = g - lds_ld(&w.rel[lo]);
+              if (k >= lds_ld(&w.rel[kWave + lo])) {
+                lfail |= kFailInternal;
+              } else {
+                const uint32_t vo = lds_ld(&w.rel[2 * kWave + lo]);
+                const float so = __uint_as_float(lds_ld(&w.rel[3 * kWave + lo]));
+                const uint2 re = j.rev_adj[lds_ld(&w.rel[4 * kWave + lo]) + k];
+                const uint32_t nb = re.x & kNodeMask;
+                if (nb != vo) {
+                  const float cs = so + __uint_as_float(re.y);
+                  uint64_t kn = 0;
+                  if (cs <= j.maxsec) {
+                    const uint64_t nk = pack_key(c.tag, cs);
+                    kn = wmax64(nkey(w, nb), nk);
+                    if (nk > kn) {
+                      const uint32_t b = bucket_of(cs, j.inv_delta);
+                      if (b <= c.k || b - c.k >= j.nbk) {
+                        lfail |= (b <= c.k) ? kFailInternal : kFailRing;
+                      } else {
+                        const uint32_t rb = b % j.nbk;
+                        const uint32_t pos = lds_add(&w.ring[rb], 1u);
+                        if (pos < a.bcap)
+                          st64(reinterpret_cast<uint64_t *>(&w.bkt[rb * a.bcap + pos]),
+                               ((uint64_t)__float_as_uint(cs) << 32) | nb);
+                        else
+                          lfail |= kFailBucket;
+                        ++pushes;
+                      }
+                    }
+                  } else if (cycles && (re.x & kCanonBit)) {
+                    kn = ld64(nkey(w, nb));
+                  }
+                  if (cycles && (re.x & kCanonBit) && key_current(kn, c.tag) && key_less(key_secs(kn), nb, so, vo)) {
+                    ce = true;
+                    route_o = so * speed;
+                  }
+                }
+              }
+            }
+            if constexpr (MODE == kBrandes) {
+              if (cycles) {
+#pragma unroll
+                for (int t = 0; t < NT; ++t) eband[t] += (uint32_t)__popcll(__ballot(ce && route_o <= j.dist[t]));
+              }
+            }
+          }
+          wave_fence();  // the next batch rewrites the LDS table
+        }
+      }
       // nodes within each threshold (circuit rank, reach) and the rank's edge
-      // counts, reduced over the wave
+      // counts from the out-row side, reduced over the wave
       if constexpr (MODE == kBrandes) {
         const float route_v = __uint_as_float(csb) * speed;
 #pragma unroll
@@ -857,7 +897,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
   const uint32_t wib = uniform(threadIdx.x / kWave);
   const uint32_t wid = uniform(blockIdx.x * (blockDim.x / kWave) + wib);
   const uint32_t xcd = a.xsplit ? blockIdx.x % kXcd : 0u;
-  const WaveWs w = wave_ws(a, wid, lds_rings + (size_t)wib * (a.nbk_lds + kFlushWords));
+  const WaveWs w = wave_ws(a, wid, lds_rings + (size_t)wib * (a.nbk_lds + kWaveLds));
   for (uint32_t r = lane; r < a.nbk_lds; r += kWave) lds_st(&w.ring[r], 0u);
   wave_fence();
   uint32_t epoch = a.epochs[wid];
@@ -1449,7 +1489,7 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   j.inv_delta = 1.0f / delta;
   const uint32_t jump = (uint32_t)std::ceil((double)de.max_es / (double)delta) + 2;
   j.nbk = jump + 1;
-  if (j.nbk > 4096 - kFlushWords) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
+  if (j.nbk > 4096 - kWaveLds) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
   j.maxb = (uint32_t)((double)cs.maxsec * (double)j.inv_delta) + 4;
   j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
   j.n_sources = js.n_sources;
@@ -1496,7 +1536,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
   const uint32_t wpb = block / kWave;
   p.bcap = 1024;
   p.cap = std::min<uint32_t>({maxN, tune.cap ? tune.cap : 16384u, kSlotLimit});
-  const size_t lds = (size_t)wpb * (p.nbk + kFlushWords) * 4;
+  const size_t lds = (size_t)wpb * (p.nbk + kWaveLds) * 4;
   const int bpc = std::max(1, blocks_per_cu<MODE>(NT, block, lds));
   uint32_t resident = di.cus * (uint32_t)bpc * wpb;
   if (tune.waves_per_simd) resident = std::min<uint32_t>(resident, di.cus * 4 * tune.waves_per_simd);
@@ -1704,7 +1744,7 @@ struct ShortestBatch {
     wpb = block / kWave;
     p.bcap = 1024;
     p.cap = std::min<uint32_t>({maxN, tune.cap ? tune.cap : 16384u, kSlotLimit});
-    lds = (size_t)wpb * (p.nbk + kFlushWords) * 4;
+    lds = (size_t)wpb * (p.nbk + kWaveLds) * 4;
     const DevInfo &di = dev_info(device);
     const int bpc = std::max(1, blocks_per_cu<kBrandes>(NT, block, lds));
     uint32_t resident = di.cus * (uint32_t)bpc * wpb;
