@@ -840,7 +840,8 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
   }
   AngPlan p;
   p.maxp = maxp;
-  p.scap = std::min<uint32_t>(2 * N, 8192);
+  const uint32_t cap0 = default_tuning().cap;  // search states per lane before a retry
+  p.scap = std::min<uint32_t>(2 * N, cap0 ? std::max<uint32_t>(cap0, 2) : 8192u);
   p.ncap = std::min<uint32_t>(N, p.scap);
   p.hcap = 2 * p.scap;
   const int dev = specs[0].dg->device;

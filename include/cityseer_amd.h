@@ -195,7 +195,8 @@ typedef struct cs_tuning {
   uint32_t block_threads;   /* workgroup size 64..256, a multiple of 64: sources in flight per workgroup = /64 */
   uint32_t waves_per_simd;  /* cap on resident waves per SIMD; 0 = as many as fit */
   float delta_scale;        /* bucket width as a fraction in (0, 1] of its safe maximum */
-  uint32_t slot_capacity;   /* settled-node slots per wave before a retry; 0 = default (16384) */
+  uint32_t slot_capacity;   /* settled-node slots per wave (simplest: search states per lane) before a
+                               retry with four times as many; 0 = default (16384; simplest 8192) */
   uint32_t xcd_split;       /* 1: each XCD's waves take a contiguous share of the sources first; 0: one counter */
 } cs_tuning;
 
