@@ -118,6 +118,11 @@ _SIGNATURES = {
     "cs_graph_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _u32p, _u32p]),
     "cs_graph_progress": (ctypes.c_uint64, [ctypes.c_void_p]),
     "cs_centrality_shortest": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CsParams), _f64p, _u32p]),
+    "cs_betweenness_od_shortest": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(CsParams), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)],
+    ),
     "cs_centrality_shortest_device": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.POINTER(CsParams), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],

@@ -277,3 +277,100 @@ def betweenness_simplest(network_structure, nodes_gdf, distances=None, betas=Non
     """Angular betweenness only (networks.py:860-888)."""
     return node_centrality_simplest(network_structure, nodes_gdf, distances, betas, minutes, False, True,
                                     min_threshold_wt, speed_m_s, tolerance, 90, 1, random_seed, sample, epsilon)
+
+
+def _zone_points(zones) -> tuple[np.ndarray, np.ndarray]:
+    """Zone identifiers (the index) and one (x, y) per zone: the centroid of a polygon geometry,
+    a point geometry's coordinates, or columns ``x`` and ``y``."""
+    ids = np.asarray(zones.index.values)
+    if "geometry" in getattr(zones, "columns", []):
+        pts = []
+        for g in zones["geometry"]:
+            c = g.centroid if getattr(g, "geom_type", "Point") in ("Polygon", "MultiPolygon") else g
+            pts.append((float(c.x), float(c.y)))
+        return ids, np.asarray(pts, dtype=np.float64).reshape(-1, 2)
+    return ids, np.column_stack([np.asarray(zones["x"], np.float64), np.asarray(zones["y"], np.float64)])
+
+
+def build_od_matrix(
+    od_df: pd.DataFrame,
+    zones_gdf,
+    network_structure,
+    origin_col: str,
+    destination_col: str,
+    weight_col: str,
+    zone_id_col: str | None = None,
+    max_snap_dist: float = 500.0,
+) -> rustalgos.centrality.OdMatrix:
+    """An ``OdMatrix`` from zone-to-zone flows (networks.py:278-373): each zone's centroid snaps
+    to its nearest network node (KD-tree) unless farther than ``max_snap_dist``; flows with a
+    missing, zero or negative weight, or an unsnapped zone, are dropped. Zones in projected
+    coordinates matching the network (reprojection from EPSG:4326 is not supported here)."""
+    from scipy.spatial import KDTree
+
+    crs = getattr(zones_gdf, "crs", None)
+    if crs is not None and getattr(crs, "to_epsg", lambda: None)() == 4326:
+        raise ValueError("zones in EPSG:4326 must be projected to the network's CRS first")
+    ids, xy = _zone_points(zones_gdf)
+    if zone_id_col is not None:
+        ids = np.asarray(zones_gdf[zone_id_col].values)
+    tree = KDTree(np.asarray(network_structure.node_xys, dtype=np.float64))
+    snap_d, snap_i = tree.query(xy)
+    zone_to_node: dict = {}
+    n_excluded = 0
+    for i, zid in enumerate(ids):
+        if snap_d[i] > max_snap_dist:
+            n_excluded += 1
+            continue
+        zone_to_node[zid] = int(snap_i[i])
+    if n_excluded:
+        logger.warning(f"{n_excluded} zone centroids exceeded max_snap_dist={max_snap_dist}m and were excluded")
+    origins: list[int] = []
+    dests: list[int] = []
+    weights: list[float] = []
+    for o_zone, d_zone, w in zip(od_df[origin_col], od_df[destination_col], od_df[weight_col]):
+        if pd.isna(w) or w <= 0:
+            continue
+        if o_zone not in zone_to_node or d_zone not in zone_to_node:
+            continue
+        origins.append(zone_to_node[o_zone])
+        dests.append(zone_to_node[d_zone])
+        weights.append(float(w))
+    logger.info(f"Built OD matrix: {len(origins)} pairs, {sum(weights):.0f} total trips")
+    return rustalgos.centrality.OdMatrix(origins, dests, weights)
+
+
+def betweenness_od(
+    network_structure,
+    nodes_gdf: pd.DataFrame,
+    od_matrix,
+    distances: list[int] | None = None,
+    betas: list[float] | None = None,
+    minutes: list[float] | None = None,
+    min_threshold_wt: float = MIN_THRESH_WT,
+    speed_m_s: float = SPEED_M_S,
+    tolerance: float | None = None,
+) -> pd.DataFrame:
+    """OD-weighted betweenness with shortest paths (networks.py:376-461): columns
+    ``cc_betweenness_{d}`` and ``cc_betweenness_beta_{d}``; closeness is not computed."""
+    logger.info("Computing OD-weighted betweenness centrality.")
+    func = partial(
+        network_structure.betweenness_od_shortest,
+        od_matrix=od_matrix,
+        distances=distances,
+        betas=betas,
+        minutes=minutes,
+        min_threshold_wt=min_threshold_wt,
+        speed_m_s=speed_m_s,
+        tolerance=tolerance,
+    )
+    result = config.wrap_progress(total=network_structure.street_node_count(), rust_struct=network_structure,
+                                  partial_func=func)
+    distances = config.log_thresholds(
+        distances=distances, betas=betas, minutes=minutes, min_threshold_wt=min_threshold_wt, speed_m_s=speed_m_s
+    )
+    temp: dict = {}
+    for measure_key, attr_key in [("betweenness", "node_betweenness"), ("betweenness_beta", "node_betweenness_beta")]:
+        for d in distances:
+            temp[config.prep_gdf_key(measure_key, d)] = getattr(result, attr_key)[d]
+    return _assign(nodes_gdf, result.node_keys_py, temp)

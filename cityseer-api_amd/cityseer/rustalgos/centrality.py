@@ -77,3 +77,42 @@ class CentralitySegmentResult(_MetricResultSet):
     segment_harmonic = _getter(1)
     segment_beta = _getter(2)
     segment_betweenness = _getter(3)
+
+
+class BetweennessShortestResult(_MetricResultSet):
+    """OD-weighted betweenness (centrality.rs:98-148): betweenness and betweenness_beta only."""
+
+    _metrics = ("betweenness", "betweenness_beta")
+    node_betweenness = _getter(0)
+    node_betweenness_beta = _getter(1)
+
+
+class OdMatrix:
+    """Sparse origin-destination trip weights (centrality.rs:50-91), built from parallel
+    (origin, destination, weight) arrays in COO form; a later pair overwrites an earlier one with
+    the same origin and destination. Reusable across calls."""
+
+    def __init__(self, origins: list[int], destinations: list[int], weights: list[float]) -> None:
+        if len(origins) != len(destinations) or len(origins) != len(weights):
+            raise ValueError(
+                f"origins ({len(origins)}), destinations ({len(destinations)}), and weights ({len(weights)}) "
+                "must have equal length"
+            )
+        o = np.asarray(origins, dtype=np.int64)
+        d = np.asarray(destinations, dtype=np.int64)
+        if o.size and (o.min() < 0 or d.min() < 0):
+            raise ValueError("OD node indices must be non-negative")
+        self.origins = o.astype(np.uint32)
+        self.destinations = d.astype(np.uint32)
+        self.weights = np.asarray(weights, dtype=np.float32)
+        # distinct (origin, destination) pairs, as the reference's nested map holds them
+        self._pairs = len(set(zip(self.origins.tolist(), self.destinations.tolist())))
+        self._origins = len(set(self.origins.tolist()))
+
+    def len(self) -> int:
+        """Number of non-zero OD pairs."""
+        return self._pairs
+
+    def n_origins(self) -> int:
+        """Number of unique origin nodes."""
+        return self._origins

@@ -730,6 +730,39 @@ class NetworkStructure:
             reach[:t].tolist() if plan["track"] else [], plan["sampled"] if plan["track"] else 0,
         )
 
+    def betweenness_od_shortest(
+        self,
+        od_matrix: _res.OdMatrix,
+        distances: list[int] | None = None,
+        betas: list[float] | None = None,
+        minutes: list[float] | None = None,
+        min_threshold_wt: float | None = None,
+        speed_m_s: float | None = None,
+        tolerance: float | None = None,
+        pbar_disabled: bool | None = None,
+    ) -> _res.BetweennessShortestResult:
+        """OD-weighted betweenness from a Brandes search per origin with outbound trips, each
+        destination seeding its trip weight (centrality.rs:2413-2540)."""
+        from . import pair_distances_betas_time
+
+        speed = WALKING_SPEED if speed_m_s is None else float(speed_m_s)
+        tol = validate_tolerance(tolerance)
+        dists, bets, secs = pair_distances_betas_time(speed, distances, betas, minutes, min_threshold_wt)
+        n, t = len(self._keys), len(dists)
+        out = np.zeros((2, t, n), dtype=np.float64)
+        plan = {"sources": np.zeros(0, np.uint32), "wt": np.zeros(0, np.float32), "eligible": np.zeros(n, np.uint8),
+                "track": False, "scale": 1.0}
+        params, _keep = self._params(dists, bets, secs, speed, tol, False, True, plan)
+        o, d, w = od_matrix.origins, od_matrix.destinations, od_matrix.weights
+        self.progress_init()
+        _native.check(
+            _native.load().cs_betweenness_od_shortest(
+                self._native_handle(), ctypes.byref(params), len(o), _native.ptr(o, ctypes.c_uint32),
+                _native.ptr(d, ctypes.c_uint32), _native.ptr(w, ctypes.c_float), _native.ptr(out, ctypes.c_double),
+            )
+        )
+        return _res.BetweennessShortestResult(dists, self.node_keys_py(), self.node_indices(), out)
+
     def centrality_simplest(
         self,
         distances: list[int] | None = None,

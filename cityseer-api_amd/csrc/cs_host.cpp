@@ -688,6 +688,100 @@ int cs_centrality_shortest_device(cs_graph *h, const cs_params *p, const float *
   return rc;
 }
 
+int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs, const uint32_t *origins,
+                               const uint32_t *destinations, const float *weights, double *out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  t_stats = cs_stats{};
+  int rc = guarded([&] {
+    if (!h || !p || !out) fail(CS_EINVAL, "graph, params and out must not be NULL");
+    if (n_pairs && (!origins || !destinations || !weights))
+      fail(CS_EINVAL, "origins, destinations and weights must not be NULL");
+    Graph &g = h->g;
+    if (p->n_thresholds == 0 || p->n_thresholds > CS_MAX_THRESHOLDS)
+      fail(CS_EINVAL, "between 1 and " + std::to_string(CS_MAX_THRESHOLDS) + " distance thresholds are supported");
+    if (!(p->speed_m_s > 0.0f) || !std::isfinite(p->speed_m_s))
+      fail(CS_EINVAL, "speed_m_s must be finite and positive");
+    if (!(p->tolerance >= 0.0f && p->tolerance <= 1.0f))
+      fail(CS_EINVAL, "Tolerance must be between 0 and 100 (percent), got " + std::to_string(p->tolerance * 100.0f));
+    const uint32_t N = g.n;
+    for (uint32_t i = 0; i < n_pairs; ++i)
+      if (origins[i] >= N || destinations[i] >= N)
+        fail(CS_EINVAL, "node index " + std::to_string(std::max(origins[i], destinations[i])) +
+                            " does not exist in the graph");
+    // OdMatrix::new (centrality.rs:61-82): one row per origin, a later pair
+    // overwriting an earlier one with the same destination
+    std::vector<uint32_t> order(n_pairs);
+    for (uint32_t i = 0; i < n_pairs; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      return origins[a] != origins[b] ? origins[a] < origins[b] : destinations[a] < destinations[b];
+    });
+    std::vector<uint32_t> src, off{0}, dest;
+    std::vector<float> w;
+    for (uint32_t k = 0; k < n_pairs;) {
+      const uint32_t o = origins[order[k]];
+      const bool live = g.live[o] != 0;  // betweenness_od_shortest skips non-live origins (:2484-2486)
+      while (k < n_pairs && origins[order[k]] == o) {
+        uint32_t last = k;
+        while (last + 1 < n_pairs && origins[order[last + 1]] == o &&
+               destinations[order[last + 1]] == destinations[order[k]])
+          ++last;
+        if (live) {
+          dest.push_back(destinations[order[last]]);
+          w.push_back(weights[order[last]]);
+        }
+        k = last + 1;
+      }
+      if (live) {
+        src.push_back(o);
+        off.push_back((uint32_t)dest.size());
+      }
+    }
+    CallSpec spec = make_spec(p);
+    spec.closeness = false;
+    spec.betweenness = true;
+    spec.track_reach = false;
+    const uint32_t T = spec.T, M = 7;
+    const uint32_t ns = (uint32_t)src.size();
+    const int dev = current_device();
+    hipStream_t stream;
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    DevBuf acc(sizeof(double) * (size_t)N * M * T), d_src(4 * std::max<size_t>(ns, 1)),
+        d_wt(4 * std::max<size_t>(ns, 1)), d_elig(N), d_off(4 * off.size()), d_dest(4 * std::max<size_t>(dest.size(), 1)),
+        d_w(4 * std::max<size_t>(w.size(), 1));
+    const std::vector<float> ones(std::max<size_t>(ns, 1), 1.0f);
+    hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
+    hip_check(hipMemsetAsync(d_elig.p, 0, N, stream), "memset");
+    if (ns) {
+      hip_check(hipMemcpyAsync(d_src.p, src.data(), 4 * ns, hipMemcpyHostToDevice, stream), "copy");
+      hip_check(hipMemcpyAsync(d_wt.p, ones.data(), 4 * ns, hipMemcpyHostToDevice, stream), "copy");
+      hip_check(hipMemcpyAsync(d_off.p, off.data(), 4 * off.size(), hipMemcpyHostToDevice, stream), "copy");
+      hip_check(hipMemcpyAsync(d_dest.p, dest.data(), 4 * dest.size(), hipMemcpyHostToDevice, stream), "copy");
+      hip_check(hipMemcpyAsync(d_w.p, w.data(), 4 * w.size(), hipMemcpyHostToDevice, stream), "copy");
+      std::lock_guard<std::mutex> lk(g.mu);
+      DevGraph &dg = device_graph(g, dev);
+      DevEdges &de = device_edges(g, dg, spec.speed);
+      if (g.max_out_deg > 64) fail(CS_EINVAL, "nodes with more than 64 out-edges are not supported by the GPU search");
+      JobSpec js{&g, &dg, &de, spec, d_src.as<uint32_t>(), d_wt.as<float>(), d_wt.as<float>(), ns,
+                 d_elig.as<uint8_t>(), acc.as<double>(), nullptr, default_tuning().delta_scale};
+      js.d_od_off = d_off.as<uint32_t>();
+      js.d_od_dest = d_dest.as<uint32_t>();
+      js.d_od_w = d_w.as<float>();
+      launch_shortest_batch({js}, T, default_tuning(), stream, t_stats);
+    }
+    std::vector<double> hacc((size_t)N * M * T);
+    hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
+              "copy out");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    hip_check(hipStreamDestroy(stream), "stream destroy");
+    // rows 5 and 6 (betweenness, betweenness_beta) of the [M][T][N] accumulators
+    std::memcpy(out, hacc.data() + (size_t)5 * T * N, sizeof(double) * 2 * T * N);
+    t_stats.sources = ns;
+    g.progress = N;
+  });
+  t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
 int cs_segment_centrality(cs_graph *h, const cs_params *p, double *out) {
   const auto t0 = std::chrono::steady_clock::now();
   t_stats = cs_stats{};

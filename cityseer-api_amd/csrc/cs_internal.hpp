@@ -151,6 +151,11 @@ struct JobSpec {
   uint32_t *d_reach;
   float delta_scale;
   uint32_t tstride = 0;
+  // OD-weighted betweenness: source position i seeds od_w at od_dest over
+  // [od_off[i], od_off[i + 1]) instead of every reached node (device arrays)
+  const uint32_t *d_od_off = nullptr;
+  const uint32_t *d_od_dest = nullptr;
+  const float *d_od_w = nullptr;
 };
 // All sources of all jobs in one persistent launch, or only `items`
 // ({job, source position}) when given.

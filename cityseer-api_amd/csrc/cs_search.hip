@@ -78,8 +78,13 @@ struct Job {
   float maxsec, speed, inv_delta;
   float tol;  // tie tolerance; above the internal band, Phase 2 rebuilds predecessors
   uint32_t nbk, maxb;
-  uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
+  uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach, 8 OD-weighted seeds
   uint32_t n_sources;
+  // OD-weighted betweenness: source position i's destinations and trip weights
+  // are od_dest / od_w[od_off[i] .. od_off[i + 1])
+  const uint32_t *od_off;
+  const uint32_t *od_dest;
+  const float *od_w;
   uint32_t tstride;  // threshold stride of acc (>= the job's threshold count)
   uint32_t nacc;     // node count of acc (the job's graph)
   float dist[kMaxT];  // thresholds beyond the job's own count hold -1 (never within)
@@ -100,7 +105,7 @@ struct SearchArgs {
   size_t ncap;
   char *ws;         // [waves][ws_stride] scratch
   size_t ws_stride;
-  size_t o_rec, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
+  size_t o_rec, o_pm, o_acc, o_ods, o_bkt, o_bnd, o_lvl;
   uint32_t *epochs;
   uint32_t *next;            // [kXcd] per-XCD item counters
   uint32_t xbeg[kXcd + 1];   // XCD x owns item indices [xbeg[x], xbeg[x+1])
@@ -201,6 +206,7 @@ struct WaveWs {
   uint64_t *rec;    // [cap][2]
   uint64_t *pm;     // [cap] predecessor row mask (Brandes) / {last len, origin len} (trees)
   double *acc;      // [cap][2T] dependencies (Brandes) / [cap][T] subtree sums (segment)
+  double *ods;      // [cap] OD trip weight seeding each slot (OD-weighted betweenness)
   uint2 *bkt;       // [nbk][bcap] bucket ring entries {node, secs bits}
   uint32_t *bnd;    // [maxb + 2] slot bounds per opened bucket
   uint32_t *lvl;    // [maxb + 2] highest Phase 2 resolution round per bucket
@@ -216,6 +222,7 @@ __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uin
   w.rec = reinterpret_cast<uint64_t *>(base + a.o_rec);
   w.pm = reinterpret_cast<uint64_t *>(base + a.o_pm);
   w.acc = reinterpret_cast<double *>(base + a.o_acc);
+  w.ods = reinterpret_cast<double *>(base + a.o_ods);
   w.bkt = reinterpret_cast<uint2 *>(base + a.o_bkt);
   w.bnd = reinterpret_cast<uint32_t *>(base + a.o_bnd);
   w.lvl = reinterpret_cast<uint32_t *>(base + a.o_lvl);
@@ -521,6 +528,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           if (MODE == kBrandesExport || npred > 1) st64(&w.pm[s], f.pmask);
           if constexpr (MODE == kBrandes) {
             const float route_v = secs_v * speed;
+            if (j.flags & 8u) std_(&w.ods[s], 0.0);  // set below for the source's destinations
             if (j.flags & 2u) {
               // dependencies for thresholds the node lies beyond are never
               // accumulated (successors lie farther still) nor read
@@ -830,7 +838,8 @@ __device__ __forceinline__ void flush_target(CJob &j, const WaveWs &w, uint32_t 
     }
   }
   if (j.flags & 2u) {
-    const double pc = j.eligible[v] ? 0.5 : 1.0;
+    // seed: the all-pairs weight (0.5 between two eligible sources), or the OD trip weight
+    const double pc = (j.flags & 8u) ? ldd(&w.ods[s]) : (j.eligible[v] ? 0.5 : 1.0);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       if (route <= j.dist[t]) {
@@ -962,6 +971,16 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
         st_within += tot;
         if ((j.flags & 5u) == 5u && lane == 0 && tot) atomicAdd(&j.reach[t], tot);
       }
+      // ---- OD-weighted seeds (centrality.rs:2500-2509): the source's destinations
+      // that were reached take their trip weight; every other slot keeps 0
+      const bool od = (j.flags & 8u) != 0;
+      if (od) {
+        for (uint32_t k = j.od_off[si] + lane; k < j.od_off[si + 1]; k += kWave) {
+          const uint64_t kd = ld64(nkey(w, j.od_dest[k]));
+          if (key_current(kd, c.tag)) std_(&w.ods[key_slot(kd)], (double)j.od_w[k]);
+        }
+        wave_fence();
+      }
       // ---- betweenness dependencies: reverse bucket order (centrality.rs:1783-1835,
       // 823-873); each slot's accumulated delta is credited by the flush below
       if (j.flags & 2u) {
@@ -984,7 +1003,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
               const uint32_t pslot0 = (uint32_t)hi;
               const uint32_t meta = (uint32_t)(hi >> 32);
               const uint32_t npred = meta & 0xffu;
-              const double pc = (meta & 0x100u) ? 0.5 : 1.0;
+              const double pc = od ? ldd(&w.ods[s]) : ((meta & 0x100u) ? 0.5 : 1.0);
               double dep[NT], dep_b[NT];
               bool any = false;
 #pragma unroll
@@ -1284,7 +1303,7 @@ int blocks_per_cu(uint32_t T, uint32_t block, size_t lds) {
 struct Plan {
   uint32_t bcap, cap, maxb, nbk, waves;
   size_t stride;
-  size_t o_rec, o_pm, o_acc, o_bkt, o_bnd, o_lvl;
+  size_t o_rec, o_pm, o_acc, o_ods, o_bkt, o_bnd, o_lvl;
 };
 
 // Per-wave scratch layout. Everything here is written before it is read
@@ -1298,6 +1317,8 @@ void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   o = align256(o + 8ull * p.cap);
   p.o_acc = o;
   o = align256(o + 8ull * p.cap * (brandes ? 2 * T : T));
+  p.o_ods = o;
+  o = align256(o + (brandes ? 8ull * p.cap : 0ull));
   p.o_bkt = o;
   o = align256(o + 8ull * p.nbk * p.bcap);
   p.o_bnd = o;
@@ -1492,6 +1513,13 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   if (j.nbk > 4096 - kWaveLds) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
   j.maxb = (uint32_t)((double)cs.maxsec * (double)j.inv_delta) + 4;
   j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
+  j.od_off = js.d_od_off;
+  j.od_dest = js.d_od_dest;
+  j.od_w = js.d_od_w;
+  if (js.d_od_off) {
+    if (cs.closeness || !cs.betweenness) fail(CS_EINTERNAL, "OD-weighted jobs compute betweenness only");
+    j.flags |= 8u;
+  }
   j.n_sources = js.n_sources;
   j.tstride = js.tstride ? js.tstride : NT;
   j.nacc = g.n;
@@ -1611,6 +1639,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.o_rec = p.o_rec;
     a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
+    a.o_ods = p.o_ods;
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.o_lvl = p.o_lvl;
@@ -1808,6 +1837,7 @@ struct ShortestBatch {
     a.o_rec = p.o_rec;
     a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
+    a.o_ods = p.o_ods;
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.o_lvl = p.o_lvl;

@@ -127,6 +127,16 @@ int cs_centrality_shortest(cs_graph *g, const cs_params *p, double *out, uint32_
 int cs_centrality_shortest_device(cs_graph *g, const cs_params *p, const float *d_cycles_wt, double *d_acc,
                                   uint32_t *d_reach, void *stream);
 
+/* Replaces NetworkStructure.betweenness_od_shortest (rust/src/centrality.rs:2413-2540):
+ * OD-weighted betweenness from a sparse origin-destination matrix given as n_pairs
+ * parallel (origin, destination, trip weight) triples, as OdMatrix(origins,
+ * destinations, weights) takes them (a later pair overwrites an earlier one with
+ * the same origin and destination). Only p's thresholds, speed_m_s and tolerance
+ * are read. Non-live origins are skipped; an index beyond the graph is CS_EINVAL.
+ * out: f64 [2][T][N] = betweenness, betweenness_beta. */
+int cs_betweenness_od_shortest(cs_graph *g, const cs_params *p, uint32_t n_pairs, const uint32_t *origins,
+                               const uint32_t *destinations, const float *weights, double *out);
+
 /* Replaces NetworkStructure.centrality_simplest (rust/src/centrality.rs:1876-2132).
  * out: f64 [4][T][N] = density, farness, harmonic, betweenness. */
 int cs_centrality_simplest(cs_graph *g, const cs_params *p, double *out, uint32_t *reach_totals);

@@ -790,8 +790,12 @@ typedef struct {
   double *out;
   _Atomic uint32_t *reach;
   _Atomic uint32_t next;
-  int kind; /* 0 shortest, 1 simplest, 2 segment */
+  int kind; /* 0 shortest, 1 simplest, 2 segment, 3 OD-weighted betweenness */
   _Atomic int err;
+  /* kind 3: the OD row of origin p->sources[i] is od_dest/od_w[od_off[i] .. od_off[i + 1]) */
+  const uint32_t *od_off;
+  const uint32_t *od_dest;
+  const float *od_w;
 } job;
 
 static uint32_t max_u32(const uint32_t *a, uint32_t n) {
@@ -874,6 +878,47 @@ static void shortest_source(job *jb, uint32_t s_i) {
     free(delta_b);
   }
   free(reached);
+  trav_free(&t);
+}
+
+/* betweenness_od_shortest per-origin body: centrality.rs:2475-2535. Brandes
+ * search as for centrality_shortest; per threshold the seeds are the origin's
+ * OD weights at destinations within the threshold (w and w * exp(-beta *
+ * route)); credits are not scaled by a source weight. Out rows: betweenness,
+ * betweenness_beta. */
+static void od_source(job *jb, uint32_t s_i) {
+  const og_graph *g = jb->g;
+  const og_params *p = jb->p;
+  double *out = jb->out;
+  const uint32_t N = g->n, T = p->n_thresholds;
+  uint32_t src = p->sources[s_i];
+  if (!g->live[src]) return;
+  uint32_t maxs = max_u32(p->seconds, T);
+  traversal t;
+  brandes_shortest(g, src, maxs, p->speed_m_s, p->tolerance, &t);
+  uint32_t *sorted = xmalloc(sizeof(uint32_t) * (t.n_order ? t.n_order : 1));
+  uint32_t ns = sorted_states(&t, sorted);
+  double *seed = xmalloc(sizeof(double) * N), *seed_b = xmalloc(sizeof(double) * N);
+  double *delta = xmalloc(sizeof(double) * N), *delta_b = xmalloc(sizeof(double) * N);
+  for (uint32_t i = 0; i < T; i++) {
+    float thr = (float)p->distances[i];
+    double beta = (double)p->betas[i];
+    memset(seed, 0, sizeof(double) * N);
+    memset(seed_b, 0, sizeof(double) * N);
+    for (uint32_t k = jb->od_off[s_i]; k < jb->od_off[s_i + 1]; k++) {
+      uint32_t dest = jb->od_dest[k];
+      if (t.best_route[dest] > thr) continue;
+      double w = (double)jb->od_w[k];
+      seed[dest] += w;
+      seed_b[dest] += w * exp(-beta * (double)t.best_route[dest]);
+    }
+    backprop(&t, sorted, ns, src, seed, seed_b, 0, thr, 1, delta, delta_b, &OUT(0, i, 0), &OUT(1, i, 0), 1.0);
+  }
+  free(sorted);
+  free(seed);
+  free(seed_b);
+  free(delta);
+  free(delta_b);
   trav_free(&t);
 }
 
@@ -1271,14 +1316,24 @@ static void *worker(void *arg) {
       shortest_source(jb, i);
     else if (jb->kind == 1)
       simplest_source(jb, i);
-    else
+    else if (jb->kind == 2)
       segment_source(jb, i);
+    else
+      od_source(jb, i);
   }
   return NULL;
 }
 
+static int run_job_od(const og_graph *g, const og_params *p, double *out, uint32_t *reach, int kind,
+                      uint32_t M, const uint32_t *od_off, const uint32_t *od_dest, const float *od_w);
+
 static int run_job(const og_graph *g, const og_params *p, double *out, uint32_t *reach, int kind,
                    uint32_t M) {
+  return run_job_od(g, p, out, reach, kind, M, NULL, NULL, NULL);
+}
+
+static int run_job_od(const og_graph *g, const og_params *p, double *out, uint32_t *reach, int kind,
+                      uint32_t M, const uint32_t *od_off, const uint32_t *od_dest, const float *od_w) {
   const uint32_t N = g->n, T = p->n_thresholds;
   if (T == 0 || T > 16) return OG_EINVAL;
   memset(out, 0, sizeof(double) * (size_t)M * T * N);
@@ -1292,6 +1347,9 @@ static int run_job(const og_graph *g, const og_params *p, double *out, uint32_t 
   atomic_init(&jb.next, 0);
   atomic_init(&jb.err, OG_OK);
   jb.kind = kind;
+  jb.od_off = od_off;
+  jb.od_dest = od_dest;
+  jb.od_w = od_w;
   int nt = p->n_threads > 0 ? p->n_threads : 1;
   if (nt == 1) {
     worker(&jb);
@@ -1318,6 +1376,14 @@ int og_centrality_shortest(const og_graph *g, const og_params *p, double *out, u
   int rc = run_job(g, p, out, reach, 0, 7);
   if (rc == OG_OK && p->compute_betweenness) scale_rows(out, g->n, p->n_thresholds, 5, 6, p->betw_scale);
   return rc;
+}
+
+int og_betweenness_od_shortest(const og_graph *g, const og_params *p, const uint32_t *od_off,
+                               const uint32_t *od_dest, const float *od_w, double *out) {
+  for (uint32_t i = 0; i < p->n_sources; i++)
+    for (uint32_t k = od_off[i]; k < od_off[i + 1]; k++)
+      if (od_dest[k] >= g->n || p->sources[i] >= g->n) return OG_EINVAL;
+  return run_job_od(g, p, out, NULL, 3, 2, od_off, od_dest, od_w);
 }
 
 int og_centrality_simplest(const og_graph *g, const og_params *p, double *out, uint32_t *reach) {

@@ -84,6 +84,13 @@ typedef struct {
  * M = 4 for segment (density, harmonic, beta, betweenness). */
 int og_centrality_shortest(const og_graph *g, const og_params *p, double *out,
                            uint32_t *reach_totals);
+/* betweenness_od_shortest (rust/src/centrality.rs:2413-2540): p->sources are
+ * the unique origins, p->n_sources of them; origin i's OD row (destination,
+ * trip weight) is od_dest/od_w[od_off[i] .. od_off[i + 1]), destinations
+ * unique per origin. out: f64 [2][T][N] = betweenness, betweenness_beta. Only
+ * thresholds, speed, tolerance and n_threads of p are read besides sources. */
+int og_betweenness_od_shortest(const og_graph *g, const og_params *p, const uint32_t *od_off,
+                               const uint32_t *od_dest, const float *od_w, double *out);
 int og_centrality_simplest(const og_graph *g, const og_params *p, double *out,
                            uint32_t *reach_totals);
 int og_segment_centrality(const og_graph *g, const og_params *p, double *out);

@@ -47,6 +47,23 @@ class OgParams(ctypes.Structure):
 _lib = None
 
 
+def od_rows(origins, destinations, weights):
+    """OdMatrix::new (rust/src/centrality.rs:61-82): a map origin -> {destination: weight}, a later
+    pair overwriting an earlier one; returned as unique origins and a CSR of their rows."""
+    rows: dict[int, dict[int, float]] = {}
+    for o, d_, w in zip(origins, destinations, weights):
+        rows.setdefault(int(o), {})[int(d_)] = float(w)
+    orig = np.array(sorted(rows), dtype=np.uint32)
+    off = np.zeros(len(orig) + 1, dtype=np.uint32)
+    dest, wts = [], []
+    for i, o in enumerate(orig):
+        for d_, w in rows[int(o)].items():
+            dest.append(d_)
+            wts.append(w)
+        off[i + 1] = len(dest)
+    return orig, off, np.asarray(dest, dtype=np.uint32), np.asarray(wts, dtype=np.float32)
+
+
 def build() -> None:
     """Compile liboracle.so with the committed Makefile."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -75,6 +92,9 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = ctypes.c_int
         fn.argtypes = [ctypes.c_void_p, ctypes.POINTER(OgParams), _f64p, _u32p]
+    lib.og_betweenness_od_shortest.restype = ctypes.c_int
+    lib.og_betweenness_od_shortest.argtypes = [ctypes.c_void_p, ctypes.POINTER(OgParams), _u32p, _u32p,
+                                               ctypes.POINTER(ctypes.c_float), _f64p]
     lib.og_segment_centrality.restype = ctypes.c_int
     lib.og_segment_centrality.argtypes = [ctypes.c_void_p, ctypes.POINTER(OgParams), _f64p]
     lib.og_brandes_shortest.restype = ctypes.c_int
@@ -215,6 +235,21 @@ class OracleGraph:
         if rc != 0:
             raise ValueError(f"oracle error {rc}")
         return out, reach[: len(d)]
+
+    def betweenness_od_shortest(self, od, distances, speed=1.33333, tolerance=1e-4, n_threads=1,
+                                min_threshold_wt=0.01831563888873418):
+        """rust/src/centrality.rs:2413-2540; od = (origins, destinations, weights) as given to OdMatrix."""
+        d, b, s = thresholds(speed, distances=distances, min_threshold_wt=min_threshold_wt)
+        origins, off, dest, w = od_rows(*od)
+        p, _keep = self._params(d, b, s, speed, tolerance, False, True, origins, None, None, False, 1.0, 180.0, 1.0,
+                                n_threads)
+        out = np.zeros((2, len(d), self.n), dtype=np.float64)
+        rc = load().og_betweenness_od_shortest(self.h, ctypes.byref(p), _p(off, ctypes.c_uint32),
+                                               _p(dest, ctypes.c_uint32), _p(w, ctypes.c_float),
+                                               _p(out, ctypes.c_double))
+        if rc != 0:
+            raise ValueError(f"oracle error {rc}")
+        return out
 
     def segment_centrality(self, distances, speed=1.33333, closeness=True, betweenness=True, sources=None,
                            n_threads=1):
