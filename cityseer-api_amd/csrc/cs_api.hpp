@@ -36,6 +36,19 @@ struct SourceSet {
 };
 SourceSet source_set(const Graph &g, const cs_params *p);
 
+// A non-blocking stream owned by one call: destroyed on every exit path,
+// after the call's device buffers (declared later) have been freed.
+struct OwnedStream {
+  hipStream_t s = nullptr;
+  OwnedStream() { hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream"); }
+  ~OwnedStream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+  OwnedStream(const OwnedStream &) = delete;
+  OwnedStream &operator=(const OwnedStream &) = delete;
+  operator hipStream_t() const { return s; }
+};
+
 // acc [M][stride][N] (device rows) -> out [M][T][N] (host)
 void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out);
 void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s);

@@ -620,8 +620,7 @@ int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_
     const CallSpec spec = make_spec(p);
     const SourceSet ss = source_set(g, p);
     const uint32_t N = g.n, T = spec.T, M = 7;
-    hipStream_t stream;
-    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    OwnedStream stream;
     DevBuf acc(sizeof(double) * (size_t)N * M * T), reach(4 * kMaxT), d_src(4 * ss.src.size()),
         d_wt(4 * ss.src.size()), d_cyc(4 * ss.src.size()), d_elig(N);
     hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
@@ -647,7 +646,6 @@ int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_
               "copy out");
     hip_check(hipMemcpyAsync(hreach, reach.p, 4 * kMaxT, hipMemcpyDeviceToHost, stream), "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
-    hip_check(hipStreamDestroy(stream), "stream destroy");
     unpack_acc(hacc, N, M, T, out);
     if (p->compute_betweenness) scale_rows(out, N, T, 5, 6, p->betw_scale);
     if (reach_totals)
@@ -743,8 +741,7 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
     const uint32_t T = spec.T, M = 7;
     const uint32_t ns = (uint32_t)src.size();
     const int dev = current_device();
-    hipStream_t stream;
-    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    OwnedStream stream;
     DevBuf acc(sizeof(double) * (size_t)N * M * T), d_src(4 * std::max<size_t>(ns, 1)),
         d_wt(4 * std::max<size_t>(ns, 1)), d_elig(N), d_off(4 * off.size()), d_dest(4 * std::max<size_t>(dest.size(), 1)),
         d_w(4 * std::max<size_t>(w.size(), 1));
@@ -772,7 +769,6 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
     hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
               "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
-    hip_check(hipStreamDestroy(stream), "stream destroy");
     // rows 5 and 6 (betweenness, betweenness_beta) of the [M][T][N] accumulators
     std::memcpy(out, hacc.data() + (size_t)5 * T * N, sizeof(double) * 2 * T * N);
     t_stats.sources = ns;
@@ -803,8 +799,7 @@ int cs_segment_centrality(cs_graph *h, const cs_params *p, double *out) {
     std::lock_guard<std::mutex> lk(g.mu);
     DevGraph &dg = device_graph(g, dev);
     DevEdges &de = device_edges(g, dg, spec.speed);
-    hipStream_t stream;
-    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    OwnedStream stream;
     DevBuf acc(sizeof(double) * (size_t)N * M * T), d_src(4 * srcs.size());
     hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
     if (!srcs.empty())
@@ -815,7 +810,6 @@ int cs_segment_centrality(cs_graph *h, const cs_params *p, double *out) {
     hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
               "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
-    hip_check(hipStreamDestroy(stream), "stream destroy");
     unpack_acc(hacc, N, M, T, out);
     t_stats.sources = srcs.size();
     g.progress = p->n_sources;
@@ -839,8 +833,7 @@ int cs_centrality_simplest(cs_graph *h, const cs_params *p, double *out, uint32_
     std::lock_guard<std::mutex> lk(g.mu);
     DevGraph &dg = device_graph(g, dev);
     DevEdges &de = device_edges(g, dg, spec.speed);
-    hipStream_t stream;
-    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    OwnedStream stream;
     DevBuf acc(sizeof(double) * (size_t)N * M * T), reach(4 * kMaxT), d_src(4 * ss.src.size()),
         d_wt(4 * ss.src.size()), d_elig(N);
     hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
@@ -858,7 +851,6 @@ int cs_centrality_simplest(cs_graph *h, const cs_params *p, double *out, uint32_
               "copy out");
     hip_check(hipMemcpyAsync(hreach, reach.p, 4 * kMaxT, hipMemcpyDeviceToHost, stream), "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
-    hip_check(hipStreamDestroy(stream), "stream destroy");
     unpack_acc(hacc, N, M, T, out);
     if (p->compute_betweenness) scale_rows(out, N, T, 3, 3, p->betw_scale);
     if (reach_totals)
@@ -894,8 +886,7 @@ Single run_single(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, 
   std::lock_guard<std::mutex> lk(g.mu);
   DevGraph &dg = device_graph(g, dev);
   DevEdges &de = device_edges(g, dg, speed);
-  hipStream_t stream;
-  hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+  OwnedStream stream;
   Single s;
   s.secs.assign(N, INFINITY);
   s.pred.assign(N, -1);
@@ -930,7 +921,6 @@ Single run_single(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, 
                 "copy");
   }
   hip_check(hipStreamSynchronize(stream), "sync");
-  hip_check(hipStreamDestroy(stream), "stream destroy");
   return s;
 }
 
@@ -1057,8 +1047,7 @@ int cs_tree_simplest(cs_graph *h, uint32_t src, uint32_t max_seconds, float spee
     std::lock_guard<std::mutex> lk(g.mu);
     DevGraph &dg = device_graph(g, dev);
     DevEdges &de = device_edges(g, dg, speed);
-    hipStream_t stream;
-    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    OwnedStream stream;
     std::vector<float> hs(N, INFINITY), hsec(N, INFINITY);
     std::vector<int32_t> hp(N, -1);
     std::vector<uint8_t> hv(N, 0);
@@ -1078,7 +1067,6 @@ int cs_tree_simplest(cs_graph *h, uint32_t src, uint32_t max_seconds, float spee
     hip_check(hipMemcpyAsync(ho.data(), d_o.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
     hip_check(hipMemcpyAsync(&hn, d_n.p, 4, hipMemcpyDeviceToHost, stream), "copy");
     hip_check(hipStreamSynchronize(stream), "sync");
-    hip_check(hipStreamDestroy(stream), "stream destroy");
     for (uint32_t v = 0; v < N; ++v) {
       visited[v] = hv[v];
       discovered[v] = hv[v];
