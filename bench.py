@@ -526,8 +526,8 @@ def main() -> int:
                     help="run only a plain 1 GiB device-to-device copy (3 times), to scale HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); the multi-GPU sharding is a "
-              "design in DESIGN.md, not a measured path.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); sharding sources over several "
+              "GPUs is not a measured path.", file=sys.stderr)
         return 2
 
     import numpy as np
