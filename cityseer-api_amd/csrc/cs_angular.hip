@@ -58,7 +58,7 @@ struct AngArgs {
   size_t lane_stride;  // bytes of the lane-major maps per lane
   size_t wave_stride;  // bytes of the interleaved arrays per wave
   uint32_t scap, ncap, hcap, maxp, tcap;  // per-lane capacities (tcap: thresholds of the delta/seed blocks)
-  size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_delta, o_seed, o_node, o_brt, o_bsec,
+  size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_orte, o_delta, o_seed, o_node, o_brt, o_bsec,
       o_heap;
   char *wave_ws;
   uint32_t *epochs;  // per lane
@@ -196,6 +196,7 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
       fail |= kFailCap;
       return false;
     }
+    *il<float>(w, a.o_orte, n_order, lane, a.scap) = u_route;  // final: a settled state is never relaxed
     *il<uint32_t>(w, a.o_order, n_order++, lane, a.scap) = uss;
     // out-edges in chunks of kRelax: the adjacency entries, then the state and
     // node keys, then the state and node fields are loaded for the whole chunk
@@ -529,9 +530,10 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
       // states by route descending, equal routes in settle order (stable sort)
       int hi = (int)n_order - 1;
       while (hi >= 0) {
-        const float r_hi = *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, hi, lane, a.scap), lane, a.scap);
+        // routes by settle position: the group scan reads neighbouring words
+        const float r_hi = *il<float>(w, a.o_orte, hi, lane, a.scap);
         int lo = hi;
-        while (lo > 0 && *il<float>(w, a.o_route, *il<uint32_t>(w, a.o_order, lo - 1, lane, a.scap), lane, a.scap) == r_hi) --lo;
+        while (lo > 0 && *il<float>(w, a.o_orte, lo - 1, lane, a.scap) == r_hi) --lo;
         for (int q = lo; q <= hi; ++q) {
           const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane, a.scap);
           const float secs_ss = *il<float>(w, a.o_secs, ss, lane, a.scap);
@@ -676,7 +678,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct AngPlan {
   uint32_t scap, ncap, hcap, maxp, lanes, tcap;
   size_t lane_stride, wave_stride;
-  size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_delta, o_seed, o_node, o_brt, o_bsec, o_heap;
+  size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_orte, o_delta, o_seed, o_node, o_brt, o_bsec, o_heap;
 };
 
 void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
@@ -694,6 +696,7 @@ void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
   take(p.o_np, 4, p.scap);
   take(p.o_preds, 4, (size_t)p.scap * p.maxp);
   take(p.o_order, 4, p.scap);
+  take(p.o_orte, 4, p.scap);
   take(p.o_delta, 8, (size_t)p.scap * std::max<uint32_t>(T, 1));
   take(p.o_seed, 8, (size_t)p.scap * std::max<uint32_t>(T, 1));
   take(p.o_node, 4, p.ncap);
@@ -904,6 +907,7 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     a.o_np = p.o_np;
     a.o_preds = p.o_preds;
     a.o_order = p.o_order;
+    a.o_orte = p.o_orte;
     a.o_delta = p.o_delta;
     a.o_seed = p.o_seed;
     a.o_node = p.o_node;
@@ -1019,6 +1023,7 @@ void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, fl
   a.o_np = p.o_np;
   a.o_preds = p.o_preds;
   a.o_order = p.o_order;
+  a.o_orte = p.o_orte;
   a.o_delta = p.o_delta;
   a.o_seed = p.o_seed;
   a.o_node = p.o_node;
