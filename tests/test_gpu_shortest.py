@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from cityseer import config
-from cityseer.tools import graphs, io, mock
+from cityseer.tools import io, mock
 import oracle.oracle as O
 from oracle.oracle import OracleGraph
 
