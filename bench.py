@@ -229,6 +229,23 @@ def batch_workload(n_graphs: int, seed: int = 0):
     return out
 
 
+def all_sources(ns, dev):
+    """Every node as a source, listed in the graph's device node order
+    (cs_graph_node_order): the same work as 0..N-1, with searches from
+    neighbouring nodes running side by side. Unit weights and eligibility need
+    no reordering."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from cityseer import _native
+
+    order = np.empty(ns.node_count(), np.uint32)
+    _native.check(_native.load().cs_graph_node_order(ns._native_handle(), _native.ptr(order, ctypes.c_uint32)))
+    return torch.from_numpy(order.view(np.int32).copy()).to(dev)
+
+
 def run_batch(args) -> int:
     """Many graphs of uneven size, each with its own thresholds: one autotuned persistent launch
     (hipGraph replay) against one plain device-resident launch per graph."""
@@ -253,7 +270,7 @@ def run_batch(args) -> int:
         d, b, s = rustalgos.pair_distances_betas_time(1.33333, distances=dists)
         t = len(d)
         bufs = dict(
-            src=torch.arange(n, dtype=torch.int32, device=dev), wt=torch.ones(n, dtype=torch.float32, device=dev),
+            src=all_sources(ns, dev), wt=torch.ones(n, dtype=torch.float32, device=dev),
             el=torch.ones(n, dtype=torch.uint8, device=dev), acc=torch.zeros(n * 7 * t, dtype=torch.float64, device=dev),
             reach=torch.zeros(8, dtype=torch.int32, device=dev),
             d=np.asarray(d, np.uint32), b=np.asarray(b, np.float32), s=np.asarray(s, np.uint32))
@@ -558,7 +575,7 @@ def main() -> int:
     t_count = len(dists)
     dev = torch.device("cuda:0")
     torch.cuda.init()
-    d_src = torch.arange(n, dtype=torch.int32, device=dev)
+    d_src = all_sources(ns, dev)
     d_wt = torch.ones(n, dtype=torch.float32, device=dev)
     d_cyc = torch.ones(n, dtype=torch.float32, device=dev)
     d_elig = torch.ones(n, dtype=torch.uint8, device=dev)

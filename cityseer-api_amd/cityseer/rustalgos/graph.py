@@ -176,6 +176,9 @@ class NetworkStructure:
         self._pk_ids: dict[str, int] = {}
         self._pk_names: list[str] = []
         self._handle: ctypes.c_void_p | None = None
+        # device memory lays nodes out along a Hilbert curve of their
+        # coordinates (results are unaffected); False keeps insertion order
+        self._spatial_order = True
 
     @classmethod
     def new(cls) -> NetworkStructure:
@@ -454,6 +457,8 @@ class NetworkStructure:
     def arrays(self) -> dict[str, np.ndarray]:
         """Contiguous node and edge columns (insertion order)."""
         return {
+            "x": np.ascontiguousarray(self._x.array()),
+            "y": np.ascontiguousarray(self._y.array()),
             "z": np.ascontiguousarray(self._z.array()),
             "live": np.ascontiguousarray(self._live.array().astype(np.uint8)),
             "weight": np.ascontiguousarray(self._weight.array()),
@@ -487,6 +492,11 @@ class NetworkStructure:
                     p(a["pk"], ctypes.c_int32), int(self._is_dual), ctypes.byref(h),
                 )
             )
+            if self._spatial_order and len(self._keys):
+                rc = lib.cs_graph_set_xy(h, p(a["x"], ctypes.c_double), p(a["y"], ctypes.c_double))
+                if rc != 0:
+                    lib.cs_graph_destroy(h)
+                    _native.check(rc)
             self._handle = h
         return self._handle
 

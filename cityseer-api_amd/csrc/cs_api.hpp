@@ -36,6 +36,17 @@ struct SourceSet {
 };
 SourceSet source_set(const Graph &g, const cs_params *p);
 
+// Device node order (Graph::perm). The shortest and segment searches run on
+// device node ids: sources are translated and sorted ascending (waves that run
+// side by side then search overlapping neighbourhoods), per-node inputs are
+// gathered into device order and results scattered back on unpacking. The
+// angular search keeps the original order (dual graphs are never renumbered).
+void to_device_sources(const Graph &g, SourceSet &ss);
+void to_device_ids(const Graph &g, std::vector<uint32_t> &ids);
+std::vector<uint8_t> to_device_u8(const Graph &g, const uint8_t *in);
+// iperm to pass to unpack_acc_strided: null when the order is the identity
+const uint32_t *result_order(const Graph &g);
+
 // A non-blocking stream owned by one call: destroyed on every exit path,
 // after the call's device buffers (declared later) have been freed.
 struct OwnedStream {
@@ -49,8 +60,10 @@ struct OwnedStream {
   operator hipStream_t() const { return s; }
 };
 
-// acc [M][stride][N] (device rows) -> out [M][T][N] (host)
-void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out);
+// acc [M][stride][N] (device rows, node d at column d) -> out [M][T][N]
+// (host, node iperm[d] at column iperm[d]; iperm null = identity)
+void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out,
+                        const uint32_t *iperm);
 void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s);
 
 }  // namespace cs

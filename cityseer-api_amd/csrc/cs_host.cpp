@@ -173,6 +173,54 @@ void free_edges(DevEdges &d) {
 
 }  // namespace
 
+// Hilbert curve index of (x, y) on a 2^16 x 2^16 grid (the classic xy-to-d
+// walk: quadrant by quadrant, rotating the frame as the curve does).
+static uint64_t hilbert_index(uint32_t x, uint32_t y) {
+  const uint32_t n = 1u << 16;
+  uint64_t d = 0;
+  for (uint32_t s = n / 2; s > 0; s >>= 1) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += (uint64_t)s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = n - 1 - x;
+        y = n - 1 - y;
+      }
+      std::swap(x, y);
+    }
+  }
+  return d;
+}
+
+void set_node_order_from_xy(Graph &g, const double *x, const double *y) {
+  if (!g.dev.empty() || g.order_frozen)
+    fail(CS_EINVAL, "node coordinates must be set before the graph's first computation");
+  const uint32_t n = g.n;
+  if (g.is_dual || n < 2) return;  // dual graphs keep their order (angular search)
+  double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!std::isfinite(x[i]) || !std::isfinite(y[i])) return;  // no usable layout: keep the given order
+    x0 = std::min(x0, x[i]);
+    x1 = std::max(x1, x[i]);
+    y0 = std::min(y0, y[i]);
+    y1 = std::max(y1, y[i]);
+  }
+  const double span = std::max({x1 - x0, y1 - y0, 1e-9});
+  std::vector<std::pair<uint64_t, uint32_t>> key(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t qx = (uint32_t)std::min(65535.0, (x[i] - x0) / span * 65535.0);
+    const uint32_t qy = (uint32_t)std::min(65535.0, (y[i] - y0) / span * 65535.0);
+    key[i] = {hilbert_index(qx, qy), i};
+  }
+  std::sort(key.begin(), key.end());
+  g.identity_order = true;
+  for (uint32_t d = 0; d < n; ++d) {
+    g.iperm[d] = key[d].second;
+    g.perm[key[d].second] = d;
+    if (key[d].second != d) g.identity_order = false;
+  }
+}
+
 DevGraph &device_graph(Graph &g, int device) {
   auto it = g.dev.find(device);
   if (it != g.dev.end()) return *it->second;
@@ -180,10 +228,16 @@ DevGraph &device_graph(Graph &g, int device) {
   dg->device = device;
   dg->rev_off = dev_upload(g.rev_off);
   dg->fwd_off = dev_upload(g.fwd_off);
+  // device node d is original node iperm[d]; its rows stay where the
+  // original CSR puts them, with neighbour ids translated (device_edges)
   std::vector<uint4> rows(g.n);
-  for (uint32_t v = 0; v < g.n; ++v)
-    rows[v] = make_uint4(g.fwd_off[v], g.fwd_off[v + 1], g.rev_off[v], g.rev_off[v + 1]);
+  for (uint32_t d = 0; d < g.n; ++d) {
+    const uint32_t v = g.iperm[d];
+    rows[d] = make_uint4(g.fwd_off[v], g.fwd_off[v + 1], g.rev_off[v], g.rev_off[v + 1]);
+  }
   dg->rows = dev_upload(rows);
+  dg->orig = dev_upload(g.iperm);
+  dg->perm = dev_upload(g.perm);
   DevGraph &ref = *dg;
   g.dev[device] = std::move(dg);
   return ref;
@@ -212,7 +266,7 @@ DevEdges &device_edges(Graph &g, DevGraph &dg, float speed) {
       // (at the later-settled endpoint) when it or its reverse twin is the
       // key's canonical edge
       const bool count_in = g.canon[e] || (tw >= 0 && g.canon[tw]);
-      rev[j] = make_uint2(nb | (count_in ? kCanonBit : 0u), bits);
+      rev[j] = make_uint2(g.perm[nb] | (count_in ? kCanonBit : 0u), bits);
       rtw[j] = make_float2(tw >= 0 ? g.elen[tw] : NAN, tw >= 0 ? g.eimp[tw] : NAN);
       reid[j] = e;
       if (nb != v) {
@@ -229,7 +283,7 @@ DevEdges &device_edges(Graph &g, DevGraph &dg, float speed) {
       const int32_t tw = g.twin[e];
       // out-edges count only one-way canonical edges (no twin among the in-edges)
       const bool count_out = g.canon[e] && tw < 0;
-      fwd[j] = make_uint2(u | (count_out ? kCanonBit : 0u), bits);
+      fwd[j] = make_uint2(g.perm[u] | (count_out ? kCanonBit : 0u), bits);
       ftw[j] = tw >= 0 ? g.elen[tw] : NAN;
       feid[j] = e;
     }
@@ -398,15 +452,59 @@ SourceSet source_set(const Graph &g, const cs_params *p) {
   return s;
 }
 
-// acc [M][stride][N] (device rows) -> out [M][T][N] (host): drops the padded thresholds
-void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out) {
-  for (uint32_t m = 0; m < M; ++m)
-    for (uint32_t i = 0; i < T; ++i)
-      std::memcpy(out + ((size_t)m * T + i) * N, acc + ((size_t)m * stride + i) * N, sizeof(double) * N);
+void to_device_sources(const Graph &g, SourceSet &ss) {
+  g.order_frozen = true;
+  if (g.identity_order) return;
+  const size_t n = ss.src.size();
+  std::vector<uint32_t> ord(n);
+  for (size_t i = 0; i < n; ++i) ord[i] = (uint32_t)i;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](uint32_t a, uint32_t b) { return g.perm[ss.src[a]] < g.perm[ss.src[b]]; });
+  SourceSet out;
+  out.src.resize(n);
+  out.wt.resize(n);
+  out.cyc.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    out.src[i] = g.perm[ss.src[ord[i]]];
+    out.wt[i] = ss.wt[ord[i]];
+    out.cyc[i] = ss.cyc[ord[i]];
+  }
+  ss = std::move(out);
 }
 
-void unpack_acc(const std::vector<double> &acc, uint32_t N, uint32_t M, uint32_t T, double *out) {
-  unpack_acc_strided(acc.data(), N, M, T, T, out);
+void to_device_ids(const Graph &g, std::vector<uint32_t> &ids) {
+  g.order_frozen = true;
+  if (g.identity_order) return;
+  for (auto &v : ids) v = g.perm[v];
+}
+
+std::vector<uint8_t> to_device_u8(const Graph &g, const uint8_t *in) {
+  std::vector<uint8_t> out(g.n);
+  for (uint32_t d = 0; d < g.n; ++d) out[d] = in[g.iperm[d]];
+  return out;
+}
+
+const uint32_t *result_order(const Graph &g) { return g.identity_order ? nullptr : g.iperm.data(); }
+
+// acc [M][stride][N] (device rows) -> out [M][T][N] (host): drops the padded
+// thresholds and puts device node d at original column iperm[d]
+void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out,
+                        const uint32_t *iperm) {
+  for (uint32_t m = 0; m < M; ++m)
+    for (uint32_t i = 0; i < T; ++i) {
+      const double *src = acc + ((size_t)m * stride + i) * N;
+      double *dst = out + ((size_t)m * T + i) * N;
+      if (!iperm) {
+        std::memcpy(dst, src, sizeof(double) * N);
+      } else {
+        for (uint32_t d = 0; d < N; ++d) dst[iperm[d]] = src[d];
+      }
+    }
+}
+
+void unpack_acc(const std::vector<double> &acc, uint32_t N, uint32_t M, uint32_t T, double *out,
+                const uint32_t *iperm) {
+  unpack_acc_strided(acc.data(), N, M, T, T, out, iperm);
 }
 
 void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s) {
@@ -571,6 +669,9 @@ int cs_graph_create(uint32_t n_nodes, const double *z, const uint8_t *live, cons
                             ": impedance must be finite and positive (> 0.0).");
     }
     build_topology(g);
+    g.perm.resize(n_nodes);
+    g.iperm.resize(n_nodes);
+    for (uint32_t i = 0; i < n_nodes; ++i) g.perm[i] = g.iperm[i] = i;
     *out = h.release();
   });
 }
@@ -583,12 +684,30 @@ void cs_graph_destroy(cs_graph *h) {
       (void)hipFree(kv.second->rev_off);
       (void)hipFree(kv.second->fwd_off);
       (void)hipFree(kv.second->rows);
+      (void)hipFree(kv.second->orig);
+      (void)hipFree(kv.second->perm);
+      (void)hipFree(kv.second->xbuf);
       for (auto &e : kv.second->edges) free_edges(*e);
       (void)hipSetDevice(prev);
     }
   }
   (void)hipGetLastError();
   delete h;
+}
+
+int cs_graph_set_xy(cs_graph *h, const double *x, const double *y) {
+  return guarded([&] {
+    if (!h || !x || !y) fail(CS_EINVAL, "graph, x and y must not be NULL");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    set_node_order_from_xy(h->g, x, y);
+  });
+}
+
+int cs_graph_node_order(const cs_graph *h, uint32_t *order) {
+  return guarded([&] {
+    if (!h || !order) fail(CS_EINVAL, "graph and order must not be NULL");
+    std::copy(h->g.iperm.begin(), h->g.iperm.end(), order);
+  });
 }
 
 int cs_graph_info(const cs_graph *h, uint32_t *n, uint32_t *m, uint32_t *mo, uint32_t *mi) {
@@ -618,7 +737,9 @@ int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_
     check_params(g, p);
     const int dev = current_device();
     const CallSpec spec = make_spec(p);
-    const SourceSet ss = source_set(g, p);
+    SourceSet ss = source_set(g, p);
+    to_device_sources(g, ss);
+    const std::vector<uint8_t> elig = to_device_u8(g, p->eligible);
     const uint32_t N = g.n, T = spec.T, M = 7;
     OwnedStream stream;
     DevBuf acc(sizeof(double) * (size_t)N * M * T), reach(4 * kMaxT), d_src(4 * ss.src.size()),
@@ -630,7 +751,7 @@ int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_
       hip_check(hipMemcpyAsync(d_wt.p, ss.wt.data(), 4 * ss.wt.size(), hipMemcpyHostToDevice, stream), "copy");
       hip_check(hipMemcpyAsync(d_cyc.p, ss.cyc.data(), 4 * ss.cyc.size(), hipMemcpyHostToDevice, stream), "copy");
     }
-    hip_check(hipMemcpyAsync(d_elig.p, p->eligible, N, hipMemcpyHostToDevice, stream), "copy");
+    hip_check(hipMemcpyAsync(d_elig.p, elig.data(), N, hipMemcpyHostToDevice, stream), "copy");
     {
       std::lock_guard<std::mutex> lk(g.mu);
       DevGraph &dg = device_graph(g, dev);
@@ -646,7 +767,7 @@ int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_
               "copy out");
     hip_check(hipMemcpyAsync(hreach, reach.p, 4 * kMaxT, hipMemcpyDeviceToHost, stream), "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
-    unpack_acc(hacc, N, M, T, out);
+    unpack_acc(hacc, N, M, T, out, result_order(g));
     if (p->compute_betweenness) scale_rows(out, N, T, 5, 6, p->betw_scale);
     if (reach_totals)
       for (uint32_t i = 0; i < T; ++i) reach_totals[i] = hreach[i];
@@ -677,8 +798,8 @@ int cs_centrality_shortest_device(cs_graph *h, const cs_params *p, const float *
     DevGraph &dg = device_graph(g, dev);
     DevEdges &de = device_edges(g, dg, spec.speed);
     run_shortest_checks(g, p);
-    launch_shortest(g, dg, de, spec, p->sources, p->source_wt, d_cycles_wt, p->n_sources, p->eligible, d_acc,
-                    d_reach, stream, t_stats);
+    launch_shortest_original(g, dg, de, spec, p->sources, p->source_wt, d_cycles_wt, p->n_sources, p->eligible,
+                             d_acc, d_reach, stream, t_stats);
     t_stats.sources = p->n_sources;
     g.progress = p->n_sources;
   });
@@ -734,6 +855,26 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
         off.push_back((uint32_t)dest.size());
       }
     }
+    if (!g.identity_order) {
+      // device node ids, origins ascending in device order
+      std::vector<uint32_t> ord(src.size());
+      for (uint32_t i = 0; i < ord.size(); ++i) ord[i] = i;
+      std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return g.perm[src[a]] < g.perm[src[b]]; });
+      std::vector<uint32_t> src2, off2{0}, dest2;
+      std::vector<float> w2;
+      for (const uint32_t o : ord) {
+        src2.push_back(g.perm[src[o]]);
+        for (uint32_t k = off[o]; k < off[o + 1]; ++k) {
+          dest2.push_back(g.perm[dest[k]]);
+          w2.push_back(w[k]);
+        }
+        off2.push_back((uint32_t)dest2.size());
+      }
+      src.swap(src2);
+      off.swap(off2);
+      dest.swap(dest2);
+      w.swap(w2);
+    }
     CallSpec spec = make_spec(p);
     spec.closeness = false;
     spec.betweenness = true;
@@ -770,7 +911,7 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
               "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
     // rows 5 and 6 (betweenness, betweenness_beta) of the [M][T][N] accumulators
-    std::memcpy(out, hacc.data() + (size_t)5 * T * N, sizeof(double) * 2 * T * N);
+    unpack_acc_strided(hacc.data() + (size_t)5 * T * N, N, 2, T, T, out, result_order(g));
     t_stats.sources = ns;
     g.progress = N;
   });
@@ -796,6 +937,8 @@ int cs_segment_centrality(cs_graph *h, const cs_params *p, double *out) {
     std::vector<uint32_t> srcs;
     for (uint32_t i = 0; i < p->n_sources; ++i)
       if (g.live[p->sources[i]]) srcs.push_back(p->sources[i]);
+    to_device_ids(g, srcs);
+    std::sort(srcs.begin(), srcs.end());
     std::lock_guard<std::mutex> lk(g.mu);
     DevGraph &dg = device_graph(g, dev);
     DevEdges &de = device_edges(g, dg, spec.speed);
@@ -810,7 +953,7 @@ int cs_segment_centrality(cs_graph *h, const cs_params *p, double *out) {
     hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
               "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
-    unpack_acc(hacc, N, M, T, out);
+    unpack_acc(hacc, N, M, T, out, result_order(g));
     t_stats.sources = srcs.size();
     g.progress = p->n_sources;
   });
@@ -851,7 +994,7 @@ int cs_centrality_simplest(cs_graph *h, const cs_params *p, double *out, uint32_
               "copy out");
     hip_check(hipMemcpyAsync(hreach, reach.p, 4 * kMaxT, hipMemcpyDeviceToHost, stream), "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
-    unpack_acc(hacc, N, M, T, out);
+    unpack_acc(hacc, N, M, T, out, nullptr);
     if (p->compute_betweenness) scale_rows(out, N, T, 3, 3, p->betw_scale);
     if (reach_totals)
       for (uint32_t i = 0; i < T; ++i) reach_totals[i] = hreach[i];
@@ -906,7 +1049,7 @@ Single run_single(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, 
   to.pred_list = d_pl.as<uint32_t>();
   to.n_preds = d_np.as<uint32_t>();
   to.max_preds = max_preds;
-  launch_single(g, dg, de, src, (float)max_seconds, tolerance, mode, to, stream);
+  launch_single(g, dg, de, g.perm[src], (float)max_seconds, tolerance, mode, to, stream);
   hip_check(hipMemcpyAsync(s.secs.data(), d_secs.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
   hip_check(hipMemcpyAsync(s.pred.data(), d_pred.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
   hip_check(hipMemcpyAsync(s.last.data(), d_last.p, 4ull * N, hipMemcpyDeviceToHost, stream), "copy");
@@ -921,6 +1064,28 @@ Single run_single(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, 
                 "copy");
   }
   hip_check(hipStreamSynchronize(stream), "sync");
+  if (!g.identity_order) {
+    // device node order -> original order (edge ids are original already)
+    const std::vector<float> secs = s.secs;
+    const std::vector<int32_t> pred = s.pred, last = s.last;
+    for (uint32_t v = 0; v < N; ++v) {
+      const uint32_t d = g.perm[v];
+      s.secs[v] = secs[d];
+      s.pred[v] = pred[d] >= 0 ? (int32_t)g.iperm[pred[d]] : -1;
+      s.last[v] = last[d];
+    }
+    if (mode == 1) {
+      const std::vector<double> sigma = s.sigma;
+      const std::vector<uint32_t> np = s.npreds, pl = s.preds;
+      for (uint32_t v = 0; v < N; ++v) {
+        const uint32_t d = g.perm[v];
+        s.sigma[v] = sigma[d];
+        s.npreds[v] = np[d];
+        const uint32_t k = std::min(np[d], max_preds);
+        for (uint32_t i = 0; i < k; ++i) s.preds[(size_t)v * max_preds + i] = g.iperm[pl[(size_t)d * max_preds + i]];
+      }
+    }
+  }
   return s;
 }
 
@@ -1175,7 +1340,8 @@ int cs_batch_add_shortest(cs_batch *b, cs_graph *h, const cs_params *p, double *
     j.h = h;
     j.spec = make_spec(p);
     j.ss = source_set(g, p);
-    j.elig.assign(p->eligible, p->eligible + g.n);
+    to_device_sources(g, j.ss);
+    j.elig = to_device_u8(g, p->eligible);
     j.out = out;
     j.reach = reach_totals;
     j.betw = p->compute_betweenness != 0;
@@ -1240,7 +1406,7 @@ int cs_batch_results(cs_batch *b) {
     for (size_t i = 0; i < b->jobs.size(); ++i) {
       const cs_batch::JobHost &j = b->jobs[i];
       const uint32_t N = j.h->g.n, T = j.spec.T;
-      unpack_acc_strided(hacc.data() + j.acc_off, N, 7, T, b->NT, j.out);
+      unpack_acc_strided(hacc.data() + j.acc_off, N, 7, T, b->NT, j.out, result_order(j.h->g));
       if (j.betw) scale_rows(j.out, N, T, 5, 6, j.betw_scale);
       if (j.reach)
         for (uint32_t t = 0; t < T; ++t) j.reach[t] = hreach[8 * i + t];

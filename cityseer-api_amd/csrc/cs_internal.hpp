@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -55,9 +56,15 @@ struct DevEdges {
 
 struct DevGraph {
   int device = -1;
-  uint32_t *rev_off = nullptr;
+  uint32_t *rev_off = nullptr;  // original node order (angular searches on dual graphs)
   uint32_t *fwd_off = nullptr;
-  uint4 *rows = nullptr;  // {fwd begin, fwd end, rev begin, rev end} per node: one 16-byte load
+  uint4 *rows = nullptr;  // {fwd begin, fwd end, rev begin, rev end} per device node: one 16-byte load
+  uint32_t *orig = nullptr;  // [N] original index of each device node (tie order)
+  uint32_t *perm = nullptr;  // [N] device index of each original node
+  // staging for calls whose device buffers are in original order (grown on
+  // demand, used under the graph's lock)
+  void *xbuf = nullptr;
+  size_t xbuf_bytes = 0;
   std::vector<std::unique_ptr<DevEdges>> edges;  // small cache keyed by speed
 };
 
@@ -80,6 +87,13 @@ struct Graph {
   std::string slots_err;
   uint32_t max_out_deg = 0, max_in_deg = 0;
   bool has_loops = false;
+  // Device node order: nodes lie in device memory along a space-filling curve
+  // of their coordinates when those are given (primal graphs), so that nodes
+  // near in space are near in memory. perm[original] = device index, iperm =
+  // its inverse; both identity otherwise. Every entry point translates.
+  std::vector<uint32_t> perm, iperm;
+  bool identity_order = true;
+  mutable std::atomic<bool> order_frozen{false};  // set once an entry point has translated ids
 
   std::mutex mu;
   std::map<int, std::unique_ptr<DevGraph>> dev;
@@ -89,6 +103,9 @@ struct Graph {
 // Host edge travel seconds (centrality.rs:970-1007).
 float edge_seconds(const Graph &g, uint32_t from, uint32_t to, uint32_t e, float speed,
                    bool use_impedance);
+
+// Device node order from coordinates (cs_host.cpp); before any device use.
+void set_node_order_from_xy(Graph &g, const double *x, const double *y);
 
 // Device residency (cs_host.cpp).
 DevGraph &device_graph(Graph &g, int device);
@@ -120,6 +137,16 @@ void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                      uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
                      uint32_t *d_reach, hipStream_t stream, cs_stats &st);
+
+// launch_shortest with sources, eligibility and acc in original node order
+// (device buffers of the caller), translated to and from the graph's device
+// order through staging kept with the device graph (dg.xbuf); the caller
+// holds the graph's lock. Returns after the stream has finished the call's
+// work when the order is not the identity.
+void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
+                              const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
+                              uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
+                              uint32_t *d_reach, hipStream_t stream, cs_stats &st);
 
 // Launch shape of the search kernel: workgroup size (64..256), a cap on
 // resident waves per SIMD (0 = as many as fit), the bucket width as a fraction

@@ -85,6 +85,7 @@ struct Job {
   const uint32_t *od_off;
   const uint32_t *od_dest;
   const float *od_w;
+  const uint32_t *orig;  // [N] original index of each device node (exact ties)
   uint32_t tstride;  // threshold stride of acc (>= the job's threshold count)
   uint32_t nacc;     // node count of acc (the job's graph)
   float dist[kMaxT];  // thresholds beyond the job's own count hold -1 (never within)
@@ -192,8 +193,9 @@ __device__ __forceinline__ uint32_t bucket_of(float s, float inv_delta) {
 
 // Reference settle order: ascending seconds (route is monotone in seconds),
 // then node index (deterministic stand-in for BinaryHeap's internal order).
-__device__ __forceinline__ bool key_less(float sa, uint32_t a, float sb, uint32_t b) {
-  return sa < sb || (sa == sb && a < b);
+// Node ids here are device ids; exact ties compare the original indices.
+__device__ __forceinline__ bool key_less(const uint32_t *orig, float sa, uint32_t a, float sb, uint32_t b) {
+  return sa < sb || (sa == sb && orig[a] < orig[b]);
 }
 
 // Per-slot record, two 64-bit words:
@@ -304,8 +306,8 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
       const float cs = su + __uint_as_float(e.y);
       if (cs > j.maxsec || (double)cs > thr_far) continue;
       const uint32_t jr = p - r0;
-      if (have_last && !(key_less(lsu, lu, su, u) || (su == lsu && u == lu && jr > lj))) continue;
-      if (!found || key_less(su, u, psu, pu) || (su == psu && u == pu && jr < pj)) {
+      if (have_last && !(key_less(j.orig, lsu, lu, su, u) || (su == lsu && u == lu && jr > lj))) continue;
+      if (!found || key_less(j.orig, su, u, psu, pu) || (su == psu && u == pu && jr < pj)) {
         found = true;
         psu = su;
         pu = u;
@@ -461,7 +463,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             const uint64_t ku = fk[q];
             if (!key_current(ku, c.tag)) continue;
             const float su = key_secs(ku);
-            if (cycles && (fe[q].x & kCanonBit) && key_less(su, u, secs_v, v)) ++cnt_e;
+            if (cycles && (fe[q].x & kCanonBit) && key_less(j.orig, su, u, secs_v, v)) ++cnt_e;
             if (!fold || bucket_of(su, j.inv_delta) >= c.k) continue;
             const float cs = su + __uint_as_float(fe[q].y);
             if (cs > j.maxsec) continue;
@@ -480,7 +482,8 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
               // dijkstra_tree_shortest / dijkstra_tree_segment (centrality.rs:1186-1195,
               // 1589-1607): strict < in settle order keeps the lexicographic min
               // (cs, secs_u, u, row position)
-              if (cs < bcs || (cs == bcs && (su < bsu || (su == bsu && (u < bu || (u == bu && jr < bj)))))) {
+              if (cs < bcs ||
+                  (cs == bcs && (su < bsu || (su == bsu && (j.orig[u] < j.orig[bu] || (u == bu && jr < bj)))))) {
                 bcs = cs;
                 bsu = su;
                 bu = u;
@@ -624,7 +627,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
                   } else if (cycles && (re.x & kCanonBit)) {
                     kn = ld64(nkey(w, nb));
                   }
-                  if (cycles && (re.x & kCanonBit) && key_current(kn, c.tag) && key_less(key_secs(kn), nb, so, vo)) {
+                  if (cycles && (re.x & kCanonBit) && key_current(kn, c.tag) && key_less(j.orig, key_secs(kn), nb, so, vo)) {
                     ce = true;
                     route_o = so * speed;
                   }
@@ -731,7 +734,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
           const uint64_t ku = ld64(nkey(w, u));
           if (!key_current(ku, c.tag)) continue;
           const float su = key_secs(ku);
-          if (!key_less(su, u, secs_v, v)) continue;
+          if (!key_less(j.orig, su, u, secs_v, v)) continue;
           const float ps = su + __uint_as_float(e.y);
           if (!(ps <= lim)) continue;
           bool dup = false;
@@ -779,8 +782,8 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
           for (uint64_t mm = pm; mm; mm &= mm - 1) {
             const uint32_t u = j.fwd_adj[f0 + __builtin_ctzll(mm)].x & kNodeMask;
             const float su = key_secs(ld64(nkey(w, u)));
-            if (have_last && !key_less(lsu, lu, su, u)) continue;
-            if (!found || key_less(su, u, bsu, bu)) {
+            if (have_last && !key_less(j.orig, lsu, lu, su, u)) continue;
+            if (!found || key_less(j.orig, su, u, bsu, bu)) {
               found = true;
               bsu = su;
               bu = u;
@@ -1095,7 +1098,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
                 sd_m = sd_n;
               } else if (key_current(kn[q], c.tag)) {
                 const float s_nb = key_secs(kn[q]);
-                visited_edge = key_less(secs_v, v, s_nb, nb);
+                visited_edge = key_less(j.orig, secs_v, v, s_nb, nb);
                 sd_m = s_nb * speed;
               }
               if (!visited_edge) continue;
@@ -1181,7 +1184,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
             for (int t = 0; t < NT; ++t) {
               if (below[t] > 0.0) gadd_f64(&j.acc[((size_t)3 * j.tstride + t) * j.nacc + v], below[t]);
               double own = 0.0;
-              if (v > c.src) {
+              if (j.orig[v] > j.orig[c.src]) {  // pairs counted once, in original index order
                 const float dist = j.dist[t];
                 const float beta = j.beta[t];
                 if (min_span <= dist) {
@@ -1493,6 +1496,7 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   std::memset(&j, 0, sizeof(j));
   const float delta = choose_delta(g, de, cs.maxsec) * js.delta_scale;
   j.rows = js.dg->rows;
+  j.orig = js.dg->orig;
   j.rev_adj = de.rev_adj;
   j.fwd_adj = de.fwd_adj;
   j.rev_tw = de.rev_tw;
@@ -1954,6 +1958,68 @@ void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, d_wt, d_cyc, n_sources, d_eligible, d_acc, d_reach, tune.delta_scale};
   run_jobs<kBrandes>({js}, cs.T, tune, nullptr, stream, st);
+}
+
+// Translation between buffers in original node order and the device order.
+__global__ void k_map_ids(uint32_t *out, const uint32_t *in, const uint32_t *perm, uint32_t n_ids, uint32_t N) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_ids) {
+    const uint32_t v = in[i];
+    out[i] = v < N ? perm[v] : v;  // an invalid id stays invalid
+  }
+}
+
+__global__ void k_gather_u8(uint8_t *out, const uint8_t *in, const uint32_t *orig, uint32_t N) {
+  const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < N) out[d] = in[orig[d]];
+}
+
+// dst[r][orig[d]] += src[r][d] over rows x N: orig is a permutation, so every
+// thread owns its destination
+__global__ void k_scatter_add_rows(double *dst, const double *src, const uint32_t *orig, uint32_t N, uint32_t rows) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (size_t)N * rows) {
+    const size_t r = i / N, d = i - r * N;
+    dst[r * N + orig[d]] += src[i];
+  }
+}
+
+void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
+                              const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
+                              uint32_t n_sources, const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
+                              hipStream_t stream, cs_stats &st) {
+  if (g.identity_order) {
+    launch_shortest(g, dg, de, cs, d_sources, d_wt, d_cyc, n_sources, d_eligible, d_acc, d_reach, stream, st);
+    return;
+  }
+  const uint32_t N = g.n, rows = 7 * cs.T;
+  constexpr uint32_t kB = 256;
+  const size_t total = (size_t)rows * N;
+  // staging, 256-byte aligned parts: acc [rows][N] f64 | sources u32 | eligibility u8
+  const size_t acc_b = (total * 8 + 255) & ~(size_t)255, src_b = ((size_t)n_sources * 4 + 255) & ~(size_t)255;
+  const size_t need = acc_b + src_b + N;
+  if (dg.xbuf_bytes < need) {
+    if (dg.xbuf) hip_check(hipFree(dg.xbuf), "hipFree staging");
+    dg.xbuf = nullptr;
+    dg.xbuf_bytes = 0;
+    hip_check(hipMalloc(&dg.xbuf, need), "hipMalloc staging");
+    dg.xbuf_bytes = need;
+  }
+  double *acc = static_cast<double *>(dg.xbuf);
+  uint32_t *src = reinterpret_cast<uint32_t *>(static_cast<char *>(dg.xbuf) + acc_b);
+  uint8_t *elig = reinterpret_cast<uint8_t *>(static_cast<char *>(dg.xbuf) + acc_b + src_b);
+  if (n_sources)
+    hipLaunchKernelGGL(k_map_ids, dim3((n_sources + kB - 1) / kB), dim3(kB), 0, stream, src, d_sources, dg.perm,
+                       n_sources, N);
+  hipLaunchKernelGGL(k_gather_u8, dim3((N + kB - 1) / kB), dim3(kB), 0, stream, elig, d_eligible, dg.orig, N);
+  hip_check(hipMemsetAsync(acc, 0, sizeof(double) * total, stream), "memset");
+  hip_check(hipGetLastError(), "translate launch");
+  launch_shortest(g, dg, de, cs, src, d_wt, d_cyc, n_sources, elig, acc, d_reach, stream, st);
+  hipLaunchKernelGGL(k_scatter_add_rows, dim3((uint32_t)((total + kB - 1) / kB)), dim3(kB), 0, stream, d_acc, acc,
+                     dg.orig, N, rows);
+  hip_check(hipGetLastError(), "scatter launch");
+  // the staging is reused by the graph's next call once the lock is released
+  hip_check(hipStreamSynchronize(stream), "sync");
 }
 
 void launch_shortest_batch(const std::vector<JobSpec> &jobs, uint32_t NT, const Tuning &tune, hipStream_t stream,

@@ -155,7 +155,8 @@ struct cs_server {
     if (!hacc.empty())
       hip_check(hipMemcpy(hacc.data(), r.d_acc, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost), "copy out");
     hip_check(hipMemcpy(hreach, r.d_reach, 4 * kMaxT, hipMemcpyDeviceToHost), "copy out");
-    unpack_acc_strided(hacc.data(), r.N, r.M, r.T, r.T, r.out);
+    unpack_acc_strided(hacc.data(), r.N, r.M, r.T, r.T, r.out,
+                       r.kind == CS_REQ_SHORTEST ? result_order(r.h->g) : nullptr);
     if (r.betw) {
       if (r.kind == CS_REQ_SHORTEST)
         scale_rows(r.out, r.N, r.T, 5, 6, r.betw_scale);
@@ -335,7 +336,12 @@ int64_t submit_impl(cs_server *s, cs_graph *h, int kind, const cs_params *p, dou
   r->kind = kind;
   r->spec = make_spec(p);
   r->ss = source_set(g, p);
-  r->elig.assign(p->eligible, p->eligible + g.n);
+  if (kind == CS_REQ_SHORTEST) {
+    to_device_sources(g, r->ss);
+    r->elig = to_device_u8(g, p->eligible);
+  } else {
+    r->elig.assign(p->eligible, p->eligible + g.n);  // the angular search keeps the original order
+  }
   r->out = out;
   r->reach = reach;
   r->betw = p->compute_betweenness != 0;

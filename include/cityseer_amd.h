@@ -78,6 +78,23 @@ int cs_graph_create(uint32_t n_nodes, const double *z, const uint8_t *live, cons
                     int is_dual, cs_graph **out);
 void cs_graph_destroy(cs_graph *g);
 
+/* Node coordinates (x, y per node, the reference's NetworkStructure node
+ * positions, rust/src/graph.rs:426-452). They set the order nodes take in device
+ * memory: along a Hilbert curve, so that nodes near in space are near in
+ * memory and a shortest-path search touches fewer cache lines. Results are
+ * unchanged: every entry point takes and returns original node indices, and
+ * exact ties still resolve by original index. Dual graphs, graphs with
+ * non-finite coordinates and graphs of fewer than two nodes keep their order.
+ * Must come before the graph's first computation (CS_EINVAL after it). No
+ * reference counterpart: the reference keeps nodes in insertion order. */
+int cs_graph_set_xy(cs_graph *g, const double *x, const double *y);
+
+/* order[d] = original index of the node at device position d (n_nodes
+ * entries). Sources listed in this order (cs_centrality_shortest_device) run
+ * fastest: neighbouring searches then run side by side. No reference
+ * counterpart. */
+int cs_graph_node_order(const cs_graph *g, uint32_t *order);
+
 /* Number of nodes/edges; maximum out/in degree. */
 int cs_graph_info(const cs_graph *g, uint32_t *n_nodes, uint32_t *n_edges, uint32_t *max_out_deg,
                   uint32_t *max_in_deg);
@@ -123,7 +140,10 @@ int cs_centrality_shortest(cs_graph *g, const cs_params *p, double *out, uint32_
  * adds into d_acc, f64 [7][T][N] laid out like `out` (zeroed by the caller), and d_reach
  * u32 [T] (may be NULL); betw_scale is not applied. stream may be NULL (the
  * null stream). Same computation as centrality_shortest
- * (rust/src/centrality.rs:1624-1874) without host copies. */
+ * (rust/src/centrality.rs:1624-1874) without host copies. All node-indexed
+ * buffers are in original node order; when the graph has a device order
+ * (cs_graph_set_xy) the call translates them through staging it keeps with
+ * the graph and returns after the stream has finished its work. */
 int cs_centrality_shortest_device(cs_graph *g, const cs_params *p, const float *d_cycles_wt, double *d_acc,
                                   uint32_t *d_reach, void *stream);
 

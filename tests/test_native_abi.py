@@ -103,6 +103,70 @@ def test_graph_create_rejects_bad_input():
     assert b"does not exist" in lib.cs_last_error()
 
 
+def _grid_graph(side: int, perm: np.ndarray, is_dual: int = 0):
+    """A side x side grid whose node i sits at old grid cell perm[i]; a chain of edges."""
+    lib = _native.load()
+    n = side * side
+    src = np.arange(n - 1, dtype=np.uint32)
+    dst = np.arange(1, n, dtype=np.uint32)
+    length = np.ones(n - 1, np.float32)
+    p = _native.ptr
+    h = ctypes.c_void_p()
+    pk = np.zeros(n - 1, np.int32)
+    _native.check(lib.cs_graph_create(n, None, None, None, n - 1, p(src, ctypes.c_uint32), p(dst, ctypes.c_uint32),
+                                      None, p(length, ctypes.c_float), None, None, None,
+                                      p(pk, ctypes.c_int32) if is_dual else None, is_dual, ctypes.byref(h)))
+    x = (perm % side).astype(np.float64) * 37.0 + 1000.0
+    y = (perm // side).astype(np.float64) * 37.0 - 500.0
+    return lib, h, x, y
+
+
+def _order(lib, h, n):
+    out = np.empty(n, np.uint32)
+    _native.check(lib.cs_graph_node_order(h, _native.ptr(out, ctypes.c_uint32)))
+    return out
+
+
+def test_device_node_order_follows_a_hilbert_curve():
+    """cs_graph_set_xy orders a shuffled 8 x 8 grid along a Hilbert curve: a permutation whose
+    every step moves to an edge-adjacent grid cell, starting in a corner (the curve's defining
+    properties; a wrong rotation or quadrant code breaks adjacency)."""
+    side, n = 8, 64
+    perm = np.random.default_rng(5).permutation(n)
+    lib, h, x, y = _grid_graph(side, perm)
+    try:
+        assert np.array_equal(_order(lib, h, n), np.arange(n))  # identity until coordinates arrive
+        _native.check(lib.cs_graph_set_xy(h, _native.ptr(x, ctypes.c_double), _native.ptr(y, ctypes.c_double)))
+        order = _order(lib, h, n)
+        assert sorted(order.tolist()) == list(range(n))
+        cells = perm[order]
+        cx, cy = cells % side, cells // side
+        steps = np.abs(np.diff(cx)) + np.abs(np.diff(cy))
+        assert np.all(steps == 1), steps
+        assert (cx[0], cy[0]) == (0, 0) and cy[-1] == 0 and cx[-1] == side - 1
+        assert lib.cs_graph_set_xy(h, None, _native.ptr(y, ctypes.c_double)) == _native.CS_EINVAL
+    finally:
+        lib.cs_graph_destroy(h)
+
+
+def test_device_node_order_kept_for_dual_and_missing_coordinates():
+    side, n = 4, 16
+    perm = np.random.default_rng(6).permutation(n)
+    lib, h, x, y = _grid_graph(side, perm, is_dual=1)
+    try:
+        _native.check(lib.cs_graph_set_xy(h, _native.ptr(x, ctypes.c_double), _native.ptr(y, ctypes.c_double)))
+        assert np.array_equal(_order(lib, h, n), np.arange(n))
+    finally:
+        lib.cs_graph_destroy(h)
+    lib, h, x, y = _grid_graph(side, perm)
+    try:
+        x[3] = np.nan
+        _native.check(lib.cs_graph_set_xy(h, _native.ptr(x, ctypes.c_double), _native.ptr(y, ctypes.c_double)))
+        assert np.array_equal(_order(lib, h, n), np.arange(n))
+    finally:
+        lib.cs_graph_destroy(h)
+
+
 def test_compute_fails_loudly_without_gpu(diamond_graph):
     """No CPU fallback: without a HIP device the compute entry points raise."""
     try:
