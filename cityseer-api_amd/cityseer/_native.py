@@ -71,6 +71,7 @@ class CsTuning(ctypes.Structure):
         ("delta_scale", ctypes.c_float),
         ("slot_capacity", ctypes.c_uint32),
         ("xcd_split", ctypes.c_uint32),
+        ("simplest_lanes", ctypes.c_uint32),
     ]
 
 

@@ -61,7 +61,9 @@ struct AngArgs {
   size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_orte, o_delta, o_seed, o_node, o_brt, o_bsec,
       o_heap;
   char *wave_ws;
-  uint32_t *epochs;  // per lane
+  uint32_t per_wave;    // lanes of a wave that take sources (the rest take padding items)
+  uint32_t epoch_base;  // item it searches under epoch epoch_base + it + 1
+  uint32_t *epochs;     // single-source tree: its one epoch counter
   uint32_t *next_source;
   uint32_t *fail_list;
   uint32_t *n_fail;
@@ -421,11 +423,11 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;  // global lane
   const uint32_t wid = __builtin_amdgcn_readfirstlane(gl / kWave);
   LaneWs L;
-  L.skey = reinterpret_cast<uint64_t *>(a.ws + (size_t)gl * a.lane_stride);
+  // lanes >= per_wave only ever take padding items and touch no workspace
+  L.skey = reinterpret_cast<uint64_t *>(a.ws + ((size_t)wid * a.per_wave + lane) * a.lane_stride);
   L.nkey = L.skey + 2 * (size_t)a.N;
   L.w = a.wave_ws + (size_t)wid * a.wave_stride;
   L.lane = lane;
-  uint32_t epoch = a.epochs[gl];
   unsigned long long st_reached = 0, st_edges = 0, st_within = 0;
   char *w = L.w;
   // Items are dealt statically, lane gl taking gl, gl + lanes, ...; each job's
@@ -443,7 +445,9 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     CAngJob &jb = ((CAngJob *)a.jobs)[job];
     const uint32_t src = jb.sources[si];
     const float wt = jb.src_wt[si];
-    ++epoch;
+    // every item of every launch since the maps were zeroed has its own epoch,
+    // so map entries left by any earlier search (any layout) never match
+    const uint32_t epoch = a.epoch_base + it + 1;
     uint32_t n_states, n_nodes, n_order, fail = 0;
     bool ok = angular_source(a, jb, L, src, epoch, n_states, n_nodes, n_order, fail);
     if (ok && jb.tolerance > kTieEps && (jb.flags & 2u)) ok = angular_phase2(a, jb, L, epoch, n_order, fail);
@@ -566,7 +570,6 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
       }
     }
   }
-  a.epochs[gl] = epoch;
   atomicAdd(&a.stats[0], st_reached);
   atomicAdd(&a.stats[1], st_edges);
   atomicAdd(&a.stats[2], st_within);
@@ -677,6 +680,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct AngPlan {
   uint32_t scap, ncap, hcap, maxp, lanes, tcap;
+  uint32_t per_wave = kWave;  // lane blocks per wave in the interleaved arrays
   size_t lane_stride, wave_stride;
   size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_orte, o_delta, o_seed, o_node, o_brt, o_bsec, o_heap;
 };
@@ -687,7 +691,7 @@ void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
   size_t o = 0;
   auto take = [&](size_t &field, size_t elem_bytes, size_t count) {
     field = o;
-    o = align256(o + elem_bytes * count * kWave);
+    o = align256(o + elem_bytes * count * p.per_wave);
   };
   take(p.o_route, 4, p.scap);
   take(p.o_secs, 4, p.scap);
@@ -709,11 +713,8 @@ void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
 struct AngWorkspace {
   int device = -1;
   char *lane_ws = nullptr, *wave_ws = nullptr;
-  uint32_t *epochs = nullptr;
   size_t lane_bytes = 0, wave_bytes = 0;
-  uint32_t lanes = 0, N = 0;
-  size_t lane_stride = 0, wave_stride = 0;
-  uint64_t epoch_hi = 0;
+  uint64_t epoch_hi = 0;  // epochs issued since the lane maps were last zeroed
   std::mutex mu;
 };
 
@@ -730,40 +731,46 @@ AngWorkspace &ang_ws(int device) {
   return *p;
 }
 
-void prepare_ang(AngWorkspace &w, const AngPlan &p, uint32_t N, uint64_t sources, hipStream_t stream) {
-  const size_t lane_need = p.lane_stride * p.lanes;
+// Grows the workspace to the plan (lane maps of per_wave lanes per wave,
+// interleaved arrays per wave) and returns the launch's epoch base. The maps
+// are zeroed only when allocated or when the 32-bit epochs would wrap: a
+// launch's epochs all exceed every epoch issued before it, so the maps need no
+// clearing when the layout (graph size, lanes per wave) changes between calls.
+uint32_t prepare_ang(AngWorkspace &w, const AngPlan &p, uint64_t n_items, hipStream_t stream) {
+  const size_t lane_need = p.lane_stride * (size_t)(p.lanes / kWave) * p.per_wave;
   const size_t wave_need = p.wave_stride * (p.lanes / kWave);
-  bool reinit = false;
-  if (lane_need > w.lane_bytes || wave_need > w.wave_bytes || w.N != N || w.lane_stride != p.lane_stride ||
-      w.wave_stride != p.wave_stride || p.lanes > w.lanes) {
+  bool zero = false;
+  if (lane_need > w.lane_bytes) {
     (void)hipFree(w.lane_ws);
-    (void)hipFree(w.wave_ws);
-    (void)hipFree(w.epochs);
-    w.lane_ws = w.wave_ws = nullptr;
-    w.epochs = nullptr;
-    if (hipMalloc(&w.lane_ws, lane_need) != hipSuccess || hipMalloc(&w.wave_ws, wave_need) != hipSuccess) {
+    w.lane_ws = nullptr;
+    w.lane_bytes = 0;
+    if (hipMalloc(&w.lane_ws, lane_need) != hipSuccess) {
       (void)hipGetLastError();
-      (void)hipFree(w.lane_ws);
       w.lane_ws = nullptr;
-      w.lane_bytes = w.wave_bytes = 0;
       fail(CS_ENOMEM, "could not allocate the angular search workspace");
     }
-    hip_check(hipMalloc(&w.epochs, 4ull * p.lanes), "hipMalloc epochs");
     w.lane_bytes = lane_need;
-    w.wave_bytes = wave_need;
-    w.lanes = p.lanes;
-    w.N = N;
-    w.lane_stride = p.lane_stride;
-    w.wave_stride = p.wave_stride;
-    reinit = true;
+    zero = true;
   }
-  if (w.epoch_hi + sources + 2 >= 0xfffffff0ull) reinit = true;
-  if (reinit) {
-    hip_check(hipMemsetAsync(w.lane_ws, 0, lane_need, stream), "memset");
-    hip_check(hipMemsetAsync(w.epochs, 0, 4ull * w.lanes, stream), "memset");
+  if (wave_need > w.wave_bytes) {
+    (void)hipFree(w.wave_ws);
+    w.wave_ws = nullptr;
+    w.wave_bytes = 0;
+    if (hipMalloc(&w.wave_ws, wave_need) != hipSuccess) {
+      (void)hipGetLastError();
+      w.wave_ws = nullptr;
+      fail(CS_ENOMEM, "could not allocate the angular search workspace");
+    }
+    w.wave_bytes = wave_need;
+  }
+  if (w.epoch_hi + n_items + 2 >= 0xfffffff0ull) zero = true;
+  if (zero) {
+    hip_check(hipMemsetAsync(w.lane_ws, 0, w.lane_bytes, stream), "memset");
     w.epoch_hi = 0;
   }
-  w.epoch_hi += sources;
+  const uint32_t base = (uint32_t)w.epoch_hi;
+  w.epoch_hi += n_items + 1;
+  return base;
 }
 
 }  // namespace
@@ -796,24 +803,54 @@ static uint4 *angular_adjacency(Graph &g, DevEdges &de) {
 }
 
 // Items grouped by job (stable) with each job's run padded to whole waves.
-std::vector<uint2> wave_aligned(const std::vector<uint2> &in, size_t n_jobs) {
+// Items grouped by job, `per_wave` real items to each wave of 64 (the rest
+// padding), so that the 64 lanes of a wave always belong to one job.
+std::vector<uint2> wave_aligned(const std::vector<uint2> &in, size_t n_jobs, uint32_t per_wave) {
   std::vector<std::vector<uint2>> by(n_jobs);
   for (const uint2 &x : in) by[x.x].push_back(x);
   std::vector<uint2> out;
   for (uint32_t j = 0; j < n_jobs; ++j) {
-    for (const uint2 &x : by[j]) out.push_back(x);
+    for (size_t i = 0; i < by[j].size(); ++i) {
+      if (i && i % per_wave == 0)
+        while (out.size() % kWave) out.push_back(make_uint2(j, kNoSource));
+      out.push_back(by[j][i]);
+    }
     while (out.size() % kWave) out.push_back(make_uint2(j, kNoSource));
   }
   return out;
 }
 
+// Sources per wave. A lane's search is a chain of dependent gathers, and one
+// wave-wide gather costs as many cache-line requests as it has active lanes.
+// A launch with few sources therefore spreads them over more waves with fewer
+// lanes each, so that every SIMD takes part; a full launch keeps 64.
+uint32_t sources_per_wave(uint32_t n_real, uint32_t simds) {
+  const uint32_t forced = default_tuning().ang_lanes;
+  if (forced) return std::min<uint32_t>(forced, kWave);
+  const uint32_t waves = std::max<uint32_t>(simds, 1);
+  const uint32_t per = (n_real + waves - 1) / waves;
+  return std::max<uint32_t>(1, std::min<uint32_t>(kWave, per));
+}
+
 void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<uint2> *explicit_items,
                           uint32_t n_implicit, hipStream_t stream, cs_stats &st) {
   if (specs.empty()) return;
+  const int dev0 = specs[0].dg->device;
+  int n_cu = 0;
+  hip_check(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev0), "attribute");
+  const uint32_t n_real = explicit_items ? (uint32_t)explicit_items->size() : n_implicit;
+  if (n_real == 0) return;
+  const uint32_t per_wave = sources_per_wave(n_real, 4u * (uint32_t)n_cu);
   std::vector<uint2> items;
-  if (explicit_items) items = wave_aligned(*explicit_items, specs.size());
-  uint32_t n_list = explicit_items ? (uint32_t)items.size() : n_implicit;
-  if (n_list == 0) return;
+  if (explicit_items) {
+    items = wave_aligned(*explicit_items, specs.size(), per_wave);
+  } else if (per_wave < kWave) {
+    std::vector<uint2> all(n_implicit);
+    for (uint32_t i = 0; i < n_implicit; ++i) all[i] = make_uint2(0u, i);
+    items = wave_aligned(all, 1, per_wave);
+  }
+  const bool have_items = !items.empty();
+  uint32_t n_list = have_items ? (uint32_t)items.size() : n_implicit;
   uint32_t N = 1, maxp = 1, T = 1;
   std::vector<AngJob> jobs(specs.size());
   for (size_t i = 0; i < specs.size(); ++i) {
@@ -863,7 +900,7 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
   hip_check(hipMalloc(&d_stats, 32), "hipMalloc");
   hip_check(hipMalloc(&d_jobs, sizeof(AngJob) * jobs.size()), "hipMalloc");
   hip_check(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(AngJob) * jobs.size(), hipMemcpyHostToDevice, stream), "copy");
-  if (explicit_items) {
+  if (have_items) {
     hip_check(hipMalloc(&d_items, sizeof(uint2) * n_list), "hipMalloc");
     hip_check(hipMemcpyAsync(d_items, items.data(), sizeof(uint2) * n_list, hipMemcpyHostToDevice, stream), "copy");
   }
@@ -873,16 +910,20 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
   hip_check(hipEventCreate(&e1), "event");
   double kms = 0.0;
   for (int attempt = 0; attempt < 6; ++attempt) {
+    p.per_wave = per_wave;
     ang_layout(p, N, T);
-    const size_t per_lane = p.lane_stride + p.wave_stride / kWave;
+    // a wave holds per_wave lanes' maps and interleaved arrays
+    const size_t wave_bytes = p.lane_stride * p.per_wave + p.wave_stride;
     const size_t budget = free_b / 4 + wsp.lane_bytes + wsp.wave_bytes;
-    uint32_t lanes = (uint32_t)std::min<size_t>(budget / per_lane, (size_t)n_list);
-    lanes = std::min<uint32_t>(lanes, (uint32_t)prop.multiProcessorCount * 4 * kWave * 8);
-    lanes = (lanes + kBlock - 1) / kBlock * kBlock;
-    if (lanes == 0 || (size_t)lanes * per_lane > budget + per_lane * kBlock)
+    const size_t wpb = kBlock / kWave;
+    size_t waves = std::min<size_t>(budget / wave_bytes, ((size_t)n_list + kWave - 1) / kWave);
+    waves = std::min<size_t>(waves, (size_t)prop.multiProcessorCount * 4 * 8);
+    waves = (waves + wpb - 1) / wpb * wpb;
+    if (waves == 0 || waves * wave_bytes > budget + wave_bytes * wpb)
       fail(CS_ENOMEM, "graph too large for the angular search workspace on this device");
+    const uint32_t lanes = (uint32_t)(waves * kWave);
     p.lanes = lanes;
-    prepare_ang(wsp, p, N, n_list, stream);
+    const uint32_t epoch_base = prepare_ang(wsp, p, n_list, stream);
     hip_check(hipMemsetAsync(d_ctl, 0, 16, stream), "memset");
     AngArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -914,7 +955,8 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     a.o_brt = p.o_brt;
     a.o_bsec = p.o_bsec;
     a.o_heap = p.o_heap;
-    a.epochs = wsp.epochs;
+    a.per_wave = p.per_wave;
+    a.epoch_base = epoch_base;
     a.next_source = d_ctl;
     a.n_fail = d_ctl + 1;
     a.fail_flags = d_ctl + 2;
@@ -940,7 +982,7 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     hip_check(hipMemcpy(fl.data(), d_fail, 4ull * n_failed, hipMemcpyDeviceToHost), "copy");
     std::vector<uint2> retry(n_failed);
     for (uint32_t i = 0; i < n_failed; ++i) retry[i] = d_items ? items[fl[i]] : make_uint2(0u, fl[i]);
-    items = wave_aligned(retry, specs.size());
+    items = wave_aligned(retry, specs.size(), per_wave);
     (void)hipFree(d_items);
     hip_check(hipMalloc(&d_items, sizeof(uint2) * items.size()), "hipMalloc");
     hip_check(hipMemcpy(d_items, items.data(), sizeof(uint2) * items.size(), hipMemcpyHostToDevice), "copy");

@@ -1288,6 +1288,8 @@ Tuning from_c(const cs_tuning *t) {
   r.delta_scale = t->delta_scale;
   r.cap = t->slot_capacity;
   r.xcd_split = t->xcd_split ? 1u : 0u;
+  if (t->simplest_lanes > 64) fail(CS_EINVAL, "simplest_lanes must be at most 64");
+  r.ang_lanes = t->simplest_lanes;
   return r;
 }
 cs_tuning to_c(const Tuning &t) {
@@ -1297,6 +1299,7 @@ cs_tuning to_c(const Tuning &t) {
   r.delta_scale = t.delta_scale;
   r.slot_capacity = t.cap;
   r.xcd_split = t.xcd_split;
+  r.simplest_lanes = t.ang_lanes;
   return r;
 }
 }  // namespace

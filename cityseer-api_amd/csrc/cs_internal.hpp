@@ -157,6 +157,7 @@ struct Tuning {
   float delta_scale = 1.0f;
   uint32_t cap = 0;
   uint32_t xcd_split = 1;  // sources dealt to XCDs in contiguous shares (1) or from one counter (0)
+  uint32_t ang_lanes = 0;  // angular: sources per wave (1..64), 0 = from the launch's source count
 };
 void set_default_tuning(const Tuning &t);
 Tuning default_tuning();

@@ -228,6 +228,7 @@ typedef struct cs_tuning {
   uint32_t slot_capacity;   /* settled-node slots per wave (simplest: search states per lane) before a
                                retry with four times as many; 0 = default (16384; simplest 8192) */
   uint32_t xcd_split;       /* 1: each XCD's waves take a contiguous share of the sources first; 0: one counter */
+  uint32_t simplest_lanes;  /* simplest: sources per wave (1..64), 0 = chosen from the launch's source count */
 } cs_tuning;
 
 /* Launch shape of the plain shortest/segment entry points (process-wide). */

@@ -141,3 +141,51 @@ def test_source_indices_and_tolerance_band():
                                         eligible=plan["eligible"], track=True, scale=plan["scale"])
     assert_close(stack(res, KEYS, [800, 1600]), ref, "angular subset")
     assert res.reachability_totals == reach.tolist()
+
+
+def _lanes(n):
+    import ctypes
+
+    from cityseer import _native
+
+    lib = _native.load()
+    t = _native.CsTuning()
+    _native.check(lib.cs_get_tuning(ctypes.byref(t)))
+    t.simplest_lanes = n
+    _native.check(lib.cs_set_tuning(ctypes.byref(t)))
+
+
+def test_sources_per_wave_and_workspace_reuse():
+    """Launches alternate between graphs and source counts, so the workspace layout (graph
+    size, sources per wave) changes from call to call over maps that are never cleared in
+    between; every result must still match the oracle, and forcing any number of sources
+    per wave must give the same results."""
+    small = mock.network_structure_from_lattice(mock.lattice_dual(mock.perturbed_lattice(10, seed=41)))
+    big = mock.network_structure_from_lattice(mock.lattice_dual(mock.perturbed_lattice(24, seed=42)))
+    og_s, og_b = OracleGraph.from_network_structure(small), OracleGraph.from_network_structure(big)
+    rng = np.random.default_rng(43)
+    sub = np.sort(rng.choice(small.node_count(), 23, replace=False)).tolist()
+
+    def sub_check(ns, og, dists, sources):
+        res = ns.centrality_simplest(distances=dists, source_indices=sources)
+        plan = ns._sampling_plan(None, None, None, sources)
+        ref, _reach = og.centrality_simplest(dists, sources=plan["sources"], wt=plan["wt"],
+                                             eligible=plan["eligible"], track=True, scale=plan["scale"])
+        got = stack(res, KEYS, dists)
+        assert np.array_equal(got[0], ref[0])
+        assert_close(got, ref, "simplest subset")
+        return got
+
+    try:
+        say("auto sources per wave: small graph subset, big graph, small graph again")
+        first = sub_check(small, og_s, [300, 700], sub)
+        check_simplest(big, og_b, [400, 900])
+        again = sub_check(small, og_s, [300, 700], sub)
+        assert np.array_equal(first[0], again[0])
+        for n in (1, 7, 64):
+            say(f"{n} sources per wave")
+            _lanes(n)
+            check_simplest(small, og_s, [300, 700])
+            sub_check(big, og_b, [400, 900], np.sort(rng.choice(big.node_count(), 150, replace=False)).tolist())
+    finally:
+        _lanes(0)
