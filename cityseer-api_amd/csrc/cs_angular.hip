@@ -58,8 +58,7 @@ struct AngArgs {
   size_t lane_stride;  // bytes of the lane-major maps per lane
   size_t wave_stride;  // bytes of the interleaved arrays per wave
   uint32_t scap, ncap, hcap, maxp, tcap;  // per-lane capacities (tcap: thresholds of the delta/seed blocks)
-  size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_orte, o_delta, o_seed, o_node, o_brt, o_bsec,
-      o_heap;
+  size_t o_srec, o_preds, o_order, o_orte, o_delta, o_seed, o_nrec, o_heap;
   char *wave_ws;
   uint32_t per_wave;    // lanes of a wave that take sources (the rest take padding items)
   uint32_t epoch_base;  // item it searches under epoch epoch_base + it + 1
@@ -84,6 +83,27 @@ struct AngArgs {
 template <typename T>
 __device__ __forceinline__ T *il(char *base, size_t off, uint32_t i, uint32_t lane, size_t count) {
   return reinterpret_cast<T *>(base + off) + (size_t)lane * count + i;
+}
+
+// A search state's fields live together in one 32-byte record, a node's in
+// one 16-byte record, so that the fields a step reads or writes for a slot
+// share one cache line (records never straddle one).
+constexpr uint32_t kSRec = 32, kNRec = 16;
+constexpr uint32_t kS_route = 0;   // f32 route metric
+constexpr uint32_t kS_secs = 4;    // f32 seconds
+constexpr uint32_t kS_state = 8;   // u32 state id, visited flag in bit 31
+constexpr uint32_t kS_np = 12;     // u32 predecessor count
+constexpr uint32_t kS_sigma = 16;  // f64 path count
+constexpr uint32_t kN_node = 0;    // u32 node id
+constexpr uint32_t kN_brt = 4;     // f32 best route over the node's states
+constexpr uint32_t kN_bsec = 8;    // f32 seconds of that best route
+template <typename T>
+__device__ __forceinline__ T *sr(char *base, size_t off, uint32_t i, uint32_t lane, size_t count, uint32_t f) {
+  return reinterpret_cast<T *>(base + off + ((size_t)lane * count + i) * kSRec + f);
+}
+template <typename T>
+__device__ __forceinline__ T *nr(char *base, size_t off, uint32_t i, uint32_t lane, size_t count, uint32_t f) {
+  return reinterpret_cast<T *>(base + off + ((size_t)lane * count + i) * kNRec + f);
 }
 
 struct LaneWs {
@@ -165,19 +185,19 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
   Heap h{w, a.o_heap, lane, 0, a.hcap};
   // node slot of the source
   L.nkey[src] = ((uint64_t)epoch << 32) | 0u;
-  *il<uint32_t>(w, a.o_node, 0, lane, a.ncap) = src;
-  *il<float>(w, a.o_brt, 0, lane, a.ncap) = 0.0f;
-  *il<float>(w, a.o_bsec, 0, lane, a.ncap) = 0.0f;
+  *nr<uint32_t>(w, a.o_nrec, 0, lane, a.ncap, kN_node) = src;
+  *nr<float>(w, a.o_nrec, 0, lane, a.ncap, kN_brt) = 0.0f;
+  *nr<float>(w, a.o_nrec, 0, lane, a.ncap, kN_bsec) = 0.0f;
   n_nodes = 1;
   for (uint32_t slot = 0; slot < 2; ++slot) {
     const uint32_t s = 2 * src + slot;
     const uint32_t ss = n_states++;
     L.skey[s] = ((uint64_t)epoch << 32) | ss;
-    *il<float>(w, a.o_route, ss, lane, a.scap) = 0.0f;
-    *il<float>(w, a.o_secs, ss, lane, a.scap) = 0.0f;
-    *il<double>(w, a.o_sigma, ss, lane, a.scap) = 1.0;
-    *il<uint32_t>(w, a.o_state, ss, lane, a.scap) = s;  // visited flag in bit 31
-    *il<uint32_t>(w, a.o_np, ss, lane, a.scap) = 0;
+    *sr<float>(w, a.o_srec, ss, lane, a.scap, kS_route) = 0.0f;
+    *sr<float>(w, a.o_srec, ss, lane, a.scap, kS_secs) = 0.0f;
+    *sr<double>(w, a.o_srec, ss, lane, a.scap, kS_sigma) = 1.0;
+    *sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_state) = s;  // visited flag in bit 31
+    *sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_np) = 0;
     h.push(0.0f, 0.0f, s, ss);
   }
   while (h.n > 0) {
@@ -186,11 +206,12 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
     const uint32_t uss = top.w;  // the state's slot, carried by the heap entry
     const uint32_t u = us >> 1, entry = us & 1u;
     // the state's fields and its out-row bounds are independent loads
-    uint32_t *ust = il<uint32_t>(w, a.o_state, uss, lane, a.scap);
-    const uint32_t ustate = *ust;
-    const float u_route = *il<float>(w, a.o_route, uss, lane, a.scap);
-    const float u_secs = *il<float>(w, a.o_secs, uss, lane, a.scap);
-    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane, a.scap);
+    uint32_t *ust = sr<uint32_t>(w, a.o_srec, uss, lane, a.scap, kS_state);
+    const uint4 uhd = *sr<uint4>(w, a.o_srec, uss, lane, a.scap, 0);  // route, secs, state, np
+    const uint32_t ustate = uhd.z;
+    const float u_route = __uint_as_float(uhd.x);
+    const float u_secs = __uint_as_float(uhd.y);
+    const double u_sigma = *sr<double>(w, a.o_srec, uss, lane, a.scap, kS_sigma);
     const uint32_t j0 = jb.off[u], j1 = jb.off[u + 1];
     if (ustate & 0x80000000u) continue;
     *ust = us | 0x80000000u;
@@ -228,12 +249,14 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
         flv[q] = 0u;
         rtv[q] = brv[q] = bsv[q] = INFINITY;
         if (ok[q] && (uint32_t)(skv[q] >> 32) == epoch) {
-          flv[q] = *il<uint32_t>(w, a.o_state, (uint32_t)skv[q], lane, a.scap);
-          rtv[q] = *il<float>(w, a.o_route, (uint32_t)skv[q], lane, a.scap);
+          const uint4 r = *sr<uint4>(w, a.o_srec, (uint32_t)skv[q], lane, a.scap, 0);
+          flv[q] = r.z;
+          rtv[q] = __uint_as_float(r.x);
         }
         if (ok[q] && (uint32_t)(nkv[q] >> 32) == epoch) {
-          brv[q] = *il<float>(w, a.o_brt, (uint32_t)nkv[q], lane, a.ncap);
-          bsv[q] = *il<float>(w, a.o_bsec, (uint32_t)nkv[q], lane, a.ncap);
+          const uint4 r = *nr<uint4>(w, a.o_nrec, (uint32_t)nkv[q], lane, a.ncap, 0);
+          brv[q] = __uint_as_float(r.y);
+          bsv[q] = __uint_as_float(r.z);
         }
       }
 #pragma unroll
@@ -253,8 +276,8 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
         if (dup_s) {
           sk = L.skey[vs];
           if ((uint32_t)(sk >> 32) == epoch) {
-            fl = *il<uint32_t>(w, a.o_state, (uint32_t)sk, lane, a.scap);
-            rt = *il<float>(w, a.o_route, (uint32_t)sk, lane, a.scap);
+            fl = *sr<uint32_t>(w, a.o_srec, (uint32_t)sk, lane, a.scap, kS_state);
+            rt = *sr<float>(w, a.o_srec, (uint32_t)sk, lane, a.scap, kS_route);
           }
         }
         uint32_t vss;
@@ -270,27 +293,25 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
           }
           vss = n_states++;
           L.skey[vs] = ((uint64_t)epoch << 32) | vss;
-          *il<float>(w, a.o_route, vss, lane, a.scap) = INFINITY;
-          *il<float>(w, a.o_secs, vss, lane, a.scap) = INFINITY;
-          *il<double>(w, a.o_sigma, vss, lane, a.scap) = 0.0;
-          *il<uint32_t>(w, a.o_state, vss, lane, a.scap) = vs;
-          *il<uint32_t>(w, a.o_np, vss, lane, a.scap) = 0;
+          *sr<uint4>(w, a.o_srec, vss, lane, a.scap, 0) =
+              make_uint4(__float_as_uint(INFINITY), __float_as_uint(INFINITY), vs, 0u);
+          *sr<double>(w, a.o_srec, vss, lane, a.scap, kS_sigma) = 0.0;
           cur = INFINITY;
         }
         float cr = u_route + __uint_as_float(e[q].z);
         cr = cr + __uint_as_float(e[q].w);
-        uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane, a.scap);
+        uint32_t *np = sr<uint32_t>(w, a.o_srec, vss, lane, a.scap, kS_np);
         uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)lane * a.scap + vss) * a.maxp;
         if (cr < cur) {
-          double *sg = il<double>(w, a.o_sigma, vss, lane, a.scap);
+          double *sg = sr<double>(w, a.o_srec, vss, lane, a.scap, kS_sigma);
           if (cr < cur * one_m) {
             *np = 0;
             *sg = u_sigma;
           } else {
             *sg = *sg + u_sigma;
           }
-          *il<float>(w, a.o_route, vss, lane, a.scap) = cr;
-          *il<float>(w, a.o_secs, vss, lane, a.scap) = cs;
+          *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_route) = cr;
+          *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_secs) = cs;
           const uint32_t k = *np;
           if (k >= a.maxp) {
             fail |= kFailPreds;
@@ -309,7 +330,7 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
           for (uint32_t r = 0; r < k; ++r)
             if (preds[r] == uss) has = true;
           if (!has) {
-            float *vsec = il<float>(w, a.o_secs, vss, lane, a.scap);
+            float *vsec = sr<float>(w, a.o_srec, vss, lane, a.scap, kS_secs);
             if (cs < *vsec) *vsec = cs;
             if (k >= a.maxp) {
               fail |= kFailPreds;
@@ -317,7 +338,7 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
             }
             preds[k] = uss;
             *np = k + 1;
-            double *sg = il<double>(w, a.o_sigma, vss, lane, a.scap);
+            double *sg = sr<double>(w, a.o_srec, vss, lane, a.scap, kS_sigma);
             *sg = *sg + u_sigma;
           }
         }
@@ -327,8 +348,8 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
         if (dup_n) {
           nk = L.nkey[v];
           if ((uint32_t)(nk >> 32) == epoch) {
-            best = *il<float>(w, a.o_brt, (uint32_t)nk, lane, a.ncap);
-            bsec = *il<float>(w, a.o_bsec, (uint32_t)nk, lane, a.ncap);
+            best = *nr<float>(w, a.o_nrec, (uint32_t)nk, lane, a.ncap, kN_brt);
+            bsec = *nr<float>(w, a.o_nrec, (uint32_t)nk, lane, a.ncap, kN_bsec);
           }
         }
         uint32_t nsl;
@@ -341,18 +362,18 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
           }
           nsl = n_nodes++;
           L.nkey[v] = ((uint64_t)epoch << 32) | nsl;
-          *il<uint32_t>(w, a.o_node, nsl, lane, a.ncap) = v;
+          *nr<uint32_t>(w, a.o_nrec, nsl, lane, a.ncap, kN_node) = v;
           best = bsec = INFINITY;
         }
         if (cr < best * one_m) {
-          *il<float>(w, a.o_brt, nsl, lane, a.ncap) = cr;
-          *il<float>(w, a.o_bsec, nsl, lane, a.ncap) = cs;
+          *nr<float>(w, a.o_nrec, nsl, lane, a.ncap, kN_brt) = cr;
+          *nr<float>(w, a.o_nrec, nsl, lane, a.ncap, kN_bsec) = cs;
         } else if (cr <= best * one_p) {
-          *il<float>(w, a.o_bsec, nsl, lane, a.ncap) = fminf(bsec, cs);
+          *nr<float>(w, a.o_nrec, nsl, lane, a.ncap, kN_bsec) = fminf(bsec, cs);
         } else if (!((uint32_t)(nk >> 32) == epoch)) {
           // a new node slot keeps its initial infinite best
-          *il<float>(w, a.o_brt, nsl, lane, a.ncap) = INFINITY;
-          *il<float>(w, a.o_bsec, nsl, lane, a.ncap) = INFINITY;
+          *nr<float>(w, a.o_nrec, nsl, lane, a.ncap, kN_brt) = INFINITY;
+          *nr<float>(w, a.o_nrec, nsl, lane, a.ncap, kN_bsec) = INFINITY;
         }
       }
     }
@@ -372,19 +393,19 @@ __device__ bool angular_phase2(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
   for (uint32_t q = 0; q < n_order; ++q) {
     const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane, a.scap);
     *il<double>(w, a.o_delta, ss, lane, (size_t)a.scap * a.tcap) = (double)q;
-    *il<uint32_t>(w, a.o_np, ss, lane, a.scap) = 0;
-    *il<double>(w, a.o_sigma, ss, lane, a.scap) = 0.0;
+    *sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_np) = 0;
+    *sr<double>(w, a.o_srec, ss, lane, a.scap, kS_sigma) = 0.0;
   }
   // the source's two states are the first two slots
-  *il<double>(w, a.o_sigma, 0, lane, a.scap) = 1.0;
-  *il<double>(w, a.o_sigma, 1, lane, a.scap) = 1.0;
+  *sr<double>(w, a.o_srec, 0, lane, a.scap, kS_sigma) = 1.0;
+  *sr<double>(w, a.o_srec, 1, lane, a.scap, kS_sigma) = 1.0;
   const float band = 1.0f + jb.tolerance;
   for (uint32_t pos = 0; pos < n_order; ++pos) {
     const uint32_t uss = *il<uint32_t>(w, a.o_order, pos, lane, a.scap);
-    const uint32_t us = *il<uint32_t>(w, a.o_state, uss, lane, a.scap) & 0x7fffffffu;
+    const uint32_t us = *sr<uint32_t>(w, a.o_srec, uss, lane, a.scap, kS_state) & 0x7fffffffu;
     const uint32_t u = us >> 1, entry = us & 1u;
-    const float u_route = *il<float>(w, a.o_route, uss, lane, a.scap);
-    const double u_sigma = *il<double>(w, a.o_sigma, uss, lane, a.scap);
+    const float u_route = *sr<float>(w, a.o_srec, uss, lane, a.scap, kS_route);
+    const double u_sigma = *sr<double>(w, a.o_srec, uss, lane, a.scap, kS_sigma);
     for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
       const uint4 e = jb.adj[j];
       if (((e.x >> 30) & 1u) != 1u - entry) continue;
@@ -393,12 +414,12 @@ __device__ bool angular_phase2(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
       const uint64_t sk = L.skey[vs];
       if ((uint32_t)(sk >> 32) != epoch) continue;
       const uint32_t vss = (uint32_t)sk;
-      if (!(*il<uint32_t>(w, a.o_state, vss, lane, a.scap) & 0x80000000u)) continue;  // never settled
+      if (!(*sr<uint32_t>(w, a.o_srec, vss, lane, a.scap, kS_state) & 0x80000000u)) continue;  // never settled
       if (*il<double>(w, a.o_delta, vss, lane, (size_t)a.scap * a.tcap) <= (double)pos) continue;       // settled before u
       float cr = u_route + __uint_as_float(e.z);
       cr = cr + __uint_as_float(e.w);
-      if (!(cr <= *il<float>(w, a.o_route, vss, lane, a.scap) * band)) continue;
-      uint32_t *np = il<uint32_t>(w, a.o_np, vss, lane, a.scap);
+      if (!(cr <= *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_route) * band)) continue;
+      uint32_t *np = sr<uint32_t>(w, a.o_srec, vss, lane, a.scap, kS_np);
       uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)lane * a.scap + vss) * a.maxp;
       const uint32_t k = *np;
       bool has = false;
@@ -411,7 +432,7 @@ __device__ bool angular_phase2(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
       }
       preds[k] = uss;
       *np = k + 1;
-      double *sg = il<double>(w, a.o_sigma, vss, lane, a.scap);
+      double *sg = sr<double>(w, a.o_srec, vss, lane, a.scap, kS_sigma);
       *sg = *sg + u_sigma;
     }
   }
@@ -462,10 +483,10 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     const uint32_t T = jb.T;
     // ---- closeness at targets (centrality.rs:1995-2033)
     for (uint32_t k = 1; k < n_nodes; ++k) {
-      const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane, a.ncap);
+      const uint32_t v = *nr<uint32_t>(w, a.o_nrec, k, lane, a.ncap, kN_node);
       if (v == src) continue;
-      const float bsd = *il<float>(w, a.o_brt, k, lane, a.ncap);
-      const float bs = *il<float>(w, a.o_bsec, k, lane, a.ncap);
+      const float bsd = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_brt);
+      const float bs = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_bsec);
       if (!isfinite(bsd) || !isfinite(bs)) continue;
       double *accv = jb.acc + v;
       const size_t NN = jb.N;
@@ -497,10 +518,10 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
       }
       // seeds: best terminal states split by sigma (best_angular_target_states)
       for (uint32_t k = 1; k < n_nodes; ++k) {
-        const uint32_t v = *il<uint32_t>(w, a.o_node, k, lane, a.ncap);
+        const uint32_t v = *nr<uint32_t>(w, a.o_nrec, k, lane, a.ncap, kN_node);
         if (v == src) continue;
-        const float br = *il<float>(w, a.o_brt, k, lane, a.ncap);
-        const float bs = *il<float>(w, a.o_bsec, k, lane, a.ncap);
+        const float br = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_brt);
+        const float bs = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_bsec);
         if (!isfinite(br) || !isfinite(bs)) continue;
         uint32_t cand[2];
         uint32_t ncand = 0;
@@ -518,16 +539,16 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           double total = 0.0;
           for (uint32_t q = 0; q < ncand; ++q) {
             const uint32_t ss = cand[q];
-            const double sg = *il<double>(w, a.o_sigma, ss, lane, a.scap);
-            if (sg == 0.0 || *il<float>(w, a.o_secs, ss, lane, a.scap) > sthr) continue;
-            if (*il<float>(w, a.o_route, ss, lane, a.scap) <= br * (1.0f + jb.tolerance)) best[nbst++] = ss;
+            const double sg = *sr<double>(w, a.o_srec, ss, lane, a.scap, kS_sigma);
+            if (sg == 0.0 || *sr<float>(w, a.o_srec, ss, lane, a.scap, kS_secs) > sthr) continue;
+            if (*sr<float>(w, a.o_srec, ss, lane, a.scap, kS_route) <= br * (1.0f + jb.tolerance)) best[nbst++] = ss;
           }
           if (nbst == 0) continue;
-          for (uint32_t q = 0; q < nbst; ++q) total += *il<double>(w, a.o_sigma, best[q], lane, a.scap);
+          for (uint32_t q = 0; q < nbst; ++q) total += *sr<double>(w, a.o_srec, best[q], lane, a.scap, kS_sigma);
           if (total == 0.0) continue;
           for (uint32_t q = 0; q < nbst; ++q) {
             double *sd = il<double>(w, a.o_seed, t * sc + best[q], lane, (size_t)a.scap * a.tcap);
-            *sd = *sd + pc * (*il<double>(w, a.o_sigma, best[q], lane, a.scap) / total);
+            *sd = *sd + pc * (*sr<double>(w, a.o_srec, best[q], lane, a.scap, kS_sigma) / total);
           }
         }
       }
@@ -540,15 +561,15 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
         while (lo > 0 && *il<float>(w, a.o_orte, lo - 1, lane, a.scap) == r_hi) --lo;
         for (int q = lo; q <= hi; ++q) {
           const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane, a.scap);
-          const float secs_ss = *il<float>(w, a.o_secs, ss, lane, a.scap);
+          const float secs_ss = *sr<float>(w, a.o_srec, ss, lane, a.scap, kS_secs);
           bool any = false;
           for (uint32_t t = 0; t < T; ++t) any = any || secs_ss <= jb.sec_thr[t];
           if (!any) continue;
-          const double sw = *il<double>(w, a.o_sigma, ss, lane, a.scap);
+          const double sw = *sr<double>(w, a.o_srec, ss, lane, a.scap, kS_sigma);
           if (sw == 0.0) continue;
-          const uint32_t k = *il<uint32_t>(w, a.o_np, ss, lane, a.scap);
+          const uint32_t k = *sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_np);
           const uint32_t *preds = reinterpret_cast<const uint32_t *>(w + a.o_preds) + ((size_t)lane * a.scap + ss) * a.maxp;
-          const uint32_t node = (*il<uint32_t>(w, a.o_state, ss, lane, a.scap) & 0x7fffffffu) >> 1;
+          const uint32_t node = (*sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_state) & 0x7fffffffu) >> 1;
           for (uint32_t t = 0; t < T; ++t) {
             if (!(secs_ss <= jb.sec_thr[t])) continue;
             const double seed = *il<double>(w, a.o_seed, t * sc + ss, lane, (size_t)a.scap * a.tcap);
@@ -556,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
             if (dep == 0.0) continue;
             for (uint32_t p = 0; p < k; ++p) {
               const uint32_t ps = preds[p];
-              const double sv = *il<double>(w, a.o_sigma, ps, lane, a.scap);
+              const double sv = *sr<double>(w, a.o_srec, ps, lane, a.scap, kS_sigma);
               if (sv == 0.0) continue;
               double *dp = il<double>(w, a.o_delta, t * sc + ps, lane, (size_t)a.scap * a.tcap);
               *dp = *dp + (sv / sw) * dep;
@@ -589,7 +610,7 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
   char *w = L.w;
   const uint32_t epoch = a.epochs[0] + 1;
   a.epochs[0] = epoch;
-  // state slots: route metric (o_route), agg seconds (o_secs), simplest
+  // state slots: route metric (kS_route), agg seconds (kS_secs), simplest
   // distance (o_delta, an f32 value held exactly in an f64 slot)
   uint32_t n_states = 0, no = 0;
   Heap h{w, a.o_heap, 0, 0, a.hcap};
@@ -601,21 +622,21 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
     const uint32_t s = 2 * src + slot;
     const uint32_t ss = n_states++;
     L.skey[s] = ((uint64_t)epoch << 32) | ss;
-    *il<float>(w, a.o_route, ss, 0, a.scap) = 0.0f;            // route metric
-    *il<float>(w, a.o_secs, ss, 0, a.scap) = 0.0f;             // agg seconds
+    *sr<float>(w, a.o_srec, ss, 0, a.scap, kS_route) = 0.0f;            // route metric
+    *sr<float>(w, a.o_srec, ss, 0, a.scap, kS_secs) = 0.0f;             // agg seconds
     *il<double>(w, a.o_delta, ss, 0, (size_t)a.scap * a.tcap) = 0.0;            // simpl dist (f32 stored in f64 slot)
-    *il<uint32_t>(w, a.o_state, ss, 0, a.scap) = s;
+    *sr<uint32_t>(w, a.o_srec, ss, 0, a.scap, kS_state) = s;
     h.push(0.0f, 0.0f, s);
   }
   while (h.n > 0) {
     const uint4 top = h.pop();
     const uint32_t us = top.z;
     const uint32_t uss = (uint32_t)L.skey[us];
-    uint32_t *ust = il<uint32_t>(w, a.o_state, uss, 0, a.scap);
+    uint32_t *ust = sr<uint32_t>(w, a.o_srec, uss, 0, a.scap, kS_state);
     if (*ust & 0x80000000u) continue;
     *ust = us | 0x80000000u;
     const uint32_t u = us >> 1, entry = us & 1u;
-    const float u_secs = *il<float>(w, a.o_secs, uss, 0, a.scap);
+    const float u_secs = *sr<float>(w, a.o_srec, uss, 0, a.scap, kS_secs);
     const float u_simpl = (float)*il<double>(w, a.o_delta, uss, 0, (size_t)a.scap * a.tcap);
     for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
       const uint4 e = jb.adj[j];
@@ -631,8 +652,8 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
       float vmetric, vsecs;
       if ((uint32_t)(sk >> 32) == epoch) {
         vss = (uint32_t)sk;
-        vmetric = *il<float>(w, a.o_route, vss, 0, a.scap);
-        vsecs = *il<float>(w, a.o_secs, vss, 0, a.scap);
+        vmetric = *sr<float>(w, a.o_srec, vss, 0, a.scap, kS_route);
+        vsecs = *sr<float>(w, a.o_srec, vss, 0, a.scap, kS_secs);
       } else {
         vss = 0xffffffffu;
         vmetric = INFINITY;
@@ -648,11 +669,11 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
           }
           vss = n_states++;
           L.skey[vs] = ((uint64_t)epoch << 32) | vss;
-          *il<uint32_t>(w, a.o_state, vss, 0, a.scap) = vs;
+          *sr<uint32_t>(w, a.o_srec, vss, 0, a.scap, kS_state) = vs;
         }
-        *il<float>(w, a.o_route, vss, 0, a.scap) = cmetric;
+        *sr<float>(w, a.o_srec, vss, 0, a.scap, kS_route) = cmetric;
         *il<double>(w, a.o_delta, vss, 0, (size_t)a.scap * a.tcap) = (double)csimpl;
-        *il<float>(w, a.o_secs, vss, 0, a.scap) = cs;
+        *sr<float>(w, a.o_srec, vss, 0, a.scap, kS_secs) = cs;
         if (h.n >= a.hcap) {
           atomicOr(a.fail_flags, kFailHeap);
           return;
@@ -682,7 +703,7 @@ struct AngPlan {
   uint32_t scap, ncap, hcap, maxp, lanes, tcap;
   uint32_t per_wave = kWave;  // lane blocks per wave in the interleaved arrays
   size_t lane_stride, wave_stride;
-  size_t o_route, o_secs, o_sigma, o_state, o_np, o_preds, o_order, o_orte, o_delta, o_seed, o_node, o_brt, o_bsec, o_heap;
+  size_t o_srec, o_preds, o_order, o_orte, o_delta, o_seed, o_nrec, o_heap;
 };
 
 void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
@@ -693,19 +714,13 @@ void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
     field = o;
     o = align256(o + elem_bytes * count * p.per_wave);
   };
-  take(p.o_route, 4, p.scap);
-  take(p.o_secs, 4, p.scap);
-  take(p.o_sigma, 8, p.scap);
-  take(p.o_state, 4, p.scap);
-  take(p.o_np, 4, p.scap);
+  take(p.o_srec, kSRec, p.scap);
   take(p.o_preds, 4, (size_t)p.scap * p.maxp);
   take(p.o_order, 4, p.scap);
   take(p.o_orte, 4, p.scap);
   take(p.o_delta, 8, (size_t)p.scap * std::max<uint32_t>(T, 1));
   take(p.o_seed, 8, (size_t)p.scap * std::max<uint32_t>(T, 1));
-  take(p.o_node, 4, p.ncap);
-  take(p.o_brt, 4, p.ncap);
-  take(p.o_bsec, 4, p.ncap);
+  take(p.o_nrec, kNRec, p.ncap);
   take(p.o_heap, 16, p.hcap);
   p.wave_stride = o;
 }
@@ -941,19 +956,13 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     a.hcap = p.hcap;
     a.maxp = p.maxp;
     a.tcap = p.tcap;
-    a.o_route = p.o_route;
-    a.o_secs = p.o_secs;
-    a.o_sigma = p.o_sigma;
-    a.o_state = p.o_state;
-    a.o_np = p.o_np;
+    a.o_srec = p.o_srec;
     a.o_preds = p.o_preds;
     a.o_order = p.o_order;
     a.o_orte = p.o_orte;
     a.o_delta = p.o_delta;
     a.o_seed = p.o_seed;
-    a.o_node = p.o_node;
-    a.o_brt = p.o_brt;
-    a.o_bsec = p.o_bsec;
+    a.o_nrec = p.o_nrec;
     a.o_heap = p.o_heap;
     a.per_wave = p.per_wave;
     a.epoch_base = epoch_base;
@@ -1058,19 +1067,13 @@ void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, fl
   a.hcap = p.hcap;
   a.maxp = p.maxp;
   a.tcap = p.tcap;
-  a.o_route = p.o_route;
-  a.o_secs = p.o_secs;
-  a.o_sigma = p.o_sigma;
-  a.o_state = p.o_state;
-  a.o_np = p.o_np;
+  a.o_srec = p.o_srec;
   a.o_preds = p.o_preds;
   a.o_order = p.o_order;
   a.o_orte = p.o_orte;
   a.o_delta = p.o_delta;
   a.o_seed = p.o_seed;
-  a.o_node = p.o_node;
-  a.o_brt = p.o_brt;
-  a.o_bsec = p.o_bsec;
+  a.o_nrec = p.o_nrec;
   a.o_heap = p.o_heap;
   a.epochs = epochs;
   a.fail_flags = ctl + 2;
