@@ -199,6 +199,15 @@ class NetworkStructure:
             return int(_native.load().cs_graph_progress(self._handle))
         return self._progress
 
+    def upload(self, device: int = 0, speed_m_s: float | None = None) -> None:
+        """Stage the graph on HIP device ``device`` now (adjacency, node order and, for
+        ``speed_m_s``, edge travel seconds; default the walking speed) instead of on
+        its first computation there. Not in the reference, which has no device."""
+        from ..config import SPEED_M_S
+
+        speed = SPEED_M_S if speed_m_s is None else float(speed_m_s)
+        _native.check(_native.load().cs_graph_upload(self._native_handle(), int(device), speed))
+
     @property
     def is_dual(self) -> bool:
         return self._is_dual

@@ -703,6 +703,32 @@ int cs_graph_set_xy(cs_graph *h, const double *x, const double *y) {
   });
 }
 
+int cs_graph_upload(cs_graph *h, int device, float speed_m_s) {
+  return guarded([&] {
+    if (!h) fail(CS_EINVAL, "graph must not be NULL");
+    (void)current_device();  // raises without a GPU
+    int count = 0;
+    hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    if (device < 0 || device >= count)
+      fail(CS_EINVAL, "device " + std::to_string(device) + " does not exist (" + std::to_string(count) + " visible)");
+    if (speed_m_s != 0.0f && !(std::isfinite(speed_m_s) && speed_m_s > 0.0f))
+      fail(CS_EINVAL, "speed_m_s must be finite and positive (or 0: adjacency only)");
+    int prev = 0;
+    hip_check(hipGetDevice(&prev), "hipGetDevice");
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    struct Restore {
+      int d;
+      ~Restore() { (void)hipSetDevice(d); }
+    } restore{prev};
+    Graph &g = h->g;
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.order_frozen = true;
+    DevGraph &dg = device_graph(g, device);
+    if (speed_m_s > 0.0f) (void)device_edges(g, dg, speed_m_s);
+    hip_check(hipDeviceSynchronize(), "upload");
+  });
+}
+
 int cs_graph_node_order(const cs_graph *h, uint32_t *order) {
   return guarded([&] {
     if (!h || !order) fail(CS_EINVAL, "graph and order must not be NULL");

@@ -95,6 +95,14 @@ int cs_graph_set_xy(cs_graph *g, const double *x, const double *y);
  * counterpart. */
 int cs_graph_node_order(const cs_graph *g, uint32_t *order);
 
+/* Uploads the graph's device arrays (adjacency, node order) to HIP device
+ * `device` and, when speed_m_s > 0, its edge travel seconds for that speed,
+ * so that the first computation there pays no upload (computations otherwise
+ * upload on first use; a graph keeps one copy per device until destroyed).
+ * Freezes the device node order (cs_graph_set_xy fails afterwards). No
+ * reference counterpart: the reference has no device. */
+int cs_graph_upload(cs_graph *g, int device, float speed_m_s);
+
 /* Number of nodes/edges; maximum out/in degree. */
 int cs_graph_info(const cs_graph *g, uint32_t *n_nodes, uint32_t *n_edges, uint32_t *max_out_deg,
                   uint32_t *max_in_deg);

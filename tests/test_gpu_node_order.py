@@ -243,3 +243,21 @@ def test_device_entry_point_takes_original_order(jittered):
             assert_close(got, want, f"device entry point, {listing} listing")
     finally:
         hip.free()
+
+
+def test_upload_stages_the_graph_and_freezes_its_order():
+    lat = shuffled(mock.perturbed_lattice(14, seed=89), seed=90)
+    a, b = pair(lat)
+    lib = _native.load()
+    say("upload, then compute; coordinates after the upload are refused")
+    a.upload(0)
+    with pytest.raises(ValueError, match="does not exist"):
+        a.upload(99)
+    x = np.ascontiguousarray(lat.x)
+    y = np.ascontiguousarray(lat.y)
+    rc = lib.cs_graph_set_xy(a._native_handle(), _native.ptr(x, ctypes.c_double), _native.ptr(y, ctypes.c_double))
+    assert rc == _native.CS_EINVAL
+    ga, gb = a.centrality_shortest(distances=[500, 900]), b.centrality_shortest(distances=[500, 900])
+    sa, sb = stack(ga, SHORTEST_KEYS, [500, 900]), stack(gb, SHORTEST_KEYS, [500, 900])
+    assert np.array_equal(sa[0], sb[0])
+    assert_close(sa, sb, "uploaded graph vs insertion order")

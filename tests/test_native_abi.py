@@ -182,6 +182,8 @@ def test_compute_fails_loudly_without_gpu(diamond_graph):
         ns.centrality_shortest(distances=[150])
     with pytest.raises(RuntimeError, match="HIP device"):
         ns.segment_centrality(distances=[150])
+    with pytest.raises(RuntimeError, match="HIP device"):
+        ns.upload()
 
 
 def test_dual_builder_keys_and_slots(diamond_graph):
