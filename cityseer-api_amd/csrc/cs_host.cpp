@@ -288,7 +288,7 @@ DevEdges &device_edges(Graph &g, DevGraph &dg, float speed) {
       feid[j] = e;
     }
   }
-  d->min_es = std::isfinite(min_es) ? min_es : 0.0f;
+  d->min_es = min_es;  // infinite when no edge joins two distinct nodes
   d->max_es = max_es;
   d->rev_adj = dev_upload(rev);
   d->fwd_adj = dev_upload(fwd);

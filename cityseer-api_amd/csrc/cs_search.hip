@@ -78,7 +78,7 @@ struct Job {
   float maxsec, speed, inv_delta;
   float tol;  // tie tolerance; above the internal band, Phase 2 rebuilds predecessors
   uint32_t nbk, maxb;
-  uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach, 8 OD-weighted seeds
+  uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach, 8 OD-weighted seeds, 16 fold after the search
   uint32_t n_sources;
   // OD-weighted betweenness: source position i's destinations and trip weights
   // are od_dest / od_w[od_off[i] .. od_off[i + 1])
@@ -282,9 +282,11 @@ __device__ __forceinline__ uint4 nrec(const WaveWs &w, uint32_t v) {
 __device__ __forceinline__ uint64_t lo64(uint4 r) { return ((uint64_t)r.y << 32) | r.x; }
 __device__ __forceinline__ uint64_t hi64(uint4 r) { return ((uint64_t)r.w << 32) | r.z; }
 
+// Candidates are out-row nodes settled before v: in an earlier bucket than kb,
+// or, with by_key, anywhere before v in (seconds, original index) order.
 __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint32_t v, float secs_v,
                                    uint32_t kb, uint32_t r0, uint32_t r1, double thr_far, uint32_t n_near,
-                                   Fold &f) {
+                                   Fold &f, bool by_key = false) {
   f.best = INFINITY;
   f.sigma = 0.0;
   f.pmask = 0;
@@ -302,7 +304,7 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
       const uint64_t ku = ld64(nkey(w, u));
       if (!key_current(ku, tag)) continue;
       const float su = key_secs(ku);
-      if (bucket_of(su, j.inv_delta) >= kb) continue;
+      if (by_key ? !key_less(j.orig, su, u, secs_v, v) : bucket_of(su, j.inv_delta) >= kb) continue;
       const float cs = su + __uint_as_float(e.y);
       if (cs > j.maxsec || (double)cs > thr_far) continue;
       const uint32_t jr = p - r0;
@@ -356,7 +358,8 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
   const float speed = j.speed;
   const bool cycles = (MODE == kBrandes) && (j.flags & 1u);
   // with Phase 2 ahead, the Phase 1 fold's predecessors would be discarded
-  const bool phase2 = kMulti && j.tol > kTieEps;
+  // Phase 2, or Phase 1 folded after the search (flag 16): no fold at settle time
+  const bool phase2 = kMulti && (j.tol > kTieEps || (j.flags & 16u));
   const bool fold = ((MODE != kBrandes) || (j.flags & 2u)) && !phase2;
   if (lane == 0) {
     st64(nkey(w, c.src), pack_key(c.tag, 0.0f));
@@ -808,6 +811,97 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
   return true;
 }
 
+// Phase 1 folded after the search (flag 16), for bucket widths that cannot
+// keep a node's tie band clear of its own bucket (thresholds long against the
+// shortest edge). The search settles every node at its exact seconds; here,
+// bucket by bucket, each node's predecessors and sigma come from the ordered
+// fold over all its out-row candidates settled before it (in seconds, original
+// index order), same-bucket ones included, once every such candidate has been
+// folded itself: in rounds, as Phase 2 does, whose numbers become the levels
+// back-propagation walks.
+__device__ bool phase1_fold_after(CJob &j, const WaveWs &w, const SrcCtx &c) {
+  const uint32_t lane = lane_id();
+  for (uint32_t b = 0; b < c.nb; ++b) {
+    const uint32_t s0 = uniform(ld32(&w.bnd[b])), s1 = uniform(ld32(&w.bnd[b + 1]));
+    // candidate masks over the out-row positions of each node
+    for (uint32_t s = s0 + lane; s < s1; s += kWave) {
+      const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+      const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
+      const uint32_t v = (uint32_t)lo;
+      const float secs_v = __uint_as_float((uint32_t)(lo >> 32));
+      uint32_t meta = (uint32_t)(hi >> 32) & 0x100u;  // keep the eligibility bit
+      uint64_t cm = 0;
+      if (s == 0) {
+        meta |= kP2Done;
+        std_(reinterpret_cast<double *>(npay(w, v)), 1.0);
+      } else {
+        const uint4 rw = j.rows[v];
+        const double thr_far = (double)secs_v * far_factor(rw.y - rw.x);
+        for (uint32_t p = rw.x; p < rw.y; ++p) {
+          const uint2 e = j.fwd_adj[p];
+          const uint32_t u = e.x & kNodeMask;
+          if (u == v) continue;
+          const uint64_t ku = ld64(nkey(w, u));
+          if (!key_current(ku, c.tag)) continue;
+          const float su = key_secs(ku);
+          if (!key_less(j.orig, su, u, secs_v, v)) continue;
+          const float cs = su + __uint_as_float(e.y);
+          if (cs > j.maxsec || (double)cs > thr_far) continue;
+          cm |= 1ull << (p - rw.x);
+        }
+      }
+      st64(&w.pm[s], cm);
+      st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)meta << 32) | (uint32_t)hi);
+    }
+    wave_fence();
+    for (uint32_t round = 0;; ++round) {
+      uint32_t pending = 0, bad = 0;
+      for (uint32_t s = s0 + lane; s < s1; s += kWave) {
+        const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+        const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
+        const uint32_t meta = (uint32_t)(hi >> 32);
+        if (meta & kP2Done) continue;
+        const uint32_t v = (uint32_t)lo;
+        const float secs_v = __uint_as_float((uint32_t)(lo >> 32));
+        const uint64_t cm = ld64(&w.pm[s]);
+        const uint4 rw = j.rows[v];
+        bool ready = true;
+        for (uint64_t mm = cm; mm; mm &= mm - 1) {
+          const uint32_t u = j.fwd_adj[rw.x + __builtin_ctzll(mm)].x & kNodeMask;
+          const uint32_t su = key_slot(ld64(nkey(w, u)));
+          const uint32_t mu = (uint32_t)(ld64(&w.rec[2 * (size_t)su + 1]) >> 32);
+          // same-bucket candidates must have been folded in an earlier round
+          if (!(mu & kP2Done) || (su >= s0 && su < s1 && (mu >> 16) >= round)) ready = false;
+        }
+        if (!ready) {
+          ++pending;
+          continue;
+        }
+        Fold f;
+        const uint32_t n_near = (uint32_t)__popcll(cm);
+        const double thr_far = (double)secs_v * far_factor(rw.y - rw.x);
+        if (n_near == 0 || !brandes_fold_exact(j, w, c.tag, v, secs_v, 0u, rw.x, rw.y, thr_far, n_near, f, true)) {
+          ++bad;
+          continue;
+        }
+        const uint32_t np = (uint32_t)__popcll(f.pmask);
+        const uint32_t u0 = j.fwd_adj[rw.x + __builtin_ctzll(f.pmask)].x & kNodeMask;
+        const uint32_t pslot0 = key_slot(ld64(nkey(w, u0)));
+        st64(&w.pm[s], f.pmask);
+        std_(reinterpret_cast<double *>(npay(w, v)), f.sigma);
+        st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np | (round << 16)) << 32) | pslot0);
+      }
+      if (uniform(wave_sum(bad))) return false;
+      pending = uniform(wave_sum(pending));
+      if (pending == 0 && lane_id() == 0) st32(&w.lvl[b], round);
+      wave_fence();
+      if (pending == 0) break;
+      if (round > s1 - s0 || round >= 0xffffu) return false;
+    }
+  }
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // Results of one source (centrality.rs:1733-1780 closeness, 1783-1835
 // betweenness credit): every reached target but the source adds, per threshold
@@ -943,8 +1037,8 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
     for (int t = 0; t < NT; ++t) nband[t] = eband[t] = 0;
     bool ok = search_source<MODE, NT>(a, j, w, c, nband, eband, st_edges);
     if constexpr (MODE == kBrandes || MODE == kBrandesExport) {
-      if (ok && j.tol > kTieEps && (MODE == kBrandesExport || (j.flags & 2u))) {
-        ok = phase2_rebuild(j, w, c);
+      if (ok && (j.tol > kTieEps || (j.flags & 16u)) && (MODE == kBrandesExport || (j.flags & 2u))) {
+        ok = j.tol > kTieEps ? phase2_rebuild(j, w, c) : phase1_fold_after(j, w, c);
         if (!ok) c.fail |= kFailInternal;
       }
     }
@@ -989,7 +1083,7 @@ __global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
       if (j.flags & 2u) {
         // with Phase 2, a successor may share its predecessor's bucket: visit
         // each bucket level by level, highest first (levels are all 0 otherwise)
-        const bool p2 = j.tol > kTieEps;
+        const bool p2 = j.tol > kTieEps || (j.flags & 16u);
         uint32_t s1 = R;
         uint32_t s0 = uniform(ld32(&w.bnd[c.nb - 1]));
         for (int b = (int)c.nb - 1; b >= 0; --b) {
@@ -1333,17 +1427,23 @@ void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
 
 // Bucket width: below the smallest edge cost by the largest tie-band drift a
 // node's fold can accumulate (one 1e-4 step per candidate edge), so candidates
-// from the same bucket can never reach a receiver's tie band.
-float choose_delta(const Graph &g, const DevEdges &de, float maxsec) {
+// from the same bucket can never reach a receiver's tie band. When the
+// threshold is too long against the shortest edge for such a width to exist,
+// the width is 0.95 of the shortest edge (distances stay exact: no node
+// improves another of its own bucket) and exact_fold asks for the fold after
+// the search, which takes same-bucket candidates in settle order.
+float choose_delta(const Graph &g, const DevEdges &de, float maxsec, bool &exact_fold) {
+  exact_fold = false;
   const float min_es = de.min_es;
+  if (std::isinf(min_es)) return 2.0f * std::max(maxsec, 1.0f);  // no edge between distinct nodes: one bucket
   if (!(min_es > 0.0f)) fail(CS_EINVAL, "graph has zero-cost edges; the bucketed search needs positive edge costs");
   const double deg = (double)std::max(g.max_out_deg, g.max_in_deg);
   const double gap = (double)maxsec * 1.02e-4 * (deg + 3.0) + 1e-3;
   const double delta = (double)min_es - gap;
-  if (delta < 0.05 * (double)min_es)
-    fail(CS_EINVAL,
-         "edge costs are too small relative to the distance threshold for the bucketed search "
-         "(min edge seconds " + std::to_string(min_es) + ", max seconds " + std::to_string(maxsec) + ")");
+  if (delta < 0.05 * (double)min_es) {
+    exact_fold = true;
+    return 0.95f * min_es;
+  }
   return (float)delta;
 }
 
@@ -1494,7 +1594,8 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   const CallSpec &cs = js.cs;
   Job j;
   std::memset(&j, 0, sizeof(j));
-  const float delta = choose_delta(g, de, cs.maxsec) * js.delta_scale;
+  bool exact_fold = false;
+  const float delta = choose_delta(g, de, cs.maxsec, exact_fold) * js.delta_scale;
   j.rows = js.dg->rows;
   j.orig = js.dg->orig;
   j.rev_adj = de.rev_adj;
@@ -1516,7 +1617,8 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   j.nbk = jump + 1;
   if (j.nbk > 4096 - kWaveLds) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
   j.maxb = (uint32_t)((double)cs.maxsec * (double)j.inv_delta) + 4;
-  j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u);
+  j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u) |
+            (exact_fold ? 16u : 0u);
   j.od_off = js.d_od_off;
   j.od_dest = js.d_od_dest;
   j.od_w = js.d_od_w;
