@@ -1,0 +1,72 @@
+"""Street-network ingest through the native library (host only, no GPU).
+
+``edge_geometry`` derives the edge payload ``add_street_edge`` computes from a
+polyline (rust/src/graph.rs:764-857) for many edges in one call;
+``decompose`` splits polylines as ``tools.graphs.nx_decompose`` cuts edges
+(pysrc/cityseer/tools/graphs.py:1911-1961).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from collections.abc import Sequence
+
+import numpy as np
+
+from . import _native
+
+
+def _flatten(polylines: Sequence[Sequence[Sequence[float]]], dims: int) -> tuple[np.ndarray, np.ndarray]:
+    counts = np.fromiter((len(p) for p in polylines), dtype=np.int64, count=len(polylines))
+    off = np.zeros(len(polylines) + 1, dtype=np.uint32)
+    if len(polylines):
+        off[1:] = np.cumsum(counts)
+    pts = np.empty((int(off[-1]), dims), dtype=np.float64)
+    i = 0
+    for p in polylines:
+        for c in p:
+            pts[i, 0] = c[0]
+            pts[i, 1] = c[1]
+            if dims == 3:
+                pts[i, 2] = c[2] if len(c) > 2 else np.nan
+            i += 1
+    return off, pts
+
+
+def edge_geometry(polylines: Sequence[Sequence[Sequence[float]]]) -> tuple[np.ndarray, np.ndarray]:
+    """(length f32[m], angle_sum f32[m]) of each polyline, as ``add_street_edge`` stores them."""
+    m = len(polylines)
+    off, pts = _flatten(polylines, 2)
+    length = np.empty(m, dtype=np.float32)
+    angle = np.empty(m, dtype=np.float32)
+    p = _native.ptr
+    _native.check(_native.load().cs_edge_geometry(m, p(off, ctypes.c_uint32), p(pts, ctypes.c_double),
+                                                  p(length, ctypes.c_float), p(angle, ctypes.c_float), None, None))
+    return length, angle
+
+
+def decompose(polylines: Sequence[Sequence[Sequence[float]]], decompose_max: float) -> list[list[np.ndarray]]:
+    """Each polyline cut into ceil(length / decompose_max) equal pieces: per polyline, the list of
+    piece point arrays ([k][2], or [k][3] when every input point carries z)."""
+    dims = 3 if polylines and all(len(c) > 2 for pl in polylines for c in pl) else 2
+    off, pts = _flatten(polylines, dims)
+    lib = _native.load()
+    h = ctypes.c_void_p()
+    n_pieces, n_points = ctypes.c_uint32(), ctypes.c_uint32()
+    p = _native.ptr
+    _native.check(lib.cs_decompose(len(polylines), p(off, ctypes.c_uint32), p(pts, ctypes.c_double), dims,
+                                   float(decompose_max), ctypes.byref(h), ctypes.byref(n_pieces),
+                                   ctypes.byref(n_points)))
+    try:
+        cuts = np.empty(len(polylines), dtype=np.uint32)
+        poff = np.empty(n_pieces.value + 1, dtype=np.uint32)
+        out_pts = np.empty((n_points.value, dims), dtype=np.float64)
+        _native.check(lib.cs_pieces_copy(h, p(cuts, ctypes.c_uint32), p(poff, ctypes.c_uint32),
+                                         p(out_pts, ctypes.c_double)))
+    finally:
+        lib.cs_pieces_free(h)
+    result, k = [], 0
+    for c in cuts.tolist():
+        result.append([out_pts[poff[k + i]: poff[k + i + 1]] for i in range(c)])
+        k += c
+    return result

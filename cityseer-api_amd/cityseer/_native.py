@@ -119,6 +119,14 @@ _SIGNATURES = {
     "cs_graph_set_xy": (ctypes.c_int, [ctypes.c_void_p, _f64p, _f64p]),
     "cs_graph_node_order": (ctypes.c_int, [ctypes.c_void_p, _u32p]),
     "cs_graph_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_float]),
+    "cs_edge_geometry": (ctypes.c_int, [ctypes.c_uint32, _u32p, _f64p, _f32p, _f32p, _f32p, _f32p]),
+    "cs_decompose": (
+        ctypes.c_int,
+        [ctypes.c_uint32, _u32p, _f64p, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p),
+         _u32p, _u32p],
+    ),
+    "cs_pieces_copy": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _f64p]),
+    "cs_pieces_free": (None, [ctypes.c_void_p]),
     "cs_graph_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _u32p, _u32p]),
     "cs_graph_progress": (ctypes.c_uint64, [ctypes.c_void_p]),
     "cs_centrality_shortest": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CsParams), _f64p, _u32p]),

@@ -19,7 +19,7 @@ from typing import Any
 
 import numpy as np
 
-from .. import _native
+from .. import _ingest, _native
 from ..tools import util
 from . import centrality as _res
 
@@ -377,7 +377,7 @@ class NetworkStructure:
                 f"Street edge geometry (idx {edge_idx}) between nodes {start_nd_idx} and {end_nd_idx} "
                 f"must have at least 2 coordinates. Found {len(coords)}."
             )
-        length, angle = util.edge_geometry_payload(coords)
+        length, angle = (v[0] for v in _ingest.edge_geometry([coords]))
         imp = np.float32(1.0 if imp_factor is None else imp_factor)
         if not math.isfinite(float(imp)) or imp <= 0.0:
             raise ValueError(
@@ -434,6 +434,27 @@ class NetworkStructure:
             self._epk.extend(np.asarray(shared_primal_node_key, dtype=np.int32))
         self._release()
         return np.arange(start, start + m)
+
+    def add_street_edges_from_lines(
+        self,
+        start_nd_idx: np.ndarray,
+        end_nd_idx: np.ndarray,
+        edge_idx: np.ndarray | None,
+        lines: Sequence[Sequence[Sequence[float]]],
+        imp_factor: np.ndarray | None = None,
+        shared_primal_node_key: Sequence[str | None] | None = None,
+    ) -> np.ndarray:
+        """Array form of ``add_street_edge``: lengths and angle sums derived from the polylines
+        natively in one call (rust/src/graph.rs:764-857). ``shared_primal_node_key`` takes the
+        keys themselves (``None`` = none)."""
+        if len(lines) != len(start_nd_idx) or len(lines) != len(end_nd_idx):
+            raise ValueError("start_nd_idx, end_nd_idx and lines must have the same length")
+        length, angle = _ingest.edge_geometry(lines)
+        pk = None
+        if shared_primal_node_key is not None:
+            pk = np.fromiter((self._primal_key_id(k) for k in shared_primal_node_key), dtype=np.int32,
+                             count=len(lines))
+        return self.add_street_edges_bulk(start_nd_idx, end_nd_idx, edge_idx, length, angle, imp_factor, pk)
 
     def edge_references(self) -> list[tuple[int, int, int]]:
         return list(zip(self._esrc.array().tolist(), self._edst.array().tolist(), self._ekey.array().tolist(), strict=True))

@@ -12,8 +12,8 @@ import logging
 import math
 
 import networkx as nx
-import numpy as np
 
+from .. import _ingest
 from . import util
 from .util import LineString
 
@@ -48,8 +48,10 @@ def nx_simple_geoms(nx_multigraph: nx.MultiGraph) -> nx.MultiGraph:
 
 
 def _substring(coords: list[tuple[float, ...]], start: float, end: float) -> list[tuple[float, ...]]:
-    """Part of a polyline between two distances along it (shapely ``ops.substring``)."""
-    seglens = [math.hypot(b[0] - a[0], b[1] - a[1]) for a, b in zip(coords[:-1], coords[1:], strict=True)]
+    """Part of a polyline between two distances along it, for start < end (shapely
+    ``ops.substring`` over GEOS: segment lengths as the root of summed squares, a point
+    at a distance taken on the first segment extending strictly beyond it)."""
+    seglens = [math.sqrt((b[0] - a[0]) ** 2 + (b[1] - a[1]) ** 2) for a, b in zip(coords[:-1], coords[1:], strict=True)]
     total = sum(seglens)
     start = min(max(start, 0.0), total)
     end = min(max(end, 0.0), total)
@@ -57,9 +59,9 @@ def _substring(coords: list[tuple[float, ...]], start: float, end: float) -> lis
     def point_at(dist: float) -> tuple[float, ...]:
         walked = 0.0
         for (a, b), sl in zip(zip(coords[:-1], coords[1:], strict=True), seglens, strict=True):
-            if walked + sl >= dist:
-                t = 0.0 if sl == 0 else (dist - walked) / sl
-                return tuple(a[i] + (b[i] - a[i]) * t for i in range(len(a)))
+            if walked + sl > dist:
+                t = (dist - walked) / sl
+                return tuple(a[i] + t * (b[i] - a[i]) for i in range(len(a)))
             walked += sl
         return tuple(coords[-1])
 
@@ -67,26 +69,33 @@ def _substring(coords: list[tuple[float, ...]], start: float, end: float) -> lis
     walked = 0.0
     for i, sl in enumerate(seglens):
         walked += sl
-        if start < walked < end:
+        if walked >= end:
+            break
+        if start < walked:
             out.append(tuple(coords[i + 1]))
     out.append(point_at(end))
     return out
 
 
 def nx_decompose(nx_multigraph: nx.MultiGraph, decompose_max: float) -> nx.MultiGraph:
-    """Split edges so that none exceeds ``decompose_max`` (graphs.py:1829-1962)."""
+    """Split edges so that none exceeds ``decompose_max`` (graphs.py:1829-1962).
+
+    The geometry (cut count, piece polylines) comes from the native library in
+    one call for all edges; this function names the new nodes and builds the graph.
+    """
     g = util.validate_cityseer_networkx_graph(nx_multigraph)
-    for s, e, data in list(nx_multigraph.edges(data=True)):
+    edges = list(nx_multigraph.edges(data=True))
+    lines = []
+    for s, _e, data in edges:
         sxy = (nx_multigraph.nodes[s]["x"], nx_multigraph.nodes[s]["y"])
-        coords = util.align_linestring_coords(_coords_of(data["geom"]), sxy)
-        line_len = LineString(coords).length
-        cuts = int(np.ceil(line_len / decompose_max))
-        step_size = line_len / cuts
+        lines.append(util.align_linestring_coords(_coords_of(data["geom"]), sxy))
+    all_pieces = _ingest.decompose(lines, decompose_max) if lines else []
+    for (s, e, data), pieces in zip(edges, all_pieces, strict=True):
+        cuts = len(pieces)
         g.remove_edge(s, e)
-        step = 0.0
         prior = s
         for sub in range(cuts - 1):
-            seg = _substring(coords, step, step + step_size)
+            seg = [tuple(c) for c in pieces[sub].tolist()]
             end_coord = seg[-1]
             x, y = end_coord[0], end_coord[1]
             z = end_coord[2] if len(end_coord) == 3 else None
@@ -112,8 +121,7 @@ def nx_decompose(nx_multigraph: nx.MultiGraph, decompose_max: float) -> nx.Multi
             copy = {k: v for k, v in data.items() if k != "geom"}
             g.add_edge(prior, new_name, geom=LineString(seg), **copy)
             prior = new_name
-            step += step_size
-        seg = _substring(coords, step, line_len)
+        seg = [tuple(c) for c in pieces[-1].tolist()]
         copy = {k: v for k, v in data.items() if k != "geom"}
         g.add_edge(prior, e, geom=LineString(seg), **copy)
     return util.validate_cityseer_networkx_graph(g)

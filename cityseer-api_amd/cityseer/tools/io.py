@@ -12,6 +12,7 @@ out of scope.
 from __future__ import annotations
 
 import logging
+import math
 from typing import Any
 
 import networkx as nx
@@ -49,7 +50,8 @@ def network_structure_from_nx(nx_multigraph: nx.MultiGraph, crs: None = None):
                 nd["primal_edge_idx"],
             )
     edge_rows: dict[str, tuple[Any, ...]] = {}
-    edge_dual: list[Any] = []
+    # every directed edge in adjacency order, then one native call derives the payloads
+    starts, ends, keys, lines, imps, pks, rows = [], [], [], [], [], [], []
     for start in g.nodes():
         s_idx, sx, sy = node_rows[start][0], node_rows[start][1], node_rows[start][2]
         for end in g.neighbors(start):
@@ -59,22 +61,26 @@ def network_structure_from_nx(nx_multigraph: nx.MultiGraph, crs: None = None):
                 if geom is None:
                     raise ValueError(f"Edge has no geometry: start_node={start}, end_node={end}, edge_idx={edge_idx}")
                 coords = util.align_linestring_coords(list(geom.coords), (sx, sy))
-                eid = ns.add_street_edge(
-                    s_idx,
-                    e_idx,
-                    int(edge_idx),
-                    start,
-                    end,
-                    coords,
-                    float(ed.get("imp_factor", 1.0)),
-                    shared_primal_node_key=ed.get("primal_node_id"),
-                )
-                edge_rows[f"{start}-{end}"] = (
-                    eid, s_idx, e_idx, edge_idx, start, end, ed.get("imp_factor", 1.0),
-                    util.measure_bearing(np.array(coords[0]), np.array(coords[-1])), util.LineString(coords),
-                )
-                if is_dual:
-                    edge_dual.append(ed["primal_node_id"])
+                imp = float(ed.get("imp_factor", 1.0))
+                if not math.isfinite(imp) or imp <= 0.0:
+                    raise ValueError(
+                        f"Invalid impedance factor ({imp}) for edge (idx {edge_idx}) between nodes "
+                        f"{s_idx} and {e_idx}.\nImpedance must be finite and positive (> 0.0)."
+                    )
+                starts.append(s_idx)
+                ends.append(e_idx)
+                keys.append(int(edge_idx))
+                lines.append(coords)
+                imps.append(imp)
+                pks.append(ed.get("primal_node_id"))
+                rows.append((f"{start}-{end}", s_idx, e_idx, edge_idx, start, end, ed.get("imp_factor", 1.0),
+                             util.measure_bearing(np.array(coords[0]), np.array(coords[-1])), util.LineString(coords)))
+    eids = ns.add_street_edges_from_lines(
+        np.asarray(starts, dtype=np.int64), np.asarray(ends, dtype=np.int64), np.asarray(keys, dtype=np.uint32),
+        lines, np.asarray(imps, dtype=np.float32), pks,
+    ) if lines else []
+    for eid, (name, *rest) in zip(eids, rows, strict=True):
+        edge_rows[name] = (int(eid), *rest)
     nodes_df = pd.DataFrame.from_dict(
         node_rows, orient="index", columns=["ns_node_idx", "x", "y", "z", "live", "weight"]
     )

@@ -173,14 +173,6 @@ def align_linestring_coords(
     return coords
 
 
-def edge_geometry_payload(coords: Sequence[Sequence[float]]) -> tuple[np.float32, np.float32]:
-    """(length f32, angle_sum f32) as ``add_street_edge`` derives them (graph.rs:774,857)."""
-    length = 0.0
-    for a, b in zip(coords[:-1], coords[1:], strict=True):
-        length += math.hypot(b[0] - a[0], b[1] - a[1])
-    return np.float32(length), np.float32(measure_cumulative_angle(coords))
-
-
 def validate_cityseer_networkx_graph(nx_multigraph):
     """Light validation: a copy of the graph with string node keys checked (util.py validation subset)."""
     g = nx_multigraph.copy()

@@ -103,6 +103,34 @@ int cs_graph_node_order(const cs_graph *g, uint32_t *order);
  * reference counterpart: the reference has no device. */
 int cs_graph_upload(cs_graph *g, int device, float speed_m_s);
 
+/* ---------------------------------------------------------------------- */
+/* Ingest (host only, no GPU)                                             */
+/* ---------------------------------------------------------------------- */
+
+/* The payload NetworkStructure.add_street_edge derives from each edge's
+ * polyline (rust/src/graph.rs:764-857), for n_edges edges at once: edge e's
+ * points are xy[coord_off[e] .. coord_off[e+1]) ([n][2] f64, at least 2).
+ * length = f32 of the summed segment lengths; angle_sum = f32 of the summed
+ * turn angles at interior vertices (degrees); in/out bearing = degrees along
+ * the first / last distinct coordinate pair (0 when there is none). Any
+ * output may be NULL. */
+int cs_edge_geometry(uint32_t n_edges, const uint32_t *coord_off, const double *xy, float *length,
+                     float *angle_sum, float *in_bearing, float *out_bearing);
+
+/* Splits each polyline into ceil(length / decompose_max) pieces of equal
+ * length, as tools.graphs.nx_decompose cuts an edge
+ * (pysrc/cityseer/tools/graphs.py:1911-1961, shapely substring and
+ * interpolation semantics). pts: [n][dims] f64 with dims 2 (x, y) or 3
+ * (x, y, z: z interpolated alike). *n_pieces, *n_points size the result;
+ * cs_pieces_copy fills edge_cuts [n_edges], piece_off [n_pieces + 1] and pts
+ * [n_points][dims] (any may be NULL); cs_pieces_free releases it. Each
+ * interior piece end is a new node. */
+typedef struct cs_pieces cs_pieces;
+int cs_decompose(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uint32_t dims,
+                 double decompose_max, cs_pieces **out, uint32_t *n_pieces, uint32_t *n_points);
+int cs_pieces_copy(const cs_pieces *p, uint32_t *edge_cuts, uint32_t *piece_off, double *pts);
+void cs_pieces_free(cs_pieces *p);
+
 /* Number of nodes/edges; maximum out/in degree. */
 int cs_graph_info(const cs_graph *g, uint32_t *n_nodes, uint32_t *n_edges, uint32_t *max_out_deg,
                   uint32_t *max_in_deg);
