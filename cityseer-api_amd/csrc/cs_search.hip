@@ -54,6 +54,7 @@ constexpr uint32_t kFlushList = kScanQ * 64;  // LDS list of one flush step (rea
 constexpr uint32_t kFlushWords = 3 * kFlushList;  // {node, seconds bits, slot} per list entry
 constexpr uint32_t kRelWords = 5 * 64;  // per lane {first edge, in-degree, node, seconds, row start} of a batch
 constexpr uint32_t kWaveLds = kFlushWords + kRelWords;  // LDS words per wave besides the ring
+constexpr uint32_t kLdsWords = 16384;                     // LDS words one workgroup may hold (64 KiB)
 constexpr uint32_t kScanSpan = 16; // scan the index span only while it is at most this many times the reach
 constexpr uint32_t kEpochMax = 1023u;       // 10-bit epoch tags 1..1023; 0 is never issued
 constexpr uint32_t kSlotLimit = 1u << 22;   // 22-bit slots in the node key
@@ -1425,6 +1426,16 @@ void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   p.stride = o;
 }
 
+// Workgroup size for a launch whose widest ring has nbk buckets: the tuned
+// size, lowered until the waves' LDS (ring counters plus flush and relaxation
+// lists) fits one workgroup's 64 KiB.
+uint32_t launch_block(const Tuning &tune, uint32_t nbk) {
+  uint32_t block = tune.block ? tune.block : kBlock;
+  if (block % kWave || block > (uint32_t)kBlock) fail(CS_EINVAL, "workgroup size must be 64, 128, 192 or 256");
+  const uint32_t max_wpb = std::max(1u, kLdsWords / (nbk + kWaveLds));
+  return std::min(block, max_wpb * kWave);
+}
+
 // Bucket width: below the smallest edge cost by the largest tie-band drift a
 // node's fold can accumulate (one 1e-4 step per candidate edge), so candidates
 // from the same bucket can never reach a receiver's tie band. When the
@@ -1615,7 +1626,11 @@ Job make_job(const JobSpec &js, uint32_t NT) {
   j.inv_delta = 1.0f / delta;
   const uint32_t jump = (uint32_t)std::ceil((double)de.max_es / (double)delta) + 2;
   j.nbk = jump + 1;
-  if (j.nbk > 4096 - kWaveLds) fail(CS_EINVAL, "edge cost spread too large for the bucket ring");
+  // a wave's ring counters and lists live in LDS, at most 64 KiB per workgroup:
+  // wide rings run fewer waves per workgroup (launch_block)
+  if (j.nbk > kLdsWords - kWaveLds)
+    fail(CS_EINVAL, "edge cost spread too large for the bucket ring (longest edge " + std::to_string(de.max_es) +
+                        " s against a bucket width of " + std::to_string(delta) + " s)");
   j.maxb = (uint32_t)((double)cs.maxsec * (double)j.inv_delta) + 4;
   j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u) |
             (exact_fold ? 16u : 0u);
@@ -1665,8 +1680,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     p.nbk = std::max(p.nbk, jobs.back().nbk);
     p.maxb = std::max(p.maxb, jobs.back().maxb);
   }
-  const uint32_t block = tune.block ? tune.block : kBlock;
-  if (block % kWave || block > (uint32_t)kBlock) fail(CS_EINVAL, "workgroup size must be 64, 128, 192 or 256");
+  const uint32_t block = launch_block(tune, p.nbk);
   const uint32_t wpb = block / kWave;
   p.bcap = 1024;
   p.cap = std::min<uint32_t>({maxN, tune.cap ? tune.cap : 16384u, kSlotLimit});
@@ -1874,8 +1888,7 @@ struct ShortestBatch {
       p.nbk = std::max(p.nbk, jobs.back().nbk);
       p.maxb = std::max(p.maxb, jobs.back().maxb);
     }
-    block = tune.block ? tune.block : kBlock;
-    if (block % kWave || block > (uint32_t)kBlock) fail(CS_EINVAL, "workgroup size must be 64, 128, 192 or 256");
+    block = launch_block(tune, p.nbk);
     wpb = block / kWave;
     p.bcap = 1024;
     p.cap = std::min<uint32_t>({maxN, tune.cap ? tune.cap : 16384u, kSlotLimit});

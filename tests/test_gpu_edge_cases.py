@@ -162,3 +162,16 @@ def test_long_thresholds_against_short_edges(jitter):
     res = ns.centrality_shortest(distances=[3000], tolerance=5.0)
     ref, _r = og.centrality_shortest([3000], tolerance=0.05)
     assert_close(stack(res, SHORTEST_KEYS, [3000]), ref, "2 m lattice, Phase 2")
+
+
+def test_edge_cost_spread_beyond_one_ring_per_four_waves():
+    """Two 1 m chains joined by a 4.5 km edge: the bucket ring spans ~6,000 bucket widths, more
+    than four waves' share of a workgroup's LDS, so the launch runs fewer waves per workgroup."""
+    left = [(-float(i), 0.0) for i in range(120)]
+    right = [(4500.0 + float(i), 0.0) for i in range(120)]
+    xy = np.array(left + right)
+    edges = [(i, i + 1) for i in range(119)] + [(120 + i, 121 + i) for i in range(119)] + [(0, 120)]
+    ns = _ns(xy, edges)
+    say("two 1 m chains and a 4.5 km edge, thresholds [30, 6000] m")
+    got = _check(ns, [30, 6000], "wide edge-cost spread")
+    assert got[0][1].min() == 239.0  # at 6 km every node reaches all others
