@@ -385,6 +385,9 @@ def run_batch(args) -> int:
 SERVE_DISTANCES = [400, 800, 1600]
 
 
+SERVE_PASSES = 3  # timed passes of each side of the serving comparison (medians reported)
+
+
 def serve_workload(n_requests: int, seed: int = 0):
     """Graphs decomposed into <= 20 m edges with node counts log-uniform in [2,000, 100,000] (two of
     them also as duals for simplest-path requests); a stream of requests, each on a random graph,
@@ -441,27 +444,39 @@ def run_serve(args) -> int:
     log("building serving workload")
     graphs, reqs = serve_workload(args.requests, args.seed)
     total_sources = sum(len(kw["source_indices"]) for _, _, kw in reqs)
-    for ns in graphs:  # device-resident graphs before any timing
-        ns.centrality_shortest(distances=[400], source_indices=[0])
+    for ns in graphs:  # device-resident graphs (and angular adjacency) before any timing
+        if ns.is_dual:
+            ns.centrality_simplest(distances=[400], source_indices=[0])
+        else:
+            ns.centrality_shortest(distances=[400], source_indices=[0])
 
     def plain_call(kind, ns, kw):
         return ns.centrality_shortest(**kw) if kind == "shortest" else ns.centrality_simplest(**kw)
 
-    log("plain: one call per request, in order")
-    service = []
-    for kind, ns, kw in reqs:
-        t0 = time.perf_counter()
-        plain_call(kind, ns, kw)
-        service.append(time.perf_counter() - t0)
-    plain_s = float(np.sum(service))
+    # the saturation pass is one burst whose launches interleave differently from run to run:
+    # both sides are timed SERVE_PASSES times and the medians reported
+    passes = []
+    for rep in range(SERVE_PASSES):
+        log(f"plain: one call per request, in order (pass {rep + 1}/{SERVE_PASSES})")
+        service = []
+        for kind, ns, kw in reqs:
+            t0 = time.perf_counter()
+            plain_call(kind, ns, kw)
+            service.append(time.perf_counter() - t0)
+        passes.append((float(np.sum(service)), service))
+    passes.sort(key=lambda x: x[0])
+    plain_s, service = passes[len(passes) // 2]
 
     with CentralityServer() as srv:
-        log("served: all requests at once (saturation)")
-        t0 = time.perf_counter()
-        handles = [(srv.submit_shortest if k == "shortest" else srv.submit_simplest)(ns, **kw) for k, ns, kw in reqs]
-        for h in handles:
-            h.wait()
-        sat_s = time.perf_counter() - t0
+        sats = []
+        for rep in range(SERVE_PASSES):
+            log(f"served: all requests at once (saturation, pass {rep + 1}/{SERVE_PASSES})")
+            t0 = time.perf_counter()
+            handles = [(srv.submit_shortest if k == "shortest" else srv.submit_simplest)(ns, **kw) for k, ns, kw in reqs]
+            for h in handles:
+                h.wait()
+            sats.append(time.perf_counter() - t0)
+        sat_s = float(np.median(sats))
         sat_rate = len(reqs) / sat_s
         lam = args.load * sat_rate
         rng = np.random.default_rng(args.seed + 1)
@@ -505,6 +520,8 @@ def run_serve(args) -> int:
             "mode": "serve",
             "plain_value": round(plain_value, 2),
             "served_over_plain": round(value / plain_value, 3),
+            "served_s_passes": [round(x, 4) for x in sats],
+            "plain_s_passes": sorted(round(p[0], 4) for p in passes),
             "arrival_rate_per_s": round(lam, 3),
             "p50_ms": round(float(np.percentile(lat, 50)), 2),
             "p99_ms": round(float(np.percentile(lat, 99)), 2),

@@ -125,6 +125,10 @@ _SIGNATURES = {
         [ctypes.c_uint32, _u32p, _f64p, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p),
          _u32p, _u32p],
     ),
+    "cs_halve": (
+        ctypes.c_int,
+        [ctypes.c_uint32, _u32p, _f64p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p), _u32p, _u32p],
+    ),
     "cs_pieces_copy": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _f64p]),
     "cs_pieces_free": (None, [ctypes.c_void_p]),
     "cs_graph_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _u32p, _u32p]),

@@ -9,7 +9,6 @@ shapely geometries. Graph cleaning (graphs.py:110-1828) is out of scope.
 from __future__ import annotations
 
 import logging
-import math
 
 import networkx as nx
 
@@ -45,36 +44,6 @@ def nx_simple_geoms(nx_multigraph: nx.MultiGraph) -> nx.MultiGraph:
     for s, e, k in remove:
         g.remove_edge(s, e, key=k)
     return util.validate_cityseer_networkx_graph(g)
-
-
-def _substring(coords: list[tuple[float, ...]], start: float, end: float) -> list[tuple[float, ...]]:
-    """Part of a polyline between two distances along it, for start < end (shapely
-    ``ops.substring`` over GEOS: segment lengths as the root of summed squares, a point
-    at a distance taken on the first segment extending strictly beyond it)."""
-    seglens = [math.sqrt((b[0] - a[0]) ** 2 + (b[1] - a[1]) ** 2) for a, b in zip(coords[:-1], coords[1:], strict=True)]
-    total = sum(seglens)
-    start = min(max(start, 0.0), total)
-    end = min(max(end, 0.0), total)
-
-    def point_at(dist: float) -> tuple[float, ...]:
-        walked = 0.0
-        for (a, b), sl in zip(zip(coords[:-1], coords[1:], strict=True), seglens, strict=True):
-            if walked + sl > dist:
-                t = (dist - walked) / sl
-                return tuple(a[i] + t * (b[i] - a[i]) for i in range(len(a)))
-            walked += sl
-        return tuple(coords[-1])
-
-    out = [point_at(start)]
-    walked = 0.0
-    for i, sl in enumerate(seglens):
-        walked += sl
-        if walked >= end:
-            break
-        if start < walked:
-            out.append(tuple(coords[i + 1]))
-    out.append(point_at(end))
-    return out
 
 
 def nx_decompose(nx_multigraph: nx.MultiGraph, decompose_max: float) -> nx.MultiGraph:
@@ -142,13 +111,24 @@ def nx_to_dual(nx_multigraph: nx.MultiGraph) -> nx.MultiGraph:
     if "crs" in nx_multigraph.graph:
         g_dual.graph["crs"] = nx_multigraph.graph["crs"]
 
+    # every primal edge halved natively in one call: as stored (its dual node sits at the
+    # middle), and aligned to each of its ends (the halves that dual edges weld)
+    edges = list(nx_multigraph.edges(data=True, keys=True))
+    lines, where = [], {}
+    for i, (s, e, k, data) in enumerate(edges):
+        raw = _coords_of(data["geom"])
+        s_xy = (nx_multigraph.nodes[s]["x"], nx_multigraph.nodes[s]["y"])
+        e_xy = (nx_multigraph.nodes[e]["x"], nx_multigraph.nodes[e]["y"])
+        lines += [raw, util.align_linestring_coords(raw, s_xy), util.align_linestring_coords(raw, e_xy)]
+        where[(s, e, k)] = (i, 1)
+        where[(e, s, k)] = (i, 2)
+    halves = _ingest.halve(lines) if lines else []
+
     def half_geoms(a, b, k):
-        data = nx_multigraph[a][b][k]
+        i, side = where[(a, b, k)]
+        a_half = [tuple(c) for c in halves[3 * i + side][0].tolist()]
+        b_half = [tuple(c) for c in halves[3 * i + side][1].tolist()]
         a_xy = (nx_multigraph.nodes[a]["x"], nx_multigraph.nodes[a]["y"])
-        coords = util.align_linestring_coords(_coords_of(data["geom"]), a_xy)
-        total = LineString(coords).length
-        a_half = _substring(coords, 0.0, 0.5 * total)
-        b_half = _substring(coords, 0.5 * total, total)
         a_half[0] = tuple([a_xy[0], a_xy[1]] + list(a_half[0][2:]))
         mid = a_half[-1]
         b_half[0] = mid
@@ -160,20 +140,20 @@ def nx_to_dual(nx_multigraph: nx.MultiGraph) -> nx.MultiGraph:
         se = sorted([str(s), str(e)])
         return f"{se[0]}_{se[1]}_k{k}"
 
-    for s, e, k, data in nx_multigraph.edges(data=True, keys=True):
+    for i, (s, e, k, data) in enumerate(edges):
         geom = data["geom"]
-        mid = LineString(_coords_of(geom)).interpolate(0.5, normalized=True)
+        mid = halves[3 * i][0][-1]  # the middle of the edge as stored
         dk = key_of(s, e, k)
         kwargs = {
-            "x": mid.x,
-            "y": mid.y,
+            "x": float(mid[0]),
+            "y": float(mid[1]),
             "primal_edge": geom,
             "primal_edge_node_a": s,
             "primal_edge_node_b": e,
             "primal_edge_idx": k,
         }
-        if mid.has_z:
-            kwargs["z"] = mid.z
+        if len(mid) > 2:
+            kwargs["z"] = float(mid[2])
         g_dual.add_node(dk, **kwargs)
         if "live" in nx_multigraph.nodes[s] and "live" in nx_multigraph.nodes[e]:
             live = bool(nx_multigraph.nodes[s]["live"]) or bool(nx_multigraph.nodes[e]["live"])

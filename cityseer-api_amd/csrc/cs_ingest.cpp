@@ -138,14 +138,16 @@ int cs_edge_geometry(uint32_t n_edges, const uint32_t *coord_off, const double *
   });
 }
 
-int cs_decompose(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uint32_t dims,
-                 double decompose_max, cs_pieces **out, uint32_t *n_pieces, uint32_t *n_points) {
-  return cs::guarded_call([&] {
-    if (!out || !n_pieces || !n_points) cs::fail(CS_EINVAL, "out, n_pieces and n_points must not be NULL");
-    if (n_edges && (!coord_off || !pts)) cs::fail(CS_EINVAL, "coord_off and pts must not be NULL");
-    if (dims != 2 && dims != 3) cs::fail(CS_EINVAL, "dims must be 2 or 3");
-    if (!(decompose_max > 0.0) || !std::isfinite(decompose_max))
-      cs::fail(CS_EINVAL, "decompose_max must be finite and positive");
+// Cuts every polyline into equal pieces: ceil(length / decompose_max) of them,
+// or exactly two (at half its length) when halve is set.
+static void cut_lines(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uint32_t dims,
+                      double decompose_max, bool halve, cs_pieces **out, uint32_t *n_pieces, uint32_t *n_points) {
+  if (!out || !n_pieces || !n_points) cs::fail(CS_EINVAL, "out, n_pieces and n_points must not be NULL");
+  if (n_edges && (!coord_off || !pts)) cs::fail(CS_EINVAL, "coord_off and pts must not be NULL");
+  if (dims != 2 && dims != 3) cs::fail(CS_EINVAL, "dims must be 2 or 3");
+  if (!halve && (!(decompose_max > 0.0) || !std::isfinite(decompose_max)))
+    cs::fail(CS_EINVAL, "decompose_max must be finite and positive");
+  {
     auto h = std::make_unique<cs_pieces>();
     Pieces &P = h->p;
     P.dims = dims;
@@ -165,9 +167,9 @@ int cs_decompose(uint32_t n_edges, const uint32_t *coord_off, const double *pts,
       }
       if (!(total > 0.0)) cs::fail(CS_EINVAL, "edge " + std::to_string(e) + " has zero length");
       // cuts = ceil(length / decompose_max) equal steps; the last piece runs to
-      // the end of the line (graphs.py:1911-1961)
-      const uint32_t cuts = (uint32_t)std::max(1.0, std::ceil(total / decompose_max));
-      const double step = total / cuts;
+      // the end of the line (graphs.py:1911-1961). Halves: [0, total/2], [total/2, total]
+      const uint32_t cuts = halve ? 2u : (uint32_t)std::max(1.0, std::ceil(total / decompose_max));
+      const double step = halve ? 0.5 * total : total / cuts;
       P.edge_cuts[e] = cuts;
       double at = 0.0;
       for (uint32_t k = 0; k < cuts; ++k) {
@@ -180,7 +182,18 @@ int cs_decompose(uint32_t n_edges, const uint32_t *coord_off, const double *pts,
     *n_pieces = (uint32_t)(P.piece_off.size() - 1);
     *n_points = (uint32_t)(P.pts.size() / dims);
     *out = h.release();
-  });
+  }
+}
+
+int cs_decompose(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uint32_t dims,
+                 double decompose_max, cs_pieces **out, uint32_t *n_pieces, uint32_t *n_points) {
+  return cs::guarded_call(
+      [&] { cut_lines(n_edges, coord_off, pts, dims, decompose_max, false, out, n_pieces, n_points); });
+}
+
+int cs_halve(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uint32_t dims, cs_pieces **out,
+             uint32_t *n_pieces, uint32_t *n_points) {
+  return cs::guarded_call([&] { cut_lines(n_edges, coord_off, pts, dims, 0.0, true, out, n_pieces, n_points); });
 }
 
 int cs_pieces_copy(const cs_pieces *h, uint32_t *edge_cuts, uint32_t *piece_off, double *pts) {

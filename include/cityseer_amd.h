@@ -128,6 +128,10 @@ int cs_edge_geometry(uint32_t n_edges, const uint32_t *coord_off, const double *
 typedef struct cs_pieces cs_pieces;
 int cs_decompose(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uint32_t dims,
                  double decompose_max, cs_pieces **out, uint32_t *n_pieces, uint32_t *n_points);
+/* The two halves of each polyline (cut at half its length, same rules), as
+ * tools.graphs.nx_to_dual splits primal edges (graphs.py:1965-2149). */
+int cs_halve(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uint32_t dims, cs_pieces **out,
+             uint32_t *n_pieces, uint32_t *n_points);
 int cs_pieces_copy(const cs_pieces *p, uint32_t *edge_cuts, uint32_t *piece_off, double *pts);
 void cs_pieces_free(cs_pieces *p);
 

@@ -19,6 +19,36 @@ from cityseer import _ingest
 from cityseer.tools import graphs, io, util
 
 
+def _py_substring(coords: list[tuple[float, ...]], start: float, end: float) -> list[tuple[float, ...]]:
+    """Part of a polyline between two distances along it, for start < end (shapely
+    ``ops.substring`` over GEOS: segment lengths as the root of summed squares, a point
+    at a distance taken on the first segment extending strictly beyond it)."""
+    seglens = [math.sqrt((b[0] - a[0]) ** 2 + (b[1] - a[1]) ** 2) for a, b in zip(coords[:-1], coords[1:], strict=True)]
+    total = sum(seglens)
+    start = min(max(start, 0.0), total)
+    end = min(max(end, 0.0), total)
+
+    def point_at(dist: float) -> tuple[float, ...]:
+        walked = 0.0
+        for (a, b), sl in zip(zip(coords[:-1], coords[1:], strict=True), seglens, strict=True):
+            if walked + sl > dist:
+                t = (dist - walked) / sl
+                return tuple(a[i] + t * (b[i] - a[i]) for i in range(len(a)))
+            walked += sl
+        return tuple(coords[-1])
+
+    out = [point_at(start)]
+    walked = 0.0
+    for i, sl in enumerate(seglens):
+        walked += sl
+        if walked >= end:
+            break
+        if start < walked:
+            out.append(tuple(coords[i + 1]))
+    out.append(point_at(end))
+    return out
+
+
 def _py_payload(coords):
     """Length (Σ hypot, f64 → f32) and cumulative turn angle (f64 → f32) of a polyline."""
     length = 0.0
@@ -110,7 +140,7 @@ def test_decompose_matches_python_statement():
         at = 0.0
         for k, p in enumerate(pieces):
             end = total if k + 1 == len(pieces) else at + step
-            want = graphs._substring(line, at, end)
+            want = _py_substring(line, at, end)
             assert p.tolist() == [list(c) for c in want]
             at += step
 
@@ -162,3 +192,23 @@ def test_network_structure_payloads_from_native_bulk(primal_graph):
         one.add_street_edge(s, d, k, keys[s], keys[d], coords)
     b = one.arrays()
     assert np.array_equal(a["length"], b["length"]) and np.array_equal(a["angle"], b["angle"])
+
+
+def test_halve_and_dual_geometry(diamond_graph):
+    """Halves meet at half length; a dual node sits at its primal edge's middle and a dual edge
+    is the two near halves welded at the shared node (graphs.py:1965-2149)."""
+    (a, b), = _ingest.halve([[(0.0, 0.0), (10.0, 0.0), (10.0, 10.0)]])
+    assert a.tolist() == [[0.0, 0.0], [10.0, 0.0]] and b.tolist() == [[10.0, 0.0], [10.0, 10.0]]
+    (a, b), = _ingest.halve([[(0.0, 0.0, 2.0), (8.0, 6.0, 4.0)]])
+    assert a[-1].tolist() == [4.0, 3.0, 3.0] and b[0].tolist() == [4.0, 3.0, 3.0]
+    dual = graphs.nx_to_dual(diamond_graph)
+    for _n, d in dual.nodes(data=True):
+        coords = list(d["primal_edge"].coords)
+        (h0, _h1), = _ingest.halve([coords])
+        assert (d["x"], d["y"]) == (h0[-1][0], h0[-1][1])
+    for s, e, d in dual.edges(data=True):
+        half = lambda n: dual.nodes[n]["primal_edge"].length / 2  # noqa: E731
+        assert math.isclose(d["geom"].length, half(s) + half(e), rel_tol=1e-12)
+        shared = d["primal_node_id"]
+        px, py = diamond_graph.nodes[shared]["x"], diamond_graph.nodes[shared]["y"]
+        assert any(math.isclose(c[0], px) and math.isclose(c[1], py) for c in d["geom"].coords)
