@@ -95,49 +95,6 @@ struct cs_pieces {
   Pieces p;
 };
 
-extern "C" {
-
-int cs_edge_geometry(uint32_t n_edges, const uint32_t *coord_off, const double *xy, float *length, float *angle_sum,
-                     float *in_bearing, float *out_bearing) {
-  return cs::guarded_call([&] {
-    if (n_edges && (!coord_off || !xy)) cs::fail(CS_EINVAL, "coord_off and xy must not be NULL");
-    for (uint32_t e = 0; e < n_edges; ++e) {
-      const uint32_t b = coord_off[e], n = coord_off[e + 1] - coord_off[e];
-      if (coord_off[e + 1] < b || n < 2)
-        cs::fail(CS_EINVAL, "Street edge geometry (idx " + std::to_string(e) +
-                                ") must have at least 2 coordinates. Found " + std::to_string(n < 2 ? n : 0) + ".");
-      const XY *c = reinterpret_cast<const XY *>(xy) + b;
-      // Euclidean length of the LineString (graph.rs:774): hypot per segment, summed in f64
-      double len = 0.0;
-      for (uint32_t i = 0; i + 1 < n; ++i) len += std::hypot(c[i + 1].x - c[i].x, c[i + 1].y - c[i].y);
-      if (length) length[e] = (float)len;
-      // cumulative turn angle (graph.rs:368-379, 857)
-      double ang = 0.0;
-      for (uint32_t i = 1; i + 1 < n; ++i) ang += turn(c[i - 1], c[i], c[i + 1]);
-      if (angle_sum) angle_sum[e] = (float)ang;
-      // first / last distinct coordinate pairs (graph.rs:797-855); 0 for degenerate edges
-      if (in_bearing) {
-        float v = 0.0f;
-        for (uint32_t i = 1; i < n; ++i)
-          if (!same(c[i], c[0])) {
-            v = (float)bearing(c[0], c[i]);
-            break;
-          }
-        in_bearing[e] = v;
-      }
-      if (out_bearing) {
-        float v = 0.0f;
-        for (uint32_t i = n - 1; i-- > 0;)
-          if (!same(c[i], c[n - 1])) {
-            v = (float)bearing(c[i], c[n - 1]);
-            break;
-          }
-        out_bearing[e] = v;
-      }
-    }
-  });
-}
-
 // Cuts every polyline into equal pieces: ceil(length / decompose_max) of them,
 // or exactly two (at half its length) when halve is set.
 static void cut_lines(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uint32_t dims,
@@ -183,6 +140,49 @@ static void cut_lines(uint32_t n_edges, const uint32_t *coord_off, const double 
     *n_points = (uint32_t)(P.pts.size() / dims);
     *out = h.release();
   }
+}
+
+extern "C" {
+
+int cs_edge_geometry(uint32_t n_edges, const uint32_t *coord_off, const double *xy, float *length, float *angle_sum,
+                     float *in_bearing, float *out_bearing) {
+  return cs::guarded_call([&] {
+    if (n_edges && (!coord_off || !xy)) cs::fail(CS_EINVAL, "coord_off and xy must not be NULL");
+    for (uint32_t e = 0; e < n_edges; ++e) {
+      const uint32_t b = coord_off[e], n = coord_off[e + 1] - coord_off[e];
+      if (coord_off[e + 1] < b || n < 2)
+        cs::fail(CS_EINVAL, "Street edge geometry (idx " + std::to_string(e) +
+                                ") must have at least 2 coordinates. Found " + std::to_string(n < 2 ? n : 0) + ".");
+      const XY *c = reinterpret_cast<const XY *>(xy) + b;
+      // Euclidean length of the LineString (graph.rs:774): hypot per segment, summed in f64
+      double len = 0.0;
+      for (uint32_t i = 0; i + 1 < n; ++i) len += std::hypot(c[i + 1].x - c[i].x, c[i + 1].y - c[i].y);
+      if (length) length[e] = (float)len;
+      // cumulative turn angle (graph.rs:368-379, 857)
+      double ang = 0.0;
+      for (uint32_t i = 1; i + 1 < n; ++i) ang += turn(c[i - 1], c[i], c[i + 1]);
+      if (angle_sum) angle_sum[e] = (float)ang;
+      // first / last distinct coordinate pairs (graph.rs:797-855); 0 for degenerate edges
+      if (in_bearing) {
+        float v = 0.0f;
+        for (uint32_t i = 1; i < n; ++i)
+          if (!same(c[i], c[0])) {
+            v = (float)bearing(c[0], c[i]);
+            break;
+          }
+        in_bearing[e] = v;
+      }
+      if (out_bearing) {
+        float v = 0.0f;
+        for (uint32_t i = n - 1; i-- > 0;)
+          if (!same(c[i], c[n - 1])) {
+            v = (float)bearing(c[i], c[n - 1]);
+            break;
+          }
+        out_bearing[e] = v;
+      }
+    }
+  });
 }
 
 int cs_decompose(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uint32_t dims,
