@@ -997,8 +997,12 @@ __device__ void flush_targets(CJob &j, const WaveWs &w, const SrcCtx &c, uint32_
 // ---------------------------------------------------------------------------
 // Persistent kernel: each wave loops over sources taken from the counters.
 // ---------------------------------------------------------------------------
+// With up to four thresholds the search asks for five resident waves per SIMD
+// (96 VGPRs, a few spilled to scratch in cold paths): 45.2 ms against 45.6 ms at
+// four on the 200x200 workload; six waves spill in hot paths and run slower.
 template <int MODE, int NT>
-__global__ __launch_bounds__(kBlock) void k_search(SearchArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 ? 5 : 1, 8))) void k_search(
+    SearchArgs a) {
   extern __shared__ uint32_t lds_rings[];
   const uint32_t lane = lane_id();
   const uint32_t wib = uniform(threadIdx.x / kWave);
