@@ -46,7 +46,6 @@ namespace cs {
 namespace {
 
 constexpr uint32_t kFailCap = 1u, kFailBucket = 2u, kFailRing = 4u, kFailInternal = 8u;
-constexpr uint32_t kFoldChunk = 4;  // out-row candidates whose loads one fold step batches
 constexpr int kChunk = 1;           // in-edges per lane per step of the segment closeness walk
 constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dealt to them round-robin
 constexpr uint32_t kScanQ = 4;     // key words per lane per step of the node-ordered result flush
@@ -60,6 +59,10 @@ constexpr uint32_t kEpochMax = 1023u;       // 10-bit epoch tags 1..1023; 0 is n
 constexpr uint32_t kSlotLimit = 1u << 22;   // 22-bit slots in the node key
 
 enum Mode { kBrandes = 0, kSegment = 1, kTree = 2, kBrandesExport = 3 };
+// out-row candidates whose loads one fold step batches: two in the Brandes
+// search (room for five waves per SIMD), four in the trees and segment search
+template <int MODE>
+constexpr uint32_t fold_chunk() { return MODE == kBrandes ? 2u : 4u; }
 
 // One job: a graph and the parameters of one centrality request over it. A
 // launch runs the sources of one job (plain calls) or of many (batches).
@@ -433,7 +436,8 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
         float bcs = INFINITY, bsu = INFINITY;
         uint32_t bu = 0xffffffffu, bj = 0xffffffffu, bslot = 0;
         // ---- fold over the out-row, one node per lane: the candidates' loads
-        // go out four at a time, then the candidates are weighed in row order
+        // go out kFoldChunk at a time, then the candidates are weighed in row order
+        constexpr uint32_t kFoldChunk = fold_chunk<MODE>();
         const uint32_t nfold = (is_src || !(fold || cycles)) ? 0u : nf;
         for (uint32_t cb = 0; cb < nfold; cb += kFoldChunk) {
           uint2 fe[kFoldChunk];
