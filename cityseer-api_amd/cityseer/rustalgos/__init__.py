@@ -14,10 +14,11 @@ from collections.abc import Sequence
 import numpy as np
 
 from .. import _native
-from . import centrality, graph
+from . import centrality, data, graph
 
 __all__ = [
     "centrality",
+    "data",
     "graph",
     "pair_distances_betas_time",
     "distances_from_betas",

@@ -1,0 +1,218 @@
+"""Data layer: ``DataMap`` and land-use accessibility (rust/src/data.rs).
+
+``DataMap.accessibility`` (data.rs:617-747) runs on the device through
+``cs_accessibility``: one shortest-path search per live node, then, in the
+same launch, the data entries assigned to the reached nodes, each dedupe group
+counted once through its nearest member (data.rs:520-615).
+
+The assignment of entries to nodes is ``node_data_map``: ``{node index:
+[(data key, distance), ...]}``, the map ``assign_data_to_network`` leaves
+(data.rs:455-512). Here it is set by the caller; the geometric assignment
+itself (R-tree candidates, barrier and street-crossing checks,
+graph.rs:1346-1535) is not part of this package and raises
+``NotImplementedError``, as do ``mixed_uses``, ``stats`` and angular
+accessibility.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Any
+
+import numpy as np
+
+from .. import _native
+from .graph import WALKING_SPEED
+
+
+def clip_wts_curve(distances: list[int], betas: list[float], spatial_tolerance: int) -> list[float]:
+    """exp(-beta * spatial_tolerance) per threshold, in f32 (common.rs:311-334)."""
+    if len(distances) != len(betas):
+        raise ValueError("Input 'distances' and 'betas' must have the same length.")
+    out = []
+    for d, b in zip(distances, betas, strict=True):
+        if spatial_tolerance > d:
+            raise ValueError(
+                f"Clipping distance ({spatial_tolerance}) cannot be greater than the distance threshold ({d})."
+            )
+        out.append(float(np.exp(np.float32(-np.float32(b)) * np.float32(spatial_tolerance), dtype=np.float32)))
+    return out
+
+
+def clipped_beta_wt(beta: float, max_curve_wt: float, data_dist: float) -> float:
+    """min(exp(-beta * d), max_curve_wt) / max_curve_wt in f32 (common.rs:339-348)."""
+    if not 0.0 <= max_curve_wt <= 1.0:
+        raise ValueError("Max curve weight must be in the range [0, 1].")
+    raw = np.exp(np.float32(-np.float32(beta)) * np.float32(data_dist), dtype=np.float32)
+    return float(np.float32(min(raw, np.float32(max_curve_wt)) / np.float32(max_curve_wt)))
+
+
+class DataEntry:
+    """One data point (data.rs:290-377): its key, dedupe key and WKT text."""
+
+    def __init__(self, data_key: str, geom_wkt: str, dedupe_key: str | None = None):
+        self.data_key = data_key
+        self.dedupe_key = dedupe_key if dedupe_key is not None else data_key
+        self.geom_wkt = geom_wkt
+
+
+class LanduseAccess:
+    """Per-class accessibility rows (data.rs:24-44): getters return ``{distance: f64[N]}``."""
+
+    def __init__(self, distances, node_indices, count, weighted, nearest, max_dist):
+        self._distances = list(distances)
+        self._idx = np.asarray(node_indices, dtype=np.int64)
+        self._count, self._weighted, self._nearest, self._max = count, weighted, nearest, int(max_dist)
+
+    @property
+    def unweighted(self) -> dict[int, np.ndarray]:
+        return {int(d): self._count[i][self._idx].copy() for i, d in enumerate(self._distances)}
+
+    @property
+    def weighted(self) -> dict[int, np.ndarray]:
+        return {int(d): self._weighted[i][self._idx].copy() for i, d in enumerate(self._distances)}
+
+    @property
+    def distance(self) -> dict[int, np.ndarray]:
+        return {self._max: self._nearest[self._idx].copy()}
+
+
+class AccessibilityResult:
+    """Result of ``DataMap.accessibility`` (data.rs:46-99)."""
+
+    def __init__(self, distances, node_keys_py, node_indices, lu_map: dict[str, LanduseAccess]):
+        self.distances = list(distances)
+        self.node_keys_py = list(node_keys_py)
+        self.node_indices = list(node_indices)
+        self._lu_map = lu_map
+
+    @property
+    def result(self) -> dict[str, LanduseAccess]:
+        return dict(self._lu_map)
+
+
+def _composite_key(key: Any) -> str:
+    """Keys are stored as "type name:str(value)" (common.rs:91-96)."""
+    return f"{type(key).__name__}:{key}"
+
+
+class DataMap:
+    """Map of data entries (data.rs:379-520)."""
+
+    def __init__(self):
+        self.entries: dict[str, DataEntry] = {}
+        self.node_data_map: dict[int, list[tuple[str, float]]] = {}
+        self._progress = 0
+
+    def insert(self, data_key_py: Any, geom_wkt: str, dedupe_key_py: Any = None) -> None:
+        key = _composite_key(data_key_py)
+        dd = None if dedupe_key_py is None else _composite_key(dedupe_key_py)
+        self.entries[key] = DataEntry(key, geom_wkt, dd)
+
+    def entry_keys(self) -> list[str]:
+        return list(self.entries.keys())
+
+    def get_entry(self, data_key: str) -> DataEntry | None:
+        return self.entries.get(data_key)
+
+    def count(self) -> int:
+        return len(self.entries)
+
+    def is_empty(self) -> bool:
+        return not self.entries
+
+    def progress(self) -> int:
+        return self._progress
+
+    def progress_init(self) -> None:
+        self._progress = 0
+
+    def assign_data_to_network(self, network_structure, max_assignment_dist: float, n_nearest_candidates: int):
+        raise NotImplementedError(
+            "geometric data assignment is not available here; set DataMap.node_data_map to "
+            "{node index: [(data key, distance), ...]} instead"
+        )
+
+    def mixed_uses(self, *args, **kwargs):
+        raise NotImplementedError("mixed_uses is not available here")
+
+    def stats(self, *args, **kwargs):
+        raise NotImplementedError("stats is not available here")
+
+    def _assignment_arrays(self):
+        keys = list(self.entries.keys())
+        index = {k: i for i, k in enumerate(keys)}
+        groups: dict[str, int] = {}
+        entry_group = np.empty(len(keys), dtype=np.uint32)
+        for i, k in enumerate(keys):
+            entry_group[i] = groups.setdefault(self.entries[k].dedupe_key, len(groups))
+        nodes, ents, dists = [], [], []
+        for node_idx, pairs in self.node_data_map.items():
+            for data_key, dist in pairs:
+                e = index.get(data_key)
+                if e is None:  # data.rs:563-566: unknown keys are skipped
+                    continue
+                nodes.append(int(node_idx))
+                ents.append(e)
+                dists.append(float(dist))
+        return (keys, entry_group, np.asarray(nodes, dtype=np.uint32), np.asarray(ents, dtype=np.uint32),
+                np.asarray(dists, dtype=np.float64))
+
+    def accessibility(
+        self,
+        network_structure,
+        landuses_map: dict,
+        accessibility_keys: list[str],
+        distances: list[int] | None = None,
+        betas: list[float] | None = None,
+        minutes: list[float] | None = None,
+        angular: bool | None = None,
+        spatial_tolerance: int | None = None,
+        min_threshold_wt: float | None = None,
+        speed_m_s: float | None = None,
+        pbar_disabled: bool | None = None,
+    ) -> AccessibilityResult:
+        """Counts and weights of reachable land uses per node (data.rs:617-747)."""
+        from . import pair_distances_betas_time
+
+        if angular:
+            network_structure.validate_dual_for_angular("accessibility")
+            raise NotImplementedError("angular accessibility is not available here")
+        speed = WALKING_SPEED if speed_m_s is None else float(speed_m_s)
+        dists, bets, secs = pair_distances_betas_time(speed, distances, betas, minutes, min_threshold_wt)
+        if len(landuses_map) != self.count():
+            raise ValueError("The number of landuse encodings must match the number of data points")
+        lu: dict[str, str] = {}
+        for k, v in landuses_map.items():
+            ck = _composite_key(k)
+            if ck not in self.entries:
+                raise KeyError(f"Data entries key missing: {ck}")
+            lu[ck] = str(v)
+        mcw = clip_wts_curve(dists, bets, int(spatial_tolerance or 0))
+        acc_keys = list(accessibility_keys)
+        cls_index = {k: i for i, k in enumerate(acc_keys)}
+        keys, entry_group, a_node, a_ent, a_dist = self._assignment_arrays()
+        entry_class = np.asarray([cls_index.get(lu.get(k, ""), -1) for k in keys], dtype=np.int32)
+        n = network_structure.node_bound()
+        C, T = len(acc_keys), len(dists)
+        count = np.zeros((C, T, n), dtype=np.float64)
+        weighted = np.zeros((C, T, n), dtype=np.float64)
+        nearest = np.full((C, n), np.nan, dtype=np.float64)
+        p = _native.ptr
+        self.progress_init()
+        _native.check(
+            _native.load().cs_accessibility(
+                network_structure._native_handle(), len(keys), p(entry_group, ctypes.c_uint32),
+                p(entry_class, ctypes.c_int32), C, len(a_node), p(a_node, ctypes.c_uint32),
+                p(a_ent, ctypes.c_uint32), p(a_dist, ctypes.c_double), T,
+                p(np.asarray(dists, dtype=np.uint32), ctypes.c_uint32),
+                p(np.asarray(bets, dtype=np.float32), ctypes.c_float),
+                p(np.asarray(mcw, dtype=np.float32), ctypes.c_float), int(max(secs)), speed,
+                p(count, ctypes.c_double), p(weighted, ctypes.c_double), p(nearest, ctypes.c_double),
+            )
+        )
+        self._progress = n
+        idx = network_structure.node_indices()
+        lu_map = {k: LanduseAccess(dists, idx, count[c], weighted[c], nearest[c], max(dists))
+                  for k, c in cls_index.items()}
+        return AccessibilityResult(dists, network_structure.node_keys_py(), idx, lu_map)

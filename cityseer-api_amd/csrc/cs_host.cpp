@@ -1131,6 +1131,132 @@ uint32_t settle_order(const std::vector<float> &secs, uint32_t *order) {
 
 extern "C" {
 
+int cs_accessibility(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
+                     uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
+                     const uint32_t *assign_entry, const double *assign_dist, uint32_t n_thresholds,
+                     const uint32_t *distances, const float *betas, const float *max_curve_wts,
+                     uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
+                     double *out_nearest) {
+  const auto t0 = std::chrono::steady_clock::now();
+  t_stats = cs_stats{};
+  int rc = guarded([&] {
+    if (!h) fail(CS_EINVAL, "graph must not be NULL");
+    Graph &g = h->g;
+    const uint32_t N = g.n, T = n_thresholds, C = n_classes;
+    if (T == 0 || T > kMaxT) fail(CS_EINVAL, "between 1 and " + std::to_string(kMaxT) + " distances are supported");
+    if (!distances || !betas || !max_curve_wts) fail(CS_EINVAL, "distances, betas and max_curve_wts must not be NULL");
+    if (n_entries && (!entry_group || !entry_class)) fail(CS_EINVAL, "entry_group and entry_class must not be NULL");
+    if (n_assign && (!assign_node || !assign_entry || !assign_dist))
+      fail(CS_EINVAL, "assign_node, assign_entry and assign_dist must not be NULL");
+    if ((size_t)C * N && (!out_count || !out_weighted || !out_nearest)) fail(CS_EINVAL, "outputs must not be NULL");
+    if (!std::isfinite(speed_m_s) || !(speed_m_s > 0.0f)) fail(CS_EINVAL, "speed_m_s must be finite and positive");
+    for (uint32_t t = 0; t < T; ++t)
+      if (!(max_curve_wts[t] >= 0.0f && max_curve_wts[t] <= 1.0f) || max_curve_wts[t] == 0.0f)
+        fail(CS_EINVAL, "Max curve weight must be in the range [0, 1].");
+    for (uint32_t e = 0; e < n_entries; ++e) {
+      if (entry_group[e] >= n_entries) fail(CS_EINVAL, "entry_group values must be below n_entries");
+      if (entry_class[e] < -1 || entry_class[e] >= (int32_t)C) fail(CS_EINVAL, "entry_class out of range");
+    }
+    for (uint32_t i = 0; i < n_assign; ++i) {
+      if (assign_node[i] >= N) fail(CS_EINVAL, "assignment node out of range");
+      if (assign_entry[i] >= n_entries) fail(CS_EINVAL, "assignment entry out of range");
+      if (!(assign_dist[i] >= 0.0) || !std::isfinite(assign_dist[i]))
+        fail(CS_EINVAL, "assignment distances must be finite and non-negative");
+    }
+    const size_t CTN = (size_t)C * T * N;
+    std::fill(out_count, out_count + CTN, 0.0);
+    std::fill(out_weighted, out_weighted + CTN, 0.0);
+    std::fill(out_nearest, out_nearest + (size_t)C * N, std::nan(""));
+    if (N == 0 || C == 0 || n_assign == 0) return;
+    // members: assignments grouped by dedupe group, assignment order within a group
+    std::vector<uint32_t> gcount(n_entries + 1, 0);
+    for (uint32_t i = 0; i < n_assign; ++i) ++gcount[entry_group[assign_entry[i]] + 1];
+    for (uint32_t x = 0; x < n_entries; ++x) gcount[x + 1] += gcount[x];
+    const std::vector<uint32_t> group_off = gcount;
+    std::vector<uint4> mem(n_assign);
+    std::vector<uint32_t> mem_node(n_assign);
+    for (uint32_t i = 0; i < n_assign; ++i) {
+      const uint32_t e = assign_entry[i], grp = entry_group[e];
+      const uint32_t m = gcount[grp]++;
+      const uint32_t node = g.identity_order ? assign_node[i] : g.perm[assign_node[i]];
+      const float dd = (float)assign_dist[i];  // data.rs:548 casts the f64 distance to f32
+      uint32_t bits;
+      std::memcpy(&bits, &dd, 4);
+      mem[m] = make_uint4(node, e, bits, grp);
+      mem_node[m] = node;
+    }
+    // members of each device node
+    std::vector<uint32_t> node_off(N + 1, 0), node_mem(n_assign);
+    for (uint32_t m = 0; m < n_assign; ++m) ++node_off[mem_node[m] + 1];
+    for (uint32_t v = 0; v < N; ++v) node_off[v + 1] += node_off[v];
+    {
+      std::vector<uint32_t> at(node_off.begin(), node_off.end() - 1);
+      for (uint32_t m = 0; m < n_assign; ++m) node_mem[at[mem_node[m]]++] = m;
+    }
+    // sources: every live node (data.rs:681-686)
+    std::vector<uint32_t> srcs;
+    for (uint32_t v = 0; v < N; ++v)
+      if (g.live[v]) srcs.push_back(v);
+    to_device_ids(g, srcs);
+    std::sort(srcs.begin(), srcs.end());
+    AccessOut ao;
+    ao.T = T;
+    ao.N = N;
+    uint32_t dmax = 0;
+    for (uint32_t t = 0; t < T; ++t) dmax = std::max(dmax, distances[t]);
+    for (uint32_t t = 0; t < T; ++t) {
+      ao.dist[t] = (float)distances[t];
+      ao.beta[t] = betas[t];
+      ao.mcw[t] = max_curve_wts[t];
+      if (distances[t] == dmax) ao.max_mask |= 1u << t;
+    }
+    ao.maxdist = (float)max_seconds * speed_m_s;  // data.rs:532
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(g.mu);
+    DevGraph &dg = device_graph(g, dev);
+    DevEdges &de = device_edges(g, dg, speed_m_s);
+    OwnedStream stream;
+    DevBuf d_node_off(4ull * (N + 1)), d_node_mem(4ull * n_assign), d_mem(16ull * n_assign),
+        d_group_off(4ull * (n_entries + 1)), d_class(4ull * std::max<uint32_t>(n_entries, 1)),
+        d_src(4ull * std::max<size_t>(srcs.size(), 1)), d_count(8 * CTN), d_wt(8 * CTN), d_near(8ull * C * N);
+    auto up = [&](DevBuf &d, const void *hsrc, size_t bytes) {
+      if (bytes) hip_check(hipMemcpyAsync(d.p, hsrc, bytes, hipMemcpyHostToDevice, stream), "copy");
+    };
+    up(d_node_off, node_off.data(), 4ull * (N + 1));
+    up(d_node_mem, node_mem.data(), 4ull * n_assign);
+    up(d_mem, mem.data(), 16ull * n_assign);
+    up(d_group_off, group_off.data(), 4ull * (n_entries + 1));
+    up(d_class, entry_class, 4ull * n_entries);
+    up(d_src, srcs.data(), 4ull * srcs.size());
+    up(d_near, out_nearest, 8ull * C * N);  // NaN bits: no entry yet
+    hip_check(hipMemsetAsync(d_count.p, 0, 8 * CTN, stream), "memset");
+    hip_check(hipMemsetAsync(d_wt.p, 0, 8 * CTN, stream), "memset");
+    ao.node_off = d_node_off.as<uint32_t>();
+    ao.node_mem = d_node_mem.as<uint32_t>();
+    ao.mem = d_mem.as<uint4>();
+    ao.group_off = d_group_off.as<uint32_t>();
+    ao.entry_class = d_class.as<int32_t>();
+    ao.count = d_count.as<double>();
+    ao.weighted = d_wt.as<double>();
+    ao.nearest = d_near.as<unsigned long long>();
+    if (!srcs.empty())
+      launch_access(g, dg, de, (float)max_seconds, d_src.as<uint32_t>(), (uint32_t)srcs.size(), ao, stream, t_stats);
+    std::vector<double> hc(CTN), hw(CTN), hn((size_t)C * N);
+    hip_check(hipMemcpyAsync(hc.data(), d_count.p, 8 * CTN, hipMemcpyDeviceToHost, stream), "copy out");
+    hip_check(hipMemcpyAsync(hw.data(), d_wt.p, 8 * CTN, hipMemcpyDeviceToHost, stream), "copy out");
+    hip_check(hipMemcpyAsync(hn.data(), d_near.p, 8ull * C * N, hipMemcpyDeviceToHost, stream), "copy out");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    const uint32_t *ip = result_order(g);
+    unpack_acc(hc, N, C, T, out_count, ip);
+    unpack_acc(hw, N, C, T, out_weighted, ip);
+    unpack_acc(hn, N, C, 1, out_nearest, ip);
+    t_stats.sources = srcs.size();
+    g.progress = N;
+  });
+  t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
 int cs_tree_shortest(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, uint8_t *visited,
                      uint8_t *discovered, int32_t *pred, float *short_dist, float *agg_seconds, uint32_t *order,
                      uint32_t *n_order) {

@@ -222,6 +222,26 @@ struct TreeOut {
   uint32_t *n_preds = nullptr;
   uint32_t max_preds = 0;
 };
+// Land-use accessibility over each source's shortest-path tree (device
+// arrays, nodes in device order). A member is one assignment of a data entry
+// to a node; the members of one dedupe group lie together.
+struct AccessOut {
+  const uint32_t *node_off = nullptr;    // [N + 1] into node_mem
+  const uint32_t *node_mem = nullptr;    // member indices assigned to each node
+  const uint4 *mem = nullptr;            // {node, entry, data distance f32 bits, group}
+  const uint32_t *group_off = nullptr;   // [G + 1] member range of each dedupe group
+  const int32_t *entry_class = nullptr;  // class of each entry (-1: not counted)
+  double *count = nullptr;               // [C][T][N] entries within each distance
+  double *weighted = nullptr;            // [C][T][N] clipped negative-exponential weights
+  unsigned long long *nearest = nullptr; // [C][N] f64 bits of the nearest entry (NaN bits: none)
+  uint32_t T = 0, N = 0;
+  uint32_t max_mask = 0;  // thresholds equal to the largest distance
+  float maxdist = 0.0f;   // max walk seconds * speed
+  float dist[kMaxT], beta[kMaxT], mcw[kMaxT];
+};
+void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uint32_t *d_sources,
+                   uint32_t n_sources, const AccessOut &out, hipStream_t stream, cs_stats &st);
+
 // mode 0: single-predecessor tree, 1: Brandes multi-predecessor state.
 void launch_single(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec,
                    float tolerance, int mode, const TreeOut &out, hipStream_t stream);

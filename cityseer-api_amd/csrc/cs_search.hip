@@ -58,7 +58,7 @@ constexpr uint32_t kScanSpan = 16; // scan the index span only while it is at mo
 constexpr uint32_t kEpochMax = 1023u;       // 10-bit epoch tags 1..1023; 0 is never issued
 constexpr uint32_t kSlotLimit = 1u << 22;   // 22-bit slots in the node key
 
-enum Mode { kBrandes = 0, kSegment = 1, kTree = 2, kBrandesExport = 3 };
+enum Mode { kBrandes = 0, kSegment = 1, kTree = 2, kBrandesExport = 3, kAccess = 4 };
 // out-row candidates whose loads one fold step batches: two in the Brandes
 // search (room for five waves per SIMD), four in the trees and segment search
 template <int MODE>
@@ -119,6 +119,7 @@ struct SearchArgs {
   uint32_t *n_fail;
   uint32_t *fail_flags;
   TreeOut tout;
+  AccessOut acc;
 };
 
 // Wave-private workspace: only the owning wave reads or writes it, so
@@ -1311,6 +1312,53 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
           s0 = uniform(s0n);
         }
       }
+    } else if constexpr (MODE == kAccess) {
+      // ---- data entries reachable from the source (data.rs:520-615, 680-741).
+      // A member (entry e at node v, data distance dd) counts when v lies
+      // below the walk limit, secs_v * speed + dd is within the largest
+      // distance and no other member of its dedupe group is nearer (equal
+      // totals: the lower member index wins). Each group therefore counts
+      // once, through its nearest member, with no per-source map.
+      const AccessOut &ao = a.acc;
+      const uint32_t src = c.src;
+      for (uint32_t s = lane; s < R; s += kWave) {
+        const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+        const uint32_t v = (uint32_t)lo;
+        const float secs = __uint_as_float((uint32_t)(lo >> 32));
+        if (!(secs < j.maxsec)) continue;
+        const float net = __fmul_rn(secs, speed);
+        for (uint32_t k = ao.node_off[v]; k < ao.node_off[v + 1]; ++k) {
+          const uint32_t m = ao.node_mem[k];
+          const uint4 me = ao.mem[m];
+          const float tot = __fadd_rn(net, __uint_as_float(me.z));
+          if (!(tot <= ao.maxdist)) continue;
+          bool win = true;
+          for (uint32_t m2 = ao.group_off[me.w]; m2 < ao.group_off[me.w + 1] && win; ++m2) {
+            if (m2 == m) continue;
+            const uint4 o = ao.mem[m2];
+            const uint64_t ko = ld64(nkey(w, o.x));
+            if (!key_current(ko, c.tag)) continue;
+            const float so = key_secs(ko);
+            if (!(so < j.maxsec)) continue;
+            const float to = __fadd_rn(__fmul_rn(so, speed), __uint_as_float(o.z));
+            if (to <= ao.maxdist && (to < tot || (to == tot && m2 < m))) win = false;
+          }
+          if (!win) continue;
+          const int32_t cls = ao.entry_class[me.y];
+          if (cls < 0) continue;
+          for (uint32_t t = 0; t < ao.T; ++t) {
+            if (!(tot <= ao.dist[t])) continue;
+            const size_t row = ((size_t)cls * ao.T + t) * ao.N + src;
+            gadd_f64(&ao.count[row], 1.0);
+            // clipped_beta_wt (common.rs:339-348): min(exp(-beta d), mcw) / mcw in f32
+            const float raw = expf(__fmul_rn(-ao.beta[t], tot));
+            gadd_f64(&ao.weighted[row], (double)__fdiv_rn(fminf(raw, ao.mcw[t]), ao.mcw[t]));
+            if (ao.max_mask & (1u << t))
+              atomicMin(&ao.nearest[(size_t)cls * ao.N + src], (unsigned long long)__double_as_longlong((double)tot));
+          }
+        }
+      }
+      wave_fence();
     } else {
       // single-source exports (trees / Brandes state)
       for (uint32_t s = lane; s < R; s += kWave) {
@@ -1384,7 +1432,7 @@ int occupancy_one(uint32_t block, size_t lds) {
 
 template <int MODE>
 void launch_nt(uint32_t T, uint32_t blocks, uint32_t block, size_t lds, hipStream_t stream, const SearchArgs &a) {
-  if constexpr (MODE == kTree || MODE == kBrandesExport) {
+  if constexpr (MODE == kTree || MODE == kBrandesExport || MODE == kAccess) {
     launch_one<MODE, 1>(blocks, block, lds, stream, a);
   } else {
 #define CS_CALL(n) launch_one<MODE, n>(blocks, block, lds, stream, a)
@@ -1396,7 +1444,7 @@ void launch_nt(uint32_t T, uint32_t blocks, uint32_t block, size_t lds, hipStrea
 template <int MODE>
 int blocks_per_cu(uint32_t T, uint32_t block, size_t lds) {
   int nb = 0;
-  if constexpr (MODE == kTree || MODE == kBrandesExport) {
+  if constexpr (MODE == kTree || MODE == kBrandesExport || MODE == kAccess) {
     nb = occupancy_one<MODE, 1>(block, lds);
   } else {
 #define CS_CALL(n) nb = occupancy_one<MODE, n>(block, lds)
@@ -1668,7 +1716,8 @@ std::mutex g_tuning_mu;
 // workspace with a larger one.
 template <int MODE>
 void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune, const TreeOut *tout,
-              hipStream_t stream, cs_stats &st, const std::vector<uint2> *explicit_items = nullptr) {
+              hipStream_t stream, cs_stats &st, const std::vector<uint2> *explicit_items = nullptr,
+              const AccessOut *aout = nullptr) {
   constexpr bool kBr = (MODE == kBrandes || MODE == kBrandesExport);
   uint64_t n_items = 0;
   for (auto &js : specs) n_items += js.n_sources;
@@ -1780,6 +1829,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.fail_flags = d_ctl + kXcd + 1;
     a.fail_list = d_fail;
     if (tout) a.tout = *tout;
+    if (aout) a.acc = *aout;
     hip_check(hipEventRecord(e0, stream), "event record");
     launch_nt<MODE>(NT, waves / wpb, block, lds, stream, a);
     hip_check(hipGetLastError(), "k_search launch");
@@ -2175,6 +2225,21 @@ void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, d_acc, nullptr, tune.delta_scale};
   run_jobs<kSegment>({js}, cs.T, tune, nullptr, stream, st);
+}
+
+void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uint32_t *d_sources,
+                   uint32_t n_sources, const AccessOut &out, hipStream_t stream, cs_stats &st) {
+  CallSpec cs;
+  cs.T = 1;
+  cs.dist[0] = maxsec * de.speed;
+  cs.maxsec = maxsec;
+  cs.speed = de.speed;
+  cs.tolerance = kTieEps;
+  cs.closeness = false;
+  cs.betweenness = false;
+  const Tuning tune = current_tuning();
+  JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, nullptr, nullptr, tune.delta_scale};
+  run_jobs<kAccess>({js}, 1, tune, nullptr, stream, st, nullptr, &out);
 }
 
 void launch_single(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec, float tolerance,

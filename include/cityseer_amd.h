@@ -206,6 +206,32 @@ int cs_centrality_simplest(cs_graph *g, const cs_params *p, double *out, uint32_
 int cs_segment_centrality(cs_graph *g, const cs_params *p, double *out);
 
 /* ---------------------------------------------------------------------- */
+/* Data layer                                                             */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces DataMap.accessibility (rust/src/data.rs:617-747) with
+ * angular = false, over the assignment DataMap.node_data_map holds after
+ * assign_data_to_network (data.rs:381-387, 455-512), given here as n_assign
+ * (node, entry, distance) triples. Entry e (0..n_entries-1) belongs to dedupe
+ * group entry_group[e] (< n_entries; entries without a dedupe key form groups
+ * of their own) and land-use class entry_class[e] (0..n_classes-1, or -1 when
+ * its class is not among the accessibility keys). Per live source node the
+ * nearest member of each group within max_seconds * speed_m_s counts
+ * (data.rs:520-615); equal totals within a group go to the member given first.
+ * distances / betas / max_curve_wts: n_thresholds values as
+ * pair_distances_betas_time and clip_wts_curve return them; max_seconds is
+ * the largest of the thresholds' seconds.
+ * out_count, out_weighted: f64 [n_classes][n_thresholds][N] (unweighted and
+ * weighted accessibility); out_nearest: f64 [n_classes][N], the distance to
+ * the nearest counted entry within the largest distance (NaN: none). */
+int cs_accessibility(cs_graph *g, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
+                     uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
+                     const uint32_t *assign_entry, const double *assign_dist, uint32_t n_thresholds,
+                     const uint32_t *distances, const float *betas, const float *max_curve_wts,
+                     uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
+                     double *out_nearest);
+
+/* ---------------------------------------------------------------------- */
 /* Single-source trees                                                    */
 /* ---------------------------------------------------------------------- */
 

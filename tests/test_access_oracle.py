@@ -1,0 +1,115 @@
+"""Accessibility (host side, no GPU): the oracle's restatement pinned by hand-computed totals on a
+line graph, the weight-curve helpers by closed forms, and the DataMap argument checks the
+reference makes before any search (rust/src/data.rs:617-680, common.rs:311-348)."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from cityseer.rustalgos import data
+from cityseer.rustalgos.graph import NetworkStructure
+import oracle.oracle as O
+from oracle.access import accessibility as oracle_access
+from oracle.oracle import OracleGraph
+
+
+def line_graph(n=4, step=100.0):
+    ns = NetworkStructure()
+    for i in range(n):
+        ns.add_street_node(str(i), i * step, 0.0, True, 1.0)
+    for a in range(n - 1):
+        pa, pb = [a * step, 0.0], [(a + 1) * step, 0.0]
+        ns.add_street_edge(a, a + 1, 0, str(a), str(a + 1), [pa, pb])
+        ns.add_street_edge(a + 1, a, 0, str(a + 1), str(a), [pb, pa])
+    return ns
+
+
+# entries on the 4-node line 0-1-2-3 (100 m apart): (key, class, dedupe key, [(node, distance)])
+LINE_ENTRIES = [
+    ("A", "shop", None, [(1, 10.0)]),
+    ("B", "shop", None, [(3, 5.0)]),
+    ("C", "other", None, [(2, 0.0)]),
+    ("D", "shop", None, [(1, 30.0), (3, 2.0)]),
+    ("E", "shop", "g", [(2, 20.0)]),
+    ("F", "cafe", "g", [(0, 40.0)]),
+]
+
+
+def line_inputs():
+    node_data_map: dict[int, list] = {}
+    for key, _c, _d, assigned in LINE_ENTRIES:
+        for node, dist in assigned:
+            node_data_map.setdefault(node, []).append((key, dist))
+    dedupe_of = {k: (d if d is not None else k) for k, _c, d, _a in LINE_ENTRIES}
+    lu_class = {k: c for k, c, _d, _a in LINE_ENTRIES}
+    return node_data_map, dedupe_of, lu_class
+
+
+def line_expected(speed, dists, betas):
+    """Totals worked by hand: walk 250 s at `speed` m/s from each end node."""
+    # from node 0 (node 3 lies beyond 250 s at 1 m/s): A 110, D 130 (via node 1), group g: F 40 beats E 220
+    # from node 3 (node 0 lies beyond): B 5, D 2 (via node 3), A 210, group g: E 120 only
+    reach = {0: {"shop": [110.0, 130.0], "cafe": [40.0]}, 3: {"shop": [5.0, 2.0, 210.0, 120.0], "cafe": []}}
+    out = {}
+    for src, per in reach.items():
+        for cls, totals in per.items():
+            unw = [sum(1 for t in totals if t <= d) for d in dists]
+            wtd = [sum(math.exp(-b * t) for t in totals if t <= d) for d, b in zip(dists, betas, strict=True)]
+            near = min(totals) if totals else math.nan
+            out[(src, cls)] = (unw, wtd, near)
+    return out
+
+
+def test_oracle_line_graph_by_hand():
+    O.set_heap_mode(False)
+    ns = line_graph()
+    og = OracleGraph.from_network_structure(ns)
+    speed, dists = 1.0, [100, 250]
+    _d, betas, secs = O.thresholds(speed, distances=dists)
+    assert max(secs) == 250
+    ndm, dedupe_of, lu_class = line_inputs()
+    res = oracle_access(og, [1, 1, 1, 1], ndm, dedupe_of, lu_class, ["shop", "cafe"], dists, betas, [1.0, 1.0],
+                        max(secs), speed)
+    for (src, cls), (unw, wtd, near) in line_expected(speed, dists, betas).items():
+        got_u, got_w, got_n = res[cls]
+        assert got_u[:, src].tolist() == unw, (src, cls)
+        assert np.allclose(got_w[:, src], wtd, rtol=1e-6), (src, cls)
+        assert (math.isnan(near) and math.isnan(got_n[src])) or got_n[src] == near, (src, cls)
+    # from node 1: A 10, D 30 (not 202 via node 3), E 120 (nearer than F 140, so group g counts
+    # as "shop" and "cafe" counts nothing), B 205
+    assert res["shop"][0][1, 1] == 4.0 and res["cafe"][0][1, 1] == 0.0
+
+
+def test_weight_curve_helpers():
+    assert data.clip_wts_curve([400, 800], [0.01, 0.005], 0) == [1.0, 1.0]
+    mcw = data.clip_wts_curve([400], [0.01], 50)[0]
+    assert math.isclose(mcw, math.exp(-0.5), rel_tol=1e-6)
+    # within the tolerance the weight is clipped to 1; beyond, exp(-beta d) / mcw
+    assert data.clipped_beta_wt(0.01, mcw, 20.0) == 1.0
+    assert math.isclose(data.clipped_beta_wt(0.01, mcw, 150.0), math.exp(-1.5) / math.exp(-0.5), rel_tol=1e-6)
+    with pytest.raises(ValueError, match="Clipping distance"):
+        data.clip_wts_curve([40], [0.1], 50)
+    with pytest.raises(ValueError, match="range"):
+        data.clipped_beta_wt(0.01, 1.5, 1.0)
+
+
+def test_datamap_argument_checks():
+    ns = line_graph()
+    dm = data.DataMap()
+    dm.insert("a", "POINT (0 0)")
+    dm.insert("b", "POINT (1 0)", dedupe_key_py="x")
+    assert dm.entry_keys() == ["str:a", "str:b"] and dm.count() == 2 and not dm.is_empty()
+    assert dm.get_entry("str:b").dedupe_key == "str:x"
+    with pytest.raises(ValueError, match="number of landuse encodings"):
+        dm.accessibility(ns, {"a": "shop"}, ["shop"], distances=[200])
+    with pytest.raises(KeyError, match="Data entries key missing"):
+        dm.accessibility(ns, {"a": "shop", "c": "shop"}, ["shop"], distances=[200])
+    with pytest.raises(ValueError, match="exactly one"):
+        dm.accessibility(ns, {"a": "shop", "b": "shop"}, ["shop"])
+    with pytest.raises(NotImplementedError):
+        dm.assign_data_to_network(ns, 400.0, 6)
+    with pytest.raises(ValueError, match="dual"):
+        dm.accessibility(ns, {"a": "shop", "b": "shop"}, ["shop"], distances=[200], angular=True)

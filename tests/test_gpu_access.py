@@ -1,0 +1,115 @@
+"""GPU: land-use accessibility (DataMap.accessibility, rust/src/data.rs:617-747) through the C-ABI,
+against hand-computed totals and against the oracle's restatement on random assignments.
+
+Counts and nearest distances are bit-exact; weighted sums are f64 sums of the same f32 weights
+in another order, compared at 1e-9 relative (plus ulp-level differences between exp
+implementations: 1e-5 relative, the bar for f32 metrics)."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from cityseer.rustalgos import data
+from cityseer.tools import mock
+import oracle.oracle as O
+from oracle.access import accessibility as oracle_access
+from oracle.oracle import OracleGraph
+from test_access_oracle import LINE_ENTRIES, line_expected, line_graph
+from test_gpu_shortest import say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _keyed():
+    O.set_heap_mode(False)
+
+
+def _datamap(entries):
+    """entries: (key, class, dedupe key, [(node, distance)]) -> DataMap and its landuse map."""
+    dm = data.DataMap()
+    for key, _c, dd, _a in entries:
+        dm.insert(key, "POINT (0 0)", dd)
+    for key, _c, _d, assigned in entries:
+        for node, dist in assigned:
+            dm.node_data_map.setdefault(node, []).append((f"str:{key}", dist))
+    return dm, {key: cls for key, cls, _d, _a in entries}
+
+
+def test_line_graph_by_hand():
+    ns = line_graph()
+    dm, lu = _datamap(LINE_ENTRIES)
+    speed, dists = 1.0, [100, 250]
+    say("accessibility on a 4-node line, hand-computed totals")
+    res = dm.accessibility(ns, lu, ["shop", "cafe"], distances=dists, speed_m_s=speed)
+    _d, bets, _s = O.thresholds(speed, distances=dists)
+    for (src, cls), (unw, wtd, near) in line_expected(speed, dists, bets).items():
+        la = res.result[cls]
+        assert [la.unweighted[d][src] for d in dists] == unw, (src, cls)
+        assert np.allclose([la.weighted[d][src] for d in dists], wtd, rtol=1e-6), (src, cls)
+        got_n = la.distance[250][src]
+        assert (math.isnan(near) and math.isnan(got_n)) or got_n == near, (src, cls)
+    assert res.result["shop"].unweighted[250][1] == 4.0 and res.result["cafe"].unweighted[250][1] == 0.0
+
+
+def _random_case(ns, n_entries, seed, classes=("shop", "cafe", "park", "other")):
+    rng = np.random.default_rng(seed)
+    n = ns.node_count()
+    entries = []
+    for e in range(n_entries):
+        k = int(rng.integers(1, 4))
+        nodes = rng.choice(n, k, replace=False).tolist()
+        assigned = [(int(v), float(rng.uniform(0.0, 60.0))) for v in nodes]
+        dd = f"g{int(rng.integers(0, n_entries // 4))}" if rng.random() < 0.3 else None
+        entries.append((f"e{e}", classes[int(rng.integers(0, len(classes)))], dd, assigned))
+    return entries
+
+
+def _compare(ns, entries, acc_keys, dists, speed, live=None, spatial_tolerance=0):
+    dm, lu = _datamap(entries)
+    res = dm.accessibility(ns, lu, acc_keys, distances=dists, speed_m_s=speed, spatial_tolerance=spatial_tolerance)
+    og = OracleGraph.from_network_structure(ns)
+    d_, bets, secs = O.thresholds(speed, distances=dists)
+    mcw = data.clip_wts_curve(dists, bets, spatial_tolerance)
+    ndm = {v: [(k[4:], dist) for k, dist in pairs] for v, pairs in dm.node_data_map.items()}
+    dedupe_of = {key: (dd if dd is not None else key) for key, _c, dd, _a in entries}
+    ref = oracle_access(og, live if live is not None else [1] * og.n, ndm, dedupe_of, lu, acc_keys, dists, bets,
+                        mcw, max(secs), speed)
+    for cls in acc_keys:
+        la = res.result[cls]
+        r_u, r_w, r_n = ref[cls]
+        for i, d in enumerate(dists):
+            assert np.array_equal(la.unweighted[d], r_u[i]), f"{cls} count at {d}"
+            np.testing.assert_allclose(la.weighted[d], r_w[i], rtol=1e-5, atol=1e-12, err_msg=f"{cls} weighted {d}")
+        assert np.array_equal(la.distance[max(dists)], r_n, equal_nan=True), f"{cls} nearest"
+    return res
+
+
+def test_random_assignments_lattice():
+    lat = mock.perturbed_lattice(24, seed=201, diag_frac=0.1)
+    ns = mock.network_structure_from_lattice(lat)
+    entries = _random_case(ns, 900, 202)
+    say(f"accessibility, {ns.node_count()} nodes, {len(entries)} entries, thresholds [200, 500, 900]")
+    res = _compare(ns, entries, ["shop", "cafe", "park"], [200, 500, 900], 1.33333)
+    assert sum(res.result["shop"].unweighted[900]) > 0
+
+
+def test_random_assignments_spatial_tolerance_and_betas():
+    lat = mock.perturbed_lattice(16, seed=203)
+    ns = mock.network_structure_from_lattice(lat)
+    entries = _random_case(ns, 300, 204)
+    say(f"accessibility with spatial_tolerance 30, {ns.node_count()} nodes")
+    _compare(ns, entries, ["park", "shop"], [150, 400], 1.1, spatial_tolerance=30)
+
+
+def test_no_assignments_and_unknown_keys():
+    ns = line_graph()
+    dm, lu = _datamap([("A", "shop", None, [])])
+    dm.node_data_map[2] = [("str:missing", 1.0)]
+    say("accessibility with no usable assignment")
+    res = dm.accessibility(ns, lu, ["shop", "none"], distances=[300])
+    assert np.all(res.result["shop"].unweighted[300] == 0.0)
+    assert np.all(np.isnan(res.result["none"].distance[300]))
