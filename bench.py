@@ -14,6 +14,10 @@ impedance) and the primal (segment) of a 150x150 perturbed lattice at
 [500, 1000, 2000] m through the NetworkStructure methods; ``value`` uses the
 kernel time of each call (graph resident, sources staged before the kernel).
 
+``--mode access``: land-use accessibility (DataMap.accessibility) on the
+200x200 lattice at [400, 800] m over about two synthetic data entries per node;
+``value`` uses the kernel time of the search-and-aggregate launch.
+
 Prints one JSON line.
 """
 
@@ -208,6 +212,146 @@ def run_variant(args) -> int:
 
 
 BATCH_DISTANCES = [400, 800, 1600, 3200, 6400]
+ACCESS_DISTANCES = [400, 800]
+ACCESS_CLASSES = ["shop", "cafe", "park", "other"]
+
+
+def access_workload(ns, lat, per_node: float, seed: int):
+    """Synthetic data entries: about `per_node` per street node, each assigned to a random node and
+    up to two of its neighbours (distances 0-60 m), 30 % sharing a dedupe key, four classes."""
+    import numpy as np
+
+    from cityseer.rustalgos import data
+
+    rng = np.random.default_rng(seed)
+    n = ns.node_count()
+    nbrs: list[list[int]] = [[] for _ in range(n)]
+    for a, b in lat.edges.tolist():
+        nbrs[a].append(b)
+        nbrs[b].append(a)
+    n_entries = int(per_node * n)
+    dm = data.DataMap()
+    lu = {}
+    for e in range(n_entries):
+        key = f"e{e}"
+        dd = f"g{int(rng.integers(0, n_entries // 4))}" if rng.random() < 0.3 else None
+        dm.insert(key, "POINT (0 0)", dd)
+        lu[key] = ACCESS_CLASSES[int(rng.integers(0, len(ACCESS_CLASSES)))]
+        v = int(rng.integers(0, n))
+        nodes = [v] + nbrs[v][: int(rng.integers(0, 3))]
+        for u in nodes:
+            dm.node_data_map.setdefault(u, []).append((f"str:{key}", float(rng.uniform(0.0, 60.0))))
+    return dm, lu
+
+
+def access_cpu_baseline(ns, dm, lu, target_s: float):
+    """The oracle's restatement (one thread, Python aggregation over the C tree search) over a
+    bounded random sample of sources."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from oracle.access import accessibility as oracle_access
+    from oracle.oracle import OracleGraph
+
+    O.set_heap_mode(False)
+    og = OracleGraph.from_network_structure(ns)
+    _d, bets, secs = O.thresholds(1.33333, distances=ACCESS_DISTANCES)
+    ndm = {v: [(k[4:], d) for k, d in pairs] for v, pairs in dm.node_data_map.items()}
+    dedupe_of = {k[4:]: e.dedupe_key[4:] for k, e in dm.entries.items()}
+    rng = np.random.default_rng(321)
+    n = ns.node_count()
+
+    def run(k):
+        src = np.sort(rng.choice(n, k, replace=False)).tolist()
+        t0 = time.perf_counter()
+        oracle_access(og, [1] * n, ndm, dedupe_of, lu, ACCESS_CLASSES[:3], ACCESS_DISTANCES, bets,
+                      [1.0] * len(ACCESS_DISTANCES), max(secs), 1.33333, sources=src)
+        return time.perf_counter() - t0
+
+    k0 = 32
+    rate = k0 / max(run(k0), 1e-6)
+    k = int(min(n, max(k0, rate * target_s)))
+    t = run(k)
+    return {
+        "value": round(k / t, 2),
+        "unit": "source searches/sec",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{k} random sources of the same workload ({t:.1f} s wall, 1 thread)",
+    }
+
+
+def run_access(args) -> int:
+    """Land-use accessibility (DataMap.accessibility) on the 200x200 perturbed lattice at
+    [400, 800] m: about two data entries per node, every live node a source."""
+    import numpy as np
+
+    from cityseer import _native
+    from cityseer.tools import mock
+
+    log("building access workload")
+    lat = mock.perturbed_lattice(args.side, seed=args.seed, diag_frac=args.diag)
+    ns = mock.network_structure_from_lattice(lat)
+    n, m = ns.node_count(), ns.edge_count
+    dm, lu = access_workload(ns, lat, 2.0, args.seed + 1)
+    n_assign = sum(len(p) for p in dm.node_data_map.values())
+
+    def step():
+        dm.accessibility(ns, lu, ACCESS_CLASSES[:3], distances=ACCESS_DISTANCES)
+        return _native.last_stats()
+
+    for _ in range(args.warmup):
+        step()
+    kernel_ms, total_ms = [], []
+    stats = None
+    for i in range(args.steps):
+        stats = step()
+        kernel_ms.append(stats["kernel_ms"])
+        total_ms.append(stats["total_ms"])
+        log(f"step {i}: kernel {stats['kernel_ms']:.2f} ms, call {stats['total_ms']:.2f} ms")
+    kms = float(np.mean(kernel_ms))
+    # search bytes only (the aggregation's reads of the assignment tables are not counted)
+    algo_bytes = 12 * stats["edges_scanned"] + 16 * stats["nodes_reached"]
+    achieved = algo_bytes / (kms / 1000.0) / 1e9
+    cpu = None if args.no_cpu else access_cpu_baseline(ns, dm, lu, args.cpu_seconds)
+    line = {
+        "metric": METRIC,
+        "value": round(n / (kms / 1000.0), 2),
+        "unit": "source searches/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(kms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 accumulators)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.side}x{args.side} perturbed lattice, DataMap.accessibility {ACCESS_DISTANCES} m, "
+                        f"{dm.count()} entries ({n_assign} node assignments), 3 of 4 classes, all sources",
+            "nodes": n,
+            "directed_edges": m,
+            "sources": n,
+            "mode": "access",
+            "call_ms": round(float(np.mean(total_ms)), 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel_ms": round(kms, 3),
+            "algorithmic_bytes": int(algo_bytes),
+            "edges_scanned": int(stats["edges_scanned"]),
+            "nodes_reached": int(stats["nodes_reached"]),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def batch_workload(n_graphs: int, seed: int = 0):
@@ -551,7 +695,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["plain", "simplest", "segment", "batch", "serve"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "simplest", "segment", "batch", "serve", "access"], default="plain")
     ap.add_argument("--requests", type=int, default=40, help="serve mode: requests in the stream")
     ap.add_argument("--load", type=float, default=0.6,
                     help="serve mode: Poisson arrival rate as a fraction of the served saturation rate")
@@ -584,6 +728,8 @@ def main() -> int:
         return run_batch(args)
     if args.mode == "serve":
         return run_serve(args)
+    if args.mode == "access":
+        return run_access(args)
 
     log("building workload")
     lat, ns = build_workload(args.side, args.diag, args.seed)

@@ -43,12 +43,13 @@ def aggregate_to_src_idx(tree, node_data_map, dedupe_of, max_walk_seconds, speed
 
 
 def accessibility(og, live, node_data_map, dedupe_of, lu_class, accessibility_keys, distances, betas, mcws,
-                  max_walk_seconds, speed):
-    """{class: (unweighted f64[T][N], weighted f64[T][N], nearest f64[N])} (data.rs:680-741)."""
+                  max_walk_seconds, speed, sources=None):
+    """{class: (unweighted f64[T][N], weighted f64[T][N], nearest f64[N])} (data.rs:680-741);
+    `sources` restricts the rows computed (all nodes by default)."""
     n, T = og.n, len(distances)
     max_dist = max(distances)
     res = {k: (np.zeros((T, n)), np.zeros((T, n)), np.full(n, np.nan)) for k in accessibility_keys}
-    for src in range(n):
+    for src in (range(n) if sources is None else sources):
         if not live[src]:
             continue
         tree = og.tree_shortest(src, max_walk_seconds, speed)
