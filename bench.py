@@ -342,7 +342,7 @@ def run_access(args) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic("traffic_bytes_per_launch", "k_search_access_cfg2_hbm.md"),
             "kernel_ms": round(kms, 3),
             "algorithmic_bytes": int(algo_bytes),
             "edges_scanned": int(stats["edges_scanned"]),
