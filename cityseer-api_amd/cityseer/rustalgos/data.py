@@ -10,8 +10,9 @@ The assignment of entries to nodes is ``node_data_map``: ``{node index:
 (data.rs:455-512). Here it is set by the caller; the geometric assignment
 itself (R-tree candidates, barrier and street-crossing checks,
 graph.rs:1346-1535) is not part of this package and raises
-``NotImplementedError``, as do ``mixed_uses``, ``stats`` and angular
-accessibility.
+``NotImplementedError``, as do ``stats`` and the angular variants.
+``mixed_uses`` (data.rs:749-944) shares the launch and adds a per-node
+diversity kernel over the class counts.
 """
 
 from __future__ import annotations
@@ -77,6 +78,37 @@ class LanduseAccess:
         return {self._max: self._nearest[self._idx].copy()}
 
 
+class MixedUsesResult:
+    """Result of ``DataMap.mixed_uses`` (data.rs:101-175): ``hill`` / ``hill_weighted`` as
+    ``{q: {distance: f64[N]}}`` for q in 0, 1, 2; ``shannon`` / ``gini`` as ``{distance: f64[N]}``."""
+
+    def __init__(self, distances, node_keys_py, node_indices, rows: np.ndarray):
+        self.distances = list(distances)
+        self.node_keys_py = list(node_keys_py)
+        self.node_indices = list(node_indices)
+        self._rows = rows  # [8][T][N]
+        self._idx = np.asarray(self.node_indices, dtype=np.int64)
+
+    def _load(self, m: int) -> dict[int, np.ndarray]:
+        return {int(d): self._rows[m][i][self._idx].copy() for i, d in enumerate(self.distances)}
+
+    @property
+    def hill(self) -> dict[int, dict[int, np.ndarray]]:
+        return {q: self._load(q) for q in (0, 1, 2)}
+
+    @property
+    def hill_weighted(self) -> dict[int, dict[int, np.ndarray]]:
+        return {q: self._load(3 + q) for q in (0, 1, 2)}
+
+    @property
+    def shannon(self) -> dict[int, np.ndarray]:
+        return self._load(6)
+
+    @property
+    def gini(self) -> dict[int, np.ndarray]:
+        return self._load(7)
+
+
 class AccessibilityResult:
     """Result of ``DataMap.accessibility`` (data.rs:46-99)."""
 
@@ -133,9 +165,6 @@ class DataMap:
             "{node index: [(data key, distance), ...]} instead"
         )
 
-    def mixed_uses(self, *args, **kwargs):
-        raise NotImplementedError("mixed_uses is not available here")
-
     def stats(self, *args, **kwargs):
         raise NotImplementedError("stats is not available here")
 
@@ -157,6 +186,74 @@ class DataMap:
                 dists.append(float(dist))
         return (keys, entry_group, np.asarray(nodes, dtype=np.uint32), np.asarray(ents, dtype=np.uint32),
                 np.asarray(dists, dtype=np.float64))
+
+    def _landuses(self, landuses_map: dict) -> dict[str, str]:
+        if len(landuses_map) != self.count():
+            raise ValueError("The number of landuse encodings must match the number of data points")
+        lu: dict[str, str] = {}
+        for k, v in landuses_map.items():
+            ck = _composite_key(k)
+            if ck not in self.entries:
+                raise KeyError(f"Data entries key missing: {ck}")
+            lu[ck] = str(v)
+        return lu
+
+    def mixed_uses(
+        self,
+        network_structure,
+        landuses_map: dict,
+        distances: list[int] | None = None,
+        betas: list[float] | None = None,
+        minutes: list[float] | None = None,
+        compute_hill: bool | None = None,
+        compute_hill_weighted: bool | None = None,
+        compute_shannon: bool | None = None,
+        compute_gini: bool | None = None,
+        angular: bool | None = None,
+        spatial_tolerance: int | None = None,
+        min_threshold_wt: float | None = None,
+        speed_m_s: float | None = None,
+        pbar_disabled: bool | None = None,
+    ) -> MixedUsesResult:
+        """Land-use diversity of what each node reaches (data.rs:749-944)."""
+        from . import pair_distances_betas_time
+
+        if angular:
+            network_structure.validate_dual_for_angular("mixed_uses")
+            raise NotImplementedError("angular mixed uses are not available here")
+        speed = WALKING_SPEED if speed_m_s is None else float(speed_m_s)
+        dists, bets, secs = pair_distances_betas_time(speed, distances, betas, minutes, min_threshold_wt)
+        lu = self._landuses(landuses_map)
+        flags = [True if compute_hill is None else bool(compute_hill),
+                 True if compute_hill_weighted is None else bool(compute_hill_weighted),
+                 bool(compute_shannon), bool(compute_gini)]
+        if not any(flags):
+            raise ValueError("One of the compute_<measure> flags must be True, but all are currently False.")
+        mcw = clip_wts_curve(dists, bets, int(spatial_tolerance or 0))
+        classes = sorted(set(lu.values()))
+        cls_index = {k: i for i, k in enumerate(classes)}
+        keys, entry_group, a_node, a_ent, a_dist = self._assignment_arrays()
+        entry_class = np.asarray([cls_index[lu[k]] if k in lu else -1 for k in keys], dtype=np.int32)
+        n, T = network_structure.node_bound(), len(dists)
+        rows = np.zeros((8, T, n), dtype=np.float64)
+        p = _native.ptr
+        self.progress_init()
+        _native.check(
+            _native.load().cs_mixed_uses(
+                network_structure._native_handle(), len(keys), p(entry_group, ctypes.c_uint32),
+                p(entry_class, ctypes.c_int32), len(classes), len(a_node), p(a_node, ctypes.c_uint32),
+                p(a_ent, ctypes.c_uint32), p(a_dist, ctypes.c_double), T,
+                p(np.asarray(dists, dtype=np.uint32), ctypes.c_uint32),
+                p(np.asarray(bets, dtype=np.float32), ctypes.c_float),
+                p(np.asarray(mcw, dtype=np.float32), ctypes.c_float), int(max(secs)), speed, p(rows, ctypes.c_double),
+            )
+        )
+        self._progress = n
+        # measures not asked for stay zero (data.rs:866-941 only adds the requested ones)
+        for on, ms in zip(flags, ((0, 1, 2), (3, 4, 5), (6,), (7,)), strict=True):
+            if not on:
+                rows[list(ms)] = 0.0
+        return MixedUsesResult(dists, network_structure.node_keys_py(), network_structure.node_indices(), rows)
 
     def accessibility(
         self,
@@ -180,14 +277,7 @@ class DataMap:
             raise NotImplementedError("angular accessibility is not available here")
         speed = WALKING_SPEED if speed_m_s is None else float(speed_m_s)
         dists, bets, secs = pair_distances_betas_time(speed, distances, betas, minutes, min_threshold_wt)
-        if len(landuses_map) != self.count():
-            raise ValueError("The number of landuse encodings must match the number of data points")
-        lu: dict[str, str] = {}
-        for k, v in landuses_map.items():
-            ck = _composite_key(k)
-            if ck not in self.entries:
-                raise KeyError(f"Data entries key missing: {ck}")
-            lu[ck] = str(v)
+        lu = self._landuses(landuses_map)
         mcw = clip_wts_curve(dists, bets, int(spatial_tolerance or 0))
         acc_keys = list(accessibility_keys)
         cls_index = {k: i for i, k in enumerate(acc_keys)}

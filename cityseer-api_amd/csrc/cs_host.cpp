@@ -1131,12 +1131,14 @@ uint32_t settle_order(const std::vector<float> &secs, uint32_t *order) {
 
 extern "C" {
 
-int cs_accessibility(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
-                     uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
-                     const uint32_t *assign_entry, const double *assign_dist, uint32_t n_thresholds,
-                     const uint32_t *distances, const float *betas, const float *max_curve_wts,
-                     uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
-                     double *out_nearest) {
+// Accessibility rows (out_count / out_weighted / out_nearest) or, with out_div,
+// the diversity rows of the same per-source class counts instead.
+static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
+                       uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
+                       const uint32_t *assign_entry, const double *assign_dist, uint32_t n_thresholds,
+                       const uint32_t *distances, const float *betas, const float *max_curve_wts,
+                       uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
+                       double *out_nearest, double *out_div) {
   const auto t0 = std::chrono::steady_clock::now();
   t_stats = cs_stats{};
   int rc = guarded([&] {
@@ -1148,7 +1150,8 @@ int cs_accessibility(cs_graph *h, uint32_t n_entries, const uint32_t *entry_grou
     if (n_entries && (!entry_group || !entry_class)) fail(CS_EINVAL, "entry_group and entry_class must not be NULL");
     if (n_assign && (!assign_node || !assign_entry || !assign_dist))
       fail(CS_EINVAL, "assign_node, assign_entry and assign_dist must not be NULL");
-    if ((size_t)C * N && (!out_count || !out_weighted || !out_nearest)) fail(CS_EINVAL, "outputs must not be NULL");
+    if (!out_div && (size_t)C * N && (!out_count || !out_weighted || !out_nearest))
+      fail(CS_EINVAL, "outputs must not be NULL");
     if (!std::isfinite(speed_m_s) || !(speed_m_s > 0.0f)) fail(CS_EINVAL, "speed_m_s must be finite and positive");
     for (uint32_t t = 0; t < T; ++t)
       if (!(max_curve_wts[t] >= 0.0f && max_curve_wts[t] <= 1.0f) || max_curve_wts[t] == 0.0f)
@@ -1164,9 +1167,13 @@ int cs_accessibility(cs_graph *h, uint32_t n_entries, const uint32_t *entry_grou
         fail(CS_EINVAL, "assignment distances must be finite and non-negative");
     }
     const size_t CTN = (size_t)C * T * N;
-    std::fill(out_count, out_count + CTN, 0.0);
-    std::fill(out_weighted, out_weighted + CTN, 0.0);
-    std::fill(out_nearest, out_nearest + (size_t)C * N, std::nan(""));
+    if (out_div) {
+      std::fill(out_div, out_div + (size_t)kDivMetrics * T * N, 0.0);
+    } else {
+      std::fill(out_count, out_count + CTN, 0.0);
+      std::fill(out_weighted, out_weighted + CTN, 0.0);
+      std::fill(out_nearest, out_nearest + (size_t)C * N, std::nan(""));
+    }
     if (N == 0 || C == 0 || n_assign == 0) return;
     // members: assignments grouped by dedupe group, assignment order within a group
     std::vector<uint32_t> gcount(n_entries + 1, 0);
@@ -1228,7 +1235,10 @@ int cs_accessibility(cs_graph *h, uint32_t n_entries, const uint32_t *entry_grou
     up(d_group_off, group_off.data(), 4ull * (n_entries + 1));
     up(d_class, entry_class, 4ull * n_entries);
     up(d_src, srcs.data(), 4ull * srcs.size());
-    up(d_near, out_nearest, 8ull * C * N);  // NaN bits: no entry yet
+    {
+      const std::vector<double> nan_rows((size_t)C * N, std::nan(""));  // NaN bits: no entry yet
+      up(d_near, nan_rows.data(), 8ull * C * N);
+    }
     hip_check(hipMemsetAsync(d_count.p, 0, 8 * CTN, stream), "memset");
     hip_check(hipMemsetAsync(d_wt.p, 0, 8 * CTN, stream), "memset");
     ao.node_off = d_node_off.as<uint32_t>();
@@ -1241,12 +1251,23 @@ int cs_accessibility(cs_graph *h, uint32_t n_entries, const uint32_t *entry_grou
     ao.nearest = d_near.as<unsigned long long>();
     if (!srcs.empty())
       launch_access(g, dg, de, (float)max_seconds, d_src.as<uint32_t>(), (uint32_t)srcs.size(), ao, stream, t_stats);
+    const uint32_t *ip = result_order(g);
+    if (out_div) {
+      DevBuf d_div(8ull * kDivMetrics * T * N);
+      launch_diversity(ao, C, d_div.as<double>(), stream);
+      std::vector<double> hd((size_t)kDivMetrics * T * N);
+      hip_check(hipMemcpyAsync(hd.data(), d_div.p, 8 * hd.size(), hipMemcpyDeviceToHost, stream), "copy out");
+      hip_check(hipStreamSynchronize(stream), "sync");
+      unpack_acc(hd, N, kDivMetrics, T, out_div, ip);
+      t_stats.sources = srcs.size();
+      g.progress = N;
+      return;
+    }
     std::vector<double> hc(CTN), hw(CTN), hn((size_t)C * N);
     hip_check(hipMemcpyAsync(hc.data(), d_count.p, 8 * CTN, hipMemcpyDeviceToHost, stream), "copy out");
     hip_check(hipMemcpyAsync(hw.data(), d_wt.p, 8 * CTN, hipMemcpyDeviceToHost, stream), "copy out");
     hip_check(hipMemcpyAsync(hn.data(), d_near.p, 8ull * C * N, hipMemcpyDeviceToHost, stream), "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
-    const uint32_t *ip = result_order(g);
     unpack_acc(hc, N, C, T, out_count, ip);
     unpack_acc(hw, N, C, T, out_weighted, ip);
     unpack_acc(hn, N, C, 1, out_nearest, ip);
@@ -1255,6 +1276,27 @@ int cs_accessibility(cs_graph *h, uint32_t n_entries, const uint32_t *entry_grou
   });
   t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
+}
+
+int cs_accessibility(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
+                     uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
+                     const uint32_t *assign_entry, const double *assign_dist, uint32_t n_thresholds,
+                     const uint32_t *distances, const float *betas, const float *max_curve_wts,
+                     uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
+                     double *out_nearest) {
+  return access_impl(h, n_entries, entry_group, entry_class, n_classes, n_assign, assign_node, assign_entry,
+                     assign_dist, n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, out_count,
+                     out_weighted, out_nearest, nullptr);
+}
+
+int cs_mixed_uses(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
+                  uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node, const uint32_t *assign_entry,
+                  const double *assign_dist, uint32_t n_thresholds, const uint32_t *distances, const float *betas,
+                  const float *max_curve_wts, uint32_t max_seconds, float speed_m_s, double *out) {
+  if (!out) return guarded([] { fail(CS_EINVAL, "out must not be NULL"); });
+  return access_impl(h, n_entries, entry_group, entry_class, n_classes, n_assign, assign_node, assign_entry,
+                     assign_dist, n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, nullptr,
+                     nullptr, nullptr, out);
 }
 
 int cs_tree_shortest(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, uint8_t *visited,

@@ -241,6 +241,11 @@ struct AccessOut {
 };
 void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uint32_t *d_sources,
                    uint32_t n_sources, const AccessOut &out, hipStream_t stream, cs_stats &st);
+// Diversity of each source's class counts (the rows launch_access leaves in
+// out.count / out.nearest, n_classes of them): div [kDivMetrics][T][N] =
+// hill q0, q1, q2, distance-weighted hill q0, q1, q2, shannon, gini-simpson.
+constexpr uint32_t kDivMetrics = 8;
+void launch_diversity(const AccessOut &out, uint32_t n_classes, double *div, hipStream_t stream);
 
 // mode 0: single-predecessor tree, 1: Brandes multi-predecessor state.
 void launch_single(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec,

@@ -2242,6 +2242,77 @@ void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uin
   run_jobs<kAccess>({js}, 1, tune, nullptr, stream, st, nullptr, &out);
 }
 
+// One thread per (threshold, node): the diversity measures of the class
+// counts within that threshold (diversity.rs:6-175, 341-385), in f32 with the
+// classes summed in index order; a measure the reference reports as an error
+// (non-finite) is 0, as mixed_uses' unwrap_or(0.0) makes it (data.rs:866-941).
+__global__ void k_diversity(AccessOut ao, uint32_t C, double *div) {
+  const uint32_t T = ao.T, N = ao.N;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)T * N) return;
+  const uint32_t t = (uint32_t)(i / N), v = (uint32_t)(i % N);
+  float out[kDivMetrics];
+  for (uint32_t k = 0; k < kDivMetrics; ++k) out[k] = 0.0f;
+  uint32_t num = 0, rich = 0;
+  for (uint32_t c = 0; c < C; ++c) {
+    const uint32_t n = (uint32_t)ao.count[((size_t)c * T + t) * N + v];
+    num += n;
+    rich += n > 0;
+  }
+  if (num > 0) {
+    const float numf = (float)num;
+    const float beta = ao.beta[t], mcw = ao.mcw[t];
+    float ent = 0.0f, sq = 0.0f, agg = 0.0f, lam = 0.0f;
+    const float nm1 = (float)(num - 1);
+    for (uint32_t c = 0; c < C; ++c) {
+      const uint32_t n = (uint32_t)ao.count[((size_t)c * T + t) * N + v];
+      if (n == 0) continue;
+      const float p = __fdiv_rn((float)n, numf);
+      ent = __fadd_rn(ent, __fmul_rn(p, logf(p)));
+      sq = __fadd_rn(sq, powf(p, 2.0f));
+      const float d = (float)__longlong_as_double((long long)ao.nearest[(size_t)c * N + v]);
+      const float w = __fdiv_rn(fminf(expf(__fmul_rn(-beta, d)), mcw), mcw);
+      agg = __fadd_rn(agg, __fmul_rn(w, p));
+      if (num >= 2) lam = __fadd_rn(lam, __fmul_rn(p, __fdiv_rn(fmaxf(__fsub_rn((float)n, 1.0f), 0.0f), nm1)));
+    }
+    out[0] = (float)rich;
+    const float h1 = expf(-ent), h2 = powf(sq, -1.0f);
+    out[1] = isfinite(h1) ? h1 : 0.0f;
+    out[2] = isfinite(sq) && isfinite(h2) ? h2 : 0.0f;
+    if (!(fabsf(agg) < __FLT_EPSILON__)) {
+      float s0 = 0.0f, we = 0.0f, s2 = 0.0f;
+      for (uint32_t c = 0; c < C; ++c) {
+        const uint32_t n = (uint32_t)ao.count[((size_t)c * T + t) * N + v];
+        if (n == 0) continue;
+        const float p = __fdiv_rn((float)n, numf);
+        const float d = (float)__longlong_as_double((long long)ao.nearest[(size_t)c * N + v]);
+        const float w = __fdiv_rn(fminf(expf(__fmul_rn(-beta, d)), mcw), mcw);
+        const float r = __fdiv_rn(p, agg);
+        s0 = __fadd_rn(s0, __fmul_rn(w, powf(r, 0.0f)));
+        const float e = __fdiv_rn(__fmul_rn(w, p), agg);
+        if (e > 0.0f) we = __fadd_rn(we, __fmul_rn(e, logf(e)));
+        s2 = __fadd_rn(s2, __fmul_rn(w, powf(r, 2.0f)));
+      }
+      const float w0 = powf(s0, 1.0f), w1 = expf(-we), w2 = powf(s2, -1.0f);
+      out[3] = isfinite(s0) && isfinite(w0) ? w0 : 0.0f;
+      out[4] = isfinite(w1) ? w1 : 0.0f;
+      out[5] = isfinite(s2) && isfinite(w2) ? w2 : 0.0f;
+    }
+    out[6] = isfinite(ent) ? fmaxf(-ent, 0.0f) : 0.0f;
+    out[7] = num >= 2 ? fmaxf(__fsub_rn(1.0f, lam), 0.0f) : 0.0f;
+  }
+  for (uint32_t k = 0; k < kDivMetrics; ++k) div[((size_t)k * T + t) * N + v] = (double)out[k];
+}
+
+void launch_diversity(const AccessOut &out, uint32_t n_classes, double *div, hipStream_t stream) {
+  const uint64_t n = (uint64_t)out.T * out.N;
+  if (n == 0) return;
+  const uint32_t block = 256;
+  hipLaunchKernelGGL(k_diversity, dim3((uint32_t)((n + block - 1) / block)), dim3(block), 0, stream, out, n_classes,
+                     div);
+  hip_check(hipGetLastError(), "k_diversity launch");
+}
+
 void launch_single(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec, float tolerance,
                    int mode, const TreeOut &out, hipStream_t stream) {
   CallSpec cs;

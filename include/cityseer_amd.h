@@ -231,6 +231,17 @@ int cs_accessibility(cs_graph *g, uint32_t n_entries, const uint32_t *entry_grou
                      uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
                      double *out_nearest);
 
+/* Replaces DataMap.mixed_uses (rust/src/data.rs:749-944) with angular = false:
+ * the same inputs as cs_accessibility, with entry_class numbering every land
+ * use in sorted order (n_classes = the number of distinct land uses). out:
+ * f64 [8][n_thresholds][N] = hill q0, q1, q2, distance-weighted hill q0, q1,
+ * q2, shannon, gini-simpson (diversity.rs:6-175, 341-385), each computed in
+ * f32 from the per-source class counts and nearest class distances. */
+int cs_mixed_uses(cs_graph *g, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
+                  uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node, const uint32_t *assign_entry,
+                  const double *assign_dist, uint32_t n_thresholds, const uint32_t *distances, const float *betas,
+                  const float *max_curve_wts, uint32_t max_seconds, float speed_m_s, double *out);
+
 /* ---------------------------------------------------------------------- */
 /* Single-source trees                                                    */
 /* ---------------------------------------------------------------------- */

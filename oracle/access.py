@@ -67,3 +67,119 @@ def accessibility(og, live, node_data_map, dedupe_of, lu_class, accessibility_ke
                     if d == max_dist and (np.isnan(near[src]) or float(data_dist) < near[src]):
                         near[src] = float(data_dist)
     return res
+
+
+# ---- diversity (rust/src/diversity.rs), f32 throughout, classes summed in order
+
+def hill(counts, q):
+    """hill_diversity_core (diversity.rs:6-72); errors read as 0 (data.rs:866-941 unwrap_or)."""
+    num = int(sum(counts))
+    if num == 0:
+        return f32(0.0)
+    if q == 0:
+        return f32(sum(1 for c in counts if c > 0))
+    numf = f32(num)
+    if q == 1:
+        ent = f32(0.0)
+        for c in counts:
+            if c > 0:
+                p = f32(f32(c) / numf)
+                ent = f32(ent + f32(p * np.log(p, dtype=np.float32)))
+        r = np.exp(-ent, dtype=np.float32)
+        return r if np.isfinite(r) else f32(0.0)
+    s = f32(0.0)
+    for c in counts:
+        if c > 0:
+            p = f32(f32(c) / numf)
+            s = f32(s + np.power(p, f32(q), dtype=np.float32))
+    r = np.power(s, f32(f32(1.0) / f32(1.0 - q)), dtype=np.float32)
+    return r if np.isfinite(s) and np.isfinite(r) else f32(0.0)
+
+
+def clipped_wt(beta, mcw, d):
+    raw = np.exp(f32(-f32(beta)) * f32(d), dtype=np.float32)
+    return f32(min(raw, f32(mcw)) / f32(mcw))
+
+
+def hill_weighted(counts, nearest, q, beta, mcw):
+    """hill_diversity_branch_distance_wt_core (diversity.rs:77-175)."""
+    num = int(sum(counts))
+    if num == 0:
+        return f32(0.0)
+    numf = f32(num)
+    pw = [(f32(f32(c) / numf), clipped_wt(beta, mcw, d)) for c, d in zip(counts, nearest, strict=True) if c > 0]
+    agg = f32(0.0)
+    for p, w in pw:
+        agg = f32(agg + f32(w * p))
+    if abs(agg) < np.finfo(np.float32).eps:
+        return f32(0.0)
+    if q == 1:
+        we = f32(0.0)
+        for p, w in pw:
+            e = f32(f32(w * p) / agg)
+            if e > 0:
+                we = f32(we + f32(e * np.log(e, dtype=np.float32)))
+        r = np.exp(-we, dtype=np.float32)
+        return r if np.isfinite(r) else f32(0.0)
+    s = f32(0.0)
+    for p, w in pw:
+        s = f32(s + f32(w * np.power(f32(p / agg), f32(q), dtype=np.float32)))
+    r = np.power(s, f32(f32(1.0) / f32(1.0 - q)), dtype=np.float32)
+    return r if np.isfinite(s) and np.isfinite(r) else f32(0.0)
+
+
+def shannon(counts):
+    """shannon_diversity_core (diversity.rs:365-385)."""
+    num = int(sum(counts))
+    if num == 0:
+        return f32(0.0)
+    numf = f32(num)
+    ent = f32(0.0)
+    for c in counts:
+        if c > 0:
+            p = f32(f32(c) / numf)
+            ent = f32(ent + f32(p * np.log(p, dtype=np.float32)))
+    return f32(max(-ent, f32(0.0))) if np.isfinite(ent) else f32(0.0)
+
+
+def gini(counts):
+    """gini_simpson_diversity_core (diversity.rs:341-358)."""
+    num = int(sum(counts))
+    if num < 2:
+        return f32(0.0)
+    numf, nm1 = f32(num), f32(num - 1)
+    lam = f32(0.0)
+    for c in counts:
+        if c > 0:
+            x = f32(c)
+            lam = f32(lam + f32(f32(x / numf) * f32(max(f32(x - f32(1.0)), f32(0.0)) / nm1)))
+    return f32(max(f32(f32(1.0) - lam), f32(0.0)))
+
+
+def mixed_uses(og, live, node_data_map, dedupe_of, lu_class, distances, betas, mcws, max_walk_seconds, speed,
+               sources=None):
+    """f64 [8][T][N]: hill q0-2, weighted hill q0-2, shannon, gini (data.rs:818-941)."""
+    n, T = og.n, len(distances)
+    classes = sorted(set(lu_class.values()))
+    ci = {c: i for i, c in enumerate(classes)}
+    out = np.zeros((8, T, n))
+    for src in (range(n) if sources is None else sources):
+        if not live[src]:
+            continue
+        tree = og.tree_shortest(src, max_walk_seconds, speed)
+        reach = aggregate_to_src_idx(tree, node_data_map, dedupe_of, max_walk_seconds, speed)
+        counts = np.zeros((T, len(classes)), dtype=np.int64)
+        nearest = np.full((T, len(classes)), np.inf, dtype=np.float32)
+        for data_key, data_dist in reach.items():
+            c = ci[lu_class[data_key]]
+            for i, d in enumerate(distances):
+                if data_dist <= f32(d):
+                    counts[i, c] += 1
+                    nearest[i, c] = min(nearest[i, c], data_dist)
+        for i in range(T):
+            cs, ns_ = counts[i].tolist(), nearest[i].tolist()
+            vals = [hill(cs, 0), hill(cs, 1), hill(cs, 2)]
+            vals += [hill_weighted(cs, ns_, q, betas[i], mcws[i]) for q in (0, 1, 2)]
+            vals += [shannon(cs), gini(cs)]
+            out[:, i, src] = [float(v) for v in vals]
+    return out

@@ -113,3 +113,45 @@ def test_datamap_argument_checks():
         dm.assign_data_to_network(ns, 400.0, 6)
     with pytest.raises(ValueError, match="dual"):
         dm.accessibility(ns, {"a": "shop", "b": "shop"}, ["shop"], distances=[200], angular=True)
+
+
+def test_oracle_diversity_closed_forms():
+    from oracle import access as A
+
+    counts = [4, 0, 2, 2]
+    p = np.array([0.5, 0.25, 0.25])
+    assert A.hill(counts, 0) == 3.0
+    assert math.isclose(A.hill(counts, 1), math.exp(-(p * np.log(p)).sum()), rel_tol=1e-6)
+    assert math.isclose(A.hill(counts, 2), 1.0 / (p**2).sum(), rel_tol=1e-6)
+    assert math.isclose(A.shannon(counts), -(p * np.log(p)).sum(), rel_tol=1e-6)
+    # gini-simpson: 1 - sum x (x - 1) / (n (n - 1))
+    assert math.isclose(A.gini(counts), 1.0 - (4 * 3 + 2 + 2) / (8 * 7), rel_tol=1e-6)
+    assert A.hill([0, 0], 1) == 0.0 and A.gini([1, 0]) == 0.0 and A.shannon([]) == 0.0
+    # with unit weights (beta 0) the weighted Hill numbers are the plain ones for q = 1, 2;
+    # q = 0 sums the weights, here the class count
+    near = [10.0, np.inf, 50.0, 90.0]
+    for q in (1, 2):
+        assert math.isclose(A.hill_weighted(counts, near, q, 0.0, 1.0), A.hill(counts, q), rel_tol=1e-6)
+    assert A.hill_weighted(counts, near, 0, 0.0, 1.0) == 3.0
+    # q = 0 with beta > 0: the sum of exp(-beta d) over present classes
+    assert math.isclose(A.hill_weighted(counts, near, 0, 0.01, 1.0), math.exp(-0.1) + math.exp(-0.5) + math.exp(-0.9),
+                        rel_tol=1e-6)
+
+
+def test_oracle_mixed_uses_line_graph():
+    """On the 4-node line, node 0 reaches A, D (shop), C (other: every class counts here) and F
+    (cafe) within 250, and only F within 100."""
+    from oracle import access as A
+
+    O.set_heap_mode(False)
+    ns = line_graph()
+    og = OracleGraph.from_network_structure(ns)
+    _d, betas, secs = O.thresholds(1.0, distances=[100, 250])
+    ndm, dedupe_of, lu_class = line_inputs()
+    out = A.mixed_uses(og, [1] * 4, ndm, dedupe_of, lu_class, [100, 250], betas, [1.0, 1.0], max(secs), 1.0)
+    # classes sorted: cafe, other, shop; node 0 at 250: counts [1, 1, 2]
+    assert out[0, 1, 0] == 3.0
+    assert math.isclose(out[2, 1, 0], 1.0 / (0.25**2 + 0.25**2 + 0.5**2), rel_tol=1e-6)
+    assert math.isclose(out[7, 1, 0], 1.0 - (2 * 1) / (4 * 3), rel_tol=1e-6)
+    # at 100 only F (40) counts: one class
+    assert out[0, 0, 0] == 1.0 and out[6, 0, 0] == 0.0 and out[7, 0, 0] == 0.0

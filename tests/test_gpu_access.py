@@ -113,3 +113,45 @@ def test_no_assignments_and_unknown_keys():
     res = dm.accessibility(ns, lu, ["shop", "none"], distances=[300])
     assert np.all(res.result["shop"].unweighted[300] == 0.0)
     assert np.all(np.isnan(res.result["none"].distance[300]))
+
+
+def _compare_mixed(ns, entries, dists, speed, spatial_tolerance=0):
+    from oracle.access import mixed_uses as oracle_mixed
+
+    dm, lu = _datamap(entries)
+    res = dm.mixed_uses(ns, lu, distances=dists, speed_m_s=speed, compute_shannon=True, compute_gini=True,
+                        spatial_tolerance=spatial_tolerance)
+    og = OracleGraph.from_network_structure(ns)
+    _d, bets, secs = O.thresholds(speed, distances=dists)
+    mcw = data.clip_wts_curve(dists, bets, spatial_tolerance)
+    ndm = {v: [(k[4:], dist) for k, dist in pairs] for v, pairs in dm.node_data_map.items()}
+    dedupe_of = {key: (dd if dd is not None else key) for key, _c, dd, _a in entries}
+    ref = oracle_mixed(og, [1] * og.n, ndm, dedupe_of, lu, dists, bets, mcw, max(secs), speed)
+    for i, d in enumerate(dists):
+        # richness is an exact count; the rest are f32 values of exp/log/pow, whose last bits may
+        # differ between the device library and the host's (1e-5 relative)
+        assert np.array_equal(res.hill[0][d], ref[0, i]), f"hill q0 at {d}"
+        for name, got, m in (("hill q1", res.hill[1][d], 1), ("hill q2", res.hill[2][d], 2),
+                             ("weighted q0", res.hill_weighted[0][d], 3), ("weighted q1", res.hill_weighted[1][d], 4),
+                             ("weighted q2", res.hill_weighted[2][d], 5), ("shannon", res.shannon[d], 6),
+                             ("gini", res.gini[d], 7)):
+            np.testing.assert_allclose(got, ref[m, i], rtol=1e-5, atol=1e-6, err_msg=f"{name} at {d}")
+    return res
+
+
+def test_mixed_uses_line_graph_and_lattice():
+    ns = line_graph()
+    say("mixed uses on the 4-node line")
+    res = _compare_mixed(ns, LINE_ENTRIES, [100, 250], 1.0)
+    assert res.hill[0][250][0] == 3.0 and res.hill[0][100][0] == 1.0
+    lat = mock.perturbed_lattice(20, seed=205, diag_frac=0.1)
+    ns = mock.network_structure_from_lattice(lat)
+    entries = _random_case(ns, 700, 206, classes=tuple(f"c{i}" for i in range(9)))
+    say(f"mixed uses, {ns.node_count()} nodes, {len(entries)} entries, 9 classes")
+    res = _compare_mixed(ns, entries, [200, 600], 1.33333, spatial_tolerance=20)
+    assert res.hill[0][600].max() > 3
+    dm, lu = _datamap(entries)
+    only = dm.mixed_uses(ns, lu, distances=[200, 600], compute_hill=False, spatial_tolerance=20)
+    assert np.all(only.hill[1][600] == 0.0) and np.array_equal(only.hill_weighted[1][600], res.hill_weighted[1][600])
+    with pytest.raises(ValueError, match="compute_"):
+        dm.mixed_uses(ns, lu, distances=[200], compute_hill=False, compute_hill_weighted=False)
