@@ -10,10 +10,8 @@ from __future__ import annotations
 
 import logging
 import os
-import threading
-import time
 from collections.abc import Callable
-from queue import Queue
+from concurrent.futures import ThreadPoolExecutor, wait
 from typing import Any
 
 import numpy as np
@@ -38,19 +36,26 @@ def prep_gdf_key(key: str, dist: int, angular: bool = False, weighted: bool | No
     return key
 
 
+_TRUTHY = frozenset({"true", "1"})
+
+
+def _env_flag(name: str) -> bool:
+    """True when environment variable ``name`` holds "true" or "1" (any case)."""
+    return os.environ.get(name, "").strip().lower() in _TRUTHY
+
+
 def check_quiet() -> bool:
-    """Quiet mode flag (config.py:35-42)."""
-    if "GCP_PROJECT" in os.environ:
-        return True
-    return "CITYSEER_QUIET_MODE" in os.environ and os.environ["CITYSEER_QUIET_MODE"].lower() in ["true", "1"]
+    """Quiet mode (config.py:35-42): set by CITYSEER_QUIET_MODE, and always on in Google Cloud
+    functions (where GCP_PROJECT is defined), which have no terminal for a progress bar."""
+    return "GCP_PROJECT" in os.environ or _env_flag("CITYSEER_QUIET_MODE")
 
 
 QUIET_MODE = check_quiet()
 
 
 def check_debug() -> bool:
-    """Debug mode flag (config.py:48-53)."""
-    return "CITYSEER_DEBUG_MODE" in os.environ and os.environ["CITYSEER_DEBUG_MODE"].lower() in ["true", "1"]
+    """Debug mode (config.py:48-53), set by CITYSEER_DEBUG_MODE."""
+    return _env_flag("CITYSEER_DEBUG_MODE")
 
 
 DEBUG_MODE: bool = check_debug()
@@ -68,16 +73,33 @@ def log_thresholds(
     min_threshold_wt: float = MIN_THRESH_WT,
     speed_m_s: float = SPEED_M_S,
 ):
-    """Resolve and log the threshold triplets (config.py:67-82)."""
+    """Resolve the threshold triplets from whichever of distances, betas or minutes is given, log
+    one line per threshold and return the distances (config.py:67-82)."""
     from . import rustalgos
 
-    distances, betas, seconds = rustalgos.pair_distances_betas_time(
+    resolved = rustalgos.pair_distances_betas_time(
         speed_m_s, distances, betas, minutes, min_threshold_wt=min_threshold_wt
     )
-    logger.info("Metrics computed for:")
-    for distance, beta, walking_time in zip(distances, betas, seconds, strict=True):
-        logger.info(f"Distance: {distance}m, Beta: {round(beta, 5)}, Walking Time: {walking_time / 60} minutes.")
-    return distances
+    lines = [
+        f"Distance: {d}m, Beta: {round(b, 5)}, Walking Time: {s / 60} minutes."
+        for d, b, s in zip(*resolved, strict=True)
+    ]
+    logger.info("\n".join(["Metrics computed for:", *lines]))
+    return resolved[0]
+
+
+_POLL_SECONDS = 0.1
+
+
+def _progress_bar(total: int, desc: str | None):
+    """A tqdm bar, or None in quiet mode or without tqdm."""
+    if QUIET_MODE:
+        return None
+    try:
+        from tqdm import tqdm
+    except ImportError:  # pragma: no cover - tqdm is optional
+        return None
+    return tqdm(total=total, desc=desc)
 
 
 def wrap_progress(
@@ -86,34 +108,20 @@ def wrap_progress(
     partial_func: Callable,
     desc: str | None = None,
 ) -> Any:
-    """Run a long native call on a worker thread and poll its progress counter (config.py:96-130)."""
-
-    def wrapper(queue: Queue):
-        try:
-            queue.put(partial_func())
-        except Exception as e:  # noqa: BLE001 - re-raised on the calling thread
-            queue.put(e)
-
-    result_queue: Queue = Queue()
-    thread = threading.Thread(target=wrapper, args=(result_queue,))
-    pbar = None
-    if not QUIET_MODE:
-        try:
-            from tqdm import tqdm
-
-            pbar = tqdm(total=total, desc=desc)
-        except ImportError:  # pragma: no cover - tqdm is optional
-            pbar = None
-    thread.start()
-    while thread.is_alive():
-        time.sleep(0.1)
-        if pbar is not None:
-            pbar.update(rust_struct.progress() - pbar.n)
-    if pbar is not None:
-        pbar.update(total - pbar.n)
-        pbar.close()
-    result = result_queue.get()
-    thread.join()
-    if isinstance(result, Exception):
-        raise result
-    return result
+    """Run ``partial_func`` on a helper thread while this thread shows ``rust_struct.progress()``
+    (sources finished so far, read while the native call runs) against ``total``
+    (config.py:96-130). Returns the call's result or re-raises its exception here."""
+    bar = _progress_bar(total, desc)
+    with ThreadPoolExecutor(max_workers=1, thread_name_prefix="cityseer-call") as pool:
+        call = pool.submit(partial_func)
+        while True:
+            done, _ = wait([call], timeout=_POLL_SECONDS)
+            if bar is not None:
+                bar.n = min(total, int(rust_struct.progress()))
+                bar.refresh()
+            if done:
+                break
+    if bar is not None:
+        bar.n = total
+        bar.close()
+    return call.result()

@@ -267,6 +267,13 @@ static int g_heap_mode = 0;
 
 void og_set_heap_mode(int rust_order) { g_heap_mode = rust_order ? 1 : 0; }
 
+/* Planted mistakes (og_set_mutation), 0 = none. The tests switch one on to show
+ * that a pinning check fails on a plausible error, then switch it off again.
+ * Nothing else sets them. */
+static int g_mutation = 0;
+
+void og_set_mutation(int m) { g_mutation = m; }
+
 typedef struct {
   float k;
   float aux;
@@ -865,6 +872,8 @@ static void shortest_source(job *jb, uint32_t s_i) {
         if (t.best_route[to] > thr) continue;
         double pc = p->eligible[to] ? 0.5 : 1.0;
         double pb = pc * exp(-beta * (double)t.best_route[to]);
+        if (g_mutation == OG_MUT_BETA_SOURCE_ROUTE) pb = pc * exp(-beta * (double)t.best_route[src]);
+        if (g_mutation == OG_MUT_BETA_FULL_SEED) pb = exp(-beta * (double)t.best_route[to]);
         seed[to] += pc;
         seed_b[to] += pb;
       }
@@ -952,6 +961,8 @@ static void simplest_source(job *jb, uint32_t s_i) {
         if (bs <= (float)p->seconds[i]) {
           atomic_add_f64(&OUT(0, i, to), (double)wt);
           float far_ang = p->farness_scaling_offset + (bsd / p->angular_scaling_unit);
+          if (g_mutation == OG_MUT_ANG_FAR_SWAP) far_ang = p->angular_scaling_unit + (bsd / p->farness_scaling_offset);
+          if (g_mutation == OG_MUT_ANG_FAR_NO_OFFSET) far_ang = bsd / p->angular_scaling_unit;
           atomic_add_f64(&OUT(1, i, to), (double)(far_ang * wt));
           float harm_ang = 1.0f + (bsd / p->angular_scaling_unit);
           atomic_add_f64(&OUT(2, i, to), (double)((1.0f / harm_ang) * wt));
@@ -983,10 +994,12 @@ static void simplest_source(job *jb, uint32_t s_i) {
         }
         if (nb == 0) continue;
         double pc = p->eligible[to] ? 0.5 : 1.0;
+        if (g_mutation == OG_MUT_ANG_FULL_SEED) pc = 1.0;
         double total = 0.0;
         for (uint32_t j = 0; j < nb; j++) total += t.sigma[best[j]];
         if (total == 0.0) continue;
-        for (uint32_t j = 0; j < nb; j++) seed[best[j]] += pc * (t.sigma[best[j]] / total);
+        for (uint32_t j = 0; j < nb; j++)
+          seed[best[j]] += pc * (g_mutation == OG_MUT_ANG_NO_SPLIT ? 1.0 : t.sigma[best[j]] / total);
       }
       backprop(&t, sorted, ns, src, seed, seed_b, 1, sthr, 2, delta, delta_b, &OUT(3, i, 0), NULL,
                (double)wt);

@@ -40,6 +40,24 @@ typedef struct og_graph og_graph;
  * reference's golden vectors on graphs with exact ties. Process-wide. */
 void og_set_heap_mode(int rust_order);
 
+/* Planted mistakes for the pinning tests (0 = none, the default):
+ *  OG_MUT_BETA_SOURCE_ROUTE   the beta seed takes the source's route instead of the target's
+ *  OG_MUT_BETA_FULL_SEED      the beta seed drops the 0.5 pair weight
+ *  OG_MUT_ANG_FAR_SWAP        angular farness with scaling unit and offset swapped
+ *  OG_MUT_ANG_FAR_NO_OFFSET   angular farness without the offset
+ *  OG_MUT_ANG_FULL_SEED       angular betweenness seeds drop the 0.5 pair weight
+ *  OG_MUT_ANG_NO_SPLIT        angular seeds not split over tied terminal states */
+enum {
+  OG_MUT_NONE = 0,
+  OG_MUT_BETA_SOURCE_ROUTE = 1,
+  OG_MUT_BETA_FULL_SEED = 2,
+  OG_MUT_ANG_FAR_SWAP = 4,
+  OG_MUT_ANG_FAR_NO_OFFSET = 5,
+  OG_MUT_ANG_FULL_SEED = 6,
+  OG_MUT_ANG_NO_SPLIT = 7,
+};
+void og_set_mutation(int m);
+
 /* Graph built from nodes and directed edges given in insertion order.
  * z: NaN = node has no elevation. seconds: NaN = derive from length
  * (rust/src/graph.rs:879 stores NaN for street edges). primal_key: -1 = none. */

@@ -84,6 +84,8 @@ def load() -> ctypes.CDLL:
     lib.og_graph_free.argtypes = [ctypes.c_void_p]
     lib.og_set_heap_mode.restype = None
     lib.og_set_heap_mode.argtypes = [ctypes.c_int]
+    lib.og_set_mutation.restype = None
+    lib.og_set_mutation.argtypes = [ctypes.c_int]
     lib.og_pair_thresholds.restype = ctypes.c_int
     lib.og_pair_thresholds.argtypes = [
         ctypes.c_float, ctypes.c_int, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, _u32p, _f32p, _u32p,
@@ -126,6 +128,31 @@ def load() -> ctypes.CDLL:
 def set_heap_mode(rust_order: bool) -> None:
     """Pop equal route costs in Rust BinaryHeap order (True) or keyed (seconds, index) order (False)."""
     load().og_set_heap_mode(int(bool(rust_order)))
+
+
+MUTATIONS = {
+    "beta_source_route": 1,
+    "beta_full_seed": 2,
+    "ang_far_swap": 4,
+    "ang_far_no_offset": 5,
+    "ang_full_seed": 6,
+    "ang_no_split": 7,
+}
+
+
+class mutation:
+    """Context manager planting one named mistake in the oracle (pinning tests only)."""
+
+    def __init__(self, name: str):
+        self.code = MUTATIONS[name]
+
+    def __enter__(self):
+        load().og_set_mutation(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        load().og_set_mutation(0)
+        return False
 
 
 def _p(a, t):

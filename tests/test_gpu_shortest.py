@@ -1,8 +1,8 @@
 """GPU parity: shortest-path and segment centrality vs the oracle (HIP kernels via the C ABI).
 
-Bar (per the brief): reachable counts (density with unit weights), distances,
-predecessor sets and sigma bit-exact; f32-term closeness / betweenness sums
-within 1e-5 relative (only f64 summation order differs).
+Bar: reachable counts (density with unit weights), distances, predecessor
+sets and sigma bit-exact; f32-term closeness / betweenness sums within 1e-5
+relative per (metric, threshold) row (only f64 summation order differs).
 """
 
 from __future__ import annotations
@@ -14,6 +14,7 @@ from cityseer import config
 from cityseer.tools import io, mock
 import oracle.oracle as O
 from oracle.oracle import OracleGraph
+from parity import assert_close  # noqa: F401  (re-exported to the other GPU test files)
 
 pytestmark = pytest.mark.gpu
 
@@ -30,15 +31,6 @@ def say(msg: str) -> None:
 
 def stack(res, keys, distances):
     return np.stack([np.stack([getattr(res, k)[d] for d in distances]) for k in keys])
-
-
-def assert_close(got, ref, what):
-    scale = max(1.0, float(np.max(np.abs(ref)))) if ref.size else 1.0
-    ok = np.isclose(got, ref, rtol=RTOL, atol=1e-9 * scale)
-    if not ok.all():
-        idx = np.argwhere(~ok)[:5]
-        detail = [(tuple(i), float(got[tuple(i)]), float(ref[tuple(i)])) for i in idx]
-        raise AssertionError(f"{what}: {int((~ok).sum())} mismatches, first {detail}")
 
 
 def check_shortest(ns, og, distances, **kw):
