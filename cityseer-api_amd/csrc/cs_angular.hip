@@ -39,7 +39,8 @@ struct AngJob {
   const uint32_t *sources;
   const float *src_wt;
   uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
-  double *acc;     // [4][T][N], metric-major
+  double *acc;     // [4][tstride][N], metric-major
+  uint32_t tstride;  // threshold stride of acc rows (>= T)
   uint32_t *reach;
 };
 
@@ -480,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     }
     st_reached += n_nodes;
     st_edges += n_order;
-    const uint32_t T = jb.T;
+    const uint32_t T = jb.T, TS = jb.tstride;
     // ---- closeness at targets (centrality.rs:1995-2033)
     for (uint32_t k = 1; k < n_nodes; ++k) {
       const uint32_t v = *nr<uint32_t>(w, a.o_nrec, k, lane, a.ncap, kN_node);
@@ -495,11 +496,11 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           ++st_within;
           if (jb.flags & 4u) atomicAdd(&jb.reach[t], 1u);
           if (jb.flags & 1u) {
-            unsafeAtomicAdd(&accv[((size_t)0 * T + t) * NN], (double)wt);
+            unsafeAtomicAdd(&accv[((size_t)0 * TS + t) * NN], (double)wt);
             const float far_ang = jb.offset + (bsd / jb.unit);
-            unsafeAtomicAdd(&accv[((size_t)1 * T + t) * NN], (double)(far_ang * wt));
+            unsafeAtomicAdd(&accv[((size_t)1 * TS + t) * NN], (double)(far_ang * wt));
             const float harm_ang = 1.0f + (bsd / jb.unit);
-            unsafeAtomicAdd(&accv[((size_t)2 * T + t) * NN], (double)((1.0f / harm_ang) * wt));
+            unsafeAtomicAdd(&accv[((size_t)2 * TS + t) * NN], (double)((1.0f / harm_ang) * wt));
           }
         }
       }
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
             }
             if (node == src) continue;
             const double credit = dep - seed;
-            if (credit > 0.0) unsafeAtomicAdd(&jb.acc[((size_t)3 * T + t) * jb.N + node], credit * (double)wt);
+            if (credit > 0.0) unsafeAtomicAdd(&jb.acc[((size_t)3 * jb.tstride + t) * jb.N + node], credit * (double)wt);
           }
         }
         hi = lo - 1;
@@ -891,6 +892,7 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     j.src_wt = sp.d_wt;
     j.flags = (sp.cs.closeness ? 1u : 0u) | (sp.cs.betweenness ? 2u : 0u) | (sp.cs.track_reach ? 4u : 0u);
     j.acc = sp.d_acc;
+    j.tstride = sp.tstride ? sp.tstride : sp.cs.T;
     j.reach = sp.d_reach;
   }
   AngPlan p;
@@ -1017,9 +1019,9 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
 
 void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const uint32_t *d_sources,
                      const float *d_wt, uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
-                     uint32_t *d_reach, hipStream_t stream, cs_stats &st) {
+                     uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride) {
   if (n_sources == 0) return;
-  std::vector<AngSpec> specs{AngSpec{&g, &dg, &de, cs, d_sources, d_wt, d_eligible, d_acc, d_reach}};
+  std::vector<AngSpec> specs{AngSpec{&g, &dg, &de, cs, d_sources, d_wt, d_eligible, d_acc, d_reach, tstride}};
   launch_simplest_jobs(specs, nullptr, n_sources, stream, st);
 }
 

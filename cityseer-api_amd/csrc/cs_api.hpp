@@ -3,6 +3,7 @@
 // server): parameter checks, source preparation and result unpacking.
 #pragma once
 
+#include <algorithm>
 #include <functional>
 #include <string>
 #include <vector>
@@ -25,8 +26,14 @@ int guarded_call(const std::function<void()> &f);
 // ValueError messages, centrality.rs:895-959, common.rs:98-270).
 void check_params(const Graph &g, const cs_params *p);
 void run_shortest_checks(const Graph &g, const cs_params *p);
+void check_betweenness_graph(const Graph &g);
 void check_simplest_graph(const Graph &g);
-CallSpec make_spec(const cs_params *p);
+// The kernels take at most kMaxT thresholds per launch; a call with more runs
+// one launch per chunk of kMaxT, each writing its own threshold rows.
+// make_spec(p, t0, n) holds thresholds t0 .. t0 + n - 1 (n = 0: the rest,
+// at most kMaxT) and the call's largest cutoff.
+CallSpec make_spec(const cs_params *p, uint32_t t0 = 0, uint32_t n = 0);
+inline uint32_t chunk_len(uint32_t T, uint32_t t0) { return std::min<uint32_t>(kMaxT, T - t0); }
 
 // Sources kept after NaN-weight filtering, with per-source weights and the
 // cycles weight wt / node_weight (centrality.rs:1711-1730).

@@ -401,8 +401,7 @@ struct DevBuf {
 
 void check_params(const Graph &g, const cs_params *p) {
   if (!p) fail(CS_EINVAL, "params must not be NULL");
-  if (p->n_thresholds == 0 || p->n_thresholds > CS_MAX_THRESHOLDS)
-    fail(CS_EINVAL, "between 1 and " + std::to_string(CS_MAX_THRESHOLDS) + " distance thresholds are supported");
+  if (p->n_thresholds == 0) fail(CS_EINVAL, "at least one distance threshold is required");
   if (!p->compute_closeness && !p->compute_betweenness)
     fail(CS_EINVAL,
          "Either or both closeness and betweenness flags is required, but both parameters are False.");
@@ -415,16 +414,21 @@ void check_params(const Graph &g, const cs_params *p) {
       fail(CS_EINVAL, "node index " + std::to_string(p->sources[i]) + " does not exist in the graph");
 }
 
-CallSpec make_spec(const cs_params *p) {
+CallSpec make_spec(const cs_params *p, uint32_t t0, uint32_t n) {
+  if (n == 0) n = p->n_thresholds - t0;
+  if (n > kMaxT || t0 + n > p->n_thresholds) fail(CS_EINTERNAL, "threshold chunk out of range");
   CallSpec c;
-  c.T = p->n_thresholds;
-  uint32_t maxs = 0;
-  for (uint32_t i = 0; i < c.T; ++i) {
-    c.dist[i] = (float)p->distances[i];
-    c.beta[i] = p->betas[i];
-    c.seconds[i] = p->seconds[i];
-    maxs = std::max(maxs, p->seconds[i]);
+  c.T = n;
+  for (uint32_t i = 0; i < n; ++i) {
+    c.dist[i] = (float)p->distances[t0 + i];
+    c.beta[i] = p->betas[t0 + i];
+    c.seconds[i] = p->seconds[t0 + i];
   }
+  // every chunk searches as far as the call's largest threshold, as the
+  // reference's single search does (tie bands at a chunk's own cutoff would
+  // otherwise differ)
+  uint32_t maxs = 0;
+  for (uint32_t i = 0; i < p->n_thresholds; ++i) maxs = std::max(maxs, p->seconds[i]);
   c.maxsec = (float)maxs;
   c.speed = p->speed_m_s;
   c.tolerance = p->tolerance;
@@ -522,9 +526,16 @@ void check_simplest_graph(const Graph &g) {
   if (!g.slots_ok) fail(CS_EINVAL, g.slots_err);
 }
 
+// Predecessor sets are kept over out-row positions (a bitmask up to 64
+// out-edges, up to four 16-bit positions beyond); a node with more than four
+// tied predecessors at such a wide row is refused by the search itself.
+void check_betweenness_graph(const Graph &g) {
+  if (g.max_out_deg > 65534)
+    fail(CS_EINVAL, "nodes with more than 65,534 out-edges are not supported by the GPU search");
+}
+
 void run_shortest_checks(const Graph &g, const cs_params *p) {
-  if (p->compute_betweenness && g.max_out_deg > 64)
-    fail(CS_EINVAL, "nodes with more than 64 out-edges are not supported by the GPU search");
+  if (p->compute_betweenness) check_betweenness_graph(g);
 }
 
 
@@ -762,16 +773,16 @@ int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_
     Graph &g = h->g;
     check_params(g, p);
     const int dev = current_device();
-    const CallSpec spec = make_spec(p);
+    const CallSpec spec = make_spec(p, 0, chunk_len(p->n_thresholds, 0));
     SourceSet ss = source_set(g, p);
     to_device_sources(g, ss);
     const std::vector<uint8_t> elig = to_device_u8(g, p->eligible);
-    const uint32_t N = g.n, T = spec.T, M = 7;
+    const uint32_t N = g.n, T = p->n_thresholds, M = 7;
     OwnedStream stream;
-    DevBuf acc(sizeof(double) * (size_t)N * M * T), reach(4 * kMaxT), d_src(4 * ss.src.size()),
+    DevBuf acc(sizeof(double) * (size_t)N * M * T), reach(4 * T), d_src(4 * ss.src.size()),
         d_wt(4 * ss.src.size()), d_cyc(4 * ss.src.size()), d_elig(N);
     hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
-    hip_check(hipMemsetAsync(reach.p, 0, 4 * kMaxT, stream), "memset");
+    hip_check(hipMemsetAsync(reach.p, 0, 4 * T, stream), "memset");
     if (!ss.src.empty()) {
       hip_check(hipMemcpyAsync(d_src.p, ss.src.data(), 4 * ss.src.size(), hipMemcpyHostToDevice, stream), "copy");
       hip_check(hipMemcpyAsync(d_wt.p, ss.wt.data(), 4 * ss.wt.size(), hipMemcpyHostToDevice, stream), "copy");
@@ -783,15 +794,16 @@ int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_
       DevGraph &dg = device_graph(g, dev);
       DevEdges &de = device_edges(g, dg, spec.speed);
       run_shortest_checks(g, p);
-      launch_shortest(g, dg, de, spec, d_src.as<uint32_t>(), d_wt.as<float>(), d_cyc.as<float>(),
-                      (uint32_t)ss.src.size(), d_elig.as<uint8_t>(), acc.as<double>(), reach.as<uint32_t>(),
-                      stream, t_stats);
+      for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
+        launch_shortest(g, dg, de, make_spec(p, t0, chunk_len(T, t0)), d_src.as<uint32_t>(), d_wt.as<float>(),
+                        d_cyc.as<float>(), (uint32_t)ss.src.size(), d_elig.as<uint8_t>(),
+                        acc.as<double>() + (size_t)t0 * N, reach.as<uint32_t>() + t0, stream, t_stats, T);
     }
     std::vector<double> hacc((size_t)N * M * T);
-    uint32_t hreach[kMaxT];
+    std::vector<uint32_t> hreach(T);
     hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
               "copy out");
-    hip_check(hipMemcpyAsync(hreach, reach.p, 4 * kMaxT, hipMemcpyDeviceToHost, stream), "copy out");
+    hip_check(hipMemcpyAsync(hreach.data(), reach.p, 4 * T, hipMemcpyDeviceToHost, stream), "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
     unpack_acc(hacc, N, M, T, out, result_order(g));
     if (p->compute_betweenness) scale_rows(out, N, T, 5, 6, p->betw_scale);
@@ -811,21 +823,22 @@ int cs_centrality_shortest_device(cs_graph *h, const cs_params *p, const float *
   int rc = guarded([&] {
     if (!h || !p || !d_acc) fail(CS_EINVAL, "graph, params and d_acc must not be NULL");
     Graph &g = h->g;
-    if (p->n_thresholds == 0 || p->n_thresholds > CS_MAX_THRESHOLDS)
-      fail(CS_EINVAL, "between 1 and " + std::to_string(CS_MAX_THRESHOLDS) + " distance thresholds are supported");
+    if (p->n_thresholds == 0) fail(CS_EINVAL, "at least one distance threshold is required");
     if (!p->compute_closeness && !p->compute_betweenness)
       fail(CS_EINVAL, "Either or both closeness and betweenness flags is required, but both parameters are False.");
     if (!p->sources || !p->source_wt || !d_cycles_wt || !p->eligible)
       fail(CS_EINVAL, "device sources, weights, cycle weights and eligibility must not be NULL");
     const int dev = current_device();
-    const CallSpec spec = make_spec(p);
+    const uint32_t T = p->n_thresholds, N = g.n;
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
     std::lock_guard<std::mutex> lk(g.mu);
     DevGraph &dg = device_graph(g, dev);
-    DevEdges &de = device_edges(g, dg, spec.speed);
+    DevEdges &de = device_edges(g, dg, p->speed_m_s);
     run_shortest_checks(g, p);
-    launch_shortest_original(g, dg, de, spec, p->sources, p->source_wt, d_cycles_wt, p->n_sources, p->eligible,
-                             d_acc, d_reach, stream, t_stats);
+    for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
+      launch_shortest_original(g, dg, de, make_spec(p, t0, chunk_len(T, t0)), p->sources, p->source_wt,
+                               d_cycles_wt, p->n_sources, p->eligible, d_acc + (size_t)t0 * N,
+                               d_reach ? d_reach + t0 : nullptr, stream, t_stats, T);
     t_stats.sources = p->n_sources;
     g.progress = p->n_sources;
   });
@@ -842,8 +855,7 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
     if (n_pairs && (!origins || !destinations || !weights))
       fail(CS_EINVAL, "origins, destinations and weights must not be NULL");
     Graph &g = h->g;
-    if (p->n_thresholds == 0 || p->n_thresholds > CS_MAX_THRESHOLDS)
-      fail(CS_EINVAL, "between 1 and " + std::to_string(CS_MAX_THRESHOLDS) + " distance thresholds are supported");
+    if (p->n_thresholds == 0) fail(CS_EINVAL, "at least one distance threshold is required");
     if (!(p->speed_m_s > 0.0f) || !std::isfinite(p->speed_m_s))
       fail(CS_EINVAL, "speed_m_s must be finite and positive");
     if (!(p->tolerance >= 0.0f && p->tolerance <= 1.0f))
@@ -901,11 +913,7 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
       dest.swap(dest2);
       w.swap(w2);
     }
-    CallSpec spec = make_spec(p);
-    spec.closeness = false;
-    spec.betweenness = true;
-    spec.track_reach = false;
-    const uint32_t T = spec.T, M = 7;
+    const uint32_t T = p->n_thresholds, M = 7;
     const uint32_t ns = (uint32_t)src.size();
     const int dev = current_device();
     OwnedStream stream;
@@ -923,14 +931,21 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
       hip_check(hipMemcpyAsync(d_w.p, w.data(), 4 * w.size(), hipMemcpyHostToDevice, stream), "copy");
       std::lock_guard<std::mutex> lk(g.mu);
       DevGraph &dg = device_graph(g, dev);
-      DevEdges &de = device_edges(g, dg, spec.speed);
-      if (g.max_out_deg > 64) fail(CS_EINVAL, "nodes with more than 64 out-edges are not supported by the GPU search");
-      JobSpec js{&g, &dg, &de, spec, d_src.as<uint32_t>(), d_wt.as<float>(), d_wt.as<float>(), ns,
-                 d_elig.as<uint8_t>(), acc.as<double>(), nullptr, default_tuning().delta_scale};
-      js.d_od_off = d_off.as<uint32_t>();
-      js.d_od_dest = d_dest.as<uint32_t>();
-      js.d_od_w = d_w.as<float>();
-      launch_shortest_batch({js}, T, default_tuning(), stream, t_stats);
+      DevEdges &de = device_edges(g, dg, p->speed_m_s);
+      check_betweenness_graph(g);
+      for (uint32_t t0 = 0; t0 < T; t0 += kMaxT) {
+        CallSpec spec = make_spec(p, t0, chunk_len(T, t0));
+        spec.closeness = false;
+        spec.betweenness = true;
+        spec.track_reach = false;
+        JobSpec js{&g, &dg, &de, spec, d_src.as<uint32_t>(), d_wt.as<float>(), d_wt.as<float>(), ns,
+                   d_elig.as<uint8_t>(), acc.as<double>() + (size_t)t0 * N, nullptr, default_tuning().delta_scale};
+        js.tstride = T;
+        js.d_od_off = d_off.as<uint32_t>();
+        js.d_od_dest = d_dest.as<uint32_t>();
+        js.d_od_w = d_w.as<float>();
+        launch_shortest_batch({js}, spec.T, default_tuning(), stream, t_stats);
+      }
     }
     std::vector<double> hacc((size_t)N * M * T);
     hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
@@ -953,8 +968,7 @@ int cs_segment_centrality(cs_graph *h, const cs_params *p, double *out) {
     Graph &g = h->g;
     check_params(g, p);
     const int dev = current_device();
-    const CallSpec spec = make_spec(p);
-    const uint32_t N = g.n, T = spec.T, M = 4;
+    const uint32_t N = g.n, T = p->n_thresholds, M = 4;
     // the reference looks up twin edges (graph.rs:1278-1292) and panics if one is missing
     for (uint32_t e = 0; e < g.m; ++e)
       if (g.twin[e] < 0)
@@ -967,14 +981,15 @@ int cs_segment_centrality(cs_graph *h, const cs_params *p, double *out) {
     std::sort(srcs.begin(), srcs.end());
     std::lock_guard<std::mutex> lk(g.mu);
     DevGraph &dg = device_graph(g, dev);
-    DevEdges &de = device_edges(g, dg, spec.speed);
+    DevEdges &de = device_edges(g, dg, p->speed_m_s);
     OwnedStream stream;
     DevBuf acc(sizeof(double) * (size_t)N * M * T), d_src(4 * srcs.size());
     hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
     if (!srcs.empty())
       hip_check(hipMemcpyAsync(d_src.p, srcs.data(), 4 * srcs.size(), hipMemcpyHostToDevice, stream), "copy");
-    launch_segment(g, dg, de, spec, d_src.as<uint32_t>(), (uint32_t)srcs.size(), acc.as<double>(), stream,
-                   t_stats);
+    for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
+      launch_segment(g, dg, de, make_spec(p, t0, chunk_len(T, t0)), d_src.as<uint32_t>(), (uint32_t)srcs.size(),
+                     acc.as<double>() + (size_t)t0 * N, stream, t_stats, T);
     std::vector<double> hacc((size_t)N * M * T);
     hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
               "copy out");
@@ -996,29 +1011,30 @@ int cs_centrality_simplest(cs_graph *h, const cs_params *p, double *out, uint32_
     check_simplest_graph(g);
     check_params(g, p);
     const int dev = current_device();
-    const CallSpec spec = make_spec(p);
     const SourceSet ss = source_set(g, p);
-    const uint32_t N = g.n, T = spec.T, M = 4;
+    const uint32_t N = g.n, T = p->n_thresholds, M = 4;
     std::lock_guard<std::mutex> lk(g.mu);
     DevGraph &dg = device_graph(g, dev);
-    DevEdges &de = device_edges(g, dg, spec.speed);
+    DevEdges &de = device_edges(g, dg, p->speed_m_s);
     OwnedStream stream;
-    DevBuf acc(sizeof(double) * (size_t)N * M * T), reach(4 * kMaxT), d_src(4 * ss.src.size()),
+    DevBuf acc(sizeof(double) * (size_t)N * M * T), reach(4 * T), d_src(4 * ss.src.size()),
         d_wt(4 * ss.src.size()), d_elig(N);
     hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
-    hip_check(hipMemsetAsync(reach.p, 0, 4 * kMaxT, stream), "memset");
+    hip_check(hipMemsetAsync(reach.p, 0, 4 * T, stream), "memset");
     if (!ss.src.empty()) {
       hip_check(hipMemcpyAsync(d_src.p, ss.src.data(), 4 * ss.src.size(), hipMemcpyHostToDevice, stream), "copy");
       hip_check(hipMemcpyAsync(d_wt.p, ss.wt.data(), 4 * ss.wt.size(), hipMemcpyHostToDevice, stream), "copy");
     }
     hip_check(hipMemcpyAsync(d_elig.p, p->eligible, N, hipMemcpyHostToDevice, stream), "copy");
-    launch_simplest(g, dg, de, spec, d_src.as<uint32_t>(), d_wt.as<float>(), (uint32_t)ss.src.size(),
-                    d_elig.as<uint8_t>(), acc.as<double>(), reach.as<uint32_t>(), stream, t_stats);
+    for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
+      launch_simplest(g, dg, de, make_spec(p, t0, chunk_len(T, t0)), d_src.as<uint32_t>(), d_wt.as<float>(),
+                      (uint32_t)ss.src.size(), d_elig.as<uint8_t>(), acc.as<double>() + (size_t)t0 * N,
+                      reach.as<uint32_t>() + t0, stream, t_stats, T);
     std::vector<double> hacc((size_t)N * M * T);
-    uint32_t hreach[kMaxT];
+    std::vector<uint32_t> hreach(T);
     hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
               "copy out");
-    hip_check(hipMemcpyAsync(hreach, reach.p, 4 * kMaxT, hipMemcpyDeviceToHost, stream), "copy out");
+    hip_check(hipMemcpyAsync(hreach.data(), reach.p, 4 * T, hipMemcpyDeviceToHost, stream), "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
     unpack_acc(hacc, N, M, T, out, nullptr);
     if (p->compute_betweenness) scale_rows(out, N, T, 3, 3, p->betw_scale);
@@ -1050,6 +1066,7 @@ Single run_single(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, 
     fail(CS_EINVAL, "src_idx " + std::to_string(src) + " out of range for network with node_bound " +
                         std::to_string(g.n));
   if (!std::isfinite(speed) || !(speed > 0.0f)) fail(CS_EINVAL, "speed_m_s must be finite and positive");
+  if (mode == 1) check_betweenness_graph(g);
   const int dev = current_device();
   const uint32_t N = g.n;
   std::lock_guard<std::mutex> lk(g.mu);
@@ -1145,7 +1162,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     if (!h) fail(CS_EINVAL, "graph must not be NULL");
     Graph &g = h->g;
     const uint32_t N = g.n, T = n_thresholds, C = n_classes;
-    if (T == 0 || T > kMaxT) fail(CS_EINVAL, "between 1 and " + std::to_string(kMaxT) + " distances are supported");
+    if (T == 0) fail(CS_EINVAL, "at least one distance is required");
     if (!distances || !betas || !max_curve_wts) fail(CS_EINVAL, "distances, betas and max_curve_wts must not be NULL");
     if (n_entries && (!entry_group || !entry_class)) fail(CS_EINVAL, "entry_group and entry_class must not be NULL");
     if (n_assign && (!assign_node || !assign_entry || !assign_dist))
@@ -1207,17 +1224,27 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     to_device_ids(g, srcs);
     std::sort(srcs.begin(), srcs.end());
     AccessOut ao;
-    ao.T = T;
     ao.N = N;
+    ao.tstride = T;
     uint32_t dmax = 0;
     for (uint32_t t = 0; t < T; ++t) dmax = std::max(dmax, distances[t]);
-    for (uint32_t t = 0; t < T; ++t) {
-      ao.dist[t] = (float)distances[t];
-      ao.beta[t] = betas[t];
-      ao.mcw[t] = max_curve_wts[t];
-      if (distances[t] == dmax) ao.max_mask |= 1u << t;
-    }
     ao.maxdist = (float)max_seconds * speed_m_s;  // data.rs:532
+    // the launch takes at most kMaxT thresholds: chunk t0 covers t0 .. t0 + T - 1
+    // of the rows (every chunk searches to the same walk distance)
+    auto chunk = [&](uint32_t t0) {
+      AccessOut c = ao;
+      c.T = chunk_len(T, t0);
+      c.max_mask = 0;
+      for (uint32_t t = 0; t < c.T; ++t) {
+        c.dist[t] = (float)distances[t0 + t];
+        c.beta[t] = betas[t0 + t];
+        c.mcw[t] = max_curve_wts[t0 + t];
+        if (distances[t0 + t] == dmax) c.max_mask |= 1u << t;
+      }
+      c.count = ao.count + (size_t)t0 * N;
+      c.weighted = ao.weighted + (size_t)t0 * N;
+      return c;
+    };
     const int dev = current_device();
     std::lock_guard<std::mutex> lk(g.mu);
     DevGraph &dg = device_graph(g, dev);
@@ -1250,11 +1277,14 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     ao.weighted = d_wt.as<double>();
     ao.nearest = d_near.as<unsigned long long>();
     if (!srcs.empty())
-      launch_access(g, dg, de, (float)max_seconds, d_src.as<uint32_t>(), (uint32_t)srcs.size(), ao, stream, t_stats);
+      for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
+        launch_access(g, dg, de, (float)max_seconds, d_src.as<uint32_t>(), (uint32_t)srcs.size(), chunk(t0), stream,
+                      t_stats);
     const uint32_t *ip = result_order(g);
     if (out_div) {
       DevBuf d_div(8ull * kDivMetrics * T * N);
-      launch_diversity(ao, C, d_div.as<double>(), stream);
+      for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
+        launch_diversity(chunk(t0), C, d_div.as<double>() + (size_t)t0 * N, stream);
       std::vector<double> hd((size_t)kDivMetrics * T * N);
       hip_check(hipMemcpyAsync(hd.data(), d_div.p, 8 * hd.size(), hipMemcpyDeviceToHost, stream), "copy out");
       hip_check(hipStreamSynchronize(stream), "sync");
@@ -1532,6 +1562,9 @@ int cs_batch_add_shortest(cs_batch *b, cs_graph *h, const cs_params *p, double *
     if (!b || !h || !out) fail(CS_EINVAL, "batch, graph and out must not be NULL");
     Graph &g = h->g;
     check_params(g, p);
+    if (p->n_thresholds > CS_MAX_THRESHOLDS)
+      fail(CS_EINVAL, "a batched request takes at most " + std::to_string(CS_MAX_THRESHOLDS) +
+                          " distance thresholds (split it into several requests)");
     run_shortest_checks(g, p);
     cs_batch::JobHost j;
     j.h = h;

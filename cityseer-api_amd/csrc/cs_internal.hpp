@@ -133,10 +133,12 @@ struct CallSpec {
 // Shortest-path centrality over `n_sources` sources; acc is a device array
 // [7][T][N] (metric-major, the reference's row order) that the call
 // accumulates into; reach [T] u32.
+// tstride: threshold stride of d_acc's rows (0: cs.T); a threshold chunk
+// passes its first row and the call's threshold count.
 void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                      uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
-                     uint32_t *d_reach, hipStream_t stream, cs_stats &st);
+                     uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride = 0);
 
 // launch_shortest with sources, eligibility and acc in original node order
 // (device buffers of the caller), translated to and from the graph's device
@@ -146,7 +148,7 @@ void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
 void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                               const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                               uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
-                              uint32_t *d_reach, hipStream_t stream, cs_stats &st);
+                              uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride = 0);
 
 // Launch shape of the search kernel: workgroup size (64..256), a cap on
 // resident waves per SIMD (0 = as many as fit), the bucket width as a fraction
@@ -209,7 +211,7 @@ void batch_run(ShortestBatch &b, hipStream_t stream, cs_stats &st);
 // Segment centrality; acc [4][T][N].
 void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                     const uint32_t *d_sources, uint32_t n_sources, double *d_acc,
-                    hipStream_t stream, cs_stats &st);
+                    hipStream_t stream, cs_stats &st, uint32_t tstride = 0);
 
 // Single-source outputs (trees and Brandes state), device arrays of length N.
 struct TreeOut {
@@ -235,6 +237,7 @@ struct AccessOut {
   double *weighted = nullptr;            // [C][T][N] clipped negative-exponential weights
   unsigned long long *nearest = nullptr; // [C][N] f64 bits of the nearest entry (NaN bits: none)
   uint32_t T = 0, N = 0;
+  uint32_t tstride = 0;   // threshold stride of the count / weighted rows (and of the diversity rows)
   uint32_t max_mask = 0;  // thresholds equal to the largest distance
   float maxdist = 0.0f;   // max walk seconds * speed
   float dist[kMaxT], beta[kMaxT], mcw[kMaxT];
@@ -261,8 +264,9 @@ struct AngSpec {
   const uint32_t *d_sources;
   const float *d_wt;
   const uint8_t *d_eligible;
-  double *d_acc;  // [4][T][N]
+  double *d_acc;  // [4][tstride][N]
   uint32_t *d_reach;
+  uint32_t tstride = 0;  // threshold stride of d_acc (0: cs.T)
 };
 
 // Simplest-path centrality for several requests in one launch: the items
@@ -273,7 +277,7 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
 void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, uint32_t n_sources,
                      const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
-                     hipStream_t stream, cs_stats &st);
+                     hipStream_t stream, cs_stats &st, uint32_t tstride = 0);
 
 // dijkstra_tree_simplest for one source (cs_angular.hip); device arrays of
 // length N, pre-filled by the caller (INF / -1 / 0).

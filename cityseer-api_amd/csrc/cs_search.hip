@@ -45,7 +45,7 @@ namespace cs {
 
 namespace {
 
-constexpr uint32_t kFailCap = 1u, kFailBucket = 2u, kFailRing = 4u, kFailInternal = 8u;
+constexpr uint32_t kFailCap = 1u, kFailBucket = 2u, kFailRing = 4u, kFailInternal = 8u, kFailWide = 16u;
 constexpr int kChunk = 1;           // in-edges per lane per step of the segment closeness walk
 constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dealt to them round-robin
 constexpr uint32_t kScanQ = 4;     // key words per lane per step of the node-ordered result flush
@@ -239,12 +239,54 @@ __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uin
   return w;
 }
 
+// Predecessor set of a settled node, over the positions of its out-row: a
+// bitmask when the row has at most 64 entries; for a wider row, up to four
+// positions of 16 bits each (position + 1, packed from the low end in the
+// order they joined, 0 after the last). A fifth predecessor at a wide row is
+// refused (kFailWide).
+constexpr uint32_t kWideRow = 64u, kWideSlots = 4u;  // rows up to 65,534 entries (checked on the host)
+__device__ __forceinline__ uint64_t ps_single(uint32_t p, bool wide) {
+  return wide ? (uint64_t)(p + 1u) : 1ull << p;
+}
+__device__ __forceinline__ uint64_t ps_add(uint64_t m, uint32_t p, bool wide, bool &overflow) {
+  if (!wide) return m | (1ull << p);
+  for (uint32_t k = 0; k < kWideSlots; ++k) {
+    const uint32_t x = (uint32_t)(m >> (16u * k)) & 0xffffu;
+    if (x == p + 1u) return m;
+    if (x == 0u) return m | ((uint64_t)(p + 1u) << (16u * k));
+  }
+  overflow = true;
+  return m;
+}
+__device__ __forceinline__ uint32_t ps_count(uint64_t m, bool wide) {
+  if (!wide) return (uint32_t)__popcll(m);
+  uint32_t n = 0;
+  for (; m; m >>= 16) ++n;
+  return n;
+}
+__device__ __forceinline__ uint32_t ps_first(uint64_t m, bool wide) {
+  return wide ? (uint32_t)(m & 0xffffu) - 1u : (uint32_t)__builtin_ctzll(m);
+}
+// Takes the next position out of m; false once m is empty.
+__device__ __forceinline__ bool ps_next(uint64_t &m, bool wide, uint32_t &p) {
+  if (!m) return false;
+  if (wide) {
+    p = (uint32_t)(m & 0xffffu) - 1u;
+    m >>= 16;
+  } else {
+    p = (uint32_t)__builtin_ctzll(m);
+    m &= m - 1;
+  }
+  return true;
+}
+
 // Multi-predecessor fold of one candidate, exactly as the reference's Phase 1
 // relaxation of node nb from settled node u (centrality.rs:1413-1437).
 struct Fold {
   float best;
   double sigma;
   uint64_t pmask;
+  bool wide, overflow;
 };
 
 __device__ __forceinline__ void fold_apply(Fold &f, float cs, double sigma_u, uint32_t j,
@@ -253,15 +295,15 @@ __device__ __forceinline__ void fold_apply(Fold &f, float cs, double sigma_u, ui
   const float one_p = 1.0f + kTieEps;
   if (cs < f.best) {
     if (cs < f.best * one_m) {
-      f.pmask = 1ull << j;
+      f.pmask = ps_single(j, f.wide);
       f.sigma = sigma_u;
     } else {
-      f.pmask |= 1ull << j;
+      f.pmask = ps_add(f.pmask, j, f.wide, f.overflow);
       f.sigma += sigma_u;
     }
     f.best = cs;
   } else if (cs <= f.best * one_p && !u_in_preds) {
-    f.pmask |= 1ull << j;
+    f.pmask = ps_add(f.pmask, j, f.wide, f.overflow);
     f.sigma += sigma_u;
   }
 }
@@ -295,6 +337,8 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
   f.best = INFINITY;
   f.sigma = 0.0;
   f.pmask = 0;
+  f.wide = (r1 - r0) > kWideRow;
+  f.overflow = false;
   bool have_last = false;
   float lsu = 0.0f;
   uint32_t lu = 0, lj = 0;
@@ -324,10 +368,9 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
     }
     if (!found) return false;
     bool contains = false;
-    for (uint64_t m = f.pmask; m; m &= m - 1) {
-      const uint32_t jb = __builtin_ctzll(m);
+    uint32_t jb;
+    for (uint64_t m = f.pmask; ps_next(m, f.wide, jb);)
       if ((j.fwd_adj[r0 + jb].x & kNodeMask) == pu) contains = true;
-    }
     const double sg = ldd(reinterpret_cast<const double *>(npay(w, pu)));
     fold_apply(f, pcs, sg, pj, contains);
     have_last = true;
@@ -335,7 +378,7 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
     lu = pu;
     lj = pj;
   }
-  return f.best == secs_v;
+  return f.best == secs_v && !f.overflow;
 }
 
 struct SrcCtx {
@@ -505,23 +548,26 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
         }
         // settle record
         if constexpr (kMulti) {
+          const bool wide = nf > kWideRow;
           Fold f;
           f.sigma = 1.0;
           f.pmask = 0;
+          f.wide = wide;
+          f.overflow = false;
           uint32_t pslot0 = 0, npred = 0;
           if (!is_src && fold) {
             if (n_near == 1) {
-              f.pmask = 1ull << near_j;
+              f.pmask = ps_single(near_j, wide);
               f.sigma = __longlong_as_double((long long)near_pay);
               pslot0 = near_slot;
               npred = 1;
               if (near_cs != secs_v) lfail |= kFailInternal;
             } else if (n_near == 0 ||
                        !brandes_fold_exact(j, w, c.tag, v, secs_v, c.k, f0, f0 + nf, thr_far, n_near, f)) {
-              lfail |= kFailInternal;
+              lfail |= f.overflow ? kFailWide : kFailInternal;
             } else {
-              npred = (uint32_t)__popcll(f.pmask);
-              const uint32_t u0 = j.fwd_adj[f0 + __builtin_ctzll(f.pmask)].x & kNodeMask;
+              npred = ps_count(f.pmask, wide);
+              const uint32_t u0 = j.fwd_adj[f0 + ps_first(f.pmask, wide)].x & kNodeMask;
               pslot0 = key_slot(ld64(nkey(w, u0)));
             }
           }
@@ -717,9 +763,10 @@ __device__ __forceinline__ uint32_t next_item(const SearchArgs &a, uint32_t x) {
 // resolve (cannot happen: every predecessor settles strictly earlier).
 constexpr uint32_t kP2Done = 0x200u;
 
-__device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
+__device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c, uint32_t &fail) {
   const uint32_t lane = lane_id();
   const float band = 1.0f + j.tol;
+  bool overflow = false;
   for (uint32_t b = 0; b < c.nb; ++b) {
     const uint32_t s0 = uniform(ld32(&w.bnd[b])), s1 = uniform(ld32(&w.bnd[b + 1]));
     // predecessor masks over the out-row positions of each node
@@ -735,6 +782,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
         std_(reinterpret_cast<double *>(npay(w, v)), 1.0);
       } else {
         const uint4 rw = j.rows[v];
+        const bool wide = (rw.y - rw.x) > kWideRow;
         const float lim = secs_v * band;
         for (uint32_t p = rw.x; p < rw.y; ++p) {
           const uint2 e = j.fwd_adj[p];
@@ -747,13 +795,18 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
           const float ps = su + __uint_as_float(e.y);
           if (!(ps <= lim)) continue;
           bool dup = false;
-          for (uint64_t mm = pm; mm; mm &= mm - 1)
-            if ((j.fwd_adj[rw.x + __builtin_ctzll(mm)].x & kNodeMask) == u) dup = true;
-          if (!dup) pm |= 1ull << (p - rw.x);
+          uint32_t q;
+          for (uint64_t mm = pm; ps_next(mm, wide, q);)
+            if ((j.fwd_adj[rw.x + q].x & kNodeMask) == u) dup = true;
+          if (!dup) pm = ps_add(pm, p - rw.x, wide, overflow);
         }
       }
       st64(&w.pm[s], pm);
       st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)meta << 32) | (uint32_t)hi);
+    }
+    if (uniform(wave_sum((uint32_t)overflow))) {
+      fail |= kFailWide;
+      return false;
     }
     wave_fence();
     for (uint32_t round = 0;; ++round) {
@@ -764,10 +817,13 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
         if (meta & kP2Done) continue;
         const uint32_t v = (uint32_t)ld64(&w.rec[2 * (size_t)s]);
         const uint64_t pm = ld64(&w.pm[s]);
-        const uint32_t f0 = j.rows[v].x;
+        const uint4 rv = j.rows[v];
+        const uint32_t f0 = rv.x;
+        const bool wide = (rv.y - rv.x) > kWideRow;
         bool ready = true;
-        for (uint64_t mm = pm; mm; mm &= mm - 1) {
-          const uint32_t u = j.fwd_adj[f0 + __builtin_ctzll(mm)].x & kNodeMask;
+        uint32_t q;
+        for (uint64_t mm = pm; ps_next(mm, wide, q);) {
+          const uint32_t u = j.fwd_adj[f0 + q].x & kNodeMask;
           const uint32_t su = key_slot(ld64(nkey(w, u)));
           const uint32_t mu = (uint32_t)(ld64(&w.rec[2 * (size_t)su + 1]) >> 32);
           // same-bucket predecessors must have resolved in an earlier round
@@ -783,13 +839,13 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
         bool have_last = false;
         float lsu = 0.0f;
         uint32_t lu = 0;
-        const uint32_t np = (uint32_t)__popcll(pm);
+        const uint32_t np = ps_count(pm, wide);
         for (uint32_t step = 0; step < np; ++step) {
           bool found = false;
           float bsu = 0.0f;
           uint32_t bu = 0;
-          for (uint64_t mm = pm; mm; mm &= mm - 1) {
-            const uint32_t u = j.fwd_adj[f0 + __builtin_ctzll(mm)].x & kNodeMask;
+          for (uint64_t mm = pm; ps_next(mm, wide, q);) {
+            const uint32_t u = j.fwd_adj[f0 + q].x & kNodeMask;
             const float su = key_secs(ld64(nkey(w, u)));
             if (have_last && !key_less(j.orig, lsu, lu, su, u)) continue;
             if (!found || key_less(j.orig, su, u, bsu, bu)) {
@@ -803,7 +859,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
           lsu = bsu;
           lu = bu;
         }
-        if (pm) pslot0 = key_slot(ld64(nkey(w, (j.fwd_adj[f0 + __builtin_ctzll(pm)].x & kNodeMask))));
+        if (pm) pslot0 = key_slot(ld64(nkey(w, (j.fwd_adj[f0 + ps_first(pm, wide)].x & kNodeMask))));
         std_(reinterpret_cast<double *>(npay(w, v)), sigma);
         st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np | (round << 16)) << 32) | pslot0);
       }
@@ -825,8 +881,9 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c) {
 // index order), same-bucket ones included, once every such candidate has been
 // folded itself: in rounds, as Phase 2 does, whose numbers become the levels
 // back-propagation walks.
-__device__ bool phase1_fold_after(CJob &j, const WaveWs &w, const SrcCtx &c) {
+__device__ bool phase1_fold_after(CJob &j, const WaveWs &w, const SrcCtx &c, uint32_t &fail) {
   const uint32_t lane = lane_id();
+  bool overflow = false;
   for (uint32_t b = 0; b < c.nb; ++b) {
     const uint32_t s0 = uniform(ld32(&w.bnd[b])), s1 = uniform(ld32(&w.bnd[b + 1]));
     // candidate masks over the out-row positions of each node
@@ -842,6 +899,7 @@ __device__ bool phase1_fold_after(CJob &j, const WaveWs &w, const SrcCtx &c) {
         std_(reinterpret_cast<double *>(npay(w, v)), 1.0);
       } else {
         const uint4 rw = j.rows[v];
+        const bool wide = (rw.y - rw.x) > kWideRow;
         const double thr_far = (double)secs_v * far_factor(rw.y - rw.x);
         for (uint32_t p = rw.x; p < rw.y; ++p) {
           const uint2 e = j.fwd_adj[p];
@@ -853,11 +911,15 @@ __device__ bool phase1_fold_after(CJob &j, const WaveWs &w, const SrcCtx &c) {
           if (!key_less(j.orig, su, u, secs_v, v)) continue;
           const float cs = su + __uint_as_float(e.y);
           if (cs > j.maxsec || (double)cs > thr_far) continue;
-          cm |= 1ull << (p - rw.x);
+          cm = ps_add(cm, p - rw.x, wide, overflow);
         }
       }
       st64(&w.pm[s], cm);
       st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)meta << 32) | (uint32_t)hi);
+    }
+    if (uniform(wave_sum((uint32_t)overflow))) {
+      fail |= kFailWide;
+      return false;
     }
     wave_fence();
     for (uint32_t round = 0;; ++round) {
@@ -871,9 +933,11 @@ __device__ bool phase1_fold_after(CJob &j, const WaveWs &w, const SrcCtx &c) {
         const float secs_v = __uint_as_float((uint32_t)(lo >> 32));
         const uint64_t cm = ld64(&w.pm[s]);
         const uint4 rw = j.rows[v];
+        const bool wide = (rw.y - rw.x) > kWideRow;
         bool ready = true;
-        for (uint64_t mm = cm; mm; mm &= mm - 1) {
-          const uint32_t u = j.fwd_adj[rw.x + __builtin_ctzll(mm)].x & kNodeMask;
+        uint32_t q;
+        for (uint64_t mm = cm; ps_next(mm, wide, q);) {
+          const uint32_t u = j.fwd_adj[rw.x + q].x & kNodeMask;
           const uint32_t su = key_slot(ld64(nkey(w, u)));
           const uint32_t mu = (uint32_t)(ld64(&w.rec[2 * (size_t)su + 1]) >> 32);
           // same-bucket candidates must have been folded in an earlier round
@@ -884,20 +948,25 @@ __device__ bool phase1_fold_after(CJob &j, const WaveWs &w, const SrcCtx &c) {
           continue;
         }
         Fold f;
-        const uint32_t n_near = (uint32_t)__popcll(cm);
+        f.overflow = false;
+        const uint32_t n_near = ps_count(cm, wide);
         const double thr_far = (double)secs_v * far_factor(rw.y - rw.x);
         if (n_near == 0 || !brandes_fold_exact(j, w, c.tag, v, secs_v, 0u, rw.x, rw.y, thr_far, n_near, f, true)) {
+          if (f.overflow) overflow = true;
           ++bad;
           continue;
         }
-        const uint32_t np = (uint32_t)__popcll(f.pmask);
-        const uint32_t u0 = j.fwd_adj[rw.x + __builtin_ctzll(f.pmask)].x & kNodeMask;
+        const uint32_t np = ps_count(f.pmask, wide);
+        const uint32_t u0 = j.fwd_adj[rw.x + ps_first(f.pmask, wide)].x & kNodeMask;
         const uint32_t pslot0 = key_slot(ld64(nkey(w, u0)));
         st64(&w.pm[s], f.pmask);
         std_(reinterpret_cast<double *>(npay(w, v)), f.sigma);
         st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np | (round << 16)) << 32) | pslot0);
       }
-      if (uniform(wave_sum(bad))) return false;
+      if (uniform(wave_sum(bad))) {
+        if (uniform(wave_sum((uint32_t)overflow))) fail |= kFailWide;
+        return false;
+      }
       pending = uniform(wave_sum(pending));
       if (pending == 0 && lane_id() == 0) st32(&w.lvl[b], round);
       wave_fence();
@@ -1048,8 +1117,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
     bool ok = search_source<MODE, NT>(a, j, w, c, nband, eband, st_edges);
     if constexpr (MODE == kBrandes || MODE == kBrandesExport) {
       if (ok && (j.tol > kTieEps || (j.flags & 16u)) && (MODE == kBrandesExport || (j.flags & 2u))) {
-        ok = j.tol > kTieEps ? phase2_rebuild(j, w, c) : phase1_fold_after(j, w, c);
-        if (!ok) c.fail |= kFailInternal;
+        uint32_t pfail = 0;
+        ok = j.tol > kTieEps ? phase2_rebuild(j, w, c, pfail) : phase1_fold_after(j, w, c, pfail);
+        if (!ok) c.fail |= pfail ? pfail : kFailInternal;
       }
     }
     if (!ok) {
@@ -1136,10 +1206,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
                 }
               } else if (npred > 1) {
                 const uint64_t pm = ld64(&w.pm[s]);
-                const uint32_t f0 = j.rows[v].x;
+                const uint4 rv = j.rows[v];
+                const uint32_t f0 = rv.x;
+                const bool wide = (rv.y - rv.x) > kWideRow;
                 const double sigma_w = ldd(reinterpret_cast<const double *>(npay(w, v)));
-                for (uint64_t mm = pm; mm; mm &= mm - 1) {
-                  const uint32_t jb = __builtin_ctzll(mm);
+                uint32_t jb;
+                for (uint64_t mm = pm; ps_next(mm, wide, jb);) {
                   const uint32_t u = j.fwd_adj[f0 + jb].x & kNodeMask;
                   const uint4 ru = nrec(w, u);
                   const uint32_t su = key_slot(lo64(ru));
@@ -1348,7 +1420,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
           if (cls < 0) continue;
           for (uint32_t t = 0; t < ao.T; ++t) {
             if (!(tot <= ao.dist[t])) continue;
-            const size_t row = ((size_t)cls * ao.T + t) * ao.N + src;
+            const size_t row = ((size_t)cls * ao.tstride + t) * ao.N + src;
             gadd_f64(&ao.count[row], 1.0);
             // clipped_beta_wt (common.rs:339-348): min(exp(-beta d), mcw) / mcw in f32
             const float raw = expf(__fmul_rn(-ao.beta[t], tot));
@@ -1376,9 +1448,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
           const uint64_t pm = ld64(&w.pm[s]);
           a.tout.sigma[v] = ldd(reinterpret_cast<const double *>(npay(w, v)));
           uint32_t np = 0;
-          const uint32_t f0 = j.rows[v].x;
-          for (uint64_t mm = pm; mm; mm &= mm - 1) {
-            const uint32_t jb = __builtin_ctzll(mm);
+          const uint4 rv = j.rows[v];
+          const uint32_t f0 = rv.x;
+          const bool wide = (rv.y - rv.x) > kWideRow;
+          uint32_t jb;
+          for (uint64_t mm = pm; ps_next(mm, wide, jb);) {
             if (np < a.tout.max_preds)
               a.tout.pred_list[(size_t)v * a.tout.max_preds + np] = j.fwd_adj[f0 + jb].x & kNodeMask;
             ++np;
@@ -1403,6 +1477,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
 // ---------------------------------------------------------------------------
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+[[noreturn]] void fail_wide_row() {
+  fail(CS_EINVAL, "a node with more than 64 out-edges has more than 4 tied predecessors in one search, which the "
+                  "GPU search does not support");
+}
 
 template <int MODE, int NT>
 void launch_one(uint32_t blocks, uint32_t block, size_t lds, hipStream_t stream, const SearchArgs &a) {
@@ -1843,6 +1922,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     st.waves = waves;
     const uint32_t n_failed = ctl[kXcd], flags = ctl[kXcd + 1];
     if (n_failed == 0) break;
+    if (flags & kFailWide) fail_wide_row();
     if (flags & kFailInternal) fail(CS_EINTERNAL, "search invariant violated (bucket order)");
     if (flags & kFailRing) fail(CS_EINTERNAL, "bucket ring window exceeded");
     st.retries += n_failed;
@@ -2061,6 +2141,7 @@ struct ShortestBatch {
     st.within_sum += stv[2];
     const uint32_t n_failed = ctl[kXcd], flags = ctl[kXcd + 1];
     if (n_failed == 0) return;
+    if (flags & kFailWide) fail_wide_row();
     if (flags & kFailInternal) fail(CS_EINTERNAL, "search invariant violated (bucket order)");
     if (flags & kFailRing) fail(CS_EINTERNAL, "bucket ring window exceeded");
     st.retries += n_failed;
@@ -2127,9 +2208,10 @@ Tuning default_tuning() { return current_tuning(); }
 void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                      uint32_t n_sources, const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
-                     hipStream_t stream, cs_stats &st) {
+                     hipStream_t stream, cs_stats &st, uint32_t tstride) {
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, d_wt, d_cyc, n_sources, d_eligible, d_acc, d_reach, tune.delta_scale};
+  js.tstride = tstride;
   run_jobs<kBrandes>({js}, cs.T, tune, nullptr, stream, st);
 }
 
@@ -2157,14 +2239,28 @@ __global__ void k_scatter_add_rows(double *dst, const double *src, const uint32_
   }
 }
 
+// dst[m][t0 + t][orig[d]] += src[m][t][d] (dst rows of stride tstride thresholds, src of T): the
+// rows of a threshold chunk scattered back to original node order
+__global__ void k_scatter_add_rows_t(double *dst, const double *src, const uint32_t *orig, uint32_t N, uint32_t M,
+                                     uint32_t T, uint32_t tstride, uint32_t t0) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (size_t)N * M * T) {
+    const size_t r = i / N, d = i - r * N;
+    const size_t m = r / T, t = r - m * T;
+    dst[(m * tstride + t0 + t) * N + orig[d]] += src[i];
+  }
+}
+
 void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                               const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                               uint32_t n_sources, const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
-                              hipStream_t stream, cs_stats &st) {
+                              hipStream_t stream, cs_stats &st, uint32_t tstride) {
   if (g.identity_order) {
-    launch_shortest(g, dg, de, cs, d_sources, d_wt, d_cyc, n_sources, d_eligible, d_acc, d_reach, stream, st);
+    launch_shortest(g, dg, de, cs, d_sources, d_wt, d_cyc, n_sources, d_eligible, d_acc, d_reach, stream, st,
+                    tstride);
     return;
   }
+  if (!tstride) tstride = cs.T;
   const uint32_t N = g.n, rows = 7 * cs.T;
   constexpr uint32_t kB = 256;
   const size_t total = (size_t)rows * N;
@@ -2188,8 +2284,8 @@ void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSp
   hip_check(hipMemsetAsync(acc, 0, sizeof(double) * total, stream), "memset");
   hip_check(hipGetLastError(), "translate launch");
   launch_shortest(g, dg, de, cs, src, d_wt, d_cyc, n_sources, elig, acc, d_reach, stream, st);
-  hipLaunchKernelGGL(k_scatter_add_rows, dim3((uint32_t)((total + kB - 1) / kB)), dim3(kB), 0, stream, d_acc, acc,
-                     dg.orig, N, rows);
+  hipLaunchKernelGGL(k_scatter_add_rows_t, dim3((uint32_t)((total + kB - 1) / kB)), dim3(kB), 0, stream, d_acc, acc,
+                     dg.orig, N, 7u, cs.T, tstride, 0u);
   hip_check(hipGetLastError(), "scatter launch");
   // the staging is reused by the graph's next call once the lock is released
   hip_check(hipStreamSynchronize(stream), "sync");
@@ -2221,9 +2317,10 @@ void batch_run(ShortestBatch &b, hipStream_t stream, cs_stats &st) { b.run(strea
 
 void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                     const uint32_t *d_sources, uint32_t n_sources, double *d_acc, hipStream_t stream,
-                    cs_stats &st) {
+                    cs_stats &st, uint32_t tstride) {
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, d_acc, nullptr, tune.delta_scale};
+  js.tstride = tstride;
   run_jobs<kSegment>({js}, cs.T, tune, nullptr, stream, st);
 }
 
@@ -2247,7 +2344,7 @@ void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uin
 // classes summed in index order; a measure the reference reports as an error
 // (non-finite) is 0, as mixed_uses' unwrap_or(0.0) makes it (data.rs:866-941).
 __global__ void k_diversity(AccessOut ao, uint32_t C, double *div) {
-  const uint32_t T = ao.T, N = ao.N;
+  const uint32_t T = ao.T, N = ao.N, TS = ao.tstride;
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (uint64_t)T * N) return;
   const uint32_t t = (uint32_t)(i / N), v = (uint32_t)(i % N);
@@ -2255,7 +2352,7 @@ __global__ void k_diversity(AccessOut ao, uint32_t C, double *div) {
   for (uint32_t k = 0; k < kDivMetrics; ++k) out[k] = 0.0f;
   uint32_t num = 0, rich = 0;
   for (uint32_t c = 0; c < C; ++c) {
-    const uint32_t n = (uint32_t)ao.count[((size_t)c * T + t) * N + v];
+    const uint32_t n = (uint32_t)ao.count[((size_t)c * TS + t) * N + v];
     num += n;
     rich += n > 0;
   }
@@ -2265,7 +2362,7 @@ __global__ void k_diversity(AccessOut ao, uint32_t C, double *div) {
     float ent = 0.0f, sq = 0.0f, agg = 0.0f, lam = 0.0f;
     const float nm1 = (float)(num - 1);
     for (uint32_t c = 0; c < C; ++c) {
-      const uint32_t n = (uint32_t)ao.count[((size_t)c * T + t) * N + v];
+      const uint32_t n = (uint32_t)ao.count[((size_t)c * TS + t) * N + v];
       if (n == 0) continue;
       const float p = __fdiv_rn((float)n, numf);
       ent = __fadd_rn(ent, __fmul_rn(p, logf(p)));
@@ -2282,7 +2379,7 @@ __global__ void k_diversity(AccessOut ao, uint32_t C, double *div) {
     if (!(fabsf(agg) < __FLT_EPSILON__)) {
       float s0 = 0.0f, we = 0.0f, s2 = 0.0f;
       for (uint32_t c = 0; c < C; ++c) {
-        const uint32_t n = (uint32_t)ao.count[((size_t)c * T + t) * N + v];
+        const uint32_t n = (uint32_t)ao.count[((size_t)c * TS + t) * N + v];
         if (n == 0) continue;
         const float p = __fdiv_rn((float)n, numf);
         const float d = (float)__longlong_as_double((long long)ao.nearest[(size_t)c * N + v]);
@@ -2301,7 +2398,7 @@ __global__ void k_diversity(AccessOut ao, uint32_t C, double *div) {
     out[6] = isfinite(ent) ? fmaxf(-ent, 0.0f) : 0.0f;
     out[7] = num >= 2 ? fmaxf(__fsub_rn(1.0f, lam), 0.0f) : 0.0f;
   }
-  for (uint32_t k = 0; k < kDivMetrics; ++k) div[((size_t)k * T + t) * N + v] = (double)out[k];
+  for (uint32_t k = 0; k < kDivMetrics; ++k) div[((size_t)k * TS + t) * N + v] = (double)out[k];
 }
 
 void launch_diversity(const AccessOut &out, uint32_t n_classes, double *div, hipStream_t stream) {

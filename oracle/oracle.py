@@ -244,7 +244,7 @@ class OracleGraph:
         p, _keep = self._params(d, b, s, speed, tolerance, closeness, betweenness, sources, wt, eligible, track,
                                 scale, 180.0, 1.0, n_threads)
         out = np.zeros((7, len(d), self.n), dtype=np.float64)
-        reach = np.zeros(16, dtype=np.uint32)
+        reach = np.zeros(32, dtype=np.uint32)
         rc = load().og_centrality_shortest(self.h, ctypes.byref(p), _p(out, ctypes.c_double), _p(reach, ctypes.c_uint32))
         if rc != 0:
             raise ValueError(f"oracle error {rc}")
@@ -257,7 +257,7 @@ class OracleGraph:
         p, _keep = self._params(d, b, s, speed, tolerance, closeness, betweenness, sources, wt, eligible, track,
                                 scale, unit, offset, n_threads)
         out = np.zeros((4, len(d), self.n), dtype=np.float64)
-        reach = np.zeros(16, dtype=np.uint32)
+        reach = np.zeros(32, dtype=np.uint32)
         rc = load().og_centrality_simplest(self.h, ctypes.byref(p), _p(out, ctypes.c_double), _p(reach, ctypes.c_uint32))
         if rc != 0:
             raise ValueError(f"oracle error {rc}")
