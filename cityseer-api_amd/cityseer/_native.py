@@ -143,6 +143,10 @@ _SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.POINTER(CsParams), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     ),
+    "cs_centrality_shortest_device_noalloc": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(CsParams), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    ),
     "cs_centrality_simplest": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CsParams), _f64p, _u32p]),
     "cs_segment_centrality": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CsParams), _f64p]),
     "cs_accessibility": (

@@ -243,6 +243,17 @@ DevGraph &device_graph(Graph &g, int device) {
   return ref;
 }
 
+DevGraph *find_device_graph(Graph &g, int device) {
+  auto it = g.dev.find(device);
+  return it == g.dev.end() ? nullptr : it->second.get();
+}
+
+DevEdges *find_device_edges(DevGraph &dg, float speed) {
+  for (auto &e : dg.edges)
+    if (e->speed == speed) return e.get();
+  return nullptr;
+}
+
 DevEdges &device_edges(Graph &g, DevGraph &dg, float speed) {
   for (auto &e : dg.edges)
     if (e->speed == speed) return *e;

@@ -110,6 +110,9 @@ void set_node_order_from_xy(Graph &g, const double *x, const double *y);
 // Device residency (cs_host.cpp).
 DevGraph &device_graph(Graph &g, int device);
 DevEdges &device_edges(Graph &g, DevGraph &dg, float speed);
+// The same, without staging anything: null when not resident yet.
+DevGraph *find_device_graph(Graph &g, int device);
+DevEdges *find_device_edges(DevGraph &dg, float speed);
 
 // Per-call statistics (thread-local).
 cs_stats &thread_stats();
@@ -149,6 +152,16 @@ void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSp
                               const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                               uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
                               uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride = 0);
+
+// launch_shortest_original with every device buffer beyond the caller's
+// inputs and outputs carved from the device region ws (ws_bytes bytes,
+// 256-byte aligned): no allocation. The call's rows go to thresholds
+// t0 .. t0 + cs.T - 1 of d_acc, f64 [7][tstride][N]. Returns after the stream
+// has finished the call's work.
+void launch_shortest_in(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const uint32_t *d_sources,
+                        const float *d_wt, const float *d_cyc, uint32_t n_sources, const uint8_t *d_eligible,
+                        double *d_acc, uint32_t *d_reach, uint32_t tstride, uint32_t t0, hipStream_t stream,
+                        cs_stats &st, char *ws, size_t ws_bytes);
 
 // Launch shape of the search kernel: workgroup size (64..256), a cap on
 // resident waves per SIMD (0 = as many as fit), the bucket width as a fraction

@@ -2291,6 +2291,201 @@ void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSp
   hip_check(hipStreamSynchronize(stream), "sync");
 }
 
+// A job table entry written from a kernel argument (no host copy to the device).
+__global__ void k_put_job(Job j, Job *dst) {
+  if (threadIdx.x == 0) *dst = j;
+}
+
+// Items {job 0, source position or earlier item} of a retry pass from a launch's failure list.
+__global__ void k_retry_items(uint2 *out, const uint32_t *fail, uint32_t n, const uint2 *prev) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = prev ? prev[fail[i]] : make_uint2(0u, fail[i]);
+}
+
+// launch_shortest_original with every device buffer it needs beyond the
+// caller's inputs and outputs carved from `ws` (ws_bytes bytes, 256-byte
+// aligned): no allocation. The region holds, in order: control words and
+// statistics, the job table, the failure list and two retry item lists, the
+// node-order staging (graphs with a device order) and then, per wave, its
+// epoch, node records and scratch. As many waves run as the rest holds.
+void launch_shortest_in(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const uint32_t *d_sources,
+                        const float *d_wt, const float *d_cyc, uint32_t n_sources, const uint8_t *d_eligible,
+                        double *d_acc, uint32_t *d_reach, uint32_t tstride, uint32_t t0, hipStream_t stream,
+                        cs_stats &st, char *ws, size_t ws_bytes) {
+  if (n_sources == 0) return;
+  if (reinterpret_cast<uintptr_t>(ws) & 255u) fail(CS_EINTERNAL, "workspace region not 256-byte aligned");
+  const Tuning tune = current_tuning();
+  const uint32_t N = g.n, T = cs.T;
+  const bool translate = !g.identity_order;
+  const DevInfo &di = dev_info(dg.device);
+  constexpr uint32_t kB = 256;
+  // fixed parts
+  size_t o = 0;
+  const size_t o_ctl = o;
+  o = align256(o + 64);
+  const size_t o_stats = o;
+  o = align256(o + 32);
+  const size_t o_job = o;
+  o = align256(o + sizeof(Job));
+  const size_t o_fail = o;
+  o = align256(o + 4ull * n_sources);
+  const size_t o_items0 = o;
+  o = align256(o + 8ull * n_sources);
+  const size_t o_items1 = o;
+  o = align256(o + 8ull * n_sources);
+  const size_t rows = 7ull * T;
+  size_t o_sacc = 0, o_ssrc = 0, o_selig = 0;
+  if (translate) {
+    o_sacc = o;
+    o = align256(o + 8ull * rows * N);
+    o_ssrc = o;
+    o = align256(o + 4ull * n_sources);
+    o_selig = o;
+    o = align256(o + N);
+  }
+  const size_t fixed = o;
+  if (fixed > ws_bytes) fail(CS_ENOMEM, "call too large for the allocation-free workspace");
+  uint32_t *d_ctl = reinterpret_cast<uint32_t *>(ws + o_ctl);
+  unsigned long long *d_stats = reinterpret_cast<unsigned long long *>(ws + o_stats);
+  Job *d_job = reinterpret_cast<Job *>(ws + o_job);
+  uint32_t *d_fail = reinterpret_cast<uint32_t *>(ws + o_fail);
+  uint2 *d_items[2] = {reinterpret_cast<uint2 *>(ws + o_items0), reinterpret_cast<uint2 *>(ws + o_items1)};
+
+  const uint32_t *src = d_sources;
+  const uint8_t *elig = d_eligible;
+  double *acc = d_acc + (size_t)t0 * N;
+  uint32_t stride_t = tstride;
+  if (translate) {
+    uint32_t *ssrc = reinterpret_cast<uint32_t *>(ws + o_ssrc);
+    uint8_t *selig = reinterpret_cast<uint8_t *>(ws + o_selig);
+    acc = reinterpret_cast<double *>(ws + o_sacc);
+    stride_t = T;
+    hipLaunchKernelGGL(k_map_ids, dim3((n_sources + kB - 1) / kB), dim3(kB), 0, stream, ssrc, d_sources, dg.perm,
+                       n_sources, N);
+    hipLaunchKernelGGL(k_gather_u8, dim3((N + kB - 1) / kB), dim3(kB), 0, stream, selig, d_eligible, dg.orig, N);
+    hipLaunchKernelGGL(k_zero64, dim3(1024), dim3(kB), 0, stream, reinterpret_cast<uint64_t *>(acc), rows * N);
+    hip_check(hipGetLastError(), "translate launch");
+    src = ssrc;
+    elig = selig;
+  }
+  JobSpec js{&g, &dg, &de, cs, src, d_wt, d_cyc, n_sources, elig, acc, d_reach ? d_reach + t0 : nullptr,
+             tune.delta_scale};
+  js.tstride = stride_t;
+  Job job = make_job(js, T);
+  hipLaunchKernelGGL(k_put_job, dim3(1), dim3(64), 0, stream, job, d_job);
+  hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t *>(d_stats), (size_t)4);
+  hip_check(hipGetLastError(), "job table launch");
+
+  Plan p{};
+  p.nbk = job.nbk;
+  p.maxb = job.maxb;
+  const uint32_t block = launch_block(tune, p.nbk);
+  const uint32_t wpb = block / kWave;
+  p.bcap = 1024;
+  p.cap = std::min<uint32_t>({N, tune.cap ? tune.cap : 16384u, kSlotLimit});
+  const size_t lds = (size_t)wpb * (p.nbk + kWaveLds) * 4;
+  const int bpc = std::max(1, blocks_per_cu<kBrandes>(T, block, lds));
+  uint32_t resident = di.cus * (uint32_t)bpc * wpb;
+  if (tune.waves_per_simd) resident = std::min<uint32_t>(resident, di.cus * 4 * tune.waves_per_simd);
+
+  uint32_t n_list = n_sources;
+  const uint2 *items = nullptr;
+  int cur = 0;
+  for (int attempt = 0;; ++attempt) {
+    layout(p, N, T, true);
+    const size_t per_wave = 4 + 16ull * N + p.stride;
+    const uint32_t want = std::max<uint32_t>(wpb, (std::min<uint32_t>(n_list, resident) + wpb - 1) / wpb * wpb);
+    // epochs, node records and scratch of w waves, each part 256-byte aligned
+    auto bytes_for = [&](uint32_t w) { return align256(4ull * w) + align256(16ull * N * w) + p.stride * w; };
+    uint32_t waves = std::min<uint32_t>(want, (uint32_t)std::min<size_t>((ws_bytes - fixed) / per_wave, want));
+    waves = waves / wpb * wpb;
+    while (waves >= wpb && fixed + bytes_for(waves) > ws_bytes) waves -= wpb;
+    if (waves < wpb) {
+      if (p.cap > 1024) {
+        p.cap = std::max<uint32_t>(1024, p.cap / 2);
+        continue;
+      }
+      fail(CS_ENOMEM, "graph of " + std::to_string(N) +
+                          " nodes too large for the allocation-free workspace (" +
+                          std::to_string(ws_bytes >> 20) + " MiB)");
+    }
+    char *base = ws + fixed;
+    uint32_t *d_epochs = reinterpret_cast<uint32_t *>(base);
+    uint64_t *d_keys = reinterpret_cast<uint64_t *>(base + align256(4ull * waves));
+    char *d_scratch = base + align256(4ull * waves) + align256(16ull * N * waves);
+    // node records and epochs start from zero (epoch 0 is never issued)
+    hipLaunchKernelGGL(k_zero64, dim3(2048), dim3(kB), 0, stream, reinterpret_cast<uint64_t *>(base),
+                       (align256(4ull * waves) + align256(16ull * N * waves)) / 8);
+    hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t *>(d_ctl), (size_t)8);
+    hip_check(hipGetLastError(), "clear launch");
+    SearchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.jobs = d_job;
+    a.items = items;
+    a.n_items = n_list;
+    a.bcap = p.bcap;
+    a.cap = p.cap;
+    a.nbk_lds = p.nbk;
+    a.stats = d_stats;
+    a.keys = d_keys;
+    a.ncap = N;
+    a.ws = d_scratch;
+    a.ws_stride = p.stride;
+    a.o_rec = p.o_rec;
+    a.o_pm = p.o_pm;
+    a.o_acc = p.o_acc;
+    a.o_ods = p.o_ods;
+    a.o_bkt = p.o_bkt;
+    a.o_bnd = p.o_bnd;
+    a.o_lvl = p.o_lvl;
+    a.epochs = d_epochs;
+    a.next = d_ctl;
+    a.xsplit = tune.xcd_split ? 1u : 0u;
+    for (uint32_t x = 0; x <= kXcd; ++x)
+      a.xbeg[x] = a.xsplit ? (uint32_t)((uint64_t)n_list * x / kXcd) : (x ? n_list : 0u);
+    a.n_fail = d_ctl + kXcd;
+    a.fail_flags = d_ctl + kXcd + 1;
+    a.fail_list = d_fail;
+    launch_nt<kBrandes>(T, waves / wpb, block, lds, stream, a);
+    hip_check(hipGetLastError(), "k_search launch");
+    uint32_t ctl[16];
+    hip_check(hipMemcpyAsync(ctl, d_ctl, 64, hipMemcpyDeviceToHost, stream), "copy ctl");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    st.waves = waves;
+    const uint32_t n_failed = ctl[kXcd], flags = ctl[kXcd + 1];
+    if (n_failed == 0) break;
+    if (flags & kFailWide) fail_wide_row();
+    if (flags & kFailInternal) fail(CS_EINTERNAL, "search invariant violated (bucket order)");
+    if (flags & kFailRing) fail(CS_EINTERNAL, "bucket ring window exceeded");
+    if ((flags & kFailCap) && p.cap >= std::min<uint32_t>(N, kSlotLimit))
+      fail(CS_EINVAL, "a source reaches more nodes than the search's slot limit");
+    if (attempt == 5) fail(CS_EINTERNAL, "search workspace retries exhausted");
+    st.retries += n_failed;
+    // the failed items, as {job, source position}, run again with more room
+    uint2 *next = d_items[cur];
+    hipLaunchKernelGGL(k_retry_items, dim3((n_failed + kB - 1) / kB), dim3(kB), 0, stream, next, d_fail, n_failed,
+                       items);
+    hip_check(hipGetLastError(), "retry items launch");
+    items = next;
+    cur ^= 1;
+    n_list = n_failed;
+    if (flags & kFailCap) p.cap = std::min<uint32_t>({N, p.cap * 4, kSlotLimit});
+    if (flags & kFailBucket) p.bcap *= 4;
+  }
+  if (translate) {
+    const size_t total = rows * N;
+    hipLaunchKernelGGL(k_scatter_add_rows_t, dim3((uint32_t)((total + kB - 1) / kB)), dim3(kB), 0, stream, d_acc, acc,
+                       dg.orig, N, 7u, T, tstride, t0);
+    hip_check(hipGetLastError(), "scatter launch");
+  }
+  unsigned long long stv[4];
+  hip_check(hipMemcpyAsync(stv, d_stats, 32, hipMemcpyDeviceToHost, stream), "copy stats");
+  hip_check(hipStreamSynchronize(stream), "sync");
+  st.nodes_reached += stv[0];
+  st.edges_scanned += stv[1];
+  st.within_sum += stv[2];
+}
+
 void launch_shortest_batch(const std::vector<JobSpec> &jobs, uint32_t NT, const Tuning &tune, hipStream_t stream,
                            cs_stats &st, const std::vector<uint2> *items) {
   run_jobs<kBrandes>(jobs, NT, tune, nullptr, stream, st, items);

@@ -191,6 +191,28 @@ int cs_centrality_shortest(cs_graph *g, const cs_params *p, double *out, uint32_
 int cs_centrality_shortest_device(cs_graph *g, const cs_params *p, const float *d_cycles_wt, double *d_acc,
                                   uint32_t *d_reach, void *stream);
 
+/* Bytes of the static device array that cs_centrality_shortest_device_noalloc
+ * works in; each device the library runs on holds one until the process ends. */
+#define CS_NOALLOC_SCRATCH_BYTES (256u << 20)
+
+/* cs_centrality_shortest_device that allocates nothing. No reference
+ * counterpart: it serves callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, results and synchronisation as
+ * cs_centrality_shortest_device; every device buffer it needs beyond them
+ * (job table, control words, retry lists, node-order staging, per-wave node
+ * records and scratch) lies in a static array of CS_NOALLOC_SCRATCH_BYTES on
+ * the current device. The graph must already be resident there at
+ * p->speed_m_s (cs_graph_upload(g, device, speed)), else CS_EINVAL. A call
+ * whose fixed parts or one workgroup's waves do not fit the array is
+ * CS_ENOMEM; otherwise it runs as many waves as fit (fewer than the allocating
+ * entry point on large graphs). Calls on one device share the array, which
+ * keeps the last call's contents: the library serialises the calls of one
+ * process on a device (each returns after its stream has finished), and work
+ * the caller enqueues elsewhere must not touch the array. */
+int cs_centrality_shortest_device_noalloc(cs_graph *g, const cs_params *p, const float *d_cycles_wt,
+                                          double *d_acc, uint32_t *d_reach, void *stream);
+
 /* Replaces NetworkStructure.betweenness_od_shortest (rust/src/centrality.rs:2413-2540):
  * OD-weighted betweenness from a sparse origin-destination matrix given as n_pairs
  * parallel (origin, destination, trip weight) triples, as OdMatrix(origins,
