@@ -75,6 +75,7 @@ struct AngArgs {
   uint8_t *t_vis;
   uint32_t *t_order;
   uint32_t *t_norder;
+  uint32_t *progress;  // [lanes] sources each lane has finished (pinned host memory), or null
 };
 
 // Element i of this lane's own block in a wave-shared array of 64 lane blocks
@@ -451,6 +452,8 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
   L.w = a.wave_ws + (size_t)wid * a.wave_stride;
   L.lane = lane;
   unsigned long long st_reached = 0, st_edges = 0, st_within = 0;
+  // sources this lane has finished in the call so far (earlier launches included)
+  uint32_t pdone = a.progress ? __hip_atomic_load(&a.progress[gl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0u;
   char *w = L.w;
   // Items are dealt statically, lane gl taking gl, gl + lanes, ...; each job's
   // items are padded to whole waves, so the 64 items of one wave belong to one
@@ -591,6 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
         hi = lo - 1;
       }
     }
+    if (a.progress) __hip_atomic_store(&a.progress[gl], ++pdone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   atomicAdd(&a.stats[0], st_reached);
   atomicAdd(&a.stats[1], st_edges);
@@ -849,7 +853,7 @@ uint32_t sources_per_wave(uint32_t n_real, uint32_t simds) {
 }
 
 void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<uint2> *explicit_items,
-                          uint32_t n_implicit, hipStream_t stream, cs_stats &st) {
+                          uint32_t n_implicit, hipStream_t stream, cs_stats &st, uint32_t *progress) {
   if (specs.empty()) return;
   const int dev0 = specs[0].dg->device;
   int n_cu = 0;
@@ -972,6 +976,7 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     a.n_fail = d_ctl + 1;
     a.fail_flags = d_ctl + 2;
     a.fail_list = d_fail;
+    a.progress = lanes <= kProgressSlots ? progress : nullptr;
     hip_check(hipEventRecord(e0, stream), "event");
     hipLaunchKernelGGL(k_angular, dim3(lanes / kBlock), dim3(kBlock), 0, stream, a);
     hip_check(hipGetLastError(), "k_angular launch");
@@ -1019,10 +1024,10 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
 
 void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs, const uint32_t *d_sources,
                      const float *d_wt, uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
-                     uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride) {
+                     uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride, uint32_t *progress) {
   if (n_sources == 0) return;
   std::vector<AngSpec> specs{AngSpec{&g, &dg, &de, cs, d_sources, d_wt, d_eligible, d_acc, d_reach, tstride}};
-  launch_simplest_jobs(specs, nullptr, n_sources, stream, st);
+  launch_simplest_jobs(specs, nullptr, n_sources, stream, st, progress);
 }
 
 void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec, float *d_simpl,

@@ -221,6 +221,46 @@ void set_node_order_from_xy(Graph &g, const double *x, const double *y) {
   }
 }
 
+// Pinned host arrays of progress counts. A graph takes one on its first call
+// and hands it back when destroyed; the arrays are reused, never freed, since
+// kernels write them through the device's view of host memory.
+std::mutex g_progress_mu;
+std::vector<std::pair<uint32_t *, uint32_t *>> g_progress_pool;  // {host, device}
+
+uint32_t *progress_begin(Graph &g, uint64_t settled_on_host) {
+  uint32_t *slots = g.progress_slots.load(std::memory_order_acquire);
+  if (!slots) {
+    std::pair<uint32_t *, uint32_t *> pr{nullptr, nullptr};
+    {
+      std::lock_guard<std::mutex> lk(g_progress_mu);
+      if (!g_progress_pool.empty()) {
+        pr = g_progress_pool.back();
+        g_progress_pool.pop_back();
+      }
+    }
+    if (!pr.first) {
+      void *hp = nullptr, *dp = nullptr;
+      hip_check(hipHostMalloc(&hp, 4ull * kProgressSlots, hipHostMallocMapped | hipHostMallocCoherent),
+                "hipHostMalloc progress");
+      hip_check(hipHostGetDevicePointer(&dp, hp, 0), "hipHostGetDevicePointer");
+      pr = {static_cast<uint32_t *>(hp), static_cast<uint32_t *>(dp)};
+    }
+    g.progress_dev = pr.second;
+    slots = pr.first;
+  }
+  g.progress_host.store(settled_on_host, std::memory_order_relaxed);
+  for (uint32_t i = 0; i < kProgressSlots; ++i) __atomic_store_n(&slots[i], 0u, __ATOMIC_RELAXED);
+  g.progress_slots.store(slots, std::memory_order_release);
+  return g.progress_dev;
+}
+
+void progress_finish(Graph &g, uint64_t n) {
+  uint64_t counted = 0;
+  if (const uint32_t *slots = g.progress_slots.load(std::memory_order_acquire))
+    for (uint32_t i = 0; i < kProgressSlots; ++i) counted += __atomic_load_n(&slots[i], __ATOMIC_RELAXED);
+  g.progress_host.store(n > counted ? n - counted : 0, std::memory_order_relaxed);
+}
+
 DevGraph &device_graph(Graph &g, int device) {
   auto it = g.dev.find(device);
   if (it != g.dev.end()) return *it->second;
@@ -700,6 +740,10 @@ int cs_graph_create(uint32_t n_nodes, const double *z, const uint8_t *live, cons
 
 void cs_graph_destroy(cs_graph *h) {
   if (!h) return;
+  if (uint32_t *slots = h->g.progress_slots.load()) {
+    std::lock_guard<std::mutex> lk(g_progress_mu);
+    g_progress_pool.emplace_back(slots, h->g.progress_dev);
+  }
   for (auto &kv : h->g.dev) {
     int prev = 0;
     if (hipGetDevice(&prev) == hipSuccess && hipSetDevice(kv.first) == hipSuccess) {
@@ -768,7 +812,14 @@ int cs_graph_info(const cs_graph *h, uint32_t *n, uint32_t *m, uint32_t *mo, uin
   });
 }
 
-uint64_t cs_graph_progress(const cs_graph *h) { return h ? h->g.progress : 0; }
+uint64_t cs_graph_progress(const cs_graph *h) {
+  if (!h) return 0;
+  const Graph &g = h->g;
+  uint64_t n = g.progress_host.load(std::memory_order_relaxed);
+  if (const uint32_t *slots = g.progress_slots.load(std::memory_order_acquire))
+    for (uint32_t i = 0; i < kProgressSlots; ++i) n += __atomic_load_n(&slots[i], __ATOMIC_RELAXED);
+  return n;
+}
 
 int cs_last_stats(cs_stats *out) {
   if (!out) return CS_EINVAL;
@@ -805,10 +856,14 @@ int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_
       DevGraph &dg = device_graph(g, dev);
       DevEdges &de = device_edges(g, dg, spec.speed);
       run_shortest_checks(g, p);
+      uint32_t *prog = progress_begin(g, p->n_sources - ss.src.size());
+      // a call of several threshold chunks reports progress in its last one
       for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
         launch_shortest(g, dg, de, make_spec(p, t0, chunk_len(T, t0)), d_src.as<uint32_t>(), d_wt.as<float>(),
                         d_cyc.as<float>(), (uint32_t)ss.src.size(), d_elig.as<uint8_t>(),
-                        acc.as<double>() + (size_t)t0 * N, reach.as<uint32_t>() + t0, stream, t_stats, T);
+                        acc.as<double>() + (size_t)t0 * N, reach.as<uint32_t>() + t0, stream, t_stats, T,
+                        t0 + kMaxT >= T ? prog : nullptr);
+      progress_finish(g, p->n_sources);
     }
     std::vector<double> hacc((size_t)N * M * T);
     std::vector<uint32_t> hreach(T);
@@ -821,7 +876,6 @@ int cs_centrality_shortest(cs_graph *h, const cs_params *p, double *out, uint32_
     if (reach_totals)
       for (uint32_t i = 0; i < T; ++i) reach_totals[i] = hreach[i];
     t_stats.sources = ss.src.size();
-    g.progress = p->n_sources;
   });
   t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
@@ -846,12 +900,14 @@ int cs_centrality_shortest_device(cs_graph *h, const cs_params *p, const float *
     DevGraph &dg = device_graph(g, dev);
     DevEdges &de = device_edges(g, dg, p->speed_m_s);
     run_shortest_checks(g, p);
+    uint32_t *prog = progress_begin(g, 0);
     for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
       launch_shortest_original(g, dg, de, make_spec(p, t0, chunk_len(T, t0)), p->sources, p->source_wt,
                                d_cycles_wt, p->n_sources, p->eligible, d_acc + (size_t)t0 * N,
-                               d_reach ? d_reach + t0 : nullptr, stream, t_stats, T);
+                               d_reach ? d_reach + t0 : nullptr, stream, t_stats, T,
+                               t0 + kMaxT >= T ? prog : nullptr);
+    progress_finish(g, p->n_sources);
     t_stats.sources = p->n_sources;
-    g.progress = p->n_sources;
   });
   t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
@@ -944,6 +1000,7 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
       DevGraph &dg = device_graph(g, dev);
       DevEdges &de = device_edges(g, dg, p->speed_m_s);
       check_betweenness_graph(g);
+      uint32_t *prog = progress_begin(g, N - ns);
       for (uint32_t t0 = 0; t0 < T; t0 += kMaxT) {
         CallSpec spec = make_spec(p, t0, chunk_len(T, t0));
         spec.closeness = false;
@@ -955,7 +1012,8 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
         js.d_od_off = d_off.as<uint32_t>();
         js.d_od_dest = d_dest.as<uint32_t>();
         js.d_od_w = d_w.as<float>();
-        launch_shortest_batch({js}, spec.T, default_tuning(), stream, t_stats);
+        launch_shortest_batch({js}, spec.T, default_tuning(), stream, t_stats, nullptr,
+                              t0 + kMaxT >= T ? prog : nullptr);
       }
     }
     std::vector<double> hacc((size_t)N * M * T);
@@ -965,7 +1023,7 @@ int cs_betweenness_od_shortest(cs_graph *h, const cs_params *p, uint32_t n_pairs
     // rows 5 and 6 (betweenness, betweenness_beta) of the [M][T][N] accumulators
     unpack_acc_strided(hacc.data() + (size_t)5 * T * N, N, 2, T, T, out, result_order(g));
     t_stats.sources = ns;
-    g.progress = N;
+    progress_finish(g, N);
   });
   t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
@@ -998,16 +1056,17 @@ int cs_segment_centrality(cs_graph *h, const cs_params *p, double *out) {
     hip_check(hipMemsetAsync(acc.p, 0, sizeof(double) * (size_t)N * M * T, stream), "memset");
     if (!srcs.empty())
       hip_check(hipMemcpyAsync(d_src.p, srcs.data(), 4 * srcs.size(), hipMemcpyHostToDevice, stream), "copy");
+    uint32_t *prog = progress_begin(g, p->n_sources - srcs.size());
     for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
       launch_segment(g, dg, de, make_spec(p, t0, chunk_len(T, t0)), d_src.as<uint32_t>(), (uint32_t)srcs.size(),
-                     acc.as<double>() + (size_t)t0 * N, stream, t_stats, T);
+                     acc.as<double>() + (size_t)t0 * N, stream, t_stats, T, t0 + kMaxT >= T ? prog : nullptr);
     std::vector<double> hacc((size_t)N * M * T);
     hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
               "copy out");
     hip_check(hipStreamSynchronize(stream), "sync");
     unpack_acc(hacc, N, M, T, out, result_order(g));
     t_stats.sources = srcs.size();
-    g.progress = p->n_sources;
+    progress_finish(g, p->n_sources);
   });
   t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
@@ -1037,10 +1096,11 @@ int cs_centrality_simplest(cs_graph *h, const cs_params *p, double *out, uint32_
       hip_check(hipMemcpyAsync(d_wt.p, ss.wt.data(), 4 * ss.wt.size(), hipMemcpyHostToDevice, stream), "copy");
     }
     hip_check(hipMemcpyAsync(d_elig.p, p->eligible, N, hipMemcpyHostToDevice, stream), "copy");
+    uint32_t *prog = progress_begin(g, p->n_sources - ss.src.size());
     for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
       launch_simplest(g, dg, de, make_spec(p, t0, chunk_len(T, t0)), d_src.as<uint32_t>(), d_wt.as<float>(),
                       (uint32_t)ss.src.size(), d_elig.as<uint8_t>(), acc.as<double>() + (size_t)t0 * N,
-                      reach.as<uint32_t>() + t0, stream, t_stats, T);
+                      reach.as<uint32_t>() + t0, stream, t_stats, T, t0 + kMaxT >= T ? prog : nullptr);
     std::vector<double> hacc((size_t)N * M * T);
     std::vector<uint32_t> hreach(T);
     hip_check(hipMemcpyAsync(hacc.data(), acc.p, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost, stream),
@@ -1052,7 +1112,7 @@ int cs_centrality_simplest(cs_graph *h, const cs_params *p, double *out, uint32_
     if (reach_totals)
       for (uint32_t i = 0; i < T; ++i) reach_totals[i] = hreach[i];
     t_stats.sources = ss.src.size();
-    g.progress = p->n_sources;
+    progress_finish(g, p->n_sources);
   });
   t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
@@ -1287,10 +1347,11 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     ao.count = d_count.as<double>();
     ao.weighted = d_wt.as<double>();
     ao.nearest = d_near.as<unsigned long long>();
+    uint32_t *prog = progress_begin(g, N - srcs.size());
     if (!srcs.empty())
       for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
         launch_access(g, dg, de, (float)max_seconds, d_src.as<uint32_t>(), (uint32_t)srcs.size(), chunk(t0), stream,
-                      t_stats);
+                      t_stats, t0 + kMaxT >= T ? prog : nullptr);
     const uint32_t *ip = result_order(g);
     if (out_div) {
       DevBuf d_div(8ull * kDivMetrics * T * N);
@@ -1301,7 +1362,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
       hip_check(hipStreamSynchronize(stream), "sync");
       unpack_acc(hd, N, kDivMetrics, T, out_div, ip);
       t_stats.sources = srcs.size();
-      g.progress = N;
+      progress_finish(g, N);
       return;
     }
     std::vector<double> hc(CTN), hw(CTN), hn((size_t)C * N);
@@ -1313,7 +1374,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     unpack_acc(hw, N, C, T, out_weighted, ip);
     unpack_acc(hn, N, C, 1, out_nearest, ip);
     t_stats.sources = srcs.size();
-    g.progress = N;
+    progress_finish(g, N);
   });
   t_stats.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;

@@ -97,8 +97,24 @@ struct Graph {
 
   std::mutex mu;
   std::map<int, std::unique_ptr<DevGraph>> dev;
-  uint64_t progress = 0;
+  // Live progress (sources finished by the running or last call): sources
+  // the host settles itself plus one count per wave (or lane) that the kernels
+  // keep in pinned host memory while they run (progress_begin).
+  std::atomic<uint32_t *> progress_slots{nullptr};  // [kProgressSlots], host address
+  uint32_t *progress_dev = nullptr;                 // the same memory, device address
+  std::atomic<uint64_t> progress_host{0};
 };
+
+// Counts the kernels keep for a call's live progress: one per wave (search)
+// or per lane (angular search); launches wider than this report none.
+constexpr uint32_t kProgressSlots = 65536;
+
+// Starts a call's live progress on g (under g.mu, no kernel of g running):
+// clears the counts, counts `settled_on_host` sources at once and returns the
+// device address of the counts for the launches to pass on.
+uint32_t *progress_begin(Graph &g, uint64_t settled_on_host);
+// Sets the count outright (calls that report only on completion).
+void progress_set(Graph &g, uint64_t n);
 
 // Host edge travel seconds (centrality.rs:970-1007).
 float edge_seconds(const Graph &g, uint32_t from, uint32_t to, uint32_t e, float speed,
@@ -138,10 +154,12 @@ struct CallSpec {
 // accumulates into; reach [T] u32.
 // tstride: threshold stride of d_acc's rows (0: cs.T); a threshold chunk
 // passes its first row and the call's threshold count.
+// progress: the call's live counts (progress_begin), or null.
 void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                      uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
-                     uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride = 0);
+                     uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride = 0,
+                     uint32_t *progress = nullptr);
 
 // launch_shortest with sources, eligibility and acc in original node order
 // (device buffers of the caller), translated to and from the graph's device
@@ -151,7 +169,8 @@ void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
 void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                               const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                               uint32_t n_sources, const uint8_t *d_eligible, double *d_acc,
-                              uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride = 0);
+                              uint32_t *d_reach, hipStream_t stream, cs_stats &st, uint32_t tstride = 0,
+                              uint32_t *progress = nullptr);
 
 // launch_shortest_original with every device buffer beyond the caller's
 // inputs and outputs carved from the device region ws (ws_bytes bytes,
@@ -203,7 +222,8 @@ struct JobSpec {
 // All sources of all jobs in one persistent launch, or only `items`
 // ({job, source position}) when given.
 void launch_shortest_batch(const std::vector<JobSpec> &jobs, uint32_t NT, const Tuning &tune,
-                           hipStream_t stream, cs_stats &st, const std::vector<uint2> *items = nullptr);
+                           hipStream_t stream, cs_stats &st, const std::vector<uint2> *items = nullptr,
+                           uint32_t *progress = nullptr);
 
 // A prepared batch of shortest-path jobs replayed as one captured hipGraph
 // (cs_search.hip). The zero regions (64-bit words) are cleared inside the
@@ -224,7 +244,7 @@ void batch_run(ShortestBatch &b, hipStream_t stream, cs_stats &st);
 // Segment centrality; acc [4][T][N].
 void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                     const uint32_t *d_sources, uint32_t n_sources, double *d_acc,
-                    hipStream_t stream, cs_stats &st, uint32_t tstride = 0);
+                    hipStream_t stream, cs_stats &st, uint32_t tstride = 0, uint32_t *progress = nullptr);
 
 // Single-source outputs (trees and Brandes state), device arrays of length N.
 struct TreeOut {
@@ -256,7 +276,8 @@ struct AccessOut {
   float dist[kMaxT], beta[kMaxT], mcw[kMaxT];
 };
 void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uint32_t *d_sources,
-                   uint32_t n_sources, const AccessOut &out, hipStream_t stream, cs_stats &st);
+                   uint32_t n_sources, const AccessOut &out, hipStream_t stream, cs_stats &st,
+                   uint32_t *progress = nullptr);
 // Diversity of each source's class counts (the rows launch_access leaves in
 // out.count / out.nearest, n_classes of them): div [kDivMetrics][T][N] =
 // hill q0, q1, q2, distance-weighted hill q0, q1, q2, shannon, gini-simpson.
@@ -285,12 +306,12 @@ struct AngSpec {
 // Simplest-path centrality for several requests in one launch: the items
 // {job, source position} (or, without a list, sources 0..n_implicit-1 of job 0).
 void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<uint2> *explicit_items,
-                          uint32_t n_implicit, hipStream_t stream, cs_stats &st);
+                          uint32_t n_implicit, hipStream_t stream, cs_stats &st, uint32_t *progress = nullptr);
 
 void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, uint32_t n_sources,
                      const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
-                     hipStream_t stream, cs_stats &st, uint32_t tstride = 0);
+                     hipStream_t stream, cs_stats &st, uint32_t tstride = 0, uint32_t *progress = nullptr);
 
 // dijkstra_tree_simplest for one source (cs_angular.hip); device arrays of
 // length N, pre-filled by the caller (INF / -1 / 0).

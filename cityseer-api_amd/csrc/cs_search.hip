@@ -120,6 +120,7 @@ struct SearchArgs {
   uint32_t *fail_flags;
   TreeOut tout;
   AccessOut acc;
+  uint32_t *progress;  // [waves] sources each wave has finished (pinned host memory), or null
 };
 
 // Wave-private workspace: only the owning wave reads or writes it, so
@@ -1086,6 +1087,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
   for (uint32_t r = lane; r < a.nbk_lds; r += kWave) lds_st(&w.ring[r], 0u);
   wave_fence();
   uint32_t epoch = a.epochs[wid];
+  // sources this wave has finished in the call so far (earlier launches included)
+  uint32_t pdone = a.progress ? uniform(__hip_atomic_load(&a.progress[wid], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_SYSTEM)) : 0u;
   unsigned long long st_reached = 0, st_within = 0;  // wave-uniform
   uint32_t st_edges = 0;                             // per lane
   for (;;) {
@@ -1462,6 +1466,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
       }
       wave_fence();
     }
+    if (a.progress) {
+      ++pdone;
+      if (lane == 0) __hip_atomic_store(&a.progress[wid], pdone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
   if (lane == 0) {
     a.epochs[wid] = epoch;
@@ -1796,7 +1804,7 @@ std::mutex g_tuning_mu;
 template <int MODE>
 void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune, const TreeOut *tout,
               hipStream_t stream, cs_stats &st, const std::vector<uint2> *explicit_items = nullptr,
-              const AccessOut *aout = nullptr) {
+              const AccessOut *aout = nullptr, uint32_t *progress = nullptr) {
   constexpr bool kBr = (MODE == kBrandes || MODE == kBrandesExport);
   uint64_t n_items = 0;
   for (auto &js : specs) n_items += js.n_sources;
@@ -1909,6 +1917,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.fail_list = d_fail;
     if (tout) a.tout = *tout;
     if (aout) a.acc = *aout;
+    a.progress = waves <= kProgressSlots ? progress : nullptr;
     hip_check(hipEventRecord(e0, stream), "event record");
     launch_nt<MODE>(NT, waves / wpb, block, lds, stream, a);
     hip_check(hipGetLastError(), "k_search launch");
@@ -2208,11 +2217,11 @@ Tuning default_tuning() { return current_tuning(); }
 void launch_shortest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                      const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                      uint32_t n_sources, const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
-                     hipStream_t stream, cs_stats &st, uint32_t tstride) {
+                     hipStream_t stream, cs_stats &st, uint32_t tstride, uint32_t *progress) {
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, d_wt, d_cyc, n_sources, d_eligible, d_acc, d_reach, tune.delta_scale};
   js.tstride = tstride;
-  run_jobs<kBrandes>({js}, cs.T, tune, nullptr, stream, st);
+  run_jobs<kBrandes>({js}, cs.T, tune, nullptr, stream, st, nullptr, nullptr, progress);
 }
 
 // Translation between buffers in original node order and the device order.
@@ -2254,10 +2263,10 @@ __global__ void k_scatter_add_rows_t(double *dst, const double *src, const uint3
 void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                               const uint32_t *d_sources, const float *d_wt, const float *d_cyc,
                               uint32_t n_sources, const uint8_t *d_eligible, double *d_acc, uint32_t *d_reach,
-                              hipStream_t stream, cs_stats &st, uint32_t tstride) {
+                              hipStream_t stream, cs_stats &st, uint32_t tstride, uint32_t *progress) {
   if (g.identity_order) {
     launch_shortest(g, dg, de, cs, d_sources, d_wt, d_cyc, n_sources, d_eligible, d_acc, d_reach, stream, st,
-                    tstride);
+                    tstride, progress);
     return;
   }
   if (!tstride) tstride = cs.T;
@@ -2283,7 +2292,7 @@ void launch_shortest_original(Graph &g, DevGraph &dg, DevEdges &de, const CallSp
   hipLaunchKernelGGL(k_gather_u8, dim3((N + kB - 1) / kB), dim3(kB), 0, stream, elig, d_eligible, dg.orig, N);
   hip_check(hipMemsetAsync(acc, 0, sizeof(double) * total, stream), "memset");
   hip_check(hipGetLastError(), "translate launch");
-  launch_shortest(g, dg, de, cs, src, d_wt, d_cyc, n_sources, elig, acc, d_reach, stream, st);
+  launch_shortest(g, dg, de, cs, src, d_wt, d_cyc, n_sources, elig, acc, d_reach, stream, st, 0, progress);
   hipLaunchKernelGGL(k_scatter_add_rows_t, dim3((uint32_t)((total + kB - 1) / kB)), dim3(kB), 0, stream, d_acc, acc,
                      dg.orig, N, 7u, cs.T, tstride, 0u);
   hip_check(hipGetLastError(), "scatter launch");
@@ -2487,8 +2496,8 @@ void launch_shortest_in(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs
 }
 
 void launch_shortest_batch(const std::vector<JobSpec> &jobs, uint32_t NT, const Tuning &tune, hipStream_t stream,
-                           cs_stats &st, const std::vector<uint2> *items) {
-  run_jobs<kBrandes>(jobs, NT, tune, nullptr, stream, st, items);
+                           cs_stats &st, const std::vector<uint2> *items, uint32_t *progress) {
+  run_jobs<kBrandes>(jobs, NT, tune, nullptr, stream, st, items, nullptr, progress);
 }
 
 ShortestBatch *batch_new(std::vector<JobSpec> specs, std::vector<ZeroRegion> zero, uint32_t NT, int device,
@@ -2512,15 +2521,16 @@ void batch_run(ShortestBatch &b, hipStream_t stream, cs_stats &st) { b.run(strea
 
 void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
                     const uint32_t *d_sources, uint32_t n_sources, double *d_acc, hipStream_t stream,
-                    cs_stats &st, uint32_t tstride) {
+                    cs_stats &st, uint32_t tstride, uint32_t *progress) {
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, d_acc, nullptr, tune.delta_scale};
   js.tstride = tstride;
-  run_jobs<kSegment>({js}, cs.T, tune, nullptr, stream, st);
+  run_jobs<kSegment>({js}, cs.T, tune, nullptr, stream, st, nullptr, nullptr, progress);
 }
 
 void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uint32_t *d_sources,
-                   uint32_t n_sources, const AccessOut &out, hipStream_t stream, cs_stats &st) {
+                   uint32_t n_sources, const AccessOut &out, hipStream_t stream, cs_stats &st,
+                   uint32_t *progress) {
   CallSpec cs;
   cs.T = 1;
   cs.dist[0] = maxsec * de.speed;
@@ -2531,7 +2541,7 @@ void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uin
   cs.betweenness = false;
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, nullptr, nullptr, tune.delta_scale};
-  run_jobs<kAccess>({js}, 1, tune, nullptr, stream, st, nullptr, &out);
+  run_jobs<kAccess>({js}, 1, tune, nullptr, stream, st, nullptr, &out, progress);
 }
 
 // One thread per (threshold, node): the diversity measures of the class
