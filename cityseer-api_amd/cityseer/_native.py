@@ -21,7 +21,13 @@ CS_EDEVICE = 2
 CS_ENOMEM = 3
 CS_EINTERNAL = 4
 CS_EBUSY = 5
+CS_ETIMEDOUT = 6
+CS_ECANCELED = 7
 CS_MAX_THRESHOLDS = 8
+
+
+class RequestCancelled(RuntimeError):
+    """A served request was cancelled before it completed."""
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _f32p = ctypes.POINTER(ctypes.c_float)
@@ -196,6 +202,11 @@ _SIGNATURES = {
     ),
     "cs_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(CsRequestInfo)]),
     "cs_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(CsRequestInfo)]),
+    "cs_server_wait_for": (
+        ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(CsRequestInfo)],
+    ),
+    "cs_server_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "cs_server_inject_fault": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "cs_server_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
 }
 
@@ -232,6 +243,10 @@ def check(rc: int) -> None:
         raise ValueError(msg)
     if rc == CS_ENOMEM:
         raise MemoryError(msg)
+    if rc == CS_ETIMEDOUT:
+        raise TimeoutError(msg)
+    if rc == CS_ECANCELED:
+        raise RequestCancelled(msg)
     raise RuntimeError(msg)
 
 

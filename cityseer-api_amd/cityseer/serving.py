@@ -47,10 +47,18 @@ class Request:
         st = self.info()["status"]
         return st == 2 or st < 0
 
-    def wait(self) -> dict:
+    def wait(self, timeout: float | None = None) -> dict:
+        """Block until the request completes (raising its error), or ``TimeoutError`` once ``timeout``
+        seconds have passed with the request still queued or running."""
         inf = _native.CsRequestInfo()
-        _native.check(_native.load().cs_server_wait(self._server._h, self.id, ctypes.byref(inf)))
+        ms = -1.0 if timeout is None else 1000.0 * float(timeout)
+        _native.check(_native.load().cs_server_wait_for(self._server._h, self.id, ms, ctypes.byref(inf)))
         return {name: getattr(inf, name) for name, _ in inf._fields_}
+
+    def cancel(self) -> None:
+        """Cancel the request: ``wait``/``result`` then raise ``RequestCancelled`` (unless it had
+        already completed). Its sources already on the device finish first."""
+        _native.check(_native.load().cs_server_cancel(self._server._h, self.id))
 
     def result(self):
         if self._result is None:

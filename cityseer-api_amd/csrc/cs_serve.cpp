@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -23,6 +24,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <stdexcept>
 #include <thread>
 #include <unordered_map>
 
@@ -67,6 +69,8 @@ struct Req {
   uint32_t finished = 0;  // sources whose launch completed
   uint32_t launches = 0;
   int status = 0;  // 0 queued, 1 running, 2 done, < 0 error
+  bool cancelled = false;  // no more sources are handed out; completes once none is in flight
+  int builder = -1;        // the worker that admitted it and builds its device state
   std::string msg;
   Clock::time_point t_submit, t_admit, t_done;
 
@@ -111,20 +115,29 @@ struct cs_server {
   bool stop = false;
   double ms_per_src[2] = {0.0, 0.0};  // running per-source cost, shortest / simplest
   std::thread workers[2];
+  bool dead[2] = {false, false};      // a worker that ended without a shutdown
+  std::string dead_msg[2];
+  std::atomic<int> fault[2] = {0, 0};  // failure injected into a worker's next slice (tests)
 
   // Moves queued requests to `active` while the budget allows (FIFO). Called
   // with `mu` held; returns the newly admitted requests for the caller to
   // build device state for outside the lock.
-  std::vector<Req *> admit_locked() {
+  std::vector<Req *> admit_locked(int who) {
     std::vector<Req *> fresh;
     while (!queue.empty()) {
       Req *r = queue.front();
+      if (dead[r->kind]) {
+        queue.pop_front();
+        fail_locked(r, CS_EINTERNAL, dead_msg[r->kind]);
+        continue;
+      }
       if (active.size() >= cfg.max_active) break;
       if (used + r->dev_bytes > cfg.hbm_budget_bytes) break;
       queue.pop_front();
       used += r->dev_bytes;
       r->status = 1;
       r->t_admit = Clock::now();
+      r->builder = who;
       active.push_back(r);
       fresh.push_back(r);
     }
@@ -167,6 +180,14 @@ struct cs_server {
       for (uint32_t t = 0; t < r.T; ++t) r.reach[t] = hreach[t];
   }
 
+  // A request that never reached the device fails (mu held).
+  void fail_locked(Req *r, int err, const std::string &msg) {
+    r->status = -err;
+    r->msg = msg;
+    r->t_done = Clock::now();
+    cv_done.notify_all();
+  }
+
   // Marks requests done (or failed), releasing their budget.
   void complete(const std::vector<Req *> &done, int err, const std::string &msg) {
     std::lock_guard<std::mutex> lk(mu);
@@ -185,36 +206,125 @@ struct cs_server {
     return !queue.empty() && active.size() < cfg.max_active && used + queue.front()->dev_bytes <= cfg.hbm_budget_bytes;
   }
 
+  // A cancelled request of this kind whose launched sources have all finished.
+  Req *cancelled_idle_locked(int kind) const {
+    for (Req *r : active)
+      if (r->kind == kind && r->cancelled && r->ready && r->finished == r->handed) return r;
+    return nullptr;
+  }
+
   bool has_work_locked(int kind) const {
-    if (admissible_locked()) return true;
+    if (admissible_locked() || cancelled_idle_locked(kind)) return true;
     for (const Req *r : active)
-      if (r->kind == kind && r->ready && r->handed < r->n()) return true;
+      if (r->kind == kind && r->ready && !r->cancelled && r->handed < r->n()) return true;
     return false;
   }
 
+  // Runs one worker. Whatever ends it other than a shutdown (a failure
+  // outside the per-request handling, down to a HIP error before its loop)
+  // fails every queued and active request of its kind, and later submissions
+  // of that kind fail at once: no request waits forever.
   void worker(int kind) {
+    std::string why = "server worker stopped";
+    try {
+      worker_loop(kind);
+      return;
+    } catch (const Error &e) {
+      why = e.msg;
+    } catch (const std::exception &e) {
+      why = e.what();
+    } catch (...) {
+      why = "unknown failure";
+    }
+    std::vector<Req *> orphans;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (stop) return;
+      dead[kind] = true;
+      dead_msg[kind] = "server worker failed: " + why;
+      for (auto it = queue.begin(); it != queue.end();) {
+        if ((*it)->kind == kind) {
+          fail_locked(*it, CS_EINTERNAL, dead_msg[kind]);
+          it = queue.erase(it);
+        } else {
+          ++it;
+        }
+      }
+      // its kind's requests on the device, and any it admitted but had not built
+      // (those the other worker is building, it fails itself)
+      for (Req *r : active)
+        if ((r->kind == kind && r->ready) || (!r->ready && r->builder == kind)) orphans.push_back(r);
+    }
+    for (Req *r : orphans) free_req_device(*r);
+    complete(orphans, CS_EINTERNAL, "server worker failed: " + why);
+  }
+
+  // The failure a caught exception becomes for the requests it affects.
+  static void describe(std::exception_ptr ep, int &err, std::string &msg) {
+    try {
+      std::rethrow_exception(ep);
+    } catch (const Error &e) {
+      err = e.code;
+      msg = e.msg;
+    } catch (const std::bad_alloc &) {
+      err = CS_ENOMEM;
+      msg = "host allocation failed";
+    } catch (const std::exception &e) {
+      err = CS_EINTERNAL;
+      msg = e.what();
+    } catch (...) {
+      err = CS_EINTERNAL;
+      msg = "unknown failure";
+    }
+  }
+
+  void worker_loop(int kind) {
     hip_check(hipSetDevice(device), "hipSetDevice");
     hipStream_t stream;
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    struct StreamGuard {
+      hipStream_t s;
+      ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } stream_guard{stream};
     for (;;) {
       std::vector<Req *> fresh;
       std::vector<Req *> slice_reqs;
       std::vector<uint32_t> slice_take;
+      Req *to_cancel = nullptr;
       {
         std::unique_lock<std::mutex> lk(mu);
         cv_work.wait(lk, [&] { return stop || has_work_locked(kind); });
         if (stop) break;
-        fresh = admit_locked();
+        fresh = admit_locked(kind);
+        to_cancel = cancelled_idle_locked(kind);
+      }
+      if (fault[kind].load() == 3) {
+        fault[kind] = 0;
+        throw std::runtime_error("injected worker failure");
+      }
+      if (to_cancel) {
+        free_req_device(*to_cancel);
+        complete({to_cancel}, CS_ECANCELED, "request cancelled");
       }
       // device state of newly admitted requests (any kind)
       for (Req *r : fresh) {
         try {
           build_device(*r);
-          std::lock_guard<std::mutex> lk(mu);
+          std::unique_lock<std::mutex> lk(mu);
+          if (dead[r->kind]) {
+            const std::string why = dead_msg[r->kind];
+            lk.unlock();
+            free_req_device(*r);
+            complete({r}, CS_EINTERNAL, why);
+            continue;
+          }
           r->ready = true;
-        } catch (const Error &e) {
+        } catch (...) {
+          int err = 0;
+          std::string msg;
+          describe(std::current_exception(), err, msg);
           free_req_device(*r);
-          complete({r}, e.code, e.msg);
+          complete({r}, err, msg);
         }
       }
       if (!fresh.empty()) cv_work.notify_all();
@@ -223,7 +333,7 @@ struct cs_server {
         std::lock_guard<std::mutex> lk(mu);
         std::vector<Req *> mine;
         for (Req *r : active)
-          if (r->kind == kind && r->ready && r->handed < r->n()) mine.push_back(r);
+          if (r->kind == kind && r->ready && !r->cancelled && r->handed < r->n()) mine.push_back(r);
         if (mine.empty()) continue;
         const double per = ms_per_src[kind] > 0.0 ? ms_per_src[kind] : 0.004;
         // an angular launch lasts about as long as its slowest source whatever
@@ -256,6 +366,10 @@ struct cs_server {
       uint64_t n_src = 0;
       try {
         cs_stats st{};
+        if (const int f = fault[kind].exchange(0)) {
+          if (f == 1) throw std::runtime_error("injected slice failure");
+          if (f == 2) throw std::bad_alloc();
+        }
         if (kind == CS_REQ_SHORTEST) {
           std::vector<JobSpec> specs;
           std::vector<uint2> items;
@@ -284,9 +398,8 @@ struct cs_server {
           n_src = items.size();
           launch_simplest_jobs(specs, &items, 0, stream, st);
         }
-      } catch (const Error &e) {
-        err = e.code;
-        msg = e.msg;
+      } catch (...) {
+        describe(std::current_exception(), err, msg);
       }
       const double ms = ms_between(t0, Clock::now());
       std::vector<Req *> done;
@@ -299,26 +412,27 @@ struct cs_server {
         for (size_t i = 0; i < slice_reqs.size(); ++i) {
           Req *r = slice_reqs[i];
           r->finished += slice_take[i];
-          if (err || r->finished == r->n()) done.push_back(r);
+          if (err || r->finished == r->n() || (r->cancelled && r->finished == r->handed)) done.push_back(r);
           if (err) r->handed = r->n();
         }
       }
       for (Req *r : done) {
         int e2 = err;
         std::string m2 = msg;
-        if (!err) {
+        if (!err && r->cancelled) {
+          e2 = CS_ECANCELED;
+          m2 = "request cancelled";
+        } else if (!err) {
           try {
             finish(*r);
-          } catch (const Error &e) {
-            e2 = e.code;
-            m2 = e.msg;
+          } catch (...) {
+            describe(std::current_exception(), e2, m2);
           }
         }
         free_req_device(*r);
         complete({r}, e2, m2);
       }
     }
-    (void)hipStreamDestroy(stream);
   }
 };
 
@@ -358,6 +472,7 @@ int64_t submit_impl(cs_server *s, cs_graph *h, int kind, const cs_params *p, dou
   r->t_submit = Clock::now();
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) fail(CS_EBUSY, "server is shutting down");
+  if (s->dead[kind]) fail(CS_EINTERNAL, s->dead_msg[kind]);
   r->id = s->next_id++;
   const int64_t id = r->id;
   Req *raw = r.get();
@@ -404,18 +519,8 @@ int cs_server_create(const cs_server_config *cfg, cs_server **out) {
     if (s->cfg.max_active == 0) s->cfg.max_active = 64;
     if (!(s->cfg.slice_ms > 0.0)) s->cfg.slice_ms = 20.0;
     cs_server *raw = s.get();
-    raw->workers[0] = std::thread([raw] {
-      try {
-        raw->worker(CS_REQ_SHORTEST);
-      } catch (...) {
-      }
-    });
-    raw->workers[1] = std::thread([raw] {
-      try {
-        raw->worker(CS_REQ_SIMPLEST);
-      } catch (...) {
-      }
-    });
+    raw->workers[0] = std::thread([raw] { raw->worker(CS_REQ_SHORTEST); });
+    raw->workers[1] = std::thread([raw] { raw->worker(CS_REQ_SIMPLEST); });
     *out = s.release();
   });
 }
@@ -465,7 +570,9 @@ int cs_server_poll(cs_server *s, int64_t id, cs_request_info *info) {
   });
 }
 
-int cs_server_wait(cs_server *s, int64_t id, cs_request_info *info) {
+int cs_server_wait(cs_server *s, int64_t id, cs_request_info *info) { return cs_server_wait_for(s, id, -1.0, info); }
+
+int cs_server_wait_for(cs_server *s, int64_t id, double timeout_ms, cs_request_info *info) {
   int status = CS_OK;
   std::string msg;
   const int rc = guarded_call([&] {
@@ -474,7 +581,15 @@ int cs_server_wait(cs_server *s, int64_t id, cs_request_info *info) {
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) fail(CS_EINVAL, "unknown request id " + std::to_string(id));
     Req &r = *it->second;
-    s->cv_done.wait(lk, [&] { return r.status == 2 || r.status < 0; });
+    auto finished = [&] { return r.status == 2 || r.status < 0; };
+    if (timeout_ms < 0.0) {
+      s->cv_done.wait(lk, finished);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), finished)) {
+      fill_info(r, info);
+      status = CS_ETIMEDOUT;
+      msg = "request " + std::to_string(id) + " not finished after " + std::to_string(timeout_ms) + " ms";
+      return;
+    }
     fill_info(r, info);
     status = r.status < 0 ? -r.status : CS_OK;
     msg = r.msg;
@@ -482,6 +597,35 @@ int cs_server_wait(cs_server *s, int64_t id, cs_request_info *info) {
   if (rc != CS_OK) return rc;
   if (status != CS_OK) return guarded_call([&] { fail(status, msg); });
   return CS_OK;
+}
+
+int cs_server_cancel(cs_server *s, int64_t id) {
+  return guarded_call([&] {
+    if (!s) fail(CS_EINVAL, "server must not be NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) fail(CS_EINVAL, "unknown request id " + std::to_string(id));
+    Req *r = it->second.get();
+    if (r->status == 2 || r->status < 0) return;  // already over
+    auto q = std::find(s->queue.begin(), s->queue.end(), r);
+    if (q != s->queue.end()) {
+      s->queue.erase(q);
+      s->fail_locked(r, CS_ECANCELED, "request cancelled");
+      return;
+    }
+    r->cancelled = true;  // its worker completes it once no source of it is in flight
+    s->cv_work.notify_all();
+  });
+}
+
+int cs_server_inject_fault(cs_server *s, int kind, int what) {
+  return guarded_call([&] {
+    if (!s) fail(CS_EINVAL, "server must not be NULL");
+    if (kind != CS_REQ_SHORTEST && kind != CS_REQ_SIMPLEST) fail(CS_EINVAL, "unknown request kind");
+    if (what < 0 || what > 3) fail(CS_EINVAL, "unknown fault");
+    s->fault[kind] = what;
+    s->cv_work.notify_all();
+  });
 }
 
 int cs_server_release(cs_server *s, int64_t id) {

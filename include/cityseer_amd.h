@@ -26,6 +26,8 @@ extern "C" {
 #define CS_ENOMEM 3    /* device memory exhausted (or HBM budget exceeded) */
 #define CS_EINTERNAL 4 /* internal invariant violated */
 #define CS_EBUSY 5     /* serving queue full / request not finished */
+#define CS_ETIMEDOUT 6 /* cs_server_wait_for: the request did not finish in time */
+#define CS_ECANCELED 7 /* the request was cancelled (cs_server_cancel) */
 
 /* Last error message of the calling thread (empty string if none). */
 const char *cs_last_error(void);
@@ -394,10 +396,25 @@ void cs_server_destroy(cs_server *s);
 int64_t cs_server_submit(cs_server *s, cs_graph *g, int kind, const cs_params *p, double *out,
                          uint32_t *reach_totals);
 int cs_server_poll(cs_server *s, int64_t id, cs_request_info *info);
-/* Blocks until the request completes; returns its status. */
+/* Blocks until the request completes; returns its status. A request whose
+ * worker failed (a HIP error, an exception) completes with CS_EINTERNAL (or
+ * CS_ENOMEM for host memory); later submissions of that kind then fail at
+ * once with the same message. */
 int cs_server_wait(cs_server *s, int64_t id, cs_request_info *info);
+/* cs_server_wait for at most timeout_ms (negative: no limit); CS_ETIMEDOUT if
+ * the request is still queued or running then (info holds its state). */
+int cs_server_wait_for(cs_server *s, int64_t id, double timeout_ms, cs_request_info *info);
+/* Cancels a request: a queued one completes at once, a running one once its
+ * sources already in a launch have finished (no more are handed out); either
+ * way with status CS_ECANCELED and its device memory released. A request that
+ * has already completed is left as it is. */
+int cs_server_cancel(cs_server *s, int64_t id);
 /* Forgets a completed request. */
 int cs_server_release(cs_server *s, int64_t id);
+/* Failure-handling tests: makes the worker of `kind` fail its next slice with a
+ * std::runtime_error (what = 1) or std::bad_alloc (2), or end itself with an
+ * exception outside the per-request handling (3). No reference counterpart. */
+int cs_server_inject_fault(cs_server *s, int kind, int what);
 
 #ifdef __cplusplus
 }
