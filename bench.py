@@ -391,6 +391,11 @@ def all_sources(ns, dev):
 
 
 def run_batch(args) -> int:
+    print(json.dumps(batch_line(args)), flush=True)
+    return 0
+
+
+def batch_line(args) -> dict:
     """Many graphs of uneven size, each with its own thresholds: one autotuned persistent launch
     (hipGraph replay) against one plain device-resident launch per graph."""
     import numpy as np
@@ -461,7 +466,7 @@ def run_batch(args) -> int:
         batch.add_shortest(ns, distances=dists)
     log("autotune")
     t0 = time.perf_counter()
-    tune = batch.autotune(0.03)
+    tune = batch.autotune(args.tune_fraction)
     tune_s = time.perf_counter() - t0
     log(f"chosen {tune} in {tune_s:.1f} s")
     for _ in range(max(args.warmup, 1)):
@@ -522,21 +527,23 @@ def run_batch(args) -> int:
         },
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-SERVE_DISTANCES = [400, 800, 1600]
+# thresholds of served requests (m) and the node visits one request may ask for at most
+SERVE_DISTANCES = [400, 800, 1600, 3200, 6400, 10000]
+SERVE_MAX_VISITS = {"shortest": 4e8, "simplest": 2.5e7}
 
 
-SERVE_PASSES = 3  # timed passes of each side of the serving comparison (medians reported)
 
 
-def serve_workload(n_requests: int, seed: int = 0):
-    """Graphs decomposed into <= 20 m edges with node counts log-uniform in [2,000, 100,000] (two of
-    them also as duals for simplest-path requests); a stream of requests, each on a random graph,
-    80% shortest / 20% simplest, a random 1%-30% source subset and 1-3 distances from
-    {400, 800, 1600} m."""
+def serve_workload(n_requests: int, seed: int = 0, max_nodes: float = 1e6):
+    """Graphs decomposed into <= 20 m edges with node counts log-uniform in [2,000, max_nodes] (the
+    three smallest also as duals for simplest-path requests); a stream of requests, each on a random
+    graph, 80% shortest / 20% simplest, 1-3 distances from SERVE_DISTANCES and a random 1%-100%
+    source subset, cut where needed so that sources x the nodes a source reaches at the request's
+    largest distance (about 11 per 100 m x 100 m cell of the lattice) stays within
+    SERVE_MAX_VISITS: long thresholds on large graphs take small subsets."""
     import numpy as np
 
     from cityseer.tools import mock
@@ -544,26 +551,26 @@ def serve_workload(n_requests: int, seed: int = 0):
     rng = np.random.default_rng(seed)
     primal, dual = [], []
     for i in range(6):
-        target = float(np.exp(np.log(2e3) + (np.log(1e5) - np.log(2e3)) * i / 5))
+        target = float(np.exp(np.log(2e3) + (np.log(max_nodes) - np.log(2e3)) * i / 5))
         side = max(4, int(round(np.sqrt(target / 11.0))))
         lat = mock.decompose_lattice(mock.perturbed_lattice(side, seed=100 + i), 20.0)
         primal.append(mock.network_structure_from_lattice(lat))
-    for i in range(2):
-        side = 8 + 10 * i
-        lat = mock.decompose_lattice(mock.perturbed_lattice(side, seed=200 + i), 20.0)
-        dual.append(mock.network_structure_from_lattice(mock.lattice_dual(lat)))
+        if i < 3:
+            dual.append(mock.network_structure_from_lattice(mock.lattice_dual(lat)))
     reqs = []
     for _ in range(n_requests):
         simplest = rng.random() < 0.2
+        kind = "simplest" if simplest else "shortest"
         ns = dual[int(rng.integers(len(dual)))] if simplest else primal[int(rng.integers(len(primal)))]
         n = ns.node_count()
-        frac = float(rng.uniform(0.01, 0.3))
-        k = max(1, int(round(frac * n)))
-        src = np.sort(rng.choice(n, k, replace=False)).tolist()
         nd = int(rng.integers(1, 4))
         dists = sorted(rng.choice(SERVE_DISTANCES, nd, replace=False).tolist())
-        reqs.append(("simplest" if simplest else "shortest", ns, dict(distances=[int(d) for d in dists],
-                                                                       source_indices=src)))
+        reach = min(float(n), 11.0 * np.pi * (max(dists) / 100.0) ** 2)
+        frac = float(rng.uniform(0.01, 1.0))
+        k = int(min(frac * n, SERVE_MAX_VISITS[kind] / reach))
+        k = max(1, min(k, n))
+        src = np.sort(rng.choice(n, k, replace=False)).tolist()
+        reqs.append((kind, ns, dict(distances=[int(d) for d in dists], source_indices=src)))
     return primal + dual, reqs
 
 
@@ -578,6 +585,11 @@ def _fifo_latencies(arrivals, service):
 
 
 def run_serve(args) -> int:
+    print(json.dumps(serve_line(args)), flush=True)
+    return 0
+
+
+def serve_line(args) -> dict:
     """Continuous-batching server vs the plain one-call-per-request path on the same request stream:
     saturation throughput (all requests submitted at once) and p50/p99 latency under Poisson
     arrivals at a fraction of the served saturation rate."""
@@ -586,7 +598,7 @@ def run_serve(args) -> int:
     from cityseer.serving import CentralityServer
 
     log("building serving workload")
-    graphs, reqs = serve_workload(args.requests, args.seed)
+    graphs, reqs = serve_workload(args.requests, args.seed, args.serve_max_nodes)
     total_sources = sum(len(kw["source_indices"]) for _, _, kw in reqs)
     for ns in graphs:  # device-resident graphs (and angular adjacency) before any timing
         if ns.is_dual:
@@ -598,10 +610,10 @@ def run_serve(args) -> int:
         return ns.centrality_shortest(**kw) if kind == "shortest" else ns.centrality_simplest(**kw)
 
     # the saturation pass is one burst whose launches interleave differently from run to run:
-    # both sides are timed SERVE_PASSES times and the medians reported
+    # both sides are timed serve_passes times and the medians reported
     passes = []
-    for rep in range(SERVE_PASSES):
-        log(f"plain: one call per request, in order (pass {rep + 1}/{SERVE_PASSES})")
+    for rep in range(args.serve_passes):
+        log(f"plain: one call per request, in order (pass {rep + 1}/{args.serve_passes})")
         service = []
         for kind, ns, kw in reqs:
             t0 = time.perf_counter()
@@ -613,8 +625,8 @@ def run_serve(args) -> int:
 
     with CentralityServer() as srv:
         sats = []
-        for rep in range(SERVE_PASSES):
-            log(f"served: all requests at once (saturation, pass {rep + 1}/{SERVE_PASSES})")
+        for rep in range(args.serve_passes):
+            log(f"served: all requests at once (saturation, pass {rep + 1}/{args.serve_passes})")
             t0 = time.perf_counter()
             handles = [(srv.submit_shortest if k == "shortest" else srv.submit_simplest)(ns, **kw) for k, ns, kw in reqs]
             for h in handles:
@@ -657,7 +669,8 @@ def run_serve(args) -> int:
         "config": {
             "workload": f"{len(reqs)} requests over {len(graphs)} graphs decomposed to <= 20 m edges "
                         f"({min(g.node_count() for g in graphs)}-{max(g.node_count() for g in graphs)} nodes), "
-                        f"80% shortest / 20% simplest, 1-30% of sources, 1-3 distances from {SERVE_DISTANCES} m; "
+                        f"80% shortest / 20% simplest, 1-100% of sources (at most {SERVE_MAX_VISITS} node visits "
+                        f"per request), 1-3 distances from {SERVE_DISTANCES} m; "
                         "continuous-batching server vs one plain call per request",
             "requests": len(reqs),
             "sources": total_sources,
@@ -681,8 +694,22 @@ def run_serve(args) -> int:
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
+
+
+def leg_summary(line: dict, ratio_key: str) -> dict:
+    """The headline fields of a batch or serve line, nested in the plain line."""
+    c = line["config"]
+    out = {"workload": c["workload"], "value": line["value"], "unit": line["unit"],
+           "ms_per_step": line["ms_per_step"], "sources": c["sources"], "plain_value": c["plain_value"],
+           ratio_key: c[ratio_key]}
+    for k in ("p50_ms", "p99_ms", "plain_p50_ms", "plain_p99_ms", "requests", "graphs", "tuning", "autotune_s"):
+        if k in c:
+            out[k] = c[k]
+    if line.get("roofline"):
+        out["roofline_frac"] = line["roofline"]["frac"]
+        out["kernel_ms"] = line["roofline"]["kernel_ms"]
+    return out
 
 
 def main() -> int:
@@ -700,6 +727,12 @@ def main() -> int:
     ap.add_argument("--load", type=float, default=0.6,
                     help="serve mode: Poisson arrival rate as a fraction of the served saturation rate")
     ap.add_argument("--graphs", type=int, default=64, help="batch mode: number of graphs")
+    ap.add_argument("--tune-fraction", type=float, default=0.03, help="batch mode: autotuner sample fraction")
+    ap.add_argument("--serve-passes", type=int, default=3, help="serve mode: timed passes of each side")
+    ap.add_argument("--serve-max-nodes", type=float, default=1e6, help="serve mode: largest graph")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="plain mode: skip the batched (64 graphs) and served legs nested in the line")
+    ap.add_argument("--leg-steps", type=int, default=3, help="plain mode: timed passes of the nested legs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (3 times), to scale HBM counters")
     args = ap.parse_args()
@@ -835,6 +868,18 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
     }
+    if not args.no_legs:
+        # the metric's other half: batched and served throughput against their own plain paths
+        legs = argparse.Namespace(**vars(args))
+        legs.steps = args.leg_steps
+        legs.warmup = 1
+        legs.tune_fraction = 0.01
+        legs.serve_passes = 1
+        log("leg: batch (64 graphs)")
+        line["batch"] = leg_summary(batch_line(legs), "batched_over_plain")
+        line["batch"]["batched_over_plain_cfg2"] = round(line["batch"]["value"] / value, 3)
+        log("leg: serve")
+        line["serve"] = leg_summary(serve_line(legs), "served_over_plain")
     print(json.dumps(line), flush=True)
     return 0
 

@@ -123,6 +123,13 @@ struct SearchArgs {
   uint32_t *progress;  // [waves] sources each wave has finished (pinned host memory), or null
 };
 
+// The reference's f32 exponential is the host libm's expf, correctly rounded
+// in all but a sliver of cases. The device's f32 exp differs from it by up to
+// an ulp, which the differences of exponentials in the segment integrals
+// magnify; the f64 exponential rounded to f32 agrees with the correctly
+// rounded value except when it lands within 2^-29 of a rounding boundary.
+__device__ __forceinline__ float exp_f32(float x) { return (float)exp((double)x); }
+
 // Wave-private workspace: only the owning wave reads or writes it, so
 // workgroup scope (coherent in the CU's L1 and the XCD's L2) is sufficient.
 __device__ __forceinline__ uint64_t ld64(const uint64_t *p) {
@@ -1006,7 +1013,7 @@ __device__ __forceinline__ void flush_target(CJob &j, const WaveWs &w, uint32_t 
         gadd_f64(&accv[((size_t)1 * ts + t) * nacc], (double)(route * wt));
         gadd_f64(&accv[((size_t)2 * ts + t) * nacc], (double)(rank[t] * cyc));
         gadd_f64(&accv[((size_t)3 * ts + t) * nacc], (double)((1.0f / route) * wt));
-        gadd_f64(&accv[((size_t)4 * ts + t) * nacc], (double)(expf(-j.beta[t] * route) * wt));
+        gadd_f64(&accv[((size_t)4 * ts + t) * nacc], (double)(exp_f32(-j.beta[t] * route) * wt));
       }
     }
   }
@@ -1306,7 +1313,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
                   sd[t] += (double)(cur - av);
                   const float hh = a_imp < 1.0f ? logf(curi) : logf(fmaxf(curi / a_imp, 1.1920929e-07f));
                   sh[t] += (double)hh;
-                  const float bb = beta == 0.0f ? curi - a_imp : (expf(nbeta * curi) - expf(nbeta * a_imp)) * inv;
+                  const float bb = beta == 0.0f ? curi - a_imp : (exp_f32(nbeta * curi) - exp_f32(nbeta * a_imp)) * inv;
                   sb[t] += (double)bb;
                 }
                 if (bv == dd) continue;
@@ -1319,7 +1326,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
                   sd[t] += (double)(cur - bv);
                   const float hh = b_imp < 1.0f ? logf(curi) : logf(fmaxf(curi / b_imp, 1.1920929e-07f));
                   sh[t] += (double)hh;
-                  const float bb = beta == 0.0f ? curi - b_imp : (expf(nbeta * curi) - expf(nbeta * b_imp)) * inv;
+                  const float bb = beta == 0.0f ? curi - b_imp : (exp_f32(nbeta * curi) - exp_f32(nbeta * b_imp)) * inv;
                   sb[t] += (double)bb;
                 }
               }
@@ -1374,7 +1381,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
                     auc = (o2s - o_1) + (l2s - l_1);
                   } else {
                     const float nb2 = -beta, inv = 1.0f / nb2;
-                    auc = (expf(nb2 * o2s) - expf(nb2 * o_1)) * inv + (expf(nb2 * l2s) - expf(nb2 * l_1)) * inv;
+                    auc = (exp_f32(nb2 * o2s) - exp_f32(nb2 * o_1)) * inv + (exp_f32(nb2 * l2s) - exp_f32(nb2 * l_1)) * inv;
                   }
                   if (isfinite(auc) && auc >= 0.0f) own = (double)auc;
                 }
@@ -1427,7 +1434,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
             const size_t row = ((size_t)cls * ao.tstride + t) * ao.N + src;
             gadd_f64(&ao.count[row], 1.0);
             // clipped_beta_wt (common.rs:339-348): min(exp(-beta d), mcw) / mcw in f32
-            const float raw = expf(__fmul_rn(-ao.beta[t], tot));
+            const float raw = exp_f32(__fmul_rn(-ao.beta[t], tot));
             gadd_f64(&ao.weighted[row], (double)__fdiv_rn(fminf(raw, ao.mcw[t]), ao.mcw[t]));
             if (ao.max_mask & (1u << t))
               atomicMin(&ao.nearest[(size_t)cls * ao.N + src], (unsigned long long)__double_as_longlong((double)tot));
@@ -2573,12 +2580,12 @@ __global__ void k_diversity(AccessOut ao, uint32_t C, double *div) {
       ent = __fadd_rn(ent, __fmul_rn(p, logf(p)));
       sq = __fadd_rn(sq, powf(p, 2.0f));
       const float d = (float)__longlong_as_double((long long)ao.nearest[(size_t)c * N + v]);
-      const float w = __fdiv_rn(fminf(expf(__fmul_rn(-beta, d)), mcw), mcw);
+      const float w = __fdiv_rn(fminf(exp_f32(__fmul_rn(-beta, d)), mcw), mcw);
       agg = __fadd_rn(agg, __fmul_rn(w, p));
       if (num >= 2) lam = __fadd_rn(lam, __fmul_rn(p, __fdiv_rn(fmaxf(__fsub_rn((float)n, 1.0f), 0.0f), nm1)));
     }
     out[0] = (float)rich;
-    const float h1 = expf(-ent), h2 = powf(sq, -1.0f);
+    const float h1 = exp_f32(-ent), h2 = powf(sq, -1.0f);
     out[1] = isfinite(h1) ? h1 : 0.0f;
     out[2] = isfinite(sq) && isfinite(h2) ? h2 : 0.0f;
     if (!(fabsf(agg) < __FLT_EPSILON__)) {
@@ -2588,14 +2595,14 @@ __global__ void k_diversity(AccessOut ao, uint32_t C, double *div) {
         if (n == 0) continue;
         const float p = __fdiv_rn((float)n, numf);
         const float d = (float)__longlong_as_double((long long)ao.nearest[(size_t)c * N + v]);
-        const float w = __fdiv_rn(fminf(expf(__fmul_rn(-beta, d)), mcw), mcw);
+        const float w = __fdiv_rn(fminf(exp_f32(__fmul_rn(-beta, d)), mcw), mcw);
         const float r = __fdiv_rn(p, agg);
         s0 = __fadd_rn(s0, __fmul_rn(w, powf(r, 0.0f)));
         const float e = __fdiv_rn(__fmul_rn(w, p), agg);
         if (e > 0.0f) we = __fadd_rn(we, __fmul_rn(e, logf(e)));
         s2 = __fadd_rn(s2, __fmul_rn(w, powf(r, 2.0f)));
       }
-      const float w0 = powf(s0, 1.0f), w1 = expf(-we), w2 = powf(s2, -1.0f);
+      const float w0 = powf(s0, 1.0f), w1 = exp_f32(-we), w2 = powf(s2, -1.0f);
       out[3] = isfinite(s0) && isfinite(w0) ? w0 : 0.0f;
       out[4] = isfinite(w1) ? w1 : 0.0f;
       out[5] = isfinite(s2) && isfinite(w2) ? w2 : 0.0f;
