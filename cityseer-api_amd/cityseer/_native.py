@@ -165,6 +165,11 @@ _SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_uint32, _u32p, _i32p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, _f64p,
          ctypes.c_uint32, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, _f64p],
     ),
+    "cs_data_stats": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p, _u32p, _f64p, ctypes.c_uint32, _f32p,
+         ctypes.c_uint32, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, _f64p],
+    ),
     "cs_tree_shortest": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, _u8p, _u8p, _i32p, _f32p, _f32p,

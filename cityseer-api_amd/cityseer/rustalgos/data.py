@@ -10,7 +10,9 @@ The assignment of entries to nodes is ``node_data_map``: ``{node index:
 (data.rs:455-512). Here it is set by the caller; the geometric assignment
 itself (R-tree candidates, barrier and street-crossing checks,
 graph.rs:1346-1535) is not part of this package and raises
-``NotImplementedError``, as do ``stats`` and the angular variants.
+``NotImplementedError``, as do the angular variants. ``stats`` (data.rs:1013-1233)
+runs the same launch with each source's wave summarising the numeric values of
+the entries it reaches.
 ``mixed_uses`` (data.rs:749-944) shares the launch and adds a per-node
 diversity kernel over the class counts.
 """
@@ -123,6 +125,41 @@ class AccessibilityResult:
         return dict(self._lu_map)
 
 
+STAT_METRICS = 14
+_STAT_NAMES = ("sum", "sum_wt", "mean", "mean_wt", "median", "median_wt", "count", "count_wt", "variance",
+               "variance_wt", "mad", "mad_wt", "max", "min")
+
+
+class Stats:
+    """One numeric map's statistics (data.rs:170-248): each getter returns ``{distance: f64[N]}``."""
+
+    def __init__(self, distances, node_indices, rows: np.ndarray):
+        self._distances = list(distances)
+        self._idx = np.asarray(node_indices, dtype=np.int64)
+        self._rows = rows  # [14][T][N]
+
+    def _load(self, m: int) -> dict[int, np.ndarray]:
+        return {int(d): self._rows[m][i][self._idx].copy() for i, d in enumerate(self._distances)}
+
+
+for _m, _name in enumerate(_STAT_NAMES):
+    setattr(Stats, _name, property(lambda self, m=_m: self._load(m)))
+
+
+class StatsResult:
+    """Result of ``DataMap.stats`` (data.rs:250-304): ``result`` is one ``Stats`` per numeric map."""
+
+    def __init__(self, distances, node_keys_py, node_indices, stats: list[Stats]):
+        self.distances = list(distances)
+        self.node_keys_py = list(node_keys_py)
+        self.node_indices = list(node_indices)
+        self._stats = stats
+
+    @property
+    def result(self) -> list[Stats]:
+        return list(self._stats)
+
+
 def _composite_key(key: Any) -> str:
     """Keys are stored as "type name:str(value)" (common.rs:91-96)."""
     return f"{type(key).__name__}:{key}"
@@ -165,8 +202,60 @@ class DataMap:
             "{node index: [(data key, distance), ...]} instead"
         )
 
-    def stats(self, *args, **kwargs):
-        raise NotImplementedError("stats is not available here")
+    def stats(
+        self,
+        network_structure,
+        numerical_maps: list[dict],
+        distances: list[int] | None = None,
+        betas: list[float] | None = None,
+        minutes: list[float] | None = None,
+        angular: bool | None = None,
+        spatial_tolerance: int | None = None,
+        min_threshold_wt: float | None = None,
+        speed_m_s: float | None = None,
+        pbar_disabled: bool | None = None,
+    ) -> StatsResult:
+        """Numerical summaries of the data each node reaches (data.rs:1013-1233): per map and
+        distance, the sum, mean, count, variance, median, median absolute deviation, max and min of
+        the values of the entries within the distance (one per dedupe group, through its nearest
+        member), and their distance-weighted forms."""
+        from . import pair_distances_betas_time
+
+        if angular:
+            network_structure.validate_dual_for_angular("stats")
+            raise NotImplementedError("angular stats are not available here")
+        speed = WALKING_SPEED if speed_m_s is None else float(speed_m_s)
+        dists, bets, secs = pair_distances_betas_time(speed, distances, betas, minutes, min_threshold_wt)
+        keys, entry_group, a_node, a_ent, a_dist = self._assignment_arrays()
+        index = {k: i for i, k in enumerate(keys)}
+        values = np.full((len(numerical_maps), len(keys)), np.nan, dtype=np.float32)
+        for mp, num_map in enumerate(numerical_maps):
+            if len(num_map) != self.count():
+                raise ValueError("The number of numeric data points must match the number of data points")
+            for k, v in num_map.items():
+                ck = _composite_key(k)
+                if ck not in index:
+                    raise KeyError(f"Data entries key missing: {ck}")
+                values[mp, index[ck]] = np.float32(v)
+        mcw = clip_wts_curve(dists, bets, int(spatial_tolerance or 0))
+        n, T = network_structure.node_bound(), len(dists)
+        out = np.empty((len(numerical_maps), STAT_METRICS, T, n), dtype=np.float64)
+        p = _native.ptr
+        self.progress_init()
+        _native.check(
+            _native.load().cs_data_stats(
+                network_structure._native_handle(), len(keys), p(entry_group, ctypes.c_uint32), len(a_node),
+                p(a_node, ctypes.c_uint32), p(a_ent, ctypes.c_uint32), p(a_dist, ctypes.c_double),
+                len(numerical_maps), p(values, ctypes.c_float), T,
+                p(np.asarray(dists, dtype=np.uint32), ctypes.c_uint32),
+                p(np.asarray(bets, dtype=np.float32), ctypes.c_float),
+                p(np.asarray(mcw, dtype=np.float32), ctypes.c_float), int(max(secs)), speed, p(out, ctypes.c_double),
+            )
+        )
+        self._progress = n
+        idx = network_structure.node_indices()
+        return StatsResult(dists, network_structure.node_keys_py(), idx, [Stats(dists, idx, out[m]) for m in
+                                                                          range(len(numerical_maps))])
 
     def _assignment_arrays(self):
         keys = list(self.entries.keys())

@@ -1221,12 +1221,20 @@ extern "C" {
 
 // Accessibility rows (out_count / out_weighted / out_nearest) or, with out_div,
 // the diversity rows of the same per-source class counts instead.
+// Numeric maps of a statistics call: values [n_maps][n_entries] (NaN: none),
+// out [n_maps][kStatMetrics][T][N].
+struct StatsIn {
+  uint32_t n_maps = 0;
+  const float *values = nullptr;
+  double *out = nullptr;
+};
+
 static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
                        uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
                        const uint32_t *assign_entry, const double *assign_dist, uint32_t n_thresholds,
                        const uint32_t *distances, const float *betas, const float *max_curve_wts,
                        uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
-                       double *out_nearest, double *out_div) {
+                       double *out_nearest, double *out_div, const StatsIn *stats_in = nullptr) {
   const auto t0 = std::chrono::steady_clock::now();
   t_stats = cs_stats{};
   int rc = guarded([&] {
@@ -1235,10 +1243,12 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     const uint32_t N = g.n, T = n_thresholds, C = n_classes;
     if (T == 0) fail(CS_EINVAL, "at least one distance is required");
     if (!distances || !betas || !max_curve_wts) fail(CS_EINVAL, "distances, betas and max_curve_wts must not be NULL");
-    if (n_entries && (!entry_group || !entry_class)) fail(CS_EINVAL, "entry_group and entry_class must not be NULL");
+    if (n_entries && (!entry_group || (!entry_class && !stats_in)))
+      fail(CS_EINVAL, "entry_group and entry_class must not be NULL");
+    if (stats_in && stats_in->n_maps && n_entries && !stats_in->values) fail(CS_EINVAL, "values must not be NULL");
     if (n_assign && (!assign_node || !assign_entry || !assign_dist))
       fail(CS_EINVAL, "assign_node, assign_entry and assign_dist must not be NULL");
-    if (!out_div && (size_t)C * N && (!out_count || !out_weighted || !out_nearest))
+    if (!out_div && !stats_in && (size_t)C * N && (!out_count || !out_weighted || !out_nearest))
       fail(CS_EINVAL, "outputs must not be NULL");
     if (!std::isfinite(speed_m_s) || !(speed_m_s > 0.0f)) fail(CS_EINVAL, "speed_m_s must be finite and positive");
     for (uint32_t t = 0; t < T; ++t)
@@ -1246,7 +1256,8 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
         fail(CS_EINVAL, "Max curve weight must be in the range [0, 1].");
     for (uint32_t e = 0; e < n_entries; ++e) {
       if (entry_group[e] >= n_entries) fail(CS_EINVAL, "entry_group values must be below n_entries");
-      if (entry_class[e] < -1 || entry_class[e] >= (int32_t)C) fail(CS_EINVAL, "entry_class out of range");
+      if (!stats_in && (entry_class[e] < -1 || entry_class[e] >= (int32_t)C))
+        fail(CS_EINVAL, "entry_class out of range");
     }
     for (uint32_t i = 0; i < n_assign; ++i) {
       if (assign_node[i] >= N) fail(CS_EINVAL, "assignment node out of range");
@@ -1255,19 +1266,34 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
         fail(CS_EINVAL, "assignment distances must be finite and non-negative");
     }
     const size_t CTN = (size_t)C * T * N;
-    if (out_div) {
+    const uint32_t SM = stats_in ? stats_in->n_maps * kStatMetrics : 0;  // statistics rows per threshold
+    std::vector<double> stat_init;
+    if (stats_in) {
+      // StatsResult::new (data.rs:262-295): sums and counts start at 0, the rest at NaN
+      stat_init.assign((size_t)SM * T * N, std::nan(""));
+      for (uint32_t mp = 0; mp < stats_in->n_maps; ++mp)
+        for (uint32_t m : {0u, 1u, 6u, 7u})
+          std::fill_n(stat_init.begin() + ((size_t)mp * kStatMetrics + m) * T * N, (size_t)T * N, 0.0);
+      std::copy(stat_init.begin(), stat_init.end(), stats_in->out);
+    } else if (out_div) {
       std::fill(out_div, out_div + (size_t)kDivMetrics * T * N, 0.0);
     } else {
       std::fill(out_count, out_count + CTN, 0.0);
       std::fill(out_weighted, out_weighted + CTN, 0.0);
       std::fill(out_nearest, out_nearest + (size_t)C * N, std::nan(""));
     }
-    if (N == 0 || C == 0 || n_assign == 0) return;
-    // members: assignments grouped by dedupe group, assignment order within a group
+    if (N == 0 || n_assign == 0 || (stats_in ? stats_in->n_maps == 0 : C == 0)) return;
+    // members: assignments grouped by dedupe group, assignment order within a
+    // group; groups of two or more members get a slot of the per-wave tables
     std::vector<uint32_t> gcount(n_entries + 1, 0);
     for (uint32_t i = 0; i < n_assign; ++i) ++gcount[entry_group[assign_entry[i]] + 1];
+    std::vector<uint32_t> gslot(n_entries, kSoloGroup);
+    uint32_t n_shared = 0, n_groups = 0;
+    for (uint32_t x = 0; x < n_entries; ++x) {
+      if (gcount[x + 1] > 1) gslot[x] = n_shared++;
+      if (gcount[x + 1] > 0) ++n_groups;
+    }
     for (uint32_t x = 0; x < n_entries; ++x) gcount[x + 1] += gcount[x];
-    const std::vector<uint32_t> group_off = gcount;
     std::vector<uint4> mem(n_assign);
     std::vector<uint32_t> mem_node(n_assign);
     for (uint32_t i = 0; i < n_assign; ++i) {
@@ -1277,7 +1303,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
       const float dd = (float)assign_dist[i];  // data.rs:548 casts the f64 distance to f32
       uint32_t bits;
       std::memcpy(&bits, &dd, 4);
-      mem[m] = make_uint4(node, e, bits, grp);
+      mem[m] = make_uint4(node, e, bits, gslot[grp]);
       mem_node[m] = node;
     }
     // members of each device node
@@ -1314,6 +1340,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
       }
       c.count = ao.count + (size_t)t0 * N;
       c.weighted = ao.weighted + (size_t)t0 * N;
+      if (ao.stats) c.stats = ao.stats + (size_t)t0 * N;
       return c;
     };
     const int dev = current_device();
@@ -1322,7 +1349,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     DevEdges &de = device_edges(g, dg, speed_m_s);
     OwnedStream stream;
     DevBuf d_node_off(4ull * (N + 1)), d_node_mem(4ull * n_assign), d_mem(16ull * n_assign),
-        d_group_off(4ull * (n_entries + 1)), d_class(4ull * std::max<uint32_t>(n_entries, 1)),
+        d_class(4ull * std::max<uint32_t>(n_entries, 1)),
         d_src(4ull * std::max<size_t>(srcs.size(), 1)), d_count(8 * CTN), d_wt(8 * CTN), d_near(8ull * C * N);
     auto up = [&](DevBuf &d, const void *hsrc, size_t bytes) {
       if (bytes) hip_check(hipMemcpyAsync(d.p, hsrc, bytes, hipMemcpyHostToDevice, stream), "copy");
@@ -1330,8 +1357,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     up(d_node_off, node_off.data(), 4ull * (N + 1));
     up(d_node_mem, node_mem.data(), 4ull * n_assign);
     up(d_mem, mem.data(), 16ull * n_assign);
-    up(d_group_off, group_off.data(), 4ull * (n_entries + 1));
-    up(d_class, entry_class, 4ull * n_entries);
+    if (entry_class) up(d_class, entry_class, 4ull * n_entries);
     up(d_src, srcs.data(), 4ull * srcs.size());
     {
       const std::vector<double> nan_rows((size_t)C * N, std::nan(""));  // NaN bits: no entry yet
@@ -1342,7 +1368,18 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     ao.node_off = d_node_off.as<uint32_t>();
     ao.node_mem = d_node_mem.as<uint32_t>();
     ao.mem = d_mem.as<uint4>();
-    ao.group_off = d_group_off.as<uint32_t>();
+    ao.n_shared = n_shared;
+    ao.n_groups = n_groups;
+    DevBuf d_stats(8ull * std::max<size_t>((size_t)SM * T * N, 1)),
+        d_values(4ull * std::max<size_t>(stats_in ? (size_t)stats_in->n_maps * n_entries : 0, 1));
+    if (stats_in) {
+      up(d_stats, stat_init.data(), 8ull * SM * T * N);
+      up(d_values, stats_in->values, 4ull * stats_in->n_maps * n_entries);
+      ao.stats = d_stats.as<double>();
+      ao.values = d_values.as<float>();
+      ao.n_maps = stats_in->n_maps;
+      ao.n_entries = n_entries;
+    }
     ao.entry_class = d_class.as<int32_t>();
     ao.count = d_count.as<double>();
     ao.weighted = d_wt.as<double>();
@@ -1353,6 +1390,15 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
         launch_access(g, dg, de, (float)max_seconds, d_src.as<uint32_t>(), (uint32_t)srcs.size(), chunk(t0), stream,
                       t_stats, t0 + kMaxT >= T ? prog : nullptr);
     const uint32_t *ip = result_order(g);
+    if (stats_in) {
+      std::vector<double> hs((size_t)SM * T * N);
+      hip_check(hipMemcpyAsync(hs.data(), d_stats.p, 8 * hs.size(), hipMemcpyDeviceToHost, stream), "copy out");
+      hip_check(hipStreamSynchronize(stream), "sync");
+      unpack_acc(hs, N, SM, T, stats_in->out, ip);
+      t_stats.sources = srcs.size();
+      progress_finish(g, N);
+      return;
+    }
     if (out_div) {
       DevBuf d_div(8ull * kDivMetrics * T * N);
       for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
@@ -1399,6 +1445,21 @@ int cs_mixed_uses(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, 
   return access_impl(h, n_entries, entry_group, entry_class, n_classes, n_assign, assign_node, assign_entry,
                      assign_dist, n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, nullptr,
                      nullptr, nullptr, out);
+}
+
+int cs_data_stats(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, uint32_t n_assign,
+                  const uint32_t *assign_node, const uint32_t *assign_entry, const double *assign_dist,
+                  uint32_t n_maps, const float *values, uint32_t n_thresholds, const uint32_t *distances,
+                  const float *betas, const float *max_curve_wts, uint32_t max_seconds, float speed_m_s,
+                  double *out) {
+  if (!out) return guarded([] { fail(CS_EINVAL, "out must not be NULL"); });
+  StatsIn st;
+  st.n_maps = n_maps;
+  st.values = values;
+  st.out = out;
+  return access_impl(h, n_entries, entry_group, nullptr, 0, n_assign, assign_node, assign_entry, assign_dist,
+                     n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, nullptr, nullptr, nullptr,
+                     nullptr, &st);
 }
 
 int cs_tree_shortest(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, uint8_t *visited,

@@ -218,6 +218,8 @@ struct JobSpec {
   const uint32_t *d_od_off = nullptr;
   const uint32_t *d_od_dest = nullptr;
   const float *d_od_w = nullptr;
+  // per-wave records beyond the graph's nodes (land-use dedupe group tables)
+  uint32_t extra_records = 0;
 };
 // All sources of all jobs in one persistent launch, or only `items`
 // ({job, source position}) when given.
@@ -263,18 +265,27 @@ struct TreeOut {
 struct AccessOut {
   const uint32_t *node_off = nullptr;    // [N + 1] into node_mem
   const uint32_t *node_mem = nullptr;    // member indices assigned to each node
-  const uint4 *mem = nullptr;            // {node, entry, data distance f32 bits, group}
-  const uint32_t *group_off = nullptr;   // [G + 1] member range of each dedupe group
+  const uint4 *mem = nullptr;            // {node, entry, data distance f32 bits, shared group slot or kSoloGroup}
+  uint32_t n_shared = 0;                 // dedupe groups of two or more members (one table slot each)
   const int32_t *entry_class = nullptr;  // class of each entry (-1: not counted)
   double *count = nullptr;               // [C][T][N] entries within each distance
   double *weighted = nullptr;            // [C][T][N] clipped negative-exponential weights
   unsigned long long *nearest = nullptr; // [C][N] f64 bits of the nearest entry (NaN bits: none)
+  // land-use statistics instead of counts (data.rs:1013-1233): when stats is
+  // set, each source's rows [map][kStatMetrics][t] are computed from the
+  // numeric values of the entries it reaches, [maps][n_entries] (NaN: none)
+  double *stats = nullptr;
+  const float *values = nullptr;
+  uint32_t n_maps = 0, n_entries = 0;
+  uint32_t n_groups = 0;  // dedupe groups with at least one member (winners per source at most)
   uint32_t T = 0, N = 0;
   uint32_t tstride = 0;   // threshold stride of the count / weighted rows (and of the diversity rows)
   uint32_t max_mask = 0;  // thresholds equal to the largest distance
   float maxdist = 0.0f;   // max walk seconds * speed
   float dist[kMaxT], beta[kMaxT], mcw[kMaxT];
 };
+// A member whose dedupe group has no other member.
+constexpr uint32_t kSoloGroup = 0xffffffffu;
 void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uint32_t *d_sources,
                    uint32_t n_sources, const AccessOut &out, hipStream_t stream, cs_stats &st,
                    uint32_t *progress = nullptr);
@@ -282,6 +293,9 @@ void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uin
 // out.count / out.nearest, n_classes of them): div [kDivMetrics][T][N] =
 // hill q0, q1, q2, distance-weighted hill q0, q1, q2, shannon, gini-simpson.
 constexpr uint32_t kDivMetrics = 8;
+// Statistics rows per numeric map, in this order: sum, sum_wt, mean, mean_wt,
+// median, median_wt, count, count_wt, variance, variance_wt, mad, mad_wt, max, min.
+constexpr uint32_t kStatMetrics = 14;
 void launch_diversity(const AccessOut &out, uint32_t n_classes, double *div, hipStream_t stream);
 
 // mode 0: single-predecessor tree, 1: Brandes multi-predecessor state.

@@ -111,6 +111,8 @@ struct SearchArgs {
   char *ws;         // [waves][ws_stride] scratch
   size_t ws_stride;
   size_t o_rec, o_pm, o_acc, o_ods, o_bkt, o_bnd, o_lvl;
+  size_t o_st;       // land-use statistics: per-wave lists (winner count, winners, sort keys, values, weights)
+  uint32_t st_cap;   // winners one source may have in those lists
   uint32_t *epochs;
   uint32_t *next;            // [kXcd] per-XCD item counters
   uint32_t xbeg[kXcd + 1];   // XCD x owns item indices [xbeg[x], xbeg[x+1])
@@ -1079,6 +1081,170 @@ __device__ void flush_targets(CJob &j, const WaveWs &w, const SrcCtx &c, uint32_
 // ---------------------------------------------------------------------------
 // Persistent kernel: each wave loops over sources taken from the counters.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Land-use statistics of one source (data.rs:946-1233), computed by its wave
+// from the list of entries it reaches (one per dedupe group, with its total
+// distance), in ascending entry order: f32 sums in that order, medians over
+// the values sorted by Rust's f32 total order (ties kept in list order, as
+// the reference's stable sort keeps them).
+
+// f32 total order (f32::total_cmp) as an ascending u32 key.
+__device__ __forceinline__ uint32_t total_key(float x) {
+  const uint32_t b = __float_as_uint(x);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// Sorts buf[0..n) ascending in the wave's scratch (room for the next power of two).
+__device__ void wave_sort64(uint64_t *buf, uint32_t n) {
+  const uint32_t lane = lane_id();
+  uint32_t P = 1;
+  while (P < n) P <<= 1;
+  for (uint32_t i = n + lane; i < P; i += kWave) st64(&buf[i], ~0ull);
+  wave_fence();
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+      for (uint32_t i = lane; i < P; i += kWave) {
+        const uint32_t l = i ^ jj;
+        if (l > i) {
+          const uint64_t x = ld64(&buf[i]), y = ld64(&buf[l]);
+          if ((x > y) == ((i & k) == 0)) {
+            st64(&buf[i], y);
+            st64(&buf[l], x);
+          }
+        }
+      }
+      wave_fence();
+    }
+  }
+}
+
+// weighted_median (data.rs:961-997) over keys sorted by value, the value and
+// weight of key k being val(k) and wts[k]; one lane.
+template <typename V>
+__device__ float weighted_median_sorted(const uint64_t *keys, uint32_t n, const float *wts, float total_wt, V val) {
+  if (n == 0 || total_wt == 0.0f) return __int_as_float(0x7fc00000);
+  const float mid = total_wt / 2.0f;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t k = (uint32_t)ld64(&keys[i]);
+    if (wts[k] > mid) return val(k);
+  }
+  float agg = 0.0f;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t k = (uint32_t)ld64(&keys[i]);
+    agg = agg + wts[k];
+    if (agg == mid) return i + 1 < n ? (val(k) + val((uint32_t)ld64(&keys[i + 1]))) / 2.0f : val(k);
+    if (agg > mid) return val(k);
+  }
+  return val((uint32_t)ld64(&keys[n - 1]));
+}
+
+// median (data.rs:946-959) of keys sorted by value.
+template <typename V>
+__device__ __forceinline__ float median_sorted(const uint64_t *keys, uint32_t n, V val) {
+  if (n == 0) return __int_as_float(0x7fc00000);
+  if (n & 1u) return val((uint32_t)ld64(&keys[n / 2]));
+  return (val((uint32_t)ld64(&keys[n / 2 - 1])) + val((uint32_t)ld64(&keys[n / 2]))) / 2.0f;
+}
+
+// The fourteen statistics of every map and threshold for one source, from
+// its winners win[0..nw) = {entry << 32 | bits(total distance)}.
+__device__ void source_stats(const AccessOut &ao, uint32_t src, uint64_t *win, uint32_t nw, uint64_t *keys,
+                             float *vals, float *wts) {
+  const uint32_t lane = lane_id();
+  const float nan = __int_as_float(0x7fc00000);
+  wave_sort64(win, nw);  // ascending entry
+  for (uint32_t mp = 0; mp < ao.n_maps; ++mp) {
+    const float *num_of = ao.values + (size_t)mp * ao.n_entries;
+    for (uint32_t t = 0; t < ao.T; ++t) {
+      // the values within this threshold, in entry order
+      uint32_t n = 0;
+      for (uint32_t base = 0; base < nw; base += kWave) {
+        const uint32_t i = base + lane;
+        bool keep = false;
+        float num = 0.0f, wt = 0.0f;
+        if (i < nw) {
+          const uint64_t wv = ld64(&win[i]);
+          const float tot = __uint_as_float((uint32_t)wv);
+          num = num_of[(uint32_t)(wv >> 32)];
+          keep = tot <= ao.dist[t] && !isnan(num);
+          if (keep) wt = __fdiv_rn(fminf(exp_f32(__fmul_rn(-ao.beta[t], tot)), ao.mcw[t]), ao.mcw[t]);
+        }
+        const uint64_t bal = __ballot(keep);
+        if (keep) {
+          const uint32_t k = n + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+          vals[k] = num;
+          wts[k] = wt;
+        }
+        n += (uint32_t)__popcll(bal);
+      }
+      wave_fence();
+      // sums, counts, extremes (one lane, list order)
+      float r[kStatMetrics];
+      float count_wt = 0.0f, median = nan, median_wt = nan;
+      if (lane == 0) {
+        float sum = 0.0f, sum_wt = 0.0f, count = 0.0f, sum_sq = 0.0f, sum_sq_wt = 0.0f, mn = nan, mx = nan;
+        for (uint32_t k = 0; k < n; ++k) {
+          const float v = vals[k], w = wts[k];
+          sum = sum + v;
+          sum_wt = sum_wt + v * w;
+          count = count + 1.0f;
+          count_wt = count_wt + w;
+          sum_sq = sum_sq + v * v;
+          sum_sq_wt = sum_sq_wt + w * v * v;
+          mx = isnan(mx) ? v : fmaxf(mx, v);
+          mn = isnan(mn) ? v : fminf(mn, v);
+        }
+        const float mean = count > 0.0f ? sum / count : nan;
+        const float mean_wt = count_wt > 0.0f ? sum_wt / count_wt : nan;
+        r[0] = sum;
+        r[1] = sum_wt;
+        r[2] = mean;
+        r[3] = mean_wt;
+        r[6] = count;
+        r[7] = count_wt;
+        r[8] = count > 0.0f ? fmaxf(sum_sq / count - mean * mean, 0.0f) : nan;
+        r[9] = count_wt > 0.0f ? fmaxf(sum_sq_wt / count_wt - mean_wt * mean_wt, 0.0f) : nan;
+        r[12] = mx;
+        r[13] = mn;
+      }
+      count_wt = __shfl(count_wt, 0);
+      auto value = [&](uint32_t k) { return vals[k]; };
+      // median and weighted median: the values in total order
+      for (uint32_t k = lane; k < n; k += kWave) st64(&keys[k], ((uint64_t)total_key(vals[k]) << 32) | k);
+      wave_fence();
+      wave_sort64(keys, n);
+      if (lane == 0) {
+        median = median_sorted(keys, n, value);
+        median_wt = weighted_median_sorted(keys, n, wts, count_wt, value);
+        r[4] = median;
+        r[5] = median_wt;
+      }
+      median = __shfl(median, 0);
+      median_wt = __shfl(median_wt, 0);
+      wave_fence();
+      // median absolute deviation
+      auto dev = [&](uint32_t k) { return fabsf(vals[k] - median); };
+      for (uint32_t k = lane; k < n; k += kWave) st64(&keys[k], ((uint64_t)total_key(dev(k)) << 32) | k);
+      wave_fence();
+      wave_sort64(keys, n);
+      if (lane == 0) r[10] = (n > 0 && !isnan(median)) ? median_sorted(keys, n, dev) : nan;
+      wave_fence();
+      // weighted median absolute deviation
+      auto dev_wt = [&](uint32_t k) { return fabsf(vals[k] - median_wt); };
+      for (uint32_t k = lane; k < n; k += kWave) st64(&keys[k], ((uint64_t)total_key(dev_wt(k)) << 32) | k);
+      wave_fence();
+      wave_sort64(keys, n);
+      if (lane == 0) {
+        r[11] = (n > 0 && !isnan(median_wt) && count_wt > 0.0f) ? weighted_median_sorted(keys, n, wts, count_wt, dev_wt)
+                                                                 : nan;
+        for (uint32_t m = 0; m < kStatMetrics; ++m)
+          ao.stats[(((size_t)mp * kStatMetrics + m) * ao.tstride + t) * ao.N + src] = (double)r[m];
+      }
+      wave_fence();
+    }
+  }
+}
+
 // With up to four thresholds the search asks for five resident waves per SIMD
 // (96 VGPRs, a few spilled to scratch in cold paths): 45.2 ms against 45.6 ms at
 // four on the 200x200 workload; six waves spill in hot paths and run slower.
@@ -1400,48 +1566,82 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
       // A member (entry e at node v, data distance dd) counts when v lies
       // below the walk limit, secs_v * speed + dd is within the largest
       // distance and no other member of its dedupe group is nearer (equal
-      // totals: the lower member index wins). Each group therefore counts
-      // once, through its nearest member, with no per-source map.
+      // totals: the lower member index wins). Groups of several members keep,
+      // per wave, two epoch-tagged records after the node records: the nearest
+      // total (pass 1), then the lowest member index at that total (pass 2), so
+      // each group counts once, through its nearest member, in passes linear in
+      // the members reached.
       const AccessOut &ao = a.acc;
       const uint32_t src = c.src;
-      for (uint32_t s = lane; s < R; s += kWave) {
-        const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
-        const uint32_t v = (uint32_t)lo;
-        const float secs = __uint_as_float((uint32_t)(lo >> 32));
-        if (!(secs < j.maxsec)) continue;
-        const float net = __fmul_rn(secs, speed);
-        for (uint32_t k = ao.node_off[v]; k < ao.node_off[v + 1]; ++k) {
-          const uint32_t m = ao.node_mem[k];
-          const uint4 me = ao.mem[m];
-          const float tot = __fadd_rn(net, __uint_as_float(me.z));
-          if (!(tot <= ao.maxdist)) continue;
-          bool win = true;
-          for (uint32_t m2 = ao.group_off[me.w]; m2 < ao.group_off[me.w + 1] && win; ++m2) {
-            if (m2 == m) continue;
-            const uint4 o = ao.mem[m2];
-            const uint64_t ko = ld64(nkey(w, o.x));
-            if (!key_current(ko, c.tag)) continue;
-            const float so = key_secs(ko);
-            if (!(so < j.maxsec)) continue;
-            const float to = __fadd_rn(__fmul_rn(so, speed), __uint_as_float(o.z));
-            if (to <= ao.maxdist && (to < tot || (to == tot && m2 < m))) win = false;
-          }
-          if (!win) continue;
-          const int32_t cls = ao.entry_class[me.y];
-          if (cls < 0) continue;
-          for (uint32_t t = 0; t < ao.T; ++t) {
-            if (!(tot <= ao.dist[t])) continue;
-            const size_t row = ((size_t)cls * ao.tstride + t) * ao.N + src;
-            gadd_f64(&ao.count[row], 1.0);
-            // clipped_beta_wt (common.rs:339-348): min(exp(-beta d), mcw) / mcw in f32
-            const float raw = exp_f32(__fmul_rn(-ao.beta[t], tot));
-            gadd_f64(&ao.weighted[row], (double)__fdiv_rn(fminf(raw, ao.mcw[t]), ao.mcw[t]));
-            if (ao.max_mask & (1u << t))
-              atomicMin(&ao.nearest[(size_t)cls * ao.N + src], (unsigned long long)__double_as_longlong((double)tot));
+      uint64_t *grec = w.node + 2 * (size_t)j.nacc;
+      // the epoch sits where node keys hold theirs, so that records left in these
+      // words by earlier sources (node keys of a larger graph included) are older
+      const uint64_t tag = (uint64_t)c.tag << 54;
+      // statistics: the winners go to a list in the wave's scratch instead of the counts
+      char *stb = a.ws + (size_t)wid * a.ws_stride + a.o_st;
+      uint32_t *st_n = reinterpret_cast<uint32_t *>(stb);
+      uint64_t *st_win = reinterpret_cast<uint64_t *>(stb + 256);
+      if (ao.stats && lane == 0) st32(st_n, 0u);
+      for (int pass = 0; pass < 3; ++pass) {
+        if (pass < 2 && ao.n_shared == 0) continue;
+        for (uint32_t s = lane; s < R; s += kWave) {
+          const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
+          const uint32_t v = (uint32_t)lo;
+          const float secs = __uint_as_float((uint32_t)(lo >> 32));
+          if (!(secs < j.maxsec)) continue;
+          const float net = __fmul_rn(secs, speed);
+          for (uint32_t k = ao.node_off[v]; k < ao.node_off[v + 1]; ++k) {
+            const uint32_t m = ao.node_mem[k];
+            const uint4 me = ao.mem[m];
+            const float tot = __fadd_rn(net, __uint_as_float(me.z));
+            if (!(tot <= ao.maxdist)) continue;
+            const bool shared = me.w != kSoloGroup;
+            const uint64_t tkey = tag | ((uint64_t)(uint32_t)~__float_as_uint(tot) << 22);  // larger = nearer
+            if (pass == 0) {
+              if (shared) wmax64(&grec[2 * (size_t)me.w], tkey);
+              continue;
+            }
+            if (pass == 1) {
+              if (shared && ld64(&grec[2 * (size_t)me.w]) == tkey) wmax64(&grec[2 * (size_t)me.w + 1], tag | (uint32_t)~m);
+              continue;
+            }
+            if (shared && ld64(&grec[2 * (size_t)me.w + 1]) != (tag | (uint32_t)~m)) continue;
+            if (ao.stats) {
+              const uint32_t q = __hip_atomic_fetch_add(st_n, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              if (q < a.st_cap) st64(&st_win[q], ((uint64_t)me.y << 32) | __float_as_uint(tot));
+              continue;
+            }
+            const int32_t cls = ao.entry_class[me.y];
+            if (cls < 0) continue;
+            for (uint32_t t = 0; t < ao.T; ++t) {
+              if (!(tot <= ao.dist[t])) continue;
+              const size_t row = ((size_t)cls * ao.tstride + t) * ao.N + src;
+              gadd_f64(&ao.count[row], 1.0);
+              // clipped_beta_wt (common.rs:339-348): min(exp(-beta d), mcw) / mcw in f32
+              const float raw = exp_f32(__fmul_rn(-ao.beta[t], tot));
+              gadd_f64(&ao.weighted[row], (double)__fdiv_rn(fminf(raw, ao.mcw[t]), ao.mcw[t]));
+              if (ao.max_mask & (1u << t))
+                atomicMin(&ao.nearest[(size_t)cls * ao.N + src], (unsigned long long)__double_as_longlong((double)tot));
+            }
           }
         }
+        wave_fence();
       }
-      wave_fence();
+      if (ao.stats) {
+        const uint32_t nw = uniform(ld32(st_n));
+        if (nw > a.st_cap) {
+          // more winners than the list holds: the source runs again with longer lists
+          if (lane == 0) {
+            const uint32_t q = atomicAdd(a.n_fail, 1u);
+            a.fail_list[q] = item;
+            atomicOr(a.fail_flags, kFailCap);
+          }
+          continue;
+        }
+        uint64_t *st_keys = st_win + 2 * (size_t)a.st_cap;
+        float *st_vals = reinterpret_cast<float *>(st_keys + 2 * (size_t)a.st_cap);
+        source_stats(ao, src, st_win, nw, st_keys, st_vals, st_vals + a.st_cap);
+      }
     } else {
       // single-source exports (trees / Brandes state)
       for (uint32_t s = lane; s < R; s += kWave) {
@@ -1552,6 +1752,8 @@ struct Plan {
   uint32_t bcap, cap, maxb, nbk, waves;
   size_t stride;
   size_t o_rec, o_pm, o_acc, o_ods, o_bkt, o_bnd, o_lvl;
+  size_t o_st = 0;
+  uint32_t st_cap = 0;  // winners per source for land-use statistics (0: none)
 };
 
 // Per-wave scratch layout. Everything here is written before it is read
@@ -1573,6 +1775,9 @@ void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   o = align256(o + 4ull * (p.maxb + 2));
   p.o_lvl = o;
   o = align256(o + 4ull * (p.maxb + 2));
+  p.o_st = o;
+  // winner count | winners, 2 cap | sort keys, 2 cap | values | weights (sorts pad to a power of two)
+  if (p.st_cap) o = align256(o + 256 + 16ull * p.st_cap + 16ull * p.st_cap + 8ull * p.st_cap);
   p.stride = o;
 }
 
@@ -1827,7 +2032,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
   p.maxb = 1;
   for (auto &js : specs) {
     jobs.push_back(make_job(js, NT));
-    maxN = std::max(maxN, js.g->n);
+    maxN = std::max(maxN, js.g->n + js.extra_records);
     p.nbk = std::max(p.nbk, jobs.back().nbk);
     p.maxb = std::max(p.maxb, jobs.back().maxb);
   }
@@ -1835,6 +2040,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
   const uint32_t wpb = block / kWave;
   p.bcap = 1024;
   p.cap = std::min<uint32_t>({maxN, tune.cap ? tune.cap : 16384u, kSlotLimit});
+  if (aout && aout->stats) p.st_cap = std::max<uint32_t>(1u, std::min<uint32_t>(aout->n_groups, 4096u));
   const size_t lds = (size_t)wpb * (p.nbk + kWaveLds) * 4;
   const int bpc = std::max(1, blocks_per_cu<MODE>(NT, block, lds));
   uint32_t resident = di.cus * (uint32_t)bpc * wpb;
@@ -1914,6 +2120,8 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.o_lvl = p.o_lvl;
+    a.o_st = p.o_st;
+    a.st_cap = p.st_cap;
     a.epochs = wsp->epochs;
     a.next = d_ctl;
     a.xsplit = tune.xcd_split ? 1u : 0u;
@@ -1954,7 +2162,10 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     n_list = n_failed;
     if ((flags & kFailCap) && p.cap >= kSlotLimit)
       fail(CS_EINVAL, "a source reaches more than 4,194,304 nodes, the search's slot limit");
-    if (flags & kFailCap) p.cap = std::min<uint32_t>({maxN, p.cap * 4, kSlotLimit});
+    if (flags & kFailCap) {
+      p.cap = std::min<uint32_t>({maxN, p.cap * 4, kSlotLimit});
+      if (p.st_cap) p.st_cap = std::min<uint32_t>(aout->n_groups, p.st_cap * 4);
+    }
     if (flags & kFailBucket) p.bcap *= 4;
     if (attempt == 5) fail(CS_EINTERNAL, "search workspace retries exhausted");
   }
@@ -2548,6 +2759,7 @@ void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uin
   cs.betweenness = false;
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, nullptr, nullptr, tune.delta_scale};
+  js.extra_records = out.n_shared;  // one 16-byte record (two tagged words) per shared group
   run_jobs<kAccess>({js}, 1, tune, nullptr, stream, st, nullptr, &out, progress);
 }
 

@@ -270,6 +270,22 @@ int cs_mixed_uses(cs_graph *g, uint32_t n_entries, const uint32_t *entry_group, 
                   const double *assign_dist, uint32_t n_thresholds, const uint32_t *distances, const float *betas,
                   const float *max_curve_wts, uint32_t max_seconds, float speed_m_s, double *out);
 
+/* Replaces DataMap.stats (rust/src/data.rs:1013-1233) with angular = false:
+ * the entries, dedupe groups and node assignment of cs_accessibility, and
+ * n_maps numeric maps as values f32 [n_maps][n_entries] (NaN: the entry has no
+ * value). out: f64 [n_maps][14][n_thresholds][N] = sum, sum_wt, mean,
+ * mean_wt, median, median_wt, count, count_wt, variance, variance_wt, mad,
+ * mad_wt, max, min per live source, over the entries it reaches within each
+ * distance (nodes that are not live keep 0 for sums and counts, NaN for the
+ * rest, as StatsResult::new starts them). Sums run in f32 in ascending entry
+ * order (the reference's order is its hash map's, which varies from run to
+ * run); medians follow the reference's total order and stable sort. */
+int cs_data_stats(cs_graph *g, uint32_t n_entries, const uint32_t *entry_group, uint32_t n_assign,
+                  const uint32_t *assign_node, const uint32_t *assign_entry, const double *assign_dist,
+                  uint32_t n_maps, const float *values, uint32_t n_thresholds, const uint32_t *distances,
+                  const float *betas, const float *max_curve_wts, uint32_t max_seconds, float speed_m_s,
+                  double *out);
+
 /* ---------------------------------------------------------------------- */
 /* Single-source trees                                                    */
 /* ---------------------------------------------------------------------- */

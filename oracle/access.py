@@ -183,3 +183,108 @@ def mixed_uses(og, live, node_data_map, dedupe_of, lu_class, distances, betas, m
             vals += [shannon(cs), gini(cs)]
             out[:, i, src] = [float(v) for v in vals]
     return out
+
+
+# ---- statistics (rust/src/data.rs:946-1233), f32 throughout
+
+def _f32_exp(x):
+    """The reference's f32::exp (the host libm's expf, correctly rounded but for a sliver of
+    inputs): the f64 exponential rounded to f32."""
+    import math
+
+    return f32(math.exp(float(x)))
+
+
+def _total_key(x):
+    """f32::total_cmp as an ascending integer key."""
+    b = int(np.array([x], dtype=np.float32).view(np.uint32)[0])
+    return (~b & 0xFFFFFFFF) if b & 0x80000000 else (b | 0x80000000)
+
+
+def median(vals):
+    """data.rs:946-959."""
+    n = len(vals)
+    if n == 0:
+        return f32(np.nan)
+    s = sorted(vals, key=_total_key)
+    if n % 2 == 1:
+        return s[n // 2]
+    return f32(f32(s[n // 2 - 1] + s[n // 2]) / f32(2.0))
+
+
+def weighted_median(pairs, total_wt):
+    """data.rs:961-997 (Python's sort is stable, as Rust's sort_by)."""
+    if not pairs:
+        return f32(np.nan)
+    s = sorted(pairs, key=lambda p: _total_key(p[0]))
+    if total_wt == f32(0.0):
+        return f32(np.nan)
+    mid = f32(total_wt / f32(2.0))
+    for v, w in s:
+        if w > mid:
+            return v
+    agg = f32(0.0)
+    for i, (v, w) in enumerate(s):
+        agg = f32(agg + w)
+        if agg == mid:
+            return f32(f32(v + s[i + 1][0]) / f32(2.0)) if i + 1 < len(s) else v
+        if agg > mid:
+            return v
+    return s[-1][0]
+
+
+STAT_NAMES = ("sum", "sum_wt", "mean", "mean_wt", "median", "median_wt", "count", "count_wt", "variance",
+              "variance_wt", "mad", "mad_wt", "max", "min")
+
+
+def stats(og, live, node_data_map, dedupe_of, entry_order, num_maps, distances, betas, mcws, max_walk_seconds,
+          speed, sources=None):
+    """f64 [maps][14][T][N] in STAT_NAMES order (data.rs:1073-1227). Each source's reachable entries
+    are taken in ascending `entry_order` (the reference walks its hash map, whose order varies)."""
+    n, T = og.n, len(distances)
+    out = np.full((len(num_maps), 14, T, n), np.nan)
+    out[:, [0, 1, 6, 7]] = 0.0
+    for src in (range(n) if sources is None else sources):
+        if not live[src]:
+            continue
+        tree = og.tree_shortest(src, max_walk_seconds, speed)
+        reach = aggregate_to_src_idx(tree, node_data_map, dedupe_of, max_walk_seconds, speed)
+        items = sorted(reach.items(), key=lambda kv: entry_order[kv[0]])
+        for mp, num_map in enumerate(num_maps):
+            for i, (d, b, mcw) in enumerate(zip(distances, betas, mcws, strict=True)):
+                vals, pairs = [], []
+                s = s_wt = cnt = cnt_wt = sq = sq_wt = f32(0.0)
+                mn = mx = f32(np.nan)
+                for key, dist in items:
+                    if not dist <= f32(d):
+                        continue
+                    num = num_map.get(key)
+                    if num is None or np.isnan(num):
+                        continue
+                    num = f32(num)
+                    raw = _f32_exp(f32(-f32(b)) * dist)
+                    wt = f32(min(raw, f32(mcw)) / f32(mcw))
+                    s = f32(s + num)
+                    s_wt = f32(s_wt + f32(num * wt))
+                    cnt = f32(cnt + f32(1.0))
+                    cnt_wt = f32(cnt_wt + wt)
+                    sq = f32(sq + f32(num * num))
+                    sq_wt = f32(sq_wt + f32(f32(wt * num) * num))
+                    mx = num if np.isnan(mx) else max(mx, num)
+                    mn = num if np.isnan(mn) else min(mn, num)
+                    vals.append(num)
+                    pairs.append((num, wt))
+                mean = f32(s / cnt) if cnt > 0 else f32(np.nan)
+                mean_wt = f32(s_wt / cnt_wt) if cnt_wt > 0 else f32(np.nan)
+                var = max(f32(f32(sq / cnt) - f32(mean * mean)), f32(0.0)) if cnt > 0 else f32(np.nan)
+                var_wt = (max(f32(f32(sq_wt / cnt_wt) - f32(mean_wt * mean_wt)), f32(0.0)) if cnt_wt > 0
+                          else f32(np.nan))
+                med = median(vals)
+                med_wt = weighted_median(pairs, cnt_wt)
+                mad = median([f32(abs(f32(v - med))) for v in vals]) if vals and not np.isnan(med) else f32(np.nan)
+                mad_wt = (weighted_median([(f32(abs(f32(v - med_wt))), w) for v, w in pairs], cnt_wt)
+                          if pairs and not np.isnan(med_wt) and cnt_wt > 0 else f32(np.nan))
+                row = (s, s_wt, mean, mean_wt, med, med_wt, cnt, cnt_wt, var, var_wt, mad, mad_wt, mx, mn)
+                for m, v in enumerate(row):
+                    out[mp, m, i, src] = float(v)
+    return out

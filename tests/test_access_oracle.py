@@ -155,3 +155,72 @@ def test_oracle_mixed_uses_line_graph():
     assert math.isclose(out[7, 1, 0], 1.0 - (2 * 1) / (4 * 3), rel_tol=1e-6)
     # at 100 only F (40) counts: one class
     assert out[0, 0, 0] == 1.0 and out[6, 0, 0] == 0.0 and out[7, 0, 0] == 0.0
+
+
+# ---------------------------------------------------------------- statistics (data.rs:946-1233)
+def test_median_and_weighted_median_closed_forms():
+    from oracle.access import median, weighted_median
+
+    f = np.float32
+    assert median([]) != median([])  # NaN
+    assert median([f(3), f(1), f(2)]) == f(2)
+    assert median([f(4), f(1), f(3), f(2)]) == f(2.5)
+    assert median([f(-0.0), f(0.0)]) == f(0.0)
+    # a weight above half the total wins outright
+    assert weighted_median([(f(1), f(0.2)), (f(5), f(0.7)), (f(9), f(0.1))], f(1.0)) == f(5)
+    # cumulative weight lands exactly on the midpoint: mean of that value and the next
+    assert weighted_median([(f(1), f(0.25)), (f(2), f(0.25)), (f(3), f(0.25)), (f(4), f(0.25))], f(1.0)) == f(2.5)
+    # passes the midpoint inside a value
+    assert weighted_median([(f(1), f(0.3)), (f(2), f(0.3)), (f(3), f(0.4))], f(1.0)) == f(2)
+    assert weighted_median([(f(1), f(0.0))], f(0.0)) != weighted_median([(f(1), f(0.0))], f(0.0))
+
+
+def _stats_line(values, dists, speed=1.0, spatial_tolerance=0):
+    from oracle.access import stats as oracle_stats
+
+    ns = line_graph()
+    og = OracleGraph.from_network_structure(ns)
+    ndm, dedupe_of, _lu = line_inputs()
+    _d, bets, secs = O.thresholds(speed, distances=dists)
+    mcw = data.clip_wts_curve(dists, bets, spatial_tolerance)
+    order = {k: i for i, (k, _c, _dd, _a) in enumerate(LINE_ENTRIES)}
+    return oracle_stats(og, [1] * og.n, ndm, dedupe_of, order, [values], dists, bets, mcw, max(secs), speed)[0]
+
+
+def test_stats_closed_forms_on_the_line():
+    """From node 0 at 250 m (1 m/s): A 110, D 130, C 200, F 40 (E loses group g to F); from node 3:
+    B 5, D 2, C 100, E 120, A 210. Values A=1, B=2, C=3, D=4, E=5, F=NaN; a tolerance equal to the
+    largest distance clips every weight to 1, so the weighted forms equal the plain ones."""
+    vals = {"A": 1.0, "B": 2.0, "C": 3.0, "D": 4.0, "E": 5.0, "F": float("nan")}
+    st = _stats_line(vals, [250], spatial_tolerance=250)
+    names = ("sum", "sum_wt", "mean", "mean_wt", "median", "median_wt", "count", "count_wt", "variance",
+             "variance_wt", "mad", "mad_wt", "max", "min")
+    # node 0: values {A: 1, C: 3, D: 4} (F has no value)
+    want0 = dict(sum=8, count=3, mean=8 / 3, variance=14 / 9, median=3.0, mad=1.0, max=4, min=1)
+    # node 3: values {B: 2, D: 4, C: 3, E: 5, A: 1}
+    want3 = dict(sum=15, count=5, mean=3.0, variance=2.0, median=3.0, mad=1.0, max=5, min=1)
+    for src, want in ((0, want0), (3, want3)):
+        for k, v in want.items():
+            assert np.isclose(st[names.index(k), 0, src], v), (src, k)
+            if k in ("sum", "count", "mean", "variance", "median", "mad"):
+                assert np.isclose(st[names.index(k + "_wt"), 0, src], v), (src, k + "_wt")
+
+
+def test_stats_against_numpy_and_the_weight_curve():
+    rng = np.random.default_rng(7)
+    vals = {k: float(rng.uniform(-10, 10)) for k, _c, _d, _a in LINE_ENTRIES}
+    st = _stats_line(vals, [150, 250])
+    # node 3 at 250 m reaches B, D, C, E, A with totals 5, 2, 100, 120, 210
+    v = np.array([vals["B"], vals["D"], vals["C"], vals["E"], vals["A"]], np.float32)
+    assert np.isclose(st[0, 1, 3], v.sum(), rtol=1e-6)
+    assert np.isclose(st[4, 1, 3], np.median(v), rtol=1e-6)
+    assert np.isclose(st[8, 1, 3], v.var(), rtol=1e-4, atol=1e-5)
+    assert np.isclose(st[10, 1, 3], np.median(np.abs(v - np.median(v))), rtol=1e-6)
+    # weights: exp(-beta * total) at beta = 0.016 (250 m), none clipped
+    w = np.exp(-0.016 * np.array([5.0, 2.0, 100.0, 120.0, 210.0]))
+    assert np.isclose(st[1, 1, 3], (v * w).sum(), rtol=1e-5)
+    assert np.isclose(st[7, 1, 3], w.sum(), rtol=1e-5)
+    assert np.isclose(st[3, 1, 3], (v * w).sum() / w.sum(), rtol=1e-5)
+    # at 150 m node 3 reaches B, D, C, E (A at 210 m lies beyond)
+    assert st[6, 0, 3] == 4.0
+    assert np.isclose(st[12, 0, 3], max(vals["B"], vals["D"], vals["C"], vals["E"]))

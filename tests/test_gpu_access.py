@@ -155,3 +155,27 @@ def test_mixed_uses_line_graph_and_lattice():
     assert np.all(only.hill[1][600] == 0.0) and np.array_equal(only.hill_weighted[1][600], res.hill_weighted[1][600])
     with pytest.raises(ValueError, match="compute_"):
         dm.mixed_uses(ns, lu, distances=[200], compute_hill=False, compute_hill_weighted=False)
+
+
+def test_large_dedupe_groups():
+    """Dedupe groups of up to 2,000 members spread over the lattice (polygon-like features): one
+    winner per group, through its nearest member, as the oracle's dictionary walk picks it."""
+    lat = mock.perturbed_lattice(40, seed=207, diag_frac=0.05)
+    ns = mock.network_structure_from_lattice(lat)
+    rng = np.random.default_rng(208)
+    n = ns.node_count()
+    entries = []
+    # one group of 2,000 single-node entries, two of 300 multi-node entries, the rest singletons
+    for e in range(2000):
+        entries.append((f"big{e}", ("shop", "cafe")[e % 2], "poly0",
+                        [(int(rng.integers(0, n)), float(rng.uniform(0.0, 40.0)))]))
+    for grp in (1, 2):
+        for e in range(300):
+            nodes = rng.choice(n, 3, replace=False).tolist()
+            entries.append((f"g{grp}_{e}", "park", f"poly{grp}", [(int(v), float(rng.uniform(0.0, 40.0))) for v in nodes]))
+    entries += _random_case(ns, 400, 209)
+    say(f"accessibility, {len(entries)} entries in groups of up to 2,000 members")
+    res = _compare(ns, entries, ["shop", "cafe", "park"], [300, 700], 1.33333)
+    # each group counts at most once per source
+    assert res.result["park"].unweighted[700].max() <= 2 + 400
+    assert (res.result["shop"].unweighted[700] + res.result["cafe"].unweighted[700]).max() >= 1
