@@ -212,3 +212,52 @@ def test_halve_and_dual_geometry(diamond_graph):
         shared = d["primal_node_id"]
         px, py = diamond_graph.nodes[shared]["x"], diamond_graph.nodes[shared]["y"]
         assert any(math.isclose(c[0], px) and math.isclose(c[1], py) for c in d["geom"].coords)
+
+
+# ---------------------------------------------------------------- network_structure_from_gpd
+def _same_structure(a, b):
+    x, y = a.arrays(), b.arrays()
+    for k in ("x", "y", "live", "weight", "src", "dst", "key", "length", "angle", "imp", "pk"):
+        assert np.array_equal(x[k], y[k], equal_nan=True), k
+    assert a.node_keys_py() == b.node_keys_py()
+
+
+@pytest.mark.parametrize("dual", [False, True])
+def test_network_structure_from_gpd_round_trip(primal_graph, dual, tmp_path):
+    from cityseer.tools import graphs, io
+
+    g = graphs.nx_to_dual(primal_graph) if dual else primal_graph
+    nodes, edges, ns = io.network_structure_from_nx(g)
+    again = io.network_structure_from_gpd(nodes, edges)
+    _same_structure(ns, again)
+    # through CSV files, geometries as WKT text (io.py:1213-1321 serves reloaded frames)
+    edges_csv = edges.assign(geom=[geom.wkt for geom in edges["geom"]])
+    nodes.to_csv(tmp_path / "n.csv")
+    edges_csv.to_csv(tmp_path / "e.csv")
+    import pandas as pd
+
+    n2 = pd.read_csv(tmp_path / "n.csv", index_col=0)
+    n2.index = n2.index.astype(str)
+    e2 = pd.read_csv(tmp_path / "e.csv", index_col=0, dtype={"nx_start_node_key": str, "nx_end_node_key": str})
+    _same_structure(ns, io.network_structure_from_gpd(n2, e2))
+
+
+def test_network_structure_from_gpd_checks(primal_graph, caplog):
+    from cityseer.tools import io
+
+    nodes, edges, ns = io.network_structure_from_nx(primal_graph)
+    with pytest.raises(ValueError, match="nodes GDF: weight"):
+        io.network_structure_from_gpd(nodes.drop(columns=["weight"]), edges)
+    with pytest.raises(ValueError, match="edges GDF: imp_factor"):
+        io.network_structure_from_gpd(nodes, edges.drop(columns=["imp_factor"]))
+    # edges naming a node the frame lacks are skipped
+    gone = nodes.index[0]
+    import logging
+
+    with caplog.at_level(logging.INFO):
+        part = io.network_structure_from_gpd(nodes, edges)
+        fewer = io.network_structure_from_gpd(nodes.drop(index=[gone]), edges)
+    assert part.edge_count == ns.edge_count
+    dropped = int(((edges["nx_start_node_key"] == gone) | (edges["nx_end_node_key"] == gone)).sum())
+    assert dropped > 0 and fewer.edge_count == ns.edge_count - dropped
+    assert "node nx key not found" in caplog.text
