@@ -172,6 +172,7 @@ class DataMap:
         self.entries: dict[str, DataEntry] = {}
         self.node_data_map: dict[int, list[tuple[str, float]]] = {}
         self._progress = 0
+        self._node_bound: int | None = None  # of the network the current call runs on
 
     def insert(self, data_key_py: Any, geom_wkt: str, dedupe_key_py: Any = None) -> None:
         key = _composite_key(data_key_py)
@@ -226,6 +227,7 @@ class DataMap:
             raise NotImplementedError("angular stats are not available here")
         speed = WALKING_SPEED if speed_m_s is None else float(speed_m_s)
         dists, bets, secs = pair_distances_betas_time(speed, distances, betas, minutes, min_threshold_wt)
+        self._node_bound = network_structure.node_bound()
         keys, entry_group, a_node, a_ent, a_dist = self._assignment_arrays()
         index = {k: i for i, k in enumerate(keys)}
         values = np.full((len(numerical_maps), len(keys)), np.nan, dtype=np.float32)
@@ -265,7 +267,11 @@ class DataMap:
         for i, k in enumerate(keys):
             entry_group[i] = groups.setdefault(self.entries[k].dedupe_key, len(groups))
         nodes, ents, dists = [], [], []
+        n_nodes = self._node_bound
         for node_idx, pairs in self.node_data_map.items():
+            # the reference only looks up nodes a search reaches (data.rs:555-559): others are ignored
+            if n_nodes is not None and not 0 <= int(node_idx) < n_nodes:
+                continue
             for data_key, dist in pairs:
                 e = index.get(data_key)
                 if e is None:  # data.rs:563-566: unknown keys are skipped
@@ -282,9 +288,11 @@ class DataMap:
         lu: dict[str, str] = {}
         for k, v in landuses_map.items():
             ck = _composite_key(k)
+            if not isinstance(v, str):  # the reference extracts a String first (data.rs:650-660)
+                raise TypeError(f"land-use values must be strings, got {type(v).__name__} for {ck}")
             if ck not in self.entries:
                 raise KeyError(f"Data entries key missing: {ck}")
-            lu[ck] = str(v)
+            lu[ck] = v
         return lu
 
     def mixed_uses(
@@ -321,6 +329,7 @@ class DataMap:
         mcw = clip_wts_curve(dists, bets, int(spatial_tolerance or 0))
         classes = sorted(set(lu.values()))
         cls_index = {k: i for i, k in enumerate(classes)}
+        self._node_bound = network_structure.node_bound()
         keys, entry_group, a_node, a_ent, a_dist = self._assignment_arrays()
         entry_class = np.asarray([cls_index[lu[k]] if k in lu else -1 for k in keys], dtype=np.int32)
         n, T = network_structure.node_bound(), len(dists)
@@ -370,6 +379,7 @@ class DataMap:
         mcw = clip_wts_curve(dists, bets, int(spatial_tolerance or 0))
         acc_keys = list(accessibility_keys)
         cls_index = {k: i for i, k in enumerate(acc_keys)}
+        self._node_bound = network_structure.node_bound()
         keys, entry_group, a_node, a_ent, a_dist = self._assignment_arrays()
         entry_class = np.asarray([cls_index.get(lu.get(k, ""), -1) for k in keys], dtype=np.int32)
         n = network_structure.node_bound()

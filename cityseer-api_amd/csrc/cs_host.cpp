@@ -1339,7 +1339,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
         if (distances[t0 + t] == dmax) c.max_mask |= 1u << t;
       }
       c.count = ao.count + (size_t)t0 * N;
-      c.weighted = ao.weighted + (size_t)t0 * N;
+      c.weighted = ao.weighted ? ao.weighted + (size_t)t0 * N : nullptr;
       if (ao.stats) c.stats = ao.stats + (size_t)t0 * N;
       return c;
     };
@@ -1350,7 +1350,8 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     OwnedStream stream;
     DevBuf d_node_off(4ull * (N + 1)), d_node_mem(4ull * n_assign), d_mem(16ull * n_assign),
         d_class(4ull * std::max<uint32_t>(n_entries, 1)),
-        d_src(4ull * std::max<size_t>(srcs.size(), 1)), d_count(8 * CTN), d_wt(8 * CTN), d_near(8ull * C * N);
+        d_src(4ull * std::max<size_t>(srcs.size(), 1)), d_count(8 * CTN), d_wt(out_div ? 8 : 8 * CTN),
+        d_near(8ull * C * N);
     auto up = [&](DevBuf &d, const void *hsrc, size_t bytes) {
       if (bytes) hip_check(hipMemcpyAsync(d.p, hsrc, bytes, hipMemcpyHostToDevice, stream), "copy");
     };
@@ -1364,7 +1365,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
       up(d_near, nan_rows.data(), 8ull * C * N);
     }
     hip_check(hipMemsetAsync(d_count.p, 0, 8 * CTN, stream), "memset");
-    hip_check(hipMemsetAsync(d_wt.p, 0, 8 * CTN, stream), "memset");
+    if (!out_div) hip_check(hipMemsetAsync(d_wt.p, 0, 8 * CTN, stream), "memset");
     ao.node_off = d_node_off.as<uint32_t>();
     ao.node_mem = d_node_mem.as<uint32_t>();
     ao.mem = d_mem.as<uint4>();
@@ -1382,7 +1383,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     }
     ao.entry_class = d_class.as<int32_t>();
     ao.count = d_count.as<double>();
-    ao.weighted = d_wt.as<double>();
+    ao.weighted = out_div ? nullptr : d_wt.as<double>();
     ao.nearest = d_near.as<unsigned long long>();
     uint32_t *prog = progress_begin(g, N - srcs.size());
     if (!srcs.empty())

@@ -1617,9 +1617,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
               if (!(tot <= ao.dist[t])) continue;
               const size_t row = ((size_t)cls * ao.tstride + t) * ao.N + src;
               gadd_f64(&ao.count[row], 1.0);
-              // clipped_beta_wt (common.rs:339-348): min(exp(-beta d), mcw) / mcw in f32
-              const float raw = exp_f32(__fmul_rn(-ao.beta[t], tot));
-              gadd_f64(&ao.weighted[row], (double)__fdiv_rn(fminf(raw, ao.mcw[t]), ao.mcw[t]));
+              if (ao.weighted) {  // mixed uses read only the counts
+                // clipped_beta_wt (common.rs:339-348): min(exp(-beta d), mcw) / mcw in f32
+                const float raw = exp_f32(__fmul_rn(-ao.beta[t], tot));
+                gadd_f64(&ao.weighted[row], (double)__fdiv_rn(fminf(raw, ao.mcw[t]), ao.mcw[t]));
+              }
               if (ao.max_mask & (1u << t))
                 atomicMin(&ao.nearest[(size_t)cls * ao.N + src], (unsigned long long)__double_as_longlong((double)tot));
             }

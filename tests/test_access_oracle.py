@@ -224,3 +224,19 @@ def test_stats_against_numpy_and_the_weight_curve():
     # at 150 m node 3 reaches B, D, C, E (A at 210 m lies beyond)
     assert st[6, 0, 3] == 4.0
     assert np.isclose(st[12, 0, 3], max(vals["B"], vals["D"], vals["C"], vals["E"]))
+
+
+def test_datamap_boundary_follows_the_reference():
+    """Assignments to nodes outside the network are never looked up by the reference's searches
+    (data.rs:555-559), so they are ignored; land-use values must be strings (data.rs:650-660)."""
+    dm = data.DataMap()
+    dm.insert("a", "POINT (0 0)")
+    dm.insert("b", "POINT (0 0)")
+    dm.node_data_map = {0: [("str:a", 1.0)], 99: [("str:b", 2.0)], -1: [("str:b", 3.0)]}
+    dm._node_bound = 10
+    _keys, _grp, nodes, ents, _dists = dm._assignment_arrays()
+    assert nodes.tolist() == [0] and ents.tolist() == [0]
+    with pytest.raises(TypeError, match="strings"):
+        dm._landuses({"a": 1, "b": "shop"})
+    with pytest.raises(KeyError):
+        dm._landuses({"a": "shop", "c": "shop"})
