@@ -623,7 +623,7 @@ def serve_line(args) -> dict:
     passes.sort(key=lambda x: x[0])
     plain_s, service = passes[len(passes) // 2]
 
-    with CentralityServer() as srv:
+    with CentralityServer(slice_ms=args.slice_ms) as srv:
         sats = []
         for rep in range(args.serve_passes):
             log(f"served: all requests at once (saturation, pass {rep + 1}/{args.serve_passes})")
@@ -730,6 +730,8 @@ def main() -> int:
     ap.add_argument("--tune-fraction", type=float, default=0.03, help="batch mode: autotuner sample fraction")
     ap.add_argument("--serve-passes", type=int, default=3, help="serve mode: timed passes of each side")
     ap.add_argument("--serve-max-nodes", type=float, default=1e6, help="serve mode: largest graph")
+    ap.add_argument("--slice-ms", type=float, default=0.0,
+                    help="serve mode: target duration of one batched launch (0: the server's default)")
     ap.add_argument("--no-legs", action="store_true",
                     help="plain mode: skip the batched (64 graphs) and served legs nested in the line")
     ap.add_argument("--leg-steps", type=int, default=3, help="plain mode: timed passes of the nested legs")
@@ -874,7 +876,7 @@ def main() -> int:
         legs.steps = args.leg_steps
         legs.warmup = 1
         legs.tune_fraction = 0.01
-        legs.serve_passes = 1
+        legs.serve_passes = 3
         log("leg: batch (64 graphs)")
         line["batch"] = leg_summary(batch_line(legs), "batched_over_plain")
         line["batch"]["batched_over_plain_cfg2"] = round(line["batch"]["value"] / value, 3)

@@ -193,6 +193,7 @@ _SIGNATURES = {
     "cs_last_stats": (ctypes.c_int, [ctypes.POINTER(CsStats)]),
     "cs_set_tuning": (ctypes.c_int, [ctypes.POINTER(CsTuning)]),
     "cs_get_tuning": (ctypes.c_int, [ctypes.POINTER(CsTuning)]),
+    "cs_release_workspaces": (ctypes.c_int, []),
     "cs_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "cs_batch_destroy": (None, [ctypes.c_void_p]),
     "cs_batch_add_shortest": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CsParams), _f64p, _u32p]),

@@ -1103,4 +1103,17 @@ void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, fl
   if (hctl[2]) fail(CS_EINTERNAL, "angular tree workspace exceeded");
 }
 
+void release_angular_workspaces() {
+  std::lock_guard<std::mutex> lk(g_ang_mu);
+  for (auto &dv : g_ang) {
+    AngWorkspace &w = *dv.second;
+    std::lock_guard<std::mutex> wl(w.mu);
+    if (w.lane_ws) (void)hipFree(w.lane_ws);
+    if (w.wave_ws) (void)hipFree(w.wave_ws);
+    w.lane_ws = w.wave_ws = nullptr;
+    w.lane_bytes = w.wave_bytes = 0;
+    w.epoch_hi = 0;
+  }
+}
+
 }  // namespace cs

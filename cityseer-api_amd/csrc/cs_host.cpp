@@ -1668,6 +1668,13 @@ int cs_set_tuning(const cs_tuning *t) {
   return guarded([&] { set_default_tuning(from_c(t)); });
 }
 
+int cs_release_workspaces(void) {
+  return guarded([] {
+    release_search_workspaces();
+    release_angular_workspaces();
+  });
+}
+
 int cs_get_tuning(cs_tuning *t) {
   return guarded([&] {
     if (!t) fail(CS_EINVAL, "tuning must not be NULL");

@@ -113,8 +113,14 @@ constexpr uint32_t kProgressSlots = 65536;
 // clears the counts, counts `settled_on_host` sources at once and returns the
 // device address of the counts for the launches to pass on.
 uint32_t *progress_begin(Graph &g, uint64_t settled_on_host);
-// Sets the count outright (calls that report only on completion).
-void progress_set(Graph &g, uint64_t n);
+// Ends a call's progress at n sources: whatever the kernels have not counted
+// is added on the host (calls that report only on completion count it all).
+void progress_finish(Graph &g, uint64_t n);
+
+// Frees the pooled search workspaces of every device, waiting for the calls
+// that hold them; the next call allocates and zeroes fresh ones.
+void release_search_workspaces();
+void release_angular_workspaces();
 
 // Host edge travel seconds (centrality.rs:970-1007).
 float edge_seconds(const Graph &g, uint32_t from, uint32_t to, uint32_t e, float speed,

@@ -1567,15 +1567,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
       // below the walk limit, secs_v * speed + dd is within the largest
       // distance and no other member of its dedupe group is nearer (equal
       // totals: the lower member index wins). Groups of several members keep,
-      // per wave, two epoch-tagged records after the node records: the nearest
+      // per wave, two epoch-tagged words after the node records: the nearest
       // total (pass 1), then the lowest member index at that total (pass 2), so
       // each group counts once, through its nearest member, in passes linear in
       // the members reached.
       const AccessOut &ao = a.acc;
       const uint32_t src = c.src;
+      // group g owns the key words of records nacc + 2g and nacc + 2g + 1. Only key
+      // words carry an epoch (payload words hold sigmas and lengths of earlier
+      // searches), so whatever earlier sources left in them, node keys of a larger
+      // graph included, compares below the current tag
       uint64_t *grec = w.node + 2 * (size_t)j.nacc;
-      // the epoch sits where node keys hold theirs, so that records left in these
-      // words by earlier sources (node keys of a larger graph included) are older
       const uint64_t tag = (uint64_t)c.tag << 54;
       // statistics: the winners go to a list in the wave's scratch instead of the counts
       char *stb = a.ws + (size_t)wid * a.ws_stride + a.o_st;
@@ -1598,14 +1600,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
             const bool shared = me.w != kSoloGroup;
             const uint64_t tkey = tag | ((uint64_t)(uint32_t)~__float_as_uint(tot) << 22);  // larger = nearer
             if (pass == 0) {
-              if (shared) wmax64(&grec[2 * (size_t)me.w], tkey);
+              if (shared) wmax64(&grec[4 * (size_t)me.w], tkey);
               continue;
             }
             if (pass == 1) {
-              if (shared && ld64(&grec[2 * (size_t)me.w]) == tkey) wmax64(&grec[2 * (size_t)me.w + 1], tag | (uint32_t)~m);
+              if (shared && ld64(&grec[4 * (size_t)me.w]) == tkey) wmax64(&grec[4 * (size_t)me.w + 2], tag | (uint32_t)~m);
               continue;
             }
-            if (shared && ld64(&grec[2 * (size_t)me.w + 1]) != (tag | (uint32_t)~m)) continue;
+            if (shared && ld64(&grec[4 * (size_t)me.w + 2]) != (tag | (uint32_t)~m)) continue;
             if (ao.stats) {
               const uint32_t q = __hip_atomic_fetch_add(st_n, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
               if (q < a.st_cap) st64(&st_win[q], ((uint64_t)me.y << 32) | __float_as_uint(tot));
@@ -1844,7 +1846,8 @@ struct Workspace {
   int device = -1;
   // node records {epoch-tagged key, payload}, [key_waves][ncap][2]: zeroed
   // whenever (re)allocated; each wave clears its own when its 10-bit epochs
-  // wrap, and records of earlier graphs carry older epochs
+  // wrap, and key words left by earlier graphs carry older epochs (payload
+  // words carry none)
   uint64_t *keys = nullptr;
   uint32_t ncap = 0, key_waves = 0;
   uint32_t *epochs = nullptr;
@@ -2427,6 +2430,15 @@ struct ShortestBatch {
   }
 };
 
+void release_search_workspaces() {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  for (auto &dv : g_ws_pool)
+    for (auto &w : dv.second) {
+      std::lock_guard<std::mutex> wl(w->mu);  // a running call holds it until its launch has ended
+      w->release();
+    }
+}
+
 void set_default_tuning(const Tuning &t) {
   std::lock_guard<std::mutex> lk(g_tuning_mu);
   g_tuning = t;
@@ -2751,6 +2763,8 @@ void launch_segment(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
 void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uint32_t *d_sources,
                    uint32_t n_sources, const AccessOut &out, hipStream_t stream, cs_stats &st,
                    uint32_t *progress) {
+  if ((uint64_t)g.n + 2ull * out.n_shared > 0x7fffffffull)
+    fail(CS_EINVAL, "too many nodes and dedupe groups for the per-wave records");
   CallSpec cs;
   cs.T = 1;
   cs.dist[0] = maxsec * de.speed;
@@ -2761,7 +2775,7 @@ void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uin
   cs.betweenness = false;
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, nullptr, nullptr, tune.delta_scale};
-  js.extra_records = out.n_shared;  // one 16-byte record (two tagged words) per shared group
+  js.extra_records = 2 * out.n_shared;  // two 16-byte records per shared group, their key words tagged
   run_jobs<kAccess>({js}, 1, tune, nullptr, stream, st, nullptr, &out, progress);
 }
 

@@ -517,7 +517,7 @@ int cs_server_create(const cs_server_config *cfg, cs_server **out) {
       s->cfg.hbm_budget_bytes = free_b / 4;
     }
     if (s->cfg.max_active == 0) s->cfg.max_active = 64;
-    if (!(s->cfg.slice_ms > 0.0)) s->cfg.slice_ms = 20.0;
+    if (!(s->cfg.slice_ms > 0.0)) s->cfg.slice_ms = 100.0;
     cs_server *raw = s.get();
     raw->workers[0] = std::thread([raw] { raw->worker(CS_REQ_SHORTEST); });
     raw->workers[1] = std::thread([raw] { raw->worker(CS_REQ_SIMPLEST); });

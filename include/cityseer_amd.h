@@ -356,6 +356,12 @@ typedef struct cs_tuning {
 int cs_set_tuning(const cs_tuning *t);
 int cs_get_tuning(cs_tuning *t);
 
+/* No counterpart in the reference (its searches allocate per call). Frees the
+ * search workspaces the library pools per device between calls (node records,
+ * per-wave scratch, launch tables), waiting for calls that hold them; the next
+ * call allocates fresh, zeroed ones. For callers that want the HBM back. */
+int cs_release_workspaces(void);
+
 /* A batch of node_centrality_shortest jobs (any graphs, each with its own
  * thresholds and sources) run as one persistent launch captured in a hipGraph.
  * Graphs must outlive the batch. Not thread-safe: one thread drives a batch. */
@@ -394,7 +400,7 @@ typedef struct cs_server cs_server;
 typedef struct cs_server_config {
   uint64_t hbm_budget_bytes; /* device bytes admitted requests may hold; 0 = a quarter of free HBM */
   uint32_t max_active;       /* requests on the device at once; 0 = 64 */
-  double slice_ms;           /* target duration of one batched launch; 0 = 20 ms */
+  double slice_ms;           /* target duration of one batched launch; 0 = 100 ms */
 } cs_server_config;
 typedef struct cs_request_info {
   int32_t status;     /* 0 queued, 1 running, 2 done, or minus the error status */

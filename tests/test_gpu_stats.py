@@ -96,3 +96,27 @@ def test_large_dedupe_group():
     vals = {key: float(rng.uniform(0, 100)) for key, _c, _d, _a in entries}
     say(f"stats with a dedupe group of 2,000 members, {n} nodes")
     _compare(ns, entries, [vals], [300, 700], 1.33333)
+
+
+def test_groups_after_a_larger_graph():
+    """Dedupe groups keep per-wave records just past the node records of the graph in hand, in
+    words that a larger graph's searches used before. From freshly released workspaces, a
+    betweenness run on a larger lattice leaves path counts (doubles) in the payload words of those
+    records; statistics and accessibility on a smaller lattice, run next with the same waves, must
+    still count every group through its nearest member."""
+    from cityseer import _native
+
+    from test_gpu_access import _compare as access_compare
+
+    _native.check(_native.load().cs_release_workspaces())
+    big = mock.network_structure_from_lattice(mock.perturbed_lattice(40, seed=407, diag_frac=0.1))
+    small = mock.network_structure_from_lattice(mock.perturbed_lattice(22, seed=408, diag_frac=0.1))
+    entries = _random_case(small, 700, 409)
+    rng = np.random.default_rng(410)
+    vals = {key: float(rng.normal(5.0, 3.0)) for key, _c, _d, _a in entries}
+    for round_ in range(2):
+        say(f"betweenness on {big.node_count()} nodes, then stats and accessibility on {small.node_count()}")
+        big.centrality_shortest(distances=[4000], compute_closeness=False)
+        _compare(small, entries, [vals], [150, 400, 800], 1.33333)
+        big.centrality_shortest(distances=[4000], compute_closeness=False)
+        access_compare(small, entries, ["shop", "cafe", "park"], [200, 500, 900], 1.33333)
