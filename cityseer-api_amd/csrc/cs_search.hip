@@ -222,7 +222,9 @@ struct WaveWs {
                     // payload = sigma bits (Brandes) or origin-segment length bits (trees)
   uint64_t *rec;    // [cap][2]
   uint64_t *pm;     // [cap] predecessor row mask (Brandes) / {last len, origin len} (trees)
-  double *acc;      // [cap][2T] dependencies (Brandes) / [cap][T] subtree sums (segment)
+  double *acc;      // Brandes: [2T][cap] dependencies, plain then beta-weighted per threshold;
+                    // segment: [cap][T] subtree sums
+  uint32_t dcap;    // slots per dependency row (cap)
   double *ods;      // [cap] OD trip weight seeding each slot (OD-weighted betweenness)
   uint2 *bkt;       // [nbk][bcap] bucket ring entries {node, secs bits}
   uint32_t *bnd;    // [maxb + 2] slot bounds per opened bucket
@@ -232,6 +234,13 @@ struct WaveWs {
   uint32_t *rel;    // LDS [5][64] edge-parallel relaxation table of one batch
 };
 
+// Brandes dependency k of slot s: k = t (plain) or T + t (beta-weighted). Rows
+// are threshold-major: slots are in settle order, so the slots within a
+// threshold are a prefix of its rows, and the words a source touches are dense.
+__device__ __forceinline__ double *dep_at(const WaveWs &w, uint32_t k, uint32_t s) {
+  return &w.acc[(size_t)k * w.dcap + s];
+}
+
 __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uint32_t *ring) {
   char *base = a.ws + (size_t)wid * a.ws_stride;
   WaveWs w;
@@ -239,6 +248,7 @@ __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uin
   w.rec = reinterpret_cast<uint64_t *>(base + a.o_rec);
   w.pm = reinterpret_cast<uint64_t *>(base + a.o_pm);
   w.acc = reinterpret_cast<double *>(base + a.o_acc);
+  w.dcap = a.cap;
   w.ods = reinterpret_cast<double *>(base + a.o_ods);
   w.bkt = reinterpret_cast<uint2 *>(base + a.o_bkt);
   w.bnd = reinterpret_cast<uint32_t *>(base + a.o_bnd);
@@ -603,8 +613,8 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
 #pragma unroll
               for (int t = 0; t < NT; ++t) {
                 if (route_v <= j.dist[t]) {
-                  std_(&w.acc[(size_t)s * 2 * NT + t], 0.0);
-                  std_(&w.acc[(size_t)s * 2 * NT + NT + t], 0.0);
+                  std_(dep_at(w, t, s), 0.0);
+                  std_(dep_at(w, NT + t, s), 0.0);
                 }
               }
             }
@@ -1027,8 +1037,8 @@ __device__ __forceinline__ void flush_target(CJob &j, const WaveWs &w, uint32_t 
       if (route <= j.dist[t]) {
         // dependency = seed + delta; credit = dependency - seed (centrality.rs:848-866)
         const double seed_b = pc * exp(-(double)j.beta[t] * (double)route);
-        const double credit = (pc + ldd(&w.acc[(size_t)s * 2 * NT + t])) - pc;
-        const double credit_b = (seed_b + ldd(&w.acc[(size_t)s * 2 * NT + NT + t])) - seed_b;
+        const double credit = (pc + ldd(dep_at(w, t, s))) - pc;
+        const double credit_b = (seed_b + ldd(dep_at(w, NT + t, s))) - seed_b;
         if (credit > 0.0) gadd_f64(&accv[((size_t)5 * ts + t) * nacc], credit * (double)wt);
         if (credit_b > 0.0) gadd_f64(&accv[((size_t)6 * ts + t) * nacc], credit_b * (double)wt);
       }
@@ -1365,8 +1375,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
                 dep[t] = dep_b[t] = 0.0;
                 if (route <= j.dist[t]) {
                   const double seed_b = pc * exp(-(double)j.beta[t] * (double)route);
-                  dep[t] = pc + ldd(&w.acc[(size_t)s * 2 * NT + t]);
-                  dep_b[t] = seed_b + ldd(&w.acc[(size_t)s * 2 * NT + NT + t]);
+                  dep[t] = pc + ldd(dep_at(w, t, s));
+                  dep_b[t] = seed_b + ldd(dep_at(w, NT + t, s));
                   any = any || dep[t] != 0.0 || dep_b[t] != 0.0;
                 }
               }
@@ -1377,8 +1387,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
                   if (dep[t] != 0.0 || dep_b[t] != 0.0) {
-                    wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + t], dep[t]);
-                    wadd_f64(&w.acc[(size_t)pslot0 * 2 * NT + NT + t], dep_b[t]);
+                    wadd_f64(dep_at(w, t, pslot0), dep[t]);
+                    wadd_f64(dep_at(w, NT + t, pslot0), dep_b[t]);
                   }
                 }
               } else if (npred > 1) {
@@ -1398,8 +1408,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
 #pragma unroll
                   for (int t = 0; t < NT; ++t) {
                     if (dep[t] != 0.0 || dep_b[t] != 0.0) {
-                      wadd_f64(&w.acc[(size_t)su * 2 * NT + t], fct * dep[t]);
-                      wadd_f64(&w.acc[(size_t)su * 2 * NT + NT + t], fct * dep_b[t]);
+                      wadd_f64(dep_at(w, t, su), fct * dep[t]);
+                      wadd_f64(dep_at(w, NT + t, su), fct * dep_b[t]);
                     }
                   }
                 }
