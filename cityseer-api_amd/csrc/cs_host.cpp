@@ -335,6 +335,7 @@ DevEdges &device_edges(Graph &g, DevGraph &dg, float speed) {
       // out-edges count only one-way canonical edges (no twin among the in-edges)
       const bool count_out = g.canon[e] && tw < 0;
       fwd[j] = make_uint2(g.perm[u] | (count_out ? kCanonBit : 0u), bits);
+      if (count_out) d->has_out_canon = true;
       ftw[j] = tw >= 0 ? g.elen[tw] : NAN;
       feid[j] = e;
     }

@@ -47,6 +47,7 @@ struct DevEdges {
   float *fwd_twlen = nullptr;   // twin length per fwd position (segment)
   uint32_t *rev_eid = nullptr;  // edge id per rev position (trees)
   uint32_t *fwd_eid = nullptr;  // edge id per fwd position (trees)
+  bool has_out_canon = false;   // some out-edge carries the circuit-rank bit
   float min_es = 0.0f;          // min seconds over non-loop edges (use_impedance)
   float max_es = 0.0f;
   // angular (forward search over out-edges on the 2-state dual)

@@ -51,7 +51,7 @@ constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dealt to them round-robin
 constexpr uint32_t kScanQ = 4;     // key words per lane per step of the node-ordered result flush
 constexpr uint32_t kFlushList = kScanQ * 64;  // LDS list of one flush step (reached nodes, index order)
 constexpr uint32_t kFlushWords = 3 * kFlushList;  // {node, seconds bits, slot} per list entry
-constexpr uint32_t kRelWords = 5 * 64;  // per lane {first edge, in-degree, node, seconds, row start} of a batch
+constexpr uint32_t kRelWords = 6 * 64;  // per lane {first edge, in-degree, node, seconds, row start, slot} of a batch
 constexpr uint32_t kWaveLds = kFlushWords + kRelWords;  // LDS words per wave besides the ring
 constexpr uint32_t kLdsWords = 16384;                     // LDS words one workgroup may hold (64 KiB)
 constexpr uint32_t kScanSpan = 16; // scan the index span only while it is at most this many times the reach
@@ -82,7 +82,9 @@ struct Job {
   float maxsec, speed, inv_delta;
   float tol;  // tie tolerance; above the internal band, Phase 2 rebuilds predecessors
   uint32_t nbk, maxb;
-  uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach, 8 OD-weighted seeds, 16 fold after the search
+  uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach, 8 OD-weighted seeds, 16 fold after the search,
+                   // 32 some out-edge counts toward the circuit rank (one-way canonical edges)
+  double far_max;  // far_factor of the graph's largest out-degree: the widest tie band of any fold
   uint32_t n_sources;
   // OD-weighted betweenness: source position i's destinations and trip weights
   // are od_dest / od_w[od_off[i] .. od_off[i + 1])
@@ -110,7 +112,7 @@ struct SearchArgs {
   size_t ncap;
   char *ws;         // [waves][ws_stride] scratch
   size_t ws_stride;
-  size_t o_rec, o_pm, o_acc, o_ods, o_bkt, o_bnd, o_lvl;
+  size_t o_rec, o_pm, o_acc, o_ods, o_sig, o_bkt, o_bnd, o_lvl;
   size_t o_st;       // land-use statistics: per-wave lists (winner count, winners, sort keys, values, weights)
   uint32_t st_cap;   // winners one source may have in those lists
   uint32_t *epochs;
@@ -188,19 +190,31 @@ __device__ __forceinline__ double wave_sum(double v) { return __ockl_wfred_add_f
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t wave_or(uint32_t v) { return __ockl_wfred_or_u32(v); }
 
-// Node key, one 64-bit word: epoch (10 bits) | ~f32 bits of the seconds (32) |
-// slot (22, zero until the node settles). A 64-bit atomic max keeps the newest
-// epoch and then the smallest seconds; a relaxation carries slot 0, so it can
-// never displace a settled key of equal seconds.
-__device__ __forceinline__ uint64_t pack_key(uint32_t epoch, float secs) {
-  return ((uint64_t)epoch << 54) | ((uint64_t)(uint32_t)~__float_as_uint(secs) << 22);
+// Node key, one 64-bit word: epoch (10 bits) | settled (1) | low 31 bits of the
+// complemented f32 seconds (31; seconds are never negative, so the dropped top
+// bit is always 1) | slot (22). A 64-bit atomic max keeps the newest epoch,
+// then a settled key over any offer, then the smallest seconds. An offer
+// carries the slot of the node that makes it (the predecessor it proposes); a
+// settled key carries the node's own slot.
+constexpr uint64_t kSettledBit = 1ull << 53;
+constexpr uint64_t kSlotMask = (uint64_t)kSlotLimit - 1u;
+__device__ __forceinline__ uint64_t offer_key_bits(uint32_t epoch, uint32_t secs_bits, uint32_t slot) {
+  return ((uint64_t)epoch << 54) | ((uint64_t)(~secs_bits & 0x7fffffffu) << 22) | slot;
 }
-__device__ __forceinline__ uint64_t pack_key_bits(uint32_t epoch, uint32_t secs_bits) {
-  return ((uint64_t)epoch << 54) | ((uint64_t)(uint32_t)~secs_bits << 22);
+__device__ __forceinline__ uint64_t offer_key(uint32_t epoch, float secs, uint32_t slot) {
+  return offer_key_bits(epoch, __float_as_uint(secs), slot);
+}
+// The settled form of an offer key: same epoch and seconds, the node's slot.
+__device__ __forceinline__ uint64_t settle_key(uint64_t k, uint32_t slot) {
+  return (k & ~kSlotMask) | kSettledBit | slot;
 }
 __device__ __forceinline__ bool key_current(uint64_t k, uint32_t epoch) { return (uint32_t)(k >> 54) == epoch; }
-__device__ __forceinline__ float key_secs(uint64_t k) { return __uint_as_float(~(uint32_t)(k >> 22)); }
+__device__ __forceinline__ bool key_settled(uint64_t k) { return (k & kSettledBit) != 0; }
+__device__ __forceinline__ float key_secs(uint64_t k) { return __uint_as_float(~((uint32_t)(k >> 22) | 0x80000000u)); }
 __device__ __forceinline__ uint32_t key_slot(uint64_t k) { return (uint32_t)k & (kSlotLimit - 1u); }
+// Payload of a node not yet settled whose offers include two within the widest
+// tie band: a NaN pattern with the epoch, which no settled payload holds.
+__device__ __forceinline__ uint64_t tie_mark(uint32_t epoch) { return 0xfff8000000000000ull | epoch; }
 
 __device__ __forceinline__ uint32_t bucket_of(float s, float inv_delta) {
   return (uint32_t)(s * inv_delta);
@@ -226,6 +240,7 @@ struct WaveWs {
                     // segment: [cap][T] subtree sums
   uint32_t dcap;    // slots per dependency row (cap)
   double *ods;      // [cap] OD trip weight seeding each slot (OD-weighted betweenness)
+  double *sig;      // [cap] sigma of each slot (Brandes with betweenness)
   uint2 *bkt;       // [nbk][bcap] bucket ring entries {node, secs bits}
   uint32_t *bnd;    // [maxb + 2] slot bounds per opened bucket
   uint32_t *lvl;    // [maxb + 2] highest Phase 2 resolution round per bucket
@@ -250,6 +265,7 @@ __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uin
   w.acc = reinterpret_cast<double *>(base + a.o_acc);
   w.dcap = a.cap;
   w.ods = reinterpret_cast<double *>(base + a.o_ods);
+  w.sig = reinterpret_cast<double *>(base + a.o_sig);
   w.bkt = reinterpret_cast<uint2 *>(base + a.o_bkt);
   w.bnd = reinterpret_cast<uint32_t *>(base + a.o_bnd);
   w.lvl = reinterpret_cast<uint32_t *>(base + a.o_lvl);
@@ -429,8 +445,14 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
   // Phase 2, or Phase 1 folded after the search (flag 16): no fold at settle time
   const bool phase2 = kMulti && (j.tol > kTieEps || (j.flags & 16u));
   const bool fold = ((MODE != kBrandes) || (j.flags & 2u)) && !phase2;
+  // Brandes folds at settle time take the predecessor an offer key proposes
+  // unless the node carries a tie mark (two offers within the widest tie band)
+  const bool hint = (MODE == kBrandes) && fold;
+  // out-rows are scanned for the circuit rank only where one-way canonical
+  // edges exist (two-way keys are all counted by the relaxations)
+  const bool out_canon = cycles && (j.flags & 32u);
   if (lane == 0) {
-    st64(nkey(w, c.src), pack_key(c.tag, 0.0f));
+    st64(nkey(w, c.src), offer_key(c.tag, 0.0f, 0u));
     st64(reinterpret_cast<uint64_t *>(&w.bkt[0]), (uint64_t)c.src);
     lds_st(&w.ring[0], 1u);
   }
@@ -465,17 +487,24 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
     for (uint32_t base = 0; base < cnt; base += kWave) {
       const uint32_t i = base + lane;
       uint32_t v = 0, csb = 0, el = 0;
-      uint64_t kv = 0;
+      uint64_t kv = 0, kp = 0;
       uint4 rw = make_uint4(0u, 0u, 0u, 0u);
       if (i < cnt) {
         const uint64_t ent = ld64(reinterpret_cast<const uint64_t *>(&w.bkt[ring * a.bcap + i]));
         v = (uint32_t)ent;
         csb = (uint32_t)(ent >> 32);
-        kv = ld64(nkey(w, v));
+        if (hint) {
+          const uint4 r = nrec(w, v);
+          kv = lo64(r);
+          kp = hi64(r);
+        } else {
+          kv = ld64(nkey(w, v));
+        }
         rw = j.rows[v];
         if constexpr (MODE == kBrandes) el = j.eligible[v];
       }
-      const bool valid = (i < cnt) && kv == pack_key_bits(c.tag, csb);
+      // an entry is live while the key is still its offer: same seconds, not settled
+      const bool valid = (i < cnt) && (kv & ~kSlotMask) == offer_key_bits(c.tag, csb, 0u);
       const uint64_t m = __ballot(valid);
       const uint32_t cntv = (uint32_t)__popcll(m);
       if (c.nslot + cntv > a.cap) {
@@ -502,7 +531,9 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
         // ---- fold over the out-row, one node per lane: the candidates' loads
         // go out kFoldChunk at a time, then the candidates are weighed in row order
         constexpr uint32_t kFoldChunk = fold_chunk<MODE>();
-        const uint32_t nfold = (is_src || !(fold || cycles)) ? 0u : nf;
+        const bool hinted = hint && kp != tie_mark(c.tag);
+        const bool scan = !is_src && ((fold && !hinted) || out_canon);
+        const uint32_t nfold = scan ? nf : 0u;
         for (uint32_t cb = 0; cb < nfold; cb += kFoldChunk) {
           uint2 fe[kFoldChunk];
           uint64_t fk[kFoldChunk], fp[kFoldChunk];
@@ -576,7 +607,13 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           f.overflow = false;
           uint32_t pslot0 = 0, npred = 0;
           if (!is_src && fold) {
-            if (n_near == 1) {
+            if (hinted) {
+              // no tie mark: the one offer within the band is the node's final
+              // seconds, and its key names the offering slot
+              pslot0 = key_slot(kv);
+              f.sigma = ldd(&w.sig[pslot0]);
+              npred = 1;
+            } else if (n_near == 1) {
               f.pmask = ps_single(near_j, wide);
               f.sigma = __longlong_as_double((long long)near_pay);
               pslot0 = near_slot;
@@ -591,13 +628,13 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
               pslot0 = key_slot(ld64(nkey(w, u0)));
             }
           }
-          // the node's key gains its slot, the payload its sigma (read by folds,
-          // Phase 2 and back-propagation, none of which runs for closeness
-          // alone); relaxations still in flight carry larger seconds and leave
-          // the key unchanged
+          // the node's key turns settled with its own slot, the payload takes its
+          // sigma (read by folds, Phase 2 and back-propagation, none of which
+          // runs for closeness alone); a settled key outranks every offer
           if (MODE != kBrandes || (j.flags & 2u)) {
-            if (s) st64(nkey(w, v), kv | s);
+            st64(nkey(w, v), settle_key(kv, s));
             std_(reinterpret_cast<double *>(npay(w, v)), f.sigma);
+            if (hint) std_(&w.sig[s], f.sigma);
           }
           // the slot record is read only after this bucket's fence: one plain 16-byte store
           *reinterpret_cast<uint4 *>(&w.rec[2 * (size_t)s]) =
@@ -632,7 +669,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
               olen = (pslot == 0) ? llen : __uint_as_float((uint32_t)bpay);
             }
           }
-          if (s) st64(nkey(w, v), kv | s);
+          st64(nkey(w, v), settle_key(kv, s));
           st64(npay(w, v), __float_as_uint(olen));
           st64(&w.rec[2 * (size_t)s], ((uint64_t)csb << 32) | v);
           st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)bj << 32) | pslot);
@@ -648,6 +685,15 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
       // owner of edge g is the last lane whose first edge is <= g. The old key
       // returned by the atomic max also tells whether the neighbour settled
       // before the node (canonical edges count toward the circuit rank).
+      // A plain load first: keys only grow within an epoch, so a neighbour whose
+      // key already beats the offer (settled nodes, and better offers) needs no
+      // atomic, which would leave a dirty line behind for nothing. The decisions
+      // taken from the loaded key are the atomic's: a neighbour settling before
+      // the node is final when its bucket opens. Every offer meets the best
+      // offer before it (loaded or returned by the atomic); when the loser lies
+      // within the widest tie band of the winner, the neighbour is tie-marked
+      // and folds over its whole out-row. Two offers within the band of the
+      // final seconds always meet this way, whichever arrives first.
       {
         const uint32_t nrl = valid ? rw.w - rw.z : 0u;
         const uint32_t first = __ockl_wfscan_add_u32(nrl, false);
@@ -658,6 +704,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           lds_st(&w.rel[2 * kWave + lane], v);
           lds_st(&w.rel[3 * kWave + lane], csb);
           lds_st(&w.rel[4 * kWave + lane], rw.z);
+          lds_st(&w.rel[5 * kWave + lane], s);
           wave_fence();
           for (uint32_t g0 = 0; g0 < tot; g0 += kWave) {
             const uint32_t g = g0 + lane;
@@ -682,9 +729,16 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
                   const float cs = so + __uint_as_float(re.y);
                   uint64_t kn = 0;
                   if (cs <= j.maxsec) {
-                    const uint64_t nk = pack_key(c.tag, cs);
-                    kn = wmax64(nkey(w, nb), nk);
-                    if (nk > kn) {
+                    const uint64_t nk = offer_key(c.tag, cs, lds_ld(&w.rel[5 * kWave + lo]));
+                    const uint64_t k0 = ld64(nkey(w, nb));
+                    kn = (k0 >= nk) ? k0 : wmax64(nkey(w, nb), nk);
+                    const bool better = (nk >> 22) > (kn >> 22);  // strictly fewer seconds (or an older epoch)
+                    if (hint && key_current(kn, c.tag) && !key_settled(kn)) {
+                      const double sk = (double)key_secs(kn);
+                      const double lose = better ? sk : (double)cs, win = better ? (double)cs : sk;
+                      if (lose <= win * j.far_max) st64(npay(w, nb), tie_mark(c.tag));
+                    }
+                    if (better) {
                       const uint32_t b = bucket_of(cs, j.inv_delta);
                       if (b <= c.k || b - c.k >= j.nbk) {
                         lfail |= (b <= c.k) ? kFailInternal : kFailRing;
@@ -1075,7 +1129,7 @@ __device__ void flush_targets(CJob &j, const WaveWs &w, const SrcCtx &c, uint32_
       if (hit) {
         const uint32_t i = n + rank_below(m);
         lds_st(&w.list[i], v);
-        lds_st(&w.list[kFlushList + i], ~(uint32_t)(key[q] >> 22));
+        lds_st(&w.list[kFlushList + i], __float_as_uint(key_secs(key[q])));
         lds_st(&w.list[2 * kFlushList + i], key_slot(key[q]));
       }
       n += (uint32_t)__popcll(m);
@@ -1765,7 +1819,7 @@ int blocks_per_cu(uint32_t T, uint32_t block, size_t lds) {
 struct Plan {
   uint32_t bcap, cap, maxb, nbk, waves;
   size_t stride;
-  size_t o_rec, o_pm, o_acc, o_ods, o_bkt, o_bnd, o_lvl;
+  size_t o_rec, o_pm, o_acc, o_ods, o_sig, o_bkt, o_bnd, o_lvl;
   size_t o_st = 0;
   uint32_t st_cap = 0;  // winners per source for land-use statistics (0: none)
 };
@@ -1782,6 +1836,8 @@ void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   p.o_acc = o;
   o = align256(o + 8ull * p.cap * (brandes ? 2 * T : T));
   p.o_ods = o;
+  o = align256(o + (brandes ? 8ull * p.cap : 0ull));
+  p.o_sig = o;
   o = align256(o + (brandes ? 8ull * p.cap : 0ull));
   p.o_bkt = o;
   o = align256(o + 8ull * p.nbk * p.bcap);
@@ -2003,7 +2059,10 @@ Job make_job(const JobSpec &js, uint32_t NT) {
                         " s against a bucket width of " + std::to_string(delta) + " s)");
   j.maxb = (uint32_t)((double)cs.maxsec * (double)j.inv_delta) + 4;
   j.flags = (cs.closeness ? 1u : 0u) | (cs.betweenness ? 2u : 0u) | (cs.track_reach ? 4u : 0u) |
-            (exact_fold ? 16u : 0u);
+            (exact_fold ? 16u : 0u) | (de.has_out_canon ? 32u : 0u);
+  // the device's far_factor(max out-degree), the same products in the same order
+  j.far_max = 1.0;
+  for (uint32_t i = 0; i < g.max_out_deg + 3; ++i) j.far_max *= 1.00010002;
   j.od_off = js.d_od_off;
   j.od_dest = js.d_od_dest;
   j.od_w = js.d_od_w;
@@ -2132,6 +2191,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
     a.o_ods = p.o_ods;
+    a.o_sig = p.o_sig;
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.o_lvl = p.o_lvl;
@@ -2337,6 +2397,7 @@ struct ShortestBatch {
     a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
     a.o_ods = p.o_ods;
+    a.o_sig = p.o_sig;
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.o_lvl = p.o_lvl;
@@ -2686,6 +2747,7 @@ void launch_shortest_in(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs
     a.o_pm = p.o_pm;
     a.o_acc = p.o_acc;
     a.o_ods = p.o_ods;
+    a.o_sig = p.o_sig;
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.o_lvl = p.o_lvl;
