@@ -165,6 +165,8 @@ def run_variant(args) -> int:
         total_ms.append(stats["total_ms"])
         log(f"step {i}: kernel {stats['kernel_ms']:.2f} ms, call {stats['total_ms']:.2f} ms")
     kms = float(np.mean(kernel_ms))
+    # a step is the whole C call (host rows out; the graph stays uploaded)
+    call_ms = float(np.mean(total_ms))
     metrics = 4
     algo_bytes = 12 * stats["edges_scanned"] + 16 * stats["nodes_reached"] + 16 * metrics * stats["within_sum"]
     achieved = algo_bytes / (kms / 1000.0) / 1e9
@@ -172,12 +174,12 @@ def run_variant(args) -> int:
     call = "node_centrality_simplest (dual, angular)" if kind == "simplest" else "segment_centrality (primal)"
     line = {
         "metric": METRIC,
-        "value": round(n / (kms / 1000.0), 2),
+        "value": round(n / (call_ms / 1000.0), 2),
         "unit": "source searches/sec",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(kms, 3),
+        "ms_per_step": round(call_ms, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -310,18 +312,22 @@ def run_access(args) -> int:
         total_ms.append(stats["total_ms"])
         log(f"step {i}: kernel {stats['kernel_ms']:.2f} ms, call {stats['total_ms']:.2f} ms")
     kms = float(np.mean(kernel_ms))
+    # a step is the whole C call: host assignment tables in, device tables built,
+    # the launch, the rows copied out (the Python layer's assembly of the tables
+    # from its dicts is outside)
+    call_ms = float(np.mean(total_ms))
     # search bytes only (the aggregation's reads of the assignment tables are not counted)
     algo_bytes = 12 * stats["edges_scanned"] + 16 * stats["nodes_reached"]
     achieved = algo_bytes / (kms / 1000.0) / 1e9
     cpu = None if args.no_cpu else access_cpu_baseline(ns, dm, lu, args.cpu_seconds)
     line = {
         "metric": METRIC,
-        "value": round(n / (kms / 1000.0), 2),
+        "value": round(n / (call_ms / 1000.0), 2),
         "unit": "source searches/sec",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(kms, 3),
+        "ms_per_step": round(call_ms, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

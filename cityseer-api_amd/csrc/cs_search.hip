@@ -444,7 +444,8 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
   // with Phase 2 ahead, the Phase 1 fold's predecessors would be discarded
   // Phase 2, or Phase 1 folded after the search (flag 16): no fold at settle time
   const bool phase2 = kMulti && (j.tol > kTieEps || (j.flags & 16u));
-  const bool fold = ((MODE != kBrandes) || (j.flags & 2u)) && !phase2;
+  // land-use walks read only each node's seconds: no predecessor at all
+  const bool fold = ((MODE != kBrandes) || (j.flags & 2u)) && !phase2 && MODE != kAccess;
   // Brandes folds at settle time take the predecessor an offer key proposes
   // unless the node carries a tie mark (two offers within the widest tie band)
   const bool hint = (MODE == kBrandes) && fold;
@@ -659,7 +660,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
         } else {
           uint32_t pslot = 0xffffffffu;
           float olen = 0.0f, llen = 0.0f;
-          if (!is_src) {
+          if (!is_src && MODE != kAccess) {
             if (bj == 0xffffffffu || bcs != secs_v) {
               lfail |= kFailInternal;
               bj = 0xffffffffu;
@@ -670,10 +671,12 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             }
           }
           st64(nkey(w, v), settle_key(kv, s));
-          st64(npay(w, v), __float_as_uint(olen));
           st64(&w.rec[2 * (size_t)s], ((uint64_t)csb << 32) | v);
-          st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)bj << 32) | pslot);
-          st64(&w.pm[s], ((uint64_t)__float_as_uint(olen) << 32) | __float_as_uint(llen));
+          if constexpr (MODE != kAccess) {
+            st64(npay(w, v), __float_as_uint(olen));
+            st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)bj << 32) | pslot);
+            st64(&w.pm[s], ((uint64_t)__float_as_uint(olen) << 32) | __float_as_uint(llen));
+          }
           if constexpr (MODE == kSegment) {
 #pragma unroll
             for (int t = 0; t < NT; ++t) std_(&w.acc[(size_t)s * NT + t], 0.0);
