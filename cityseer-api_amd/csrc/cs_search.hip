@@ -15,16 +15,16 @@
 //    therefore be replayed per node from candidates of earlier buckets only,
 //    in the reference's settle order (ascending seconds, then node index),
 //    reproducing predecessor sets and sigma bit-exactly.
-//  * Per-wave node state is an epoch-tagged 64-bit key (epoch<<32 | ~f32 bits
-//    of seconds) updated with a 64-bit atomic max: newer epochs and smaller
-//    seconds win, zero-filled memory is stale, and no O(N) state is cleared
-//    per source. Settled nodes get compact slots in settle order; the source
-//    is always slot 0 (it is alone in bucket 0).
-//  * Each lane works through its node's in- and out-edges in chunks: the
-//    chunk's adjacency loads, neighbour key loads and relaxation atomics are
-//    all issued before any result is consumed, so a bucket costs a handful of
-//    memory round trips instead of two per edge. Bucket ring counters live in
-//    LDS.
+//  * Per-wave node state is one epoch-tagged 64-bit key per node (epoch,
+//    settled bit, seconds, slot, tie mark) updated with a 64-bit atomic max:
+//    newer epochs, settled keys and smaller seconds win, zero-filled memory is
+//    stale, and no O(N) state is cleared per source. Settled nodes get compact
+//    slots in settle order; the source is always slot 0 (it is alone in bucket
+//    0). An offer names the slot that makes it, so a node without a tie mark
+//    takes its one predecessor from its key, without looking at its out-row.
+//  * Relaxations run edge-parallel over the lanes of a batch (one in-edge per
+//    lane); a relaxation loads the neighbour's key first and runs the atomic
+//    max only for an offer that may win. Bucket ring counters live in LDS.
 //  * Closeness terms are scattered to targets with f64 atomics
 //    (centrality.rs:1755-1778); betweenness back-propagates bucket by bucket
 //    in reverse (centrality.rs:823-873), reading the predecessor's slot that
@@ -55,7 +55,9 @@ constexpr uint32_t kRelWords = 6 * 64;  // per lane {first edge, in-degree, node
 constexpr uint32_t kWaveLds = kFlushWords + kRelWords;  // LDS words per wave besides the ring
 constexpr uint32_t kLdsWords = 16384;                     // LDS words one workgroup may hold (64 KiB)
 constexpr uint32_t kScanSpan = 16; // scan the index span only while it is at most this many times the reach
-constexpr uint32_t kEpochMax = 1023u;       // 10-bit epoch tags 1..1023; 0 is never issued
+constexpr uint32_t kEpochMax = 511u;        // 9-bit epoch tags 1..511; 0 is never issued
+constexpr uint32_t kEpochShift = 55;        // epoch position in node keys and group records
+constexpr size_t kRecBytes = 8;             // one node record: its key
 constexpr uint32_t kSlotLimit = 1u << 22;   // 22-bit slots in the node key
 
 enum Mode { kBrandes = 0, kSegment = 1, kTree = 2, kBrandesExport = 3, kAccess = 4 };
@@ -108,7 +110,7 @@ struct SearchArgs {
   uint32_t n_items;
   uint32_t bcap, cap, nbk_lds;
   unsigned long long *stats;  // reached, edges, within
-  uint64_t *keys;   // [waves][ncap][2] node records {epoch-tagged key, payload}
+  uint64_t *keys;   // [waves][ncap] node records (epoch-tagged keys)
   size_t ncap;
   char *ws;         // [waves][ws_stride] scratch
   size_t ws_stride;
@@ -190,31 +192,31 @@ __device__ __forceinline__ double wave_sum(double v) { return __ockl_wfred_add_f
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ uint32_t wave_or(uint32_t v) { return __ockl_wfred_or_u32(v); }
 
-// Node key, one 64-bit word: epoch (10 bits) | settled (1) | low 31 bits of the
+// Node key, one 64-bit word: epoch (9 bits) | settled (1) | low 31 bits of the
 // complemented f32 seconds (31; seconds are never negative, so the dropped top
-// bit is always 1) | slot (22). A 64-bit atomic max keeps the newest epoch,
-// then a settled key over any offer, then the smallest seconds. An offer
-// carries the slot of the node that makes it (the predecessor it proposes); a
-// settled key carries the node's own slot.
-constexpr uint64_t kSettledBit = 1ull << 53;
+// bit is always 1) | slot (22) | tie mark (1). A 64-bit atomic max keeps the
+// newest epoch, then a settled key over any offer, then the smallest seconds.
+// An offer carries the slot of the node that makes it (the predecessor it
+// proposes); a settled key carries the node's own slot. The tie mark (set by an
+// atomic or) says that two offers within the widest tie band met at the node.
+constexpr uint64_t kSettledBit = 1ull << 54;
 constexpr uint64_t kSlotMask = (uint64_t)kSlotLimit - 1u;
+constexpr uint64_t kLowMask = (kSlotMask << 1) | 1u;  // slot and tie mark
 __device__ __forceinline__ uint64_t offer_key_bits(uint32_t epoch, uint32_t secs_bits, uint32_t slot) {
-  return ((uint64_t)epoch << 54) | ((uint64_t)(~secs_bits & 0x7fffffffu) << 22) | slot;
+  return ((uint64_t)epoch << kEpochShift) | ((uint64_t)(~secs_bits & 0x7fffffffu) << 23) | ((uint64_t)slot << 1);
 }
 __device__ __forceinline__ uint64_t offer_key(uint32_t epoch, float secs, uint32_t slot) {
   return offer_key_bits(epoch, __float_as_uint(secs), slot);
 }
 // The settled form of an offer key: same epoch and seconds, the node's slot.
 __device__ __forceinline__ uint64_t settle_key(uint64_t k, uint32_t slot) {
-  return (k & ~kSlotMask) | kSettledBit | slot;
+  return (k & ~kLowMask) | kSettledBit | ((uint64_t)slot << 1);
 }
-__device__ __forceinline__ bool key_current(uint64_t k, uint32_t epoch) { return (uint32_t)(k >> 54) == epoch; }
+__device__ __forceinline__ bool key_current(uint64_t k, uint32_t epoch) { return (uint32_t)(k >> kEpochShift) == epoch; }
 __device__ __forceinline__ bool key_settled(uint64_t k) { return (k & kSettledBit) != 0; }
-__device__ __forceinline__ float key_secs(uint64_t k) { return __uint_as_float(~((uint32_t)(k >> 22) | 0x80000000u)); }
-__device__ __forceinline__ uint32_t key_slot(uint64_t k) { return (uint32_t)k & (kSlotLimit - 1u); }
-// Payload of a node not yet settled whose offers include two within the widest
-// tie band: a NaN pattern with the epoch, which no settled payload holds.
-__device__ __forceinline__ uint64_t tie_mark(uint32_t epoch) { return 0xfff8000000000000ull | epoch; }
+__device__ __forceinline__ bool key_marked(uint64_t k) { return (k & 1u) != 0; }
+__device__ __forceinline__ float key_secs(uint64_t k) { return __uint_as_float(~((uint32_t)(k >> 23) | 0x80000000u)); }
+__device__ __forceinline__ uint32_t key_slot(uint64_t k) { return (uint32_t)(k >> 1) & (kSlotLimit - 1u); }
 
 __device__ __forceinline__ uint32_t bucket_of(float s, float inv_delta) {
   return (uint32_t)(s * inv_delta);
@@ -232,15 +234,14 @@ __device__ __forceinline__ bool key_less(const uint32_t *orig, float sa, uint32_
 //   hi = Brandes: first predecessor slot | (n_preds | eligible << 8) << 32
 //        trees:   predecessor slot | row position of the tree edge << 32
 struct WaveWs {
-  uint64_t *node;   // [N][2] node records {key, payload}: one 16-byte line segment per node;
-                    // payload = sigma bits (Brandes) or origin-segment length bits (trees)
+  uint64_t *node;   // [N] node keys (records after the graph's nodes: dedupe groups)
   uint64_t *rec;    // [cap][2]
-  uint64_t *pm;     // [cap] predecessor row mask (Brandes) / {last len, origin len} (trees)
+  uint64_t *pm;     // [cap] predecessor row mask (Brandes) / {origin len, last len} (trees)
   double *acc;      // Brandes: [2T][cap] dependencies, plain then beta-weighted per threshold;
                     // segment: [cap][T] subtree sums
   uint32_t dcap;    // slots per dependency row (cap)
   double *ods;      // [cap] OD trip weight seeding each slot (OD-weighted betweenness)
-  double *sig;      // [cap] sigma of each slot (Brandes with betweenness)
+  double *sig;      // [cap] sigma of each slot (Brandes searches)
   uint2 *bkt;       // [nbk][bcap] bucket ring entries {node, secs bits}
   uint32_t *bnd;    // [maxb + 2] slot bounds per opened bucket
   uint32_t *lvl;    // [maxb + 2] highest Phase 2 resolution round per bucket
@@ -259,7 +260,7 @@ __device__ __forceinline__ double *dep_at(const WaveWs &w, uint32_t k, uint32_t 
 __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uint32_t *ring) {
   char *base = a.ws + (size_t)wid * a.ws_stride;
   WaveWs w;
-  w.node = a.keys + (size_t)wid * a.ncap * 2;
+  w.node = a.keys + (size_t)wid * a.ncap;
   w.rec = reinterpret_cast<uint64_t *>(base + a.o_rec);
   w.pm = reinterpret_cast<uint64_t *>(base + a.o_pm);
   w.acc = reinterpret_cast<double *>(base + a.o_acc);
@@ -356,14 +357,11 @@ __device__ __forceinline__ double far_factor(uint32_t deg) {
 
 // Ordered fold over the near candidates of v (two or more): selection passes in
 // settle order (secs_u, u), ties of one u (parallel edges) in row order.
-__device__ __forceinline__ uint64_t *nkey(const WaveWs &w, uint32_t v) { return &w.node[2 * (size_t)v]; }
-__device__ __forceinline__ uint64_t *npay(const WaveWs &w, uint32_t v) { return &w.node[2 * (size_t)v + 1]; }
-// Key and payload of one node in one 16-byte load.
-__device__ __forceinline__ uint4 nrec(const WaveWs &w, uint32_t v) {
-  return *reinterpret_cast<const uint4 *>(&w.node[2 * (size_t)v]);
+__device__ __forceinline__ uint64_t *nkey(const WaveWs &w, uint32_t v) { return &w.node[v]; }
+// Origin-segment length of a settled slot (trees: the high word of its pm entry).
+__device__ __forceinline__ float slot_olen(const WaveWs &w, uint32_t s) {
+  return __uint_as_float((uint32_t)(ld64(&w.pm[s]) >> 32));
 }
-__device__ __forceinline__ uint64_t lo64(uint4 r) { return ((uint64_t)r.y << 32) | r.x; }
-__device__ __forceinline__ uint64_t hi64(uint4 r) { return ((uint64_t)r.w << 32) | r.z; }
 
 // Candidates are out-row nodes settled before v: in an earlier bucket than kb,
 // or, with by_key, anywhere before v in (seconds, original index) order.
@@ -381,7 +379,7 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
   for (uint32_t step = 0; step < n_near; ++step) {
     bool found = false;
     float psu = INFINITY, pcs = 0.0f;
-    uint32_t pu = 0, pj = 0;
+    uint32_t pu = 0, pj = 0, psl = 0;
     for (uint32_t p = r0; p < r1; ++p) {
       const uint2 e = j.fwd_adj[p];
       const uint32_t u = e.x & kNodeMask;
@@ -400,6 +398,7 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
         pu = u;
         pj = jr;
         pcs = cs;
+        psl = key_slot(ku);
       }
     }
     if (!found) return false;
@@ -407,7 +406,7 @@ __device__ bool brandes_fold_exact(CJob &j, const WaveWs &w, uint32_t tag, uint3
     uint32_t jb;
     for (uint64_t m = f.pmask; ps_next(m, f.wide, jb);)
       if ((j.fwd_adj[r0 + jb].x & kNodeMask) == pu) contains = true;
-    const double sg = ldd(reinterpret_cast<const double *>(npay(w, pu)));
+    const double sg = ldd(&w.sig[psl]);
     fold_apply(f, pcs, sg, pj, contains);
     have_last = true;
     lsu = psu;
@@ -488,24 +487,18 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
     for (uint32_t base = 0; base < cnt; base += kWave) {
       const uint32_t i = base + lane;
       uint32_t v = 0, csb = 0, el = 0;
-      uint64_t kv = 0, kp = 0;
+      uint64_t kv = 0;
       uint4 rw = make_uint4(0u, 0u, 0u, 0u);
       if (i < cnt) {
         const uint64_t ent = ld64(reinterpret_cast<const uint64_t *>(&w.bkt[ring * a.bcap + i]));
         v = (uint32_t)ent;
         csb = (uint32_t)(ent >> 32);
-        if (hint) {
-          const uint4 r = nrec(w, v);
-          kv = lo64(r);
-          kp = hi64(r);
-        } else {
-          kv = ld64(nkey(w, v));
-        }
+        kv = ld64(nkey(w, v));
         rw = j.rows[v];
         if constexpr (MODE == kBrandes) el = j.eligible[v];
       }
       // an entry is live while the key is still its offer: same seconds, not settled
-      const bool valid = (i < cnt) && (kv & ~kSlotMask) == offer_key_bits(c.tag, csb, 0u);
+      const bool valid = (i < cnt) && (kv & ~kLowMask) == offer_key_bits(c.tag, csb, 0u);
       const uint64_t m = __ballot(valid);
       const uint32_t cntv = (uint32_t)__popcll(m);
       if (c.nslot + cntv > a.cap) {
@@ -525,39 +518,25 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
         // fold state (multi-predecessor: near candidates; trees: lexicographic min)
         const double thr_far = kMulti ? (double)secs_v * far_factor(nf) : 0.0;
         uint32_t n_near = 0, near_j = 0, near_slot = 0;
-        uint64_t near_pay = 0, bpay = 0;
         float near_cs = INFINITY;
         float bcs = INFINITY, bsu = INFINITY;
         uint32_t bu = 0xffffffffu, bj = 0xffffffffu, bslot = 0;
         // ---- fold over the out-row, one node per lane: the candidates' loads
         // go out kFoldChunk at a time, then the candidates are weighed in row order
         constexpr uint32_t kFoldChunk = fold_chunk<MODE>();
-        const bool hinted = hint && kp != tie_mark(c.tag);
+        const bool hinted = hint && !key_marked(kv);
         const bool scan = !is_src && ((fold && !hinted) || out_canon);
         const uint32_t nfold = scan ? nf : 0u;
         for (uint32_t cb = 0; cb < nfold; cb += kFoldChunk) {
           uint2 fe[kFoldChunk];
-          uint64_t fk[kFoldChunk], fp[kFoldChunk];
-          uint32_t fs[kFoldChunk];
+          uint64_t fk[kFoldChunk];
 #pragma unroll
           for (uint32_t q = 0; q < kFoldChunk; ++q) fe[q] = (cb + q < nf) ? j.fwd_adj[f0 + cb + q] : make_uint2(v, 0u);
 #pragma unroll
           for (uint32_t q = 0; q < kFoldChunk; ++q) {
             const uint32_t u = fe[q].x & kNodeMask;
             fk[q] = 0;
-            fs[q] = 0;
-            fp[q] = 0;
-            if (u != v && (fold || (cycles && (fe[q].x & kCanonBit)))) {
-              if (fold) {
-                // key, slot and payload of the candidate in one load
-                const uint4 r = nrec(w, u);
-                fk[q] = lo64(r);
-                fs[q] = key_slot(fk[q]);
-                fp[q] = hi64(r);
-              } else {
-                fk[q] = ld64(nkey(w, u));
-              }
-            }
+            if (u != v && (fold || (cycles && (fe[q].x & kCanonBit)))) fk[q] = ld64(nkey(w, u));
           }
           // fold candidates v -> u (u settled in an earlier bucket)
 #pragma unroll
@@ -577,8 +556,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
                 if (n_near == 0) {
                   near_j = jr;
                   near_cs = cs;
-                  near_slot = fs[q];
-                  near_pay = fp[q];
+                  near_slot = key_slot(ku);
                 }
                 ++n_near;
               }
@@ -592,8 +570,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
                 bsu = su;
                 bu = u;
                 bj = jr;
-                bslot = fs[q];
-                bpay = fp[q];
+                bslot = key_slot(ku);
               }
             }
           }
@@ -616,7 +593,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
               npred = 1;
             } else if (n_near == 1) {
               f.pmask = ps_single(near_j, wide);
-              f.sigma = __longlong_as_double((long long)near_pay);
+              f.sigma = ldd(&w.sig[near_slot]);
               pslot0 = near_slot;
               npred = 1;
               if (near_cs != secs_v) lfail |= kFailInternal;
@@ -629,13 +606,12 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
               pslot0 = key_slot(ld64(nkey(w, u0)));
             }
           }
-          // the node's key turns settled with its own slot, the payload takes its
+          // the node's key turns settled with its own slot and the slot takes its
           // sigma (read by folds, Phase 2 and back-propagation, none of which
           // runs for closeness alone); a settled key outranks every offer
           if (MODE != kBrandes || (j.flags & 2u)) {
             st64(nkey(w, v), settle_key(kv, s));
-            std_(reinterpret_cast<double *>(npay(w, v)), f.sigma);
-            if (hint) std_(&w.sig[s], f.sigma);
+            std_(&w.sig[s], f.sigma);
           }
           // the slot record is read only after this bucket's fence: one plain 16-byte store
           *reinterpret_cast<uint4 *>(&w.rec[2 * (size_t)s]) =
@@ -667,13 +643,12 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
             } else {
               pslot = bslot;
               llen = j.fwd_twlen[f0 + bj];
-              olen = (pslot == 0) ? llen : __uint_as_float((uint32_t)bpay);
+              olen = (pslot == 0) ? llen : slot_olen(w, pslot);
             }
           }
           st64(nkey(w, v), settle_key(kv, s));
           st64(&w.rec[2 * (size_t)s], ((uint64_t)csb << 32) | v);
           if constexpr (MODE != kAccess) {
-            st64(npay(w, v), __float_as_uint(olen));
             st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)bj << 32) | pslot);
             st64(&w.pm[s], ((uint64_t)__float_as_uint(olen) << 32) | __float_as_uint(llen));
           }
@@ -735,11 +710,12 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
                     const uint64_t nk = offer_key(c.tag, cs, lds_ld(&w.rel[5 * kWave + lo]));
                     const uint64_t k0 = ld64(nkey(w, nb));
                     kn = (k0 >= nk) ? k0 : wmax64(nkey(w, nb), nk);
-                    const bool better = (nk >> 22) > (kn >> 22);  // strictly fewer seconds (or an older epoch)
+                    const bool better = (nk >> 23) > (kn >> 23);  // strictly fewer seconds (or an older epoch)
                     if (hint && key_current(kn, c.tag) && !key_settled(kn)) {
                       const double sk = (double)key_secs(kn);
                       const double lose = better ? sk : (double)cs, win = better ? (double)cs : sk;
-                      if (lose <= win * j.far_max) st64(npay(w, nb), tie_mark(c.tag));
+                      if (lose <= win * j.far_max)
+                        __hip_atomic_fetch_or(nkey(w, nb), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                     if (better) {
                       const uint32_t b = bucket_of(cs, j.inv_delta);
@@ -856,7 +832,7 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c, uint32
       uint64_t pm = 0;
       if (s == 0) {
         meta |= kP2Done;
-        std_(reinterpret_cast<double *>(npay(w, v)), 1.0);
+        std_(&w.sig[0], 1.0);
       } else {
         const uint4 rw = j.rows[v];
         const bool wide = (rw.y - rw.x) > kWideRow;
@@ -920,24 +896,26 @@ __device__ bool phase2_rebuild(CJob &j, const WaveWs &w, const SrcCtx &c, uint32
         for (uint32_t step = 0; step < np; ++step) {
           bool found = false;
           float bsu = 0.0f;
-          uint32_t bu = 0;
+          uint32_t bu = 0, bsl = 0;
           for (uint64_t mm = pm; ps_next(mm, wide, q);) {
             const uint32_t u = j.fwd_adj[f0 + q].x & kNodeMask;
-            const float su = key_secs(ld64(nkey(w, u)));
+            const uint64_t ku = ld64(nkey(w, u));
+            const float su = key_secs(ku);
             if (have_last && !key_less(j.orig, lsu, lu, su, u)) continue;
             if (!found || key_less(j.orig, su, u, bsu, bu)) {
               found = true;
               bsu = su;
               bu = u;
+              bsl = key_slot(ku);
             }
           }
-          sigma += ldd(reinterpret_cast<const double *>(npay(w, bu)));
+          sigma += ldd(&w.sig[bsl]);
           have_last = true;
           lsu = bsu;
           lu = bu;
         }
         if (pm) pslot0 = key_slot(ld64(nkey(w, (j.fwd_adj[f0 + ps_first(pm, wide)].x & kNodeMask))));
-        std_(reinterpret_cast<double *>(npay(w, v)), sigma);
+        std_(&w.sig[s], sigma);
         st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np | (round << 16)) << 32) | pslot0);
       }
       pending = uniform(wave_sum(pending));
@@ -973,7 +951,7 @@ __device__ bool phase1_fold_after(CJob &j, const WaveWs &w, const SrcCtx &c, uin
       uint64_t cm = 0;
       if (s == 0) {
         meta |= kP2Done;
-        std_(reinterpret_cast<double *>(npay(w, v)), 1.0);
+        std_(&w.sig[0], 1.0);
       } else {
         const uint4 rw = j.rows[v];
         const bool wide = (rw.y - rw.x) > kWideRow;
@@ -1037,7 +1015,7 @@ __device__ bool phase1_fold_after(CJob &j, const WaveWs &w, const SrcCtx &c, uin
         const uint32_t u0 = j.fwd_adj[rw.x + ps_first(f.pmask, wide)].x & kNodeMask;
         const uint32_t pslot0 = key_slot(ld64(nkey(w, u0)));
         st64(&w.pm[s], f.pmask);
-        std_(reinterpret_cast<double *>(npay(w, v)), f.sigma);
+        std_(&w.sig[s], f.sigma);
         st64(&w.rec[2 * (size_t)s + 1], ((uint64_t)(meta | kP2Done | np | (round << 16)) << 32) | pslot0);
       }
       if (uniform(wave_sum(bad))) {
@@ -1347,9 +1325,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
     c.src = j.sources[si];
     c.wt = (MODE == kBrandes) ? j.src_wt[si] : 0.0f;
     if (epoch >= kEpochMax) {
-      // the 10-bit tags are used up: clear this wave's node records and start over
-      uint4 *nr = reinterpret_cast<uint4 *>(w.node);
-      for (size_t i = lane; i < a.ncap; i += kWave) nr[i] = make_uint4(0u, 0u, 0u, 0u);
+      // the 9-bit tags are used up: clear this wave's node records and start over
+      for (size_t i = lane; i < a.ncap; i += kWave) w.node[i] = 0ull;
       wave_fence();
       epoch = 0;
     }
@@ -1453,13 +1430,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
                 const uint4 rv = j.rows[v];
                 const uint32_t f0 = rv.x;
                 const bool wide = (rv.y - rv.x) > kWideRow;
-                const double sigma_w = ldd(reinterpret_cast<const double *>(npay(w, v)));
+                const double sigma_w = ldd(&w.sig[s]);
                 uint32_t jb;
                 for (uint64_t mm = pm; ps_next(mm, wide, jb);) {
                   const uint32_t u = j.fwd_adj[f0 + jb].x & kNodeMask;
-                  const uint4 ru = nrec(w, u);
-                  const uint32_t su = key_slot(lo64(ru));
-                  const double sigma_u = __longlong_as_double((long long)hi64(ru));
+                  const uint32_t su = key_slot(ld64(nkey(w, u)));
+                  const double sigma_u = ldd(&w.sig[su]);
                   if (sigma_u == 0.0) continue;
                   const double fct = sigma_u / sigma_w;
 #pragma unroll
@@ -1640,12 +1616,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
       // the members reached.
       const AccessOut &ao = a.acc;
       const uint32_t src = c.src;
-      // group g owns the key words of records nacc + 2g and nacc + 2g + 1. Only key
-      // words carry an epoch (payload words hold sigmas and lengths of earlier
-      // searches), so whatever earlier sources left in them, node keys of a larger
-      // graph included, compares below the current tag
-      uint64_t *grec = w.node + 2 * (size_t)j.nacc;
-      const uint64_t tag = (uint64_t)c.tag << 54;
+      // group g owns records nacc + 2g and nacc + 2g + 1 (node keys to earlier
+      // sources): whatever those left there, keys of a larger graph included,
+      // carries an older epoch and compares below the current tag
+      uint64_t *grec = w.node + (size_t)j.nacc;
+      const uint64_t tag = (uint64_t)c.tag << kEpochShift;
       // statistics: the winners go to a list in the wave's scratch instead of the counts
       char *stb = a.ws + (size_t)wid * a.ws_stride + a.o_st;
       uint32_t *st_n = reinterpret_cast<uint32_t *>(stb);
@@ -1667,14 +1642,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
             const bool shared = me.w != kSoloGroup;
             const uint64_t tkey = tag | ((uint64_t)(uint32_t)~__float_as_uint(tot) << 22);  // larger = nearer
             if (pass == 0) {
-              if (shared) wmax64(&grec[4 * (size_t)me.w], tkey);
+              if (shared) wmax64(&grec[2 * (size_t)me.w], tkey);
               continue;
             }
             if (pass == 1) {
-              if (shared && ld64(&grec[4 * (size_t)me.w]) == tkey) wmax64(&grec[4 * (size_t)me.w + 2], tag | (uint32_t)~m);
+              if (shared && ld64(&grec[2 * (size_t)me.w]) == tkey) wmax64(&grec[2 * (size_t)me.w + 1], tag | (uint32_t)~m);
               continue;
             }
-            if (shared && ld64(&grec[4 * (size_t)me.w + 2]) != (tag | (uint32_t)~m)) continue;
+            if (shared && ld64(&grec[2 * (size_t)me.w + 1]) != (tag | (uint32_t)~m)) continue;
             if (ao.stats) {
               const uint32_t q = __hip_atomic_fetch_add(st_n, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
               if (q < a.st_cap) st64(&st_win[q], ((uint64_t)me.y << 32) | __float_as_uint(tot));
@@ -1728,7 +1703,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
           }
         } else if constexpr (MODE == kBrandesExport) {
           const uint64_t pm = ld64(&w.pm[s]);
-          a.tout.sigma[v] = ldd(reinterpret_cast<const double *>(npay(w, v)));
+          a.tout.sigma[v] = ldd(&w.sig[s]);
           uint32_t np = 0;
           const uint4 rv = j.rows[v];
           const uint32_t f0 = rv.x;
@@ -1913,10 +1888,9 @@ struct DevVec {
 
 struct Workspace {
   int device = -1;
-  // node records {epoch-tagged key, payload}, [key_waves][ncap][2]: zeroed
-  // whenever (re)allocated; each wave clears its own when its 10-bit epochs
-  // wrap, and key words left by earlier graphs carry older epochs (payload
-  // words carry none)
+  // node records (epoch-tagged keys), [key_waves][ncap]: zeroed whenever
+  // (re)allocated; each wave clears its own when its 9-bit epochs wrap, and
+  // keys left by earlier graphs carry older epochs
   uint64_t *keys = nullptr;
   uint32_t ncap = 0, key_waves = 0;
   uint32_t *epochs = nullptr;
@@ -1960,7 +1934,7 @@ struct WsGuard {
 size_t ws_growth(const Workspace &w, uint32_t N, size_t stride, uint32_t waves) {
   size_t g = 0;
   if (N > w.ncap || waves > w.key_waves)
-    g += 16ull * std::max(N, w.ncap) * std::max(waves, w.key_waves);
+    g += kRecBytes * std::max(N, w.ncap) * std::max(waves, w.key_waves);
   if (stride * waves > w.scratch.bytes) g += stride * waves;
   return g;
 }
@@ -1977,10 +1951,10 @@ void prepare_ws(Workspace &w, uint32_t N, size_t stride, uint32_t waves, uint64_
     w.keys = nullptr;
     w.epochs = nullptr;
     w.ncap = w.key_waves = 0;
-    if (hipMalloc(&w.keys, 16ull * ncap * kw) != hipSuccess) {
+    if (hipMalloc(&w.keys, kRecBytes * ncap * kw) != hipSuccess) {
       (void)hipGetLastError();
       w.keys = nullptr;
-      fail(CS_ENOMEM, "could not allocate " + std::to_string((16ull * ncap * kw) >> 20) + " MiB of node records");
+      fail(CS_ENOMEM, "could not allocate " + std::to_string((kRecBytes * ncap * kw) >> 20) + " MiB of node records");
     }
     hip_check(hipMalloc(&w.epochs, 4ull * kw), "hipMalloc epochs");
     w.ncap = ncap;
@@ -1990,7 +1964,7 @@ void prepare_ws(Workspace &w, uint32_t N, size_t stride, uint32_t waves, uint64_
   if (w.epoch_hi + sources + 2 >= 0xfffffff0ull) clear = true;
   if (clear) {
     // every key carries epoch 0, which is never issued
-    hip_check(hipMemsetAsync(w.keys, 0, 16ull * w.ncap * w.key_waves, stream), "hipMemset keys");
+    hip_check(hipMemsetAsync(w.keys, 0, kRecBytes * w.ncap * w.key_waves, stream), "hipMemset keys");
     hip_check(hipMemsetAsync(w.epochs, 0, 4ull * w.key_waves, stream), "hipMemset epochs");
     w.epoch_hi = 0;
   }
@@ -2161,8 +2135,8 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     if (ws_growth(*wsp, maxN, p.stride, want) > 0) {
       size_t free_b = 0, total_b = 0;
       hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-      const size_t have = wsp->scratch.bytes + 16ull * wsp->ncap * wsp->key_waves;
-      const size_t per_wave = p.stride + 16ull * maxN;
+      const size_t have = wsp->scratch.bytes + kRecBytes * wsp->ncap * wsp->key_waves;
+      const size_t per_wave = p.stride + kRecBytes * maxN;
       by_mem = (uint32_t)std::min<size_t>((free_b / 4 + have) / per_wave, want);
     }
     if (by_mem < wpb) {
@@ -2343,7 +2317,7 @@ struct ShortestBatch {
     layout(p, maxN, NT, true);
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    const uint32_t by_mem = (uint32_t)std::min<size_t>((free_b / 4) / (p.stride + 16ull * maxN), 1u << 20);
+    const uint32_t by_mem = (uint32_t)std::min<size_t>((free_b / 4) / (p.stride + kRecBytes * maxN), 1u << 20);
     if (by_mem < wpb) fail(CS_ENOMEM, "batch too large for the search workspace on this device");
     waves = std::min<uint32_t>((uint32_t)std::min<uint64_t>(n_items, resident), by_mem);
     waves = std::max<uint32_t>(wpb, (waves + wpb - 1) / wpb * wpb);
@@ -2420,7 +2394,7 @@ struct ShortestBatch {
     if (n_items == 0) return;
     // epochs: every wave may use up to n_items of them per run
     if (ws.epoch_hi + n_items + 2 >= 0xfffffff0ull) {
-      hip_check(hipMemsetAsync(ws.keys, 0, 16ull * ws.ncap * ws.key_waves, stream), "hipMemset keys");
+      hip_check(hipMemsetAsync(ws.keys, 0, kRecBytes * ws.ncap * ws.key_waves, stream), "hipMemset keys");
       hip_check(hipMemsetAsync(ws.epochs, 0, 4ull * ws.key_waves, stream), "hipMemset epochs");
       ws.epoch_hi = 0;
     }
@@ -2708,10 +2682,10 @@ void launch_shortest_in(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs
   int cur = 0;
   for (int attempt = 0;; ++attempt) {
     layout(p, N, T, true);
-    const size_t per_wave = 4 + 16ull * N + p.stride;
+    const size_t per_wave = 4 + kRecBytes * N + p.stride;
     const uint32_t want = std::max<uint32_t>(wpb, (std::min<uint32_t>(n_list, resident) + wpb - 1) / wpb * wpb);
     // epochs, node records and scratch of w waves, each part 256-byte aligned
-    auto bytes_for = [&](uint32_t w) { return align256(4ull * w) + align256(16ull * N * w) + p.stride * w; };
+    auto bytes_for = [&](uint32_t w) { return align256(4ull * w) + align256(kRecBytes * N * w) + p.stride * w; };
     uint32_t waves = std::min<uint32_t>(want, (uint32_t)std::min<size_t>((ws_bytes - fixed) / per_wave, want));
     waves = waves / wpb * wpb;
     while (waves >= wpb && fixed + bytes_for(waves) > ws_bytes) waves -= wpb;
@@ -2727,10 +2701,10 @@ void launch_shortest_in(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs
     char *base = ws + fixed;
     uint32_t *d_epochs = reinterpret_cast<uint32_t *>(base);
     uint64_t *d_keys = reinterpret_cast<uint64_t *>(base + align256(4ull * waves));
-    char *d_scratch = base + align256(4ull * waves) + align256(16ull * N * waves);
+    char *d_scratch = base + align256(4ull * waves) + align256(kRecBytes * N * waves);
     // node records and epochs start from zero (epoch 0 is never issued)
     hipLaunchKernelGGL(k_zero64, dim3(2048), dim3(kB), 0, stream, reinterpret_cast<uint64_t *>(base),
-                       (align256(4ull * waves) + align256(16ull * N * waves)) / 8);
+                       (align256(4ull * waves) + align256(kRecBytes * N * waves)) / 8);
     hipLaunchKernelGGL(k_zero64, dim3(1), dim3(64), 0, stream, reinterpret_cast<uint64_t *>(d_ctl), (size_t)8);
     hip_check(hipGetLastError(), "clear launch");
     SearchArgs a;
@@ -2850,7 +2824,7 @@ void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uin
   cs.betweenness = false;
   const Tuning tune = current_tuning();
   JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, nullptr, nullptr, tune.delta_scale};
-  js.extra_records = 2 * out.n_shared;  // two 16-byte records per shared group, their key words tagged
+  js.extra_records = 2 * out.n_shared;  // two records (epoch-tagged words) per shared group
   run_jobs<kAccess>({js}, 1, tune, nullptr, stream, st, nullptr, &out, progress);
 }
 
