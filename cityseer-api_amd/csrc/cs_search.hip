@@ -51,6 +51,7 @@ constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dealt to them round-robin
 constexpr uint32_t kScanQ = 4;     // key words per lane per step of the node-ordered result flush
 constexpr uint32_t kFlushList = kScanQ * 64;  // LDS list of one flush step (reached nodes, index order)
 constexpr uint32_t kFlushWords = 3 * kFlushList;  // {node, seconds bits, slot} per list entry
+constexpr uint32_t kRelQ = 3;       // in-edges per lane per relaxation step
 constexpr uint32_t kRelWords = 6 * 64;  // per lane {first edge, in-degree, node, seconds, row start, slot} of a batch
 constexpr uint32_t kWaveLds = kFlushWords + kRelWords;  // LDS words per wave besides the ring
 constexpr uint32_t kLdsWords = 16384;                     // LDS words one workgroup may hold (64 KiB)
@@ -684,68 +685,96 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           lds_st(&w.rel[4 * kWave + lane], rw.z);
           lds_st(&w.rel[5 * kWave + lane], s);
           wave_fence();
-          for (uint32_t g0 = 0; g0 < tot; g0 += kWave) {
-            const uint32_t g = g0 + lane;
-            bool ce = false;
-            float route_o = 0.0f;
-            if (g < tot) {
-              uint32_t lo = 0, hi = kWave - 1;
-              while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (lds_ld(&w.rel[mid]) <= g) lo = mid;
-                else hi = mid - 1;
+          for (uint32_t g0 = 0; g0 < tot; g0 += kRelQ * kWave) {
+            // kRelQ edges per lane per step: owners and adjacency first, then the
+            // neighbours' keys, then the atomics, each stage's loads in flight together
+            bool act[kRelQ], ce[kRelQ];
+            uint32_t vo[kRelQ], oslot[kRelQ];
+            float so[kRelQ], route_o[kRelQ];
+            uint2 re[kRelQ];
+#pragma unroll
+            for (uint32_t q = 0; q < kRelQ; ++q) {
+              const uint32_t g = g0 + q * kWave + lane;
+              act[q] = false;
+              ce[q] = false;
+              route_o[q] = 0.0f;
+              vo[q] = oslot[q] = 0;
+              so[q] = 0.0f;
+              re[q] = make_uint2(0u, 0u);
+              if (g < tot) {
+                uint32_t lo = 0, hi = kWave - 1;
+                while (lo < hi) {
+                  const uint32_t mid = (lo + hi + 1) >> 1;
+                  if (lds_ld(&w.rel[mid]) <= g) lo = mid;
+                  else hi = mid - 1;
+                }
+                const uint32_t k = g - lds_ld(&w.rel[lo]);
+                if (k >= lds_ld(&w.rel[kWave + lo])) {
+                  lfail |= kFailInternal;
+                } else {
+                  vo[q] = lds_ld(&w.rel[2 * kWave + lo]);
+                  so[q] = __uint_as_float(lds_ld(&w.rel[3 * kWave + lo]));
+                  oslot[q] = lds_ld(&w.rel[5 * kWave + lo]);
+                  re[q] = j.rev_adj[lds_ld(&w.rel[4 * kWave + lo]) + k];
+                  act[q] = true;
+                }
               }
-              const uint32_t k = g - lds_ld(&w.rel[lo]);
-              if (k >= lds_ld(&w.rel[kWave + lo])) {
-                lfail |= kFailInternal;
-              } else {
-                const uint32_t vo = lds_ld(&w.rel[2 * kWave + lo]);
-                const float so = __uint_as_float(lds_ld(&w.rel[3 * kWave + lo]));
-                const uint2 re = j.rev_adj[lds_ld(&w.rel[4 * kWave + lo]) + k];
-                const uint32_t nb = re.x & kNodeMask;
-                if (nb != vo) {
-                  const float cs = so + __uint_as_float(re.y);
-                  uint64_t kn = 0;
-                  if (cs <= j.maxsec) {
-                    const uint64_t nk = offer_key(c.tag, cs, lds_ld(&w.rel[5 * kWave + lo]));
-                    const uint64_t k0 = ld64(nkey(w, nb));
-                    kn = (k0 >= nk) ? k0 : wmax64(nkey(w, nb), nk);
-                    const bool better = (nk >> 23) > (kn >> 23);  // strictly fewer seconds (or an older epoch)
-                    if (hint && key_current(kn, c.tag) && !key_settled(kn)) {
-                      const double sk = (double)key_secs(kn);
-                      const double lose = better ? sk : (double)cs, win = better ? (double)cs : sk;
-                      if (lose <= win * j.far_max)
-                        __hip_atomic_fetch_or(nkey(w, nb), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-                    if (better) {
-                      const uint32_t b = bucket_of(cs, j.inv_delta);
-                      if (b <= c.k || b - c.k >= j.nbk) {
-                        lfail |= (b <= c.k) ? kFailInternal : kFailRing;
-                      } else {
-                        const uint32_t rb = b % j.nbk;
-                        const uint32_t pos = lds_add(&w.ring[rb], 1u);
-                        if (pos < a.bcap)
-                          st64(reinterpret_cast<uint64_t *>(&w.bkt[rb * a.bcap + pos]),
-                               ((uint64_t)__float_as_uint(cs) << 32) | nb);
-                        else
-                          lfail |= kFailBucket;
-                        ++pushes;
-                      }
-                    }
-                  } else if (cycles && (re.x & kCanonBit)) {
-                    kn = ld64(nkey(w, nb));
-                  }
-                  if (cycles && (re.x & kCanonBit) && key_current(kn, c.tag) && key_less(j.orig, key_secs(kn), nb, so, vo)) {
-                    ce = true;
-                    route_o = so * speed;
+            }
+            uint64_t k0[kRelQ];
+            float csq[kRelQ];
+            bool offer[kRelQ];
+#pragma unroll
+            for (uint32_t q = 0; q < kRelQ; ++q) {
+              const uint32_t nb = re[q].x & kNodeMask;
+              act[q] = act[q] && nb != vo[q];
+              csq[q] = so[q] + __uint_as_float(re[q].y);
+              offer[q] = act[q] && csq[q] <= j.maxsec;
+              k0[q] = (offer[q] || (act[q] && cycles && (re[q].x & kCanonBit))) ? ld64(nkey(w, nb)) : 0ull;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < kRelQ; ++q) {
+              if (!act[q]) continue;
+              const uint32_t nb = re[q].x & kNodeMask;
+              const float cs = csq[q];
+              uint64_t kn = k0[q];
+              if (offer[q]) {
+                const uint64_t nk = offer_key(c.tag, cs, oslot[q]);
+                kn = (k0[q] >= nk) ? k0[q] : wmax64(nkey(w, nb), nk);
+                const bool better = (nk >> 23) > (kn >> 23);  // strictly fewer seconds (or an older epoch)
+                if (hint && key_current(kn, c.tag) && !key_settled(kn)) {
+                  const double sk = (double)key_secs(kn);
+                  const double lose = better ? sk : (double)cs, win = better ? (double)cs : sk;
+                  if (lose <= win * j.far_max)
+                    __hip_atomic_fetch_or(nkey(w, nb), 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                if (better) {
+                  const uint32_t b = bucket_of(cs, j.inv_delta);
+                  if (b <= c.k || b - c.k >= j.nbk) {
+                    lfail |= (b <= c.k) ? kFailInternal : kFailRing;
+                  } else {
+                    const uint32_t rb = b % j.nbk;
+                    const uint32_t pos = lds_add(&w.ring[rb], 1u);
+                    if (pos < a.bcap)
+                      st64(reinterpret_cast<uint64_t *>(&w.bkt[rb * a.bcap + pos]),
+                           ((uint64_t)__float_as_uint(cs) << 32) | nb);
+                    else
+                      lfail |= kFailBucket;
+                    ++pushes;
                   }
                 }
+              }
+              if (cycles && (re[q].x & kCanonBit) && key_current(kn, c.tag) &&
+                  key_less(j.orig, key_secs(kn), nb, so[q], vo[q])) {
+                ce[q] = true;
+                route_o[q] = so[q] * speed;
               }
             }
             if constexpr (MODE == kBrandes) {
               if (cycles) {
 #pragma unroll
-                for (int t = 0; t < NT; ++t) eband[t] += (uint32_t)__popcll(__ballot(ce && route_o <= j.dist[t]));
+                for (uint32_t q = 0; q < kRelQ; ++q)
+#pragma unroll
+                  for (int t = 0; t < NT; ++t) eband[t] += (uint32_t)__popcll(__ballot(ce[q] && route_o[q] <= j.dist[t]));
               }
             }
           }
