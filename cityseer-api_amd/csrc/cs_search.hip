@@ -252,8 +252,9 @@ struct WaveWs {
 };
 
 // Brandes dependency k of slot s: k = t (plain) or T + t (beta-weighted). Rows
-// are threshold-major: slots are in settle order, so the slots within a
-// threshold are a prefix of its rows, and the words a source touches are dense.
+// are threshold-major: slots are in bucket order, so the slots within a
+// threshold fill its rows up to the bucket straddling it, and the words a
+// source touches are dense.
 __device__ __forceinline__ double *dep_at(const WaveWs &w, uint32_t k, uint32_t s) {
   return &w.acc[(size_t)k * w.dcap + s];
 }
