@@ -298,19 +298,20 @@ def run_access(args) -> int:
     dm, lu = access_workload(ns, lat, 2.0, args.seed + 1)
     n_assign = sum(len(p) for p in dm.node_data_map.values())
 
-    def step():
-        dm.accessibility(ns, lu, ACCESS_CLASSES[:3], distances=ACCESS_DISTANCES)
-        return _native.last_stats()
+    def timed(dmap, lus, label):
+        for _ in range(args.warmup):
+            dmap.accessibility(ns, lus, ACCESS_CLASSES[:3], distances=ACCESS_DISTANCES)
+        kernel_ms, total_ms = [], []
+        st = None
+        for i in range(args.steps):
+            dmap.accessibility(ns, lus, ACCESS_CLASSES[:3], distances=ACCESS_DISTANCES)
+            st = _native.last_stats()
+            kernel_ms.append(st["kernel_ms"])
+            total_ms.append(st["total_ms"])
+            log(f"{label} step {i}: kernel {st['kernel_ms']:.2f} ms, call {st['total_ms']:.2f} ms")
+        return st, kernel_ms, total_ms
 
-    for _ in range(args.warmup):
-        step()
-    kernel_ms, total_ms = [], []
-    stats = None
-    for i in range(args.steps):
-        stats = step()
-        kernel_ms.append(stats["kernel_ms"])
-        total_ms.append(stats["total_ms"])
-        log(f"step {i}: kernel {stats['kernel_ms']:.2f} ms, call {stats['total_ms']:.2f} ms")
+    stats, kernel_ms, total_ms = timed(dm, lu, "access")
     kms = float(np.mean(kernel_ms))
     # a step is the whole C call: host assignment tables in, device tables built,
     # the launch, the rows copied out (the Python layer's assembly of the tables
@@ -356,6 +357,25 @@ def run_access(args) -> int:
         },
         "cpu_baseline": cpu,
     }
+    if args.access_group:
+        # the same workload plus one dedupe group of many members spread over the
+        # lattice (the vertices of one large feature): each group counts once per
+        # source, through its nearest member
+        rng = np.random.default_rng(args.seed + 2)
+        for e in range(args.access_group):
+            key = f"poly{e}"
+            dm.insert(key, "POINT (0 0)", "poly")
+            lu[key] = ACCESS_CLASSES[0]
+            dm.node_data_map.setdefault(int(rng.integers(0, n)), []).append((f"str:{key}", float(rng.uniform(0.0, 40.0))))
+        st, kg, tg = timed(dm, lu, f"access + group of {args.access_group}")
+        line["large_group"] = {
+            "workload": f"the same plus one dedupe group of {args.access_group} members at random nodes",
+            "value": round(n / (float(np.mean(tg)) / 1000.0), 2),
+            "unit": "source searches/sec",
+            "ms_per_step": round(float(np.mean(tg)), 3),
+            "kernel_ms": round(float(np.mean(kg)), 3),
+            "kernel_over_plain": round(float(np.mean(kg)) / kms, 3),
+        }
     print(json.dumps(line), flush=True)
     return 0
 
@@ -738,6 +758,8 @@ def main() -> int:
     ap.add_argument("--serve-max-nodes", type=float, default=1e6, help="serve mode: largest graph")
     ap.add_argument("--slice-ms", type=float, default=0.0,
                     help="serve mode: target duration of one batched launch (0: the server's default)")
+    ap.add_argument("--access-group", type=int, default=2000,
+                    help="access mode: also time the workload with one dedupe group of this many members (0: no)")
     ap.add_argument("--no-legs", action="store_true",
                     help="plain mode: skip the batched (64 graphs) and served legs nested in the line")
     ap.add_argument("--leg-steps", type=int, default=3, help="plain mode: timed passes of the nested legs")
