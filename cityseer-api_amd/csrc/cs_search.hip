@@ -115,7 +115,7 @@ struct SearchArgs {
   size_t ncap;
   char *ws;         // [waves][ws_stride] scratch
   size_t ws_stride;
-  size_t o_rec, o_pm, o_acc, o_ods, o_sig, o_bkt, o_bnd, o_lvl;
+  size_t o_rec, o_pm, o_acc, o_ods, o_sig, o_bkt, o_bnd, o_lvl, o_bk;
   size_t o_st;       // land-use statistics: per-wave lists (winner count, winners, sort keys, values, weights)
   uint32_t st_cap;   // winners one source may have in those lists
   uint32_t *epochs;
@@ -246,6 +246,7 @@ struct WaveWs {
   uint2 *bkt;       // [nbk][bcap] bucket ring entries {node, secs bits}
   uint32_t *bnd;    // [maxb + 2] slot bounds per opened bucket
   uint32_t *lvl;    // [maxb + 2] highest Phase 2 resolution round per bucket
+  uint32_t *bk;     // [maxb + 2] absolute bucket number of each opened bucket
   uint32_t *ring;   // LDS [nbk] bucket ring counters
   uint32_t *list;   // LDS [3][kFlushList] {node, seconds bits, slot} list of the result flush
   uint32_t *rel;    // LDS [5][64] edge-parallel relaxation table of one batch
@@ -272,6 +273,7 @@ __device__ __forceinline__ WaveWs wave_ws(const SearchArgs &a, uint32_t wid, uin
   w.bkt = reinterpret_cast<uint2 *>(base + a.o_bkt);
   w.bnd = reinterpret_cast<uint32_t *>(base + a.o_bnd);
   w.lvl = reinterpret_cast<uint32_t *>(base + a.o_lvl);
+  w.bk = reinterpret_cast<uint32_t *>(base + a.o_bk);
   w.ring = ring;
   w.list = ring + a.nbk_lds;
   w.rel = w.list + kFlushWords;
@@ -483,6 +485,7 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
     if (lane == 0) {
       lds_st(&w.ring[ring], 0u);
       st32(&w.bnd[c.nb], c.nslot);
+      st32(&w.bk[c.nb], c.k);
     }
     ++c.nb;
     uint32_t pushes = 0, lfail = 0;
@@ -1420,11 +1423,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
         for (int b = (int)c.nb - 1; b >= 0; --b) {
           const uint32_t s0n = (b > 0) ? ld32(&w.bnd[b - 1]) : 0u;
           const uint32_t top = p2 ? uniform(ld32(&w.lvl[b])) : 0u;
+          // thresholds some node of this bucket may lie within: a route here is at
+          // least (bucket - 1) bucket widths, so their dependencies are loaded
+          // with the slot records instead of after them
+          bool maybe[NT];
+          {
+            const uint32_t kb = uniform(ld32(&w.bk[b]));
+            const double lo_route = (double)(kb > 0 ? kb - 1 : 0) / (double)j.inv_delta * (double)speed;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) maybe[t] = !p2 && lo_route <= (double)j.dist[t];
+          }
           for (int L = (int)top; L >= 0; --L) {
             // the source (slot 0) has no predecessors
             for (uint32_t s = (s0 ? s0 : 1u) + lane; s < s1; s += kWave) {
               const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
               const uint64_t hi = ld64(&w.rec[2 * (size_t)s + 1]);
+              double d0[NT], d1[NT];
+#pragma unroll
+              for (int t = 0; t < NT; ++t) {
+                d0[t] = maybe[t] ? ldd(dep_at(w, t, s)) : 0.0;
+                d1[t] = maybe[t] ? ldd(dep_at(w, NT + t, s)) : 0.0;
+              }
               if (p2 && (uint32_t)(hi >> 48) != (uint32_t)L) continue;
               const uint32_t v = (uint32_t)lo;
               const float route = __uint_as_float((uint32_t)(lo >> 32)) * speed;
@@ -1439,8 +1458,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
                 dep[t] = dep_b[t] = 0.0;
                 if (route <= j.dist[t]) {
                   const double seed_b = pc * exp(-(double)j.beta[t] * (double)route);
-                  dep[t] = pc + ldd(dep_at(w, t, s));
-                  dep_b[t] = seed_b + ldd(dep_at(w, NT + t, s));
+                  dep[t] = pc + (maybe[t] ? d0[t] : ldd(dep_at(w, t, s)));
+                  dep_b[t] = seed_b + (maybe[t] ? d1[t] : ldd(dep_at(w, NT + t, s)));
                   any = any || dep[t] != 0.0 || dep_b[t] != 0.0;
                 }
               }
@@ -1827,7 +1846,7 @@ int blocks_per_cu(uint32_t T, uint32_t block, size_t lds) {
 struct Plan {
   uint32_t bcap, cap, maxb, nbk, waves;
   size_t stride;
-  size_t o_rec, o_pm, o_acc, o_ods, o_sig, o_bkt, o_bnd, o_lvl;
+  size_t o_rec, o_pm, o_acc, o_ods, o_sig, o_bkt, o_bnd, o_lvl, o_bk;
   size_t o_st = 0;
   uint32_t st_cap = 0;  // winners per source for land-use statistics (0: none)
 };
@@ -1852,6 +1871,8 @@ void layout(Plan &p, uint32_t N, uint32_t T, bool brandes) {
   p.o_bnd = o;
   o = align256(o + 4ull * (p.maxb + 2));
   p.o_lvl = o;
+  o = align256(o + 4ull * (p.maxb + 2));
+  p.o_bk = o;
   o = align256(o + 4ull * (p.maxb + 2));
   p.o_st = o;
   // winner count | winners, 2 cap | sort keys, 2 cap | values | weights (sorts pad to a power of two)
@@ -2202,6 +2223,7 @@ void run_jobs(const std::vector<JobSpec> &specs, uint32_t NT, const Tuning &tune
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.o_lvl = p.o_lvl;
+    a.o_bk = p.o_bk;
     a.o_st = p.o_st;
     a.st_cap = p.st_cap;
     a.epochs = wsp->epochs;
@@ -2408,6 +2430,7 @@ struct ShortestBatch {
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.o_lvl = p.o_lvl;
+    a.o_bk = p.o_bk;
     a.epochs = ws.epochs;
     a.next = d_ctl;
     a.xsplit = tune.xcd_split ? 1u : 0u;
@@ -2758,6 +2781,7 @@ void launch_shortest_in(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs
     a.o_bkt = p.o_bkt;
     a.o_bnd = p.o_bnd;
     a.o_lvl = p.o_lvl;
+    a.o_bk = p.o_bk;
     a.epochs = d_epochs;
     a.next = d_ctl;
     a.xsplit = tune.xcd_split ? 1u : 0u;
