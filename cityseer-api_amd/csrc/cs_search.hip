@@ -785,9 +785,10 @@ __device__ bool search_source(const SearchArgs &a, CJob &j, const WaveWs &w, Src
           wave_fence();  // the next batch rewrites the LDS table
         }
       }
-      // nodes within each threshold (circuit rank, reach) and the rank's edge
+      // nodes within each threshold (circuit rank, reach; segment: the count
+      // of node-threshold pairs the statistics report) and the rank's edge
       // counts from the out-row side, reduced over the wave
-      if constexpr (MODE == kBrandes) {
+      if constexpr (MODE == kBrandes || MODE == kSegment) {
         const float route_v = __uint_as_float(csb) * speed;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -1506,6 +1507,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
       // ---- every target's closeness terms and betweenness credit, in node order
       flush_targets<NT>(j, w, c, R, rank, wt, cyc);
     } else if constexpr (MODE == kSegment) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) st_within += nband[t] ? nband[t] - 1 : 0u;
       // ---- closeness at the source over visited edges (centrality.rs:2201-2317)
       if (j.flags & 1u) {
         double sd[NT], sh[NT], sb[NT];
