@@ -70,6 +70,17 @@ class CsStats(ctypes.Structure):
     ]
 
 
+class CsGeoms(ctypes.Structure):
+    _fields_ = [
+        ("n_geoms", ctypes.c_uint32),
+        ("geom_part_off", ctypes.POINTER(ctypes.c_uint32)),
+        ("part_kind", ctypes.POINTER(ctypes.c_uint8)),
+        ("part_ring_off", ctypes.POINTER(ctypes.c_uint32)),
+        ("ring_off", ctypes.POINTER(ctypes.c_uint32)),
+        ("xy", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
 class CsTuning(ctypes.Structure):
     _fields_ = [
         ("block_threads", ctypes.c_uint32),
@@ -137,6 +148,19 @@ _SIGNATURES = {
     ),
     "cs_pieces_copy": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _f64p]),
     "cs_pieces_free": (None, [ctypes.c_void_p]),
+    "cs_assign_data": (
+        ctypes.c_int,
+        [ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_double),
+         ctypes.POINTER(CsGeoms), ctypes.POINTER(CsGeoms), ctypes.c_double, ctypes.c_uint32,
+         ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64)],
+    ),
+    "cs_assignment_copy": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+         ctypes.POINTER(ctypes.c_double)],
+    ),
+    "cs_assignment_free": (None, [ctypes.c_void_p]),
     "cs_graph_info": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _u32p, _u32p]),
     "cs_graph_progress": (ctypes.c_uint64, [ctypes.c_void_p]),
     "cs_centrality_shortest": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CsParams), _f64p, _u32p]),

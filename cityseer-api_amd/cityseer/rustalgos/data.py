@@ -7,10 +7,10 @@ counted once through its nearest member (data.rs:520-615).
 
 The assignment of entries to nodes is ``node_data_map``: ``{node index:
 [(data key, distance), ...]}``, the map ``assign_data_to_network`` leaves
-(data.rs:455-512). Here it is set by the caller; the geometric assignment
-itself (R-tree candidates, barrier and street-crossing checks,
-graph.rs:1346-1535) is not part of this package and raises
-``NotImplementedError``, as do the angular variants. ``stats`` (data.rs:1013-1233)
+(data.rs:455-512), computed natively on the host (``cs_assign_data``: nearest
+street edges, barrier and street-crossing checks, graph.rs:1346-1535); a caller
+may also set it directly. The angular variants raise ``NotImplementedError``.
+``stats`` (data.rs:1013-1233)
 runs the same launch with each source's wave summarising the numeric values of
 the entries it reaches.
 ``mixed_uses`` (data.rs:749-944) shares the launch and adds a per-node
@@ -198,10 +198,43 @@ class DataMap:
         self._progress = 0
 
     def assign_data_to_network(self, network_structure, max_assignment_dist: float, n_nearest_candidates: int):
-        raise NotImplementedError(
-            "geometric data assignment is not available here; set DataMap.node_data_map to "
-            "{node index: [(data key, distance), ...]} instead"
-        )
+        """Assign every entry to the end nodes of its nearest street edges whose line of sight
+        crosses no barrier and no street (data.rs:455-512, graph.rs:1346-1535), natively on
+        the host (``cs_assign_data``). Replaces ``node_data_map``."""
+        from .._geom import GeomArrays, parse_wkt
+
+        keys = list(self.entries.keys())
+        data = GeomArrays([parse_wkt(self.entries[k].geom_wkt) for k in keys])
+        bars = None if network_structure._barriers is None else GeomArrays(network_structure._barriers)
+        src, dst, off, xy = network_structure._street_polylines()
+        nxy = np.ascontiguousarray(np.stack([network_structure._x.array(), network_structure._y.array()], axis=1),
+                                   dtype=np.float64)
+        p = _native.ptr
+        lib = _native.load()
+        handle = ctypes.c_void_p()
+        n_out = ctypes.c_uint64()
+        d_struct = data.struct(_native.CsGeoms)
+        b_struct = bars.struct(_native.CsGeoms) if bars is not None else None
+        src32 = np.ascontiguousarray(src, dtype=np.uint32)
+        dst32 = np.ascontiguousarray(dst, dtype=np.uint32)
+        _native.check(lib.cs_assign_data(
+            len(nxy), p(nxy, ctypes.c_double), len(src32), p(src32, ctypes.c_uint32), p(dst32, ctypes.c_uint32),
+            p(off, ctypes.c_uint32), p(np.ascontiguousarray(xy), ctypes.c_double),
+            ctypes.byref(b_struct) if b_struct is not None else None, ctypes.byref(d_struct),
+            float(max_assignment_dist), int(n_nearest_candidates), ctypes.byref(handle), ctypes.byref(n_out)))
+        try:
+            n = int(n_out.value)
+            di = np.empty(n, dtype=np.uint32)
+            nodes = np.empty(n, dtype=np.uint32)
+            dist = np.empty(n, dtype=np.float64)
+            if n:
+                _native.check(lib.cs_assignment_copy(handle, p(di, ctypes.c_uint32), p(nodes, ctypes.c_uint32),
+                                                     p(dist, ctypes.c_double)))
+        finally:
+            lib.cs_assignment_free(handle)
+        self.node_data_map = {}
+        for d, v, dd in zip(di.tolist(), nodes.tolist(), dist.tolist(), strict=True):
+            self.node_data_map.setdefault(v, []).append((keys[d], dd))
 
     def stats(
         self,

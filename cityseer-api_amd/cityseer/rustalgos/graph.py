@@ -173,6 +173,13 @@ class NetworkStructure:
         self._eimp = _Column(np.float32)
         self._esec = _Column(np.float32)
         self._epk = _Column(np.int32)
+        # edge polylines (for the data assignment): edge e's coordinates are
+        # _gxy[2 * _goff[e] : 2 * (_goff[e] + _gn[e])]; _gn 0 = none given (a
+        # straight line between the end nodes stands in)
+        self._goff = _Column(np.uint64)
+        self._gn = _Column(np.uint32)
+        self._gxy = _Column(np.float64)
+        self._barriers: list | None = None
         self._pk_ids: dict[str, int] = {}
         self._pk_names: list[str] = []
         self._handle: ctypes.c_void_p | None = None
@@ -313,6 +320,53 @@ class NetworkStructure:
     def street_node_indices(self) -> list[int]:
         return list(range(len(self._keys)))
 
+    # -- barriers and edge polylines for the data assignment (graph.rs:1154-1235) --
+    def set_barriers(self, barriers_wkt: list[str]) -> None:
+        """Barrier geometries (WKT) that assignment lines may not cross; replaces any set
+        before. Geometries that fail to parse or have no coordinates are skipped with a
+        warning, as the reference skips them."""
+        from .._geom import parse_wkt
+
+        geoms = []
+        for i, w in enumerate(barriers_wkt):
+            try:
+                parts = parse_wkt(w)
+            except ValueError as err:
+                logging.getLogger(__name__).warning(f"Failed to parse WKT barrier (input index {i}): {err}")
+                continue
+            if not parts:
+                logging.getLogger(__name__).warning(f"Skipping barrier geom (input index {i}) with no bounding box")
+                continue
+            geoms.append(parts)
+        self._barriers = geoms or None
+
+    def unset_barriers(self) -> None:
+        self._barriers = None
+
+    def _street_polylines(self):
+        """(start, end, offsets, xy) of every edge, in edge order; an edge added without a
+        polyline runs straight between its end nodes."""
+        src, dst = self._esrc.array(), self._edst.array()
+        goff, gn, gxy = self._goff.array(), self._gn.array(), self._gxy.array().reshape(-1, 2)
+        xs, ys = self._x.array(), self._y.array()
+        n = np.where(gn > 0, gn, 2).astype(np.uint64)
+        off = np.zeros(len(src) + 1, dtype=np.uint64)
+        np.cumsum(n, out=off[1:])
+        xy = np.empty((int(off[-1]), 2), dtype=np.float64)
+        has = gn > 0
+        if has.any():
+            cnt = gn[has].astype(np.int64)
+            within = np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+            xy[np.repeat(off[:-1][has].astype(np.int64), cnt) + within] = gxy[
+                np.repeat(goff[has].astype(np.int64), cnt) + within]
+        if (~has).any():
+            a = off[:-1][~has].astype(np.int64)
+            xy[a] = np.stack([xs[src[~has]], ys[src[~has]]], axis=1)
+            xy[a + 1] = np.stack([xs[dst[~has]], ys[dst[~has]]], axis=1)
+        if off[-1] >= 2**32:
+            raise ValueError("too many edge coordinates for the data assignment")
+        return src, dst, off.astype(np.uint32), xy
+
     @property
     def node_xs(self) -> list[float]:
         return self._x.array().tolist()
@@ -385,6 +439,9 @@ class NetworkStructure:
                 f"{start_nd_idx} and {end_nd_idx}.\nImpedance must be finite and positive (> 0.0)."
             )
         eid = len(self._esrc)
+        self._goff.append(len(self._gxy) // 2)
+        self._gn.append(len(coords))
+        self._gxy.extend(np.asarray([c[:2] for c in coords], dtype=np.float64).reshape(-1))
         self._esrc.append(int(start_nd_idx))
         self._edst.append(int(end_nd_idx))
         self._ekey.append(int(edge_idx))
@@ -421,6 +478,8 @@ class NetworkStructure:
         if m and (not np.all(np.isfinite(imp)) or np.any(imp <= 0)):
             raise ValueError("Invalid impedance factor: impedance must be finite and positive (> 0.0).")
         start = len(self._esrc)
+        self._goff.extend(np.full(m, len(self._gxy) // 2, dtype=np.uint64))
+        self._gn.extend(np.zeros(m, dtype=np.uint32))
         self._esrc.extend(s.astype(np.uint32))
         self._edst.extend(d.astype(np.uint32))
         self._ekey.extend(np.zeros(m, dtype=np.uint32) if edge_idx is None else np.asarray(edge_idx, dtype=np.uint32))
@@ -454,7 +513,16 @@ class NetworkStructure:
         if shared_primal_node_key is not None:
             pk = np.fromiter((self._primal_key_id(k) for k in shared_primal_node_key), dtype=np.int32,
                              count=len(lines))
-        return self.add_street_edges_bulk(start_nd_idx, end_nd_idx, edge_idx, length, angle, imp_factor, pk)
+        eids = self.add_street_edges_bulk(start_nd_idx, end_nd_idx, edge_idx, length, angle, imp_factor, pk)
+        # keep the polylines (x, y) for the data assignment
+        counts = np.fromiter((len(c) for c in lines), dtype=np.uint32, count=len(lines))
+        base = len(self._gxy) // 2
+        offs = base + np.concatenate([[0], np.cumsum(counts[:-1], dtype=np.uint64)]) if len(lines) else counts
+        self._goff.array()[eids] = offs
+        self._gn.array()[eids] = counts
+        if len(lines):
+            self._gxy.extend(np.concatenate([np.asarray(c, dtype=np.float64)[:, :2].reshape(-1) for c in lines]))
+        return eids
 
     def edge_references(self) -> list[tuple[int, int, int]]:
         return list(zip(self._esrc.array().tolist(), self._edst.array().tolist(), self._ekey.array().tolist(), strict=True))

@@ -98,6 +98,62 @@ class Lattice:
         return src, dst, length, angle, pk
 
 
+def get_graph_extents(nx_multigraph: nx.MultiGraph) -> tuple[float, float, float, float]:
+    """(min x, min y, max x, max y) of the graph's nodes."""
+    xs = [d["x"] for _, d in nx_multigraph.nodes(data=True)]
+    ys = [d["y"] for _, d in nx_multigraph.nodes(data=True)]
+    return min(xs), min(ys), max(xs), max(ys)
+
+
+def mock_data_gdf(nx_multigraph: nx.MultiGraph, length: int = 50, random_seed: int = 0):
+    """Mock data points within the graph's extents (mock.py:274-311 of the reference): a
+    ``pandas.DataFrame`` indexed by ``uid`` with WKT ``geometry`` and ``data_id`` columns
+    (GeoPandas is not required). The points come from numpy's legacy global generator seeded
+    with ``random_seed``, x then y, as the reference draws them; the last five share one
+    ``data_id`` and sit 10 m apart on a diagonal, for deduplication checks."""
+    import pandas as pd
+
+    np.random.seed(seed=random_seed)
+    min_x, min_y, max_x, max_y = get_graph_extents(nx_multigraph)
+    xs = np.random.uniform(min_x, max_x, length)
+    ys = np.random.uniform(min_y, max_y, length)
+    pts = [(float(x), float(y)) for x, y in zip(xs, ys, strict=True)]
+    data_id = list(range(length))
+    for idx, loc in enumerate(range(length - 5, length)):
+        data_id[loc] = length - 5
+        pts[loc] = (700100.0 + idx * 10, 5719100.0 + idx * 10)
+    df = pd.DataFrame({"uid": list(range(length)), "geometry": [f"POINT ({x!r} {y!r})" for x, y in pts],
+                       "data_id": data_id})
+    return df.set_index("uid")
+
+
+def mock_data_map(data_gdf, dedupe_key_col: str | None = None):
+    """A ``DataMap`` of a frame like ``mock_data_gdf``'s (mock.py:540-560 of the reference);
+    geometries may be WKT strings or objects with a ``wkt`` attribute."""
+    from ..rustalgos import data
+
+    dm = data.DataMap()
+    for uid, row in data_gdf.iterrows():
+        geom = row["geometry"]
+        wkt = geom if isinstance(geom, str) else geom.wkt
+        dm.insert(uid, wkt, row[dedupe_key_col] if dedupe_key_col is not None else None)
+    return dm
+
+
+def mock_barriers():
+    """Mock barriers (mock.py:563-611 of the reference): a point, a line, a multi-line and a
+    polygon, as a ``pandas.DataFrame`` of WKT and the list of their WKT."""
+    import pandas as pd
+
+    wkts = [
+        "POINT (700900 5719350)",
+        "LINESTRING (701020 5719150, 701250 5719600)",
+        "MULTILINESTRING ((700400 5719150, 700150 5719600))",
+        "POLYGON ((700900 5719140, 700900 5719160, 700750 5719160, 700750 5719140, 700900 5719140))",
+    ]
+    return pd.DataFrame({"geometry": wkts}), wkts
+
+
 def _largest_component(n: int, edges: np.ndarray) -> np.ndarray:
     from scipy.sparse import coo_matrix
     from scipy.sparse.csgraph import connected_components

@@ -137,6 +137,39 @@ int cs_halve(uint32_t n_edges, const uint32_t *coord_off, const double *pts, uin
 int cs_pieces_copy(const cs_pieces *p, uint32_t *edge_cuts, uint32_t *piece_off, double *pts);
 void cs_pieces_free(cs_pieces *p);
 
+/* Geometries in flat arrays: geometry g has parts [geom_part_off[g],
+ * geom_part_off[g + 1]); part p has kind part_kind[p] (0 point, 1 line string,
+ * 2 polygon) and rings [part_ring_off[p], part_ring_off[p + 1]) (one ring for
+ * points and lines; a polygon's exterior, then its holes); ring r has the
+ * coordinates [ring_off[r], ring_off[r + 1]) of xy [n][2]. Multi-geometries
+ * are several parts. */
+typedef struct {
+  uint32_t n_geoms;
+  const uint32_t *geom_part_off;
+  const uint8_t *part_kind;
+  const uint32_t *part_ring_off;
+  const uint32_t *ring_off;
+  const double *xy;
+} cs_geoms;
+
+/* Replaces DataMap.assign_data_to_network (rust/src/data.rs:455-512) with
+ * NetworkStructure.find_assignments_for_entry (rust/src/graph.rs:1346-1535):
+ * each data geometry is assigned to the end nodes of its nearest street edges
+ * whose line of sight (from the geometry's nearest point to the node) crosses
+ * no barrier and no street. Street edges are given in edge order with their
+ * polylines (edge_off [n_edges + 1] into edge_xy [n][2]); edges repeating an
+ * earlier edge's node pair and geometry (either direction) are skipped, as
+ * the reference's edge R-tree skips them. barriers may be NULL. The result
+ * rows (data index, node, distance) come out in data order through
+ * cs_assignment_copy; cs_assignment_free releases them. Host only. */
+typedef struct cs_assignment cs_assignment;
+int cs_assign_data(uint32_t n_nodes, const double *node_xy, uint32_t n_edges, const uint32_t *edge_start,
+                   const uint32_t *edge_end, const uint32_t *edge_off, const double *edge_xy,
+                   const cs_geoms *barriers, const cs_geoms *data, double max_dist, uint32_t n_nearest,
+                   cs_assignment **out, uint64_t *n_out);
+int cs_assignment_copy(const cs_assignment *a, uint32_t *data_index, uint32_t *node, double *dist);
+void cs_assignment_free(cs_assignment *a);
+
 /* Number of nodes/edges; maximum out/in degree. */
 int cs_graph_info(const cs_graph *g, uint32_t *n_nodes, uint32_t *n_edges, uint32_t *max_out_deg,
                   uint32_t *max_in_deg);

@@ -109,8 +109,9 @@ def test_datamap_argument_checks():
         dm.accessibility(ns, {"a": "shop", "c": "shop"}, ["shop"], distances=[200])
     with pytest.raises(ValueError, match="exactly one"):
         dm.accessibility(ns, {"a": "shop", "b": "shop"}, ["shop"])
-    with pytest.raises(NotImplementedError):
-        dm.assign_data_to_network(ns, 400.0, 6)
+    # both points take both ends of edge 0-1 (the twin 1-0 shares its geometry)
+    dm.assign_data_to_network(ns, 400.0, 6)
+    assert dm.node_data_map == {0: [("str:a", 0.0), ("str:b", 1.0)], 1: [("str:a", 100.0), ("str:b", 99.0)]}
     with pytest.raises(ValueError, match="dual"):
         dm.accessibility(ns, {"a": "shop", "b": "shop"}, ["shop"], distances=[200], angular=True)
 
