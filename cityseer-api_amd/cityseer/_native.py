@@ -182,17 +182,17 @@ _SIGNATURES = {
     "cs_accessibility": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_uint32, _u32p, _i32p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, _f64p,
-         ctypes.c_uint32, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, _f64p, _f64p, _f64p],
+         ctypes.c_uint32, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, ctypes.c_int, _f64p, _f64p, _f64p],
     ),
     "cs_mixed_uses": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_uint32, _u32p, _i32p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, _f64p,
-         ctypes.c_uint32, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, _f64p],
+         ctypes.c_uint32, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, ctypes.c_int, _f64p],
     ),
     "cs_data_stats": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p, _u32p, _f64p, ctypes.c_uint32, _f32p,
-         ctypes.c_uint32, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, _f64p],
+         ctypes.c_uint32, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, ctypes.c_int, _f64p],
     ),
     "cs_tree_shortest": (
         ctypes.c_int,

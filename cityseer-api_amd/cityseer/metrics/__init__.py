@@ -1,5 +1,6 @@
-"""Centrality metrics front-end (drop-in for ``cityseer.metrics``)."""
+"""Metrics front-end (drop-in for ``cityseer.metrics``): network centrality and the land-use and
+statistical layers."""
 
-from . import networks
+from . import layers, networks
 
-__all__ = ["networks"]
+__all__ = ["layers", "networks"]

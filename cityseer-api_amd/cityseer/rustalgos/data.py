@@ -9,7 +9,9 @@ The assignment of entries to nodes is ``node_data_map``: ``{node index:
 [(data key, distance), ...]}``, the map ``assign_data_to_network`` leaves
 (data.rs:455-512), computed natively on the host (``cs_assign_data``: nearest
 street edges, barrier and street-crossing checks, graph.rs:1346-1535); a caller
-may also set it directly. The angular variants raise ``NotImplementedError``.
+may also set it directly. With ``angular`` the searches are simplest-path trees
+on a dual graph (data.rs:538-548): the reached nodes of each source come from
+a simplest-path reach kernel and the same aggregation runs over them.
 ``stats`` (data.rs:1013-1233)
 runs the same launch with each source's wave summarising the numeric values of
 the entries it reaches.
@@ -51,12 +53,22 @@ def clipped_beta_wt(beta: float, max_curve_wt: float, data_dist: float) -> float
 
 
 class DataEntry:
-    """One data point (data.rs:290-377): its key, dedupe key and WKT text."""
+    """One data point (data.rs:306-377): its key and dedupe key, both as given and as composite
+    strings, and its WKT text, parsed here so that a bad geometry fails at insertion with
+    ``ValueError`` as in the reference."""
 
-    def __init__(self, data_key: str, geom_wkt: str, dedupe_key: str | None = None):
-        self.data_key = data_key
-        self.dedupe_key = dedupe_key if dedupe_key is not None else data_key
+    def __init__(self, data_key_py: Any, geom_wkt: str, dedupe_key_py: Any = None):
+        from .._geom import parse_wkt
+
+        self.data_key_py = data_key_py
+        self.data_key = _composite_key(data_key_py)
+        self.dedupe_key_py = dedupe_key_py if dedupe_key_py is not None else data_key_py
+        self.dedupe_key = _composite_key(self.dedupe_key_py)
         self.geom_wkt = geom_wkt
+        try:
+            self.parts = parse_wkt(geom_wkt)
+        except ValueError as e:
+            raise ValueError(f"Failed to parse WKT for key '{self.data_key}': {e}") from None
 
 
 class LanduseAccess:
@@ -172,12 +184,12 @@ class DataMap:
         self.entries: dict[str, DataEntry] = {}
         self.node_data_map: dict[int, list[tuple[str, float]]] = {}
         self._progress = 0
+        self._running = None  # network of the call in progress
         self._node_bound: int | None = None  # of the network the current call runs on
 
     def insert(self, data_key_py: Any, geom_wkt: str, dedupe_key_py: Any = None) -> None:
-        key = _composite_key(data_key_py)
-        dd = None if dedupe_key_py is None else _composite_key(dedupe_key_py)
-        self.entries[key] = DataEntry(key, geom_wkt, dd)
+        entry = DataEntry(data_key_py, geom_wkt, dedupe_key_py)
+        self.entries[entry.data_key] = entry
 
     def entry_keys(self) -> list[str]:
         return list(self.entries.keys())
@@ -192,19 +204,68 @@ class DataMap:
         return not self.entries
 
     def progress(self) -> int:
-        return self._progress
+        """Sources finished by the running call (read live from its network's counter), or by
+        the last one."""
+        ns = self._running
+        return int(ns.progress()) if ns is not None else self._progress
 
     def progress_init(self) -> None:
         self._progress = 0
+
+    def _run(self, network_structure, call) -> None:
+        """Runs the native call with progress() following the network's live counter."""
+        self.progress_init()
+        self._running = network_structure
+        try:
+            _native.check(call())
+        finally:
+            self._running = None
+        self._progress = network_structure.node_bound()
+
+    def aggregate_to_src_idx(self, netw_src_idx: int, network_structure, max_walk_seconds: int, speed_m_s: float,
+                             angular: bool | None = None) -> dict[str, float]:
+        """{data key: network distance + data distance} of the entries one source reaches, each
+        dedupe group through its nearest member (data.rs:520-615): the source's shortest-path
+        tree (simplest-path with ``angular``) comes from the device, the walk over its visited
+        nodes runs here in f32 as the reference's does."""
+        if angular:
+            network_structure.validate_dual_for_angular("aggregate_to_src_idx")
+            order, visits = network_structure.dijkstra_tree_simplest(netw_src_idx, max_walk_seconds, speed_m_s)
+        else:
+            order, visits = network_structure.dijkstra_tree_shortest(netw_src_idx, max_walk_seconds, speed_m_s)
+        speed = np.float32(speed_m_s)
+        max_walk_dist = np.float32(np.float32(max_walk_seconds) * speed)
+        result: dict[str, float] = {}
+        nearest: dict[str, tuple[str, np.float32]] = {}
+        for node_idx in order:
+            agg = np.float32(visits[node_idx].agg_seconds)
+            if agg >= np.float32(max_walk_seconds):
+                continue
+            for data_key, data_dist in self.node_data_map.get(node_idx, []):
+                entry = self.entries.get(data_key)
+                if entry is None:
+                    continue
+                total = np.float32(np.float32(agg * speed) + np.float32(data_dist))
+                if not total <= max_walk_dist:
+                    continue
+                held = nearest.get(entry.dedupe_key)
+                if held is None:
+                    nearest[entry.dedupe_key] = (data_key, total)
+                    result[data_key] = float(total)
+                elif total < held[1]:
+                    result.pop(held[0], None)
+                    nearest[entry.dedupe_key] = (data_key, total)
+                    result[data_key] = float(total)
+        return result
 
     def assign_data_to_network(self, network_structure, max_assignment_dist: float, n_nearest_candidates: int):
         """Assign every entry to the end nodes of its nearest street edges whose line of sight
         crosses no barrier and no street (data.rs:455-512, graph.rs:1346-1535), natively on
         the host (``cs_assign_data``). Replaces ``node_data_map``."""
-        from .._geom import GeomArrays, parse_wkt
+        from .._geom import GeomArrays
 
         keys = list(self.entries.keys())
-        data = GeomArrays([parse_wkt(self.entries[k].geom_wkt) for k in keys])
+        data = GeomArrays([self.entries[k].parts for k in keys])
         bars = None if network_structure._barriers is None else GeomArrays(network_structure._barriers)
         src, dst, off, xy = network_structure._street_polylines()
         nxy = np.ascontiguousarray(np.stack([network_structure._x.array(), network_structure._y.array()], axis=1),
@@ -257,7 +318,6 @@ class DataMap:
 
         if angular:
             network_structure.validate_dual_for_angular("stats")
-            raise NotImplementedError("angular stats are not available here")
         speed = WALKING_SPEED if speed_m_s is None else float(speed_m_s)
         dists, bets, secs = pair_distances_betas_time(speed, distances, betas, minutes, min_threshold_wt)
         self._node_bound = network_structure.node_bound()
@@ -276,18 +336,15 @@ class DataMap:
         n, T = network_structure.node_bound(), len(dists)
         out = np.empty((len(numerical_maps), STAT_METRICS, T, n), dtype=np.float64)
         p = _native.ptr
-        self.progress_init()
-        _native.check(
-            _native.load().cs_data_stats(
-                network_structure._native_handle(), len(keys), p(entry_group, ctypes.c_uint32), len(a_node),
-                p(a_node, ctypes.c_uint32), p(a_ent, ctypes.c_uint32), p(a_dist, ctypes.c_double),
-                len(numerical_maps), p(values, ctypes.c_float), T,
-                p(np.asarray(dists, dtype=np.uint32), ctypes.c_uint32),
-                p(np.asarray(bets, dtype=np.float32), ctypes.c_float),
-                p(np.asarray(mcw, dtype=np.float32), ctypes.c_float), int(max(secs)), speed, p(out, ctypes.c_double),
-            )
-        )
-        self._progress = n
+        self._run(network_structure, lambda: _native.load().cs_data_stats(
+            network_structure._native_handle(), len(keys), p(entry_group, ctypes.c_uint32), len(a_node),
+            p(a_node, ctypes.c_uint32), p(a_ent, ctypes.c_uint32), p(a_dist, ctypes.c_double),
+            len(numerical_maps), p(values, ctypes.c_float), T,
+            p(np.asarray(dists, dtype=np.uint32), ctypes.c_uint32),
+            p(np.asarray(bets, dtype=np.float32), ctypes.c_float),
+            p(np.asarray(mcw, dtype=np.float32), ctypes.c_float), int(max(secs)), speed, int(bool(angular)),
+            p(out, ctypes.c_double),
+        ))
         idx = network_structure.node_indices()
         return StatsResult(dists, network_structure.node_keys_py(), idx, [Stats(dists, idx, out[m]) for m in
                                                                           range(len(numerical_maps))])
@@ -350,7 +407,6 @@ class DataMap:
 
         if angular:
             network_structure.validate_dual_for_angular("mixed_uses")
-            raise NotImplementedError("angular mixed uses are not available here")
         speed = WALKING_SPEED if speed_m_s is None else float(speed_m_s)
         dists, bets, secs = pair_distances_betas_time(speed, distances, betas, minutes, min_threshold_wt)
         lu = self._landuses(landuses_map)
@@ -368,18 +424,15 @@ class DataMap:
         n, T = network_structure.node_bound(), len(dists)
         rows = np.zeros((8, T, n), dtype=np.float64)
         p = _native.ptr
-        self.progress_init()
-        _native.check(
-            _native.load().cs_mixed_uses(
-                network_structure._native_handle(), len(keys), p(entry_group, ctypes.c_uint32),
-                p(entry_class, ctypes.c_int32), len(classes), len(a_node), p(a_node, ctypes.c_uint32),
-                p(a_ent, ctypes.c_uint32), p(a_dist, ctypes.c_double), T,
-                p(np.asarray(dists, dtype=np.uint32), ctypes.c_uint32),
-                p(np.asarray(bets, dtype=np.float32), ctypes.c_float),
-                p(np.asarray(mcw, dtype=np.float32), ctypes.c_float), int(max(secs)), speed, p(rows, ctypes.c_double),
-            )
-        )
-        self._progress = n
+        self._run(network_structure, lambda: _native.load().cs_mixed_uses(
+            network_structure._native_handle(), len(keys), p(entry_group, ctypes.c_uint32),
+            p(entry_class, ctypes.c_int32), len(classes), len(a_node), p(a_node, ctypes.c_uint32),
+            p(a_ent, ctypes.c_uint32), p(a_dist, ctypes.c_double), T,
+            p(np.asarray(dists, dtype=np.uint32), ctypes.c_uint32),
+            p(np.asarray(bets, dtype=np.float32), ctypes.c_float),
+            p(np.asarray(mcw, dtype=np.float32), ctypes.c_float), int(max(secs)), speed, int(bool(angular)),
+            p(rows, ctypes.c_double),
+        ))
         # measures not asked for stay zero (data.rs:866-941 only adds the requested ones)
         for on, ms in zip(flags, ((0, 1, 2), (3, 4, 5), (6,), (7,)), strict=True):
             if not on:
@@ -405,7 +458,6 @@ class DataMap:
 
         if angular:
             network_structure.validate_dual_for_angular("accessibility")
-            raise NotImplementedError("angular accessibility is not available here")
         speed = WALKING_SPEED if speed_m_s is None else float(speed_m_s)
         dists, bets, secs = pair_distances_betas_time(speed, distances, betas, minutes, min_threshold_wt)
         lu = self._landuses(landuses_map)
@@ -421,19 +473,16 @@ class DataMap:
         weighted = np.zeros((C, T, n), dtype=np.float64)
         nearest = np.full((C, n), np.nan, dtype=np.float64)
         p = _native.ptr
-        self.progress_init()
-        _native.check(
-            _native.load().cs_accessibility(
-                network_structure._native_handle(), len(keys), p(entry_group, ctypes.c_uint32),
-                p(entry_class, ctypes.c_int32), C, len(a_node), p(a_node, ctypes.c_uint32),
-                p(a_ent, ctypes.c_uint32), p(a_dist, ctypes.c_double), T,
-                p(np.asarray(dists, dtype=np.uint32), ctypes.c_uint32),
-                p(np.asarray(bets, dtype=np.float32), ctypes.c_float),
-                p(np.asarray(mcw, dtype=np.float32), ctypes.c_float), int(max(secs)), speed,
-                p(count, ctypes.c_double), p(weighted, ctypes.c_double), p(nearest, ctypes.c_double),
-            )
-        )
-        self._progress = n
+        self._run(network_structure, lambda: _native.load().cs_accessibility(
+            network_structure._native_handle(), len(keys), p(entry_group, ctypes.c_uint32),
+            p(entry_class, ctypes.c_int32), C, len(a_node), p(a_node, ctypes.c_uint32),
+            p(a_ent, ctypes.c_uint32), p(a_dist, ctypes.c_double), T,
+            p(np.asarray(dists, dtype=np.uint32), ctypes.c_uint32),
+            p(np.asarray(bets, dtype=np.float32), ctypes.c_float),
+            p(np.asarray(mcw, dtype=np.float32), ctypes.c_float), int(max(secs)), speed,
+            int(bool(angular)), p(count, ctypes.c_double), p(weighted, ctypes.c_double),
+            p(nearest, ctypes.c_double),
+        ))
         idx = network_structure.node_indices()
         lu_map = {k: LanduseAccess(dists, idx, count[c], weighted[c], nearest[c], max(dists))
                   for k, c in cls_index.items()}

@@ -127,6 +127,41 @@ def mock_data_gdf(nx_multigraph: nx.MultiGraph, length: int = 50, random_seed: i
     return df.set_index("uid")
 
 
+def mock_landuse_categorical_data(nx_multigraph: nx.MultiGraph, length: int = 50, num_classes: int = 10,
+                                  random_seed: int = 0):
+    """``mock_data_gdf``'s points with a ``categorical_landuses`` column (mock.py:314-359 of the
+    reference): letters drawn from the first ``num_classes - 1`` of the alphabet by the same
+    seeded generator, after the coordinates; the last five rows (one dedupe group) are "z"."""
+    import string
+
+    letters = list(string.ascii_lowercase)
+    if num_classes > len(letters):
+        raise ValueError(
+            f"The requested {num_classes} classes exceeds max available categorical classes of {len(letters)}"
+        )
+    np.random.seed(seed=random_seed)
+    df = mock_data_gdf(nx_multigraph, length=length, random_seed=random_seed)
+    letters = letters[: num_classes - 1]
+    codes = ["z" if i >= length - 5 else letters[int(np.random.randint(0, len(letters)))] for i in range(len(df))]
+    df["categorical_landuses"] = codes
+    return df
+
+
+def mock_numerical_data(nx_multigraph: nx.MultiGraph, length: int = 50, val_min: int = 0, val_max: int = 100000,
+                        num_arrs: int = 1, floating_pt: int = 3, random_seed: int = 0):
+    """``mock_data_gdf``'s points with ``mock_numerical_1`` .. ``mock_numerical_<num_arrs>``
+    columns (mock.py:362-403 of the reference): f32 integers in [val_min, val_max) from the same
+    seeded generator, divided by 10**floating_pt; the last five rows take the column's maximum."""
+    np.random.seed(seed=random_seed)
+    df = mock_data_gdf(nx_multigraph, length=length, random_seed=random_seed)
+    for i in range(1, num_arrs + 1):
+        arr = np.array(np.random.randint(val_min, high=val_max, size=length), dtype=np.float32)
+        arr /= 10**floating_pt
+        arr[-5:] = np.nanmax(arr)
+        df[f"mock_numerical_{i}"] = arr
+    return df
+
+
 def mock_data_map(data_gdf, dedupe_key_col: str | None = None):
     """A ``DataMap`` of a frame like ``mock_data_gdf``'s (mock.py:540-560 of the reference);
     geometries may be WKT strings or objects with a ``wkt`` attribute."""

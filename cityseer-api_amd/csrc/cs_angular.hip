@@ -702,6 +702,141 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
   *a.t_norder = no;
 }
 
+// dijkstra_tree_simplest (centrality.rs:1202-1332) for many sources, one per
+// lane, as angular data aggregation walks it (data.rs:538-548): per source the
+// nodes it reaches in first-reach order (visited_nodes) with the seconds of
+// their collapsed node view, written as {node | seconds << 32} to the item's
+// list of ncap entries, and their number to counts (kNoList when a capacity ran
+// out: the item is listed in fail_list and searched again with more room).
+// State records keep route, seconds, state and the simplest distance (in the
+// predecessor-count word); node records node, simplest distance and seconds.
+__global__ __launch_bounds__(kBlock) void k_reach_simplest(AngArgs a, const uint32_t *items, uint64_t *lists,
+                                                           uint32_t *counts) {
+  const uint32_t lane = __lane_id();
+  const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t wid = __builtin_amdgcn_readfirstlane(gl / kWave);
+  LaneWs L;
+  L.skey = reinterpret_cast<uint64_t *>(a.ws + ((size_t)wid * kWave + lane) * a.lane_stride);
+  L.nkey = L.skey + 2 * (size_t)a.N;
+  L.w = a.wave_ws + (size_t)wid * a.wave_stride;
+  L.lane = lane;
+  char *w = L.w;
+  CAngJob &jb = ((CAngJob *)a.jobs)[0];
+  const float eps = kTieEps;
+  unsigned long long st_reached = 0, st_states = 0;
+  const uint32_t lanes_total = gridDim.x * blockDim.x;
+  for (uint32_t it = gl; it < a.n_items; it += lanes_total) {
+    const uint32_t src = jb.sources[items ? items[it] : it];
+    const uint32_t epoch = a.epoch_base + it + 1;
+    uint32_t n_states = 0, n_nodes = 1, fail = 0;
+    Heap h{w, a.o_heap, lane, 0, a.hcap};
+    L.nkey[src] = ((uint64_t)epoch << 32) | 0u;
+    *nr<uint32_t>(w, a.o_nrec, 0, lane, a.ncap, kN_node) = src;
+    *nr<float>(w, a.o_nrec, 0, lane, a.ncap, kN_brt) = 0.0f;
+    *nr<float>(w, a.o_nrec, 0, lane, a.ncap, kN_bsec) = 0.0f;
+    for (uint32_t slot = 0; slot < 2; ++slot) {
+      const uint32_t s = 2 * src + slot;
+      const uint32_t ss = n_states++;
+      L.skey[s] = ((uint64_t)epoch << 32) | ss;
+      *sr<float>(w, a.o_srec, ss, lane, a.scap, kS_route) = 0.0f;
+      *sr<float>(w, a.o_srec, ss, lane, a.scap, kS_secs) = 0.0f;
+      *sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_state) = s;
+      *sr<float>(w, a.o_srec, ss, lane, a.scap, kS_np) = 0.0f;
+      h.push(0.0f, 0.0f, s, ss);
+    }
+    while (h.n > 0 && !fail) {
+      const uint4 top = h.pop();
+      const uint32_t us = top.z, uss = top.w;
+      uint32_t *ust = sr<uint32_t>(w, a.o_srec, uss, lane, a.scap, kS_state);
+      if (*ust & 0x80000000u) continue;
+      *ust = us | 0x80000000u;
+      const uint32_t u = us >> 1, entry = us & 1u;
+      const float u_secs = *sr<float>(w, a.o_srec, uss, lane, a.scap, kS_secs);
+      const float u_simpl = *sr<float>(w, a.o_srec, uss, lane, a.scap, kS_np);
+      for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
+        const uint4 e = jb.adj[j];
+        if (((e.x >> 30) & 1u) != 1u - entry) continue;
+        const uint32_t v = e.x & 0x3fffffffu;
+        const uint32_t vs = 2 * v + (e.x >> 31);
+        const float cs = u_secs + __uint_as_float(e.y);
+        if (cs > jb.maxsec) continue;
+        const float csimpl = u_simpl + __uint_as_float(e.z);
+        const float cmetric = csimpl + __uint_as_float(e.w);
+        const uint64_t sk = L.skey[vs];
+        uint32_t vss = 0xffffffffu;
+        float vmetric = INFINITY, vsecs = INFINITY;
+        if ((uint32_t)(sk >> 32) == epoch) {
+          vss = (uint32_t)sk;
+          vmetric = *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_route);
+          vsecs = *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_secs);
+        }
+        const bool improved = cmetric + eps < vmetric;
+        const bool tied = fabsf(cmetric - vmetric) <= eps;
+        if (!(improved || (tied && cs < vsecs && us != vs))) continue;
+        if (vss == 0xffffffffu) {
+          if (n_states >= a.scap) {
+            fail |= kFailCap;
+            break;
+          }
+          vss = n_states++;
+          L.skey[vs] = ((uint64_t)epoch << 32) | vss;
+          *sr<uint32_t>(w, a.o_srec, vss, lane, a.scap, kS_state) = vs;
+        }
+        *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_route) = cmetric;
+        *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_secs) = cs;
+        *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_np) = csimpl;
+        if (h.n >= a.hcap) {
+          fail |= kFailHeap;
+          break;
+        }
+        h.push(cmetric, cs, vs, vss);
+        // collapsed node view (centrality.rs:1307-1326); a node's slot is its
+        // position in visited_nodes
+        const uint64_t nk = L.nkey[v];
+        if ((uint32_t)(nk >> 32) != epoch) {
+          if (n_nodes >= a.ncap) {
+            fail |= kFailCap;
+            break;
+          }
+          const uint32_t ns = n_nodes++;
+          L.nkey[v] = ((uint64_t)epoch << 32) | ns;
+          *nr<uint32_t>(w, a.o_nrec, ns, lane, a.ncap, kN_node) = v;
+          *nr<float>(w, a.o_nrec, ns, lane, a.ncap, kN_brt) = csimpl;
+          *nr<float>(w, a.o_nrec, ns, lane, a.ncap, kN_bsec) = cs;
+        } else {
+          const uint32_t ns = (uint32_t)nk;
+          float *nsimpl = nr<float>(w, a.o_nrec, ns, lane, a.ncap, kN_brt);
+          float *nsecs = nr<float>(w, a.o_nrec, ns, lane, a.ncap, kN_bsec);
+          const bool nimp = csimpl + eps < *nsimpl;
+          const bool ntied = fabsf(csimpl - *nsimpl) <= eps;
+          if (nimp || (ntied && cs < *nsecs)) {
+            *nsimpl = csimpl;
+            *nsecs = cs;
+          }
+        }
+      }
+    }
+    if (fail) {
+      counts[it] = kNoList;
+      const uint32_t q = atomicAdd(a.n_fail, 1u);
+      a.fail_list[q] = it;
+      atomicOr(a.fail_flags, fail);
+      continue;
+    }
+    uint64_t *lst = lists + (size_t)it * a.ncap;
+    for (uint32_t k = 0; k < n_nodes; ++k) {
+      const uint32_t v = *nr<uint32_t>(w, a.o_nrec, k, lane, a.ncap, kN_node);
+      const float secs = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_bsec);
+      lst[k] = (uint64_t)v | ((uint64_t)__float_as_uint(secs) << 32);
+    }
+    counts[it] = n_nodes;
+    st_reached += n_nodes;
+    st_states += n_states;
+  }
+  atomicAdd(&a.stats[0], st_reached);
+  atomicAdd(&a.stats[1], st_states);
+}
+
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct AngPlan {
@@ -1101,6 +1236,104 @@ void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, fl
   (void)hipFree(ctl);
   (void)hipFree(d_job);
   if (hctl[2]) fail(CS_EINTERNAL, "angular tree workspace exceeded");
+}
+
+static_assert(kFailCap == kReachFailCap && kFailHeap == kReachFailHeap, "reach failure bits");
+
+std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de, float maxsec,
+                                            const uint32_t *d_src, const uint32_t *d_items, uint32_t n_items,
+                                            uint32_t cap, uint32_t hcap, uint64_t *d_lists, uint32_t *d_counts,
+                                            hipStream_t stream, cs_stats &st, uint32_t *fail_bits) {
+  *fail_bits = 0;
+  if (n_items == 0) return {};
+  const uint32_t N = g.n;
+  if (N >= (1u << 30)) fail(CS_EINVAL, "dual graph too large for the angular search (node ids need 30 bits)");
+  const int dev = dg.device;
+  AngJob job;
+  std::memset(&job, 0, sizeof(job));
+  job.N = N;
+  job.off = dg.fwd_off;
+  job.adj = angular_adjacency(g, de);
+  job.maxsec = maxsec;
+  job.sources = d_src;
+  // state records, node records and the heap of each lane, interleaved per wave
+  AngPlan p;
+  p.per_wave = kWave;
+  p.scap = std::min<uint32_t>(2 * N, 2 * cap);
+  p.ncap = cap;
+  p.hcap = hcap;
+  p.lane_stride = align256(8ull * 3 * N);
+  p.o_srec = 0;
+  p.o_nrec = align256((size_t)kSRec * p.scap * kWave);
+  p.o_heap = p.o_nrec + align256((size_t)kNRec * p.ncap * kWave);
+  p.wave_stride = p.o_heap + align256((size_t)16 * p.hcap * kWave);
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, dev), "props");
+  size_t free_b = 0, total_b = 0;
+  hip_check(hipMemGetInfo(&free_b, &total_b), "meminfo");
+  AngWorkspace &wsp = ang_ws(dev);
+  std::lock_guard<std::mutex> lk(wsp.mu);
+  const size_t wave_bytes = p.lane_stride * kWave + p.wave_stride;
+  const size_t budget = free_b / 4 + wsp.lane_bytes + wsp.wave_bytes;
+  const size_t wpb = kBlock / kWave;
+  size_t waves = std::min<size_t>(budget / wave_bytes, ((size_t)n_items + kWave - 1) / kWave);
+  waves = std::min<size_t>(waves, (size_t)prop.multiProcessorCount * 4 * 8);
+  waves = (waves + wpb - 1) / wpb * wpb;
+  if (waves == 0 || waves * wave_bytes > budget + wave_bytes * wpb)
+    fail(CS_ENOMEM, "graph too large for the angular search workspace on this device");
+  p.lanes = (uint32_t)(waves * kWave);
+  const uint32_t epoch_base = prepare_ang(wsp, p, n_items, stream);
+  DevMem<uint32_t> d_ctl(4), d_fail(n_items);
+  DevMem<unsigned long long> d_stats(4);
+  DevMem<AngJob> d_job(1);
+  hip_check(hipMemsetAsync(d_ctl.p, 0, 16, stream), "memset");
+  hip_check(hipMemsetAsync(d_stats.p, 0, 32, stream), "memset");
+  hip_check(hipMemcpyAsync(d_job.p, &job, sizeof(AngJob), hipMemcpyHostToDevice, stream), "copy");
+  AngArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.N = N;
+  a.jobs = d_job.p;
+  a.n_items = n_items;
+  a.stats = d_stats.p;
+  a.ws = wsp.lane_ws;
+  a.wave_ws = wsp.wave_ws;
+  a.lane_stride = p.lane_stride;
+  a.wave_stride = p.wave_stride;
+  a.scap = p.scap;
+  a.ncap = p.ncap;
+  a.hcap = p.hcap;
+  a.o_srec = p.o_srec;
+  a.o_nrec = p.o_nrec;
+  a.o_heap = p.o_heap;
+  a.per_wave = kWave;
+  a.epoch_base = epoch_base;
+  a.n_fail = d_ctl.p + 1;
+  a.fail_flags = d_ctl.p + 2;
+  a.fail_list = d_fail.p;
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  hip_check(hipEventRecord(e0, stream), "event");
+  hipLaunchKernelGGL(k_reach_simplest, dim3(p.lanes / kBlock), dim3(kBlock), 0, stream, a, d_items, d_lists, d_counts);
+  hip_check(hipGetLastError(), "k_reach_simplest launch");
+  hip_check(hipEventRecord(e1, stream), "event");
+  uint32_t ctl[4];
+  unsigned long long stv[4];
+  hip_check(hipMemcpyAsync(ctl, d_ctl.p, 16, hipMemcpyDeviceToHost, stream), "copy");
+  hip_check(hipMemcpyAsync(stv, d_stats.p, 32, hipMemcpyDeviceToHost, stream), "copy");
+  hip_check(hipStreamSynchronize(stream), "sync");
+  float ms = 0.0f;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  st.kernel_ms += ms;
+  st.nodes_reached += stv[0];
+  st.edges_scanned += stv[1];
+  st.waves = std::max<uint64_t>(st.waves, waves);
+  *fail_bits = ctl[2];
+  std::vector<uint32_t> failed(ctl[1]);
+  if (ctl[1]) hip_check(hipMemcpy(failed.data(), d_fail.p, 4ull * ctl[1], hipMemcpyDeviceToHost), "copy");
+  return failed;
 }
 
 void release_angular_workspaces() {

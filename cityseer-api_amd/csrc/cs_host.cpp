@@ -1234,8 +1234,9 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
                        uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
                        const uint32_t *assign_entry, const double *assign_dist, uint32_t n_thresholds,
                        const uint32_t *distances, const float *betas, const float *max_curve_wts,
-                       uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
-                       double *out_nearest, double *out_div, const StatsIn *stats_in = nullptr) {
+                       uint32_t max_seconds, float speed_m_s, int angular, double *out_count,
+                       double *out_weighted, double *out_nearest, double *out_div,
+                       const StatsIn *stats_in = nullptr) {
   const auto t0 = std::chrono::steady_clock::now();
   t_stats = cs_stats{};
   int rc = guarded([&] {
@@ -1252,6 +1253,14 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     if (!out_div && !stats_in && (size_t)C * N && (!out_count || !out_weighted || !out_nearest))
       fail(CS_EINVAL, "outputs must not be NULL");
     if (!std::isfinite(speed_m_s) || !(speed_m_s > 0.0f)) fail(CS_EINVAL, "speed_m_s must be finite and positive");
+    if (angular) {
+      // validate_dual_for_angular (graph.rs:1333-1341), then the endpoint slots the search needs
+      if (!g.is_dual)
+        fail(CS_EINVAL, std::string(out_div ? "mixed_uses" : stats_in ? "stats" : "accessibility") +
+                            " requires a dual graph for angular analysis. Convert the graph with "
+                            "cityseer.tools.graphs.nx_to_dual(...) before ingesting it into NetworkStructure.");
+      if (!g.slots_ok) fail(CS_EINVAL, g.slots_err);
+    }
     for (uint32_t t = 0; t < T; ++t)
       if (!(max_curve_wts[t] >= 0.0f && max_curve_wts[t] <= 1.0f) || max_curve_wts[t] == 0.0f)
         fail(CS_EINVAL, "Max curve weight must be in the range [0, 1].");
@@ -1387,10 +1396,54 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
     ao.weighted = out_div ? nullptr : d_wt.as<double>();
     ao.nearest = d_near.as<unsigned long long>();
     uint32_t *prog = progress_begin(g, N - srcs.size());
-    if (!srcs.empty())
+    if (!srcs.empty() && !angular)
       for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
         launch_access(g, dg, de, (float)max_seconds, d_src.as<uint32_t>(), (uint32_t)srcs.size(), chunk(t0), stream,
                       t_stats, t0 + kMaxT >= T ? prog : nullptr);
+    if (!srcs.empty() && angular) {
+      // angular aggregation walks dijkstra_tree_simplest (data.rs:538-548):
+      // each source's reached nodes come from the simplest-path reach kernel,
+      // then the same aggregation runs over those lists. Sources go in rounds:
+      // a round's lists hold `cap` nodes per source, and sources that reach
+      // more (or fill the heap) go to the next round with four times the room,
+      // up to the whole graph (2 M + 2 heap entries: one push per relaxation).
+      const uint32_t hmax = (uint32_t)std::min<uint64_t>(2ull * g.m + 2, 0xfffffff0ull);
+      // the tuning's slot capacity (search states per lane) bounds the first round's lists at half as many nodes
+      const uint32_t cap0 = default_tuning().cap;
+      uint32_t cap = std::min<uint32_t>(N, cap0 ? std::max<uint32_t>(cap0 / 2, 1u) : 4096u);
+      uint32_t hcap = (uint32_t)std::min<uint64_t>(hmax, 4ull * cap);
+      std::vector<uint32_t> pending(srcs.size());
+      for (uint32_t i = 0; i < (uint32_t)pending.size(); ++i) pending[i] = i;
+      while (!pending.empty()) {
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "meminfo");
+        const size_t per = 8ull * cap;
+        const size_t chunk_n = std::max<size_t>(64, std::min<size_t>(pending.size(), free_b / 8 / per));
+        DevBuf d_items(4 * chunk_n), d_counts(4 * chunk_n), d_lists(per * chunk_n);
+        std::vector<uint32_t> next;
+        uint32_t bits = 0;
+        for (size_t c0 = 0; c0 < pending.size(); c0 += chunk_n) {
+          const uint32_t n = (uint32_t)std::min<size_t>(chunk_n, pending.size() - c0);
+          hip_check(hipMemcpyAsync(d_items.p, pending.data() + c0, 4ull * n, hipMemcpyHostToDevice, stream), "copy");
+          uint32_t b = 0;
+          for (uint32_t f : launch_reach_simplest(g, dg, de, (float)max_seconds, d_src.as<uint32_t>(),
+                                                  d_items.as<uint32_t>(), n, cap, hcap, d_lists.as<uint64_t>(),
+                                                  d_counts.as<uint32_t>(), stream, t_stats, &b))
+            next.push_back(pending[c0 + f]);
+          bits |= b;
+          for (uint32_t t0 = 0; t0 < T; t0 += kMaxT)
+            launch_access_lists(chunk(t0), d_src.as<uint32_t>(), d_items.as<uint32_t>(), d_lists.as<uint64_t>(),
+                                d_counts.as<uint32_t>(), n, cap, (float)max_seconds, speed_m_s, stream, t_stats,
+                                t0 + kMaxT >= T ? prog : nullptr);
+        }
+        if (next.empty()) break;
+        if (cap >= N && hcap >= hmax) fail(CS_EINTERNAL, "angular reach lists: capacity retries exhausted");
+        t_stats.retries += next.size();
+        if (bits & kReachFailCap) cap = (uint32_t)std::min<uint64_t>(N, 4ull * cap);
+        hcap = (uint32_t)std::min<uint64_t>(hmax, std::max<uint64_t>(4ull * cap, (bits & kReachFailHeap) ? 4ull * hcap : hcap));
+        pending.swap(next);
+      }
+    }
     const uint32_t *ip = result_order(g);
     if (stats_in) {
       std::vector<double> hs((size_t)SM * T * N);
@@ -1432,36 +1485,36 @@ int cs_accessibility(cs_graph *h, uint32_t n_entries, const uint32_t *entry_grou
                      uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
                      const uint32_t *assign_entry, const double *assign_dist, uint32_t n_thresholds,
                      const uint32_t *distances, const float *betas, const float *max_curve_wts,
-                     uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
-                     double *out_nearest) {
+                     uint32_t max_seconds, float speed_m_s, int angular, double *out_count,
+                     double *out_weighted, double *out_nearest) {
   return access_impl(h, n_entries, entry_group, entry_class, n_classes, n_assign, assign_node, assign_entry,
-                     assign_dist, n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, out_count,
-                     out_weighted, out_nearest, nullptr);
+                     assign_dist, n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, angular,
+                     out_count, out_weighted, out_nearest, nullptr);
 }
 
 int cs_mixed_uses(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
                   uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node, const uint32_t *assign_entry,
                   const double *assign_dist, uint32_t n_thresholds, const uint32_t *distances, const float *betas,
-                  const float *max_curve_wts, uint32_t max_seconds, float speed_m_s, double *out) {
+                  const float *max_curve_wts, uint32_t max_seconds, float speed_m_s, int angular, double *out) {
   if (!out) return guarded([] { fail(CS_EINVAL, "out must not be NULL"); });
   return access_impl(h, n_entries, entry_group, entry_class, n_classes, n_assign, assign_node, assign_entry,
-                     assign_dist, n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, nullptr,
-                     nullptr, nullptr, out);
+                     assign_dist, n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, angular,
+                     nullptr, nullptr, nullptr, out);
 }
 
 int cs_data_stats(cs_graph *h, uint32_t n_entries, const uint32_t *entry_group, uint32_t n_assign,
                   const uint32_t *assign_node, const uint32_t *assign_entry, const double *assign_dist,
                   uint32_t n_maps, const float *values, uint32_t n_thresholds, const uint32_t *distances,
                   const float *betas, const float *max_curve_wts, uint32_t max_seconds, float speed_m_s,
-                  double *out) {
+                  int angular, double *out) {
   if (!out) return guarded([] { fail(CS_EINVAL, "out must not be NULL"); });
   StatsIn st;
   st.n_maps = n_maps;
   st.values = values;
   st.out = out;
   return access_impl(h, n_entries, entry_group, nullptr, 0, n_assign, assign_node, assign_entry, assign_dist,
-                     n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, nullptr, nullptr, nullptr,
-                     nullptr, &st);
+                     n_thresholds, distances, betas, max_curve_wts, max_seconds, speed_m_s, angular, nullptr, nullptr,
+                     nullptr, nullptr, &st);
 }
 
 int cs_tree_shortest(cs_graph *h, uint32_t src, uint32_t max_seconds, float speed, uint8_t *visited,

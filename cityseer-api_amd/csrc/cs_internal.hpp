@@ -34,6 +34,16 @@ struct Error {
 [[noreturn]] void fail(int code, const std::string &msg);
 void hip_check(hipError_t e, const char *what);
 
+// Device memory of n elements owned by one scope (freed on exit, errors included).
+template <typename T>
+struct DevMem {
+  T *p = nullptr;
+  explicit DevMem(size_t n) { hip_check(hipMalloc(&p, sizeof(T) * (n ? n : 1)), "hipMalloc"); }
+  ~DevMem() { (void)hipFree(p); }
+  DevMem(const DevMem &) = delete;
+  DevMem &operator=(const DevMem &) = delete;
+};
+
 // Speed-dependent device edge arrays for one (device, speed).
 struct DevEdges {
   float speed = 0.0f;
@@ -339,5 +349,27 @@ void launch_simplest(Graph &g, DevGraph &dg, DevEdges &de, const CallSpec &cs,
 void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, float maxsec, float *d_simpl,
                           float *d_secs, int32_t *d_pred, uint8_t *d_vis, uint32_t *d_order, uint32_t *d_norder,
                           hipStream_t stream);
+
+// Reached-node lists of many simplest-path trees (cs_angular.hip), for angular
+// data aggregation: item i searches from d_src[d_items[i]] (d_src[i] without
+// d_items) and leaves {node | seconds << 32} of the nodes it reaches, in
+// first-reach order, in d_lists[i * cap ...] and their number in d_counts[i].
+// Items whose search ran out of room (more than cap nodes or 2 cap states, or
+// more than hcap heap entries) get kNoList; their positions are returned, the
+// causes (kReachFailCap, kReachFailHeap) in *fail_bits.
+constexpr uint32_t kNoList = 0xffffffffu;
+constexpr uint32_t kReachFailCap = 1u, kReachFailHeap = 2u;
+std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de, float maxsec,
+                                            const uint32_t *d_src, const uint32_t *d_items, uint32_t n_items,
+                                            uint32_t cap, uint32_t hcap, uint64_t *d_lists, uint32_t *d_counts,
+                                            hipStream_t stream, cs_stats &st, uint32_t *fail_bits);
+
+// Data aggregation (accessibility rows, class counts or statistics, as `out`
+// asks) over reached-node lists (cs_search.hip): item i is source
+// d_src[d_items[i]] with the list d_lists[i * lcap ...] of d_counts[i] entries;
+// items whose count is kNoList are skipped.
+void launch_access_lists(const AccessOut &out, const uint32_t *d_src, const uint32_t *d_items,
+                         const uint64_t *d_lists, const uint32_t *d_counts, uint32_t n_items, uint32_t lcap,
+                         float maxsec, float speed, hipStream_t stream, cs_stats &st, uint32_t *progress);
 
 }  // namespace cs

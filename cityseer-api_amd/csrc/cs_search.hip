@@ -1324,6 +1324,83 @@ __device__ void source_stats(const AccessOut &ao, uint32_t src, uint64_t *win, u
   }
 }
 
+// ---- data entries reachable from a source (data.rs:520-615, 680-741), one
+// wave. reach(s) gives the source's s-th reached node as {node | seconds << 32}
+// (s < R). A member (entry e at node v, data distance dd) counts when v lies
+// below the walk limit, secs_v * speed + dd is within the largest distance and
+// no other member of its dedupe group is nearer (equal totals: the lower member
+// index wins). Groups of several members keep two epoch-tagged words each in
+// grec: the nearest total (pass 1), then the lowest member index at that total
+// (pass 2), so each group counts once, through its nearest member, in passes
+// linear in the members reached. `tag` is the source's epoch in the top bits,
+// above every tag grec holds from earlier sources. Counts go to the rows of
+// `ao`; with ao.stats the winners go to the list at stb (count, then up to
+// st_cap winners and the scratch of source_stats) and become statistics rows.
+// Returns false when a statistics source has more winners than st_cap.
+template <typename Reach>
+__device__ bool access_fold(const AccessOut &ao, uint32_t src, uint32_t R, const Reach &reach, float maxsec,
+                            float speed, uint64_t *grec, uint64_t tag, char *stb, uint32_t st_cap) {
+  const uint32_t lane = lane_id();
+  uint32_t *st_n = reinterpret_cast<uint32_t *>(stb);
+  uint64_t *st_win = reinterpret_cast<uint64_t *>(stb + 256);
+  if (ao.stats && lane == 0) st32(st_n, 0u);
+  for (int pass = 0; pass < 3; ++pass) {
+    if (pass < 2 && ao.n_shared == 0) continue;
+    for (uint32_t s = lane; s < R; s += kWave) {
+      const uint64_t lo = reach(s);
+      const uint32_t v = (uint32_t)lo;
+      const float secs = __uint_as_float((uint32_t)(lo >> 32));
+      if (!(secs < maxsec)) continue;
+      const float net = __fmul_rn(secs, speed);
+      for (uint32_t k = ao.node_off[v]; k < ao.node_off[v + 1]; ++k) {
+        const uint32_t m = ao.node_mem[k];
+        const uint4 me = ao.mem[m];
+        const float tot = __fadd_rn(net, __uint_as_float(me.z));
+        if (!(tot <= ao.maxdist)) continue;
+        const bool shared = me.w != kSoloGroup;
+        const uint64_t tkey = tag | ((uint64_t)(uint32_t)~__float_as_uint(tot) << 22);  // larger = nearer
+        if (pass == 0) {
+          if (shared) wmax64(&grec[2 * (size_t)me.w], tkey);
+          continue;
+        }
+        if (pass == 1) {
+          if (shared && ld64(&grec[2 * (size_t)me.w]) == tkey) wmax64(&grec[2 * (size_t)me.w + 1], tag | (uint32_t)~m);
+          continue;
+        }
+        if (shared && ld64(&grec[2 * (size_t)me.w + 1]) != (tag | (uint32_t)~m)) continue;
+        if (ao.stats) {
+          const uint32_t q = __hip_atomic_fetch_add(st_n, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (q < st_cap) st64(&st_win[q], ((uint64_t)me.y << 32) | __float_as_uint(tot));
+          continue;
+        }
+        const int32_t cls = ao.entry_class[me.y];
+        if (cls < 0) continue;
+        for (uint32_t t = 0; t < ao.T; ++t) {
+          if (!(tot <= ao.dist[t])) continue;
+          const size_t row = ((size_t)cls * ao.tstride + t) * ao.N + src;
+          gadd_f64(&ao.count[row], 1.0);
+          if (ao.weighted) {  // mixed uses read only the counts
+            // clipped_beta_wt (common.rs:339-348): min(exp(-beta d), mcw) / mcw in f32
+            const float raw = exp_f32(__fmul_rn(-ao.beta[t], tot));
+            gadd_f64(&ao.weighted[row], (double)__fdiv_rn(fminf(raw, ao.mcw[t]), ao.mcw[t]));
+          }
+          if (ao.max_mask & (1u << t))
+            atomicMin(&ao.nearest[(size_t)cls * ao.N + src], (unsigned long long)__double_as_longlong((double)tot));
+        }
+      }
+    }
+    wave_fence();
+  }
+  if (ao.stats) {
+    const uint32_t nw = uniform(ld32(st_n));
+    if (nw > st_cap) return false;
+    uint64_t *st_keys = st_win + 2 * (size_t)st_cap;
+    float *st_vals = reinterpret_cast<float *>(st_keys + 2 * (size_t)st_cap);
+    source_stats(ao, src, st_win, nw, st_keys, st_vals, st_vals + st_cap);
+  }
+  return true;
+}
+
 // With up to four thresholds the search asks for five resident waves per SIMD
 // (96 VGPRs, a few spilled to scratch in cold paths): 45.2 ms against 45.6 ms at
 // four on the 200x200 workload; six waves spill in hot paths and run slower.
@@ -1657,88 +1734,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NT <= 4 
         }
       }
     } else if constexpr (MODE == kAccess) {
-      // ---- data entries reachable from the source (data.rs:520-615, 680-741).
-      // A member (entry e at node v, data distance dd) counts when v lies
-      // below the walk limit, secs_v * speed + dd is within the largest
-      // distance and no other member of its dedupe group is nearer (equal
-      // totals: the lower member index wins). Groups of several members keep,
-      // per wave, two epoch-tagged words after the node records: the nearest
-      // total (pass 1), then the lowest member index at that total (pass 2), so
-      // each group counts once, through its nearest member, in passes linear in
-      // the members reached.
-      const AccessOut &ao = a.acc;
-      const uint32_t src = c.src;
       // group g owns records nacc + 2g and nacc + 2g + 1 (node keys to earlier
       // sources): whatever those left there, keys of a larger graph included,
       // carries an older epoch and compares below the current tag
       uint64_t *grec = w.node + (size_t)j.nacc;
-      const uint64_t tag = (uint64_t)c.tag << kEpochShift;
-      // statistics: the winners go to a list in the wave's scratch instead of the counts
       char *stb = a.ws + (size_t)wid * a.ws_stride + a.o_st;
-      uint32_t *st_n = reinterpret_cast<uint32_t *>(stb);
-      uint64_t *st_win = reinterpret_cast<uint64_t *>(stb + 256);
-      if (ao.stats && lane == 0) st32(st_n, 0u);
-      for (int pass = 0; pass < 3; ++pass) {
-        if (pass < 2 && ao.n_shared == 0) continue;
-        for (uint32_t s = lane; s < R; s += kWave) {
-          const uint64_t lo = ld64(&w.rec[2 * (size_t)s]);
-          const uint32_t v = (uint32_t)lo;
-          const float secs = __uint_as_float((uint32_t)(lo >> 32));
-          if (!(secs < j.maxsec)) continue;
-          const float net = __fmul_rn(secs, speed);
-          for (uint32_t k = ao.node_off[v]; k < ao.node_off[v + 1]; ++k) {
-            const uint32_t m = ao.node_mem[k];
-            const uint4 me = ao.mem[m];
-            const float tot = __fadd_rn(net, __uint_as_float(me.z));
-            if (!(tot <= ao.maxdist)) continue;
-            const bool shared = me.w != kSoloGroup;
-            const uint64_t tkey = tag | ((uint64_t)(uint32_t)~__float_as_uint(tot) << 22);  // larger = nearer
-            if (pass == 0) {
-              if (shared) wmax64(&grec[2 * (size_t)me.w], tkey);
-              continue;
-            }
-            if (pass == 1) {
-              if (shared && ld64(&grec[2 * (size_t)me.w]) == tkey) wmax64(&grec[2 * (size_t)me.w + 1], tag | (uint32_t)~m);
-              continue;
-            }
-            if (shared && ld64(&grec[2 * (size_t)me.w + 1]) != (tag | (uint32_t)~m)) continue;
-            if (ao.stats) {
-              const uint32_t q = __hip_atomic_fetch_add(st_n, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-              if (q < a.st_cap) st64(&st_win[q], ((uint64_t)me.y << 32) | __float_as_uint(tot));
-              continue;
-            }
-            const int32_t cls = ao.entry_class[me.y];
-            if (cls < 0) continue;
-            for (uint32_t t = 0; t < ao.T; ++t) {
-              if (!(tot <= ao.dist[t])) continue;
-              const size_t row = ((size_t)cls * ao.tstride + t) * ao.N + src;
-              gadd_f64(&ao.count[row], 1.0);
-              if (ao.weighted) {  // mixed uses read only the counts
-                // clipped_beta_wt (common.rs:339-348): min(exp(-beta d), mcw) / mcw in f32
-                const float raw = exp_f32(__fmul_rn(-ao.beta[t], tot));
-                gadd_f64(&ao.weighted[row], (double)__fdiv_rn(fminf(raw, ao.mcw[t]), ao.mcw[t]));
-              }
-              if (ao.max_mask & (1u << t))
-                atomicMin(&ao.nearest[(size_t)cls * ao.N + src], (unsigned long long)__double_as_longlong((double)tot));
-            }
-          }
+      const auto reach = [&](uint32_t s) { return ld64(&w.rec[2 * (size_t)s]); };
+      if (!access_fold(a.acc, c.src, R, reach, j.maxsec, speed, grec, (uint64_t)c.tag << kEpochShift, stb,
+                       a.st_cap)) {
+        // more winners than the list holds: the source runs again with longer lists
+        if (lane == 0) {
+          const uint32_t q = atomicAdd(a.n_fail, 1u);
+          a.fail_list[q] = item;
+          atomicOr(a.fail_flags, kFailCap);
         }
-        wave_fence();
-      }
-      if (ao.stats) {
-        const uint32_t nw = uniform(ld32(st_n));
-        if (nw > a.st_cap) {
-          // more winners than the list holds: the source runs again with longer lists
-          if (lane == 0) {
-            const uint32_t q = atomicAdd(a.n_fail, 1u);
-            a.fail_list[q] = item;
-            atomicOr(a.fail_flags, kFailCap);
-          }
-          continue;
-        }
-        uint64_t *st_keys = st_win + 2 * (size_t)a.st_cap;
-        float *st_vals = reinterpret_cast<float *>(st_keys + 2 * (size_t)a.st_cap);
-        source_stats(ao, src, st_win, nw, st_keys, st_vals, st_vals + a.st_cap);
+        continue;
       }
     } else {
       // single-source exports (trees / Brandes state)
@@ -2883,6 +2893,133 @@ void launch_access(Graph &g, DevGraph &dg, DevEdges &de, float maxsec, const uin
   JobSpec js{&g, &dg, &de, cs, d_sources, nullptr, nullptr, n_sources, nullptr, nullptr, nullptr, tune.delta_scale};
   js.extra_records = 2 * out.n_shared;  // two records (epoch-tagged words) per shared group
   run_jobs<kAccess>({js}, 1, tune, nullptr, stream, st, nullptr, &out, progress);
+}
+
+// Data aggregation over reached-node lists that another search produced (the
+// simplest-path trees of angular aggregation): one wave per source, persistent
+// waves taking sources from a counter. A wave's dedupe-group words start at
+// zero (the launch clears them) and carry the wave's own 9-bit epochs, cleared
+// again when those run out.
+struct ListAccessArgs {
+  AccessOut ao;
+  const uint32_t *src;     // device node of each source position
+  const uint32_t *items;   // [n_run] source position of each item, or null: item i is position i
+  const uint32_t *runs;    // [n_run] items to aggregate, or null: 0 .. n_run - 1
+  const uint64_t *lists;   // [items][lcap] {node | seconds << 32}
+  const uint32_t *counts;  // [items] list length or kNoList
+  uint32_t n_run, lcap;
+  float maxsec, speed;
+  char *ws;
+  size_t ws_stride, o_st;
+  uint32_t st_cap;
+  uint32_t *next, *n_fail, *fail_list;
+  uint32_t *progress;  // [waves] sources each wave has finished (pinned host memory), or null
+};
+
+__global__ __launch_bounds__(kBlock) void k_access_lists(ListAccessArgs a) {
+  const uint32_t lane = lane_id();
+  const uint32_t wid = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+  uint64_t *grec = reinterpret_cast<uint64_t *>(a.ws + (size_t)wid * a.ws_stride);
+  char *stb = a.ws + (size_t)wid * a.ws_stride + a.o_st;
+  uint32_t pdone = a.progress ? uniform(__hip_atomic_load(&a.progress[wid], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_SYSTEM)) : 0u;
+  uint32_t epoch = 0;
+  for (;;) {
+    uint32_t r = 0;
+    if (lane == 0) r = atomicAdd(a.next, 1u);
+    r = __shfl(r, 0);
+    if (r >= a.n_run) break;
+    const uint32_t item = uniform(a.runs ? a.runs[r] : r);
+    const uint32_t R = uniform(a.counts[item]);
+    if (R == kNoList) continue;
+    if (epoch >= kEpochMax) {
+      for (size_t i = lane; i < 2 * (size_t)a.ao.n_shared; i += kWave) grec[i] = 0ull;
+      wave_fence();
+      epoch = 0;
+    }
+    ++epoch;
+    const uint32_t src = uniform(a.src[a.items ? a.items[item] : item]);
+    const uint64_t *lst = a.lists + (size_t)item * a.lcap;
+    const auto reach = [&](uint32_t s) { return lst[s]; };
+    if (!access_fold(a.ao, src, R, reach, a.maxsec, a.speed, grec, (uint64_t)epoch << kEpochShift, stb,
+                     a.st_cap)) {
+      if (lane == 0) a.fail_list[atomicAdd(a.n_fail, 1u)] = item;
+      continue;
+    }
+    if (a.progress) {
+      ++pdone;
+      if (lane == 0) __hip_atomic_store(&a.progress[wid], pdone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
+
+void launch_access_lists(const AccessOut &out, const uint32_t *d_src, const uint32_t *d_items,
+                         const uint64_t *d_lists, const uint32_t *d_counts, uint32_t n_items, uint32_t lcap,
+                         float maxsec, float speed, hipStream_t stream, cs_stats &st, uint32_t *progress) {
+  if (n_items == 0) return;
+  if (out.n_shared > 0x7fffffffu) fail(CS_EINVAL, "too many dedupe groups for the per-wave records");
+  int dev = 0, n_cu = 0;
+  hip_check(hipGetDevice(&dev), "device");
+  hip_check(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev), "attribute");
+  const size_t wpb = kBlock / kWave;
+  size_t waves = std::min<size_t>(n_items, (size_t)n_cu * 16);
+  waves = (waves + wpb - 1) / wpb * wpb;
+  uint32_t st_cap = out.stats ? std::max<uint32_t>(1u, std::min<uint32_t>(out.n_groups, 4096u)) : 0u;
+  std::vector<uint32_t> runs;  // empty: every item
+  uint32_t n_run = n_items;
+  DevMem<uint32_t> d_ctl(4), d_fail(n_items), d_runs(n_items);
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    const size_t o_st = align256(16ull * out.n_shared);
+    const size_t stride = o_st + (st_cap ? align256(256 + 40ull * st_cap) : 0);
+    DevMem<char> ws(waves * stride);
+    hip_check(hipMemsetAsync(ws.p, 0, waves * stride, stream), "memset");
+    hip_check(hipMemsetAsync(d_ctl.p, 0, 16, stream), "memset");
+    if (!runs.empty())
+      hip_check(hipMemcpyAsync(d_runs.p, runs.data(), 4ull * n_run, hipMemcpyHostToDevice, stream), "copy");
+    ListAccessArgs a;
+    a.ao = out;
+    a.src = d_src;
+    a.items = d_items;
+    a.runs = runs.empty() ? nullptr : d_runs.p;
+    a.lists = d_lists;
+    a.counts = d_counts;
+    a.n_run = n_run;
+    a.lcap = lcap;
+    a.maxsec = maxsec;
+    a.speed = speed;
+    a.ws = ws.p;
+    a.ws_stride = stride;
+    a.o_st = o_st;
+    a.st_cap = st_cap;
+    a.next = d_ctl.p;
+    a.n_fail = d_ctl.p + 1;
+    a.fail_list = d_fail.p;
+    a.progress = waves <= kProgressSlots ? progress : nullptr;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, stream), "event");
+    hipLaunchKernelGGL(k_access_lists, dim3(waves / wpb), dim3(kBlock), 0, stream, a);
+    hip_check(hipGetLastError(), "k_access_lists launch");
+    hip_check(hipEventRecord(e1, stream), "event");
+    uint32_t ctl[4];
+    hip_check(hipMemcpyAsync(ctl, d_ctl.p, 16, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st.kernel_ms += ms;
+    if (ctl[1] == 0) return;
+    // statistics sources with more winners than the lists hold run again with longer lists
+    if (st_cap >= out.n_groups) fail(CS_EINTERNAL, "statistics winner list overflow");
+    st.retries += ctl[1];
+    runs.resize(ctl[1]);
+    hip_check(hipMemcpy(runs.data(), d_fail.p, 4ull * ctl[1], hipMemcpyDeviceToHost), "copy");
+    n_run = ctl[1];
+    st_cap = std::min<uint32_t>(out.n_groups, st_cap * 4);
+  }
+  fail(CS_EINTERNAL, "statistics list retries exhausted");
 }
 
 // One thread per (threshold, node): the diversity measures of the class

@@ -270,8 +270,8 @@ int cs_segment_centrality(cs_graph *g, const cs_params *p, double *out);
 /* Data layer                                                             */
 /* ---------------------------------------------------------------------- */
 
-/* Replaces DataMap.accessibility (rust/src/data.rs:617-747) with
- * angular = false, over the assignment DataMap.node_data_map holds after
+/* Replaces DataMap.accessibility (rust/src/data.rs:617-747) over the
+ * assignment DataMap.node_data_map holds after
  * assign_data_to_network (data.rs:381-387, 455-512), given here as n_assign
  * (node, entry, distance) triples. Entry e (0..n_entries-1) belongs to dedupe
  * group entry_group[e] (< n_entries; entries without a dedupe key form groups
@@ -281,7 +281,9 @@ int cs_segment_centrality(cs_graph *g, const cs_params *p, double *out);
  * (data.rs:520-615); equal totals within a group go to the member given first.
  * distances / betas / max_curve_wts: n_thresholds values as
  * pair_distances_betas_time and clip_wts_curve return them; max_seconds is
- * the largest of the thresholds' seconds.
+ * the largest of the thresholds' seconds. angular != 0 searches simplest-path
+ * trees on a dual graph instead of shortest-path trees (data.rs:538-548;
+ * CS_EINVAL on a primal graph, as validate_dual_for_angular raises).
  * out_count, out_weighted: f64 [n_classes][n_thresholds][N] (unweighted and
  * weighted accessibility); out_nearest: f64 [n_classes][N], the distance to
  * the nearest counted entry within the largest distance (NaN: none). */
@@ -289,11 +291,11 @@ int cs_accessibility(cs_graph *g, uint32_t n_entries, const uint32_t *entry_grou
                      uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
                      const uint32_t *assign_entry, const double *assign_dist, uint32_t n_thresholds,
                      const uint32_t *distances, const float *betas, const float *max_curve_wts,
-                     uint32_t max_seconds, float speed_m_s, double *out_count, double *out_weighted,
-                     double *out_nearest);
+                     uint32_t max_seconds, float speed_m_s, int angular, double *out_count,
+                     double *out_weighted, double *out_nearest);
 
-/* Replaces DataMap.mixed_uses (rust/src/data.rs:749-944) with angular = false:
- * the same inputs as cs_accessibility, with entry_class numbering every land
+/* Replaces DataMap.mixed_uses (rust/src/data.rs:749-944): the same inputs as
+ * cs_accessibility (angular included), with entry_class numbering every land
  * use in sorted order (n_classes = the number of distinct land uses). out:
  * f64 [8][n_thresholds][N] = hill q0, q1, q2, distance-weighted hill q0, q1,
  * q2, shannon, gini-simpson (diversity.rs:6-175, 341-385), each computed in
@@ -301,10 +303,10 @@ int cs_accessibility(cs_graph *g, uint32_t n_entries, const uint32_t *entry_grou
 int cs_mixed_uses(cs_graph *g, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
                   uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node, const uint32_t *assign_entry,
                   const double *assign_dist, uint32_t n_thresholds, const uint32_t *distances, const float *betas,
-                  const float *max_curve_wts, uint32_t max_seconds, float speed_m_s, double *out);
+                  const float *max_curve_wts, uint32_t max_seconds, float speed_m_s, int angular, double *out);
 
-/* Replaces DataMap.stats (rust/src/data.rs:1013-1233) with angular = false:
- * the entries, dedupe groups and node assignment of cs_accessibility, and
+/* Replaces DataMap.stats (rust/src/data.rs:1013-1233): the entries, dedupe
+ * groups, node assignment and angular flag of cs_accessibility, and
  * n_maps numeric maps as values f32 [n_maps][n_entries] (NaN: the entry has no
  * value). out: f64 [n_maps][14][n_thresholds][N] = sum, sum_wt, mean,
  * mean_wt, median, median_wt, count, count_wt, variance, variance_wt, mad,
@@ -317,7 +319,7 @@ int cs_data_stats(cs_graph *g, uint32_t n_entries, const uint32_t *entry_group, 
                   const uint32_t *assign_node, const uint32_t *assign_entry, const double *assign_dist,
                   uint32_t n_maps, const float *values, uint32_t n_thresholds, const uint32_t *distances,
                   const float *betas, const float *max_curve_wts, uint32_t max_seconds, float speed_m_s,
-                  double *out);
+                  int angular, double *out);
 
 /* ---------------------------------------------------------------------- */
 /* Single-source trees                                                    */

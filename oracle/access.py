@@ -3,7 +3,9 @@
 Loaded by tests/ as the parity checker; never by the product package. Follows
 DataMap.aggregate_to_src_idx (rust/src/data.rs:520-615) and the accumulation in
 DataMap.accessibility (data.rs:680-741) loop for loop, over the oracle's own
-dijkstra_tree_shortest (the visit order is the settle order). Values are f32
+dijkstra_tree_shortest (the visit order is the settle order) or, with
+``angular``, its dijkstra_tree_simplest (the visit order is the first-reach
+order; data.rs:538-548). Values are f32
 where the reference computes in f32 and f64 where it accumulates in f64.
 """
 
@@ -42,8 +44,15 @@ def aggregate_to_src_idx(tree, node_data_map, dedupe_of, max_walk_seconds, speed
     return entries_result
 
 
+def _tree(og, src, max_walk_seconds, speed, angular):
+    """The search aggregate_to_src_idx walks (data.rs:538-548)."""
+    if angular:
+        return og.tree_simplest(src, max_walk_seconds, speed)
+    return og.tree_shortest(src, max_walk_seconds, speed)
+
+
 def accessibility(og, live, node_data_map, dedupe_of, lu_class, accessibility_keys, distances, betas, mcws,
-                  max_walk_seconds, speed, sources=None):
+                  max_walk_seconds, speed, sources=None, angular=False):
     """{class: (unweighted f64[T][N], weighted f64[T][N], nearest f64[N])} (data.rs:680-741);
     `sources` restricts the rows computed (all nodes by default)."""
     n, T = og.n, len(distances)
@@ -52,7 +61,7 @@ def accessibility(og, live, node_data_map, dedupe_of, lu_class, accessibility_ke
     for src in (range(n) if sources is None else sources):
         if not live[src]:
             continue
-        tree = og.tree_shortest(src, max_walk_seconds, speed)
+        tree = _tree(og, src, max_walk_seconds, speed, angular)
         reach = aggregate_to_src_idx(tree, node_data_map, dedupe_of, max_walk_seconds, speed)
         for data_key, data_dist in reach.items():
             cls = lu_class.get(data_key)
@@ -157,7 +166,7 @@ def gini(counts):
 
 
 def mixed_uses(og, live, node_data_map, dedupe_of, lu_class, distances, betas, mcws, max_walk_seconds, speed,
-               sources=None):
+               sources=None, angular=False):
     """f64 [8][T][N]: hill q0-2, weighted hill q0-2, shannon, gini (data.rs:818-941)."""
     n, T = og.n, len(distances)
     classes = sorted(set(lu_class.values()))
@@ -166,7 +175,7 @@ def mixed_uses(og, live, node_data_map, dedupe_of, lu_class, distances, betas, m
     for src in (range(n) if sources is None else sources):
         if not live[src]:
             continue
-        tree = og.tree_shortest(src, max_walk_seconds, speed)
+        tree = _tree(og, src, max_walk_seconds, speed, angular)
         reach = aggregate_to_src_idx(tree, node_data_map, dedupe_of, max_walk_seconds, speed)
         counts = np.zeros((T, len(classes)), dtype=np.int64)
         nearest = np.full((T, len(classes)), np.inf, dtype=np.float32)
@@ -238,7 +247,7 @@ STAT_NAMES = ("sum", "sum_wt", "mean", "mean_wt", "median", "median_wt", "count"
 
 
 def stats(og, live, node_data_map, dedupe_of, entry_order, num_maps, distances, betas, mcws, max_walk_seconds,
-          speed, sources=None):
+          speed, sources=None, angular=False):
     """f64 [maps][14][T][N] in STAT_NAMES order (data.rs:1073-1227). Each source's reachable entries
     are taken in ascending `entry_order` (the reference walks its hash map, whose order varies)."""
     n, T = og.n, len(distances)
@@ -247,7 +256,7 @@ def stats(og, live, node_data_map, dedupe_of, entry_order, num_maps, distances, 
     for src in (range(n) if sources is None else sources):
         if not live[src]:
             continue
-        tree = og.tree_shortest(src, max_walk_seconds, speed)
+        tree = _tree(og, src, max_walk_seconds, speed, angular)
         reach = aggregate_to_src_idx(tree, node_data_map, dedupe_of, max_walk_seconds, speed)
         items = sorted(reach.items(), key=lambda kv: entry_order[kv[0]])
         for mp, num_map in enumerate(num_maps):

@@ -241,3 +241,61 @@ def test_datamap_boundary_follows_the_reference():
         dm._landuses({"a": 1, "b": "shop"})
     with pytest.raises(KeyError):
         dm._landuses({"a": "shop", "c": "shop"})
+
+
+def detour_dual():
+    """Dual of a primal graph where street T0-T1 is reached from street S0-S1 by a shortest route
+    with three right-angle turns (300 m between midpoints) and by a longer route through D with
+    101.4 degrees of turning (50 + 100 sqrt(1.01) + 110 + 50 = 310.5 m): the simplest-path tree
+    (centrality.rs:1202-1332) takes the second. Returns the network and the two dual nodes."""
+    import networkx as nx
+
+    from cityseer.tools import graphs, io
+
+    pts = {"S0": (0, 0), "S1": (100, 0), "C1": (150, 0), "C2": (150, 50), "C3": (200, 50), "T0": (200, 100),
+           "T1": (200, 200), "D": (200, -10)}
+    G = nx.MultiGraph()
+    for k, (x, y) in pts.items():
+        G.add_node(k, x=x, y=y)
+    for a, b in [("S0", "S1"), ("S1", "C1"), ("C1", "C2"), ("C2", "C3"), ("C3", "T0"), ("S1", "D"), ("D", "T0"),
+                 ("T0", "T1")]:
+        G.add_edge(a, b)
+    _n, _e, ns = io.network_structure_from_nx(graphs.nx_to_dual(graphs.nx_simple_geoms(G)))
+    keys = [str(k) for k in ns.node_keys_py()]
+    return ns, keys.index("S0_S1_k0"), keys.index("T0_T1_k0")
+
+
+DETOUR_SIMPLEST = 50.0 + 100.0 * math.sqrt(1.01) + 110.0 + 50.0
+
+
+def test_oracle_angular_aggregation_follows_the_simplest_route():
+    """Angular aggregation walks the simplest-path tree (data.rs:538-548): an entry on T0-T1 is
+    300 m from S0-S1 by the shortest route and 310.5 m by the simplest, so a 305 m threshold
+    counts it only for the shortest search and the nearest distances differ by the detour."""
+    from oracle.access import accessibility as oracle_access
+    from oracle.oracle import OracleGraph
+
+    ns, s, t = detour_dual()
+    og = OracleGraph.from_network_structure(ns)
+    ndm = {t: [("x", 0.0)]}
+    dists = [305, 400]
+    _d, bets, secs = O.thresholds(1.0, distances=dists)
+    live = [1] * og.n
+    args = (og, live, ndm, {"x": "x"}, {"x": "shop"}, ["shop"], dists, bets, [1.0, 1.0], max(secs), 1.0)
+    sh = oracle_access(*args, sources=[s])["shop"]
+    an = oracle_access(*args, sources=[s], angular=True)["shop"]
+    assert sh[0][:, s].tolist() == [1.0, 1.0] and an[0][:, s].tolist() == [0.0, 1.0]
+    assert sh[2][s] == 300.0
+    assert abs(an[2][s] - DETOUR_SIMPLEST) < 1e-3
+
+
+def test_angular_data_layer_needs_a_dual_graph():
+    ns = line_graph()
+    dm = data.DataMap()
+    dm.insert("a", "POINT (0 0)")
+    for call in (lambda: dm.accessibility(ns, {"a": "shop"}, ["shop"], distances=[200], angular=True),
+                 lambda: dm.mixed_uses(ns, {"a": "shop"}, distances=[200], angular=True),
+                 lambda: dm.stats(ns, [{"a": 1.0}], distances=[200], angular=True),
+                 lambda: dm.aggregate_to_src_idx(0, ns, 300, 1.0, angular=True)):
+        with pytest.raises(ValueError, match="requires a dual graph for angular analysis"):
+            call()

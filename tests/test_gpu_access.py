@@ -68,16 +68,17 @@ def _random_case(ns, n_entries, seed, classes=("shop", "cafe", "park", "other"))
     return entries
 
 
-def _compare(ns, entries, acc_keys, dists, speed, live=None, spatial_tolerance=0):
+def _compare(ns, entries, acc_keys, dists, speed, live=None, spatial_tolerance=0, angular=False):
     dm, lu = _datamap(entries)
-    res = dm.accessibility(ns, lu, acc_keys, distances=dists, speed_m_s=speed, spatial_tolerance=spatial_tolerance)
+    res = dm.accessibility(ns, lu, acc_keys, distances=dists, speed_m_s=speed, spatial_tolerance=spatial_tolerance,
+                           angular=angular)
     og = OracleGraph.from_network_structure(ns)
     d_, bets, secs = O.thresholds(speed, distances=dists)
     mcw = data.clip_wts_curve(dists, bets, spatial_tolerance)
     ndm = {v: [(k[4:], dist) for k, dist in pairs] for v, pairs in dm.node_data_map.items()}
     dedupe_of = {key: (dd if dd is not None else key) for key, _c, dd, _a in entries}
     ref = oracle_access(og, live if live is not None else [1] * og.n, ndm, dedupe_of, lu, acc_keys, dists, bets,
-                        mcw, max(secs), speed)
+                        mcw, max(secs), speed, angular=angular)
     for cls in acc_keys:
         la = res.result[cls]
         r_u, r_w, r_n = ref[cls]
@@ -115,18 +116,18 @@ def test_no_assignments_and_unknown_keys():
     assert np.all(np.isnan(res.result["none"].distance[300]))
 
 
-def _compare_mixed(ns, entries, dists, speed, spatial_tolerance=0):
+def _compare_mixed(ns, entries, dists, speed, spatial_tolerance=0, angular=False):
     from oracle.access import mixed_uses as oracle_mixed
 
     dm, lu = _datamap(entries)
     res = dm.mixed_uses(ns, lu, distances=dists, speed_m_s=speed, compute_shannon=True, compute_gini=True,
-                        spatial_tolerance=spatial_tolerance)
+                        spatial_tolerance=spatial_tolerance, angular=angular)
     og = OracleGraph.from_network_structure(ns)
     _d, bets, secs = O.thresholds(speed, distances=dists)
     mcw = data.clip_wts_curve(dists, bets, spatial_tolerance)
     ndm = {v: [(k[4:], dist) for k, dist in pairs] for v, pairs in dm.node_data_map.items()}
     dedupe_of = {key: (dd if dd is not None else key) for key, _c, dd, _a in entries}
-    ref = oracle_mixed(og, [1] * og.n, ndm, dedupe_of, lu, dists, bets, mcw, max(secs), speed)
+    ref = oracle_mixed(og, [1] * og.n, ndm, dedupe_of, lu, dists, bets, mcw, max(secs), speed, angular=angular)
     for i, d in enumerate(dists):
         # richness is an exact count; the rest are f32 values of exp/log/pow, whose last bits may
         # differ between the device library and the host's (1e-5 relative)
@@ -179,3 +180,53 @@ def test_large_dedupe_groups():
     # each group counts at most once per source
     assert res.result["park"].unweighted[700].max() <= 2 + 400
     assert (res.result["shop"].unweighted[700] + res.result["cafe"].unweighted[700]).max() >= 1
+
+
+def test_angular_detour_by_hand():
+    """Angular aggregation walks the simplest-path tree (data.rs:538-548): the entry on T0-T1 is
+    310.5 m from S0-S1 along the simplest route (300 m along the shortest)."""
+    from test_access_oracle import DETOUR_SIMPLEST, detour_dual
+
+    ns, s, t = detour_dual()
+    dm = data.DataMap()
+    dm.insert("x", "POINT (0 0)")
+    dm.node_data_map = {t: [("str:x", 0.0)]}
+    say("angular accessibility on the detour dual")
+    res = dm.accessibility(ns, {"x": "shop"}, ["shop"], distances=[305, 400], speed_m_s=1.0, angular=True)
+    assert res.result["shop"].unweighted[305][s] == 0.0 and res.result["shop"].unweighted[400][s] == 1.0
+    assert abs(res.result["shop"].distance[400][s] - DETOUR_SIMPLEST) < 1e-3
+    short = dm.accessibility(ns, {"x": "shop"}, ["shop"], distances=[305, 400], speed_m_s=1.0)
+    assert short.result["shop"].unweighted[305][s] == 1.0 and short.result["shop"].distance[400][s] == 300.0
+    # the single-source form, tree on the device and the walk on the host
+    assert dm.aggregate_to_src_idx(s, ns, 400, 1.0) == {"str:x": 300.0}
+    got = dm.aggregate_to_src_idx(s, ns, 400, 1.0, angular=True)
+    assert list(got) == ["str:x"] and abs(got["str:x"] - DETOUR_SIMPLEST) < 1e-3
+    _compare(ns, [("x", "shop", None, [(t, 0.0)])], ["shop"], [305, 400], 1.0, angular=True)
+
+
+def test_angular_random_assignments_lattice_dual():
+    ns = mock.network_structure_from_lattice(mock.lattice_dual(mock.perturbed_lattice(14, seed=211)))
+    entries = _random_case(ns, 500, 212)
+    say(f"angular accessibility, {ns.node_count()} dual nodes, {len(entries)} entries")
+    res = _compare(ns, entries, ["shop", "cafe", "park"], [200, 500, 900], 1.33333, angular=True)
+    assert res.result["shop"].unweighted[900].sum() > 0
+    say("angular mixed uses with a spatial tolerance")
+    entries = _random_case(ns, 400, 213, classes=tuple(f"c{i}" for i in range(7)))
+    _compare_mixed(ns, entries, [250, 700], 1.33333, spatial_tolerance=25, angular=True)
+
+
+def test_angular_aggregate_to_src_idx_matches_oracle():
+    from oracle.access import aggregate_to_src_idx as oracle_aggregate
+
+    ns = mock.network_structure_from_lattice(mock.lattice_dual(mock.perturbed_lattice(10, seed=214)))
+    entries = _random_case(ns, 150, 215)
+    dm, _lu = _datamap(entries)
+    og = OracleGraph.from_network_structure(ns)
+    ndm = {v: pairs for v, pairs in dm.node_data_map.items()}
+    dedupe_of = {k: e.dedupe_key for k, e in dm.entries.items()}
+    say("aggregate_to_src_idx, shortest and angular, 12 sources")
+    for src in range(0, ns.node_count(), max(1, ns.node_count() // 12)):
+        for angular in (False, True):
+            tree = og.tree_simplest(src, 600, 1.33333) if angular else og.tree_shortest(src, 600, 1.33333)
+            want = {k: float(v) for k, v in oracle_aggregate(tree, ndm, dedupe_of, 600, 1.33333).items()}
+            assert dm.aggregate_to_src_idx(src, ns, 600, 1.33333, angular=angular) == want, (src, angular)

@@ -74,3 +74,30 @@ def test_simplest_retries_match_oracle_plain_and_served(tiny_slots):
             g = stack(r.result(), SIMPLEST, d)
             assert np.array_equal(g[0], ref[0]), what
             assert_close(g, ref, what)
+
+
+def test_angular_access_retries_match_oracle(tiny_slots):
+    """Angular aggregation with reached-node lists of 24 nodes per source at first: sources that
+    reach more go round again with four times the room, until every list fits."""
+    from cityseer.rustalgos import data
+    from oracle.access import accessibility as oracle_access
+    from test_gpu_access import _datamap, _random_case
+
+    ns = mock.network_structure_from_lattice(mock.lattice_dual(mock.perturbed_lattice(13, seed=54)))
+    entries = _random_case(ns, 300, 55)
+    dm, lu = _datamap(entries)
+    dists = [300, 1200]
+    say("angular accessibility with 48 state slots per lane")
+    res = dm.accessibility(ns, lu, ["shop", "cafe"], distances=dists, angular=True)
+    assert _native.last_stats()["retries"] > 0
+    og = OracleGraph.from_network_structure(ns)
+    _d, bets, secs = O.thresholds(1.33333, distances=dists)
+    ndm = {v: [(k[4:], d) for k, d in pairs] for v, pairs in dm.node_data_map.items()}
+    dedupe_of = {key: (dd if dd is not None else key) for key, _c, dd, _a in entries}
+    ref = oracle_access(og, [1] * og.n, ndm, dedupe_of, lu, ["shop", "cafe"], dists, bets,
+                        data.clip_wts_curve(dists, bets, 0), max(secs), 1.33333, angular=True)
+    for cls in ("shop", "cafe"):
+        for i, d in enumerate(dists):
+            assert np.array_equal(res.result[cls].unweighted[d], ref[cls][0][i]), (cls, d)
+            np.testing.assert_allclose(res.result[cls].weighted[d], ref[cls][1][i], rtol=1e-5, atol=1e-12)
+        assert np.array_equal(res.result[cls].distance[1200], ref[cls][2], equal_nan=True)

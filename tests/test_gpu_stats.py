@@ -30,17 +30,19 @@ def _keyed():
     O.set_heap_mode(False)
 
 
-def _compare(ns, entries, maps, dists, speed, spatial_tolerance=0):
+def _compare(ns, entries, maps, dists, speed, spatial_tolerance=0, angular=False):
     dm, _lu = _datamap(entries)
     num_maps = [{k: v for k, v in m.items()} for m in maps]
-    res = dm.stats(ns, num_maps, distances=dists, speed_m_s=speed, spatial_tolerance=spatial_tolerance)
+    res = dm.stats(ns, num_maps, distances=dists, speed_m_s=speed, spatial_tolerance=spatial_tolerance,
+                   angular=angular)
     og = OracleGraph.from_network_structure(ns)
     _d, bets, secs = O.thresholds(speed, distances=dists)
     mcw = data.clip_wts_curve(dists, bets, spatial_tolerance)
     ndm = {v: [(k[4:], dist) for k, dist in pairs] for v, pairs in dm.node_data_map.items()}
     dedupe_of = {key: (dd if dd is not None else key) for key, _c, dd, _a in entries}
     order = {key: i for i, (key, _c, _dd, _a) in enumerate(entries)}
-    ref = oracle_stats(og, [1] * og.n, ndm, dedupe_of, order, maps, dists, bets, mcw, max(secs), speed)
+    ref = oracle_stats(og, [1] * og.n, ndm, dedupe_of, order, maps, dists, bets, mcw, max(secs), speed,
+                       angular=angular)
     for mp, st in enumerate(res.result):
         for m, name in enumerate(STAT_NAMES):
             got = np.stack([getattr(st, name)[d] for d in dists])
@@ -120,3 +122,18 @@ def test_groups_after_a_larger_graph():
         _compare(small, entries, [vals], [150, 400, 800], 1.33333)
         big.centrality_shortest(distances=[4000], compute_closeness=False)
         access_compare(small, entries, ["shop", "cafe", "park"], [200, 500, 900], 1.33333)
+
+
+def test_angular_lattice_dual_ten_thresholds():
+    """Angular statistics (data.rs:538-548 through stats): simplest-path trees on a lattice dual,
+    ten thresholds (two launches of the aggregation over one set of reached-node lists)."""
+    ns = mock.network_structure_from_lattice(mock.lattice_dual(mock.perturbed_lattice(12, seed=411)))
+    entries = _random_case(ns, 400, 412)
+    rng = np.random.default_rng(413)
+    vals = {key: float(rng.normal(5.0, 3.0)) for key, _c, _d, _a in entries}
+    for key in rng.choice([e[0] for e in entries], 30, replace=False):
+        vals[str(key)] = float("nan")
+    ints = {key: float(rng.integers(0, 4)) for key, _c, _d, _a in entries}
+    dists = [100, 150, 200, 300, 400, 500, 600, 700, 800, 900]
+    say(f"angular stats, {ns.node_count()} dual nodes, 2 maps, ten thresholds")
+    _compare(ns, entries, [vals, ints], dists, 1.33333, spatial_tolerance=30, angular=True)
