@@ -20,7 +20,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <memory>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -403,14 +405,17 @@ extern "C" int cs_assign_data(uint32_t n_nodes, const double *node_xy, uint32_t 
     std::vector<Geom> items = read_geoms(data);
     Grid grid;
     grid.build(streets);
-    std::vector<uint8_t> seen(streets.size(), 0);
-    std::vector<uint32_t> hits;
-    std::vector<std::pair<double, uint32_t>> near;
     auto node_pt = [&](uint32_t v) { return P{node_xy[2 * (size_t)v], node_xy[2 * (size_t)v + 1]}; };
+
+    // per-thread scratch of the envelope queries
+    struct Scratch {
+      std::vector<uint8_t> seen;
+      std::vector<std::pair<double, uint32_t>> near;
+    };
 
     // the line from the data's nearest point to a node crosses a barrier or a
     // street properly (graph.rs:1488-1535)
-    auto blocked = [&](const P &a, const P &b) {
+    auto blocked = [&](const P &a, const P &b, Scratch &sc) {
       Box lb;
       lb.add(a);
       lb.add(b);
@@ -418,7 +423,7 @@ extern "C" int cs_assign_data(uint32_t n_nodes, const double *node_xy, uint32_t 
         if (g.box.hits(lb) && seg_meets_geom(a, b, g)) return true;
       if (streets.empty()) return false;
       std::vector<uint32_t> cand;
-      grid.query(lb, cand, seen, streets);
+      grid.query(lb, cand, sc.seen, streets);
       for (uint32_t i : cand) {
         const auto &l = streets[i].line;
         for (size_t k = 0; k + 1 < l.size(); ++k)
@@ -427,22 +432,22 @@ extern "C" int cs_assign_data(uint32_t n_nodes, const double *node_xy, uint32_t 
       return false;
     };
 
-    auto res = std::make_unique<cs_assignment>();
-    for (uint32_t d = 0; d < (uint32_t)items.size(); ++d) {
+    // the assignments of data geometry d, appended to rows in the reference's order
+    auto assign_one = [&](uint32_t d, Scratch &sc, std::vector<Assign> &rows) {
       const Geom &g = items[d];
-      if (g.parts.empty() || streets.empty()) continue;  // no geometry, no network: nothing assigned
+      if (g.parts.empty() || streets.empty()) return;  // no geometry, no network: nothing assigned
       // candidates (graph.rs:1367-1411)
       std::vector<uint32_t> cands;
       if (g.is_point()) {
-        grid.nearest(g.parts[0].rings[0][0], n_nearest, near, seen, streets);
-        for (auto &q : near) cands.push_back(q.second);
+        grid.nearest(g.parts[0].rings[0][0], n_nearest, sc.near, sc.seen, streets);
+        for (auto &q : sc.near) cands.push_back(q.second);
       } else {
         Box qb = g.box;
         qb.x0 -= max_dist;
         qb.y0 -= max_dist;
         qb.x1 += max_dist;
         qb.y1 += max_dist;
-        grid.query(qb, cands, seen, streets);
+        grid.query(qb, cands, sc.seen, streets);
       }
       std::vector<std::pair<double, uint32_t>> within;
       for (uint32_t i : cands) {
@@ -462,7 +467,7 @@ extern "C" int cs_assign_data(uint32_t n_nodes, const double *node_xy, uint32_t 
         const double nd = dist_point_geom(np, g, &cp);
         std::pair<bool, double> r;
         if (nd < 1e-6) r = {true, 0.0};
-        else r = {!blocked(cp, np), nd};
+        else r = {!blocked(cp, np, sc), nd};
         checked.emplace(v, r);
         return r;
       };
@@ -473,14 +478,44 @@ extern "C" int cs_assign_data(uint32_t n_nodes, const double *node_xy, uint32_t 
         for (uint32_t v : {s.start, s.end}) {
           const auto r = check(v);
           if (r.first && added.insert(v).second) {
-            res->rows.push_back(Assign{d, v, r.second});
+            rows.push_back(Assign{d, v, r.second});
             produced = true;
           }
         }
         if (g.is_point() && produced) break;
         if (++visited >= n_nearest) break;
       }
+    };
+
+    // entries are independent (the reference's rayon loop, data.rs:469-484):
+    // contiguous blocks of them go to host threads, whose rows concatenate in
+    // data order
+    const uint32_t n_items = (uint32_t)items.size();
+    const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t n_threads = std::max<uint32_t>(1u, std::min<uint32_t>({hw, 16u, n_items / 64}));
+    std::vector<std::vector<Assign>> rows(n_threads);
+    std::vector<std::exception_ptr> errs(n_threads);
+    auto run_block = [&](uint32_t t) {
+      try {
+        Scratch sc;
+        sc.seen.assign(streets.size(), 0);
+        const uint32_t d0 = (uint32_t)((uint64_t)n_items * t / n_threads);
+        const uint32_t d1 = (uint32_t)((uint64_t)n_items * (t + 1) / n_threads);
+        for (uint32_t d = d0; d < d1; ++d) assign_one(d, sc, rows[t]);
+      } catch (...) {
+        errs[t] = std::current_exception();
+      }
+    };
+    {
+      std::vector<std::thread> pool;
+      for (uint32_t t = 1; t < n_threads; ++t) pool.emplace_back(run_block, t);
+      run_block(0);
+      for (auto &th : pool) th.join();
     }
+    for (auto &e : errs)
+      if (e) std::rethrow_exception(e);
+    auto res = std::make_unique<cs_assignment>();
+    for (auto &r : rows) res->rows.insert(res->rows.end(), r.begin(), r.end());
     *n_out = res->rows.size();
     *out = res.release();
   });

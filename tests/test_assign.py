@@ -185,3 +185,26 @@ def test_candidate_limit_and_distance_cut():
     dm3.insert("big", "POLYGON ((-10 -60, 210 -60, 210 60, -10 60, -10 -60))")
     dm3.assign_data_to_network(ns, 0, 1)
     assert len(dm3.node_data_map) == 2
+
+
+def test_threaded_assignment_keeps_data_order():
+    """Entries go to host threads in contiguous blocks: a 3,000-point assignment equals the
+    assignments of its 64-entry slices (each run on one thread), concatenated."""
+    G = _mock_primal()
+    _n, _e, ns = io.network_structure_from_nx(G)
+    rng = np.random.default_rng(7)
+    x0, y0, x1, y1 = mock.get_graph_extents(G)
+    pts = [f"POINT ({rng.uniform(x0, x1)!r} {rng.uniform(y0, y1)!r})" for _ in range(3000)]
+    whole = data.DataMap()
+    for i, w in enumerate(pts):
+        whole.insert(i, w)
+    whole.assign_data_to_network(ns, 200, 6)
+    want: dict = {}
+    for a in range(0, len(pts), 64):
+        part = data.DataMap()
+        for i in range(a, min(a + 64, len(pts))):
+            part.insert(i, pts[i])
+        part.assign_data_to_network(ns, 200, 6)
+        for node, pairs in part.node_data_map.items():
+            want.setdefault(node, []).extend(pairs)
+    assert whole.node_data_map == want
