@@ -200,7 +200,8 @@ def run_variant(args) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": profiled_traffic("simplest_traffic_bytes_per_launch") if kind == "simplest" else None,
+            "traffic": (profiled_traffic("simplest_traffic_bytes_per_launch", "k_angular_cfg3_dual_hbm.md")
+                        if kind == "simplest" else None),
             "kernel_ms": round(kms, 3),
             "algorithmic_bytes": int(algo_bytes),
             "edges_scanned": int(stats["edges_scanned"]),

@@ -5,8 +5,9 @@
 // bucket width separates a state from its predecessors and the reference's
 // order-dependent fold (centrality.rs:613-705) must see candidates strictly in
 // settle order. Each lane therefore runs one source's search serially with a
-// private binary heap keyed (route, seconds at push, state) -- 64 independent
-// sources per wave, persistent lanes pulling sources from a global counter.
+// private priority queue keyed (route, seconds at push, state): a ring of
+// buckets on route cost, or a 4-ary heap for a source that overfills a bucket.
+// Up to 64 independent sources run per wave, one per lane, dealt statically.
 //
 // Per-lane state: epoch-tagged maps state -> slot and node -> slot (zero
 // memory is stale, nothing cleared per source), slot arrays for reached
@@ -40,6 +41,7 @@ struct AngJob {
   const float *src_wt;
   uint32_t flags;  // 1 closeness, 2 betweenness, 4 track reach
   double *acc;     // [4][tstride][N], metric-major
+  float bq_inv;    // bucket queue: buckets per unit of route (the ring spans more than the largest edge)
   uint32_t tstride;  // threshold stride of acc rows (>= T)
   uint32_t *reach;
 };
@@ -60,6 +62,8 @@ struct AngArgs {
   size_t wave_stride;  // bytes of the interleaved arrays per wave
   uint32_t scap, ncap, hcap, maxp, tcap;  // per-lane capacities (tcap: thresholds of the delta/seed blocks)
   size_t o_srec, o_preds, o_order, o_orte, o_delta, o_seed, o_nrec, o_heap;
+  size_t o_bqe, o_bqc;  // bucket queue entries [kBuckets][bq_cap] and counts [kBuckets] per lane
+  uint32_t bq_cap;
   char *wave_ws;
   uint32_t per_wave;    // lanes of a wave that take sources (the rest take padding items)
   uint32_t epoch_base;  // item it searches under epoch epoch_base + it + 1
@@ -135,8 +139,9 @@ struct Heap {
   __device__ __forceinline__ static bool less(const uint4 &a, const uint4 &b) {
     return hless(__uint_as_float(a.x), __uint_as_float(a.y), a.z, __uint_as_float(b.x), __uint_as_float(b.y), b.z);
   }
-  // `slot` rides along outside the key (the state's workspace slot)
-  __device__ __forceinline__ void push(float k, float s, uint32_t idx, uint32_t slot = 0u) {
+  // `slot` rides along outside the key (the state's workspace slot); false: full
+  __device__ __forceinline__ bool push(float k, float s, uint32_t idx, uint32_t slot = 0u) {
+    if (n >= cap) return false;
     const uint4 e = make_uint4(__float_as_uint(k), __float_as_uint(s), idx, slot);
     uint32_t i = n++;
     while (i > 0) {
@@ -147,6 +152,7 @@ struct Heap {
       i = p;
     }
     *at(i) = e;
+    return true;
   }
   __device__ __forceinline__ uint4 pop() {
     const uint4 top = *at(0);
@@ -176,15 +182,118 @@ struct Heap {
   }
 };
 
+// Per-lane monotone bucket queue on route cost, the same interface and pop
+// order as Heap. A search's keys never fall below the last key popped (edge
+// costs are non-negative and f32 addition is monotone), so entries live in a
+// ring of kBuckets buckets of width 1 / inv, a bucket holding every key whose
+// floor(route * inv) maps to it; the host makes kBucketSpan buckets span the
+// largest edge cost, so a pushed key lands fewer than kBuckets buckets past the
+// last pop. A mask in registers marks the non-empty buckets. A pop scans the lowest
+// non-empty bucket for its exact minimum under the heap's key order (route,
+// seconds at push, state) and moves the bucket's last entry into the gap: keys
+// are distinct (a state is pushed again only with a smaller route), so the pop
+// sequence equals the heap's. Pops cost a count load and one batch of entry
+// loads instead of a dependent load per heap level. A push that would leave
+// the ring or overfill its bucket returns false (the source is searched again
+// with the heap).
+constexpr uint32_t kMaskWords = 2;                // occupancy mask words held in registers
+constexpr uint32_t kBuckets = 64 * kMaskWords;     // ring of 128 buckets
+constexpr uint32_t kBucketSpan = kBuckets - 8;     // buckets per largest route increment
+constexpr uint32_t kBqCap = 64;  // entries per bucket before a source falls back to the heap
+struct BQueue {
+  char *w;
+  size_t off_e, off_c;
+  uint32_t lane, cap;  // entries per bucket
+  float inv;
+  uint64_t mask[kMaskWords] = {};
+  uint32_t cur = 0;  // absolute bucket of the last pop
+  uint32_t n = 0;
+  __device__ __forceinline__ uint4 *ent(uint32_t r, uint32_t i) {
+    return il<uint4>(w, off_e, r * cap + i, lane, (size_t)kBuckets * cap);
+  }
+  __device__ __forceinline__ uint32_t *cnt(uint32_t r) { return il<uint32_t>(w, off_c, r, lane, kBuckets); }
+  // mask words are indexed through selects so that they stay in registers
+  __device__ __forceinline__ uint64_t word(uint32_t i) const {
+    uint64_t v = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kMaskWords; ++k) v = (i == k) ? mask[k] : v;
+    return v;
+  }
+  __device__ __forceinline__ void flip(uint32_t r, bool on) {
+    const uint32_t i = r >> 6;
+    const uint64_t b = 1ull << (r & 63);
+#pragma unroll
+    for (uint32_t k = 0; k < kMaskWords; ++k)
+      if (i == k) mask[k] = on ? (mask[k] | b) : (mask[k] & ~b);
+  }
+  // distance from the last pop's bucket to the next non-empty one (n > 0)
+  __device__ __forceinline__ uint32_t next_offset() const {
+    const uint32_t c0 = cur & (kBuckets - 1), w0 = c0 >> 6, b0 = c0 & 63;
+    const uint64_t x = word(w0) >> b0;
+    if (x) return (uint32_t)__builtin_ctzll(x);
+    uint32_t off = 64 - b0;
+#pragma unroll
+    for (uint32_t k = 1; k <= kMaskWords; ++k) {
+      const uint64_t y = word((w0 + k) & (kMaskWords - 1));
+      if (y) return off + (uint32_t)__builtin_ctzll(y);
+      off += 64;
+    }
+    return 0;
+  }
+  __device__ __forceinline__ bool push(float k, float s, uint32_t idx, uint32_t slot = 0u) {
+    const float fb = k * inv;
+    if (!(fb < 4.0e9f)) return false;
+    const uint32_t b = (uint32_t)fb;  // floor: k >= 0
+    if (b - cur >= kBuckets) return false;
+    const uint32_t r = b & (kBuckets - 1);
+    const bool used = (word(r >> 6) >> (r & 63)) & 1ull;
+    const uint32_t m = used ? *cnt(r) : 0u;
+    if (m >= cap) return false;
+    *ent(r, m) = make_uint4(__float_as_uint(k), __float_as_uint(s), idx, slot);
+    *cnt(r) = m + 1;
+    flip(r, true);
+    ++n;
+    return true;
+  }
+  __device__ __forceinline__ uint4 pop() {
+    cur += next_offset();
+    const uint32_t r = cur & (kBuckets - 1);
+    const uint32_t m = *cnt(r);
+    uint4 best = make_uint4(0u, 0u, 0u, 0u), last = best;
+    uint32_t bi = 0;
+    // entries in batches of eight independent loads
+    for (uint32_t i0 = 0; i0 < m; i0 += 8) {
+      uint4 e[8];
+#pragma unroll
+      for (uint32_t q = 0; q < 8; ++q) e[q] = (i0 + q < m) ? *ent(r, i0 + q) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+      for (uint32_t q = 0; q < 8; ++q) {
+        const uint32_t i = i0 + q;
+        if (i >= m) continue;
+        if (i == 0 || Heap::less(e[q], best)) {
+          best = e[q];
+          bi = i;
+        }
+        if (i == m - 1) last = e[q];
+      }
+    }
+    if (bi != m - 1) *ent(r, bi) = last;
+    *cnt(r) = m - 1;
+    if (m == 1) flip(r, false);
+    --n;
+    return best;
+  }
+};
+
 // One source, exactly as dijkstra_brandes_angular Phase 1 (centrality.rs:577-705).
 // Returns false (with fail bits) if a workspace capacity is exceeded.
-__device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, uint32_t src, uint32_t epoch, uint32_t &n_states,
-                               uint32_t &n_nodes, uint32_t &n_order, uint32_t &fail) {
+template <typename Q>
+__device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, Q &h, uint32_t src, uint32_t epoch,
+                               uint32_t &n_states, uint32_t &n_nodes, uint32_t &n_order, uint32_t &fail) {
   const float one_m = 1.0f - kTieEps, one_p = 1.0f + kTieEps;
   char *w = L.w;
   const uint32_t lane = L.lane;
   n_states = n_nodes = n_order = 0;
-  Heap h{w, a.o_heap, lane, 0, a.hcap};
   // node slot of the source
   L.nkey[src] = ((uint64_t)epoch << 32) | 0u;
   *nr<uint32_t>(w, a.o_nrec, 0, lane, a.ncap, kN_node) = src;
@@ -321,11 +430,10 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
           }
           preds[k] = uss;
           *np = k + 1;
-          if (h.n >= a.hcap) {
+          if (!h.push(cr, cs, vs, vss)) {
             fail |= kFailHeap;
             return false;
           }
-          h.push(cr, cs, vs, vss);
         } else if (cr <= cur * one_p) {
           const uint32_t k = *np;
           bool has = false;
@@ -441,6 +549,8 @@ __device__ bool angular_phase2(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
   return true;
 }
 
+// QK 1: bucket queue, 0: 4-ary heap.
+template <int QK>
 __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
   const uint32_t lane = __lane_id();
   const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;  // global lane
@@ -474,7 +584,14 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     // so map entries left by any earlier search (any layout) never match
     const uint32_t epoch = a.epoch_base + it + 1;
     uint32_t n_states, n_nodes, n_order, fail = 0;
-    bool ok = angular_source(a, jb, L, src, epoch, n_states, n_nodes, n_order, fail);
+    bool ok;
+    if constexpr (QK == 1) {
+      BQueue q{w, a.o_bqe, a.o_bqc, lane, a.bq_cap, jb.bq_inv};
+      ok = angular_source(a, jb, L, q, src, epoch, n_states, n_nodes, n_order, fail);
+    } else {
+      Heap q{w, a.o_heap, lane, 0, a.hcap};
+      ok = angular_source(a, jb, L, q, src, epoch, n_states, n_nodes, n_order, fail);
+    }
     if (ok && jb.tolerance > kTieEps && (jb.flags & 2u)) ok = angular_phase2(a, jb, L, epoch, n_order, fail);
     if (!ok) {
       const uint32_t q = atomicAdd(a.n_fail, 1u);
@@ -679,11 +796,10 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
         *sr<float>(w, a.o_srec, vss, 0, a.scap, kS_route) = cmetric;
         *il<double>(w, a.o_delta, vss, 0, (size_t)a.scap * a.tcap) = (double)csimpl;
         *sr<float>(w, a.o_srec, vss, 0, a.scap, kS_secs) = cs;
-        if (h.n >= a.hcap) {
+        if (!h.push(cmetric, cs, vs)) {
           atomicOr(a.fail_flags, kFailHeap);
           return;
         }
-        h.push(cmetric, cs, vs);
         // collapsed node view (centrality.rs:1307-1326); t_vis: 0 unreached, 1 reached
         const bool first = a.t_vis[v] == 0;
         if (first) a.t_order[no++] = v;
@@ -785,11 +901,10 @@ __global__ __launch_bounds__(kBlock) void k_reach_simplest(AngArgs a, const uint
         *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_route) = cmetric;
         *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_secs) = cs;
         *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_np) = csimpl;
-        if (h.n >= a.hcap) {
+        if (!h.push(cmetric, cs, vs, vss)) {
           fail |= kFailHeap;
           break;
         }
-        h.push(cmetric, cs, vs, vss);
         // collapsed node view (centrality.rs:1307-1326); a node's slot is its
         // position in visited_nodes
         const uint64_t nk = L.nkey[v];
@@ -841,6 +956,8 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct AngPlan {
   uint32_t scap, ncap, hcap, maxp, lanes, tcap;
+  uint32_t bq_cap = 0;  // bucket queue entries per bucket (0: the heap)
+  size_t o_bqe = 0, o_bqc = 0;
   uint32_t per_wave = kWave;  // lane blocks per wave in the interleaved arrays
   size_t lane_stride, wave_stride;
   size_t o_srec, o_preds, o_order, o_orte, o_delta, o_seed, o_nrec, o_heap;
@@ -861,7 +978,13 @@ void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
   take(p.o_delta, 8, (size_t)p.scap * std::max<uint32_t>(T, 1));
   take(p.o_seed, 8, (size_t)p.scap * std::max<uint32_t>(T, 1));
   take(p.o_nrec, kNRec, p.ncap);
-  take(p.o_heap, 16, p.hcap);
+  if (p.bq_cap) {
+    take(p.o_bqe, 16, (size_t)kBuckets * p.bq_cap);
+    take(p.o_bqc, 4, kBuckets);
+    p.o_heap = 0;
+  } else {
+    take(p.o_heap, 16, p.hcap);
+  }
   p.wave_stride = o;
 }
 
@@ -935,7 +1058,7 @@ uint32_t prepare_ang(AngWorkspace &w, const AngPlan &p, uint64_t n_items, hipStr
 static uint4 *angular_adjacency(Graph &g, DevEdges &de) {
   if (de.ang_adj) return de.ang_adj;
   std::vector<uint4> adj(g.m);
-  float min_es = INFINITY;
+  float min_es = INFINITY, max_w = 0.0f;
   for (uint32_t v = 0; v < g.n; ++v) {
     for (uint32_t j = g.fwd_off[v]; j < g.fwd_off[v + 1]; ++j) {
       const uint32_t e = g.fwd_eid[j];
@@ -948,12 +1071,14 @@ static uint4 *angular_adjacency(Graph &g, DevEdges &de) {
       std::memcpy(&tbb, &tb, 4);
       adj[j] = make_uint4(u | ((uint32_t)g.slot_src[e] << 30) | ((uint32_t)g.slot_dst[e] << 31), eb, ab, tbb);
       min_es = std::min(min_es, es);
+      max_w = std::max(max_w, g.eang[e] + tb);
     }
   }
   hip_check(hipMalloc(&de.ang_adj, sizeof(uint4) * std::max<size_t>(adj.size(), 1)), "hipMalloc");
   if (!adj.empty())
     hip_check(hipMemcpy(de.ang_adj, adj.data(), sizeof(uint4) * adj.size(), hipMemcpyHostToDevice), "upload");
   de.min_es_ang = min_es;
+  de.max_w_ang = max_w;
   return de.ang_adj;
 }
 
@@ -1020,6 +1145,10 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     j.N = g.n;
     j.off = sp.dg->fwd_off;
     j.adj = angular_adjacency(g, *sp.de);
+    // buckets of 1 / kBucketSpan of the largest route increment: a pushed key
+    // lies at most kBucketSpan + 1 buckets (with rounding) past the last pop
+    const float mw = sp.de->max_w_ang;
+    j.bq_inv = (mw > 0.0f && std::isfinite(mw)) ? (float)kBucketSpan / mw : 0.0f;
     j.eligible = sp.d_eligible;
     j.T = sp.cs.T;
     j.maxsec = sp.cs.maxsec;
@@ -1040,6 +1169,9 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
   p.scap = std::min<uint32_t>(2 * N, cap0 ? std::max<uint32_t>(cap0, 2) : 8192u);
   p.ncap = std::min<uint32_t>(N, p.scap);
   p.hcap = 2 * p.scap;
+  // sources search with the bucket queue first; one that overfills a bucket
+  // (or whose graph's costs leave the ring) runs again with the heap
+  p.bq_cap = kBqCap;
   const int dev = specs[0].dg->device;
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, dev), "props");
@@ -1105,6 +1237,9 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     a.o_seed = p.o_seed;
     a.o_nrec = p.o_nrec;
     a.o_heap = p.o_heap;
+    a.o_bqe = p.o_bqe;
+    a.o_bqc = p.o_bqc;
+    a.bq_cap = p.bq_cap;
     a.per_wave = p.per_wave;
     a.epoch_base = epoch_base;
     a.next_source = d_ctl;
@@ -1113,7 +1248,10 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     a.fail_list = d_fail;
     a.progress = lanes <= kProgressSlots ? progress : nullptr;
     hip_check(hipEventRecord(e0, stream), "event");
-    hipLaunchKernelGGL(k_angular, dim3(lanes / kBlock), dim3(kBlock), 0, stream, a);
+    if (p.bq_cap)
+      hipLaunchKernelGGL(k_angular<1>, dim3(lanes / kBlock), dim3(kBlock), 0, stream, a);
+    else
+      hipLaunchKernelGGL(k_angular<0>, dim3(lanes / kBlock), dim3(kBlock), 0, stream, a);
     hip_check(hipGetLastError(), "k_angular launch");
     hip_check(hipEventRecord(e1, stream), "event");
     uint32_t ctl[4];
@@ -1140,7 +1278,11 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     n_list = (uint32_t)items.size();
     if (flags & kFailCap) p.scap = std::min<uint32_t>(2 * N, p.scap * 4);
     p.ncap = std::min<uint32_t>(N, p.scap);
-    if (flags & (kFailHeap | kFailCap)) p.hcap = std::max<uint32_t>(p.hcap * 4, 2 * p.scap);
+    if (p.bq_cap && (flags & kFailHeap)) {
+      p.bq_cap = 0;  // the heap from here on, at its first size
+    } else if (flags & (kFailHeap | kFailCap)) {
+      p.hcap = std::max<uint32_t>(p.hcap * 4, 2 * p.scap);
+    }
   }
   unsigned long long stv[4];
   hip_check(hipMemcpy(stv, d_stats, 32, hipMemcpyDeviceToHost), "copy");
