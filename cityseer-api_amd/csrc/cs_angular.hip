@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace cs {
 
@@ -200,6 +201,12 @@ constexpr uint32_t kMaskWords = 2;                // occupancy mask words held i
 constexpr uint32_t kBuckets = 64 * kMaskWords;     // ring of 128 buckets
 constexpr uint32_t kBucketSpan = kBuckets - 8;     // buckets per largest route increment
 constexpr uint32_t kBqCap = 64;  // entries per bucket before a source falls back to the heap
+// Entries per bucket: kBqCap, or 1/128 of a tuned slot capacity (at least 2),
+// so that a small capacity also sends sources to the heap early.
+uint32_t bucket_capacity() {
+  const uint32_t cap0 = default_tuning().cap;
+  return cap0 ? std::min<uint32_t>(kBqCap, std::max<uint32_t>(cap0 / 128, 2u)) : kBqCap;
+}
 struct BQueue {
   char *w;
   size_t off_e, off_c;
@@ -284,6 +291,16 @@ struct BQueue {
     return best;
   }
 };
+
+// The lane's queue for a search: the bucket ring (QK 1) or the heap.
+template <int QK>
+__device__ __forceinline__ auto make_queue(const AngArgs &a, CAngJob &jb, char *w, uint32_t lane) {
+  if constexpr (QK == 1) {
+    return BQueue{w, a.o_bqe, a.o_bqc, lane, a.bq_cap, jb.bq_inv};
+  } else {
+    return Heap{w, a.o_heap, lane, 0, a.hcap};
+  }
+}
 
 // One source, exactly as dijkstra_brandes_angular Phase 1 (centrality.rs:577-705).
 // Returns false (with fail bits) if a workspace capacity is exceeded.
@@ -585,13 +602,8 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     const uint32_t epoch = a.epoch_base + it + 1;
     uint32_t n_states, n_nodes, n_order, fail = 0;
     bool ok;
-    if constexpr (QK == 1) {
-      BQueue q{w, a.o_bqe, a.o_bqc, lane, a.bq_cap, jb.bq_inv};
-      ok = angular_source(a, jb, L, q, src, epoch, n_states, n_nodes, n_order, fail);
-    } else {
-      Heap q{w, a.o_heap, lane, 0, a.hcap};
-      ok = angular_source(a, jb, L, q, src, epoch, n_states, n_nodes, n_order, fail);
-    }
+    auto q = make_queue<QK>(a, jb, w, lane);
+    ok = angular_source(a, jb, L, q, src, epoch, n_states, n_nodes, n_order, fail);
     if (ok && jb.tolerance > kTieEps && (jb.flags & 2u)) ok = angular_phase2(a, jb, L, epoch, n_order, fail);
     if (!ok) {
       const uint32_t q = atomicAdd(a.n_fail, 1u);
@@ -826,6 +838,8 @@ __global__ void k_tree_simplest(AngArgs a, uint32_t src) {
 // out: the item is listed in fail_list and searched again with more room).
 // State records keep route, seconds, state and the simplest distance (in the
 // predecessor-count word); node records node, simplest distance and seconds.
+// QK 1: bucket queue, 0: 4-ary heap.
+template <int QK>
 __global__ __launch_bounds__(kBlock) void k_reach_simplest(AngArgs a, const uint32_t *items, uint64_t *lists,
                                                            uint32_t *counts) {
   const uint32_t lane = __lane_id();
@@ -845,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_reach_simplest(AngArgs a, const uint
     const uint32_t src = jb.sources[items ? items[it] : it];
     const uint32_t epoch = a.epoch_base + it + 1;
     uint32_t n_states = 0, n_nodes = 1, fail = 0;
-    Heap h{w, a.o_heap, lane, 0, a.hcap};
+    std::conditional_t<QK == 1, BQueue, Heap> h = make_queue<QK>(a, jb, w, lane);
     L.nkey[src] = ((uint64_t)epoch << 32) | 0u;
     *nr<uint32_t>(w, a.o_nrec, 0, lane, a.ncap, kN_node) = src;
     *nr<float>(w, a.o_nrec, 0, lane, a.ncap, kN_brt) = 0.0f;
@@ -1171,7 +1185,7 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
   p.hcap = 2 * p.scap;
   // sources search with the bucket queue first; one that overfills a bucket
   // (or whose graph's costs leave the ring) runs again with the heap
-  p.bq_cap = kBqCap;
+  p.bq_cap = bucket_capacity();
   const int dev = specs[0].dg->device;
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, dev), "props");
@@ -1384,8 +1398,9 @@ static_assert(kFailCap == kReachFailCap && kFailHeap == kReachFailHeap, "reach f
 
 std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de, float maxsec,
                                             const uint32_t *d_src, const uint32_t *d_items, uint32_t n_items,
-                                            uint32_t cap, uint32_t hcap, uint64_t *d_lists, uint32_t *d_counts,
-                                            hipStream_t stream, cs_stats &st, uint32_t *fail_bits) {
+                                            uint32_t cap, uint32_t hcap, bool heap, uint64_t *d_lists,
+                                            uint32_t *d_counts, hipStream_t stream, cs_stats &st,
+                                            uint32_t *fail_bits) {
   *fail_bits = 0;
   if (n_items == 0) return {};
   const uint32_t N = g.n;
@@ -1398,17 +1413,27 @@ std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de
   job.adj = angular_adjacency(g, de);
   job.maxsec = maxsec;
   job.sources = d_src;
-  // state records, node records and the heap of each lane, interleaved per wave
+  const float mw = de.max_w_ang;
+  job.bq_inv = (mw > 0.0f && std::isfinite(mw)) ? (float)kBucketSpan / mw : 0.0f;
+  // state records, node records and the queue (bucket ring or heap) of each lane, interleaved per wave
   AngPlan p;
   p.per_wave = kWave;
   p.scap = std::min<uint32_t>(2 * N, 2 * cap);
   p.ncap = cap;
-  p.hcap = hcap;
+  p.hcap = heap ? hcap : 0;
+  p.bq_cap = heap ? 0 : bucket_capacity();
   p.lane_stride = align256(8ull * 3 * N);
   p.o_srec = 0;
   p.o_nrec = align256((size_t)kSRec * p.scap * kWave);
-  p.o_heap = p.o_nrec + align256((size_t)kNRec * p.ncap * kWave);
-  p.wave_stride = p.o_heap + align256((size_t)16 * p.hcap * kWave);
+  const size_t o_q = p.o_nrec + align256((size_t)kNRec * p.ncap * kWave);
+  if (heap) {
+    p.o_heap = o_q;
+    p.wave_stride = o_q + align256((size_t)16 * p.hcap * kWave);
+  } else {
+    p.o_bqe = o_q;
+    p.o_bqc = o_q + align256((size_t)16 * kBuckets * p.bq_cap * kWave);
+    p.wave_stride = p.o_bqc + align256((size_t)4 * kBuckets * kWave);
+  }
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, dev), "props");
   size_t free_b = 0, total_b = 0;
@@ -1447,6 +1472,9 @@ std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de
   a.o_srec = p.o_srec;
   a.o_nrec = p.o_nrec;
   a.o_heap = p.o_heap;
+  a.o_bqe = p.o_bqe;
+  a.o_bqc = p.o_bqc;
+  a.bq_cap = p.bq_cap;
   a.per_wave = kWave;
   a.epoch_base = epoch_base;
   a.n_fail = d_ctl.p + 1;
@@ -1456,7 +1484,12 @@ std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
   hip_check(hipEventRecord(e0, stream), "event");
-  hipLaunchKernelGGL(k_reach_simplest, dim3(p.lanes / kBlock), dim3(kBlock), 0, stream, a, d_items, d_lists, d_counts);
+  if (heap)
+    hipLaunchKernelGGL(k_reach_simplest<0>, dim3(p.lanes / kBlock), dim3(kBlock), 0, stream, a, d_items, d_lists,
+                       d_counts);
+  else
+    hipLaunchKernelGGL(k_reach_simplest<1>, dim3(p.lanes / kBlock), dim3(kBlock), 0, stream, a, d_items, d_lists,
+                       d_counts);
   hip_check(hipGetLastError(), "k_reach_simplest launch");
   hip_check(hipEventRecord(e1, stream), "event");
   uint32_t ctl[4];

@@ -1412,6 +1412,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
       const uint32_t cap0 = default_tuning().cap;
       uint32_t cap = std::min<uint32_t>(N, cap0 ? std::max<uint32_t>(cap0 / 2, 1u) : 4096u);
       uint32_t hcap = (uint32_t)std::min<uint64_t>(hmax, 4ull * cap);
+      bool heap = false;  // the bucket ring first; the heap once a bucket overfills
       std::vector<uint32_t> pending(srcs.size());
       for (uint32_t i = 0; i < (uint32_t)pending.size(); ++i) pending[i] = i;
       while (!pending.empty()) {
@@ -1427,7 +1428,7 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
           hip_check(hipMemcpyAsync(d_items.p, pending.data() + c0, 4ull * n, hipMemcpyHostToDevice, stream), "copy");
           uint32_t b = 0;
           for (uint32_t f : launch_reach_simplest(g, dg, de, (float)max_seconds, d_src.as<uint32_t>(),
-                                                  d_items.as<uint32_t>(), n, cap, hcap, d_lists.as<uint64_t>(),
+                                                  d_items.as<uint32_t>(), n, cap, hcap, heap, d_lists.as<uint64_t>(),
                                                   d_counts.as<uint32_t>(), stream, t_stats, &b))
             next.push_back(pending[c0 + f]);
           bits |= b;
@@ -1437,10 +1438,12 @@ static int access_impl(cs_graph *h, uint32_t n_entries, const uint32_t *entry_gr
                                 t0 + kMaxT >= T ? prog : nullptr);
         }
         if (next.empty()) break;
-        if (cap >= N && hcap >= hmax) fail(CS_EINTERNAL, "angular reach lists: capacity retries exhausted");
+        if (cap >= N && heap && hcap >= hmax) fail(CS_EINTERNAL, "angular reach lists: capacity retries exhausted");
         t_stats.retries += next.size();
         if (bits & kReachFailCap) cap = (uint32_t)std::min<uint64_t>(N, 4ull * cap);
-        hcap = (uint32_t)std::min<uint64_t>(hmax, std::max<uint64_t>(4ull * cap, (bits & kReachFailHeap) ? 4ull * hcap : hcap));
+        const bool grow_heap = heap && (bits & kReachFailHeap);
+        if (bits & kReachFailHeap) heap = true;
+        hcap = (uint32_t)std::min<uint64_t>(hmax, std::max<uint64_t>(4ull * cap, grow_heap ? 4ull * hcap : hcap));
         pending.swap(next);
       }
     }

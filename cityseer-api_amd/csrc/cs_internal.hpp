@@ -355,15 +355,17 @@ void launch_tree_simplest(Graph &g, DevGraph &dg, DevEdges &de, uint32_t src, fl
 // data aggregation: item i searches from d_src[d_items[i]] (d_src[i] without
 // d_items) and leaves {node | seconds << 32} of the nodes it reaches, in
 // first-reach order, in d_lists[i * cap ...] and their number in d_counts[i].
-// Items whose search ran out of room (more than cap nodes or 2 cap states, or
-// more than hcap heap entries) get kNoList; their positions are returned, the
-// causes (kReachFailCap, kReachFailHeap) in *fail_bits.
+// Items whose search ran out of room (more than cap nodes or 2 cap states, or a
+// full queue: an overfilled bucket of the bucket ring, or more than hcap
+// entries of the heap that `heap` selects) get kNoList; their positions are
+// returned, the causes (kReachFailCap, kReachFailHeap) in *fail_bits.
 constexpr uint32_t kNoList = 0xffffffffu;
 constexpr uint32_t kReachFailCap = 1u, kReachFailHeap = 2u;
 std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de, float maxsec,
                                             const uint32_t *d_src, const uint32_t *d_items, uint32_t n_items,
-                                            uint32_t cap, uint32_t hcap, uint64_t *d_lists, uint32_t *d_counts,
-                                            hipStream_t stream, cs_stats &st, uint32_t *fail_bits);
+                                            uint32_t cap, uint32_t hcap, bool heap, uint64_t *d_lists,
+                                            uint32_t *d_counts, hipStream_t stream, cs_stats &st,
+                                            uint32_t *fail_bits);
 
 // Data aggregation (accessibility rows, class counts or statistics, as `out`
 // asks) over reached-node lists (cs_search.hip): item i is source

@@ -382,7 +382,10 @@ typedef struct cs_tuning {
   uint32_t waves_per_simd;  /* cap on resident waves per SIMD; 0 = as many as fit */
   float delta_scale;        /* bucket width as a fraction in (0, 1] of its safe maximum */
   uint32_t slot_capacity;   /* settled-node slots per wave (simplest: search states per lane) before a
-                               retry with four times as many; 0 = default (16384; simplest 8192) */
+                               retry with four times as many; 0 = default (16384; simplest 8192). For
+                               simplest paths 1/128 of it (2..64, default 64) also bounds the entries of
+                               one bucket of a lane's route-cost queue, beyond which the source runs
+                               again with a heap */
   uint32_t xcd_split;       /* 1: each XCD's waves take a contiguous share of the sources first; 0: one counter */
   uint32_t simplest_lanes;  /* simplest: sources per wave (1..64), 0 = chosen from the launch's source count */
 } cs_tuning;
