@@ -12,11 +12,13 @@ device synchronisations.
 ``--mode simplest`` / ``--mode segment``: the dual (simplest-path, angular
 impedance) and the primal (segment) of a 150x150 perturbed lattice at
 [500, 1000, 2000] m through the NetworkStructure methods; ``value`` uses the
-kernel time of each call (graph resident, sources staged before the kernel).
+whole C call (graph resident; per-call tables in, rows out), with the kernel
+time beside it.
 
 ``--mode access``: land-use accessibility (DataMap.accessibility) on the
 200x200 lattice at [400, 800] m over about two synthetic data entries per node;
-``value`` uses the kernel time of the search-and-aggregate launch.
+``value`` uses the whole C call. Nested legs: the same with one dedupe group of
+2,000 members, and angular accessibility on the dual of a 150x150 lattice.
 
 Prints one JSON line.
 """
@@ -247,9 +249,9 @@ def access_workload(ns, lat, per_node: float, seed: int):
     return dm, lu
 
 
-def access_cpu_baseline(ns, dm, lu, target_s: float):
-    """The oracle's restatement (one thread, Python aggregation over the C tree search) over a
-    bounded random sample of sources."""
+def access_cpu_baseline(ns, dm, lu, target_s: float, angular: bool = False):
+    """The oracle's restatement (one thread, Python aggregation over the C tree search, the
+    simplest-path tree with `angular`) over a bounded random sample of sources."""
     import numpy as np
 
     from oracle import oracle as O
@@ -268,7 +270,7 @@ def access_cpu_baseline(ns, dm, lu, target_s: float):
         src = np.sort(rng.choice(n, k, replace=False)).tolist()
         t0 = time.perf_counter()
         oracle_access(og, [1] * n, ndm, dedupe_of, lu, ACCESS_CLASSES[:3], ACCESS_DISTANCES, bets,
-                      [1.0] * len(ACCESS_DISTANCES), max(secs), 1.33333, sources=src)
+                      [1.0] * len(ACCESS_DISTANCES), max(secs), 1.33333, sources=src, angular=angular)
         return time.perf_counter() - t0
 
     k0 = 32
@@ -376,6 +378,38 @@ def run_access(args) -> int:
             "ms_per_step": round(float(np.mean(tg)), 3),
             "kernel_ms": round(float(np.mean(kg)), 3),
             "kernel_over_plain": round(float(np.mean(kg)) / kms, 3),
+        }
+    if args.access_angular_side:
+        # angular aggregation: simplest-path reach lists on the dual of a lattice,
+        # then the same aggregation over them
+        side = args.access_angular_side
+        log(f"building angular access workload (dual of a {side}x{side} lattice)")
+        dlat = mock.lattice_dual(mock.perturbed_lattice(side, seed=args.seed))
+        dns = mock.network_structure_from_lattice(dlat)
+        ddm, dlu = access_workload(dns, dlat, 2.0, args.seed + 3)
+        dn = dns.node_count()
+
+        def angular_call():
+            ddm.accessibility(dns, dlu, ACCESS_CLASSES[:3], distances=ACCESS_DISTANCES, angular=True)
+            return _native.last_stats()
+
+        for _ in range(args.warmup):
+            angular_call()
+        ka, ta = [], []
+        for i in range(args.steps):
+            st = angular_call()
+            ka.append(st["kernel_ms"])
+            ta.append(st["total_ms"])
+            log(f"angular access step {i}: kernels {st['kernel_ms']:.2f} ms, call {st['total_ms']:.2f} ms")
+        line["angular"] = {
+            "workload": f"dual of a {side}x{side} perturbed lattice, DataMap.accessibility(angular=True) "
+                        f"{ACCESS_DISTANCES} m, {ddm.count()} entries, all {dn} sources",
+            "value": round(dn / (float(np.mean(ta)) / 1000.0), 2),
+            "unit": "source searches/sec",
+            "ms_per_step": round(float(np.mean(ta)), 3),
+            "kernel_ms": round(float(np.mean(ka)), 3),
+            "cpu_baseline": None if args.no_cpu else access_cpu_baseline(dns, ddm, dlu, args.cpu_seconds / 2,
+                                                                         angular=True),
         }
     print(json.dumps(line), flush=True)
     return 0
@@ -759,6 +793,8 @@ def main() -> int:
     ap.add_argument("--serve-max-nodes", type=float, default=1e6, help="serve mode: largest graph")
     ap.add_argument("--slice-ms", type=float, default=0.0,
                     help="serve mode: target duration of one batched launch (0: the server's default)")
+    ap.add_argument("--access-angular-side", type=int, default=150,
+                    help="access mode: also time angular accessibility on the dual of a lattice this wide (0: no)")
     ap.add_argument("--access-group", type=int, default=2000,
                     help="access mode: also time the workload with one dedupe group of this many members (0: no)")
     ap.add_argument("--no-legs", action="store_true",
