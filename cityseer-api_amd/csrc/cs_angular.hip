@@ -27,6 +27,8 @@ namespace {
 constexpr uint32_t kFailCap = 1u, kFailHeap = 2u, kFailPreds = 4u;
 constexpr uint32_t kNoSource = 0xffffffffu;  // padding item
 constexpr uint32_t kRelax = 4;               // out-edges whose loads one relaxation step batches
+constexpr uint32_t kNoEdge = 0xffffffffu;     // padding entry of a state row
+constexpr uint32_t kRowSpill = 0xfffffffeu;   // first entry of a state row whose edges do not fit
 
 // One (graph, request) pair of a launch: a plain call has one, a served slice
 // one per active request.
@@ -34,6 +36,7 @@ struct AngJob {
   uint32_t N;           // nodes of the graph (accumulator rows)
   const uint32_t *off;  // out-edge rows (petgraph order)
   const uint4 *adj;     // {v | src_slot<<30 | dst_slot<<31, bits(es), bits(angle), bits(1e-6*len)}
+  const uint4 *srow;    // [2N][kRelax] each state's usable out-edges in row order (kNoEdge pads; kRowSpill: use the row)
   const uint8_t *eligible;
   uint32_t T;
   float maxsec, tolerance, unit, offset;
@@ -340,7 +343,13 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, Q
     const float u_route = __uint_as_float(uhd.x);
     const float u_secs = __uint_as_float(uhd.y);
     const double u_sigma = *sr<double>(w, a.o_srec, uss, lane, a.scap, kS_sigma);
-    const uint32_t j0 = jb.off[u], j1 = jb.off[u + 1];
+    // the state's usable out-edges: its state row (independent of the record
+    // load), or for a state with more than kRelax of them the node's out-row
+    uint4 trow[kRelax];
+#pragma unroll
+    for (uint32_t q = 0; q < kRelax; ++q) trow[q] = jb.srow[(size_t)us * kRelax + q];
+    const bool from_row = trow[0].x == kRowSpill;
+    const uint32_t j0 = from_row ? jb.off[u] : 0u, j1 = from_row ? jb.off[u + 1] : kRelax;
     if (ustate & 0x80000000u) continue;
     *ust = us | 0x80000000u;
     if (n_order >= a.scap) {
@@ -356,7 +365,8 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, Q
     for (uint32_t jc = j0; jc < j1; jc += kRelax) {
       uint4 e[kRelax];
 #pragma unroll
-      for (uint32_t q = 0; q < kRelax; ++q) e[q] = (jc + q < j1) ? jb.adj[jc + q] : make_uint4(0u, 0u, 0u, 0u);
+      for (uint32_t q = 0; q < kRelax; ++q)
+        e[q] = !from_row ? trow[q] : (jc + q < j1) ? jb.adj[jc + q] : make_uint4(0u, 0u, 0u, 0u);
       bool ok[kRelax];
       uint32_t vv[kRelax], vsv[kRelax];
       float csv[kRelax];
@@ -366,7 +376,8 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, Q
         vv[q] = e[q].x & 0x3fffffffu;
         vsv[q] = 2 * vv[q] + (e[q].x >> 31);
         csv[q] = u_secs + __uint_as_float(e[q].y);
-        ok[q] = jc + q < j1 && ((e[q].x >> 30) & 1u) == 1u - entry && !(csv[q] > jb.maxsec);
+        ok[q] = (from_row ? (jc + q < j1 && ((e[q].x >> 30) & 1u) == 1u - entry) : e[q].x != kNoEdge) &&
+                !(csv[q] > jb.maxsec);
         skv[q] = ok[q] ? L.skey[vsv[q]] : 0ull;
         nkv[q] = ok[q] ? L.nkey[vv[q]] : 0ull;
       }
@@ -1096,6 +1107,37 @@ static uint4 *angular_adjacency(Graph &g, DevEdges &de) {
   return de.ang_adj;
 }
 
+// State rows of the dual: for state 2v + entry, the out-edges of v that leave
+// through its other endpoint (source slot 1 - entry), in row order, padded with
+// kNoEdge; kRowSpill first when more than kRelax of them. A search then loads a
+// settled state's edges with its record instead of after its row offsets.
+static uint4 *angular_state_rows(Graph &g, DevEdges &de) {
+  if (de.ang_srow) return de.ang_srow;
+  angular_adjacency(g, de);  // de.ang_adj, read back below
+  std::vector<uint4> adj(g.m);
+  if (g.m) hip_check(hipMemcpy(adj.data(), de.ang_adj, sizeof(uint4) * g.m, hipMemcpyDeviceToHost), "download");
+  const uint4 pad = make_uint4(kNoEdge, 0u, 0u, 0u);
+  std::vector<uint4> rows((size_t)2 * g.n * kRelax, pad);
+  for (uint32_t v = 0; v < g.n; ++v) {
+    for (uint32_t entry = 0; entry < 2; ++entry) {
+      uint4 *row = &rows[((size_t)2 * v + entry) * kRelax];
+      uint32_t k = 0;
+      for (uint32_t j = g.fwd_off[v]; j < g.fwd_off[v + 1]; ++j) {
+        if (((adj[j].x >> 30) & 1u) != 1u - entry) continue;
+        if (k == kRelax) {
+          row[0] = make_uint4(kRowSpill, 0u, 0u, 0u);
+          break;
+        }
+        row[k++] = adj[j];
+      }
+    }
+  }
+  hip_check(hipMalloc(&de.ang_srow, sizeof(uint4) * std::max<size_t>(rows.size(), 1)), "hipMalloc");
+  if (!rows.empty())
+    hip_check(hipMemcpy(de.ang_srow, rows.data(), sizeof(uint4) * rows.size(), hipMemcpyHostToDevice), "upload");
+  return de.ang_srow;
+}
+
 // Items grouped by job (stable) with each job's run padded to whole waves.
 // Items grouped by job, `per_wave` real items to each wave of 64 (the rest
 // padding), so that the 64 lanes of a wave always belong to one job.
@@ -1159,6 +1201,7 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     j.N = g.n;
     j.off = sp.dg->fwd_off;
     j.adj = angular_adjacency(g, *sp.de);
+    j.srow = angular_state_rows(g, *sp.de);
     // buckets of 1 / kBucketSpan of the largest route increment: a pushed key
     // lies at most kBucketSpan + 1 buckets (with rounding) past the last pop
     const float mw = sp.de->max_w_ang;

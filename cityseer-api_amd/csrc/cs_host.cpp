@@ -169,6 +169,7 @@ void free_edges(DevEdges &d) {
   (void)hipFree(d.rev_eid);
   (void)hipFree(d.fwd_eid);
   (void)hipFree(d.ang_adj);
+  (void)hipFree(d.ang_srow);
 }
 
 }  // namespace

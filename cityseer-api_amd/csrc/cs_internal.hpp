@@ -62,6 +62,7 @@ struct DevEdges {
   float max_es = 0.0f;
   // angular (forward search over out-edges on the 2-state dual)
   uint4 *ang_adj = nullptr;  // {v | srcslot<<30 | dstslot<<31 ... see cs_angular.hip}
+  uint4 *ang_srow = nullptr;  // [2N][4] usable out-edges of each dual state (cs_angular.hip)
   float min_es_ang = 0.0f;
   float max_w_ang = 0.0f;  // largest route increment (angle + 1e-6 length) of the dual's edges
 };
