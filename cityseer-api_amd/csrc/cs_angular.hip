@@ -443,15 +443,17 @@ __device__ bool angular_source(const AngArgs &a, CAngJob &jb, const LaneWs &L, Q
         uint32_t *preds = reinterpret_cast<uint32_t *>(w + a.o_preds) + ((size_t)lane * a.scap + vss) * a.maxp;
         if (cr < cur) {
           double *sg = sr<double>(w, a.o_srec, vss, lane, a.scap, kS_sigma);
+          // a clear improvement restarts the predecessor list without reading it back
+          uint32_t k;
           if (cr < cur * one_m) {
-            *np = 0;
+            k = 0;
             *sg = u_sigma;
           } else {
+            k = *np;
             *sg = *sg + u_sigma;
           }
           *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_route) = cr;
           *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_secs) = cs;
-          const uint32_t k = *np;
           if (k >= a.maxp) {
             fail |= kFailPreds;
             return false;
