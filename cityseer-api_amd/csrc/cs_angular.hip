@@ -896,9 +896,23 @@ __global__ __launch_bounds__(kBlock) void k_reach_simplest(AngArgs a, const uint
       const uint32_t u = us >> 1, entry = us & 1u;
       const float u_secs = *sr<float>(w, a.o_srec, uss, lane, a.scap, kS_secs);
       const float u_simpl = *sr<float>(w, a.o_srec, uss, lane, a.scap, kS_np);
-      for (uint32_t j = jb.off[u]; j < jb.off[u + 1]; ++j) {
-        const uint4 e = jb.adj[j];
-        if (((e.x >> 30) & 1u) != 1u - entry) continue;
+      // the state's row (usable edges, row order), or the node's out-row when it spills
+      uint4 trow[kRelax];
+#pragma unroll
+      for (uint32_t q = 0; q < kRelax; ++q) trow[q] = jb.srow[(size_t)us * kRelax + q];
+      const bool from_row = trow[0].x == kRowSpill;
+      const uint32_t j0 = from_row ? jb.off[u] : 0u, j1 = from_row ? jb.off[u + 1] : kRelax;
+      for (uint32_t j = j0; j < j1; ++j) {
+        uint4 e;
+        if (from_row) {
+          e = jb.adj[j];
+          if (((e.x >> 30) & 1u) != 1u - entry) continue;
+        } else {
+          e = trow[0];
+#pragma unroll
+          for (uint32_t q = 1; q < kRelax; ++q) e = (j == q) ? trow[q] : e;
+          if (e.x == kNoEdge) break;  // padding follows the last usable edge
+        }
         const uint32_t v = e.x & 0x3fffffffu;
         const uint32_t vs = 2 * v + (e.x >> 31);
         const float cs = u_secs + __uint_as_float(e.y);
@@ -1458,6 +1472,7 @@ std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de
   job.adj = angular_adjacency(g, de);
   job.maxsec = maxsec;
   job.sources = d_src;
+  job.srow = angular_state_rows(g, de);
   const float mw = de.max_w_ang;
   job.bq_inv = (mw > 0.0f && std::isfinite(mw)) ? (float)kBucketSpan / mw : 0.0f;
   // state records, node records and the queue (bucket ring or heap) of each lane, interleaved per wave
