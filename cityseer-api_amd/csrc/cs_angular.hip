@@ -532,7 +532,7 @@ __device__ bool angular_phase2(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
   const uint32_t lane = L.lane;
   for (uint32_t q = 0; q < n_order; ++q) {
     const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane, a.scap);
-    *il<double>(w, a.o_delta, ss, lane, (size_t)a.scap * a.tcap) = (double)q;
+    *il<double>(w, a.o_delta, ss * a.tcap, lane, (size_t)a.scap * a.tcap) = (double)q;
     *sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_np) = 0;
     *sr<double>(w, a.o_srec, ss, lane, a.scap, kS_sigma) = 0.0;
   }
@@ -555,7 +555,7 @@ __device__ bool angular_phase2(const AngArgs &a, CAngJob &jb, const LaneWs &L, u
       if ((uint32_t)(sk >> 32) != epoch) continue;
       const uint32_t vss = (uint32_t)sk;
       if (!(*sr<uint32_t>(w, a.o_srec, vss, lane, a.scap, kS_state) & 0x80000000u)) continue;  // never settled
-      if (*il<double>(w, a.o_delta, vss, lane, (size_t)a.scap * a.tcap) <= (double)pos) continue;       // settled before u
+      if (*il<double>(w, a.o_delta, vss * a.tcap, lane, (size_t)a.scap * a.tcap) <= (double)pos) continue;  // settled before u
       float cr = u_route + __uint_as_float(e.z);
       cr = cr + __uint_as_float(e.w);
       if (!(cr <= *sr<float>(w, a.o_srec, vss, lane, a.scap, kS_route) * band)) continue;
@@ -654,12 +654,13 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     // walk over the settled states; each threshold keeps its own seeds and
     // dependencies ([T][scap]) and sees its terms in the reference's order
     if (jb.flags & 2u) {
-      const size_t sc = a.scap;
+      // seeds and dependencies are state-major: a state's thresholds side by side
+      const uint32_t tc = a.tcap;
       for (uint32_t q = 0; q < n_order; ++q) {
         const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane, a.scap);
         for (uint32_t t = 0; t < T; ++t) {
-          *il<double>(w, a.o_delta, t * sc + ss, lane, (size_t)a.scap * a.tcap) = 0.0;
-          *il<double>(w, a.o_seed, t * sc + ss, lane, (size_t)a.scap * a.tcap) = 0.0;
+          *il<double>(w, a.o_delta, ss * tc + t, lane, (size_t)a.scap * a.tcap) = 0.0;
+          *il<double>(w, a.o_seed, ss * tc + t, lane, (size_t)a.scap * a.tcap) = 0.0;
         }
       }
       // seeds: best terminal states split by sigma (best_angular_target_states)
@@ -693,7 +694,7 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           for (uint32_t q = 0; q < nbst; ++q) total += *sr<double>(w, a.o_srec, best[q], lane, a.scap, kS_sigma);
           if (total == 0.0) continue;
           for (uint32_t q = 0; q < nbst; ++q) {
-            double *sd = il<double>(w, a.o_seed, t * sc + best[q], lane, (size_t)a.scap * a.tcap);
+            double *sd = il<double>(w, a.o_seed, best[q] * tc + t, lane, (size_t)a.scap * a.tcap);
             *sd = *sd + pc * (*sr<double>(w, a.o_srec, best[q], lane, a.scap, kS_sigma) / total);
           }
         }
@@ -718,14 +719,14 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           const uint32_t node = (*sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_state) & 0x7fffffffu) >> 1;
           for (uint32_t t = 0; t < T; ++t) {
             if (!(secs_ss <= jb.sec_thr[t])) continue;
-            const double seed = *il<double>(w, a.o_seed, t * sc + ss, lane, (size_t)a.scap * a.tcap);
-            const double dep = seed + *il<double>(w, a.o_delta, t * sc + ss, lane, (size_t)a.scap * a.tcap);
+            const double seed = *il<double>(w, a.o_seed, ss * tc + t, lane, (size_t)a.scap * a.tcap);
+            const double dep = seed + *il<double>(w, a.o_delta, ss * tc + t, lane, (size_t)a.scap * a.tcap);
             if (dep == 0.0) continue;
             for (uint32_t p = 0; p < k; ++p) {
               const uint32_t ps = preds[p];
               const double sv = *sr<double>(w, a.o_srec, ps, lane, a.scap, kS_sigma);
               if (sv == 0.0) continue;
-              double *dp = il<double>(w, a.o_delta, t * sc + ps, lane, (size_t)a.scap * a.tcap);
+              double *dp = il<double>(w, a.o_delta, ps * tc + t, lane, (size_t)a.scap * a.tcap);
               *dp = *dp + (sv / sw) * dep;
             }
             if (node == src) continue;
