@@ -299,3 +299,21 @@ def test_angular_data_layer_needs_a_dual_graph():
                  lambda: dm.aggregate_to_src_idx(0, ns, 300, 1.0, angular=True)):
         with pytest.raises(ValueError, match="requires a dual graph for angular analysis"):
             call()
+
+
+def test_angular_data_layer_needs_endpoint_keys():
+    """A graph flagged dual whose edges carry no shared primal node key has no endpoint slots: the
+    angular data layer refuses it before any search (centrality.rs:533-566 expects the key)."""
+    ns = NetworkStructure()
+    for i in range(3):
+        ns.add_street_node(i, i * 100.0, 0.0, True, 1.0)
+    ns.add_street_edge(0, 1, 0, 0, 1, "LINESTRING (0 0, 100 0)")
+    ns.add_street_edge(1, 2, 0, 1, 2, "LINESTRING (100 0, 200 0)")
+    ns.set_is_dual(True)
+    dm = data.DataMap()
+    dm.insert("a", "POINT (0 0)")
+    dm.node_data_map = {1: [("str:a", 1.0)]}
+    with pytest.raises(ValueError, match="shared_primal_node_key"):
+        dm.accessibility(ns, {"a": "shop"}, ["shop"], distances=[200], angular=True)
+    with pytest.raises(ValueError, match="shared_primal_node_key"):
+        dm.stats(ns, [{"a": 1.0}], distances=[200], angular=True)
