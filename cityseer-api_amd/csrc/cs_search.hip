@@ -2971,6 +2971,15 @@ void launch_access_lists(const AccessOut &out, const uint32_t *d_src, const uint
   for (int attempt = 0; attempt < 8; ++attempt) {
     const size_t o_st = align256(16ull * out.n_shared);
     const size_t stride = o_st + (st_cap ? align256(256 + 40ull * st_cap) : 0);
+    // per-wave scratch (none without shared groups or statistics) within a
+    // quarter of free HBM: fewer waves for long winner lists
+    if (stride) {
+      size_t free_b = 0, total_b = 0;
+      hip_check(hipMemGetInfo(&free_b, &total_b), "meminfo");
+      const size_t fit = (free_b / 4) / stride / wpb * wpb;
+      if (fit == 0) fail(CS_ENOMEM, "not enough device memory for the aggregation scratch");
+      waves = std::min(waves, fit);
+    }
     DevMem<char> ws(waves * stride);
     hip_check(hipMemsetAsync(ws.p, 0, waves * stride, stream), "memset");
     hip_check(hipMemsetAsync(d_ctl.p, 0, 16, stream), "memset");
