@@ -46,6 +46,19 @@ def edge_geometry(polylines: Sequence[Sequence[Sequence[float]]]) -> tuple[np.nd
     return length, angle
 
 
+def edge_bearings(polylines: Sequence[Sequence[Sequence[float]]]) -> tuple[np.ndarray, np.ndarray]:
+    """(in_bearing f32[m], out_bearing f32[m]) of each polyline, in degrees, as ``add_street_edge``
+    stores them (graph.rs:764-857)."""
+    m = len(polylines)
+    off, pts = _flatten(polylines, 2)
+    ib = np.empty(m, dtype=np.float32)
+    ob = np.empty(m, dtype=np.float32)
+    p = _native.ptr
+    _native.check(_native.load().cs_edge_geometry(m, p(off, ctypes.c_uint32), p(pts, ctypes.c_double), None, None,
+                                                  p(ib, ctypes.c_float), p(ob, ctypes.c_float)))
+    return ib, ob
+
+
 def decompose(polylines: Sequence[Sequence[Sequence[float]]], decompose_max: float) -> list[list[np.ndarray]]:
     """Each polyline cut into ceil(length / decompose_max) equal pieces: per polyline, the list of
     piece point arrays ([k][2], or [k][3] when every input point carries z)."""

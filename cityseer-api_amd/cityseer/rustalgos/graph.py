@@ -70,6 +70,32 @@ class EdgeVisit:
         return cls()
 
 
+def _wkt_num(v: float) -> str:
+    """A coordinate as WKT writers print it: whole numbers without a fraction, others in the
+    shortest form that reads back exactly."""
+    r = repr(float(v))
+    return r[:-2] if r.endswith(".0") else r
+
+
+class EdgePayload:
+    """Read-only edge payload view (graph.rs:88-200): what ``get_edge_payload_py`` returns."""
+
+    def __init__(self, start_nd_key_py, end_nd_key_py, edge_idx, length, angle_sum, imp_factor, in_bearing,
+                 out_bearing, seconds, geom_wkt, shared_primal_node_key):
+        self.start_nd_key_py = start_nd_key_py
+        self.end_nd_key_py = end_nd_key_py
+        self.edge_idx = edge_idx
+        self.length = length
+        self.angle_sum = angle_sum
+        self.imp_factor = imp_factor
+        self.in_bearing = in_bearing
+        self.out_bearing = out_bearing
+        self.seconds = seconds
+        self.geom_wkt = geom_wkt
+        self.shared_primal_node_key = shared_primal_node_key
+        self.is_transport = False
+
+
 class NodePayload:
     """Read-only node payload view (graph.rs:24-86)."""
 
@@ -535,6 +561,27 @@ class NetworkStructure:
             raise ValueError(f"Edge not found: {start_nd_idx} -> {end_nd_idx} (edge_idx {edge_idx})")
         return int(hits[-1])
 
+    def get_edge_payload_py(self, start_nd_idx: int, end_nd_idx: int, edge_idx: int) -> EdgePayload:
+        """The edge's payload (graph.rs:1001-1011). Bearings come from its stored polyline (a
+        straight line between its nodes where none was given); the WKT is rendered from it."""
+        e = self._find_edge(start_nd_idx, end_nd_idx, edge_idx)
+        n = int(self._gn.array()[e])
+        if n:
+            o = int(self._goff.array()[e])
+            coords = self._gxy.array()[2 * o: 2 * (o + n)].reshape(-1, 2)
+        else:
+            a, b = int(self._esrc.array()[e]), int(self._edst.array()[e])
+            coords = np.array([[self._x.array()[a], self._y.array()[a]], [self._x.array()[b], self._y.array()[b]]])
+        ib, ob = _ingest.edge_bearings([coords.tolist()])
+        pk = int(self._epk.array()[e])
+        return EdgePayload(
+            self._keys[int(self._esrc.array()[e])], self._keys[int(self._edst.array()[e])], int(self._ekey.array()[e]),
+            float(self._elen.array()[e]), float(self._eang.array()[e]), float(self._eimp.array()[e]),
+            float(ib[0]), float(ob[0]), float(self._esec.array()[e]),
+            "LINESTRING (" + ", ".join(f"{_wkt_num(x)} {_wkt_num(y)}" for x, y in coords.tolist()) + ")",
+            None if pk < 0 else self._pk_names[pk],
+        )
+
     def get_edge_length(self, start_nd_idx: int, end_nd_idx: int, edge_idx: int) -> float:
         return float(self._elen.array()[self._find_edge(start_nd_idx, end_nd_idx, edge_idx)])
 
@@ -548,7 +595,8 @@ class NetworkStructure:
             raise ValueError("Invalid street node payload: weight must be finite and non-negative (>= 0.0).")
 
     def build_edge_rtree(self) -> None:
-        """Spatial index for data assignment: not used by the centrality path."""
+        """Kept for the reference's API (graph.rs:1064): the data assignment indexes the street
+        edges itself, in a grid built per call, so there is nothing to prepare here."""
         return None
 
     # -- native handle -------------------------------------------------------

@@ -261,3 +261,29 @@ def test_network_structure_from_gpd_checks(primal_graph, caplog):
     dropped = int(((edges["nx_start_node_key"] == gone) | (edges["nx_end_node_key"] == gone)).sum())
     assert dropped > 0 and fewer.edge_count == ns.edge_count - dropped
     assert "node nx key not found" in caplog.text
+
+
+def test_edge_payload_view_matches_the_polyline():
+    """get_edge_payload_py (graph.rs:1001-1011) on the mock graph: keys, index and stored length
+    of each edge, bearings of its first and last segments, and its polyline as WKT."""
+    import math
+
+    from cityseer._geom import parse_wkt
+    from cityseer.tools import graphs, io, mock
+
+    G = graphs.nx_simple_geoms(mock.mock_graph())
+    _n, _e, ns = io.network_structure_from_nx(G)
+    keys = ns.node_keys_py()
+    for s, d, k in ns.edge_references():
+        pl = ns.get_edge_payload_py(s, d, k)
+        assert (pl.start_nd_key_py, pl.end_nd_key_py, pl.edge_idx) == (keys[s], keys[d], k)
+        assert pl.length == ns.get_edge_length(s, d, k) and pl.imp_factor == 1.0 and math.isnan(pl.seconds)
+        (kind, rings), = parse_wkt(pl.geom_wkt)
+        xy = rings[0]
+        assert kind == 1 and tuple(xy[0]) == ns.get_node_payload_py(s).coord
+        want_in = math.degrees(math.atan2(xy[1][1] - xy[0][1], xy[1][0] - xy[0][0]))
+        want_out = math.degrees(math.atan2(xy[-1][1] - xy[-2][1], xy[-1][0] - xy[-2][0]))
+        assert abs(pl.in_bearing - want_in) < 1e-3 and abs(pl.out_bearing - want_out) < 1e-3
+        assert abs(float(np.hypot(*np.diff(xy, axis=0).T).sum()) - pl.length) < 1e-2
+    pl = ns.get_edge_payload_py(*ns.edge_references()[0])
+    assert "LINESTRING (700700 5719700" in pl.geom_wkt
