@@ -1,10 +1,11 @@
 """Drop-in for ``cityseer.rustalgos`` on the network-centrality path.
 
 The reference's PyO3 module (rust/src/lib.rs:37-164) exposes the threshold
-conversions at top level and the ``graph`` / ``centrality`` submodules. Here the
-conversions run in the native library's host code (f32 arithmetic and Rust
-rounding, rust/src/common.rs:98-270) and the graph algorithms run as HIP
-kernels behind the same classes.
+conversions and weight helpers at top level and the ``graph``,
+``centrality``, ``data`` and ``diversity`` submodules. Here the conversions run
+in the native library's host code (f32 arithmetic and Rust rounding,
+rust/src/common.rs:98-270) and the graph algorithms run as HIP kernels behind
+the same classes.
 """
 
 from __future__ import annotations
@@ -14,12 +15,18 @@ from collections.abc import Sequence
 import numpy as np
 
 from .. import _native
-from . import centrality, data, graph
+from . import centrality, data, diversity, graph
+from .data import clip_wts_curve, clipped_beta_wt
 
 __all__ = [
     "centrality",
     "data",
+    "diversity",
     "graph",
+    "avg_distances_for_betas",
+    "check_numerical_data",
+    "clip_wts_curve",
+    "clipped_beta_wt",
     "pair_distances_betas_time",
     "distances_from_betas",
     "betas_from_distances",
@@ -144,3 +151,35 @@ def pair_distances_betas_time(
         )
     )
     return [int(x) for x in od[:n]], [float(x) for x in ob[:n]], [int(x) for x in os_[:n]]
+
+
+def check_numerical_data(data_arr) -> None:
+    """Every value of a 2-D f32 array must be finite (common.rs:75-88)."""
+    a = np.asarray(data_arr, dtype=np.float32)
+    if a.ndim != 2:
+        raise TypeError("data_arr must be a 2-D array")
+    if not np.all(np.isfinite(a)):
+        raise ValueError("The numeric data values must be finite.")
+
+
+def avg_distances_for_betas(betas: list[float], min_threshold_wt: float | None = None) -> list[float]:
+    """The average distance each beta's weight curve gives up to its own threshold distance:
+    -ln(auc / d) / beta with auc = (exp(-beta d) - 1) / -beta, in f32 (common.rs:271-307)."""
+    b = _as_f32_list(betas, "betas")
+    if len(b) == 0:
+        raise ValueError("Input 'betas' cannot be empty.")
+    dists = distances_from_betas(b.tolist(), min_threshold_wt)
+    f32 = np.float32
+    out = []
+    with np.errstate(all="ignore"):
+        for beta, d in zip(b, dists, strict=True):
+            beta = f32(beta)
+            if abs(beta) < np.finfo(np.float32).eps:
+                raise ValueError(f"Beta ({beta}) must not be zero during average distance calculation.")
+            df = f32(d)
+            auc = f32(f32(np.exp(f32(-beta * df), dtype=np.float32) - f32(1.0)) / f32(-beta))
+            wt = f32(auc / df)
+            if not wt > 0.0:
+                raise ValueError(f"Invalid weight ({wt}) encountered during average distance calculation.")
+            out.append(float(f32(f32(-np.log(wt, dtype=np.float32)) / beta)))
+    return out

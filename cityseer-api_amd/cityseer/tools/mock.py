@@ -162,6 +162,17 @@ def mock_numerical_data(nx_multigraph: nx.MultiGraph, length: int = 50, val_min:
     return df
 
 
+def mock_species_data(random_seed: int = 0):
+    """Counts and probabilities of species classes for the diversity measures (mock.py:413-473
+    of the reference): for n = 1, 6, ..., 46, n integers drawn in [1, 10) by numpy's legacy seeded
+    generator, grouped by value."""
+    np.random.seed(seed=random_seed)
+    for n in range(1, 50, 5):
+        draws = np.random.randint(1, 10, n)
+        _vals, counts = np.unique(draws, return_counts=True)
+        yield counts.astype(np.int_), counts / len(draws)
+
+
 def mock_data_map(data_gdf, dedupe_key_col: str | None = None):
     """A ``DataMap`` of a frame like ``mock_data_gdf``'s (mock.py:540-560 of the reference);
     geometries may be WKT strings or objects with a ``wkt`` attribute."""
