@@ -78,6 +78,48 @@ def accessibility(og, live, node_data_map, dedupe_of, lu_class, accessibility_ke
     return res
 
 
+def accessibility_threads(og, live, node_data_map, dedupe_of, lu_class, accessibility_keys, distances, betas, mcws,
+                          max_walk_seconds, speed, angular=False, n_threads=1):
+    """The same as ``accessibility`` in C (og_accessibility), threads over sources: the
+    restatement the CPU baseline times. Returns the same {class: (unweighted, weighted,
+    nearest)} rows."""
+    import ctypes
+
+    from .oracle import _p, load
+
+    keys = list(dedupe_of.keys())
+    index = {k: i for i, k in enumerate(keys)}
+    groups: dict = {}
+    entry_group = np.asarray([groups.setdefault(dedupe_of[k], len(groups)) for k in keys], dtype=np.uint32)
+    cls_index = {c: i for i, c in enumerate(accessibility_keys)}
+    entry_class = np.asarray([cls_index.get(lu_class.get(k), -1) for k in keys], dtype=np.int32)
+    a_node, a_ent, a_dist = [], [], []
+    for v, pairs in node_data_map.items():
+        for k, d in pairs:
+            if k in index:
+                a_node.append(int(v))
+                a_ent.append(index[k])
+                a_dist.append(float(d))
+    a_node = np.asarray(a_node, dtype=np.uint32)
+    a_ent = np.asarray(a_ent, dtype=np.uint32)
+    a_dist = np.asarray(a_dist, dtype=np.float64)
+    n, T, C = og.n, len(distances), len(accessibility_keys)
+    cnt = np.zeros((C, T, n))
+    wtd = np.zeros((C, T, n))
+    near = np.full((C, n), np.nan)
+    lv = np.asarray(live, dtype=np.uint8)
+    rc = load().og_accessibility(
+        og.h, len(keys), _p(entry_group, ctypes.c_uint32), _p(entry_class, ctypes.c_int32), C, len(a_node),
+        _p(a_node, ctypes.c_uint32), _p(a_ent, ctypes.c_uint32), _p(a_dist, ctypes.c_double), T,
+        _p(np.asarray(distances, dtype=np.uint32), ctypes.c_uint32), _p(np.asarray(betas, dtype=np.float32), ctypes.c_float),
+        _p(np.asarray(mcws, dtype=np.float32), ctypes.c_float), int(max_walk_seconds), float(speed), int(bool(angular)),
+        _p(lv, ctypes.c_uint8), int(n_threads), _p(cnt, ctypes.c_double), _p(wtd, ctypes.c_double),
+        _p(near, ctypes.c_double))
+    if rc != 0:
+        raise ValueError(f"oracle error {rc}")
+    return {c: (cnt[i], wtd[i], near[i]) for i, c in enumerate(accessibility_keys)}
+
+
 # ---- diversity (rust/src/diversity.rs), f32 throughout, classes summed in order
 
 def hill(counts, q):

@@ -1510,3 +1510,174 @@ int og_tree_segment(const og_graph *g, uint32_t src, uint32_t max_seconds, float
   free(em);
   return OG_OK;
 }
+
+/* ------------------------------------------------------------------------- */
+/* land-use accessibility, threaded (DataMap.accessibility)                  */
+/* ------------------------------------------------------------------------- */
+
+/* Per live source: its shortest-path tree (simplest-path with `angular`),
+ * then aggregate_to_src_idx (data.rs:520-615): nodes in visit order below the
+ * walk limit, each node's members in assignment order, one entry per dedupe
+ * group kept, replaced only by a strictly smaller total; then the counts of
+ * DataMap.accessibility (data.rs:680-741). Members: n_assign triples (node,
+ * entry, distance f64, used as f32); entry_group[e] its dedupe group (<
+ * n_entries); entry_class[e] its class (-1: not counted). Each source writes
+ * its own rows, so threads need no atomics. out_count, out_weighted:
+ * [n_classes][T][N]; out_nearest: [n_classes][N] (NaN: none). */
+typedef struct {
+  const og_graph *g;
+  const uint32_t *node_off, *node_mem, *assign_entry;
+  const float *assign_dist;
+  const uint32_t *entry_group;
+  const int32_t *entry_class;
+  uint32_t n_entries, n_classes, T, max_seconds;
+  const uint32_t *distances;
+  const float *betas, *mcws;
+  float speed;
+  int angular;
+  const uint8_t *live;
+  double *count, *weighted, *nearest;
+  _Atomic uint32_t next;
+} acc_job;
+
+static void *acc_worker(void *arg) {
+  acc_job *jb = (acc_job *)arg;
+  const og_graph *g = jb->g;
+  const uint32_t N = g->n, T = jb->T;
+  nvisit *tm = xmalloc(sizeof(nvisit) * N);
+  uint32_t *order = xmalloc(sizeof(uint32_t) * (N ? N : 1));
+  uint32_t *tag = calloc(jb->n_entries ? jb->n_entries : 1, sizeof(uint32_t));
+  uint32_t *win = xmalloc(sizeof(uint32_t) * (jb->n_entries ? jb->n_entries : 1));
+  float *wtot = xmalloc(sizeof(float) * (jb->n_entries ? jb->n_entries : 1));
+  uint32_t *touched = xmalloc(sizeof(uint32_t) * (jb->n_entries ? jb->n_entries : 1));
+  const float maxsec = (float)jb->max_seconds;
+  const float max_walk = maxsec * jb->speed;
+  uint32_t dmax = 0;
+  for (uint32_t t = 0; t < T; t++) dmax = jb->distances[t] > dmax ? jb->distances[t] : dmax;
+  uint32_t epoch = 0;
+  for (;;) {
+    const uint32_t src = atomic_fetch_add(&jb->next, 1);
+    if (src >= N) break;
+    if (!jb->live[src]) continue;
+    const uint32_t no = jb->angular ? tree_simplest(g, src, jb->max_seconds, jb->speed, tm, order)
+                                    : tree_shortest(g, src, jb->max_seconds, jb->speed, tm, order);
+    ++epoch;
+    uint32_t nt = 0;
+    for (uint32_t k = 0; k < no; k++) {
+      const uint32_t v = order[k];
+      const float agg = tm[v].agg_seconds;
+      if (agg >= maxsec) continue;
+      const float net = agg * jb->speed;
+      for (uint32_t q = jb->node_off[v]; q < jb->node_off[v + 1]; q++) {
+        const uint32_t m = jb->node_mem[q];
+        const uint32_t e = jb->assign_entry[m];
+        const float total = net + jb->assign_dist[m];
+        if (!(total <= max_walk)) continue;
+        const uint32_t grp = jb->entry_group[e];
+        if (tag[grp] != epoch) {
+          tag[grp] = epoch;
+          win[grp] = e;
+          wtot[grp] = total;
+          touched[nt++] = grp;
+        } else if (total < wtot[grp]) {
+          win[grp] = e;
+          wtot[grp] = total;
+        }
+      }
+    }
+    for (uint32_t k = 0; k < nt; k++) {
+      const uint32_t grp = touched[k];
+      const int32_t c = jb->entry_class[win[grp]];
+      if (c < 0) continue;
+      const float d = wtot[grp];
+      for (uint32_t t = 0; t < T; t++) {
+        if (!(d <= (float)jb->distances[t])) continue;
+        const size_t row = ((size_t)c * T + t) * N + src;
+        jb->count[row] += 1.0;
+        /* clipped_beta_wt (common.rs:339-348) */
+        const float raw = expf(-jb->betas[t] * d);
+        jb->weighted[row] += (double)(fminf(raw, jb->mcws[t]) / jb->mcws[t]);
+        if (jb->distances[t] == dmax) {
+          double *nr = &jb->nearest[(size_t)c * N + src];
+          if (isnan(*nr) || (double)d < *nr) *nr = (double)d;
+        }
+      }
+    }
+  }
+  free(tm);
+  free(order);
+  free(tag);
+  free(win);
+  free(wtot);
+  free(touched);
+  return NULL;
+}
+
+int og_accessibility(const og_graph *g, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
+                     uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
+                     const uint32_t *assign_entry, const double *assign_dist, uint32_t T, const uint32_t *distances,
+                     const float *betas, const float *mcws, uint32_t max_seconds, float speed, int angular,
+                     const uint8_t *live, int n_threads, double *out_count, double *out_weighted,
+                     double *out_nearest) {
+  const uint32_t N = g->n;
+  if (T == 0 || !(speed > 0.0f) || !isfinite(speed)) return OG_EINVAL;
+  if (angular && (!g->is_dual || !g->slots_ok)) return OG_EINVAL;
+  for (uint32_t e = 0; e < n_entries; e++) {
+    if (entry_group[e] >= n_entries) return OG_EINVAL;
+    if (entry_class[e] >= (int32_t)n_classes) return OG_EINVAL;
+  }
+  for (uint32_t i = 0; i < n_assign; i++)
+    if (assign_node[i] >= N || assign_entry[i] >= n_entries) return OG_EINVAL;
+  for (size_t i = 0; i < (size_t)n_classes * T * N; i++) out_count[i] = out_weighted[i] = 0.0;
+  for (size_t i = 0; i < (size_t)n_classes * N; i++) out_nearest[i] = NAN;
+  /* members of each node, in assignment order */
+  uint32_t *node_off = calloc((size_t)N + 1, sizeof(uint32_t));
+  uint32_t *node_mem = xmalloc(sizeof(uint32_t) * (n_assign ? n_assign : 1));
+  float *adist = xmalloc(sizeof(float) * (n_assign ? n_assign : 1));
+  for (uint32_t i = 0; i < n_assign; i++) {
+    node_off[assign_node[i] + 1]++;
+    adist[i] = (float)assign_dist[i]; /* data.rs:548 casts the f64 distance to f32 */
+  }
+  for (uint32_t v = 0; v < N; v++) node_off[v + 1] += node_off[v];
+  {
+    uint32_t *at = xmalloc(sizeof(uint32_t) * (N ? N : 1));
+    for (uint32_t v = 0; v < N; v++) at[v] = node_off[v];
+    for (uint32_t i = 0; i < n_assign; i++) node_mem[at[assign_node[i]]++] = i;
+    free(at);
+  }
+  acc_job jb;
+  jb.g = g;
+  jb.node_off = node_off;
+  jb.node_mem = node_mem;
+  jb.assign_entry = assign_entry;
+  jb.assign_dist = adist;
+  jb.entry_group = entry_group;
+  jb.entry_class = entry_class;
+  jb.n_entries = n_entries;
+  jb.n_classes = n_classes;
+  jb.T = T;
+  jb.max_seconds = max_seconds;
+  jb.distances = distances;
+  jb.betas = betas;
+  jb.mcws = mcws;
+  jb.speed = speed;
+  jb.angular = angular;
+  jb.live = live;
+  jb.count = out_count;
+  jb.weighted = out_weighted;
+  jb.nearest = out_nearest;
+  atomic_init(&jb.next, 0);
+  const int nt = n_threads > 0 ? n_threads : 1;
+  if (nt == 1) {
+    acc_worker(&jb);
+  } else {
+    pthread_t *th = xmalloc(sizeof(pthread_t) * nt);
+    for (int i = 0; i < nt; i++) pthread_create(&th[i], NULL, acc_worker, &jb);
+    for (int i = 0; i < nt; i++) pthread_join(th[i], NULL);
+    free(th);
+  }
+  free(node_off);
+  free(node_mem);
+  free(adist);
+  return OG_OK;
+}

@@ -97,6 +97,12 @@ def load() -> ctypes.CDLL:
     lib.og_betweenness_od_shortest.restype = ctypes.c_int
     lib.og_betweenness_od_shortest.argtypes = [ctypes.c_void_p, ctypes.POINTER(OgParams), _u32p, _u32p,
                                                ctypes.POINTER(ctypes.c_float), _f64p]
+    lib.og_accessibility.restype = ctypes.c_int
+    lib.og_accessibility.argtypes = [
+        ctypes.c_void_p, ctypes.c_uint32, _u32p, _i32p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, _f64p,
+        ctypes.c_uint32, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_float, ctypes.c_int, _u8p, ctypes.c_int,
+        _f64p, _f64p, _f64p,
+    ]
     lib.og_segment_centrality.restype = ctypes.c_int
     lib.og_segment_centrality.argtypes = [ctypes.c_void_p, ctypes.POINTER(OgParams), _f64p]
     lib.og_brandes_shortest.restype = ctypes.c_int

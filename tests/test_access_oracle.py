@@ -317,3 +317,52 @@ def test_angular_data_layer_needs_endpoint_keys():
         dm.accessibility(ns, {"a": "shop"}, ["shop"], distances=[200], angular=True)
     with pytest.raises(ValueError, match="shared_primal_node_key"):
         dm.stats(ns, [{"a": 1.0}], distances=[200], angular=True)
+
+
+def test_threaded_c_accessibility_equals_the_restatement():
+    """og_accessibility (C, threads over sources; the CPU baseline) gives the Python
+    restatement's rows: on the hand-worked line, on a random lattice assignment, and angular on the
+    detour dual and a lattice dual. Counts and nearest distances exactly; weights to 1e-6 (expf
+    against numpy's exp, summed in another order)."""
+    from cityseer.tools import mock
+    from oracle.access import accessibility_threads
+
+    O.set_heap_mode(False)
+
+    def same(a, b):
+        for cls in a:
+            assert np.array_equal(a[cls][0], b[cls][0]), cls
+            np.testing.assert_allclose(a[cls][1], b[cls][1], rtol=1e-6, atol=1e-12)
+            assert np.array_equal(a[cls][2], b[cls][2], equal_nan=True), cls
+
+    ns = line_graph()
+    og = OracleGraph.from_network_structure(ns)
+    _d, betas, secs = O.thresholds(1.0, distances=[100, 250])
+    ndm, dedupe_of, lu_class = line_inputs()
+    args = (og, [1, 1, 1, 1], ndm, dedupe_of, lu_class, ["shop", "cafe"], [100, 250], betas, [1.0, 1.0], max(secs), 1.0)
+    same(accessibility_threads(*args, n_threads=3), oracle_access(*args))
+    rng = np.random.default_rng(31)
+    for lat, angular in ((mock.perturbed_lattice(12, seed=32, diag_frac=0.1), False),
+                         (mock.lattice_dual(mock.perturbed_lattice(9, seed=33)), True)):
+        ns = mock.network_structure_from_lattice(lat)
+        og = OracleGraph.from_network_structure(ns)
+        n = ns.node_count()
+        ndm, dedupe_of, lu = {}, {}, {}
+        for e in range(300):
+            key = f"e{e}"
+            dedupe_of[key] = f"g{int(rng.integers(0, 60))}" if rng.random() < 0.3 else key
+            lu[key] = ("shop", "cafe", "park")[int(rng.integers(0, 3))]
+            for v in rng.choice(n, int(rng.integers(1, 3)), replace=False).tolist():
+                ndm.setdefault(int(v), []).append((key, float(rng.uniform(0, 50))))
+        dists = [300, 700]
+        _d, betas, secs = O.thresholds(1.33333, distances=dists)
+        live = (rng.random(n) < 0.9).astype(int).tolist()
+        args = (og, live, ndm, dedupe_of, lu, ["shop", "park"], dists, betas, [1.0, 0.8], max(secs), 1.33333)
+        same(accessibility_threads(*args, angular=angular, n_threads=4), oracle_access(*args, angular=angular))
+    ns, s, t = detour_dual()
+    og = OracleGraph.from_network_structure(ns)
+    _d, betas, secs = O.thresholds(1.0, distances=[305, 400])
+    args = (og, [1] * og.n, {t: [("x", 0.0)]}, {"x": "x"}, {"x": "shop"}, ["shop"], [305, 400], betas, [1.0, 1.0],
+            max(secs), 1.0)
+    got = accessibility_threads(*args, angular=True)
+    assert got["shop"][0][:, s].tolist() == [0.0, 1.0] and abs(got["shop"][2][s] - DETOUR_SIMPLEST) < 1e-3
