@@ -145,6 +145,21 @@ int og_tree_segment(const og_graph *g, uint32_t src, uint32_t max_seconds,
                     int32_t *origin_seg, int32_t *last_seg, uint32_t *order,
                     uint32_t *n_order, uint32_t *edge_order, uint32_t *n_edges);
 
+
+/* Land-use accessibility over all live sources, threaded (data.rs:520-615,
+ * 680-741): per source its shortest-path tree (simplest-path with `angular`,
+ * dual graphs only), one entry per dedupe group within the walk limit, then
+ * per class and threshold the count, the clipped-beta weighted count and, at
+ * the largest threshold, the nearest distance. Members are n_assign triples
+ * (assign_node, assign_entry, assign_dist). out_count, out_weighted:
+ * [n_classes][T][N]; out_nearest: [n_classes][N], NaN where none. */
+int og_accessibility(const og_graph *g, uint32_t n_entries, const uint32_t *entry_group, const int32_t *entry_class,
+                     uint32_t n_classes, uint32_t n_assign, const uint32_t *assign_node,
+                     const uint32_t *assign_entry, const double *assign_dist, uint32_t T, const uint32_t *distances,
+                     const float *betas, const float *mcws, uint32_t max_seconds, float speed, int angular,
+                     const uint8_t *live, int n_threads, double *out_count, double *out_weighted,
+                     double *out_nearest);
+
 #ifdef __cplusplus
 }
 #endif

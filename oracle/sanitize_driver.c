@@ -7,9 +7,9 @@
  * The lattice: side x side nodes on a jittered 100 m grid, two-way edges
  * between grid neighbours (every tenth one-way in the first build), a few exact
  * ties (unjittered rows). Shortest (closeness + betweenness, 8 threads, Phase 1
- * and Phase 2) and OD-weighted betweenness run over all sources of the first
- * build, segment centrality (which needs every edge's reverse twin) over the
- * second; the output is a checksum of the integer density rows.
+ * and Phase 2), OD-weighted betweenness and land-use accessibility (8
+ * threads) run over all sources of the first build, segment centrality (which needs every edge's reverse twin) over the
+ * second; the output is a checksum of the integer density and count rows.
  */
 #include <math.h>
 #include <stdint.h>
@@ -137,7 +137,32 @@ int main(int argc, char **argv) {
   p.tolerance = 1e-4f;
   if (og_betweenness_od_shortest(g, &p, off, odd, odw, od)) return 6;
 
-  printf("density checksum %.1f reach %u %u %u\n", checksum, reach[0], reach[1], reach[2]);
+  /* accessibility: three entries per node, every fourth sharing a dedupe
+   * group with the entry before it, four classes (class -1 every seventh) */
+  const uint32_t ne = 3 * n, nc = 4;
+  uint32_t *grp = malloc(4 * ne), *anode = malloc(4 * ne), *aent = malloc(4 * ne);
+  int32_t *cls = malloc(4 * ne);
+  double *adist = malloc(8 * ne);
+  uint8_t *live = malloc(n);
+  for (uint32_t e = 0; e < ne; ++e) {
+    grp[e] = (e % 4 == 3) ? e - 1 : e;
+    cls[e] = (e % 7 == 6) ? -1 : (int32_t)(e % nc);
+    anode[e] = e / 3;
+    aent[e] = e;
+    adist[e] = 5.0 + 40.0 * rnd();
+  }
+  for (uint32_t i = 0; i < n; ++i) live[i] = (i % 5) != 0;
+  const float mcws[3] = {1.0f, 1.0f, 1.0f};
+  double *cnt = malloc(sizeof(double) * nc * 3 * n), *wtd = malloc(sizeof(double) * nc * 3 * n);
+  double *near = malloc(sizeof(double) * nc * n);
+  if (og_accessibility(g, ne, grp, cls, nc, ne, anode, aent, adist, 3, dists, betas, mcws, secs[2], 1.33333f, 0,
+                       live, 8, cnt, wtd, near))
+    return 7;
+  double acc_sum = 0.0;
+  for (size_t i = 0; i < (size_t)nc * 3 * n; ++i) acc_sum += cnt[i];  /* integer counts */
+
+  printf("density checksum %.1f reach %u %u %u access %.1f\n", checksum, reach[0], reach[1], reach[2], acc_sum);
+  free(grp); free(anode); free(aent); free(cls); free(adist); free(live); free(cnt); free(wtd); free(near);
   og_graph_free(g);
   og_graph_free(g2);
   free(elig); free(sources); free(swt); free(out); free(seg); free(off); free(odd); free(odw); free(od);

@@ -1,8 +1,8 @@
 """CPU: the oracle's threaded entry points (shortest closeness and betweenness in Phase 1 and
-Phase 2, segment, OD-weighted betweenness; eight threads over f64 CAS accumulators) under
-ThreadSanitizer and under AddressSanitizer with UndefinedBehaviorSanitizer, through
-oracle/sanitize_driver.c on a generated lattice. Both builds must run clean and agree on the
-integer density rows."""
+Phase 2, segment, OD-weighted betweenness; eight threads over f64 CAS accumulators; land-use
+accessibility, eight threads each writing its own sources' rows) under ThreadSanitizer and under
+AddressSanitizer with UndefinedBehaviorSanitizer, through oracle/sanitize_driver.c on a generated
+lattice. Both builds must run clean and agree on the integer density and count rows."""
 
 from __future__ import annotations
 
