@@ -66,7 +66,7 @@ struct AngArgs {
   size_t wave_stride;  // bytes of the interleaved arrays per wave
   uint32_t scap, ncap, hcap, maxp, tcap;  // per-lane capacities (tcap: thresholds of the delta/seed blocks)
   size_t o_srec, o_preds, o_order, o_orte, o_delta, o_seed, o_nrec, o_heap;
-  size_t o_bqe, o_bqc;  // bucket queue entries [kBuckets][bq_cap] and counts [kBuckets] per lane
+  size_t o_bqe;  // bucket queue entries [kBuckets][bq_cap] per lane (the counts live in LDS)
   uint32_t bq_cap;
   char *wave_ws;
   uint32_t per_wave;    // lanes of a wave that take sources (the rest take padding items)
@@ -210,9 +210,21 @@ uint32_t bucket_capacity() {
   const uint32_t cap0 = default_tuning().cap;
   return cap0 ? std::min<uint32_t>(kBqCap, std::max<uint32_t>(cap0 / 128, 2u)) : kBqCap;
 }
+
+// Bucket entry counts of the workgroup's lanes, in LDS: [bucket][lane], one
+// byte each (a bucket holds at most 64 entries). A pop and a push read the
+// count before they touch the bucket, so it sits on each lane's dependent
+// chain; in LDS that read costs no memory round trip.
+static_assert(kBqCap <= 255, "bucket counts are one byte");
+__device__ __forceinline__ uint8_t *bq_counts() {
+  __shared__ uint8_t c[kBuckets * kBlock];
+  return c + threadIdx.x;
+}
+
 struct BQueue {
   char *w;
-  size_t off_e, off_c;
+  size_t off_e;
+  uint8_t *lc;  // this lane's column of the LDS counts
   uint32_t lane, cap;  // entries per bucket
   float inv;
   uint64_t mask[kMaskWords] = {};
@@ -221,7 +233,7 @@ struct BQueue {
   __device__ __forceinline__ uint4 *ent(uint32_t r, uint32_t i) {
     return il<uint4>(w, off_e, r * cap + i, lane, (size_t)kBuckets * cap);
   }
-  __device__ __forceinline__ uint32_t *cnt(uint32_t r) { return il<uint32_t>(w, off_c, r, lane, kBuckets); }
+  __device__ __forceinline__ uint8_t *cnt(uint32_t r) { return lc + r * kBlock; }
   // mask words are indexed through selects so that they stay in registers
   __device__ __forceinline__ uint64_t word(uint32_t i) const {
     uint64_t v = 0;
@@ -260,7 +272,7 @@ struct BQueue {
     const uint32_t m = used ? *cnt(r) : 0u;
     if (m >= cap) return false;
     *ent(r, m) = make_uint4(__float_as_uint(k), __float_as_uint(s), idx, slot);
-    *cnt(r) = m + 1;
+    *cnt(r) = (uint8_t)(m + 1);
     flip(r, true);
     ++n;
     return true;
@@ -288,7 +300,7 @@ struct BQueue {
       }
     }
     if (bi != m - 1) *ent(r, bi) = last;
-    *cnt(r) = m - 1;
+    *cnt(r) = (uint8_t)(m - 1);
     if (m == 1) flip(r, false);
     --n;
     return best;
@@ -299,7 +311,7 @@ struct BQueue {
 template <int QK>
 __device__ __forceinline__ auto make_queue(const AngArgs &a, CAngJob &jb, char *w, uint32_t lane) {
   if constexpr (QK == 1) {
-    return BQueue{w, a.o_bqe, a.o_bqc, lane, a.bq_cap, jb.bq_inv};
+    return BQueue{w, a.o_bqe, bq_counts(), lane, a.bq_cap, jb.bq_inv};
   } else {
     return Heap{w, a.o_heap, lane, 0, a.hcap};
   }
@@ -999,7 +1011,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct AngPlan {
   uint32_t scap, ncap, hcap, maxp, lanes, tcap;
   uint32_t bq_cap = 0;  // bucket queue entries per bucket (0: the heap)
-  size_t o_bqe = 0, o_bqc = 0;
+  size_t o_bqe = 0;
   uint32_t per_wave = kWave;  // lane blocks per wave in the interleaved arrays
   size_t lane_stride, wave_stride;
   size_t o_srec, o_preds, o_order, o_orte, o_delta, o_seed, o_nrec, o_heap;
@@ -1022,7 +1034,6 @@ void ang_layout(AngPlan &p, uint32_t N, uint32_t T = 1) {
   take(p.o_nrec, kNRec, p.ncap);
   if (p.bq_cap) {
     take(p.o_bqe, 16, (size_t)kBuckets * p.bq_cap);
-    take(p.o_bqc, 4, kBuckets);
     p.o_heap = 0;
   } else {
     take(p.o_heap, 16, p.hcap);
@@ -1312,7 +1323,6 @@ void launch_simplest_jobs(const std::vector<AngSpec> &specs, const std::vector<u
     a.o_nrec = p.o_nrec;
     a.o_heap = p.o_heap;
     a.o_bqe = p.o_bqe;
-    a.o_bqc = p.o_bqc;
     a.bq_cap = p.bq_cap;
     a.per_wave = p.per_wave;
     a.epoch_base = epoch_base;
@@ -1492,8 +1502,7 @@ std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de
     p.wave_stride = o_q + align256((size_t)16 * p.hcap * kWave);
   } else {
     p.o_bqe = o_q;
-    p.o_bqc = o_q + align256((size_t)16 * kBuckets * p.bq_cap * kWave);
-    p.wave_stride = p.o_bqc + align256((size_t)4 * kBuckets * kWave);
+    p.wave_stride = o_q + align256((size_t)16 * kBuckets * p.bq_cap * kWave);
   }
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, dev), "props");
@@ -1534,7 +1543,6 @@ std::vector<uint32_t> launch_reach_simplest(Graph &g, DevGraph &dg, DevEdges &de
   a.o_nrec = p.o_nrec;
   a.o_heap = p.o_heap;
   a.o_bqe = p.o_bqe;
-  a.o_bqc = p.o_bqc;
   a.bq_cap = p.bq_cap;
   a.per_wave = kWave;
   a.epoch_base = epoch_base;
