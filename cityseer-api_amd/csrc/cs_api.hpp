@@ -67,10 +67,13 @@ struct OwnedStream {
   operator hipStream_t() const { return s; }
 };
 
-// acc [M][stride][N] (device rows, node d at column d) -> out [M][T][N]
-// (host, node iperm[d] at column iperm[d]; iperm null = identity)
+// acc [M][stride][N] (device rows, node d at column d) -> rows t0 .. t0 + T - 1
+// of out [M][out_T][N] (host, node iperm[d] at column iperm[d]; iperm null =
+// identity; out_T 0 = T)
 void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out,
-                        const uint32_t *iperm);
-void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s);
+                        const uint32_t *iperm, uint32_t out_T = 0, uint32_t t0 = 0);
+// scales rows m0..m1 of out [M][T][N], thresholds t0 .. t0 + n_t - 1 (n_t 0 = the rest)
+void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s, uint32_t t0 = 0,
+                uint32_t n_t = 0);
 
 }  // namespace cs

@@ -546,11 +546,12 @@ const uint32_t *result_order(const Graph &g) { return g.identity_order ? nullptr
 // acc [M][stride][N] (device rows) -> out [M][T][N] (host): drops the padded
 // thresholds and puts device node d at original column iperm[d]
 void unpack_acc_strided(const double *acc, uint32_t N, uint32_t M, uint32_t T, uint32_t stride, double *out,
-                        const uint32_t *iperm) {
+                        const uint32_t *iperm, uint32_t out_T, uint32_t t0) {
+  if (out_T == 0) out_T = T;
   for (uint32_t m = 0; m < M; ++m)
     for (uint32_t i = 0; i < T; ++i) {
       const double *src = acc + ((size_t)m * stride + i) * N;
-      double *dst = out + ((size_t)m * T + i) * N;
+      double *dst = out + ((size_t)m * out_T + t0 + i) * N;
       if (!iperm) {
         std::memcpy(dst, src, sizeof(double) * N);
       } else {
@@ -564,10 +565,12 @@ void unpack_acc(const std::vector<double> &acc, uint32_t N, uint32_t M, uint32_t
   unpack_acc_strided(acc.data(), N, M, T, T, out, iperm);
 }
 
-void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s) {
+void scale_rows(double *out, uint32_t N, uint32_t T, uint32_t m0, uint32_t m1, double s, uint32_t t0,
+                uint32_t n_t) {
   if (s == 1.0) return;
+  if (n_t == 0) n_t = T - t0;
   for (uint32_t m = m0; m <= m1; ++m)
-    for (uint32_t i = 0; i < T; ++i)
+    for (uint32_t i = t0; i < t0 + n_t; ++i)
       for (uint32_t v = 0; v < N; ++v) out[((size_t)m * T + i) * N + v] *= s;
 }
 
@@ -609,9 +612,11 @@ struct cs_batch {
     bool betw = false;
     double betw_scale = 1.0;
     size_t acc_off = 0;
+    uint32_t out_T = 0, t0 = 0;  // the request's thresholds; this job's first (a chunk of at most kMaxT)
   };
   int device = -1;
   std::vector<JobHost> jobs;
+  int n_requests = 0;  // cs_batch_add_shortest calls; a request runs as one job per threshold chunk
   uint32_t NT = 1;
   Tuning tune;
   hipStream_t stream = nullptr;
@@ -1761,24 +1766,34 @@ int cs_batch_add_shortest(cs_batch *b, cs_graph *h, const cs_params *p, double *
     if (!b || !h || !out) fail(CS_EINVAL, "batch, graph and out must not be NULL");
     Graph &g = h->g;
     check_params(g, p);
-    if (p->n_thresholds > CS_MAX_THRESHOLDS)
-      fail(CS_EINVAL, "a batched request takes at most " + std::to_string(CS_MAX_THRESHOLDS) +
-                          " distance thresholds (split it into several requests)");
     run_shortest_checks(g, p);
-    cs_batch::JobHost j;
-    j.h = h;
-    j.spec = make_spec(p);
-    j.ss = source_set(g, p);
-    to_device_sources(g, j.ss);
-    j.elig = to_device_u8(g, p->eligible);
-    j.out = out;
-    j.reach = reach_totals;
-    j.betw = p->compute_betweenness != 0;
-    j.betw_scale = p->betw_scale;
+    SourceSet ss = source_set(g, p);
+    to_device_sources(g, ss);
+    const std::vector<uint8_t> elig = to_device_u8(g, p->eligible);
+    // more than kMaxT thresholds: one job per chunk, each searching to the
+    // request's largest cutoff (make_spec) and filling its own rows of `out`
+    const uint32_t T = p->n_thresholds;
+    std::vector<cs_batch::JobHost> chunks;
+    for (uint32_t t0 = 0; t0 < T; t0 += kMaxT) {
+      cs_batch::JobHost j;
+      j.h = h;
+      j.spec = make_spec(p, t0, chunk_len(T, t0));
+      j.ss = ss;
+      j.elig = elig;
+      j.out = out;
+      j.reach = reach_totals;
+      j.betw = p->compute_betweenness != 0;
+      j.betw_scale = p->betw_scale;
+      j.out_T = T;
+      j.t0 = t0;
+      chunks.push_back(std::move(j));
+    }
     b->drop_device();
-    b->jobs.push_back(std::move(j));
-    b->NT = std::max(b->NT, p->n_thresholds);
-    idx = (int)b->jobs.size() - 1;
+    for (auto &j : chunks) {
+      b->NT = std::max(b->NT, j.spec.T);
+      b->jobs.push_back(std::move(j));
+    }
+    idx = b->n_requests++;
   });
   return rc == CS_OK ? idx : -rc;
 }
@@ -1835,10 +1850,10 @@ int cs_batch_results(cs_batch *b) {
     for (size_t i = 0; i < b->jobs.size(); ++i) {
       const cs_batch::JobHost &j = b->jobs[i];
       const uint32_t N = j.h->g.n, T = j.spec.T;
-      unpack_acc_strided(hacc.data() + j.acc_off, N, 7, T, b->NT, j.out, result_order(j.h->g));
-      if (j.betw) scale_rows(j.out, N, T, 5, 6, j.betw_scale);
+      unpack_acc_strided(hacc.data() + j.acc_off, N, 7, T, b->NT, j.out, result_order(j.h->g), j.out_T, j.t0);
+      if (j.betw) scale_rows(j.out, N, j.out_T, 5, 6, j.betw_scale, j.t0, T);
       if (j.reach)
-        for (uint32_t t = 0; t < T; ++t) j.reach[t] = hreach[8 * i + t];
+        for (uint32_t t = 0; t < T; ++t) j.reach[j.t0 + t] = hreach[8 * i + t];
     }
   });
 }

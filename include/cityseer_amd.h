@@ -184,12 +184,13 @@ uint64_t cs_graph_progress(const cs_graph *g);
 
 /* Thresholds one kernel launch takes. Calls with more run one launch per
  * chunk of CS_MAX_THRESHOLDS, each searching to the call's largest cutoff (the
- * reference's single search); batched and served requests take at most this
- * many (CS_EINVAL). */
+ * reference's single search); a batched request with more runs as one job per
+ * chunk in the same launch; served requests take at most this many
+ * (CS_EINVAL). */
 #define CS_MAX_THRESHOLDS 8
 
 typedef struct cs_params {
-  uint32_t n_thresholds;     /* >= 1 (batches and the server: at most CS_MAX_THRESHOLDS) */
+  uint32_t n_thresholds;     /* >= 1 (the server: at most CS_MAX_THRESHOLDS) */
   const uint32_t *distances; /* [T] strictly increasing */
   const float *betas;        /* [T] */
   const uint32_t *seconds;   /* [T] */
@@ -406,9 +407,10 @@ int cs_release_workspaces(void);
 typedef struct cs_batch cs_batch;
 int cs_batch_create(cs_batch **out);
 void cs_batch_destroy(cs_batch *b);
-/* Adds a job with the arguments of cs_centrality_shortest; `out` ([7][T][N])
- * and `reach_totals` ([T], may be NULL) are filled by cs_batch_results.
- * Returns the job index (>= 0) or minus a status code. */
+/* Adds a job with the arguments of cs_centrality_shortest, any number of
+ * thresholds; `out` ([7][T][N]) and `reach_totals` ([T], may be NULL) are
+ * filled by cs_batch_results. Returns the job index (>= 0, in call order) or
+ * minus a status code. */
 int cs_batch_add_shortest(cs_batch *b, cs_graph *g, const cs_params *p, double *out, uint32_t *reach_totals);
 /* Times candidate launch shapes on a `sample_fraction` of the batch's sources
  * and keeps the fastest; `chosen` (may be NULL) receives it. */

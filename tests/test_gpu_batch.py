@@ -105,3 +105,31 @@ def test_explicit_tunings_agree():
         b.set_tuning(blk, wps, ds)
         b.run()
         _check_equal(b.results()[0], ref, [500, 1000], f"tuning {blk}/{wps}/{ds}")
+
+
+def test_batch_request_with_ten_thresholds():
+    """A request of ten thresholds runs as two jobs (8 + 2) in the same launch, each searching to
+    the request's largest cutoff, and fills one [7][10][N] result: equal to the plain call (which
+    chunks the same way) and, for the chunk boundary's neighbours, to the oracle."""
+    lat = mock.perturbed_lattice(16, seed=21, diag_frac=0.05)
+    ns = mock.network_structure_from_lattice(lat)
+    ten = [150, 300, 450, 600, 750, 900, 1050, 1200, 1350, 1500]
+    b = CentralityBatch()
+    assert b.add_shortest(ns, distances=[500]) == 0
+    assert b.add_shortest(ns, distances=ten) == 1
+    assert b.add_shortest(ns, distances=[250, 700], compute_closeness=False) == 2
+    plain = [ns.centrality_shortest(distances=[500]), ns.centrality_shortest(distances=ten),
+             ns.centrality_shortest(distances=[250, 700], compute_closeness=False)]
+    for rep in range(2):
+        say(f"ten-threshold batch, replay {rep}")
+        b.run()
+        res = b.results()
+        assert len(res) == 3
+        for i, (r, p, d) in enumerate(zip(res, plain, ([500], ten, [250, 700]))):
+            _check_equal(r, p, d, f"job {i} replay {rep}")
+            assert r.reachability_totals == p.reachability_totals
+    og = OracleGraph.from_network_structure(ns)
+    ref, _reach = og.centrality_shortest(ten)
+    got = stack(res[1], KEYS, ten)
+    assert np.array_equal(got[0], ref[0])
+    assert_close(got, ref, "ten thresholds in a batch vs oracle")
