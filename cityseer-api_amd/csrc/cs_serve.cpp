@@ -46,7 +46,8 @@ struct Req {
   int64_t id = 0;
   cs_graph *h = nullptr;
   int kind = CS_REQ_SHORTEST;
-  CallSpec spec;
+  CallSpec spec;                 // the first threshold chunk
+  std::vector<CallSpec> chunks;  // one per kMaxT thresholds, each searching to the largest cutoff
   SourceSet ss;
   std::vector<uint8_t> elig;
   double *out = nullptr;
@@ -55,6 +56,7 @@ struct Req {
   double betw_scale = 1.0;
   uint32_t N = 0, T = 0, M = 0;
   size_t dev_bytes = 0;
+  uint32_t reach_words() const { return std::max<uint32_t>(T, kMaxT); }
   // device state (valid once `ready`)
   bool ready = false;
   uint32_t *d_src = nullptr;
@@ -158,16 +160,16 @@ struct cs_server {
     r.d_el = dev_copy(r.elig.data(), r.elig.size());
     hip_check(hipMalloc(&r.d_acc, sizeof(double) * std::max<size_t>((size_t)r.N * r.M * r.T, 1)), "hipMalloc acc");
     hip_check(hipMemset(r.d_acc, 0, sizeof(double) * (size_t)r.N * r.M * r.T), "memset acc");
-    hip_check(hipMalloc(&r.d_reach, 4 * kMaxT), "hipMalloc reach");
-    hip_check(hipMemset(r.d_reach, 0, 4 * kMaxT), "memset reach");
+    hip_check(hipMalloc(&r.d_reach, 4 * r.reach_words()), "hipMalloc reach");
+    hip_check(hipMemset(r.d_reach, 0, 4 * r.reach_words()), "memset reach");
   }
 
   void finish(Req &r) {
     std::vector<double> hacc((size_t)r.N * r.M * r.T);
-    uint32_t hreach[kMaxT];
+    std::vector<uint32_t> hreach(r.reach_words());
     if (!hacc.empty())
       hip_check(hipMemcpy(hacc.data(), r.d_acc, sizeof(double) * hacc.size(), hipMemcpyDeviceToHost), "copy out");
-    hip_check(hipMemcpy(hreach, r.d_reach, 4 * kMaxT, hipMemcpyDeviceToHost), "copy out");
+    hip_check(hipMemcpy(hreach.data(), r.d_reach, 4 * hreach.size(), hipMemcpyDeviceToHost), "copy out");
     unpack_acc_strided(hacc.data(), r.N, r.M, r.T, r.T, r.out,
                        r.kind == CS_REQ_SHORTEST ? result_order(r.h->g) : nullptr);
     if (r.betw) {
@@ -376,13 +378,18 @@ struct cs_server {
           uint32_t NT = 1;
           for (size_t i = 0; i < slice_reqs.size(); ++i) {
             Req &r = *slice_reqs[i];
-            JobSpec js{&r.h->g, r.dg, r.de, r.spec, r.d_src, r.d_wt, r.d_cyc, r.n(), r.d_el, r.d_acc,
-                       r.d_reach, 1.0f};
-            js.tstride = r.T;
-            specs.push_back(js);
-            NT = std::max(NT, r.T);
-            const uint32_t first = r.handed - slice_take[i];
-            for (uint32_t k = 0; k < slice_take[i]; ++k) items.push_back(make_uint2((uint32_t)i, first + k));
+            // one job per threshold chunk, each filling its rows of the request's accumulators
+            for (size_t c = 0; c < r.chunks.size(); ++c) {
+              const uint32_t t0 = (uint32_t)c * kMaxT;
+              JobSpec js{&r.h->g, r.dg, r.de, r.chunks[c], r.d_src, r.d_wt, r.d_cyc, r.n(), r.d_el,
+                         r.d_acc + (size_t)t0 * r.N, r.d_reach + t0, 1.0f};
+              js.tstride = r.T;
+              const uint32_t job = (uint32_t)specs.size();
+              specs.push_back(js);
+              NT = std::max(NT, r.chunks[c].T);
+              const uint32_t first = r.handed - slice_take[i];
+              for (uint32_t k = 0; k < slice_take[i]; ++k) items.push_back(make_uint2(job, first + k));
+            }
           }
           n_src = items.size();
           launch_shortest_batch(specs, NT, default_tuning(), stream, st, &items);
@@ -391,9 +398,16 @@ struct cs_server {
           std::vector<uint2> items;
           for (size_t i = 0; i < slice_reqs.size(); ++i) {
             Req &r = *slice_reqs[i];
-            specs.push_back(AngSpec{&r.h->g, r.dg, r.de, r.spec, r.d_src, r.d_wt, r.d_el, r.d_acc, r.d_reach});
-            const uint32_t first = r.handed - slice_take[i];
-            for (uint32_t k = 0; k < slice_take[i]; ++k) items.push_back(make_uint2((uint32_t)i, first + k));
+            for (size_t c = 0; c < r.chunks.size(); ++c) {
+              const uint32_t t0 = (uint32_t)c * kMaxT;
+              AngSpec as{&r.h->g, r.dg, r.de, r.chunks[c], r.d_src, r.d_wt, r.d_el, r.d_acc + (size_t)t0 * r.N,
+                         r.d_reach + t0};
+              as.tstride = r.T;
+              const uint32_t job = (uint32_t)specs.size();
+              specs.push_back(as);
+              const uint32_t first = r.handed - slice_take[i];
+              for (uint32_t k = 0; k < slice_take[i]; ++k) items.push_back(make_uint2(job, first + k));
+            }
           }
           n_src = items.size();
           launch_simplest_jobs(specs, &items, 0, stream, st);
@@ -444,14 +458,13 @@ int64_t submit_impl(cs_server *s, cs_graph *h, int kind, const cs_params *p, dou
   Graph &g = h->g;
   if (kind == CS_REQ_SIMPLEST) check_simplest_graph(g);
   check_params(g, p);
-  if (p->n_thresholds > CS_MAX_THRESHOLDS)
-    fail(CS_EINVAL, "a served request takes at most " + std::to_string(CS_MAX_THRESHOLDS) +
-                        " distance thresholds (split it into several requests)");
   if (kind == CS_REQ_SHORTEST) run_shortest_checks(g, p);
   auto r = std::make_unique<Req>();
   r->h = h;
   r->kind = kind;
-  r->spec = make_spec(p);
+  for (uint32_t t0 = 0; t0 < p->n_thresholds; t0 += kMaxT)
+    r->chunks.push_back(make_spec(p, t0, chunk_len(p->n_thresholds, t0)));
+  r->spec = r->chunks[0];
   r->ss = source_set(g, p);
   if (kind == CS_REQ_SHORTEST) {
     to_device_sources(g, r->ss);
@@ -464,9 +477,9 @@ int64_t submit_impl(cs_server *s, cs_graph *h, int kind, const cs_params *p, dou
   r->betw = p->compute_betweenness != 0;
   r->betw_scale = p->betw_scale;
   r->N = g.n;
-  r->T = r->spec.T;
+  r->T = p->n_thresholds;
   r->M = kind == CS_REQ_SHORTEST ? 7 : 4;
-  r->dev_bytes = (size_t)r->N * r->M * r->T * 8 + (size_t)r->n() * 12 + g.n + 4 * kMaxT;
+  r->dev_bytes = (size_t)r->N * r->M * r->T * 8 + (size_t)r->n() * 12 + g.n + 4 * r->reach_words();
   if (r->dev_bytes > s->cfg.hbm_budget_bytes)
     fail(CS_ENOMEM, "request needs " + std::to_string(r->dev_bytes >> 20) + " MiB, above the server's HBM budget");
   r->t_submit = Clock::now();

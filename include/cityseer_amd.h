@@ -184,13 +184,12 @@ uint64_t cs_graph_progress(const cs_graph *g);
 
 /* Thresholds one kernel launch takes. Calls with more run one launch per
  * chunk of CS_MAX_THRESHOLDS, each searching to the call's largest cutoff (the
- * reference's single search); a batched request with more runs as one job per
- * chunk in the same launch; served requests take at most this many
- * (CS_EINVAL). */
+ * reference's single search); batched and served requests with more run as one
+ * job per chunk in the same launches. */
 #define CS_MAX_THRESHOLDS 8
 
 typedef struct cs_params {
-  uint32_t n_thresholds;     /* >= 1 (the server: at most CS_MAX_THRESHOLDS) */
+  uint32_t n_thresholds;     /* >= 1 */
   const uint32_t *distances; /* [T] strictly increasing */
   const float *betas;        /* [T] */
   const uint32_t *seconds;   /* [T] */

@@ -108,3 +108,25 @@ def test_arrivals_during_running_launches(primal_graph):
             else:
                 d = [400 + 100 * i]
                 _same(r.result(), a.centrality_shortest(distances=d), "shortest", d, f"late {i}")
+
+
+def test_requests_with_ten_thresholds(primal_graph):
+    """Served requests of more than 8 thresholds run one job per threshold chunk in each of their
+    slices (each chunk searching to the request's largest cutoff) and equal the plain call, which
+    chunks the same way, beside requests of one and three thresholds."""
+    _mock_ns, a, _b, dual = _graphs(primal_graph)
+    ten = [150, 300, 450, 600, 750, 900, 1050, 1200, 1350, 1500]
+    mix = [
+        ("shortest", a, dict(distances=ten)),
+        ("simplest", dual, dict(distances=ten)),
+        ("shortest", a, dict(distances=[400, 800, 1600])),
+        ("simplest", dual, dict(distances=[700])),
+    ]
+    alone = [_alone(k, ns, kw) for k, ns, kw in mix]
+    say("serving two ten-threshold requests in a mix of four (slice 2 ms)")
+    with CentralityServer(slice_ms=2.0) as srv:
+        reqs = [(srv.submit_shortest if k == "shortest" else srv.submit_simplest)(ns, **kw) for k, ns, kw in mix]
+        for i, ((k, ns, kw), r, w) in enumerate(zip(mix, reqs, alone)):
+            info = r.wait()
+            assert info["status"] == 2, info
+            _same(r.result(), w, k, kw["distances"], f"request {i} ({k}, {len(kw['distances'])} thresholds)")
