@@ -729,21 +729,41 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           const uint32_t k = *sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_np);
           const uint32_t *preds = reinterpret_cast<const uint32_t *>(w + a.o_preds) + ((size_t)lane * a.scap + ss) * a.maxp;
           const uint32_t node = (*sr<uint32_t>(w, a.o_srec, ss, lane, a.scap, kS_state) & 0x7fffffffu) >> 1;
-          for (uint32_t t = 0; t < T; ++t) {
-            if (!(secs_ss <= jb.sec_thr[t])) continue;
-            const double seed = *il<double>(w, a.o_seed, ss * tc + t, lane, (size_t)a.scap * a.tcap);
-            const double dep = seed + *il<double>(w, a.o_delta, ss * tc + t, lane, (size_t)a.scap * a.tcap);
-            if (dep == 0.0) continue;
-            for (uint32_t p = 0; p < k; ++p) {
-              const uint32_t ps = preds[p];
-              const double sv = *sr<double>(w, a.o_srec, ps, lane, a.scap, kS_sigma);
-              if (sv == 0.0) continue;
-              double *dp = il<double>(w, a.o_delta, ps * tc + t, lane, (size_t)a.scap * a.tcap);
-              *dp = *dp + (sv / sw) * dep;
+          // the state's seed and dependency for every threshold it lies within
+          // (0 elsewhere), then each predecessor once: its sigma, then its
+          // dependencies for all thresholds together. Each dependency word still
+          // takes its terms in walk order, so the sums are those of a walk per
+          // threshold.
+          double seed[kMaxT], dep[kMaxT];
+          bool live = false;
+#pragma unroll
+          for (uint32_t t = 0; t < kMaxT; ++t) {
+            seed[t] = dep[t] = 0.0;
+            if (t < T && secs_ss <= jb.sec_thr[t]) {
+              seed[t] = *il<double>(w, a.o_seed, ss * tc + t, lane, (size_t)a.scap * a.tcap);
+              dep[t] = seed[t] + *il<double>(w, a.o_delta, ss * tc + t, lane, (size_t)a.scap * a.tcap);
             }
-            if (node == src) continue;
-            const double credit = dep - seed;
-            if (credit > 0.0) unsafeAtomicAdd(&jb.acc[((size_t)3 * jb.tstride + t) * jb.N + node], credit * (double)wt);
+            live = live || dep[t] != 0.0;
+          }
+          if (!live) continue;
+          for (uint32_t p = 0; p < k; ++p) {
+            const uint32_t ps = preds[p];
+            const double sv = *sr<double>(w, a.o_srec, ps, lane, a.scap, kS_sigma);
+            if (sv == 0.0) continue;
+            const double ratio = sv / sw;
+#pragma unroll
+            for (uint32_t t = 0; t < kMaxT; ++t) {
+              if (dep[t] == 0.0) continue;
+              double *dp = il<double>(w, a.o_delta, ps * tc + t, lane, (size_t)a.scap * a.tcap);
+              *dp = *dp + ratio * dep[t];
+            }
+          }
+          if (node == src) continue;
+#pragma unroll
+          for (uint32_t t = 0; t < kMaxT; ++t) {
+            const double credit = dep[t] - seed[t];
+            if (dep[t] != 0.0 && credit > 0.0)
+              unsafeAtomicAdd(&jb.acc[((size_t)3 * jb.tstride + t) * jb.N + node], credit * (double)wt);
           }
         }
         hi = lo - 1;
