@@ -690,6 +690,15 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           cand[ncand++] = (uint32_t)sk;
         }
         const double pc = jb.eligible[v] ? 0.5 : 1.0;
+        // the candidates' sigma, seconds and route, read once for every threshold
+        double csg[2] = {0.0, 0.0};
+        float csec[2] = {INFINITY, INFINITY}, crt[2] = {INFINITY, INFINITY};
+        for (uint32_t q = 0; q < ncand; ++q) {
+          const uint4 hd = *sr<uint4>(w, a.o_srec, cand[q], lane, a.scap, 0);  // route, secs, state, np
+          crt[q] = __uint_as_float(hd.x);
+          csec[q] = __uint_as_float(hd.y);
+          csg[q] = *sr<double>(w, a.o_srec, cand[q], lane, a.scap, kS_sigma);
+        }
         for (uint32_t t = 0; t < T; ++t) {
           const float sthr = jb.sec_thr[t];
           if (bs > sthr) continue;
@@ -697,18 +706,16 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           uint32_t nbst = 0;
           double total = 0.0;
           for (uint32_t q = 0; q < ncand; ++q) {
-            const uint32_t ss = cand[q];
-            const double sg = *sr<double>(w, a.o_srec, ss, lane, a.scap, kS_sigma);
-            if (sg == 0.0 || *sr<float>(w, a.o_srec, ss, lane, a.scap, kS_secs) > sthr) continue;
-            if (*sr<float>(w, a.o_srec, ss, lane, a.scap, kS_route) <= br * (1.0f + jb.tolerance)) best[nbst++] = ss;
+            if (csg[q] == 0.0 || csec[q] > sthr) continue;
+            if (crt[q] <= br * (1.0f + jb.tolerance)) best[nbst++] = q;
           }
           if (nbst == 0) continue;
-          for (uint32_t q = 0; q < nbst; ++q) total += *sr<double>(w, a.o_srec, best[q], lane, a.scap, kS_sigma);
+          for (uint32_t q = 0; q < nbst; ++q) total += csg[best[q]];
           if (total == 0.0) continue;
-          for (uint32_t q = 0; q < nbst; ++q) {
-            double *sd = il<double>(w, a.o_seed, best[q] * tc + t, lane, (size_t)a.scap * a.tcap);
-            *sd = *sd + pc * (*sr<double>(w, a.o_srec, best[q], lane, a.scap, kS_sigma) / total);
-          }
+          // a state's seed words are written only here, for its own node, at
+          // most once per threshold, onto cleared words: 0 + x is x
+          for (uint32_t q = 0; q < nbst; ++q)
+            *il<double>(w, a.o_seed, cand[best[q]] * tc + t, lane, (size_t)a.scap * a.tcap) = pc * (csg[best[q]] / total);
         }
       }
       // states by route descending, equal routes in settle order (stable sort)
