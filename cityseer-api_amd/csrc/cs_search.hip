@@ -1086,6 +1086,22 @@ __device__ __forceinline__ void flush_target(CJob &j, const WaveWs &w, uint32_t 
   const float route = __uint_as_float(secs_bits) * j.speed;
   const size_t ts = j.tstride, nacc = j.nacc;
   double *accv = j.acc + v;
+  // the dependencies are loaded before any atomic is issued: vmcnt counts
+  // stores and atomics too, in order, so a load issued behind them would
+  // wait for every one of them to reach L2
+  double dpl[NT], dbt[NT];
+  double pc = 0.0;
+  if (j.flags & 2u) {
+    pc = (j.flags & 8u) ? ldd(&w.ods[s]) : (j.eligible[v] ? 0.5 : 1.0);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      dpl[t] = dbt[t] = 0.0;
+      if (route <= j.dist[t]) {
+        dpl[t] = ldd(dep_at(w, t, s));
+        dbt[t] = ldd(dep_at(w, NT + t, s));
+      }
+    }
+  }
   if (j.flags & 1u) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -1099,15 +1115,14 @@ __device__ __forceinline__ void flush_target(CJob &j, const WaveWs &w, uint32_t 
     }
   }
   if (j.flags & 2u) {
-    // seed: the all-pairs weight (0.5 between two eligible sources), or the OD trip weight
-    const double pc = (j.flags & 8u) ? ldd(&w.ods[s]) : (j.eligible[v] ? 0.5 : 1.0);
+    // seed pc: the all-pairs weight (0.5 between two eligible sources), or the OD trip weight
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       if (route <= j.dist[t]) {
         // dependency = seed + delta; credit = dependency - seed (centrality.rs:848-866)
         const double seed_b = pc * exp(-(double)j.beta[t] * (double)route);
-        const double credit = (pc + ldd(dep_at(w, t, s))) - pc;
-        const double credit_b = (seed_b + ldd(dep_at(w, NT + t, s))) - seed_b;
+        const double credit = (pc + dpl[t]) - pc;
+        const double credit_b = (seed_b + dbt[t]) - seed_b;
         if (credit > 0.0) gadd_f64(&accv[((size_t)5 * ts + t) * nacc], credit * (double)wt);
         if (credit_b > 0.0) gadd_f64(&accv[((size_t)6 * ts + t) * nacc], credit_b * (double)wt);
       }
