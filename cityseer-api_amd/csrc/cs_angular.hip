@@ -204,6 +204,7 @@ constexpr uint32_t kMaskWords = 2;                // occupancy mask words held i
 constexpr uint32_t kBuckets = 64 * kMaskWords;     // ring of 128 buckets
 constexpr uint32_t kBucketSpan = kBuckets - 8;     // buckets per largest route increment
 constexpr uint32_t kBqCap = 64;  // entries per bucket before a source falls back to the heap
+constexpr uint32_t kCloseBatch = 4;  // node records the closeness pass loads ahead of its atomics
 // Entries per bucket: kBqCap, or 1/128 of a tuned slot capacity (at least 2),
 // so that a small capacity also sends sources to the heap early.
 uint32_t bucket_capacity() {
@@ -639,25 +640,34 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
     st_reached += n_nodes;
     st_edges += n_order;
     const uint32_t T = jb.T, TS = jb.tstride;
-    // ---- closeness at targets (centrality.rs:1995-2033)
-    for (uint32_t k = 1; k < n_nodes; ++k) {
-      const uint32_t v = *nr<uint32_t>(w, a.o_nrec, k, lane, a.ncap, kN_node);
-      if (v == src) continue;
-      const float bsd = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_brt);
-      const float bs = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_bsec);
-      if (!isfinite(bsd) || !isfinite(bs)) continue;
-      double *accv = jb.acc + v;
-      const size_t NN = jb.N;
-      for (uint32_t t = 0; t < T; ++t) {
-        if (bs <= jb.sec_thr[t]) {
-          ++st_within;
-          if (jb.flags & 4u) atomicAdd(&jb.reach[t], 1u);
-          if (jb.flags & 1u) {
-            unsafeAtomicAdd(&accv[((size_t)0 * TS + t) * NN], (double)wt);
-            const float far_ang = jb.offset + (bsd / jb.unit);
-            unsafeAtomicAdd(&accv[((size_t)1 * TS + t) * NN], (double)(far_ang * wt));
-            const float harm_ang = 1.0f + (bsd / jb.unit);
-            unsafeAtomicAdd(&accv[((size_t)2 * TS + t) * NN], (double)((1.0f / harm_ang) * wt));
+    // ---- closeness at targets (centrality.rs:1995-2033), kCloseBatch node
+    // records loaded before their atomics: vmcnt counts atomics with the
+    // loads, in order, so a record loaded behind them waits for them
+    for (uint32_t k0 = 1; k0 < n_nodes; k0 += kCloseBatch) {
+      uint4 rec[kCloseBatch];
+#pragma unroll
+      for (uint32_t i = 0; i < kCloseBatch; ++i)
+        rec[i] = k0 + i < n_nodes ? *nr<uint4>(w, a.o_nrec, k0 + i, lane, a.ncap, 0) : make_uint4(src, 0u, 0u, 0u);
+#pragma unroll
+      for (uint32_t i = 0; i < kCloseBatch; ++i) {
+        const uint32_t v = rec[i].x;  // node, best route, its seconds
+        if (v == src) continue;
+        const float bsd = __uint_as_float(rec[i].y);
+        const float bs = __uint_as_float(rec[i].z);
+        if (!isfinite(bsd) || !isfinite(bs)) continue;
+        double *accv = jb.acc + v;
+        const size_t NN = jb.N;
+        for (uint32_t t = 0; t < T; ++t) {
+          if (bs <= jb.sec_thr[t]) {
+            ++st_within;
+            if (jb.flags & 4u) atomicAdd(&jb.reach[t], 1u);
+            if (jb.flags & 1u) {
+              unsafeAtomicAdd(&accv[((size_t)0 * TS + t) * NN], (double)wt);
+              const float far_ang = jb.offset + (bsd / jb.unit);
+              unsafeAtomicAdd(&accv[((size_t)1 * TS + t) * NN], (double)(far_ang * wt));
+              const float harm_ang = 1.0f + (bsd / jb.unit);
+              unsafeAtomicAdd(&accv[((size_t)2 * TS + t) * NN], (double)((1.0f / harm_ang) * wt));
+            }
           }
         }
       }
@@ -675,47 +685,75 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
           *il<double>(w, a.o_seed, ss * tc + t, lane, (size_t)a.scap * a.tcap) = 0.0;
         }
       }
-      // seeds: best terminal states split by sigma (best_angular_target_states)
-      for (uint32_t k = 1; k < n_nodes; ++k) {
-        const uint32_t v = *nr<uint32_t>(w, a.o_nrec, k, lane, a.ncap, kN_node);
-        if (v == src) continue;
-        const float br = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_brt);
-        const float bs = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_bsec);
-        if (!isfinite(br) || !isfinite(bs)) continue;
-        uint32_t cand[2];
-        uint32_t ncand = 0;
-        for (uint32_t slot = 0; slot < 2; ++slot) {
-          const uint64_t sk = L.skey[2 * v + slot];
-          if ((uint32_t)(sk >> 32) != epoch) continue;
-          cand[ncand++] = (uint32_t)sk;
-        }
-        const double pc = jb.eligible[v] ? 0.5 : 1.0;
-        // the candidates' sigma, seconds and route, read once for every threshold
-        double csg[2] = {0.0, 0.0};
-        float csec[2] = {INFINITY, INFINITY}, crt[2] = {INFINITY, INFINITY};
-        for (uint32_t q = 0; q < ncand; ++q) {
-          const uint4 hd = *sr<uint4>(w, a.o_srec, cand[q], lane, a.scap, 0);  // route, secs, state, np
-          crt[q] = __uint_as_float(hd.x);
-          csec[q] = __uint_as_float(hd.y);
-          csg[q] = *sr<double>(w, a.o_srec, cand[q], lane, a.scap, kS_sigma);
-        }
-        for (uint32_t t = 0; t < T; ++t) {
-          const float sthr = jb.sec_thr[t];
-          if (bs > sthr) continue;
-          uint32_t best[2];
-          uint32_t nbst = 0;
-          double total = 0.0;
-          for (uint32_t q = 0; q < ncand; ++q) {
-            if (csg[q] == 0.0 || csec[q] > sthr) continue;
-            if (crt[q] <= br * (1.0f + jb.tolerance)) best[nbst++] = q;
+      // seeds: best terminal states split by sigma (best_angular_target_states),
+      // kCloseBatch nodes at a time: their records, then their state keys, then
+      // the candidate states, all loaded before the batch's seed stores
+      for (uint32_t k0 = 1; k0 < n_nodes; k0 += kCloseBatch) {
+        uint4 rec[kCloseBatch];
+#pragma unroll
+        for (uint32_t i = 0; i < kCloseBatch; ++i)
+          rec[i] = k0 + i < n_nodes ? *nr<uint4>(w, a.o_nrec, k0 + i, lane, a.ncap, 0) : make_uint4(src, 0u, 0u, 0u);
+        uint32_t cand[kCloseBatch][2];
+        bool has[kCloseBatch][2];
+        double pc[kCloseBatch];
+#pragma unroll
+        for (uint32_t i = 0; i < kCloseBatch; ++i) {
+          const uint32_t v = rec[i].x;  // node, best route, its seconds
+          const bool live = v != src && isfinite(__uint_as_float(rec[i].y)) && isfinite(__uint_as_float(rec[i].z));
+          pc[i] = 0.0;
+#pragma unroll
+          for (uint32_t slot = 0; slot < 2; ++slot) {
+            const uint64_t sk = live ? L.skey[2 * v + slot] : 0ull;
+            has[i][slot] = live && (uint32_t)(sk >> 32) == epoch;
+            cand[i][slot] = (uint32_t)sk;
           }
-          if (nbst == 0) continue;
-          for (uint32_t q = 0; q < nbst; ++q) total += csg[best[q]];
-          if (total == 0.0) continue;
-          // a state's seed words are written only here, for its own node, at
-          // most once per threshold, onto cleared words: 0 + x is x
-          for (uint32_t q = 0; q < nbst; ++q)
-            *il<double>(w, a.o_seed, cand[best[q]] * tc + t, lane, (size_t)a.scap * a.tcap) = pc * (csg[best[q]] / total);
+          if (live) pc[i] = jb.eligible[v] ? 0.5 : 1.0;
+        }
+        // the candidates' sigma, seconds and route, read once for every threshold
+        double csg[kCloseBatch][2];
+        float csec[kCloseBatch][2], crt[kCloseBatch][2];
+#pragma unroll
+        for (uint32_t i = 0; i < kCloseBatch; ++i) {
+#pragma unroll
+          for (uint32_t slot = 0; slot < 2; ++slot) {
+            csg[i][slot] = 0.0;
+            csec[i][slot] = crt[i][slot] = INFINITY;
+            if (has[i][slot]) {
+              const uint4 hd = *sr<uint4>(w, a.o_srec, cand[i][slot], lane, a.scap, 0);  // route, secs, state, np
+              crt[i][slot] = __uint_as_float(hd.x);
+              csec[i][slot] = __uint_as_float(hd.y);
+              csg[i][slot] = *sr<double>(w, a.o_srec, cand[i][slot], lane, a.scap, kS_sigma);
+            }
+          }
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < kCloseBatch; ++i) {
+          if (pc[i] == 0.0) continue;
+          const float br = __uint_as_float(rec[i].y), bs = __uint_as_float(rec[i].z);
+          for (uint32_t t = 0; t < T; ++t) {
+            const float sthr = jb.sec_thr[t];
+            if (bs > sthr) continue;
+            // the candidates in slot order, as the reference lists them
+            bool use[2];
+            double total = 0.0;
+#pragma unroll
+            for (uint32_t slot = 0; slot < 2; ++slot) {
+              use[slot] = has[i][slot] && csg[i][slot] != 0.0 && !(csec[i][slot] > sthr) &&
+                          crt[i][slot] <= br * (1.0f + jb.tolerance);
+            }
+#pragma unroll
+            for (uint32_t slot = 0; slot < 2; ++slot)
+              if (use[slot]) total += csg[i][slot];
+            if (!use[0] && !use[1]) continue;
+            if (total == 0.0) continue;
+            // a state's seed words are written only here, for its own node, at
+            // most once per threshold, onto cleared words: 0 + x is x
+#pragma unroll
+            for (uint32_t slot = 0; slot < 2; ++slot)
+              if (use[slot])
+                *il<double>(w, a.o_seed, cand[i][slot] * tc + t, lane, (size_t)a.scap * a.tcap) =
+                    pc[i] * (csg[i][slot] / total);
+          }
         }
       }
       // states by route descending, equal routes in settle order (stable sort)
