@@ -1058,10 +1058,16 @@ __global__ __launch_bounds__(kBlock) void k_reach_simplest(AngArgs a, const uint
       continue;
     }
     uint64_t *lst = lists + (size_t)it * a.ncap;
-    for (uint32_t k = 0; k < n_nodes; ++k) {
-      const uint32_t v = *nr<uint32_t>(w, a.o_nrec, k, lane, a.ncap, kN_node);
-      const float secs = *nr<float>(w, a.o_nrec, k, lane, a.ncap, kN_bsec);
-      lst[k] = (uint64_t)v | ((uint64_t)__float_as_uint(secs) << 32);
+    // kCloseBatch node records loaded before their list stores (a load issued
+    // behind a store waits for it)
+    for (uint32_t k0 = 0; k0 < n_nodes; k0 += kCloseBatch) {
+      uint4 rec[kCloseBatch];
+#pragma unroll
+      for (uint32_t i = 0; i < kCloseBatch; ++i)
+        if (k0 + i < n_nodes) rec[i] = *nr<uint4>(w, a.o_nrec, k0 + i, lane, a.ncap, 0);  // node, best, its seconds
+#pragma unroll
+      for (uint32_t i = 0; i < kCloseBatch; ++i)
+        if (k0 + i < n_nodes) lst[k0 + i] = (uint64_t)rec[i].x | ((uint64_t)rec[i].z << 32);
     }
     counts[it] = n_nodes;
     st_reached += n_nodes;
