@@ -759,12 +759,14 @@ __global__ __launch_bounds__(kBlock) void k_angular(AngArgs a) {
       // states by route descending, equal routes in settle order (stable sort)
       int hi = (int)n_order - 1;
       while (hi >= 0) {
-        // routes by settle position: the group scan reads neighbouring words
+        // routes by settle position: the group scan reads neighbouring words;
+        // the group's last state loads with its route (most groups hold one)
         const float r_hi = *il<float>(w, a.o_orte, hi, lane, a.scap);
+        const uint32_t ss_hi = *il<uint32_t>(w, a.o_order, hi, lane, a.scap);
         int lo = hi;
         while (lo > 0 && *il<float>(w, a.o_orte, lo - 1, lane, a.scap) == r_hi) --lo;
         for (int q = lo; q <= hi; ++q) {
-          const uint32_t ss = *il<uint32_t>(w, a.o_order, q, lane, a.scap);
+          const uint32_t ss = q == hi ? ss_hi : *il<uint32_t>(w, a.o_order, q, lane, a.scap);
           const float secs_ss = *sr<float>(w, a.o_srec, ss, lane, a.scap, kS_secs);
           bool any = false;
           for (uint32_t t = 0; t < T; ++t) any = any || secs_ss <= jb.sec_thr[t];
